@@ -1,0 +1,171 @@
+"""hera_amd: MI355X-native SQUAREM EM abundance estimation for hera.
+
+The product is the C library ``libheraem.so`` (include/hera_em.h and the
+drop-in ``estimate_count`` of include/hera_dropin.h).  This package is a thin
+ctypes view of it for tests and benchmarks.  Nothing here computes on the
+CPU: every call goes to the HIP path and raises if the library is missing.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import HemError, as_f64, as_u32, check, f64p, u32p
+
+__all__ = ["Classes", "Problem", "generate", "opts", "HemError",
+           "PREC_FP32", "PREC_FP64"]
+
+PREC_FP32 = 0
+PREC_FP64 = 1
+STEP_SD = 0
+STEP_L2 = 1
+CLAMP_ZERO = 0
+CLAMP_DIV10 = 1
+
+
+@dataclass
+class Classes:
+    """hera's class table in CSR form (Gclass + Ref_inf->len).
+
+    Class c has members cls_ref[cls_off[c]:cls_off[c+1]] and cls_count[c]
+    fragments; tx_len[t] is the u32 transcript length.
+    """
+    n_tx: int
+    cls_off: np.ndarray
+    cls_ref: np.ndarray
+    cls_count: np.ndarray
+    tx_len: np.ndarray
+    cluster_off: np.ndarray = field(default=None)
+
+    @property
+    def n_cls(self):
+        return len(self.cls_off) - 1
+
+    @property
+    def nnz(self):
+        return int(self.cls_off[-1]) if len(self.cls_off) else 0
+
+    @classmethod
+    def from_lists(cls, n_tx, members, counts, tx_len):
+        off = np.zeros(len(members) + 1, dtype=np.uint32)
+        off[1:] = np.cumsum([len(m) for m in members]) if members else []
+        ref = (np.concatenate([np.asarray(m, dtype=np.uint32) for m in members])
+               if members and off[-1] else np.zeros(0, dtype=np.uint32))
+        return cls(n_tx, off, ref, as_f64(counts), as_u32(tx_len))
+
+    def to_c(self):
+        self.cls_off = as_u32(self.cls_off)
+        self.cls_ref = as_u32(self.cls_ref)
+        self.cls_count = as_f64(self.cls_count)
+        self.tx_len = as_u32(self.tx_len)
+        return _lib.HemClasses(self.n_tx, self.n_cls, u32p(self.cls_off),
+                               u32p(self.cls_ref), f64p(self.cls_count),
+                               u32p(self.tx_len))
+
+
+def generate(n_tx, n_cls_all, n_frag, seed=0, **overrides):
+    """Seeded synthetic class table in hera's layout (include/hera_em_gen.h).
+
+    Returns (Classes, unique_counts).
+    """
+    lib = _lib.load()
+    g = _lib.HemGenParams()
+    lib.hem_gen_default(g, n_tx, n_cls_all, n_frag, seed)
+    for k, v in overrides.items():
+        setattr(g, k, v)
+    out = _lib.C.POINTER(_lib.HemGenProblem)()
+    check(lib.hem_generate(g, out))
+    try:
+        p = out.contents
+        C, T, Z = p.n_cls, p.n_tx, p.nnz
+        cls = Classes(
+            T,
+            np.ctypeslib.as_array(p.cls_off, (C + 1,)).copy(),
+            np.ctypeslib.as_array(p.cls_ref, (max(Z, 1),))[:Z].copy(),
+            np.ctypeslib.as_array(p.cls_count, (max(C, 1),))[:C].copy(),
+            np.ctypeslib.as_array(p.tx_len, (T,)).copy(),
+            np.ctypeslib.as_array(p.cluster_off, (p.n_clusters + 1,)).copy())
+        unique = np.ctypeslib.as_array(p.unique, (T,)).copy()
+    finally:
+        lib.hem_gen_free(out)
+    return cls, unique
+
+
+def opts(**kw):
+    o = _lib.HemOpts()
+    _lib.load().hem_opts_default(o)
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+class Problem:
+    """A device-resident class table (hem_problem)."""
+
+    def __init__(self, classes):
+        lib = _lib.load()
+        self.classes = classes
+        self._c = classes.to_c()
+        self._h = _lib.C.c_void_p()
+        check(lib.hem_problem_create(self._c, self._h))
+        self.n_tx = classes.n_tx
+
+    def close(self):
+        if self._h:
+            _lib.load().hem_problem_destroy(self._h)
+            self._h = _lib.C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        T = _lib.C.c_uint32()
+        Cn = _lib.C.c_uint32()
+        Z = _lib.C.c_uint64()
+        B = _lib.C.c_uint64()
+        check(_lib.load().hem_problem_info(self._h, T, Cn, Z, B))
+        return dict(n_tx=T.value, n_cls=Cn.value, nnz=Z.value,
+                    device_bytes=B.value)
+
+    def solve(self, unique, **kw):
+        """SQUAREM EM solve; returns dict(tpm, est, lam, stats)."""
+        u = as_u32(unique)
+        assert u.shape == (self.n_tx,)
+        o = opts(**kw)
+        tpm = np.empty(self.n_tx)
+        est = np.empty(self.n_tx)
+        lam = np.empty(self.n_tx)
+        st = _lib.HemStats()
+        check(_lib.load().hem_solve(self._h, u32p(u), o, f64p(tpm),
+                                    f64p(est), f64p(lam), st))
+        stats = {k: getattr(st, k) for k, _ in _lib.HemStats._fields_}
+        return dict(tpm=tpm, est=est, lam=lam, stats=stats)
+
+    def profile(self, unique, **kw):
+        """Per-kernel device time of one solve (hem_profile_solve).
+
+        Returns {kind: dict(launches, ms_total, ms_avg, bytes_per_launch)}
+        plus 'iters'."""
+        u = as_u32(unique)
+        o = opts(**kw)
+        pr = _lib.HemKernelProfile()
+        check(_lib.load().hem_profile_solve(self._h, u32p(u), o, pr))
+        out = {"iters": pr.iters}
+        for i, name in enumerate(_lib.KERNEL_KIND_NAMES):
+            n = pr.launches[i]
+            out[name] = dict(launches=n, ms_total=pr.ms[i],
+                             ms_avg=pr.ms[i] / n if n else 0.0,
+                             bytes_per_launch=pr.bytes[i])
+        return out
+
+    def loglik(self, unique, lam):
+        u = as_u32(unique)
+        lam = as_f64(lam)
+        out = _lib.C.c_double()
+        check(_lib.load().hem_loglik(self._h, u32p(u), f64p(lam), out))
+        return out.value

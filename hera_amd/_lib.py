@@ -1,0 +1,151 @@
+"""ctypes bindings of libheraem.so (the C ABI in include/hera_em.h).
+
+The shared library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C hera_amd/csrc``).  Loading fails loudly when it is missing: there is
+no CPU fallback for any entry point.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libheraem.so")
+
+HEM_OK = 0
+HEM_ERR_INVALID = 1
+HEM_ERR_NOMEM = 2
+HEM_ERR_DEVICE = 3
+HEM_ERR_NODEVICE = 4
+HEM_ERR_RANGE = 5
+
+_u32p = C.POINTER(C.c_uint32)
+_f64p = C.POINTER(C.c_double)
+
+
+class HemClasses(C.Structure):
+    _fields_ = [("n_tx", C.c_uint32), ("n_cls", C.c_uint32),
+                ("cls_off", _u32p), ("cls_ref", _u32p),
+                ("cls_count", _f64p), ("tx_len", _u32p)]
+
+
+class HemOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("step", C.c_int),
+                ("clamp", C.c_int), ("alpha_clamp", C.c_int),
+                ("min_iter", C.c_uint32), ("max_iter", C.c_uint32),
+                ("stop_min_value", C.c_double),
+                ("stop_rel_change", C.c_double),
+                ("fallback", C.c_int), ("verbose", C.c_int)]
+
+
+class HemStats(C.Structure):
+    _fields_ = [("iters", C.c_uint32), ("maps", C.c_uint32),
+                ("converged", C.c_uint32), ("fallbacks", C.c_uint32),
+                ("loglik", C.c_double), ("last_alpha", C.c_double),
+                ("ms_solve", C.c_double), ("ms_total", C.c_double)]
+
+
+HEM_KERNEL_KINDS = 6
+KERNEL_KIND_NAMES = ["class_pass", "transcript_pass", "norm_pass1",
+                     "norm_pass2", "extrapolate", "iter_end"]
+
+
+class HemKernelProfile(C.Structure):
+    _fields_ = [("iters", C.c_uint32),
+                ("launches", C.c_uint32 * HEM_KERNEL_KINDS),
+                ("ms", C.c_double * HEM_KERNEL_KINDS),
+                ("bytes", C.c_double * HEM_KERNEL_KINDS)]
+
+
+class HemGenParams(C.Structure):
+    _fields_ = [("n_tx", C.c_uint32), ("n_cls_all", C.c_uint32),
+                ("n_frag", C.c_uint64), ("cluster_min", C.c_uint32),
+                ("cluster_max", C.c_uint32), ("len_mu", C.c_double),
+                ("len_sigma", C.c_double), ("len_min", C.c_uint32),
+                ("len_max", C.c_uint32), ("geom_p", C.c_double),
+                ("cls_max", C.c_uint32), ("zipf_s", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+class HemGenProblem(C.Structure):
+    _fields_ = [("n_tx", C.c_uint32), ("n_cls", C.c_uint32),
+                ("nnz", C.c_uint64), ("cls_off", _u32p), ("cls_ref", _u32p),
+                ("cls_count", _f64p), ("tx_len", _u32p), ("unique", _u32p),
+                ("n_clusters", C.c_uint32), ("cluster_off", _u32p),
+                ("frag_total", C.c_uint64)]
+
+
+# Every symbol include/*.h declares, with its ctypes signature.
+SIGNATURES = {
+    "hem_last_error": (C.c_char_p, []),
+    "hem_opts_default": (None, [C.POINTER(HemOpts)]),
+    "hem_problem_create": (C.c_int, [C.POINTER(HemClasses), C.POINTER(C.c_void_p)]),
+    "hem_problem_destroy": (C.c_int, [C.c_void_p]),
+    "hem_problem_info": (C.c_int, [C.c_void_p, _u32p, _u32p,
+                                   C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "hem_solve": (C.c_int, [C.c_void_p, _u32p, C.POINTER(HemOpts), _f64p,
+                            _f64p, _f64p, C.POINTER(HemStats)]),
+    "hem_loglik": (C.c_int, [C.c_void_p, _u32p, _f64p, _f64p]),
+    "hem_profile_solve": (C.c_int, [C.c_void_p, _u32p, C.POINTER(HemOpts),
+                                    C.POINTER(HemKernelProfile)]),
+    "hem_gen_default": (None, [C.POINTER(HemGenParams), C.c_uint32,
+                               C.c_uint32, C.c_uint64, C.c_uint64]),
+    "hem_generate": (C.c_int, [C.POINTER(HemGenParams),
+                               C.POINTER(C.POINTER(HemGenProblem))]),
+    "hem_gen_free": (None, [C.POINTER(HemGenProblem)]),
+    "estimate_count": (C.c_void_p, [_u32p, C.c_uint]),
+    "destroy_emVal": (None, [C.c_void_p]),
+    "hem_dropin_release": (C.c_int, []),
+}
+
+# Global variables the drop-in header declares.
+DATA_SYMBOLS = ["REF_INF", "CLASS"]
+
+_LIB = None
+
+
+def load():
+    """Load libheraem.so once; raise if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "libheraem.so is not built (expected at %s); run "
+            "`python -c 'import __graft_entry__ as g; g.build()'` or "
+            "`make -C hera_amd/csrc`" % LIB_PATH)
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+class HemError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("hera_em error %d: %s" % (code, msg))
+        self.code = code
+
+
+def check(rc):
+    if rc != HEM_OK:
+        raise HemError(rc, load().hem_last_error().decode())
+    return rc
+
+
+def u32p(a):
+    return a.ctypes.data_as(_u32p)
+
+
+def f64p(a):
+    return a.ctypes.data_as(_f64p)
+
+
+def as_u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def as_f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)

@@ -1,0 +1,132 @@
+/*
+ * hem_dev.h -- the thin C ABI between the C host code and the HIP side.
+ *
+ * Everything the host needs from the device goes through these functions:
+ * plain pointers, sizes and int codes (0 = success, otherwise a HIP error
+ * number; hemd_error_string() names it).  The host C files never include a
+ * HIP header.
+ */
+#ifndef HEM_DEV_H_
+#define HEM_DEV_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- runtime ------------------------------------------------------------ */
+int hemd_init(int *device, char *arch, size_t arch_len, int *n_cu);
+const char *hemd_error_string(int code);
+int hemd_malloc(void **p, size_t n);
+int hemd_free(void *p);
+int hemd_host_alloc(void **p, size_t n);   /* pinned */
+int hemd_host_free(void *p);
+int hemd_stream_create(void **s);
+int hemd_stream_destroy(void *s);
+int hemd_stream_sync(void *s);
+int hemd_h2d(void *dst, const void *src, size_t n, void *s);
+int hemd_d2h(void *dst, const void *src, size_t n, void *s);
+int hemd_d2d(void *dst, const void *src, size_t n, void *s);
+int hemd_event_create(void **e);
+int hemd_event_destroy(void *e);
+int hemd_event_record(void *e, void *s);
+int hemd_event_sync(void *e);
+int hemd_event_elapsed(void *e0, void *e1, float *ms);
+int hemd_graph_begin(void *s);
+int hemd_graph_end(void *s, void **exec);
+int hemd_graph_launch(void *exec, void *s);
+int hemd_graph_destroy(void *exec);
+int hemd_last_launch_error(void);
+
+/* ---- solver control block (device memory) -------------------------------
+ * One per EM instance.  Every iteration kernel reads `done` first and does
+ * nothing once it is set, so a captured chunk of iterations can run past
+ * convergence without changing the result. */
+typedef struct {
+	uint32_t k;           /* SQUAREM iterations completed                  */
+	uint32_t done;        /* loop finished                                 */
+	uint32_t maps;        /* EM maps applied                               */
+	uint32_t converged;   /* stop rule fired                               */
+	uint32_t min_iter, max_iter;
+	uint32_t fallbacks;
+	uint32_t pad;
+	double   alpha;       /* last step length                              */
+	double   stop_min, stop_rel;
+} hemk_ctl;
+
+/* Device view of one problem (all pointers are device pointers). */
+typedef struct {
+	uint32_t n_tx, n_cls;
+	uint64_t nnz;
+	const uint32_t *cls_off;   /* n_cls+1, CSR over classes                */
+	const uint32_t *cls_ref;   /* nnz member transcript ids                */
+	const float    *cnt32;     /* n_cls class counts (FP32 copy)           */
+	const double   *cnt64;     /* n_cls class counts                       */
+	const uint32_t *tx_off;    /* n_tx+1, CSC over transcripts             */
+	const uint32_t *tx_ref;    /* nnz class ids, ascending per transcript  */
+	const uint32_t *len;       /* n_tx transcript lengths                  */
+} hemk_prob;
+
+/* Step options fixed for one solve. */
+typedef struct {
+	int step;        /* 0 sample sd, 1 L2                                 */
+	int clamp;       /* 0 zero, 1 u2/10                                   */
+	int alpha_clamp; /* alpha <= -1                                       */
+	int pad;
+} hemk_step;
+
+#define HEMK_NORM_BLOCKS 512
+
+/* ---- kernel launchers --------------------------------------------------- */
+/* lam_init: fill lam64 (and lam32 if non-NULL) with `value`, or copy from
+ * init64 when non-NULL. */
+int hemk_init_state(uint32_t n, double *lam64, float *lam32, double value,
+                    const double *init64, void *s);
+int hemk_ctl_reset(hemk_ctl *ctl, uint32_t min_iter, uint32_t max_iter,
+                   double stop_min, double stop_rel, void *s);
+
+/* CSR pass: rho_c = n_c / sum_{t in c} lam_t (0 if the sum is 0).
+ * prec 0: lam32/rho32; prec 1: lam64/rho64.  group = lanes per class. */
+int hemk_denom(const hemk_prob *p, int prec, const void *lam, void *rho,
+               int group, const hemk_ctl *ctl, void *s);
+
+/* CSC pass + M-step: s_t = sum_{c ni t} rho_c, out_t = (u_t + lam_t s_t) /
+ * len_t; FP32 shadow written when out32 != NULL; when stop_flags != NULL
+ * each block stores whether any of its transcripts fails the stop test. */
+int hemk_gather_mstep(const hemk_prob *p, int prec, const void *rho,
+                      const uint32_t *unique, const double *lam64,
+                      double *out64, float *out32, uint32_t *stop_flags,
+                      int group, const hemk_ctl *ctl, void *s);
+uint32_t hemk_gather_blocks(const hemk_prob *p, int group);
+
+/* SQUAREM: norms of r = u1-u, v = (u2-u1)-r, step length, extrapolation. */
+int hemk_norm_pass1(uint32_t n, const double *u, const double *u1,
+                    const double *u2, double *part1, int step,
+                    const hemk_ctl *ctl, void *s);
+int hemk_norm_pass2(uint32_t n, const double *u, const double *u1,
+                    const double *u2, const double *part1, double *part2,
+                    int step, const hemk_ctl *ctl, void *s);
+int hemk_extrapolate(uint32_t n, const double *u, const double *u1,
+                     const double *u2, const double *part2, double *out64,
+                     float *out32, hemk_step st, hemk_ctl *ctl, void *s);
+
+/* End of one iteration: k++, evaluate the loop condition of src/EM.c:328
+ * from the stop flags of the third map. */
+int hemk_iter_end(hemk_ctl *ctl, const uint32_t *stop_flags,
+                  uint32_t n_flags, void *s);
+
+/* Finalisation (src/EM.c:362-373): est_t = lam_t len_t, tpm_t =
+ * lam_t 1e6 / sum lam.  part: HEMK_NORM_BLOCKS doubles of scratch. */
+int hemk_finalize(uint32_t n, const double *lam, const uint32_t *len,
+                  double *part, double *tpm, double *est, void *s);
+
+/* FP64 log-likelihood; part: HEMK_NORM_BLOCKS*2 doubles; result in *out. */
+int hemk_loglik(const hemk_prob *p, const uint32_t *unique, const double *lam,
+                double *part, double *out, void *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,55 @@
+/* error.c -- status codes, per-thread messages, device bring-up. */
+#include <pthread.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "hem_internal.h"
+
+static __thread char hem_msg[512];
+
+const char *hem_last_error(void) { return hem_msg; }
+
+int hem_fail(int code, const char *fmt, ...)
+{
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(hem_msg, sizeof(hem_msg), fmt, ap);
+	va_end(ap);
+	return code;
+}
+
+int hem_dev_fail(int hip_code, const char *what)
+{
+	return hem_fail(HEM_ERR_DEVICE, "%s: HIP error %d (%s)", what, hip_code,
+	                hemd_error_string(hip_code));
+}
+
+static pthread_once_t hem_once = PTHREAD_ONCE_INIT;
+static int hem_init_status = -1;
+static char hem_arch[64];
+
+static void hem_init_once(void)
+{
+	int dev = 0, n_cu = 0;
+	int e = hemd_init(&dev, hem_arch, sizeof(hem_arch), &n_cu);
+	if (e) {
+		hem_init_status = e;
+		return;
+	}
+	/* The code objects are built for gfx950 only. */
+	hem_init_status = strncmp(hem_arch, "gfx950", 6) == 0 ? 0 : -2;
+}
+
+int hem_device_ready(void)
+{
+	pthread_once(&hem_once, hem_init_once);
+	if (hem_init_status == -2)
+		return hem_fail(HEM_ERR_NODEVICE,
+		                "device is %s; this library is built for gfx950",
+		                hem_arch);
+	if (hem_init_status != 0)
+		return hem_fail(HEM_ERR_NODEVICE, "no HIP device: %s",
+		                hemd_error_string(hem_init_status));
+	return HEM_OK;
+}

@@ -1,0 +1,290 @@
+/*
+ * problem.c -- validate a class table, transpose it, upload it.
+ *
+ * The reference walks CLASS->cls[i].ref[k] for every class on every map
+ * (src/EM.c:276-286).  Here the table is flattened once into a CSR over
+ * classes (members in the caller's order, duplicates kept) and its CSC
+ * transpose over transcripts (class ids ascending, a class listed once per
+ * occurrence of the transcript), both u32 and device-resident for the life
+ * of the handle.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "hem_internal.h"
+
+static int validate(const hem_classes *c)
+{
+	uint32_t i;
+	uint64_t nnz;
+	if (!c || !c->tx_len || (c->n_cls && (!c->cls_off || !c->cls_count)))
+		return hem_fail(HEM_ERR_INVALID, "null class table pointer");
+	if (c->n_tx == 0)
+		return hem_fail(HEM_ERR_INVALID, "n_tx must be > 0");
+	if (c->n_cls && c->cls_off[0] != 0)
+		return hem_fail(HEM_ERR_INVALID, "cls_off[0] must be 0");
+	for (i = 0; i < c->n_cls; ++i)
+		if (c->cls_off[i + 1] < c->cls_off[i])
+			return hem_fail(HEM_ERR_INVALID,
+			                "cls_off not monotone at class %u", i);
+	nnz = c->n_cls ? c->cls_off[c->n_cls] : 0;
+	if (nnz && !c->cls_ref)
+		return hem_fail(HEM_ERR_INVALID, "null cls_ref");
+	for (uint64_t k = 0; k < nnz; ++k)
+		if (c->cls_ref[k] >= c->n_tx)
+			return hem_fail(HEM_ERR_INVALID,
+			                "transcript id %u >= n_tx %u at member %llu",
+			                c->cls_ref[k], c->n_tx,
+			                (unsigned long long)k);
+	for (i = 0; i < c->n_cls; ++i)
+		if (!(c->cls_count[i] >= 0) || !isfinite(c->cls_count[i]))
+			return hem_fail(HEM_ERR_INVALID,
+			                "class %u count is negative or not finite",
+			                i);
+	for (i = 0; i < c->n_tx; ++i)
+		if (c->tx_len[i] == 0)
+			return hem_fail(HEM_ERR_INVALID,
+			                "transcript %u has length 0", i);
+	return HEM_OK;
+}
+
+static int dalloc(hem_problem *p, void **ptr, size_t n)
+{
+	int e = hemd_malloc(ptr, n);
+	if (e) {
+		*ptr = NULL;
+		return hem_fail(HEM_ERR_NOMEM, "device allocation of %zu bytes: %s",
+		                n, hemd_error_string(e));
+	}
+	p->dev_bytes += n;
+	return HEM_OK;
+}
+
+int hem_problem_create(const hem_classes *c, hem_problem **out)
+{
+	int rc;
+	hem_problem *p;
+	uint32_t *tx_off = NULL, *tx_ref = NULL, *pos = NULL;
+	float *cnt32 = NULL;
+	double *nd = NULL;
+	uint64_t nnz;
+	uint32_t T, C, i;
+
+	if (!out)
+		return hem_fail(HEM_ERR_INVALID, "null output handle");
+	*out = NULL;
+	if ((rc = validate(c)))
+		return rc;
+	if ((rc = hem_device_ready()))
+		return rc;
+	T = c->n_tx;
+	C = c->n_cls;
+	nnz = C ? c->cls_off[C] : 0;
+
+	/* CSC transpose by counting sort: class ids come out ascending per
+	 * transcript, duplicates adjacent. */
+	tx_off = calloc((size_t)T + 1, sizeof(uint32_t));
+	pos = malloc((size_t)T * sizeof(uint32_t));
+	tx_ref = malloc((nnz ? nnz : 1) * sizeof(uint32_t));
+	cnt32 = malloc((C ? C : 1) * sizeof(float));
+	nd = calloc((size_t)(C ? C : 1) * 2, sizeof(double));
+	if (!tx_off || !pos || !tx_ref || !cnt32 || !nd) {
+		free(tx_off);
+		free(pos);
+		free(tx_ref);
+		free(cnt32);
+		free(nd);
+		return hem_fail(HEM_ERR_NOMEM, "host allocation failed");
+	}
+	for (uint64_t k = 0; k < nnz; ++k)
+		tx_off[c->cls_ref[k] + 1]++;
+	for (i = 0; i < T; ++i)
+		tx_off[i + 1] += tx_off[i];
+	memcpy(pos, tx_off, (size_t)T * sizeof(uint32_t));
+	for (i = 0; i < C; ++i)
+		for (uint32_t k = c->cls_off[i]; k < c->cls_off[i + 1]; ++k)
+			tx_ref[pos[c->cls_ref[k]]++] = i;
+	for (i = 0; i < C; ++i) {
+		cnt32[i] = (float)c->cls_count[i];
+		nd[2 * (size_t)i] = c->cls_count[i];
+	}
+	free(pos);
+
+	p = calloc(1, sizeof(*p));
+	if (!p) {
+		free(tx_off);
+		free(tx_ref);
+		free(cnt32);
+		free(nd);
+		return hem_fail(HEM_ERR_NOMEM, "host allocation failed");
+	}
+	pthread_mutex_init(&p->mu, NULL);
+	p->n_tx = T;
+	p->n_cls = C;
+	p->nnz = nnz;
+	p->group_cls = 4;
+	p->group_tx = 16;
+	p->graph_key = -1;
+
+	rc = hemd_stream_create(&p->stream);
+	if (!rc)
+		rc = hemd_event_create(&p->ev0);
+	if (!rc)
+		rc = hemd_event_create(&p->ev1);
+	if (rc) {
+		free(tx_off);
+		free(tx_ref);
+		free(cnt32);
+		free(nd);
+		hem_problem_destroy(p);
+		return hem_dev_fail(rc, "stream/event creation");
+	}
+
+#define DALLOC_H(ptr, n)                                           \
+	do {                                                       \
+		int r_ = dalloc(p, (void **)&(ptr), (n));          \
+		if (r_) {                                          \
+			free(tx_off);                              \
+			free(tx_ref);                              \
+			free(cnt32);                               \
+			free(nd);                                  \
+			hem_problem_destroy(p);                    \
+			return r_;                                 \
+		}                                                  \
+	} while (0)
+	DALLOC_H(p->d_cls_off, ((size_t)C + 1) * 4);
+	DALLOC_H(p->d_cls_ref, (nnz ? nnz : 1) * 4);
+	DALLOC_H(p->d_cnt32, (C ? C : 1) * 4);
+	DALLOC_H(p->d_cnt64, (C ? C : 1) * 8);
+	DALLOC_H(p->d_nd, (C ? C : 1) * 16);
+	DALLOC_H(p->d_tx_off, ((size_t)T + 1) * 4);
+	DALLOC_H(p->d_tx_ref, (nnz ? nnz : 1) * 4);
+	DALLOC_H(p->d_len, (size_t)T * 4);
+	DALLOC_H(p->d_A, (size_t)T * 8);
+	DALLOC_H(p->d_B, (size_t)T * 8);
+	DALLOC_H(p->d_C, (size_t)T * 8);
+	DALLOC_H(p->d_D, (size_t)T * 8);
+	DALLOC_H(p->d_A32, (size_t)T * 4);
+	DALLOC_H(p->d_B32, (size_t)T * 4);
+	DALLOC_H(p->d_D32, (size_t)T * 4);
+	DALLOC_H(p->d_rho, (C ? C : 1) * 8);
+	DALLOC_H(p->d_unique, (size_t)T * 4);
+	p->n_flags = hemk_gather_blocks(&(hemk_prob){.n_tx = T}, 1);
+	DALLOC_H(p->d_flags, (size_t)p->n_flags * 4);
+	DALLOC_H(p->d_part1, HEMK_NORM_BLOCKS * 2 * 8);
+	DALLOC_H(p->d_part2, HEMK_NORM_BLOCKS * 2 * 8);
+	DALLOC_H(p->d_out, 64);
+	DALLOC_H(p->d_ctl, sizeof(hemk_ctl));
+#undef DALLOC_H
+	rc = hemd_host_alloc((void **)&p->h_ctl, sizeof(hemk_ctl));
+	if (rc) {
+		free(tx_off);
+		free(tx_ref);
+		free(cnt32);
+		free(nd);
+		hem_problem_destroy(p);
+		return hem_dev_fail(rc, "pinned allocation");
+	}
+
+	if (C) {
+		rc = hemd_h2d(p->d_cls_off, c->cls_off, ((size_t)C + 1) * 4,
+		              p->stream);
+		if (!rc)
+			rc = hemd_h2d(p->d_cls_ref, c->cls_ref, nnz * 4, p->stream);
+		if (!rc)
+			rc = hemd_h2d(p->d_cnt64, c->cls_count, (size_t)C * 8,
+			              p->stream);
+		if (!rc)
+			rc = hemd_h2d(p->d_cnt32, cnt32, (size_t)C * 4, p->stream);
+		if (!rc)
+			rc = hemd_h2d(p->d_nd, nd, (size_t)C * 16, p->stream);
+		if (!rc)
+			rc = hemd_h2d(p->d_tx_ref, tx_ref, nnz * 4, p->stream);
+	}
+	if (!rc)
+		rc = hemd_h2d(p->d_tx_off, tx_off, ((size_t)T + 1) * 4,
+		              p->stream);
+	if (!rc)
+		rc = hemd_h2d(p->d_len, c->tx_len, (size_t)T * 4, p->stream);
+	if (!rc)
+		rc = hemd_stream_sync(p->stream);
+	free(tx_off);
+	free(tx_ref);
+	free(cnt32);
+	free(nd);
+	if (rc) {
+		hem_problem_destroy(p);
+		return hem_dev_fail(rc, "class table upload");
+	}
+
+	p->dp.n_tx = T;
+	p->dp.n_cls = C;
+	p->dp.nnz = nnz;
+	p->dp.cls_off = p->d_cls_off;
+	p->dp.cls_ref = p->d_cls_ref;
+	p->dp.cnt32 = p->d_cnt32;
+	p->dp.cnt64 = p->d_cnt64;
+	p->dp.tx_off = p->d_tx_off;
+	p->dp.tx_ref = p->d_tx_ref;
+	p->dp.len = p->d_len;
+	*out = p;
+	return HEM_OK;
+}
+
+int hem_problem_destroy(hem_problem *p)
+{
+	if (!p)
+		return HEM_OK;
+	if (p->stream)
+		hemd_stream_sync(p->stream);
+	hemd_graph_destroy(p->graph);
+	hemd_free(p->d_cls_off);
+	hemd_free(p->d_cls_ref);
+	hemd_free(p->d_cnt32);
+	hemd_free(p->d_cnt64);
+	hemd_free(p->d_nd);
+	hemd_free(p->d_tx_off);
+	hemd_free(p->d_tx_ref);
+	hemd_free(p->d_len);
+	hemd_free(p->d_A);
+	hemd_free(p->d_B);
+	hemd_free(p->d_C);
+	hemd_free(p->d_D);
+	hemd_free(p->d_A32);
+	hemd_free(p->d_B32);
+	hemd_free(p->d_D32);
+	hemd_free(p->d_rho);
+	hemd_free(p->d_unique);
+	hemd_free(p->d_flags);
+	hemd_free(p->d_part1);
+	hemd_free(p->d_part2);
+	hemd_free(p->d_out);
+	hemd_free(p->d_ctl);
+	hemd_host_free(p->h_ctl);
+	if (p->ev0)
+		hemd_event_destroy(p->ev0);
+	if (p->ev1)
+		hemd_event_destroy(p->ev1);
+	if (p->stream)
+		hemd_stream_destroy(p->stream);
+	pthread_mutex_destroy(&p->mu);
+	free(p);
+	return HEM_OK;
+}
+
+int hem_problem_info(const hem_problem *p, uint32_t *n_tx, uint32_t *n_cls,
+                     uint64_t *nnz, uint64_t *device_bytes)
+{
+	if (!p)
+		return hem_fail(HEM_ERR_INVALID, "null problem");
+	if (n_tx)
+		*n_tx = p->n_tx;
+	if (n_cls)
+		*n_cls = p->n_cls;
+	if (nnz)
+		*nnz = p->nnz;
+	if (device_bytes)
+		*device_bytes = p->dev_bytes;
+	return HEM_OK;
+}

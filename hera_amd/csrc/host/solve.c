@@ -1,0 +1,403 @@
+/*
+ * solve.c -- the SQUAREM driver of one EM instance (src/EM.c:315-387).
+ *
+ * One iteration of the reference loop (src/EM.c:329-359) is captured once
+ * into a HIP graph, HEM_ITERS_PER_GRAPH times over, and the graph is
+ * replayed until the device-side control block reports that the loop
+ * condition of src/EM.c:328 has failed.  Buffers rotate in a fixed pattern
+ * so every iteration has the same pointers:
+ *   A = u (iterate)   B = u1 = M(A)   C = u2 = M(B)
+ *   D = extrapolated, clamped point   A <- M(D)  (stop test on this map)
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+#include <time.h>
+
+#include "hem_internal.h"
+
+void hem_opts_default(hem_opts *o)
+{
+	if (!o)
+		return;
+	memset(o, 0, sizeof(*o));
+	o->precision = HEM_PREC_FP64;
+	o->step = HEM_STEP_SD;
+	o->clamp = HEM_CLAMP_ZERO;
+	o->alpha_clamp = 1;
+	o->min_iter = 50;
+	o->max_iter = 1000;
+	o->stop_min_value = 1e-3;
+	o->stop_rel_change = 0.01;
+	o->fallback = 0;
+	o->verbose = 0;
+}
+
+static double now_ms(void)
+{
+	struct timespec ts;
+	clock_gettime(CLOCK_MONOTONIC, &ts);
+	return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+static int check_opts(const hem_opts *o)
+{
+	if (o->precision != HEM_PREC_FP32 && o->precision != HEM_PREC_FP64)
+		return hem_fail(HEM_ERR_INVALID, "unknown precision %d",
+		                o->precision);
+	if (o->step != HEM_STEP_SD && o->step != HEM_STEP_L2)
+		return hem_fail(HEM_ERR_INVALID, "unknown step form %d", o->step);
+	if (o->clamp != HEM_CLAMP_ZERO && o->clamp != HEM_CLAMP_DIV10)
+		return hem_fail(HEM_ERR_INVALID, "unknown clamp form %d",
+		                o->clamp);
+	if (o->fallback)
+		return hem_fail(HEM_ERR_INVALID,
+		                "the monotonicity fallback is not available "
+		                "in this build");
+	return HEM_OK;
+}
+
+static int graph_key(const hem_problem *p, const hem_opts *o)
+{
+	return o->precision | (o->step << 1) | (o->clamp << 2) |
+	       ((o->alpha_clamp != 0) << 3) | (p->group_cls << 4) |
+	       (p->group_tx << 12);
+}
+
+/* Optional per-kernel timing: an event is recorded after every launch. */
+typedef struct {
+	void *ev[16];
+	int kind[16];
+	int n;
+} prof_ctx;
+
+#define MARK(kd)                                                        \
+	do {                                                            \
+		if (pc) {                                               \
+			HEM_TRY_DEV(hemd_event_record(pc->ev[pc->n], s),\
+			            "event");                           \
+			pc->kind[pc->n++] = (kd);                       \
+		}                                                       \
+	} while (0)
+
+/* Enqueue one SQUAREM iteration on the problem's stream. */
+static int enqueue_iteration(hem_problem *p, const hem_opts *o, prof_ctx *pc)
+{
+	const int f32 = o->precision == HEM_PREC_FP32;
+	const int prec = f32 ? 0 : 1;
+	const void *gA = f32 ? (const void *)p->d_A32 : (const void *)p->d_A;
+	const void *gB = f32 ? (const void *)p->d_B32 : (const void *)p->d_B;
+	const void *gD = f32 ? (const void *)p->d_D32 : (const void *)p->d_D;
+	void *rho = f32 ? p->d_rho : (void *)p->d_nd;
+	hemk_step st = {o->step, o->clamp, o->alpha_clamp != 0, 0};
+	const hemk_prob *dp = &p->dp;
+	void *s = p->stream;
+	uint32_t nfl = hemk_gather_blocks(dp, p->group_tx);
+	int gc = p->group_cls, gt = p->group_tx;
+
+	HEM_TRY_DEV(hemk_denom(dp, prec, gA, rho, gc, p->d_ctl, s),
+	            "denom");
+	MARK(HEM_PK_DENOM);
+	HEM_TRY_DEV(hemk_gather_mstep(dp, prec, rho, p->d_unique, p->d_A,
+	                              p->d_B, f32 ? p->d_B32 : NULL, NULL, gt,
+	                              p->d_ctl, s),
+	            "gather");
+	MARK(HEM_PK_GATHER);
+	HEM_TRY_DEV(hemk_denom(dp, prec, gB, rho, gc, p->d_ctl, s),
+	            "denom");
+	MARK(HEM_PK_DENOM);
+	HEM_TRY_DEV(hemk_gather_mstep(dp, prec, rho, p->d_unique, p->d_B,
+	                              p->d_C, NULL, NULL, gt, p->d_ctl, s),
+	            "gather");
+	MARK(HEM_PK_GATHER);
+	HEM_TRY_DEV(hemk_norm_pass1(p->n_tx, p->d_A, p->d_B, p->d_C,
+	                            p->d_part1, o->step, p->d_ctl, s),
+	            "norm1");
+	MARK(HEM_PK_NORM1);
+	if (o->step == HEM_STEP_SD) {
+		HEM_TRY_DEV(hemk_norm_pass2(p->n_tx, p->d_A, p->d_B, p->d_C,
+		                            p->d_part1, p->d_part2, o->step,
+		                            p->d_ctl, s),
+		            "norm2");
+		MARK(HEM_PK_NORM2);
+	}
+	HEM_TRY_DEV(hemk_extrapolate(p->n_tx, p->d_A, p->d_B, p->d_C,
+	                             o->step == HEM_STEP_SD ? p->d_part2
+	                                                    : p->d_part1,
+	                             p->d_D, f32 ? p->d_D32 : NULL, st,
+	                             p->d_ctl, s),
+	            "extrapolate");
+	MARK(HEM_PK_EXTRAP);
+	HEM_TRY_DEV(hemk_denom(dp, prec, gD, rho, gc, p->d_ctl, s),
+	            "denom");
+	MARK(HEM_PK_DENOM);
+	HEM_TRY_DEV(hemk_gather_mstep(dp, prec, rho, p->d_unique, p->d_D,
+	                              p->d_A, f32 ? p->d_A32 : NULL,
+	                              p->d_flags, gt, p->d_ctl, s),
+	            "gather");
+	MARK(HEM_PK_GATHER);
+	HEM_TRY_DEV(hemk_iter_end(p->d_ctl, p->d_flags, nfl, s), "iter_end");
+	MARK(HEM_PK_ITER_END);
+	return HEM_OK;
+}
+
+static int ensure_graph(hem_problem *p, const hem_opts *o)
+{
+	int key = graph_key(p, o), rc, e;
+	void *exec = NULL;
+	if (p->graph && p->graph_key == key)
+		return HEM_OK;
+	hemd_graph_destroy(p->graph);
+	p->graph = NULL;
+	e = hemd_graph_begin(p->stream);
+	if (e)
+		return hem_dev_fail(e, "graph capture begin");
+	for (int i = 0; i < HEM_ITERS_PER_GRAPH; ++i) {
+		rc = enqueue_iteration(p, o, NULL);
+		if (rc) {
+			hemd_graph_end(p->stream, &exec);
+			hemd_graph_destroy(exec);
+			return rc;
+		}
+	}
+	e = hemd_graph_end(p->stream, &exec);
+	if (e)
+		return hem_dev_fail(e, "graph capture end");
+	p->graph = exec;
+	p->graph_key = key;
+	return HEM_OK;
+}
+
+/* Runs the iteration loop on the device from the state already in A. */
+static int run_loop(hem_problem *p, const hem_opts *o, float *ms)
+{
+	int rc;
+	uint32_t launched = 0;
+	const uint32_t limit = o->max_iter > o->min_iter ? o->max_iter
+	                                                  : o->min_iter;
+	if ((rc = ensure_graph(p, o)))
+		return rc;
+	HEM_TRY_DEV(hemd_event_record(p->ev0, p->stream), "event");
+	for (;;) {
+		HEM_TRY_DEV(hemd_graph_launch(p->graph, p->stream),
+		            "graph launch");
+		launched += HEM_ITERS_PER_GRAPH;
+		if (launched < o->min_iter && launched < limit)
+			continue;
+		HEM_TRY_DEV(hemd_d2h(p->h_ctl, p->d_ctl, sizeof(hemk_ctl),
+		                     p->stream),
+		            "control readback");
+		HEM_TRY_DEV(hemd_stream_sync(p->stream), "stream sync");
+		if (p->h_ctl->done)
+			break;
+		if (launched > limit + HEM_ITERS_PER_GRAPH)
+			return hem_fail(HEM_ERR_DEVICE,
+			                "iteration loop did not terminate");
+	}
+	HEM_TRY_DEV(hemd_event_record(p->ev1, p->stream), "event");
+	HEM_TRY_DEV(hemd_event_sync(p->ev1), "event sync");
+	HEM_TRY_DEV(hemd_event_elapsed(p->ev0, p->ev1, ms), "event time");
+	return HEM_OK;
+}
+
+/* Uploads the unique counts, sets the uniform start (src/EM.c:311-312)
+ * and resets the control block. */
+static int solve_setup(hem_problem *p, const uint32_t *unique,
+                       const hem_opts *o)
+{
+	int rc = hemd_h2d(p->d_unique, unique, (size_t)p->n_tx * 4, p->stream);
+	if (!rc)
+		rc = hemk_init_state(p->n_tx, p->d_A, p->d_A32,
+		                     (double)1 / p->n_tx, NULL, p->stream);
+	if (!rc)
+		rc = hemk_ctl_reset(p->d_ctl, o->min_iter, o->max_iter,
+		                    o->stop_min_value, o->stop_rel_change,
+		                    p->stream);
+	return rc ? hem_dev_fail(rc, "solve setup") : HEM_OK;
+}
+
+static int load_opts(const hem_opts *opts, hem_opts *o)
+{
+	if (opts)
+		*o = *opts;
+	else
+		hem_opts_default(o);
+	return check_opts(o);
+}
+
+int hem_solve(hem_problem *p, const uint32_t *unique, const hem_opts *opts,
+              double *tpm, double *est, double *lambda, hem_stats *st)
+{
+	hem_opts o;
+	int rc;
+	float ms = 0;
+	double t0 = now_ms(), ll = 0;
+	if (!p || !unique)
+		return hem_fail(HEM_ERR_INVALID, "null problem or counts");
+	if ((rc = load_opts(opts, &o)))
+		return rc;
+
+	pthread_mutex_lock(&p->mu);
+	if ((rc = solve_setup(p, unique, &o)))
+		goto out;
+	if ((rc = run_loop(p, &o, &ms)))
+		goto out;
+	/* finalisation into B (est) and C (tpm); log-likelihood of A */
+	rc = hemk_finalize(p->n_tx, p->d_A, p->d_len, p->d_part1, p->d_C,
+	                   p->d_B, p->stream);
+	if (!rc)
+		rc = hemk_loglik(&p->dp, p->d_unique, p->d_A, p->d_part2,
+		                 p->d_out, p->stream);
+	if (!rc && tpm)
+		rc = hemd_d2h(tpm, p->d_C, (size_t)p->n_tx * 8, p->stream);
+	if (!rc && est)
+		rc = hemd_d2h(est, p->d_B, (size_t)p->n_tx * 8, p->stream);
+	if (!rc && lambda)
+		rc = hemd_d2h(lambda, p->d_A, (size_t)p->n_tx * 8, p->stream);
+	if (!rc)
+		rc = hemd_d2h(p->h_ctl, p->d_ctl, sizeof(hemk_ctl), p->stream);
+	if (!rc)
+		rc = hemd_d2h(&ll, p->d_out, sizeof(double), p->stream);
+	if (!rc)
+		rc = hemd_stream_sync(p->stream);
+	if (rc) {
+		rc = hem_dev_fail(rc, "finalisation");
+		goto out;
+	}
+	if (st) {
+		memset(st, 0, sizeof(*st));
+		st->iters = p->h_ctl->k;
+		st->maps = p->h_ctl->maps;
+		st->converged = p->h_ctl->converged;
+		st->fallbacks = p->h_ctl->fallbacks;
+		st->last_alpha = p->h_ctl->alpha;
+		st->loglik = ll;
+		st->ms_solve = ms;
+		st->ms_total = now_ms() - t0;
+	}
+	if (o.verbose)
+		fprintf(stderr, "Finish EM with %u rounds\n", p->h_ctl->k);
+out:
+	pthread_mutex_unlock(&p->mu);
+	return rc;
+}
+
+/* Algorithmic bytes per launch of each kernel kind: compulsory traffic with
+ * every array element touched once (gathered vectors counted once). */
+static void kernel_bytes(const hem_problem *p, int f32, double *b)
+{
+	const double T = p->n_tx, C = p->n_cls, Z = (double)p->nnz;
+	if (f32) {
+		/* counts, densities, rho = n/D all FP32 */
+		b[HEM_PK_DENOM] = 4 * (C + 1) + 4 * Z + 4 * C + 4 * T + 4 * C;
+		b[HEM_PK_GATHER] = 4 * (T + 1) + 4 * Z + 4 * C /* rho */ +
+		                   4 * T /* unique */ + 4 * T /* len */ +
+		                   8 * T /* lam */ + 8 * T /* out */ +
+		                   4 * T /* FP32 shadow */;
+	} else {
+		/* densities FP64, D_c written, {n_c, D_c} pairs gathered */
+		b[HEM_PK_DENOM] = 4 * (C + 1) + 4 * Z + 8 * T + 8 * C;
+		b[HEM_PK_GATHER] = 4 * (T + 1) + 4 * Z + 16 * C +
+		                   4 * T + 4 * T + 8 * T + 8 * T;
+	}
+	b[HEM_PK_NORM1] = 24 * T;
+	b[HEM_PK_NORM2] = 24 * T;
+	b[HEM_PK_EXTRAP] = 24 * T + 8 * T + (f32 ? 4 * T : 0);
+	b[HEM_PK_ITER_END] = 4.0 * hemk_gather_blocks(&p->dp, p->group_tx);
+}
+
+int hem_profile_solve(hem_problem *p, const uint32_t *unique,
+                      const hem_opts *opts, hem_kernel_profile *out)
+{
+	hem_opts o;
+	prof_ctx pc;
+	void *ev_start = NULL;
+	int rc, i;
+	if (!p || !unique || !out)
+		return hem_fail(HEM_ERR_INVALID, "null argument");
+	if ((rc = load_opts(opts, &o)))
+		return rc;
+	memset(out, 0, sizeof(*out));
+	memset(&pc, 0, sizeof(pc));
+	kernel_bytes(p, o.precision == HEM_PREC_FP32, out->bytes);
+	for (i = 0; i < 16 && !rc; ++i)
+		rc = hemd_event_create(&pc.ev[i]);
+	if (!rc)
+		rc = hemd_event_create(&ev_start);
+	if (rc) {
+		rc = hem_dev_fail(rc, "event creation");
+		goto free_ev;
+	}
+	pthread_mutex_lock(&p->mu);
+	if ((rc = solve_setup(p, unique, &o)))
+		goto out;
+	for (;;) {
+		int e;
+		pc.n = 0;
+		e = hemd_event_record(ev_start, p->stream);
+		if (e) {
+			rc = hem_dev_fail(e, "event");
+			goto out;
+		}
+		if ((rc = enqueue_iteration(p, &o, &pc)))
+			goto out;
+		e = hemd_d2h(p->h_ctl, p->d_ctl, sizeof(hemk_ctl), p->stream);
+		if (!e)
+			e = hemd_stream_sync(p->stream);
+		if (e) {
+			rc = hem_dev_fail(e, "profiled iteration");
+			goto out;
+		}
+		for (i = 0; i < pc.n; ++i) {
+			float ms = 0;
+			e = hemd_event_elapsed(i ? pc.ev[i - 1] : ev_start, pc.ev[i],
+			                       &ms);
+			if (e) {
+				rc = hem_dev_fail(e, "event time");
+				goto out;
+			}
+			out->ms[pc.kind[i]] += ms;
+			out->launches[pc.kind[i]]++;
+		}
+		out->iters++;
+		if (p->h_ctl->done)
+			break;
+		if (out->iters > o.max_iter + o.min_iter + 1) {
+			rc = hem_fail(HEM_ERR_DEVICE,
+			              "profiled loop did not terminate");
+			goto out;
+		}
+	}
+	out->iters = p->h_ctl->k;
+out:
+	pthread_mutex_unlock(&p->mu);
+free_ev:
+	for (i = 0; i < 16; ++i)
+		if (pc.ev[i])
+			hemd_event_destroy(pc.ev[i]);
+	if (ev_start)
+		hemd_event_destroy(ev_start);
+	return rc;
+}
+
+int hem_loglik(hem_problem *p, const uint32_t *unique, const double *lambda,
+               double *loglik)
+{
+	int rc;
+	if (!p || !unique || !lambda || !loglik)
+		return hem_fail(HEM_ERR_INVALID, "null argument");
+	pthread_mutex_lock(&p->mu);
+	rc = hemd_h2d(p->d_unique, unique, (size_t)p->n_tx * 4, p->stream);
+	if (!rc)
+		rc = hemd_h2d(p->d_D, lambda, (size_t)p->n_tx * 8, p->stream);
+	if (!rc)
+		rc = hemk_loglik(&p->dp, p->d_unique, p->d_D, p->d_part2,
+		                 p->d_out, p->stream);
+	if (!rc)
+		rc = hemd_d2h(loglik, p->d_out, sizeof(double), p->stream);
+	if (!rc)
+		rc = hemd_stream_sync(p->stream);
+	pthread_mutex_unlock(&p->mu);
+	return rc ? hem_dev_fail(rc, "log-likelihood") : HEM_OK;
+}

@@ -1,0 +1,150 @@
+/*
+ * hera_em.h -- C ABI of the MI355X SQUAREM EM abundance estimator.
+ *
+ * Plain C: pointers, sizes and int status codes; no HIP or torch types.
+ * All entry points are thread-safe with respect to distinct hem_problem
+ * handles; calls on one handle are serialised internally.
+ *
+ * Reference interfaces replaced (citations into bioturing/hera):
+ *   estimate_count  src/EM.h:28, defined src/EM.c:315-387 (declared in
+ *                   hera_dropin.h, which reads hera's globals)
+ *   M_step/E_step   src/EM.c:271-313  -> device kernels driven by hem_solve
+ *   Norm            src/hash_align.c:150-164 -> FP64 device reduction
+ *   bootstrap loop  src/EM.c:123-150, 233-261 -> hem_bootstrap
+ */
+#ifndef HERA_EM_H_
+#define HERA_EM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define HEM_OK             0
+#define HEM_ERR_INVALID    1  /* bad argument or malformed class table      */
+#define HEM_ERR_NOMEM      2  /* host or device allocation failed           */
+#define HEM_ERR_DEVICE     3  /* HIP runtime error                          */
+#define HEM_ERR_NODEVICE   4  /* no usable gfx950 device                    */
+#define HEM_ERR_RANGE      5  /* size beyond a documented limit             */
+
+/* Last error message of the calling thread (static storage, never NULL). */
+const char *hem_last_error(void);
+
+/* ---- problem description ----------------------------------------------- */
+
+/* The class table in CSR form.  Equivalent to hera's Gclass + Ref_inf:
+ * class c has members cls_ref[cls_off[c] .. cls_off[c+1]) (u32 transcript
+ * ids < n_tx, duplicates allowed, any order) and cls_count[c] fragments.
+ * tx_len[t] is REF_INF->len[t] (must be > 0). */
+typedef struct {
+	uint32_t        n_tx;
+	uint32_t        n_cls;
+	const uint32_t *cls_off;    /* n_cls + 1 */
+	const uint32_t *cls_ref;    /* cls_off[n_cls] */
+	const double   *cls_count;  /* n_cls */
+	const uint32_t *tx_len;     /* n_tx */
+} hem_classes;
+
+/* Bulk E-step precision.  FP32: gathered densities, class denominators and
+ * per-transcript sums in FP32; the M-step, SQUAREM state, norms, stop test
+ * and log-likelihood stay FP64.  FP64: everything FP64. */
+#define HEM_PREC_FP32 0
+#define HEM_PREC_FP64 1
+
+/* SQUAREM step length.  SD: alpha = -sd(r)/sd(v) with the sample standard
+ * deviation, as src/EM.c:338-339 with Norm (src/hash_align.c:150-164).
+ * L2: alpha = -||r||/||v|| (hera-EM/utility.c:16-41). */
+#define HEM_STEP_SD 0
+#define HEM_STEP_L2 1
+
+/* Treatment of extrapolated values.  ZERO: negatives become 0
+ * (src/EM.c:351).  DIV10: values <= 1e-300 become u2/10
+ * (hera-EM/utility.c:50-53). */
+#define HEM_CLAMP_ZERO  0
+#define HEM_CLAMP_DIV10 1
+
+typedef struct {
+	int      precision;       /* HEM_PREC_*; default FP32                  */
+	int      step;            /* HEM_STEP_*; default SD                    */
+	int      clamp;           /* HEM_CLAMP_*; default ZERO                 */
+	int      alpha_clamp;     /* 1: alpha = min(alpha, -1) (src/EM.c:346)  */
+	uint32_t min_iter;        /* 50   (src/EM.c:325)                       */
+	uint32_t max_iter;        /* 1000 (src/EM.c:326)                       */
+	double   stop_min_value;  /* 1e-3 (src/EM.c:292)                       */
+	double   stop_rel_change; /* 0.01 (src/EM.c:295); e.g. 1e-8 for a
+	                             tolerance-mode run                        */
+	int      fallback;        /* 1: monotonicity fallback to the plain EM
+	                             point when the extrapolated point lowers
+	                             the log-likelihood; 0 for parity runs     */
+	int      verbose;         /* 1: print hera's "Finish EM" line          */
+} hem_opts;
+
+/* Fills the reference defaults (src/EM.c:324-346, 292-295). */
+void hem_opts_default(hem_opts *o);
+
+typedef struct {
+	uint32_t iters;        /* SQUAREM iterations k (hera's "rounds")      */
+	uint32_t maps;         /* EM map applications (3 per iteration + fallbacks) */
+	uint32_t converged;    /* 1 if the stop rule ended the loop          */
+	uint32_t fallbacks;    /* iterations where the fallback fired        */
+	double   loglik;       /* FP64 log-likelihood of the final densities */
+	double   last_alpha;   /* step length of the last iteration          */
+	double   ms_solve;     /* device time of the iteration loop          */
+	double   ms_total;     /* host wall time of the call                 */
+} hem_stats;
+
+/* ---- device-resident problem ------------------------------------------- */
+
+typedef struct hem_problem hem_problem;
+
+/* Validates the class table, builds the transposed (per-transcript) index
+ * and uploads both to the current device.  The caller's arrays are not
+ * retained.  Returns HEM_ERR_INVALID for ids >= n_tx, zero lengths,
+ * non-monotone offsets or negative / non-finite counts. */
+int hem_problem_create(const hem_classes *cls, hem_problem **out);
+int hem_problem_destroy(hem_problem *p);
+
+/* Sizes of the device layout (for the caller's bookkeeping and bytes
+ * accounting): n_tx, n_cls, nnz, device bytes held. */
+int hem_problem_info(const hem_problem *p, uint32_t *n_tx, uint32_t *n_cls,
+                     uint64_t *nnz, uint64_t *device_bytes);
+
+/* One SQUAREM EM solve (src/EM.c:315-387).  unique[t] is the unique-read
+ * count of transcript t (REF_INF->count).  Outputs (any may be NULL):
+ * tpm[t] (hera's grama after finalisation, sums to 1e6), est[t]
+ * (est_counts = lambda_t * len_t), lambda[t] (the final per-base density
+ * before finalisation). */
+int hem_solve(hem_problem *p, const uint32_t *unique, const hem_opts *o,
+              double *tpm, double *est, double *lambda, hem_stats *st);
+
+/* FP64 log-likelihood of densities lambda (n_tx values, host memory):
+ *   l = sum_t u_t log lambda_t + sum_c n_c log D_c - sum_t len_t lambda_t,
+ *   D_c = sum_{t in c} lambda_t, with 0 log 0 = 0.
+ * hera's map (src/EM.c:271-302) is the EM map of this objective. */
+int hem_loglik(hem_problem *p, const uint32_t *unique, const double *lambda,
+               double *loglik);
+
+/* ---- per-kernel timing --------------------------------------------------
+ * Runs the same solve as hem_solve, one iteration at a time outside the
+ * graph, with a HIP event after every launch on the solve's stream, and
+ * reports the device time and algorithmic bytes of each kernel kind.
+ * Kinds: 0 class pass (denominators), 1 transcript pass (gather + M-step),
+ * 2 norm pass 1, 3 norm pass 2, 4 extrapolation, 5 iteration end. */
+#define HEM_KERNEL_KINDS 6
+typedef struct {
+	uint32_t iters;
+	uint32_t launches[HEM_KERNEL_KINDS];
+	double   ms[HEM_KERNEL_KINDS];     /* summed device time per kind      */
+	double   bytes[HEM_KERNEL_KINDS];  /* algorithmic bytes per launch     */
+} hem_kernel_profile;
+
+int hem_profile_solve(hem_problem *p, const uint32_t *unique,
+                      const hem_opts *o, hem_kernel_profile *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HERA_EM_H_ */

@@ -1,0 +1,442 @@
+/*
+ * em_oracle.c -- CPU restatement of hera's SQUAREM EM (TEST INFRASTRUCTURE).
+ *
+ * This file is the parity oracle and the timed CPU baseline ("port").  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * it, and only as the checker; the product library never links it.
+ *
+ * It restates, in FP64 and with the same loop order and expression order:
+ *   M_step          src/EM.c:271-302   (E-step class loop :276-286, M-step
+ *                                       and stop test :288-298)
+ *   E_step          src/EM.c:304-313   (uniform start 1/T)
+ *   estimate_count  src/EM.c:315-387   (SQUAREM loop :325-360, finalise
+ *                                       :362-373)
+ *   Norm            src/hash_align.c:150-164 (two-pass sample sd)
+ * and, behind options that default off:
+ *   L2 step length  hera-EM/utility.c:16-41
+ *   u2/10 clamp     hera-EM/utility.c:50-53
+ * plus three functions the reference lacks (documented in em_oracle.h):
+ * the Poisson log-likelihood, its adjoint and its Hessian-vector product.
+ *
+ * Build: gcc -O2 -fgnu89-inline (the reference's flags, Makefile_linux:9),
+ * no fast-math, so the arithmetic is IEEE FP64 without contraction.
+ *
+ * The compiled reference (src/EM.c) cannot be built here: it includes HDF5
+ * headers the image lacks.  The restatement is pinned instead by closed
+ * forms, invariants and optimality conditions (tests/test_oracle.py).
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "em_oracle.h"
+
+/* The reference's _abs (src/macros.h:13). */
+#define O_ABS(x) ((x) < 0 ? -(x) : (x))
+
+typedef struct {
+	uint32_t n_tx, n_cls;
+	const uint32_t *off, *ref;
+	const double *cnt;
+	const uint32_t *len;
+} o_prob;
+
+/* Norm, src/hash_align.c:150-164: mean, centred sum of squares / (n-1),
+ * sqrt.  Sequential sums in index order. */
+double oracle_norm_sd(const double *arr, uint32_t n)
+{
+	uint32_t i;
+	double sum, total;
+
+	for (i = 0, sum = 0; i < n; i++)
+		sum += arr[i];
+	sum /= n;
+	for (i = 0, total = 0; i < n; i++)
+		total += (arr[i] - sum) * (arr[i] - sum);
+	total /= n - 1;
+	return sqrt(total);
+}
+
+/* One EM map, src/EM.c:271-302.  grama is read, overlap is the scratch the
+ * class loop accumulates into and is left zeroed.  Returns a calloc'd
+ * vector like the reference (the allocation is part of the timed cost). */
+static double *o_mstep(const o_prob *P, const double *grama, double *overlap,
+                       const uint32_t *count, uint32_t *stop,
+                       double stop_min, double stop_rel)
+{
+	uint32_t i, k;
+	double total, max_err, err;
+	double *new_grama = calloc(P->n_tx, sizeof(double));
+
+	for (i = 0; i < P->n_cls; ++i) {
+		const uint32_t *r = P->ref + P->off[i];
+		uint32_t n = P->off[i + 1] - P->off[i];
+		for (k = 0, total = 0.0; k < n; ++k)
+			total += grama[r[k]];
+		if (total == 0)
+			continue;
+		for (k = 0; k < n; ++k)
+			overlap[r[k]] += grama[r[k]] * P->cnt[i] / total;
+	}
+
+	for (i = 0, max_err = 0.0, *stop = 1; i < P->n_tx; i++) {
+		new_grama[i] = (double)(count[i] + overlap[i]) / P->len[i];
+		overlap[i] = 0.0;
+		if (new_grama[i] > stop_min) {
+			err = O_ABS(new_grama[i] - grama[i]) / new_grama[i];
+			max_err = err > max_err ? err : max_err;
+			if (max_err > stop_rel)
+				*stop = 0;
+		}
+	}
+	return new_grama;
+}
+
+void oracle_opts_default(oracle_opts *o)
+{
+	o->min_round = 50;
+	o->max_round = 1000;
+	o->stop_min = 1e-3;
+	o->stop_rel = 0.01;
+	o->step_form = 0;
+	o->clamp_form = 0;
+	o->alpha_clamp = 1;
+}
+
+/* estimate_count, src/EM.c:315-387, on a CSR class table. */
+int oracle_estimate_count(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
+                          const uint32_t *ref, const double *cnt,
+                          const uint32_t *len, const uint32_t *count,
+                          const oracle_opts *opt, const double *init,
+                          double *tpm, double *est, double *lambda,
+                          oracle_trace *tr)
+{
+	o_prob P = {n_tx, n_cls, off, ref, cnt, len};
+	uint32_t i, k, stop;
+	double *u, *u1, *u2, *grama, *overlap, sum, alpha, n_v, n_r;
+	double *r = malloc(n_tx * sizeof(double));
+	double *v = malloc(n_tx * sizeof(double));
+
+	if (!r || !v || n_tx == 0)
+		return 1;
+	/* E_step, src/EM.c:304-313 */
+	grama = calloc(n_tx, sizeof(double));
+	overlap = calloc(n_tx, sizeof(double));
+	for (i = 0; i < n_tx; i++)
+		grama[i] = init ? init[i] : (double)1 / n_tx;
+
+	stop = 0;
+	k = 0;
+	while (k < opt->min_round || (k < opt->max_round && stop == 0)) {
+		u = grama;
+		u1 = o_mstep(&P, u, overlap, count, &stop, opt->stop_min,
+		             opt->stop_rel);
+		u2 = o_mstep(&P, u1, overlap, count, &stop, opt->stop_min,
+		             opt->stop_rel);
+
+		for (i = 0; i < n_tx; ++i) {
+			r[i] = u1[i] - u[i];
+			v[i] = (u2[i] - u1[i]) - r[i];
+		}
+
+		if (opt->step_form == 0) {
+			n_r = oracle_norm_sd(r, n_tx);
+			n_v = oracle_norm_sd(v, n_tx);
+		} else {
+			/* hera-EM/utility.c:34-41: squared L2 norms; a zero
+			 * ||v|| (<= 1e-300) gives gamma^2 = 1. */
+			double r2 = 0, v2 = 0;
+			for (i = 0; i < n_tx; ++i) {
+				r2 += r[i] * r[i];
+				v2 += v[i] * v[i];
+			}
+			n_r = sqrt(r2);
+			n_v = v2 <= 1e-300 ? 0 : sqrt(v2);
+		}
+
+		if (n_v != 0)
+			alpha = -n_r / n_v;
+		else
+			alpha = -1;
+		if (opt->alpha_clamp && alpha > -1)
+			alpha = -1;
+
+		for (i = 0; i < n_tx; ++i) {
+			double x = u[i] - 2 * alpha * r[i] + alpha * alpha * v[i];
+			if (opt->clamp_form == 0) {
+				if (x < 0)
+					x = 0;
+			} else if (x <= 1e-300) {
+				x = u2[i] / 10;
+			}
+			u1[i] = x;
+		}
+		free(u2);
+
+		grama = o_mstep(&P, u1, overlap, count, &stop, opt->stop_min,
+		                opt->stop_rel);
+		if (tr && k < tr->cap) {
+			tr->alpha[k] = alpha;
+			tr->stop[k] = stop;
+		}
+		if (tr && tr->last_u)
+			memcpy(tr->last_u, u1, n_tx * sizeof(double));
+		free(u);
+		free(u1);
+		++k;
+	}
+
+	if (lambda)
+		memcpy(lambda, grama, n_tx * sizeof(double));
+	/* finalisation, src/EM.c:362-373 */
+	for (i = 0, sum = 0.0; i < n_tx; ++i)
+		sum += grama[i];
+	for (i = 0; i < n_tx; ++i) {
+		double e = 0, g = 0;
+		if (sum != 0) {
+			e = grama[i] * len[i];
+			g = grama[i] * 1000000 / sum;
+		}
+		if (est)
+			est[i] = e;
+		if (tpm)
+			tpm[i] = g;
+	}
+	if (tr)
+		tr->k = k;
+	free(grama);
+	free(overlap);
+	free(r);
+	free(v);
+	return 0;
+}
+
+/* One EM map, exposed for closed-form tests of the class loop. */
+int oracle_em_map(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
+                  const uint32_t *ref, const double *cnt, const uint32_t *len,
+                  const uint32_t *count, const double *lam_in,
+                  double *lam_out, uint32_t *stop)
+{
+	o_prob P = {n_tx, n_cls, off, ref, cnt, len};
+	double *ov = calloc(n_tx, sizeof(double));
+	double *nw;
+	if (!ov)
+		return 1;
+	nw = o_mstep(&P, lam_in, ov, count, stop, 1e-3, 0.01);
+	memcpy(lam_out, nw, n_tx * sizeof(double));
+	free(nw);
+	free(ov);
+	return 0;
+}
+
+/* Poisson log-likelihood whose EM map is src/EM.c:271-302:
+ *   l = sum_t u_t log l_t + sum_c n_c log D_c - sum_t len_t l_t. */
+double oracle_loglik(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
+                     const uint32_t *ref, const double *cnt,
+                     const uint32_t *len, const uint32_t *count,
+                     const double *lam)
+{
+	uint32_t i, k;
+	double a = 0, b = 0, c = 0;
+	for (i = 0; i < n_cls; ++i) {
+		double d = 0;
+		for (k = off[i]; k < off[i + 1]; ++k)
+			d += lam[ref[k]];
+		if (cnt[i] != 0)
+			a += cnt[i] * log(d);
+	}
+	for (i = 0; i < n_tx; ++i) {
+		if (count[i] != 0)
+			b += count[i] * log(lam[i]);
+		c += len[i] * lam[i];
+	}
+	return a + b - c;
+}
+
+/* ---- adjoint of the softmax-parameterised likelihood --------------------
+ * pi = softmax(theta), w_t = exp(-ell_t) (ell = log effective length),
+ * unique reads as singleton classes:
+ *   L = sum_c n_c log D_c + sum_t u_t log(pi_t w_t),  D_c = sum pi_s w_s.
+ * With s_t = sum_{c ni t} n_c / D_c + u_t / (pi_t w_t) and E_t = pi_t w_t s_t
+ * (expected fragments of t), N = sum_t E_t:
+ *   dL/dtheta_j = E_j - N pi_j,   dL/dell_j = -E_j.                       */
+static void o_softmax(uint32_t n, const double *theta, double *pi)
+{
+	uint32_t i;
+	double m = theta[0], s = 0;
+	for (i = 1; i < n; ++i)
+		if (theta[i] > m)
+			m = theta[i];
+	for (i = 0; i < n; ++i)
+		s += (pi[i] = exp(theta[i] - m));
+	for (i = 0; i < n; ++i)
+		pi[i] /= s;
+}
+
+double oracle_grad(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
+                   const uint32_t *ref, const double *cnt,
+                   const uint32_t *count, const double *theta,
+                   const double *ell, double *g_theta, double *g_ell)
+{
+	uint32_t i, k;
+	double L = 0, N = 0;
+	double *pi = malloc(n_tx * sizeof(double));
+	double *x = malloc(n_tx * sizeof(double));
+	double *E = calloc(n_tx, sizeof(double));
+	o_softmax(n_tx, theta, pi);
+	for (i = 0; i < n_tx; ++i)
+		x[i] = pi[i] * exp(-ell[i]);
+	for (i = 0; i < n_cls; ++i) {
+		double d = 0;
+		for (k = off[i]; k < off[i + 1]; ++k)
+			d += x[ref[k]];
+		if (cnt[i] == 0 || d == 0)
+			continue;
+		L += cnt[i] * log(d);
+		for (k = off[i]; k < off[i + 1]; ++k)
+			E[ref[k]] += x[ref[k]] * (cnt[i] / d);
+	}
+	for (i = 0; i < n_tx; ++i) {
+		if (count[i]) {
+			L += count[i] * log(x[i]);
+			E[i] += count[i];
+		}
+	}
+	for (i = 0; i < n_tx; ++i)
+		N += E[i];
+	for (i = 0; i < n_tx; ++i) {
+		if (g_theta)
+			g_theta[i] = E[i] - N * pi[i];
+		if (g_ell)
+			g_ell[i] = -E[i];
+	}
+	free(pi);
+	free(x);
+	free(E);
+	return L;
+}
+
+/* Hessian-vector product d/de [grad_theta L](theta + e*vec) at e=0:
+ *   pid_t = pi_t (vec_t - pi.vec),  Dd_c = sum pid_s w_s,
+ *   sd_t  = -sum_{c ni t} n_c Dd_c / D_c^2   (unique reads: singleton),
+ *   Ed_t  = pid_t w_t s_t + pi_t w_t sd_t,   Hv_t = Ed_t - N pid_t.        */
+void oracle_hvp(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
+                const uint32_t *ref, const double *cnt,
+                const uint32_t *count, const double *theta, const double *ell,
+                const double *vec, double *hv)
+{
+	uint32_t i, k;
+	double N = 0, pv = 0;
+	double *pi = malloc(n_tx * sizeof(double));
+	double *w = malloc(n_tx * sizeof(double));
+	double *pd = malloc(n_tx * sizeof(double));
+	double *s = calloc(n_tx, sizeof(double));
+	double *sd = calloc(n_tx, sizeof(double));
+	o_softmax(n_tx, theta, pi);
+	for (i = 0; i < n_tx; ++i) {
+		w[i] = exp(-ell[i]);
+		pv += pi[i] * vec[i];
+	}
+	for (i = 0; i < n_tx; ++i)
+		pd[i] = pi[i] * (vec[i] - pv);
+	for (i = 0; i < n_cls; ++i) {
+		double d = 0, dd = 0;
+		for (k = off[i]; k < off[i + 1]; ++k) {
+			d += pi[ref[k]] * w[ref[k]];
+			dd += pd[ref[k]] * w[ref[k]];
+		}
+		if (cnt[i] == 0 || d == 0)
+			continue;
+		N += cnt[i];
+		for (k = off[i]; k < off[i + 1]; ++k) {
+			s[ref[k]] += cnt[i] / d;
+			sd[ref[k]] -= cnt[i] * dd / (d * d);
+		}
+	}
+	for (i = 0; i < n_tx; ++i) {
+		if (count[i]) {
+			double d = pi[i] * w[i], dd = pd[i] * w[i];
+			N += count[i];
+			s[i] += count[i] / d;
+			sd[i] -= count[i] * dd / (d * d);
+		}
+	}
+	for (i = 0; i < n_tx; ++i)
+		hv[i] = pd[i] * w[i] * s[i] + pi[i] * w[i] * sd[i] - N * pd[i];
+	free(pi);
+	free(w);
+	free(pd);
+	free(s);
+	free(sd);
+}
+
+/* ---- bootstrap baseline: the reference's join-per-batch thread schedule
+ * (src/EM.c:243-255), each thread running one estimate_count on its own
+ * count vector.  Used only to time the CPU baseline. */
+typedef struct {
+	const o_prob *P;
+	const uint32_t *count;
+	const double *cnt;
+	const oracle_opts *opt;
+	double *est;
+	uint32_t k;
+} o_job;
+
+static void *o_bs_thread(void *arg)
+{
+	o_job *j = (o_job *)arg;
+	oracle_trace tr;
+	memset(&tr, 0, sizeof(tr));
+	oracle_estimate_count(j->P->n_tx, j->P->n_cls, j->P->off, j->P->ref,
+	                      j->cnt, j->P->len, j->count, j->opt, NULL, NULL,
+	                      j->est, NULL, &tr);
+	j->k = tr.k;
+	return NULL;
+}
+
+int oracle_bootstrap_batch(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
+                           const uint32_t *ref, const uint32_t *len,
+                           uint32_t n_rep, const uint32_t *counts,
+                           const double *cls_counts, const oracle_opts *opt,
+                           uint32_t n_thread, double *est, uint32_t *k_out)
+{
+	o_prob P = {n_tx, n_cls, off, ref, NULL, len};
+	pthread_t *thr = calloc(n_thread, sizeof(pthread_t));
+	o_job *jobs = calloc(n_rep, sizeof(o_job));
+	pthread_attr_t attr;
+	uint32_t i, k;
+
+	if (!thr || !jobs)
+		return 1;
+	pthread_attr_init(&attr);
+	pthread_attr_setstacksize(&attr, 16u << 20);
+	for (i = 0; i < n_rep;) {
+		for (k = 0; k < n_thread && i < n_rep; ++k, ++i) {
+			jobs[i].P = &P;
+			jobs[i].count = counts + (size_t)i * n_tx;
+			jobs[i].cnt = cls_counts + (size_t)i * n_cls;
+			jobs[i].opt = opt;
+			jobs[i].est = est ? est + (size_t)i * n_tx : NULL;
+			pthread_create(&thr[k], &attr, o_bs_thread, &jobs[i]);
+		}
+		for (; k > 0; --k)
+			pthread_join(thr[k - 1], NULL);
+	}
+	for (i = 0; i < n_rep; ++i)
+		if (k_out)
+			k_out[i] = jobs[i].k;
+	pthread_attr_destroy(&attr);
+	free(thr);
+	free(jobs);
+	return 0;
+}
+
+double oracle_now(void)
+{
+	struct timespec ts;
+	clock_gettime(CLOCK_MONOTONIC, &ts);
+	return ts.tv_sec + ts.tv_nsec * 1e-9;
+}

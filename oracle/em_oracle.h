@@ -1,0 +1,59 @@
+/*
+ * em_oracle.h -- CPU restatement of hera's EM (TEST INFRASTRUCTURE ONLY).
+ * Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * as the checker; never linked by the product library.
+ */
+#ifndef EM_ORACLE_H_
+#define EM_ORACLE_H_
+#include <stdint.h>
+
+typedef struct {
+	uint32_t min_round;   /* src/EM.c:325 */
+	uint32_t max_round;   /* src/EM.c:326 */
+	double stop_min;      /* src/EM.c:292 */
+	double stop_rel;      /* src/EM.c:295 */
+	int step_form;        /* 0 sample sd (src/EM.c:338), 1 L2 (hera-EM) */
+	int clamp_form;       /* 0 zero (src/EM.c:351), 1 u2/10 (hera-EM)   */
+	int alpha_clamp;      /* 1: alpha <= -1 (src/EM.c:346)               */
+} oracle_opts;
+
+typedef struct {
+	uint32_t cap;         /* capacity of alpha / stop                    */
+	uint32_t k;           /* iterations run                              */
+	double *alpha;        /* step length per iteration                   */
+	uint32_t *stop;       /* stop flag after each iteration's 3rd map    */
+	double *last_u;       /* n_tx: extrapolated point of the last
+	                         iteration (the input of its 3rd map), or NULL */
+} oracle_trace;
+
+void oracle_opts_default(oracle_opts *o);
+double oracle_norm_sd(const double *arr, uint32_t n);
+int oracle_estimate_count(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
+                          const uint32_t *ref, const double *cnt,
+                          const uint32_t *len, const uint32_t *count,
+                          const oracle_opts *opt, const double *init,
+                          double *tpm, double *est, double *lambda,
+                          oracle_trace *tr);
+int oracle_em_map(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
+                  const uint32_t *ref, const double *cnt, const uint32_t *len,
+                  const uint32_t *count, const double *lam_in,
+                  double *lam_out, uint32_t *stop);
+double oracle_loglik(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
+                     const uint32_t *ref, const double *cnt,
+                     const uint32_t *len, const uint32_t *count,
+                     const double *lam);
+double oracle_grad(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
+                   const uint32_t *ref, const double *cnt,
+                   const uint32_t *count, const double *theta,
+                   const double *ell, double *g_theta, double *g_ell);
+void oracle_hvp(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
+                const uint32_t *ref, const double *cnt,
+                const uint32_t *count, const double *theta, const double *ell,
+                const double *vec, double *hv);
+int oracle_bootstrap_batch(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
+                           const uint32_t *ref, const uint32_t *len,
+                           uint32_t n_rep, const uint32_t *counts,
+                           const double *cls_counts, const oracle_opts *opt,
+                           uint32_t n_thread, double *est, uint32_t *k_out);
+double oracle_now(void);
+#endif

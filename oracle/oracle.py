@@ -1,0 +1,183 @@
+"""ctypes view of oracle/liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+The FP64 CPU restatement of hera's EM (oracle/em_oracle.c).  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import this module,
+always as the checker or the timed CPU baseline, never as a product path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_u32p = C.POINTER(C.c_uint32)
+_f64p = C.POINTER(C.c_double)
+
+
+class OracleOpts(C.Structure):
+    _fields_ = [("min_round", C.c_uint32), ("max_round", C.c_uint32),
+                ("stop_min", C.c_double), ("stop_rel", C.c_double),
+                ("step_form", C.c_int), ("clamp_form", C.c_int),
+                ("alpha_clamp", C.c_int)]
+
+
+class OracleTrace(C.Structure):
+    _fields_ = [("cap", C.c_uint32), ("k", C.c_uint32),
+                ("alpha", _f64p), ("stop", _u32p), ("last_u", _f64p)]
+
+
+_LIB = None
+
+
+def load():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: make -C oracle")
+        lib = C.CDLL(LIB_PATH)
+        lib.oracle_norm_sd.restype = C.c_double
+        lib.oracle_norm_sd.argtypes = [_f64p, C.c_uint32]
+        lib.oracle_estimate_count.restype = C.c_int
+        lib.oracle_estimate_count.argtypes = [
+            C.c_uint32, C.c_uint32, _u32p, _u32p, _f64p, _u32p, _u32p,
+            C.POINTER(OracleOpts), _f64p, _f64p, _f64p, _f64p,
+            C.POINTER(OracleTrace)]
+        lib.oracle_em_map.restype = C.c_int
+        lib.oracle_em_map.argtypes = [C.c_uint32, C.c_uint32, _u32p, _u32p,
+                                      _f64p, _u32p, _u32p, _f64p, _f64p,
+                                      _u32p]
+        lib.oracle_loglik.restype = C.c_double
+        lib.oracle_loglik.argtypes = [C.c_uint32, C.c_uint32, _u32p, _u32p,
+                                      _f64p, _u32p, _u32p, _f64p]
+        lib.oracle_grad.restype = C.c_double
+        lib.oracle_grad.argtypes = [C.c_uint32, C.c_uint32, _u32p, _u32p,
+                                    _f64p, _u32p, _f64p, _f64p, _f64p, _f64p]
+        lib.oracle_hvp.restype = None
+        lib.oracle_hvp.argtypes = [C.c_uint32, C.c_uint32, _u32p, _u32p,
+                                   _f64p, _u32p, _f64p, _f64p, _f64p, _f64p]
+        lib.oracle_bootstrap_batch.restype = C.c_int
+        lib.oracle_bootstrap_batch.argtypes = [
+            C.c_uint32, C.c_uint32, _u32p, _u32p, _u32p, C.c_uint32, _u32p,
+            _f64p, C.POINTER(OracleOpts), C.c_uint32, _f64p, _u32p]
+        lib.oracle_now.restype = C.c_double
+        lib.oracle_opts_default.argtypes = [C.POINTER(OracleOpts)]
+        _LIB = lib
+    return _LIB
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _p32(a):
+    return a.ctypes.data_as(_u32p)
+
+
+def _p64(a):
+    return a.ctypes.data_as(_f64p)
+
+
+def opts(**kw):
+    o = OracleOpts()
+    load().oracle_opts_default(o)
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+def norm_sd(x):
+    x = _f64(x)
+    return load().oracle_norm_sd(_p64(x), len(x))
+
+
+def estimate_count(cls, unique, init=None, trace_cap=1000, **kw):
+    """Runs the restated estimate_count; returns dict(tpm, est, lam, k,
+    alpha, stop)."""
+    lib = load()
+    T, Cn = cls.n_tx, cls.n_cls
+    off, ref = _u32(cls.cls_off), _u32(cls.cls_ref)
+    cnt, ln, u = _f64(cls.cls_count), _u32(cls.tx_len), _u32(unique)
+    tpm, est, lam = np.empty(T), np.empty(T), np.empty(T)
+    alpha = np.zeros(max(trace_cap, 1))
+    stop = np.zeros(max(trace_cap, 1), dtype=np.uint32)
+    last_u = np.zeros(T)
+    tr = OracleTrace(trace_cap, 0, _p64(alpha), _p32(stop), _p64(last_u))
+    ini = _f64(init) if init is not None else None
+    rc = lib.oracle_estimate_count(T, Cn, _p32(off), _p32(ref), _p64(cnt),
+                                   _p32(ln), _p32(u), opts(**kw),
+                                   _p64(ini) if ini is not None else None,
+                                   _p64(tpm), _p64(est), _p64(lam), tr)
+    assert rc == 0
+    k = tr.k
+    return dict(tpm=tpm, est=est, lam=lam, k=k,
+                alpha=alpha[:min(k, trace_cap)].copy(),
+                stop=stop[:min(k, trace_cap)].copy(), last_u=last_u)
+
+
+def em_map(cls, unique, lam):
+    lib = load()
+    T = cls.n_tx
+    off, ref = _u32(cls.cls_off), _u32(cls.cls_ref)
+    cnt, ln, u, lam = (_f64(cls.cls_count), _u32(cls.tx_len), _u32(unique),
+                       _f64(lam))
+    out = np.empty(T)
+    stop = C.c_uint32()
+    assert lib.oracle_em_map(T, cls.n_cls, _p32(off), _p32(ref), _p64(cnt),
+                             _p32(ln), _p32(u), _p64(lam), _p64(out),
+                             C.byref(stop)) == 0
+    return out, stop.value
+
+
+def loglik(cls, unique, lam):
+    off, ref = _u32(cls.cls_off), _u32(cls.cls_ref)
+    cnt, ln, u, lam = (_f64(cls.cls_count), _u32(cls.tx_len), _u32(unique),
+                       _f64(lam))
+    return load().oracle_loglik(cls.n_tx, cls.n_cls, _p32(off), _p32(ref),
+                                _p64(cnt), _p32(ln), _p32(u), _p64(lam))
+
+
+def grad(cls, unique, theta, ell):
+    off, ref = _u32(cls.cls_off), _u32(cls.cls_ref)
+    cnt, u = _f64(cls.cls_count), _u32(unique)
+    theta, ell = _f64(theta), _f64(ell)
+    gt, ge = np.empty(cls.n_tx), np.empty(cls.n_tx)
+    L = load().oracle_grad(cls.n_tx, cls.n_cls, _p32(off), _p32(ref),
+                           _p64(cnt), _p32(u), _p64(theta), _p64(ell),
+                           _p64(gt), _p64(ge))
+    return L, gt, ge
+
+
+def hvp(cls, unique, theta, ell, vec):
+    off, ref = _u32(cls.cls_off), _u32(cls.cls_ref)
+    cnt, u = _f64(cls.cls_count), _u32(unique)
+    theta, ell, vec = _f64(theta), _f64(ell), _f64(vec)
+    out = np.empty(cls.n_tx)
+    load().oracle_hvp(cls.n_tx, cls.n_cls, _p32(off), _p32(ref), _p64(cnt),
+                      _p32(u), _p64(theta), _p64(ell), _p64(vec), _p64(out))
+    return out
+
+
+def bootstrap_batch(cls, counts, cls_counts, n_thread, **kw):
+    """Reference join-per-batch thread schedule over R count vectors."""
+    R = counts.shape[0]
+    off, ref, ln = _u32(cls.cls_off), _u32(cls.cls_ref), _u32(cls.tx_len)
+    counts, cls_counts = _u32(counts), _f64(cls_counts)
+    est = np.empty((R, cls.n_tx))
+    ks = np.empty(R, dtype=np.uint32)
+    assert load().oracle_bootstrap_batch(
+        cls.n_tx, cls.n_cls, _p32(off), _p32(ref), _p32(ln), R,
+        _p32(counts), _p64(cls_counts), opts(**kw), n_thread, _p64(est),
+        _p32(ks)) == 0
+    return est, ks
+
+
+def now():
+    return load().oracle_now()

@@ -1,0 +1,256 @@
+"""Pins the CPU oracle (oracle/em_oracle.c) without running the reference.
+
+The compiled reference (src/EM.c) cannot be built in this image (it needs
+HDF5 headers), and the reference ships no tests or vectors.  Each check below
+is chosen so that a plausible restatement mistake fails it: a wrong share in
+the class loop, a dropped unique count, a transposed r/v, a wrong norm, a
+wrong clamp, a wrong stop-test input or loop bound.
+"""
+import numpy as np
+import pytest
+from scipy.optimize import brentq, minimize
+
+from conftest import tiny_problem
+from hera_amd import Classes
+from oracle import oracle as O
+
+
+def class_terms(cls, lam):
+    """D_c and per-transcript sum_{c ni t} n_c / D_c (numpy, vectorised)."""
+    off = cls.cls_off.astype(np.int64)
+    ref = cls.cls_ref.astype(np.int64)
+    sizes = np.diff(off)
+    row = np.repeat(np.arange(cls.n_cls), sizes)
+    D = np.bincount(row, weights=lam[ref], minlength=cls.n_cls)
+    rho = np.where(D > 0, cls.cls_count / np.where(D > 0, D, 1), 0.0)
+    s = np.bincount(ref, weights=rho[row], minlength=cls.n_tx)
+    return D, s
+
+
+def test_norm_is_sample_sd():
+    rng = np.random.default_rng(1)
+    for n in (2, 3, 17, 1000, 100000):
+        x = rng.standard_normal(n) * 10 ** rng.uniform(-8, 3) + rng.uniform(-5, 5)
+        assert O.norm_sd(x) == pytest.approx(np.std(x, ddof=1), rel=1e-12)
+    # n-1 divisor, not n: two points {0, 2} have sample sd sqrt(2)
+    assert O.norm_sd([0.0, 2.0]) == pytest.approx(np.sqrt(2.0), rel=1e-15)
+
+
+def test_first_map_from_uniform_start_closed_form():
+    # From lam = 1/T every member of class c gets n_c * mult / |c|, where
+    # mult counts repeated members; lam' = (u + that) / len.
+    for seed in range(3):
+        cls, u = tiny_problem(seed, dup=True)
+        T = cls.n_tx
+        lam0 = np.full(T, 1.0 / T)
+        out, _ = O.em_map(cls, u, lam0)
+        share = np.zeros(T)
+        for c in range(cls.n_cls):
+            m = cls.cls_ref[cls.cls_off[c]:cls.cls_off[c + 1]]
+            for t in m:
+                share[t] += cls.cls_count[c] / len(m)
+        np.testing.assert_allclose(out, (u + share) / cls.tx_len, rtol=1e-13)
+
+
+def test_singleton_classes_one_map_fixed_point():
+    # A single-member class hands its whole count to its member:
+    # lam' = (u + n_c) / len regardless of lam (src/EM.c:276-290).
+    rng = np.random.default_rng(5)
+    T = 30
+    members = [[t] for t in range(T)]
+    counts = rng.integers(1, 100, T).astype(float)
+    lens = rng.integers(200, 3000, T)
+    cls = Classes.from_lists(T, members, counts, lens)
+    u = rng.integers(0, 50, T).astype(np.uint32)
+    lam = rng.uniform(0.1, 2.0, T)
+    out, _ = O.em_map(cls, u, lam)
+    # (lam * n) / lam rounds to n within an ulp
+    np.testing.assert_allclose(out, (u + counts) / lens, rtol=4e-16)
+
+
+def test_map_conserves_fragments_and_skips_zero_denominators():
+    for seed in range(4):
+        cls, u = tiny_problem(seed, dup=(seed % 2 == 1))
+        rng = np.random.default_rng(seed)
+        lam = rng.uniform(0, 1, cls.n_tx)
+        lam[rng.random(cls.n_tx) < 0.3] = 0.0   # some D_c become 0
+        out, _ = O.em_map(cls, u, lam)
+        D, _ = class_terms(cls, lam)
+        expect = u.sum() + cls.cls_count[D > 0].sum()
+        assert np.dot(cls.tx_len.astype(float), out) == pytest.approx(
+            expect, rel=1e-12)
+        assert (D == 0).any()
+
+
+def test_two_transcript_fixed_point():
+    # u1, u2 unique reads, one shared class n12: the fixed point solves
+    # L1 l1 + L2 l2 = u1 + u2 + n12 and l1 (L1 - n12 / (l1 + l2)) = u1.
+    u1, u2, n12, L1, L2 = 37, 11, 250.0, 1500, 800
+    S = u1 + u2 + n12
+
+    def g(l1):
+        l2 = (S - L1 * l1) / L2
+        return l1 * L1 - l1 * n12 / (l1 + l2) - u1
+
+    l1 = brentq(g, 1e-15, S / L1, xtol=1e-300, rtol=1e-15)
+    l2 = (S - L1 * l1) / L2
+    cls = Classes.from_lists(2, [[0, 1]], [n12], [L1, L2])
+    u = np.array([u1, u2], dtype=np.uint32)
+    r = O.estimate_count(cls, u, min_round=1, max_round=100000, stop_min=0.0,
+                         stop_rel=1e-15)
+    np.testing.assert_allclose(r["lam"], [l1, l2], rtol=1e-11)
+
+
+def loglik_np(cls, u, lam):
+    D, _ = class_terms(cls, lam)
+    a = np.sum(cls.cls_count[cls.cls_count > 0] * np.log(D[cls.cls_count > 0]))
+    b = np.sum(u[u > 0] * np.log(lam[u > 0]))
+    return a + b - np.dot(cls.tx_len.astype(float), lam)
+
+
+def test_loglik_definition_and_em_monotone():
+    # EM never lowers the objective whose EM map src/EM.c:271-302 is.
+    for seed in range(4):
+        cls, u = tiny_problem(seed, dup=(seed == 3))
+        rng = np.random.default_rng(seed + 10)
+        lam = rng.uniform(1e-3, 1.0, cls.n_tx)
+        ll0 = O.loglik(cls, u, lam)
+        assert ll0 == pytest.approx(loglik_np(cls, u, lam), rel=1e-13)
+        for _ in range(5):
+            lam, _ = O.em_map(cls, u, lam)
+            ll1 = O.loglik(cls, u, lam)
+            assert ll1 >= ll0 - 1e-9 * abs(ll0)
+            ll0 = ll1
+
+
+def test_converged_point_is_the_maximum():
+    # KKT of the Poisson objective at the tightly converged point, and no
+    # better point found by an independent optimiser.
+    cls, u = tiny_problem(7, n_tx=8, n_cls=20, max_size=4)
+    r = O.estimate_count(cls, u, min_round=1, max_round=200000, stop_min=0.0,
+                         stop_rel=1e-14)
+    lam = r["lam"]
+    _, s = class_terms(cls, lam)
+    grad = np.where(u > 0, u / np.maximum(lam, 1e-300), 0.0) + s - cls.tx_len
+    supp = lam > 1e-12
+    assert np.all(np.abs(grad[supp]) < 1e-6 * cls.tx_len[supp])
+    assert np.all(grad[~supp] < 1e-6 * cls.tx_len[~supp])
+    ll = O.loglik(cls, u, lam)
+
+    def negll(x):
+        return -loglik_np(cls, u, np.exp(x))
+
+    x0 = np.log(np.full(cls.n_tx, (u.sum() + cls.cls_count.sum())
+                        / cls.tx_len.sum()))
+    best = minimize(negll, x0, method="BFGS", options={"gtol": 1e-10,
+                                                       "maxiter": 20000})
+    assert ll >= -best.fun - 1e-9 * abs(ll)
+
+
+def test_alpha_minus_one_when_em_is_exact():
+    # Singleton classes: u1 = u2 = fixed point, v = -r, so sd(v) = sd(r),
+    # alpha = -1, and the loop stops at min_round with lam = (u + n)/len.
+    rng = np.random.default_rng(3)
+    T = 25
+    cls = Classes.from_lists(T, [[t] for t in range(T)],
+                             rng.integers(1, 90, T).astype(float),
+                             rng.integers(200, 3000, T))
+    u = rng.integers(1, 40, T).astype(np.uint32)
+    r = O.estimate_count(cls, u)
+    assert r["k"] == 50
+    assert np.all(r["alpha"] == -1.0)
+    np.testing.assert_allclose(r["lam"], (u + cls.cls_count) / cls.tx_len,
+                               rtol=4e-16)
+
+
+def test_alpha_clamp_and_extrapolation_clamp():
+    cls, u = tiny_problem(11, n_tx=60, n_cls=300, zero_frac=0.6)
+    r = O.estimate_count(cls, u, min_round=30, max_round=30)
+    assert r["k"] == 30
+    assert np.all(r["alpha"] <= -1.0)
+    assert np.any(r["alpha"] < -1.0)
+    r2 = O.estimate_count(cls, u, min_round=30, max_round=30, alpha_clamp=0)
+    assert np.any(r2["alpha"] > -1.0)
+    # zero clamp: once an extrapolated value is clamped to 0 and the
+    # transcript has no unique reads it stays exactly 0
+    assert np.any((r["lam"] == 0) & (u == 0))
+    assert np.all(r["last_u"] >= 0)
+    # hera-EM's u2/10 replacement never leaves exact zeros
+    r3 = O.estimate_count(cls, u, min_round=30, max_round=30, clamp_form=1)
+    assert np.all(r3["lam"] > 0)
+
+
+def stop_flag(lam_new, lam_in, smin=1e-3, srel=0.01):
+    m = lam_new > smin
+    return int(not np.any(np.abs(lam_new[m] - lam_in[m]) / lam_new[m] > srel))
+
+
+def test_stop_rule_inputs_and_loop_bounds(cfg1):
+    cls, u = cfg1
+    r = O.estimate_count(cls, u)
+    k = r["k"]
+    assert 50 <= k < 1000
+    # the flag that ended the loop is the third map's test between the
+    # extrapolated point and its image (src/EM.c:292-297, 355)
+    assert r["stop"][k - 1] == 1
+    assert stop_flag(r["lam"], r["last_u"]) == 1
+    assert np.all(r["stop"][49:k - 1] == 0)
+    # fixed iteration count and no-convergence paths
+    r5 = O.estimate_count(cls, u, min_round=3, max_round=3)
+    assert r5["k"] == 3
+    assert stop_flag(r5["lam"], r5["last_u"]) == r5["stop"][2]
+    rn = O.estimate_count(cls, u, min_round=0, max_round=7, stop_rel=0.0)
+    assert rn["k"] == 7 and rn["stop"][6] == 0
+    r0 = O.estimate_count(cls, u, min_round=0, max_round=0)
+    assert r0["k"] == 0
+    np.testing.assert_array_equal(r0["lam"], np.full(cls.n_tx, 1 / cls.n_tx))
+
+
+def test_finalisation():
+    cls, u = tiny_problem(2)
+    r = O.estimate_count(cls, u)
+    assert r["tpm"].sum() == pytest.approx(1e6, rel=1e-12)
+    np.testing.assert_array_equal(r["est"], r["lam"] * cls.tx_len)
+    np.testing.assert_allclose(r["tpm"], r["lam"] / r["lam"].sum() * 1e6,
+                               rtol=1e-14)
+    # all-zero input: the sum==0 branch zeroes both outputs
+    z = O.estimate_count(Classes.from_lists(3, [[0, 1]], [0.0], [100, 100, 100]),
+                         np.zeros(3, np.uint32), min_round=2, max_round=2)
+    assert np.all(z["tpm"] == 0) and np.all(z["est"] == 0)
+
+
+def test_gradient_matches_central_differences():
+    cls, u = tiny_problem(4, n_tx=30, n_cls=90)
+    rng = np.random.default_rng(0)
+    theta = rng.standard_normal(cls.n_tx)
+    ell = np.log(cls.tx_len.astype(float))
+    L, gt, ge = O.grad(cls, u, theta, ell)
+    h = 1e-6
+    for j in range(cls.n_tx):
+        e = np.zeros(cls.n_tx)
+        e[j] = h
+        fd_t = (O.grad(cls, u, theta + e, ell)[0]
+                - O.grad(cls, u, theta - e, ell)[0]) / (2 * h)
+        fd_e = (O.grad(cls, u, theta, ell + e)[0]
+                - O.grad(cls, u, theta, ell - e)[0]) / (2 * h)
+        assert gt[j] == pytest.approx(fd_t, rel=1e-5, abs=1e-4)
+        assert ge[j] == pytest.approx(fd_e, rel=1e-5, abs=1e-4)
+    # softmax invariance: the gradient is orthogonal to the ones vector
+    assert abs(gt.sum()) < 1e-8 * np.abs(gt).sum()
+
+
+def test_hvp_matches_gradient_differences():
+    cls, u = tiny_problem(6, n_tx=25, n_cls=70)
+    rng = np.random.default_rng(2)
+    theta = rng.standard_normal(cls.n_tx)
+    ell = np.log(cls.tx_len.astype(float))
+    v = rng.standard_normal(cls.n_tx)
+    hv = O.hvp(cls, u, theta, ell, v)
+    h = 1e-5
+    fd = (O.grad(cls, u, theta + h * v, ell)[1]
+          - O.grad(cls, u, theta - h * v, ell)[1]) / (2 * h)
+    np.testing.assert_allclose(hv, fd, rtol=1e-5, atol=1e-6 * np.abs(fd).max())
+    # symmetric: w.Hv == v.Hw
+    w = rng.standard_normal(cls.n_tx)
+    hw = O.hvp(cls, u, theta, ell, w)
+    assert np.dot(w, hv) == pytest.approx(np.dot(v, hw), rel=1e-10)

@@ -21,6 +21,8 @@ STEP_SD = 0
 STEP_L2 = 1
 CLAMP_ZERO = 0
 CLAMP_DIV10 = 1
+NORM_EXACT = 0
+NORM_TREE = 1
 
 
 @dataclass

@@ -32,6 +32,7 @@ class HemClasses(C.Structure):
 class HemOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("step", C.c_int),
                 ("clamp", C.c_int), ("alpha_clamp", C.c_int),
+                ("norm", C.c_int),
                 ("min_iter", C.c_uint32), ("max_iter", C.c_uint32),
                 ("stop_min_value", C.c_double),
                 ("stop_rel_change", C.c_double),

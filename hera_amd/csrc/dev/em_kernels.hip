@@ -156,6 +156,26 @@ __global__ __launch_bounds__(BLOCK) void k_gather_exact(
 	}
 }
 
+// SQUAREM step length from the two reductions (src/EM.c:338-346): sample
+// sd (A, B = centred sums of squares) or L2 (A, B = sums of squares).
+__device__ __forceinline__ double step_length(double A, double B, uint32_t n,
+                                              hemk_step st)
+{
+#pragma clang fp contract(off)
+	double n_r, n_v, alpha;
+	if (st.step == 0) {
+		n_r = sqrt(A / (double)(n - 1u));
+		n_v = sqrt(B / (double)(n - 1u));
+	} else {
+		n_r = sqrt(A);
+		n_v = B <= 1e-300 ? 0.0 : sqrt(B);
+	}
+	alpha = n_v != 0 ? -n_r / n_v : -1.0;
+	if (st.alpha_clamp && alpha > -1)
+		alpha = -1;
+	return alpha;
+}
+
 template <int G, typename TL>
 __global__ __launch_bounds__(BLOCK) void k_denom(
     const uint32_t *__restrict__ off, const uint32_t *__restrict__ ref,
@@ -286,18 +306,14 @@ __global__ __launch_bounds__(BLOCK) void k_extrap(
 #pragma clang fp contract(off)
 	if (ctl->done)
 		return;
-	double A, B, n_r, n_v, alpha;
-	reduce_parts(part, A, B);
-	if (st.step == 0) {
-		n_r = sqrt(A / (double)(n - 1u));
-		n_v = sqrt(B / (double)(n - 1u));
+	double alpha;
+	if (st.alpha_given) {
+		alpha = ctl->alpha;
 	} else {
-		n_r = sqrt(A);
-		n_v = B <= 1e-300 ? 0.0 : sqrt(B);
+		double A, B;
+		reduce_parts(part, A, B);
+		alpha = step_length(A, B, n, st);
 	}
-	alpha = n_v != 0 ? -n_r / n_v : -1.0;
-	if (st.alpha_clamp && alpha > -1)
-		alpha = -1;
 	for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n;
 	     i += gridDim.x * BLOCK) {
 		const double r = u1[i] - u[i];
@@ -313,8 +329,84 @@ __global__ __launch_bounds__(BLOCK) void k_extrap(
 		if (out32)
 			out32[i] = (float)x;
 	}
-	if (blockIdx.x == 0 && threadIdx.x == 0)
+	if (blockIdx.x == 0 && threadIdx.x == 0 && !st.alpha_given)
 		ctl->alpha = alpha;
+}
+
+// Sequential sums on one lane per chain (see hemk_seq_norm).  Waves 2-3
+// stage chunk c (the element values, computed in parallel) in LDS while
+// lane 0 of waves 0-1 adds chunk c-1 left to right.
+#define SEQ_TB 1024
+__global__ __launch_bounds__(BLOCK) void k_seq_norm(
+    uint32_t n, const double *__restrict__ u, const double *__restrict__ u1,
+    const double *__restrict__ u2, int phase, hemk_step st,
+    hemk_ctl *__restrict__ ctl)
+{
+#pragma clang fp contract(off)
+	__shared__ double buf[2][2][SEQ_TB];
+	if (phase != 0 && ctl->done)
+		return;
+	const int w = threadIdx.x / WAVE;
+	const int lane = threadIdx.x % WAVE;
+	const uint32_t nch = (n + SEQ_TB - 1) / SEQ_TB;
+	const double mr = phase == 2 ? ctl->mean_r : 0.0;
+	const double mv = phase == 2 ? ctl->mean_v : 0.0;
+	double acc = 0;
+	for (uint32_t c = 0; c <= nch; ++c) {
+		if (w >= 2 && c < nch) {
+			const int slot = c & 1;
+			const uint32_t base = c * SEQ_TB;
+			for (int j = threadIdx.x - 2 * WAVE; j < SEQ_TB;
+			     j += 2 * WAVE) {
+				const uint32_t i = base + j;
+				double a = 0, b = 0;
+				if (i < n) {
+					if (phase == 0) {
+						a = u[i];
+					} else {
+						const double r = u1[i] - u[i];
+						const double v = (u2[i] - u1[i]) - r;
+						if (phase == 1) {
+							a = r;
+							b = v;
+						} else if (phase == 2) {
+							a = (r - mr) * (r - mr);
+							b = (v - mv) * (v - mv);
+						} else {
+							a = r * r;
+							b = v * v;
+						}
+					}
+				}
+				buf[slot][0][j] = a;
+				buf[slot][1][j] = b;
+			}
+		}
+		if (w < 2 && lane == 0 && c >= 1) {
+			const int slot = (c - 1) & 1;
+			const uint32_t cnt = min((uint32_t)SEQ_TB,
+			                         n - (c - 1) * SEQ_TB);
+			const double *x = buf[slot][w];
+#pragma unroll 16
+			for (uint32_t j = 0; j < cnt; ++j)
+				acc += x[j];
+		}
+		__syncthreads();
+	}
+	__shared__ double res[2];
+	if (w < 2 && lane == 0)
+		res[w] = acc;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		if (phase == 0) {
+			ctl->lam_sum = res[0];
+		} else if (phase == 1) {
+			ctl->mean_r = res[0] / n;
+			ctl->mean_v = res[1] / n;
+		} else {
+			ctl->alpha = step_length(res[0], res[1], n, st);
+		}
+	}
 }
 
 __global__ __launch_bounds__(BLOCK) void k_iter_end(
@@ -406,7 +498,29 @@ __global__ __launch_bounds__(BLOCK) void k_fin2(
 	}
 }
 
+__global__ __launch_bounds__(BLOCK) void k_fin_seq(
+    uint32_t n, const double *__restrict__ lam,
+    const uint32_t *__restrict__ len, const hemk_ctl *__restrict__ ctl,
+    double *__restrict__ tpm, double *__restrict__ est)
+{
+	const double sum = ctl->lam_sum;
+	for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n;
+	     i += gridDim.x * BLOCK) {
+		const double l = lam[i];
+		double e = 0, g = 0;
+		if (sum != 0) {
+			e = l * (double)len[i];
+			g = l * 1000000 / sum;
+		}
+		if (est)
+			est[i] = e;
+		if (tpm)
+			tpm[i] = g;
+	}
+}
+
 // FP64 log-likelihood partials: class terms in a, transcript terms in b.
+// Classes with D_c == 0 are skipped, as the map skips them.
 template <int G>
 __global__ __launch_bounds__(BLOCK) void k_loglik(
     const uint32_t *__restrict__ off, const uint32_t *__restrict__ ref,
@@ -423,7 +537,7 @@ __global__ __launch_bounds__(BLOCK) void k_loglik(
 		for (uint32_t k = off[c] + lane; k < off[c + 1]; k += G)
 			d += lam[ref[k]];
 		d = group_sum<G>(d);
-		if (lane == 0 && cnt[c] != 0)
+		if (lane == 0 && cnt[c] != 0 && d > 0)
 			a += cnt[c] * log(d);
 	}
 	for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n_tx;
@@ -602,6 +716,22 @@ int hemk_finalize(uint32_t n, const double *lam, const uint32_t *len,
 	k_fin1<<<HEMK_NORM_BLOCKS, BLOCK, 0, (hipStream_t)s>>>(n, lam, part);
 	k_fin2<<<HEMK_NORM_BLOCKS, BLOCK, 0, (hipStream_t)s>>>(n, lam, len,
 	                                                       part, tpm, est);
+	return (int)hipGetLastError();
+}
+
+int hemk_seq_norm(uint32_t n, const double *u, const double *u1,
+                  const double *u2, int phase, hemk_step st, hemk_ctl *ctl,
+                  void *s)
+{
+	k_seq_norm<<<1, BLOCK, 0, (hipStream_t)s>>>(n, u, u1, u2, phase, st, ctl);
+	return (int)hipGetLastError();
+}
+
+int hemk_finalize_seq(uint32_t n, const double *lam, const uint32_t *len,
+                      const hemk_ctl *ctl, double *tpm, double *est, void *s)
+{
+	k_fin_seq<<<HEMK_NORM_BLOCKS, BLOCK, 0, (hipStream_t)s>>>(n, lam, len,
+	                                                          ctl, tpm, est);
 	return (int)hipGetLastError();
 }
 

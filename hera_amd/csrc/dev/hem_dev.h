@@ -54,6 +54,8 @@ typedef struct {
 	uint32_t pad;
 	double   alpha;       /* last step length                              */
 	double   stop_min, stop_rel;
+	double   mean_r, mean_v;  /* first pass of the sequential norms        */
+	double   lam_sum;         /* sequential sum for the finalisation       */
 } hemk_ctl;
 
 /* Device view of one problem (all pointers are device pointers). */
@@ -74,7 +76,7 @@ typedef struct {
 	int step;        /* 0 sample sd, 1 L2                                 */
 	int clamp;       /* 0 zero, 1 u2/10                                   */
 	int alpha_clamp; /* alpha <= -1                                       */
-	int pad;
+	int alpha_given; /* 1: alpha already in ctl (sequential norms)        */
 } hemk_step;
 
 #define HEMK_NORM_BLOCKS 512
@@ -112,6 +114,18 @@ int hemk_extrapolate(uint32_t n, const double *u, const double *u1,
                      const double *u2, const double *part2, double *out64,
                      float *out32, hemk_step st, hemk_ctl *ctl, void *s);
 
+/* Sequential norms, bitwise the reference's order: one wave lane runs
+ * each left-to-right sum of src/hash_align.c:154-160 while other waves
+ * stage the next chunk in LDS.
+ *   phase 1 (sd): mean_r = (sum r)/n, mean_v = (sum v)/n into ctl
+ *   phase 2 (sd): centred sums -> n_r, n_v -> alpha into ctl
+ *   phase 3 (L2): sum r^2, sum v^2 -> alpha into ctl
+ *   phase 0:      lam_sum = sum u (the finalisation sum, src/EM.c:362-363),
+ *                 runs even when ctl->done is set. */
+int hemk_seq_norm(uint32_t n, const double *u, const double *u1,
+                  const double *u2, int phase, hemk_step st, hemk_ctl *ctl,
+                  void *s);
+
 /* End of one iteration: k++, evaluate the loop condition of src/EM.c:328
  * from the stop flags of the third map. */
 int hemk_iter_end(hemk_ctl *ctl, const uint32_t *stop_flags,
@@ -121,6 +135,9 @@ int hemk_iter_end(hemk_ctl *ctl, const uint32_t *stop_flags,
  * lam_t 1e6 / sum lam.  part: HEMK_NORM_BLOCKS doubles of scratch. */
 int hemk_finalize(uint32_t n, const double *lam, const uint32_t *len,
                   double *part, double *tpm, double *est, void *s);
+/* Same, with the sum taken from ctl->lam_sum (hemk_seq_norm phase 0). */
+int hemk_finalize_seq(uint32_t n, const double *lam, const uint32_t *len,
+                      const hemk_ctl *ctl, double *tpm, double *est, void *s);
 
 /* FP64 log-likelihood; part: HEMK_NORM_BLOCKS*2 doubles; result in *out. */
 int hemk_loglik(const hemk_prob *p, const uint32_t *unique, const double *lam,

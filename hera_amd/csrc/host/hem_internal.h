@@ -11,6 +11,8 @@
 /* Iterations per captured graph.  Iterations past convergence are no-ops
  * (every kernel reads the control block's done flag first). */
 #define HEM_ITERS_PER_GRAPH 4
+/* Distinct option sets whose graphs a handle keeps. */
+#define HEM_GRAPH_CACHE 8
 
 struct hem_problem {
 	pthread_mutex_t mu;
@@ -38,9 +40,11 @@ struct hem_problem {
 	hemk_ctl *h_ctl; /* pinned */
 	uint32_t n_flags;
 
-	/* cached iteration graph and the options it was captured for */
-	void *graph;
-	int graph_key;
+	/* captured iteration graphs and the options each was captured for;
+	 * kept until the handle is destroyed */
+	void *graph[HEM_GRAPH_CACHE];
+	int graph_key[HEM_GRAPH_CACHE];
+	int n_graph;
 };
 
 /* kernel kinds for per-kernel timing (hem_kernel_profile) */

@@ -125,7 +125,6 @@ int hem_problem_create(const hem_classes *c, hem_problem **out)
 	p->nnz = nnz;
 	p->group_cls = 4;
 	p->group_tx = 16;
-	p->graph_key = -1;
 
 	rc = hemd_stream_create(&p->stream);
 	if (!rc)
@@ -170,7 +169,9 @@ int hem_problem_create(const hem_classes *c, hem_problem **out)
 	DALLOC_H(p->d_D32, (size_t)T * 4);
 	DALLOC_H(p->d_rho, (C ? C : 1) * 8);
 	DALLOC_H(p->d_unique, (size_t)T * 4);
-	p->n_flags = hemk_gather_blocks(&(hemk_prob){.n_tx = T}, 1);
+	/* one stop flag per block of the transcript pass; the widest lane
+	 * group (64 lanes per transcript) launches the most blocks */
+	p->n_flags = hemk_gather_blocks(&(hemk_prob){.n_tx = T}, 64);
 	DALLOC_H(p->d_flags, (size_t)p->n_flags * 4);
 	DALLOC_H(p->d_part1, HEMK_NORM_BLOCKS * 2 * 8);
 	DALLOC_H(p->d_part2, HEMK_NORM_BLOCKS * 2 * 8);
@@ -238,7 +239,8 @@ int hem_problem_destroy(hem_problem *p)
 		return HEM_OK;
 	if (p->stream)
 		hemd_stream_sync(p->stream);
-	hemd_graph_destroy(p->graph);
+	for (int g = 0; g < p->n_graph; ++g)
+		hemd_graph_destroy(p->graph[g]);
 	hemd_free(p->d_cls_off);
 	hemd_free(p->d_cls_ref);
 	hemd_free(p->d_cnt32);

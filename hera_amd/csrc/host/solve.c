@@ -26,6 +26,7 @@ void hem_opts_default(hem_opts *o)
 	o->step = HEM_STEP_SD;
 	o->clamp = HEM_CLAMP_ZERO;
 	o->alpha_clamp = 1;
+	o->norm = HEM_NORM_EXACT;
 	o->min_iter = 50;
 	o->max_iter = 1000;
 	o->stop_min_value = 1e-3;
@@ -51,6 +52,8 @@ static int check_opts(const hem_opts *o)
 	if (o->clamp != HEM_CLAMP_ZERO && o->clamp != HEM_CLAMP_DIV10)
 		return hem_fail(HEM_ERR_INVALID, "unknown clamp form %d",
 		                o->clamp);
+	if (o->norm != HEM_NORM_EXACT && o->norm != HEM_NORM_TREE)
+		return hem_fail(HEM_ERR_INVALID, "unknown norm mode %d", o->norm);
 	if (o->fallback)
 		return hem_fail(HEM_ERR_INVALID,
 		                "the monotonicity fallback is not available "
@@ -61,8 +64,8 @@ static int check_opts(const hem_opts *o)
 static int graph_key(const hem_problem *p, const hem_opts *o)
 {
 	return o->precision | (o->step << 1) | (o->clamp << 2) |
-	       ((o->alpha_clamp != 0) << 3) | (p->group_cls << 4) |
-	       (p->group_tx << 12);
+	       ((o->alpha_clamp != 0) << 3) | (o->norm << 4) |
+	       (p->group_cls << 5) | (p->group_tx << 13);
 }
 
 /* Optional per-kernel timing: an event is recorded after every launch. */
@@ -90,11 +93,17 @@ static int enqueue_iteration(hem_problem *p, const hem_opts *o, prof_ctx *pc)
 	const void *gB = f32 ? (const void *)p->d_B32 : (const void *)p->d_B;
 	const void *gD = f32 ? (const void *)p->d_D32 : (const void *)p->d_D;
 	void *rho = f32 ? p->d_rho : (void *)p->d_nd;
-	hemk_step st = {o->step, o->clamp, o->alpha_clamp != 0, 0};
+	const int exact = o->norm == HEM_NORM_EXACT;
+	hemk_step st = {o->step, o->clamp, o->alpha_clamp != 0, exact};
 	const hemk_prob *dp = &p->dp;
 	void *s = p->stream;
 	uint32_t nfl = hemk_gather_blocks(dp, p->group_tx);
 	int gc = p->group_cls, gt = p->group_tx;
+
+	if (nfl > p->n_flags)
+		return hem_fail(HEM_ERR_RANGE,
+		                "transcript pass grid %u exceeds %u stop flags",
+		                nfl, p->n_flags);
 
 	HEM_TRY_DEV(hemk_denom(dp, prec, gA, rho, gc, p->d_ctl, s),
 	            "denom");
@@ -111,6 +120,25 @@ static int enqueue_iteration(hem_problem *p, const hem_opts *o, prof_ctx *pc)
 	                              p->d_C, NULL, NULL, gt, p->d_ctl, s),
 	            "gather");
 	MARK(HEM_PK_GATHER);
+	if (exact) {
+		/* the reference's left-to-right sums: means then centred
+		 * sums for the sd form, one pass for the L2 form */
+		if (o->step == HEM_STEP_SD) {
+			HEM_TRY_DEV(hemk_seq_norm(p->n_tx, p->d_A, p->d_B,
+			                          p->d_C, 1, st, p->d_ctl, s),
+			            "seq norm 1");
+			MARK(HEM_PK_NORM1);
+			HEM_TRY_DEV(hemk_seq_norm(p->n_tx, p->d_A, p->d_B,
+			                          p->d_C, 2, st, p->d_ctl, s),
+			            "seq norm 2");
+			MARK(HEM_PK_NORM2);
+		} else {
+			HEM_TRY_DEV(hemk_seq_norm(p->n_tx, p->d_A, p->d_B,
+			                          p->d_C, 3, st, p->d_ctl, s),
+			            "seq norm");
+			MARK(HEM_PK_NORM1);
+		}
+	} else {
 	HEM_TRY_DEV(hemk_norm_pass1(p->n_tx, p->d_A, p->d_B, p->d_C,
 	                            p->d_part1, o->step, p->d_ctl, s),
 	            "norm1");
@@ -121,6 +149,7 @@ static int enqueue_iteration(hem_problem *p, const hem_opts *o, prof_ctx *pc)
 		                            p->d_ctl, s),
 		            "norm2");
 		MARK(HEM_PK_NORM2);
+	}
 	}
 	HEM_TRY_DEV(hemk_extrapolate(p->n_tx, p->d_A, p->d_B, p->d_C,
 	                             o->step == HEM_STEP_SD ? p->d_part2
@@ -142,14 +171,20 @@ static int enqueue_iteration(hem_problem *p, const hem_opts *o, prof_ctx *pc)
 	return HEM_OK;
 }
 
-static int ensure_graph(hem_problem *p, const hem_opts *o)
+/* Returns the iteration graph for these options, capturing it on first
+ * use.  Graphs stay cached (no destroy while the handle lives). */
+static int ensure_graph(hem_problem *p, const hem_opts *o, void **graph)
 {
 	int key = graph_key(p, o), rc, e;
 	void *exec = NULL;
-	if (p->graph && p->graph_key == key)
-		return HEM_OK;
-	hemd_graph_destroy(p->graph);
-	p->graph = NULL;
+	for (int g = 0; g < p->n_graph; ++g)
+		if (p->graph_key[g] == key) {
+			*graph = p->graph[g];
+			return HEM_OK;
+		}
+	if (p->n_graph == HEM_GRAPH_CACHE)
+		return hem_fail(HEM_ERR_RANGE, "more than %d option sets on one "
+		                "problem handle", HEM_GRAPH_CACHE);
 	e = hemd_graph_begin(p->stream);
 	if (e)
 		return hem_dev_fail(e, "graph capture begin");
@@ -164,8 +199,9 @@ static int ensure_graph(hem_problem *p, const hem_opts *o)
 	e = hemd_graph_end(p->stream, &exec);
 	if (e)
 		return hem_dev_fail(e, "graph capture end");
-	p->graph = exec;
-	p->graph_key = key;
+	p->graph[p->n_graph] = exec;
+	p->graph_key[p->n_graph++] = key;
+	*graph = exec;
 	return HEM_OK;
 }
 
@@ -173,14 +209,15 @@ static int ensure_graph(hem_problem *p, const hem_opts *o)
 static int run_loop(hem_problem *p, const hem_opts *o, float *ms)
 {
 	int rc;
+	void *graph = NULL;
 	uint32_t launched = 0;
 	const uint32_t limit = o->max_iter > o->min_iter ? o->max_iter
 	                                                  : o->min_iter;
-	if ((rc = ensure_graph(p, o)))
+	if ((rc = ensure_graph(p, o, &graph)))
 		return rc;
 	HEM_TRY_DEV(hemd_event_record(p->ev0, p->stream), "event");
 	for (;;) {
-		HEM_TRY_DEV(hemd_graph_launch(p->graph, p->stream),
+		HEM_TRY_DEV(hemd_graph_launch(graph, p->stream),
 		            "graph launch");
 		launched += HEM_ITERS_PER_GRAPH;
 		if (launched < o->min_iter && launched < limit)
@@ -244,8 +281,17 @@ int hem_solve(hem_problem *p, const uint32_t *unique, const hem_opts *opts,
 	if ((rc = run_loop(p, &o, &ms)))
 		goto out;
 	/* finalisation into B (est) and C (tpm); log-likelihood of A */
-	rc = hemk_finalize(p->n_tx, p->d_A, p->d_len, p->d_part1, p->d_C,
-	                   p->d_B, p->stream);
+	if (o.norm == HEM_NORM_EXACT) {
+		hemk_step st0 = {0, 0, 0, 0};
+		rc = hemk_seq_norm(p->n_tx, p->d_A, NULL, NULL, 0, st0,
+		                   p->d_ctl, p->stream);
+		if (!rc)
+			rc = hemk_finalize_seq(p->n_tx, p->d_A, p->d_len, p->d_ctl,
+			                       p->d_C, p->d_B, p->stream);
+	} else {
+		rc = hemk_finalize(p->n_tx, p->d_A, p->d_len, p->d_part1,
+		                   p->d_C, p->d_B, p->stream);
+	}
 	if (!rc)
 		rc = hemk_loglik(&p->dp, p->d_unique, p->d_A, p->d_part2,
 		                 p->d_out, p->stream);
@@ -301,7 +347,7 @@ static void kernel_bytes(const hem_problem *p, int f32, double *b)
 		b[HEM_PK_GATHER] = 4 * (T + 1) + 4 * Z + 16 * C +
 		                   4 * T + 4 * T + 8 * T + 8 * T;
 	}
-	b[HEM_PK_NORM1] = 24 * T;
+	b[HEM_PK_NORM1] = 24 * T;   /* u, u1, u2 */
 	b[HEM_PK_NORM2] = 24 * T;
 	b[HEM_PK_EXTRAP] = 24 * T + 8 * T + (f32 ? 4 * T : 0);
 	b[HEM_PK_ITER_END] = 4.0 * hemk_gather_blocks(&p->dp, p->group_tx);

@@ -48,9 +48,12 @@ typedef struct {
 	const uint32_t *tx_len;     /* n_tx */
 } hem_classes;
 
-/* Bulk E-step precision.  FP32: gathered densities, class denominators and
- * per-transcript sums in FP32; the M-step, SQUAREM state, norms, stop test
- * and log-likelihood stay FP64.  FP64: everything FP64. */
+/* Bulk E-step precision.  FP64 (default): the class and transcript passes
+ * add in the reference's order (members in class order, classes in
+ * ascending id per transcript, (lam_t * n_c) / D_c per term).  FP32:
+ * gathered densities, class denominators and per-transcript sums in FP32;
+ * the M-step, SQUAREM state, norms, stop test and log-likelihood stay
+ * FP64. */
 #define HEM_PREC_FP32 0
 #define HEM_PREC_FP64 1
 
@@ -59,6 +62,14 @@ typedef struct {
  * L2: alpha = -||r||/||v|| (hera-EM/utility.c:16-41). */
 #define HEM_STEP_SD 0
 #define HEM_STEP_L2 1
+
+/* SQUAREM norm reductions.  EXACT: the left-to-right sums of Norm
+ * (src/hash_align.c:150-164) and of the finalisation (src/EM.c:362-363),
+ * each run by one device lane, so with HEM_PREC_FP64 every iterate is
+ * bitwise the reference's.  TREE: fixed-shape parallel sums (bitwise
+ * reproducible run to run, ~1 ulp from the reference's). */
+#define HEM_NORM_EXACT 0
+#define HEM_NORM_TREE  1
 
 /* Treatment of extrapolated values.  ZERO: negatives become 0
  * (src/EM.c:351).  DIV10: values <= 1e-300 become u2/10
@@ -71,6 +82,7 @@ typedef struct {
 	int      step;            /* HEM_STEP_*; default SD                    */
 	int      clamp;           /* HEM_CLAMP_*; default ZERO                 */
 	int      alpha_clamp;     /* 1: alpha = min(alpha, -1) (src/EM.c:346)  */
+	int      norm;            /* HEM_NORM_*; default EXACT                 */
 	uint32_t min_iter;        /* 50   (src/EM.c:325)                       */
 	uint32_t max_iter;        /* 1000 (src/EM.c:326)                       */
 	double   stop_min_value;  /* 1e-3 (src/EM.c:292)                       */
@@ -121,8 +133,10 @@ int hem_solve(hem_problem *p, const uint32_t *unique, const hem_opts *o,
               double *tpm, double *est, double *lambda, hem_stats *st);
 
 /* FP64 log-likelihood of densities lambda (n_tx values, host memory):
- *   l = sum_t u_t log lambda_t + sum_c n_c log D_c - sum_t len_t lambda_t,
- *   D_c = sum_{t in c} lambda_t, with 0 log 0 = 0.
+ *   l = sum_t u_t log lambda_t + sum_{c: D_c>0} n_c log D_c
+ *       - sum_t len_t lambda_t,
+ *   D_c = sum_{t in c} lambda_t, with 0 log 0 = 0.  Classes whose members
+ *   are all zero are skipped, as the map skips them (src/EM.c:279-280).
  * hera's map (src/EM.c:271-302) is the EM map of this objective. */
 int hem_loglik(hem_problem *p, const uint32_t *unique, const double *lambda,
                double *loglik);
@@ -132,7 +146,8 @@ int hem_loglik(hem_problem *p, const uint32_t *unique, const double *lambda,
  * graph, with a HIP event after every launch on the solve's stream, and
  * reports the device time and algorithmic bytes of each kernel kind.
  * Kinds: 0 class pass (denominators), 1 transcript pass (gather + M-step),
- * 2 norm pass 1, 3 norm pass 2, 4 extrapolation, 5 iteration end. */
+ * 2 norm pass 1, 3 norm pass 2 (tree partial sums, or the two sequential
+ * passes), 4 extrapolation, 5 iteration end. */
 #define HEM_KERNEL_KINDS 6
 typedef struct {
 	uint32_t iters;

@@ -233,7 +233,10 @@ int oracle_em_map(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
 }
 
 /* Poisson log-likelihood whose EM map is src/EM.c:271-302:
- *   l = sum_t u_t log l_t + sum_c n_c log D_c - sum_t len_t l_t. */
+ *   l = sum_t u_t log l_t + sum_{c: D_c > 0} n_c log D_c - sum_t len_t l_t.
+ * A class whose members are all exactly zero (the extrapolation clamp of
+ * src/EM.c:351 can do this) is skipped by the map (src/EM.c:279-280) and
+ * contributes nothing here either. */
 double oracle_loglik(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
                      const uint32_t *ref, const double *cnt,
                      const uint32_t *len, const uint32_t *count,
@@ -245,7 +248,7 @@ double oracle_loglik(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
 		double d = 0;
 		for (k = off[i]; k < off[i + 1]; ++k)
 			d += lam[ref[k]];
-		if (cnt[i] != 0)
+		if (cnt[i] != 0 && d > 0)
 			a += cnt[i] * log(d);
 	}
 	for (i = 0; i < n_tx; ++i) {

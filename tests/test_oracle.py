@@ -103,7 +103,8 @@ def test_two_transcript_fixed_point():
 
 def loglik_np(cls, u, lam):
     D, _ = class_terms(cls, lam)
-    a = np.sum(cls.cls_count[cls.cls_count > 0] * np.log(D[cls.cls_count > 0]))
+    live = (cls.cls_count > 0) & (D > 0)
+    a = np.sum(cls.cls_count[live] * np.log(D[live]))
     b = np.sum(u[u > 0] * np.log(lam[u > 0]))
     return a + b - np.dot(cls.tx_len.astype(float), lam)
 

@@ -55,6 +55,16 @@ class Classes:
                if members and off[-1] else np.zeros(0, dtype=np.uint32))
         return cls(n_tx, off, ref, as_f64(counts), as_u32(tx_len))
 
+    def tiling(self):
+        """Component tiling of the table (host only): dict(n_tiles, max_tx,
+        max_cls); n_tiles == 0 when a component is too large to tile."""
+        c = self.to_c()
+        n = _lib.C.c_uint32()
+        mt = _lib.C.c_uint32()
+        mc = _lib.C.c_uint32()
+        check(_lib.load().hem_classes_tiling(c, n, mt, mc))
+        return dict(n_tiles=n.value, max_tx=mt.value, max_cls=mc.value)
+
     def to_c(self):
         self.cls_off = as_u32(self.cls_off)
         self.cls_ref = as_u32(self.cls_ref)
@@ -164,6 +174,66 @@ class Problem:
                              ms_avg=pr.ms[i] / n if n else 0.0,
                              bytes_per_launch=pr.bytes[i])
         return out
+
+    def tiles(self):
+        n = _lib.C.c_uint32()
+        mt = _lib.C.c_uint32()
+        mc = _lib.C.c_uint32()
+        check(_lib.load().hem_problem_tiles(self._h, n, mt, mc))
+        return dict(n_tiles=n.value, max_tx=mt.value, max_cls=mc.value)
+
+    @staticmethod
+    def _bstats(st):
+        return {k: getattr(st, k) for k, _ in _lib.HemBatchStats._fields_}
+
+    def solve_batch(self, unique, cls_count=None, **kw):
+        """R solves sharing this table: unique (R, T) u32, cls_count (R, C)
+        float or None.  Returns dict(est, tpm, iters, stats)."""
+        u = as_u32(unique)
+        R = u.shape[0]
+        assert u.shape == (R, self.n_tx)
+        cc = None
+        if cls_count is not None:
+            cc = as_f64(cls_count)
+            assert cc.shape == (R, self.classes.n_cls)
+        o = opts(**kw)
+        est = np.empty((R, self.n_tx))
+        tpm = np.empty((R, self.n_tx))
+        it = np.empty(R, dtype=np.uint32)
+        st = _lib.HemBatchStats()
+        check(_lib.load().hem_solve_batch(
+            self._h, R, u32p(u), f64p(cc) if cc is not None else None, o,
+            f64p(est), f64p(tpm), u32p(it), st))
+        return dict(est=est, tpm=tpm, iters=it, stats=self._bstats(st))
+
+    def bootstrap(self, unique, seeds, **kw):
+        """hera's -b bootstrap, multinomial over all fragments; one seed per
+        replicate.  Returns dict(est, tpm, iters, stats)."""
+        u = as_u32(unique)
+        sd = np.ascontiguousarray(seeds, dtype=np.uint64)
+        R = len(sd)
+        o = opts(**kw)
+        est = np.empty((R, self.n_tx))
+        tpm = np.empty((R, self.n_tx))
+        it = np.empty(R, dtype=np.uint32)
+        st = _lib.HemBatchStats()
+        check(_lib.load().hem_bootstrap(
+            self._h, u32p(u), R, sd.ctypes.data_as(_lib.C.POINTER(_lib.C.c_uint64)),
+            o, f64p(est), f64p(tpm), u32p(it), st))
+        return dict(est=est, tpm=tpm, iters=it, stats=self._bstats(st))
+
+    def bootstrap_counts(self, unique, seeds):
+        """The resampled (unique, class) counts hem_bootstrap would solve."""
+        u = as_u32(unique)
+        sd = np.ascontiguousarray(seeds, dtype=np.uint64)
+        R = len(sd)
+        uo = np.empty((R, self.n_tx), dtype=np.uint32)
+        co = np.empty((R, self.classes.n_cls))
+        check(_lib.load().hem_bootstrap_counts(
+            self._h, u32p(u), R,
+            sd.ctypes.data_as(_lib.C.POINTER(_lib.C.c_uint64)), u32p(uo),
+            f64p(co)))
+        return uo, co
 
     def loglik(self, unique, lam):
         u = as_u32(unique)

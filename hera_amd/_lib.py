@@ -58,6 +58,14 @@ class HemKernelProfile(C.Structure):
                 ("bytes", C.c_double * HEM_KERNEL_KINDS)]
 
 
+class HemBatchStats(C.Structure):
+    _fields_ = [("n_rep", C.c_uint32), ("iters_min", C.c_uint32),
+                ("iters_max", C.c_uint32), ("n_converged", C.c_uint32),
+                ("chunk", C.c_uint32), ("n_tiles", C.c_uint32),
+                ("ms_resample", C.c_double), ("ms_solve", C.c_double),
+                ("ms_total", C.c_double)]
+
+
 class HemGenParams(C.Structure):
     _fields_ = [("n_tx", C.c_uint32), ("n_cls_all", C.c_uint32),
                 ("n_frag", C.c_uint64), ("cluster_min", C.c_uint32),
@@ -89,6 +97,18 @@ SIGNATURES = {
     "hem_loglik": (C.c_int, [C.c_void_p, _u32p, _f64p, _f64p]),
     "hem_profile_solve": (C.c_int, [C.c_void_p, _u32p, C.POINTER(HemOpts),
                                     C.POINTER(HemKernelProfile)]),
+    "hem_problem_tiles": (C.c_int, [C.c_void_p, _u32p, _u32p, _u32p]),
+    "hem_classes_tiling": (C.c_int, [C.POINTER(HemClasses), _u32p, _u32p,
+                                     _u32p]),
+    "hem_solve_batch": (C.c_int, [C.c_void_p, C.c_uint32, _u32p, _f64p,
+                                  C.POINTER(HemOpts), _f64p, _f64p, _u32p,
+                                  C.POINTER(HemBatchStats)]),
+    "hem_bootstrap": (C.c_int, [C.c_void_p, _u32p, C.c_uint32,
+                                C.POINTER(C.c_uint64), C.POINTER(HemOpts),
+                                _f64p, _f64p, _u32p,
+                                C.POINTER(HemBatchStats)]),
+    "hem_bootstrap_counts": (C.c_int, [C.c_void_p, _u32p, C.c_uint32,
+                                       C.POINTER(C.c_uint64), _u32p, _f64p]),
     "hem_gen_default": (None, [C.POINTER(HemGenParams), C.c_uint32,
                                C.c_uint32, C.c_uint64, C.c_uint64]),
     "hem_generate": (C.c_int, [C.POINTER(HemGenParams),

@@ -81,6 +81,89 @@ typedef struct {
 
 #define HEMK_NORM_BLOCKS 512
 
+/* ---- component tiles (device view) ---------------------------------------
+ * See host/tiles.c.  desc holds {tx_base, n_tx, cls_base, n_cls} per tile. */
+typedef struct {
+	uint32_t n_tiles, n_tx, n_cls;
+	uint32_t max_tx, max_cls;
+	const uint32_t *desc;
+	const uint32_t *tx;     /* tile position -> transcript id            */
+	const uint32_t *len;    /* tile position -> length                   */
+	const uint32_t *coff;   /* class position -> member offset (+1)      */
+	const uint8_t  *cmem;   /* member -> local transcript index          */
+	const uint32_t *toff;   /* tile position -> class-list offset (+1)   */
+	const uint16_t *tcls;   /* class list entry -> local class index     */
+} hemk_tiles;
+
+/* Per-replicate control of a batched solve (device arrays of length R). */
+typedef struct {
+	uint32_t R;
+	uint32_t min_iter, max_iter;
+	uint32_t pad;
+	double   stop_min, stop_rel;
+	uint32_t *k, *done, *conv, *notconv;
+	double   *alpha, *mean_r, *mean_v, *lam_sum;
+	uint32_t *active;   /* [1]: replicates still iterating */
+} hemk_rctl;
+
+/* Largest replicate chunk (lanes per tile block) that fits the LDS budget
+ * for the given tiling and count width (4 or 8 bytes). */
+int hemk_tile_chunk(const hemk_tiles *t, uint32_t R, int cnt_bytes);
+
+/* Tile map kernels.  Layouts: state A/B/C [T][R] FP64 (replicate
+ * fastest), unique counts [T][R] u32, class counts [tile position][R]
+ * (u32 when cnt_bytes == 4, FP64 when 8).
+ *   map12: u1 = M(A) -> B, u2 = M(B) -> C; with part != NULL also the
+ *          per-(tile, replicate) sums of r and v (Chan form) for the
+ *          parallel norms, part[(tile*R + r)*4 ...].
+ *   ex3:   extrapolate from A, B, C with alpha[r], map once -> A, stop test
+ *          -> notconv[r]. */
+int hemk_tile_map12(const hemk_tiles *t, hemk_rctl rc, int chunk,
+                    const void *cnt, int cnt_bytes, const uint32_t *uniq,
+                    const double *A, double *B, double *C, double *part,
+                    int step, void *s);
+int hemk_tile_ex3(const hemk_tiles *t, hemk_rctl rc, int chunk,
+                  const void *cnt, int cnt_bytes, const uint32_t *uniq,
+                  double *A, const double *B, const double *C, hemk_step st,
+                  void *s);
+/* Parallel norms of a batch: fold the tile partials of each replicate in
+ * tile order (Chan et al. for the sd form) -> alpha[r]. */
+int hemk_batch_alpha(const hemk_tiles *t, hemk_rctl rc, const double *part,
+                     uint32_t T, hemk_step st, void *s);
+/* Sequential norms of a batch: one lane per replicate runs the reference's
+ * left-to-right sums over t (phases as hemk_seq_norm; phase 0 fills
+ * lam_sum from A). */
+int hemk_batch_seq(hemk_rctl rc, uint32_t T, const double *A,
+                   const double *B, const double *C, int phase, hemk_step st,
+                   void *s);
+int hemk_batch_iter_end(hemk_rctl rc, void *s);
+int hemk_batch_reset(hemk_rctl rc, uint32_t T, double *A, void *s);
+/* est[r][t] = lam[t][r] len[t]; tpm[r][t] = lam[t][r] 1e6 / lam_sum[r]
+ * (zero when the sum is zero); lam_sum from hemk_batch_seq phase 0. */
+int hemk_batch_finalize(hemk_rctl rc, uint32_t T, const double *A,
+                        const uint32_t *len, double *est, double *tpm,
+                        void *s);
+/* out = transpose of in (rows x cols, row-major) for u32 count blocks. */
+int hemk_transpose_u32(const uint32_t *in, uint32_t rows, uint32_t cols,
+                       uint32_t *out, void *s);
+
+/* ---- bootstrap resampling (resample.hip) --------------------------------
+ * Categories k < K with integer weights w[k], N = sum w < 2^32, split into B
+ * chunks [chunk_start[j], chunk_start[j+1]) with weight prefix chunk_cw
+ * (chunk_cw[B] = N; a chunk holds at most HEMK_RS_CHUNK categories).  Draws
+ * Multinomial(N, w/N) for every replicate r (Philox key seeds[r]) into
+ * out[r*K + k].  m: R*B u32 scratch. */
+#define HEMK_RS_CHUNK 2048
+#define HEMK_RS_MAXB 7000
+int hemk_resample(const uint32_t *w, uint32_t K, const uint32_t *chunk_start,
+                  const uint32_t *chunk_cw, uint32_t B, uint32_t N,
+                  const uint64_t *seeds, uint32_t R, uint32_t *m,
+                  uint32_t *out, void *s);
+int hemk_fill_u32(uint32_t *p, size_t n, uint32_t v, void *s);
+/* out[q*R + r] = in[idx[q]*R + r] */
+int hemk_gather_rows_u32(const uint32_t *in, const uint32_t *idx,
+                         uint32_t rows, uint32_t R, uint32_t *out, void *s);
+
 /* ---- kernel launchers --------------------------------------------------- */
 /* lam_init: fill lam64 (and lam32 if non-NULL) with `value`, or copy from
  * init64 when non-NULL. */

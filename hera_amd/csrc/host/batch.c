@@ -1,0 +1,650 @@
+/*
+ * batch.c -- many EM instances over one class table, and hera's bootstrap.
+ *
+ * The reference runs each bootstrap replicate as its own estimate_count in a
+ * pthread (src/EM.c:123-150, batches of nThread joined at src/EM.c:243-255).
+ * Here all replicates run together: the state is [transcript][replicate]
+ * (replicate fastest), one workgroup per (component tile, replicate chunk)
+ * reads the tile's structure once for its whole chunk, and one captured graph
+ * iteration advances every replicate that is still iterating.  Each replicate
+ * keeps hera's own loop: its k, its step length, its stop test
+ * (src/EM.c:325-360).
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "hem_internal.h"
+
+static double now_ms(void)
+{
+	struct timespec ts;
+	clock_gettime(CLOCK_MONOTONIC, &ts);
+	return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+/* ---- tiles on the device ------------------------------------------------ */
+
+int hem_tiles_upload(hem_problem *p, const hem_tiles_host *h)
+{
+	void *s = p->stream;
+	size_t sz[8] = {
+	    (size_t)h->n_tiles * 16, (size_t)h->n_tx * 4, (size_t)h->n_tx * 4,
+	    ((size_t)h->n_cls + 1) * 4, (size_t)(h->nnz ? h->nnz : 1),
+	    ((size_t)h->n_tx + 1) * 4, (size_t)(h->nnz ? h->nnz : 1) * 2,
+	    (size_t)(h->n_cls ? h->n_cls : 1) * 4};
+	const void *src[8] = {h->desc, h->tx, h->len, h->coff,
+	                      h->cmem, h->toff, h->tcls, h->cls_orig};
+	for (int i = 0; i < 8; ++i) {
+		int e = hemd_malloc((void **)&p->d_tile_mem[i], sz[i]);
+		if (e)
+			return hem_fail(HEM_ERR_NOMEM, "tile upload: %s",
+			                hemd_error_string(e));
+		p->dev_bytes += sz[i];
+		e = hemd_h2d(p->d_tile_mem[i], src[i], sz[i], s);
+		if (e)
+			return hem_dev_fail(e, "tile upload");
+	}
+	p->dt.n_tiles = h->n_tiles;
+	p->dt.n_tx = h->n_tx;
+	p->dt.n_cls = h->n_cls;
+	p->dt.max_tx = h->max_tx;
+	p->dt.max_cls = h->max_cls;
+	p->dt.desc = p->d_tile_mem[0];
+	p->dt.tx = p->d_tile_mem[1];
+	p->dt.len = p->d_tile_mem[2];
+	p->dt.coff = p->d_tile_mem[3];
+	p->dt.cmem = (const uint8_t *)p->d_tile_mem[4];
+	p->dt.toff = p->d_tile_mem[5];
+	p->dt.tcls = (const uint16_t *)p->d_tile_mem[6];
+	p->d_cls_orig = p->d_tile_mem[7];
+	return HEM_OK;
+}
+
+int hem_problem_tiles(const hem_problem *p, uint32_t *n_tiles,
+                      uint32_t *max_tx, uint32_t *max_cls)
+{
+	if (!p)
+		return hem_fail(HEM_ERR_INVALID, "null problem");
+	if (n_tiles)
+		*n_tiles = p->dt.n_tiles;
+	if (max_tx)
+		*max_tx = p->dt.max_tx;
+	if (max_cls)
+		*max_cls = p->dt.max_cls;
+	return HEM_OK;
+}
+
+/* ---- workspace ---------------------------------------------------------- */
+
+static void ws_free_graphs(hem_batch_ws *w)
+{
+	for (int g = 0; g < w->n_graph; ++g)
+		hemd_graph_destroy(w->graph[g]);
+	w->n_graph = 0;
+}
+
+void hem_batch_free(hem_problem *p)
+{
+	hem_batch_ws *w = &p->bw;
+	ws_free_graphs(w);
+	hemd_free(w->A);
+	hemd_free(w->B);
+	hemd_free(w->C);
+	hemd_free(w->cat_rm);
+	hemd_free(w->catT);
+	hemd_free(w->cnt);
+	hemd_free(w->part);
+	hemd_free(w->m);
+	hemd_free(w->rs_plan);
+	hemd_free(w->seeds);
+	hemd_free(w->u32);
+	hemd_free(w->f64);
+	hemd_host_free(w->h_active);
+	memset(w, 0, sizeof(*w));
+	for (int i = 0; i < 8; ++i) {
+		hemd_free(p->d_tile_mem[i]);
+		p->d_tile_mem[i] = NULL;
+	}
+}
+
+static int ws_ensure(hem_problem *p, uint32_t R)
+{
+	hem_batch_ws *w = &p->bw;
+	const size_t T = p->n_tx, K = (size_t)p->n_tx + p->n_cls;
+	const size_t NC = p->dt.n_cls ? p->dt.n_cls : 1;
+	const size_t NT = p->dt.n_tiles ? p->dt.n_tiles : 1;
+	if (w->R >= R)
+		return HEM_OK;
+	{
+		/* keep the tile arrays; drop the per-R buffers */
+		void *keep[8];
+		memcpy(keep, p->d_tile_mem, sizeof(keep));
+		memset(p->d_tile_mem, 0, sizeof(keep));
+		hem_batch_free(p);
+		memcpy(p->d_tile_mem, keep, sizeof(keep));
+	}
+#define WALLOC(ptr, bytes)                                                  \
+	do {                                                                \
+		int e_ = hemd_malloc((void **)&(ptr), (bytes));             \
+		if (e_) {                                                   \
+			return hem_fail(HEM_ERR_NOMEM,                      \
+			                "batch workspace (%zu bytes): %s",  \
+			                (size_t)(bytes),                    \
+			                hemd_error_string(e_));             \
+		}                                                           \
+	} while (0)
+	WALLOC(w->A, T * R * 8);
+	WALLOC(w->B, T * R * 8);
+	WALLOC(w->C, T * R * 8);
+	WALLOC(w->cat_rm, K * R * 4);
+	WALLOC(w->catT, K * R * 4);
+	WALLOC(w->cnt, NC * R * 4);
+	WALLOC(w->part, NT * R * 4 * 8);
+	WALLOC(w->m, (size_t)R * HEMK_RS_MAXB * 4);
+	WALLOC(w->rs_plan, (K + 2 * ((size_t)HEMK_RS_MAXB + 1)) * 4);
+	WALLOC(w->seeds, (size_t)R * 8);
+	WALLOC(w->u32, ((size_t)4 * R + 1) * 4);
+	WALLOC(w->f64, (size_t)4 * R * 8);
+#undef WALLOC
+	if (hemd_host_alloc((void **)&w->h_active, 64))
+		return hem_fail(HEM_ERR_NOMEM, "pinned allocation");
+	w->R = R;
+	return HEM_OK;
+}
+
+static hemk_rctl rctl_for(hem_problem *p, uint32_t R, const hem_opts *o)
+{
+	hem_batch_ws *w = &p->bw;
+	const uint32_t cap = w->R;
+	hemk_rctl rc;
+	memset(&rc, 0, sizeof(rc));
+	rc.R = R;
+	rc.min_iter = o->min_iter;
+	rc.max_iter = o->max_iter;
+	rc.stop_min = o->stop_min_value;
+	rc.stop_rel = o->stop_rel_change;
+	rc.k = w->u32;
+	rc.done = w->u32 + cap;
+	rc.conv = w->u32 + 2 * (size_t)cap;
+	rc.notconv = w->u32 + 3 * (size_t)cap;
+	rc.active = w->u32 + 4 * (size_t)cap;
+	rc.alpha = w->f64;
+	rc.mean_r = w->f64 + cap;
+	rc.mean_v = w->f64 + 2 * (size_t)cap;
+	rc.lam_sum = w->f64 + 3 * (size_t)cap;
+	return rc;
+}
+
+/* ---- the iteration ------------------------------------------------------ */
+
+static int batch_iteration(hem_problem *p, const hem_opts *o, hemk_rctl rc,
+                           int chunk)
+{
+	hem_batch_ws *w = &p->bw;
+	void *s = p->stream;
+	const int exact = o->norm == HEM_NORM_EXACT;
+	hemk_step st = {o->step, o->clamp, o->alpha_clamp != 0, 1};
+	HEM_TRY_DEV(hemk_tile_map12(&p->dt, rc, chunk, w->cnt, 4, w->catT, w->A,
+	                            w->B, w->C, exact ? NULL : w->part, o->step,
+	                            s),
+	            "tile map12");
+	if (exact) {
+		if (o->step == HEM_STEP_SD) {
+			HEM_TRY_DEV(hemk_batch_seq(rc, p->n_tx, w->A, w->B, w->C, 1,
+			                           st, s),
+			            "batch seq 1");
+			HEM_TRY_DEV(hemk_batch_seq(rc, p->n_tx, w->A, w->B, w->C, 2,
+			                           st, s),
+			            "batch seq 2");
+		} else {
+			HEM_TRY_DEV(hemk_batch_seq(rc, p->n_tx, w->A, w->B, w->C, 3,
+			                           st, s),
+			            "batch seq");
+		}
+	} else {
+		HEM_TRY_DEV(hemk_batch_alpha(&p->dt, rc, w->part, p->n_tx, st, s),
+		            "batch alpha");
+	}
+	HEM_TRY_DEV(hemk_tile_ex3(&p->dt, rc, chunk, w->cnt, 4, w->catT, w->A,
+	                          w->B, w->C, st, s),
+	            "tile ex3");
+	HEM_TRY_DEV(hemk_batch_iter_end(rc, s), "batch iter end");
+	return HEM_OK;
+}
+
+static int batch_graph_key(const hem_opts *o, int chunk)
+{
+	return o->step | (o->clamp << 1) | ((o->alpha_clamp != 0) << 2) |
+	       (o->norm << 3) | (chunk << 4);
+}
+
+static int batch_graph(hem_problem *p, const hem_opts *o, hemk_rctl rc,
+                       int chunk, void **graph)
+{
+	hem_batch_ws *w = &p->bw;
+	const int key = batch_graph_key(o, chunk);
+	void *exec = NULL;
+	int rc2, e;
+	for (int g = 0; g < w->n_graph; ++g) {
+		const hem_opts *q = &w->graph_opts[g];
+		if (w->graph_key[g] == key && w->graph_R[g] == rc.R &&
+		    q->min_iter == o->min_iter && q->max_iter == o->max_iter &&
+		    q->stop_min_value == o->stop_min_value &&
+		    q->stop_rel_change == o->stop_rel_change) {
+			*graph = w->graph[g];
+			return HEM_OK;
+		}
+	}
+	if (w->n_graph == HEM_GRAPH_CACHE)
+		ws_free_graphs(w);
+	e = hemd_graph_begin(p->stream);
+	if (e)
+		return hem_dev_fail(e, "batch graph capture");
+	for (int i = 0; i < HEM_ITERS_PER_GRAPH; ++i) {
+		rc2 = batch_iteration(p, o, rc, chunk);
+		if (rc2) {
+			hemd_graph_end(p->stream, &exec);
+			hemd_graph_destroy(exec);
+			return rc2;
+		}
+	}
+	e = hemd_graph_end(p->stream, &exec);
+	if (e)
+		return hem_dev_fail(e, "batch graph capture end");
+	w->graph[w->n_graph] = exec;
+	w->graph_key[w->n_graph] = key;
+	w->graph_opts[w->n_graph] = *o;
+	w->graph_R[w->n_graph++] = rc.R;
+	*graph = exec;
+	return HEM_OK;
+}
+
+static int check_batch_opts(const hem_problem *p, const hem_opts *o)
+{
+	if (o->precision != HEM_PREC_FP64)
+		return hem_fail(HEM_ERR_INVALID,
+		                "batched solves compute in FP64 (precision %d)",
+		                o->precision);
+	if (o->step != HEM_STEP_SD && o->step != HEM_STEP_L2)
+		return hem_fail(HEM_ERR_INVALID, "unknown step form %d", o->step);
+	if (o->clamp != HEM_CLAMP_ZERO && o->clamp != HEM_CLAMP_DIV10)
+		return hem_fail(HEM_ERR_INVALID, "unknown clamp form %d",
+		                o->clamp);
+	if (o->norm != HEM_NORM_EXACT && o->norm != HEM_NORM_TREE)
+		return hem_fail(HEM_ERR_INVALID, "unknown norm mode %d", o->norm);
+	if (o->fallback)
+		return hem_fail(HEM_ERR_INVALID, "the monotonicity fallback is "
+		                                 "not available in this build");
+	if (!p->dt.n_tiles)
+		return hem_fail(HEM_ERR_RANGE,
+		                "a connected component of this class table exceeds "
+		                "the tile limits (%u transcripts / %u classes); "
+		                "batched solves need tiles", HEM_TILE_TX,
+		                HEM_TILE_CLS);
+	return HEM_OK;
+}
+
+/* Counts already in w->cat_rm ([R][K], caller order): transpose to
+ * [K][R] and gather the class rows into tile order. */
+static int stage_counts(hem_problem *p, uint32_t R)
+{
+	hem_batch_ws *w = &p->bw;
+	const uint32_t K = p->n_tx + p->n_cls;
+	HEM_TRY_DEV(hemk_transpose_u32(w->cat_rm, R, K, w->catT, p->stream),
+	            "count transpose");
+	HEM_TRY_DEV(hemk_gather_rows_u32(w->catT + (size_t)p->n_tx * R,
+	                                 p->d_cls_orig, p->dt.n_cls, R, w->cnt,
+	                                 p->stream),
+	            "class row gather");
+	return HEM_OK;
+}
+
+/* Runs the loop for R replicates whose counts are staged, then writes the
+ * outputs. */
+static int run_batch(hem_problem *p, const hem_opts *o, uint32_t R,
+                     double *est, double *tpm, uint32_t *iters,
+                     hem_batch_stats *st)
+{
+	hem_batch_ws *w = &p->bw;
+	hemk_rctl rc = rctl_for(p, R, o);
+	const int chunk = hemk_tile_chunk(&p->dt, R, 4);
+	void *graph = NULL;
+	uint32_t launched = 0;
+	const uint32_t limit = o->max_iter > o->min_iter ? o->max_iter
+	                                                  : o->min_iter;
+	float ms = 0;
+	int rc2;
+	hemk_step st0 = {0, 0, 0, 0};
+
+	if (chunk <= 0)
+		return hem_fail(HEM_ERR_RANGE, "tiles do not fit the LDS budget");
+	HEM_TRY_DEV(hemk_batch_reset(rc, p->n_tx, w->A, p->stream), "reset");
+	if ((rc2 = batch_graph(p, o, rc, chunk, &graph)))
+		return rc2;
+	HEM_TRY_DEV(hemd_event_record(p->ev0, p->stream), "event");
+	for (;;) {
+		HEM_TRY_DEV(hemd_graph_launch(graph, p->stream), "graph launch");
+		launched += HEM_ITERS_PER_GRAPH;
+		if (launched < o->min_iter && launched < limit)
+			continue;
+		HEM_TRY_DEV(hemd_d2h(w->h_active, rc.active, 4, p->stream),
+		            "active readback");
+		HEM_TRY_DEV(hemd_stream_sync(p->stream), "stream sync");
+		if (w->h_active[0] == 0)
+			break;
+		if (launched > limit + HEM_ITERS_PER_GRAPH)
+			return hem_fail(HEM_ERR_DEVICE,
+			                "batched loop did not terminate");
+	}
+	HEM_TRY_DEV(hemd_event_record(p->ev1, p->stream), "event");
+	/* finalisation: sequential sums per replicate, then est/tpm [r][t]
+	 * into B and C */
+	HEM_TRY_DEV(hemk_batch_seq(rc, p->n_tx, w->A, NULL, NULL, 0, st0,
+	                           p->stream),
+	            "final sums");
+	HEM_TRY_DEV(hemk_batch_finalize(rc, p->n_tx, w->A, p->d_len, w->B, w->C,
+	                                p->stream),
+	            "finalize");
+	if (est)
+		HEM_TRY_DEV(hemd_d2h(est, w->B, (size_t)p->n_tx * R * 8,
+		                     p->stream),
+		            "est readback");
+	if (tpm)
+		HEM_TRY_DEV(hemd_d2h(tpm, w->C, (size_t)p->n_tx * R * 8,
+		                     p->stream),
+		            "tpm readback");
+	{
+		uint32_t *hk = malloc((size_t)R * 4 * 2);
+		if (!hk)
+			return hem_fail(HEM_ERR_NOMEM, "host allocation");
+		rc2 = hemd_d2h(hk, rc.k, (size_t)R * 4, p->stream);
+		if (!rc2)
+			rc2 = hemd_d2h(hk + R, rc.conv, (size_t)R * 4, p->stream);
+		if (!rc2)
+			rc2 = hemd_stream_sync(p->stream);
+		if (rc2) {
+			free(hk);
+			return hem_dev_fail(rc2, "iteration readback");
+		}
+		if (iters)
+			memcpy(iters, hk, (size_t)R * 4);
+		if (st) {
+			st->n_rep = R;
+			st->iters_min = 0xFFFFFFFFu;
+			st->iters_max = 0;
+			st->n_converged = 0;
+			for (uint32_t r = 0; r < R; ++r) {
+				if (hk[r] < st->iters_min)
+					st->iters_min = hk[r];
+				if (hk[r] > st->iters_max)
+					st->iters_max = hk[r];
+				st->n_converged += hk[R + r] != 0;
+			}
+			st->chunk = (uint32_t)chunk;
+			st->n_tiles = p->dt.n_tiles;
+		}
+		free(hk);
+	}
+	HEM_TRY_DEV(hemd_event_elapsed(p->ev0, p->ev1, &ms), "event time");
+	if (st)
+		st->ms_solve = ms;
+	return HEM_OK;
+}
+
+static int load_batch_opts(const hem_opts *opts, hem_opts *o)
+{
+	if (opts)
+		*o = *opts;
+	else
+		hem_opts_default(o);
+	return HEM_OK;
+}
+
+int hem_solve_batch(hem_problem *p, uint32_t n_rep, const uint32_t *unique,
+                    const double *cls_count, const hem_opts *opts, double *est,
+                    double *tpm, uint32_t *iters, hem_batch_stats *st)
+{
+	hem_opts o;
+	int rc;
+	const double t0 = now_ms();
+	const uint32_t T = p ? p->n_tx : 0, C = p ? p->n_cls : 0;
+	const size_t K = (size_t)T + C;
+	uint32_t *h = NULL;
+	if (!p || !unique || n_rep == 0)
+		return hem_fail(HEM_ERR_INVALID, "null problem/counts or n_rep 0");
+	load_batch_opts(opts, &o);
+	if ((rc = check_batch_opts(p, &o)))
+		return rc;
+	if (!cls_count && !p->counts_integral)
+		return hem_fail(HEM_ERR_INVALID,
+		                "class counts must be integers below 2^32");
+	h = malloc(K * n_rep * 4);
+	if (!h)
+		return hem_fail(HEM_ERR_NOMEM, "host allocation");
+	for (uint32_t r = 0; r < n_rep; ++r) {
+		uint32_t *row = h + (size_t)r * K;
+		memcpy(row, unique + (size_t)r * T, (size_t)T * 4);
+		for (uint32_t c = 0; c < C; ++c) {
+			const double x = cls_count ? cls_count[(size_t)r * C + c]
+			                           : p->h_cls_count[c];
+			if (!(x >= 0) || x > 4294967295.0 || x != floor(x)) {
+				free(h);
+				return hem_fail(HEM_ERR_INVALID,
+				                "class count %g (replicate %u, class "
+				                "%u) is not an integer below 2^32",
+				                x, r, c);
+			}
+			row[T + c] = (uint32_t)x;
+		}
+	}
+	pthread_mutex_lock(&p->mu);
+	if ((rc = ws_ensure(p, n_rep)))
+		goto out;
+	rc = hemd_h2d(p->bw.cat_rm, h, K * n_rep * 4, p->stream);
+	if (rc) {
+		rc = hem_dev_fail(rc, "count upload");
+		goto out;
+	}
+	if ((rc = stage_counts(p, n_rep)))
+		goto out;
+	rc = run_batch(p, &o, n_rep, est, tpm, iters, st);
+	if (!rc && st)
+		st->ms_total = now_ms() - t0;
+out:
+	pthread_mutex_unlock(&p->mu);
+	free(h);
+	return rc;
+}
+
+/* ---- resampling ---------------------------------------------------------- */
+
+/* Splits K weighted categories into chunks: a category heavier than the
+ * target weight W is a chunk on its own; otherwise categories join the open
+ * chunk until it would pass W or hold HEMK_RS_CHUNK categories.  W starts at
+ * ceil(N / 2048) and doubles until the chunk count fits HEMK_RS_MAXB. */
+static int plan_chunks(const uint32_t *w, uint32_t K, uint32_t N,
+                       uint32_t *start, uint32_t *cw, uint32_t *B_out)
+{
+	uint64_t W = ((uint64_t)N + 2047) / 2048;
+	if (W == 0)
+		W = 1;
+	for (;;) {
+		uint32_t B = 0, n = 0;
+		uint64_t cur = 0, acc = 0;
+		int over = 0;
+		for (uint32_t k = 0; k < K && !over; ++k) {
+			if (w[k] > W) {
+				if (n) {
+					cw[B] = (uint32_t)acc;
+					start[B++] = k - n;
+					acc += cur;
+					n = 0;
+					cur = 0;
+				}
+				if (B >= HEMK_RS_MAXB) {
+					over = 1;
+					break;
+				}
+				cw[B] = (uint32_t)acc;
+				start[B++] = k;
+				acc += w[k];
+				continue;
+			}
+			if (n && (cur + w[k] > W || n == HEMK_RS_CHUNK)) {
+				cw[B] = (uint32_t)acc;
+				start[B++] = k - n;
+				acc += cur;
+				n = 0;
+				cur = 0;
+			}
+			if (B >= HEMK_RS_MAXB) {
+				over = 1;
+				break;
+			}
+			cur += w[k];
+			++n;
+		}
+		if (!over && n) {
+			if (B >= HEMK_RS_MAXB) {
+				over = 1;
+			} else {
+				cw[B] = (uint32_t)acc;
+				start[B++] = K - n;
+				acc += cur;
+			}
+		}
+		if (!over) {
+			start[B] = K;
+			cw[B] = (uint32_t)acc;
+			*B_out = B;
+			return HEM_OK;
+		}
+		W *= 2;
+	}
+}
+
+/* Resamples n_rep count vectors into w->cat_rm. */
+static int resample(hem_problem *p, const uint32_t *unique, uint32_t n_rep,
+                    const uint64_t *seeds, float *ms)
+{
+	hem_batch_ws *wk = &p->bw;
+	const uint32_t T = p->n_tx, C = p->n_cls;
+	const uint32_t K = T + C;
+	uint32_t *hw = malloc(((size_t)K + 2 * (HEMK_RS_MAXB + 1)) * 4);
+	uint32_t *start = hw + K, *cw = start + HEMK_RS_MAXB + 1, B = 0;
+	uint32_t *d = NULL;
+	uint64_t N = 0;
+	int rc;
+	if (!hw)
+		return hem_fail(HEM_ERR_NOMEM, "host allocation");
+	if (!p->counts_integral) {
+		free(hw);
+		return hem_fail(HEM_ERR_INVALID,
+		                "bootstrap needs integer class counts below 2^32");
+	}
+	for (uint32_t t = 0; t < T; ++t) {
+		hw[t] = unique[t];
+		N += unique[t];
+	}
+	for (uint32_t c = 0; c < C; ++c) {
+		hw[T + c] = (uint32_t)p->h_cls_count[c];
+		N += hw[T + c];
+	}
+	if (N > 0xFFFFFFFFull) {
+		free(hw);
+		return hem_fail(HEM_ERR_RANGE,
+		                "%llu fragments: the bootstrap needs fewer than "
+		                "2^32", (unsigned long long)N);
+	}
+	plan_chunks(hw, K, (uint32_t)N, start, cw, &B);
+	d = wk->rs_plan;
+	rc = hemd_h2d(d, hw, ((size_t)K + 2 * (HEMK_RS_MAXB + 1)) * 4,
+	              p->stream);
+	if (!rc)
+		rc = hemd_h2d(wk->seeds, seeds, (size_t)n_rep * 8, p->stream);
+	if (!rc)
+		rc = hemd_event_record(p->ev2, p->stream);
+	if (!rc)
+		rc = hemk_resample(d, K, d + K, d + K + HEMK_RS_MAXB + 1, B,
+		                   (uint32_t)N, wk->seeds, n_rep, wk->m, wk->cat_rm,
+		                   p->stream);
+	if (!rc)
+		rc = hemd_event_record(p->ev3, p->stream);
+	if (!rc)
+		rc = hemd_event_sync(p->ev3);
+	if (!rc && ms)
+		rc = hemd_event_elapsed(p->ev2, p->ev3, ms);
+	free(hw);
+	return rc ? hem_dev_fail(rc, "resampling") : HEM_OK;
+}
+
+int hem_bootstrap(hem_problem *p, const uint32_t *unique, uint32_t n_rep,
+                  const uint64_t *seeds, const hem_opts *opts, double *est,
+                  double *tpm, uint32_t *iters, hem_batch_stats *st)
+{
+	hem_opts o;
+	int rc;
+	float ms_rs = 0;
+	const double t0 = now_ms();
+	if (!p || !unique || !seeds || n_rep == 0)
+		return hem_fail(HEM_ERR_INVALID, "null argument or n_rep 0");
+	load_batch_opts(opts, &o);
+	if ((rc = check_batch_opts(p, &o)))
+		return rc;
+	pthread_mutex_lock(&p->mu);
+	if ((rc = ws_ensure(p, n_rep)))
+		goto out;
+	if ((rc = resample(p, unique, n_rep, seeds, &ms_rs)))
+		goto out;
+	if ((rc = stage_counts(p, n_rep)))
+		goto out;
+	rc = run_batch(p, &o, n_rep, est, tpm, iters, st);
+	if (!rc && st) {
+		st->ms_resample = ms_rs;
+		st->ms_total = now_ms() - t0;
+	}
+out:
+	pthread_mutex_unlock(&p->mu);
+	return rc;
+}
+
+int hem_bootstrap_counts(hem_problem *p, const uint32_t *unique,
+                         uint32_t n_rep, const uint64_t *seeds,
+                         uint32_t *unique_out, double *cls_out)
+{
+	int rc;
+	const uint32_t T = p ? p->n_tx : 0, C = p ? p->n_cls : 0;
+	const size_t K = (size_t)T + C;
+	uint32_t *h;
+	if (!p || !unique || !seeds || n_rep == 0)
+		return hem_fail(HEM_ERR_INVALID, "null argument or n_rep 0");
+	h = malloc(K * n_rep * 4);
+	if (!h)
+		return hem_fail(HEM_ERR_NOMEM, "host allocation");
+	pthread_mutex_lock(&p->mu);
+	rc = ws_ensure(p, n_rep);
+	if (!rc)
+		rc = resample(p, unique, n_rep, seeds, NULL);
+	if (!rc) {
+		int e = hemd_d2h(h, p->bw.cat_rm, K * n_rep * 4, p->stream);
+		if (!e)
+			e = hemd_stream_sync(p->stream);
+		if (e)
+			rc = hem_dev_fail(e, "count readback");
+	}
+	pthread_mutex_unlock(&p->mu);
+	if (!rc)
+		for (uint32_t r = 0; r < n_rep; ++r) {
+			const uint32_t *row = h + (size_t)r * K;
+			if (unique_out)
+				memcpy(unique_out + (size_t)r * T, row, (size_t)T * 4);
+			if (cls_out)
+				for (uint32_t c = 0; c < C; ++c)
+					cls_out[(size_t)r * C + c] = row[T + c];
+		}
+	free(h);
+	return rc;
+}

@@ -14,6 +14,29 @@
 /* Distinct option sets whose graphs a handle keeps. */
 #define HEM_GRAPH_CACHE 8
 
+/* Workspace of a batched solve, kept on the handle and grown on demand. */
+typedef struct {
+	uint32_t R;                    /* capacity in replicates            */
+	double *A, *B, *C;             /* [T][R] state                      */
+	uint32_t *cat_rm;              /* [R][K] counts, caller order       */
+	uint32_t *catT;                /* [K][R]: unique rows, class rows   */
+	uint32_t *cnt;                 /* [tile position][R] class counts   */
+	double *part;                  /* [tile][R][4] norm partials        */
+	uint32_t *m;                   /* [R][B] level-1 resample counts    */
+	uint32_t *rs_plan;             /* K weights + chunk starts + prefix */
+	uint64_t *seeds;               /* [R]                               */
+	uint32_t *u32;                 /* k, done, conv, notconv, active    */
+	double *f64;                   /* alpha, mean_r, mean_v, lam_sum    */
+	uint32_t *h_active;            /* pinned                            */
+	/* captured iterations; the loop bounds and stop thresholds are kernel
+	 * arguments, so a graph is reused only for identical options */
+	void *graph[HEM_GRAPH_CACHE];
+	int graph_key[HEM_GRAPH_CACHE];
+	uint32_t graph_R[HEM_GRAPH_CACHE];
+	hem_opts graph_opts[HEM_GRAPH_CACHE];
+	int n_graph;
+} hem_batch_ws;
+
 struct hem_problem {
 	pthread_mutex_t mu;
 	uint32_t n_tx, n_cls;
@@ -40,6 +63,16 @@ struct hem_problem {
 	hemk_ctl *h_ctl; /* pinned */
 	uint32_t n_flags;
 
+	/* component tiles (n_tiles == 0: some component is too large) */
+	hemk_tiles dt;
+	uint32_t *h_cls_orig;      /* tile position -> class id (host)      */
+	uint32_t *d_cls_orig;      /* same, device                          */
+	uint32_t *d_tile_mem[8];   /* device arrays behind dt               */
+	int counts_integral;       /* class counts are integers < 2^32      */
+	double *h_cls_count;       /* host copy of the class counts         */
+	hem_batch_ws bw;
+	void *ev2, *ev3;
+
 	/* captured iteration graphs and the options each was captured for;
 	 * kept until the handle is destroyed */
 	void *graph[HEM_GRAPH_CACHE];
@@ -47,9 +80,32 @@ struct hem_problem {
 	int n_graph;
 };
 
+/* Tile limits: a component larger than this keeps to the two-pass
+ * global kernels (u8 member indices bound the transcripts per tile). */
+#define HEM_TILE_TX  64
+#define HEM_TILE_CLS 320
+
+typedef struct {
+	uint32_t n_tiles, n_tx, n_cls, max_tx, max_cls;
+	uint64_t nnz;
+	uint32_t *desc, *tx, *len, *coff, *toff, *cls_orig;
+	uint8_t *cmem;
+	uint16_t *tcls;
+} hem_tiles_host;
+
+int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
+                    const uint32_t *cls_ref, const uint32_t *tx_len,
+                    const uint32_t *tx_off, const uint32_t *tx_ref,
+                    hem_tiles_host *h);
+void hem_tiles_free_host(hem_tiles_host *h);
+
 /* kernel kinds for per-kernel timing (hem_kernel_profile) */
 enum { HEM_PK_DENOM = 0, HEM_PK_GATHER, HEM_PK_NORM1, HEM_PK_NORM2,
        HEM_PK_EXTRAP, HEM_PK_ITER_END };
+
+/* batch.c */
+void hem_batch_free(hem_problem *p);
+int hem_tiles_upload(hem_problem *p, const hem_tiles_host *h);
 
 /* error.c */
 int hem_fail(int code, const char *fmt, ...);

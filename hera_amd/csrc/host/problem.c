@@ -131,6 +131,10 @@ int hem_problem_create(const hem_classes *c, hem_problem **out)
 		rc = hemd_event_create(&p->ev0);
 	if (!rc)
 		rc = hemd_event_create(&p->ev1);
+	if (!rc)
+		rc = hemd_event_create(&p->ev2);
+	if (!rc)
+		rc = hemd_event_create(&p->ev3);
 	if (rc) {
 		free(tx_off);
 		free(tx_ref);
@@ -210,14 +214,66 @@ int hem_problem_create(const hem_classes *c, hem_problem **out)
 		rc = hemd_h2d(p->d_len, c->tx_len, (size_t)T * 4, p->stream);
 	if (!rc)
 		rc = hemd_stream_sync(p->stream);
+	if (rc) {
+		free(tx_off);
+		free(tx_ref);
+		free(cnt32);
+		free(nd);
+		hem_problem_destroy(p);
+		return hem_dev_fail(rc, "class table upload");
+	}
+	/* host copy of the counts (batched solves broadcast them) */
+	p->h_cls_count = malloc((C ? C : 1) * sizeof(double));
+	if (!p->h_cls_count) {
+		free(tx_off);
+		free(tx_ref);
+		free(cnt32);
+		free(nd);
+		hem_problem_destroy(p);
+		return hem_fail(HEM_ERR_NOMEM, "host allocation failed");
+	}
+	if (C)
+		memcpy(p->h_cls_count, c->cls_count, (size_t)C * sizeof(double));
+	p->counts_integral = 1;
+	for (i = 0; i < C; ++i)
+		if (c->cls_count[i] > 4294967295.0 ||
+		    c->cls_count[i] != floor(c->cls_count[i]))
+			p->counts_integral = 0;
+	/* component tiles for the batched kernels; a table with a component
+	 * over the tile limits keeps only the two-pass kernels */
+	{
+		hem_tiles_host th;
+		int trc = hem_tiles_build(T, C, c->cls_off, c->cls_ref,
+		                          c->tx_len, tx_off, tx_ref, &th);
+		if (trc == HEM_OK) {
+			trc = hem_tiles_upload(p, &th);
+			p->h_cls_orig = th.cls_orig;
+			th.cls_orig = NULL;
+			hem_tiles_free_host(&th);
+			if (trc == HEM_OK)
+				trc = hemd_stream_sync(p->stream) ? HEM_ERR_DEVICE
+				                                  : HEM_OK;
+			if (trc) {
+				free(tx_off);
+				free(tx_ref);
+				free(cnt32);
+				free(nd);
+				hem_problem_destroy(p);
+				return trc;
+			}
+		} else if (trc != HEM_ERR_RANGE) {
+			free(tx_off);
+			free(tx_ref);
+			free(cnt32);
+			free(nd);
+			hem_problem_destroy(p);
+			return trc;
+		}
+	}
 	free(tx_off);
 	free(tx_ref);
 	free(cnt32);
 	free(nd);
-	if (rc) {
-		hem_problem_destroy(p);
-		return hem_dev_fail(rc, "class table upload");
-	}
 
 	p->dp.n_tx = T;
 	p->dp.n_cls = C;
@@ -241,6 +297,9 @@ int hem_problem_destroy(hem_problem *p)
 		hemd_stream_sync(p->stream);
 	for (int g = 0; g < p->n_graph; ++g)
 		hemd_graph_destroy(p->graph[g]);
+	hem_batch_free(p);
+	free(p->h_cls_orig);
+	free(p->h_cls_count);
 	hemd_free(p->d_cls_off);
 	hemd_free(p->d_cls_ref);
 	hemd_free(p->d_cnt32);
@@ -268,6 +327,10 @@ int hem_problem_destroy(hem_problem *p)
 		hemd_event_destroy(p->ev0);
 	if (p->ev1)
 		hemd_event_destroy(p->ev1);
+	if (p->ev2)
+		hemd_event_destroy(p->ev2);
+	if (p->ev3)
+		hemd_event_destroy(p->ev3);
 	if (p->stream)
 		hemd_stream_destroy(p->stream);
 	pthread_mutex_destroy(&p->mu);

@@ -1,0 +1,311 @@
+/*
+ * tiles.c -- connected components of the class graph, packed into LDS tiles.
+ *
+ * hera's classes never span unrelated transcripts (a class is the candidate
+ * set of one fragment, src/hash_align.c:1564-1602), so the class-transcript
+ * graph falls apart into small connected components.  One EM map touches a
+ * component's transcripts and classes only, so a workgroup that holds a whole
+ * component in LDS can run the class pass and the transcript pass without
+ * leaving the CU.  Components are packed, in order of their smallest
+ * transcript id, into tiles of at most HEM_TILE_TX transcripts and
+ * HEM_TILE_CLS classes.
+ *
+ * Per tile the device holds:
+ *   desc     {tx_base, n_tx, cls_base, n_cls}
+ *   tx       global transcript ids, ascending          (n_tx)
+ *   len      their lengths                             (n_tx)
+ *   coff     member offsets of the tile's classes      (n_cls + 1 shared)
+ *   cmem     member local indices (u8), hera's order    (members)
+ *   toff     class-list offsets of the tile's tx        (n_tx + 1 shared)
+ *   tcls     local class indices (u16) per transcript, ascending original
+ *            class id, a class once per occurrence -- the order hera's class
+ *            loop adds into overlap[t] (src/EM.c:276-285)
+ * Classes are renumbered in tile order ("tile position"); per-replicate
+ * class counts are stored in that order so a tile reads one contiguous
+ * block.  cls_orig maps a tile position back to the caller's class id.
+ * Classes without members never contribute (D_c = 0) and are dropped.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "hem_internal.h"
+
+static uint32_t uf_find(uint32_t *par, uint32_t x)
+{
+	while (par[x] != x) {
+		par[x] = par[par[x]];
+		x = par[x];
+	}
+	return x;
+}
+
+void hem_tiles_free_host(hem_tiles_host *h)
+{
+	if (!h)
+		return;
+	free(h->desc);
+	free(h->tx);
+	free(h->len);
+	free(h->coff);
+	free(h->cmem);
+	free(h->toff);
+	free(h->tcls);
+	free(h->cls_orig);
+	memset(h, 0, sizeof(*h));
+}
+
+/* Builds the tiling of a CSR class table with its CSC transpose
+ * (tx_off/tx_ref as built by hem_problem_create).  Returns HEM_ERR_RANGE
+ * when a component exceeds the tile limits. */
+int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
+                    const uint32_t *cls_ref, const uint32_t *tx_len,
+                    const uint32_t *tx_off, const uint32_t *tx_ref,
+                    hem_tiles_host *h)
+{
+	uint32_t *par = NULL, *comp = NULL, *ctx = NULL, *ccls = NULL;
+	uint32_t *ctile = NULL, *lidx = NULL, *lcidx = NULL, *pos = NULL;
+	uint32_t *tile_ntx = NULL, *tile_ncls = NULL;
+	uint32_t ncomp = 0, ntile = 0, i, k;
+	uint64_t zlive = 0;
+	int rc = HEM_OK;
+
+	memset(h, 0, sizeof(*h));
+	par = malloc((size_t)T * 4);
+	comp = malloc((size_t)T * 4);
+	lidx = malloc((size_t)T * 4);
+	lcidx = malloc((size_t)(C ? C : 1) * 4);
+	if (!par || !comp || !lidx || !lcidx) {
+		rc = hem_fail(HEM_ERR_NOMEM, "tiling: host allocation");
+		goto out;
+	}
+	for (i = 0; i < T; ++i)
+		par[i] = i;
+	for (i = 0; i < C; ++i) {
+		if (cls_off[i + 1] == cls_off[i])
+			continue;
+		uint32_t a = uf_find(par, cls_ref[cls_off[i]]);
+		for (k = cls_off[i] + 1; k < cls_off[i + 1]; ++k) {
+			uint32_t b = uf_find(par, cls_ref[k]);
+			if (a != b) {
+				/* keep the smaller id as root */
+				if (b < a) {
+					par[a] = b;
+					a = b;
+				} else {
+					par[b] = a;
+				}
+			}
+		}
+	}
+	/* component ids in order of their smallest transcript */
+	for (i = 0; i < T; ++i) {
+		uint32_t r = uf_find(par, i);
+		comp[i] = (r == i) ? ncomp++ : comp[r];
+	}
+	ctx = calloc(ncomp, 4);
+	ccls = calloc(ncomp, 4);
+	ctile = malloc((size_t)ncomp * 4);
+	if (!ctx || !ccls || !ctile) {
+		rc = hem_fail(HEM_ERR_NOMEM, "tiling: host allocation");
+		goto out;
+	}
+	for (i = 0; i < T; ++i)
+		ctx[comp[i]]++;
+	for (i = 0; i < C; ++i)
+		if (cls_off[i + 1] > cls_off[i]) {
+			ccls[comp[cls_ref[cls_off[i]]]]++;
+			zlive += cls_off[i + 1] - cls_off[i];
+		}
+	for (i = 0; i < ncomp; ++i)
+		if (ctx[i] > HEM_TILE_TX || ccls[i] > HEM_TILE_CLS) {
+			rc = hem_fail(HEM_ERR_RANGE,
+			              "component with %u transcripts / %u classes "
+			              "exceeds the tile limits (%u / %u)", ctx[i],
+			              ccls[i], HEM_TILE_TX, HEM_TILE_CLS);
+			goto out;
+		}
+	/* pack components into tiles */
+	{
+		uint32_t nt = 0, nc = 0;
+		for (i = 0; i < ncomp; ++i) {
+			if (nt && (nt + ctx[i] > HEM_TILE_TX ||
+			           nc + ccls[i] > HEM_TILE_CLS)) {
+				ntile++;
+				nt = nc = 0;
+			}
+			ctile[i] = ntile;
+			nt += ctx[i];
+			nc += ccls[i];
+		}
+		ntile++;
+	}
+	tile_ntx = calloc(ntile + 1, 4);
+	tile_ncls = calloc(ntile + 1, 4);
+	h->desc = malloc((size_t)ntile * 4 * 4);
+	h->tx = malloc((size_t)T * 4);
+	h->len = malloc((size_t)T * 4);
+	h->toff = malloc(((size_t)T + 1) * 4);
+	h->tcls = malloc((zlive ? zlive : 1) * 2);
+	pos = malloc(((size_t)ntile + 1) * 4);
+	if (!tile_ntx || !tile_ncls || !h->desc || !h->tx || !h->len ||
+	    !h->toff || !h->tcls || !pos) {
+		rc = hem_fail(HEM_ERR_NOMEM, "tiling: host allocation");
+		goto out;
+	}
+	for (i = 0; i < T; ++i)
+		tile_ntx[ctile[comp[i]]]++;
+	for (i = 0; i < C; ++i)
+		if (cls_off[i + 1] > cls_off[i])
+			tile_ncls[ctile[comp[cls_ref[cls_off[i]]]]]++;
+	/* transcripts in tile order, ascending within a tile */
+	{
+		uint32_t acc = 0;
+		for (i = 0; i < ntile; ++i) {
+			pos[i] = acc;
+			acc += tile_ntx[i];
+		}
+		for (i = 0; i < T; ++i) {
+			uint32_t tl = ctile[comp[i]];
+			uint32_t p = pos[tl]++;
+			h->tx[p] = i;
+			h->len[p] = tx_len[i];
+		}
+	}
+	/* local index = position minus the tile's first position */
+	{
+		uint32_t acc = 0;
+		for (i = 0; i < ntile; ++i) {
+			h->desc[4 * i + 0] = acc;
+			h->desc[4 * i + 1] = tile_ntx[i];
+			acc += tile_ntx[i];
+		}
+		for (i = 0; i < ntile; ++i)
+			for (k = 0; k < tile_ntx[i]; ++k)
+				lidx[h->tx[h->desc[4 * i] + k]] = k;
+	}
+	/* classes in tile order, ascending original id within a tile */
+	h->n_cls = 0;
+	for (i = 0; i < ntile; ++i)
+		h->n_cls += tile_ncls[i];
+	h->cls_orig = malloc((h->n_cls ? h->n_cls : 1) * 4);
+	h->coff = malloc(((size_t)h->n_cls + 1) * 4);
+	h->cmem = malloc((zlive ? zlive : 1));
+	if (!h->cls_orig || !h->coff || !h->cmem) {
+		rc = hem_fail(HEM_ERR_NOMEM, "tiling: host allocation");
+		goto out;
+	}
+	{
+		uint32_t acc = 0;
+		for (i = 0; i < ntile; ++i) {
+			pos[i] = acc;
+			h->desc[4 * i + 2] = acc;
+			h->desc[4 * i + 3] = tile_ncls[i];
+			acc += tile_ncls[i];
+		}
+		for (i = 0; i < C; ++i) {
+			if (cls_off[i + 1] == cls_off[i]) {
+				lcidx[i] = 0xFFFFFFFFu;
+				continue;
+			}
+			uint32_t tl = ctile[comp[cls_ref[cls_off[i]]]];
+			uint32_t p = pos[tl]++;
+			h->cls_orig[p] = i;
+			lcidx[i] = p - h->desc[4 * tl + 2];
+		}
+	}
+	/* member lists (local transcript index, hera's member order) */
+	{
+		uint64_t z = 0;
+		for (uint32_t q = 0; q < h->n_cls; ++q) {
+			uint32_t c = h->cls_orig[q];
+			h->coff[q] = (uint32_t)z;
+			for (k = cls_off[c]; k < cls_off[c + 1]; ++k)
+				h->cmem[z++] = (uint8_t)lidx[cls_ref[k]];
+		}
+		h->coff[h->n_cls] = (uint32_t)z;
+	}
+	/* per-transcript class lists in ascending original class id */
+	{
+		uint64_t z = 0;
+		for (uint32_t q = 0; q < T; ++q) {
+			uint32_t t = h->tx[q];
+			h->toff[q] = (uint32_t)z;
+			for (k = tx_off[t]; k < tx_off[t + 1]; ++k)
+				h->tcls[z++] = (uint16_t)lcidx[tx_ref[k]];
+		}
+		h->toff[T] = (uint32_t)z;
+	}
+	h->n_tiles = ntile;
+	h->n_tx = T;
+	h->nnz = zlive;
+	h->max_tx = h->max_cls = 0;
+	for (i = 0; i < ntile; ++i) {
+		if (tile_ntx[i] > h->max_tx)
+			h->max_tx = tile_ntx[i];
+		if (tile_ncls[i] > h->max_cls)
+			h->max_cls = tile_ncls[i];
+	}
+out:
+	if (rc)
+		hem_tiles_free_host(h);
+	free(par);
+	free(comp);
+	free(ctx);
+	free(ccls);
+	free(ctile);
+	free(lidx);
+	free(lcidx);
+	free(pos);
+	free(tile_ntx);
+	free(tile_ncls);
+	return rc;
+}
+
+int hem_classes_tiling(const hem_classes *c, uint32_t *n_tiles,
+                       uint32_t *max_tx, uint32_t *max_cls)
+{
+	uint32_t *tx_off, *tx_ref, *pos, i;
+	uint64_t nnz;
+	hem_tiles_host h;
+	int rc;
+	if (!c || !c->tx_len || !c->n_tx || (c->n_cls && !c->cls_off))
+		return hem_fail(HEM_ERR_INVALID, "null class table");
+	nnz = c->n_cls ? c->cls_off[c->n_cls] : 0;
+	for (uint64_t k = 0; k < nnz; ++k)
+		if (c->cls_ref[k] >= c->n_tx)
+			return hem_fail(HEM_ERR_INVALID, "transcript id out of range");
+	tx_off = calloc((size_t)c->n_tx + 1, 4);
+	pos = malloc((size_t)c->n_tx * 4);
+	tx_ref = malloc((nnz ? nnz : 1) * 4);
+	if (!tx_off || !pos || !tx_ref) {
+		free(tx_off);
+		free(pos);
+		free(tx_ref);
+		return hem_fail(HEM_ERR_NOMEM, "host allocation");
+	}
+	for (uint64_t k = 0; k < nnz; ++k)
+		tx_off[c->cls_ref[k] + 1]++;
+	for (i = 0; i < c->n_tx; ++i)
+		tx_off[i + 1] += tx_off[i];
+	memcpy(pos, tx_off, (size_t)c->n_tx * 4);
+	for (i = 0; i < c->n_cls; ++i)
+		for (uint32_t k = c->cls_off[i]; k < c->cls_off[i + 1]; ++k)
+			tx_ref[pos[c->cls_ref[k]]++] = i;
+	rc = hem_tiles_build(c->n_tx, c->n_cls, c->cls_off, c->cls_ref,
+	                     c->tx_len, tx_off, tx_ref, &h);
+	free(tx_off);
+	free(pos);
+	free(tx_ref);
+	if (rc == HEM_ERR_RANGE) {
+		*n_tiles = 0;
+		*max_tx = *max_cls = 0;
+		return HEM_OK;
+	}
+	if (rc)
+		return rc;
+	*n_tiles = h.n_tiles;
+	*max_tx = h.max_tx;
+	*max_cls = h.max_cls;
+	hem_tiles_free_host(&h);
+	return HEM_OK;
+}

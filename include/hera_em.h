@@ -141,6 +141,58 @@ int hem_solve(hem_problem *p, const uint32_t *unique, const hem_opts *o,
 int hem_loglik(hem_problem *p, const uint32_t *unique, const double *lambda,
                double *loglik);
 
+/* ---- batches: many EM instances over one class table --------------------
+ * Replicates share the device class table and run together, R lanes per
+ * component tile ([transcript][replicate] layouts, replicate fastest);
+ * each keeps its own iteration count, step length and stop test, exactly as
+ * R separate hem_solve calls would (bitwise so with HEM_PREC_FP64 and
+ * HEM_NORM_EXACT).  Batches compute in FP64 (precision must be FP64).
+ * They need every connected component of the class graph to fit a tile
+ * (hem_problem_tiles reports n_tiles = 0 otherwise); class counts must be
+ * integers below 2^32. */
+typedef struct {
+	uint32_t n_rep;
+	uint32_t iters_min, iters_max;  /* over replicates                   */
+	uint32_t n_converged;           /* replicates whose stop rule fired  */
+	uint32_t chunk;                 /* replicates per workgroup          */
+	uint32_t n_tiles;
+	double   ms_resample;           /* device time of the resampling     */
+	double   ms_solve;              /* device time of the iteration loop */
+	double   ms_total;              /* host wall time of the call        */
+} hem_batch_stats;
+
+/* Tiling of the class graph: number of tiles (0 when a component exceeds
+ * the tile limits), largest tile in transcripts and classes. */
+int hem_problem_tiles(const hem_problem *p, uint32_t *n_tiles,
+                      uint32_t *max_tx, uint32_t *max_cls);
+/* The same for a class table on the host, without a device: also the
+ * number of connected components and the largest one.  Returns HEM_OK
+ * with *n_tiles = 0 when a component exceeds the tile limits. */
+int hem_classes_tiling(const hem_classes *cls, uint32_t *n_tiles,
+                       uint32_t *max_tx, uint32_t *max_cls);
+
+/* n_rep EM solves with per-replicate counts: unique[r*n_tx + t] and
+ * cls_count[r*n_cls + c] (NULL: the table's own class counts for every
+ * replicate).  Outputs (any may be NULL): est[r*n_tx + t] (est_counts),
+ * tpm[r*n_tx + t], iters[r]. */
+int hem_solve_batch(hem_problem *p, uint32_t n_rep, const uint32_t *unique,
+                    const double *cls_count, const hem_opts *o, double *est,
+                    double *tpm, uint32_t *iters, hem_batch_stats *st);
+
+/* hera's -b bootstrap (src/EM.c:123-150, 233-261) with multinomial
+ * semantics: replicate r redraws all N = sum(unique) + sum(class counts)
+ * fragments with replacement (Philox-4x32-10 key seeds[r]), giving new
+ * unique and class counts, and solves them.  Outputs as hem_solve_batch. */
+int hem_bootstrap(hem_problem *p, const uint32_t *unique, uint32_t n_rep,
+                  const uint64_t *seeds, const hem_opts *o, double *est,
+                  double *tpm, uint32_t *iters, hem_batch_stats *st);
+
+/* The resampled counts hem_bootstrap would solve: unique_out[r*n_tx + t],
+ * cls_out[r*n_cls + c] (classes in the caller's order). */
+int hem_bootstrap_counts(hem_problem *p, const uint32_t *unique,
+                         uint32_t n_rep, const uint64_t *seeds,
+                         uint32_t *unique_out, double *cls_out);
+
 /* ---- per-kernel timing --------------------------------------------------
  * Runs the same solve as hem_solve, one iteration at a time outside the
  * graph, with a HIP event after every launch on the solve's stream, and

@@ -443,3 +443,158 @@ double oracle_now(void)
 	clock_gettime(CLOCK_MONOTONIC, &ts);
 	return ts.tv_sec + ts.tv_nsec * 1e-9;
 }
+
+/* ---- bootstrap resampling restatement ------------------------------------
+ * The library draws replicate counts as Multinomial(N, w/N) over K
+ * categories in two levels (hera_amd/csrc/dev/resample.hip): level 1 sends
+ * each of the N fragments to a chunk of categories, level 2 spreads each
+ * chunk's fragments over its categories.  Restated here sequentially,
+ * from the published generator (Philox-4x32-10, Salmon et al., SC'11) and
+ * Lemire's multiply-and-reject bounded integers, to check the device
+ * counts bit for bit. */
+void oracle_philox(uint64_t key, const uint32_t ctr[4], uint32_t out[4])
+{
+	uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
+	uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+	int r;
+	for (r = 0; r < 10; ++r) {
+		uint64_t a = (uint64_t)0xD2511F53u * c0;
+		uint64_t b = (uint64_t)0xCD9E8D57u * c2;
+		uint32_t n0 = (uint32_t)(b >> 32) ^ c1 ^ k0;
+		uint32_t n2 = (uint32_t)(a >> 32) ^ c3 ^ k1;
+		c1 = (uint32_t)b;
+		c3 = (uint32_t)a;
+		c0 = n0;
+		c2 = n2;
+		k0 += 0x9E3779B9u;
+		k1 += 0xBB67AE85u;
+	}
+	out[0] = c0;
+	out[1] = c1;
+	out[2] = c2;
+	out[3] = c3;
+}
+
+static uint32_t o_bounded(uint64_t key, uint32_t i, uint32_t j, uint32_t M)
+{
+	uint32_t thresh = (0u - M) % M, att, w;
+	for (att = 0;; ++att) {
+		uint32_t ctr[4] = {i, j, att, 0x62737472u}, h[4];
+		oracle_philox(key, ctr, h);
+		for (w = 0; w < 4; ++w) {
+			uint64_t m = (uint64_t)h[w] * M;
+			if ((uint32_t)m >= thresh)
+				return (uint32_t)(m >> 32);
+		}
+	}
+}
+
+/* last index j in [0, n) with a[j] <= y, for a[0] <= y < a[n] */
+static uint32_t o_last_le(const uint32_t *a, uint32_t n, uint32_t y)
+{
+	uint32_t lo = 0, hi = n;
+	while (hi - lo > 1) {
+		uint32_t mid = (lo + hi) / 2;
+		if (a[mid] <= y)
+			lo = mid;
+		else
+			hi = mid;
+	}
+	return lo;
+}
+
+#define O_RS_CHUNK 2048
+#define O_RS_MAXB 7000
+
+/* Chunk plan: a category heavier than the target weight W is a chunk on its
+ * own; other categories fill the open chunk until it would pass W or hold
+ * O_RS_CHUNK categories; W = ceil(N/2048), doubled until <= O_RS_MAXB
+ * chunks. */
+uint32_t oracle_resample_plan(const uint32_t *w, uint32_t K, uint32_t N,
+                              uint32_t *start, uint32_t *cw)
+{
+	uint64_t W = ((uint64_t)N + 2047) / 2048;
+	if (W == 0)
+		W = 1;
+	for (;;) {
+		uint32_t B = 0, n = 0, k;
+		uint64_t cur = 0, acc = 0;
+		for (k = 0; k < K; ++k) {
+			if (w[k] > W) {
+				if (n) {
+					cw[B] = (uint32_t)acc;
+					start[B++] = k - n;
+					acc += cur;
+					n = 0;
+					cur = 0;
+				}
+				cw[B] = (uint32_t)acc;
+				start[B++] = k;
+				acc += w[k];
+			} else {
+				if (n && (cur + w[k] > W || n == O_RS_CHUNK)) {
+					cw[B] = (uint32_t)acc;
+					start[B++] = k - n;
+					acc += cur;
+					n = 0;
+					cur = 0;
+				}
+				cur += w[k];
+				++n;
+			}
+			if (B > O_RS_MAXB)
+				break;
+		}
+		if (k == K && n) {
+			cw[B] = (uint32_t)acc;
+			start[B++] = K - n;
+			acc += cur;
+		}
+		if (k == K && B <= O_RS_MAXB) {
+			start[B] = K;
+			cw[B] = (uint32_t)acc;
+			return B;
+		}
+		W *= 2;
+	}
+}
+
+/* One replicate's counts out[K] for weights w (N = sum w < 2^32). */
+int oracle_resample(const uint32_t *w, uint32_t K, uint64_t seed,
+                    uint32_t *out)
+{
+	uint32_t *start = malloc((O_RS_MAXB + 2) * 4);
+	uint32_t *cw = malloc((O_RS_MAXB + 2) * 4);
+	uint32_t *m, *cdf, B, i, j, k;
+	uint64_t N64 = 0;
+	for (k = 0; k < K; ++k)
+		N64 += w[k];
+	if (!start || !cw || N64 > 0xFFFFFFFFull)
+		return 1;
+	B = oracle_resample_plan(w, K, (uint32_t)N64, start, cw);
+	m = calloc(B ? B : 1, 4);
+	cdf = malloc((O_RS_CHUNK + 1) * 4);
+	for (i = 0; i < (uint32_t)N64; ++i)
+		m[o_last_le(cw, B, o_bounded(seed, i, 0xFFFFFFFFu,
+		                             (uint32_t)N64))]++;
+	memset(out, 0, (size_t)K * 4);
+	for (j = 0; j < B; ++j) {
+		uint32_t k0 = start[j], nk = start[j + 1] - start[j], acc = 0;
+		if (nk == 1) {
+			out[k0] = m[j];
+			continue;
+		}
+		for (k = 0; k < nk; ++k) {
+			cdf[k] = acc;
+			acc += w[k0 + k];
+		}
+		cdf[nk] = acc;
+		for (i = 0; i < m[j]; ++i)
+			out[k0 + o_last_le(cdf, nk, o_bounded(seed, i, j, acc))]++;
+	}
+	free(start);
+	free(cw);
+	free(m);
+	free(cdf);
+	return 0;
+}

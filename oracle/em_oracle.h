@@ -56,4 +56,12 @@ int oracle_bootstrap_batch(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
                            const double *cls_counts, const oracle_opts *opt,
                            uint32_t n_thread, double *est, uint32_t *k_out);
 double oracle_now(void);
+
+
+/* bootstrap resampling restatement */
+void oracle_philox(uint64_t key, const uint32_t ctr[4], uint32_t out[4]);
+uint32_t oracle_resample_plan(const uint32_t *w, uint32_t K, uint32_t N,
+                              uint32_t *start, uint32_t *cw);
+int oracle_resample(const uint32_t *w, uint32_t K, uint64_t seed,
+                    uint32_t *out);
 #endif

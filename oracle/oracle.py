@@ -61,6 +61,13 @@ def load():
         lib.oracle_bootstrap_batch.argtypes = [
             C.c_uint32, C.c_uint32, _u32p, _u32p, _u32p, C.c_uint32, _u32p,
             _f64p, C.POINTER(OracleOpts), C.c_uint32, _f64p, _u32p]
+        lib.oracle_philox.restype = None
+        lib.oracle_philox.argtypes = [C.c_uint64, _u32p, _u32p]
+        lib.oracle_resample_plan.restype = C.c_uint32
+        lib.oracle_resample_plan.argtypes = [_u32p, C.c_uint32, C.c_uint32,
+                                             _u32p, _u32p]
+        lib.oracle_resample.restype = C.c_int
+        lib.oracle_resample.argtypes = [_u32p, C.c_uint32, C.c_uint64, _u32p]
         lib.oracle_now.restype = C.c_double
         lib.oracle_opts_default.argtypes = [C.POINTER(OracleOpts)]
         _LIB = lib
@@ -181,3 +188,28 @@ def bootstrap_batch(cls, counts, cls_counts, n_thread, **kw):
 
 def now():
     return load().oracle_now()
+
+
+def philox(key, ctr):
+    c = _u32(ctr)
+    out = np.empty(4, dtype=np.uint32)
+    load().oracle_philox(key, _p32(c), _p32(out))
+    return out
+
+
+def resample_plan(w):
+    w = _u32(w)
+    N = int(w.sum(dtype=np.uint64))
+    start = np.empty(7002, dtype=np.uint32)
+    cw = np.empty(7002, dtype=np.uint32)
+    B = load().oracle_resample_plan(_p32(w), len(w), N, _p32(start), _p32(cw))
+    return start[:B + 1].copy(), cw[:B + 1].copy()
+
+
+def resample(w, seed):
+    """One replicate of Multinomial(sum w, w / sum w), as the library draws
+    it (two-level, Philox + Lemire)."""
+    w = _u32(w)
+    out = np.empty(len(w), dtype=np.uint32)
+    assert load().oracle_resample(_p32(w), len(w), seed, _p32(out)) == 0
+    return out

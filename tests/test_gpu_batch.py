@@ -1,0 +1,159 @@
+"""GPU parity of batched solves and the bootstrap against the FP64 oracle.
+
+Every replicate of a batch is an independent hera EM on its own counts
+(src/EM.c:315-387); in the default mode (FP64, exact norms) each must equal
+the oracle run on the same counts bit for bit, iteration count included.
+The bootstrap's resampled counts must equal the oracle's restatement of the
+resampler bit for bit.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import hera_amd as H
+from hera_amd import _lib
+from conftest import tiny_problem
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def with_counts(cls, counts):
+    return H.Classes(cls.n_tx, cls.cls_off, cls.cls_ref,
+                     np.asarray(counts, dtype=np.float64), cls.tx_len)
+
+
+def check_replicates(cls, U, CC, res, bitwise=True, tol=0.0, **kw):
+    for r in range(U.shape[0]):
+        ora = O.estimate_count(with_counts(cls, CC[r]), U[r], **kw)
+        assert res["iters"][r] == ora["k"] or not bitwise, (r, res["iters"][r],
+                                                           ora["k"])
+        if bitwise:
+            np.testing.assert_array_equal(res["est"][r], ora["est"])
+            np.testing.assert_array_equal(res["tpm"][r], ora["tpm"])
+        else:
+            m = ora["tpm"] / 1e6 > 1e-7
+            e = np.abs(res["tpm"][r][m] - ora["tpm"][m]) / ora["tpm"][m]
+            say("replicate %d k=%d/%d max err %.3e" % (
+                r, res["iters"][r], ora["k"], e.max()))
+            assert e.max() <= tol
+
+
+def perturbed(cls, u, R, seed):
+    rng = np.random.default_rng(seed)
+    U = np.array([rng.poisson(u) for _ in range(R)], dtype=np.uint32)
+    CC = np.array([rng.poisson(cls.cls_count) for _ in range(R)],
+                  dtype=np.float64)
+    return U, CC
+
+
+@pytest.mark.parametrize("R", [1, 3, 17])
+def test_batch_tiny_bitwise(R):
+    cls, u = tiny_problem(R, n_tx=50, n_cls=200, dup=True)
+    U, CC = perturbed(cls, u, R, 100 + R)
+    p = H.Problem(cls)
+    assert p.tiles()["n_tiles"] >= 1
+    res = p.solve_batch(U, CC)
+    say("R=%d iters %s chunk %d" % (R, res["iters"].tolist(),
+                                    res["stats"]["chunk"]))
+    check_replicates(cls, U, CC, res)
+    # R = 1 batch equals the plain solve
+    if R == 1:
+        g = p.solve(U[0]) if np.array_equal(CC[0], cls.cls_count) else None
+        if g is not None:
+            np.testing.assert_array_equal(res["est"][0], g["est"])
+    p.close()
+
+
+def test_batch_shared_counts_equals_plain(cfg1):
+    cls, u = cfg1
+    p = H.Problem(cls)
+    U = np.stack([u, u])
+    res = p.solve_batch(U)   # table's own class counts for both replicates
+    g = p.solve(u)
+    p.close()
+    np.testing.assert_array_equal(res["est"][0], g["est"])
+    np.testing.assert_array_equal(res["est"][1], g["est"])
+    np.testing.assert_array_equal(res["tpm"][0], g["tpm"])
+    assert res["iters"][0] == g["stats"]["iters"]
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(step_form=1),
+                                dict(clamp_form=1), dict(stop_rel=1e-8)])
+def test_batch_cfg1_bitwise(cfg1, kw):
+    cls, u = cfg1
+    R = 6
+    U, CC = perturbed(cls, u, R, 7)
+    gmap = dict(step_form="step", clamp_form="clamp",
+                stop_rel="stop_rel_change")
+    p = H.Problem(cls)
+    res = p.solve_batch(U, CC, **{gmap[k]: v for k, v in kw.items()})
+    p.close()
+    say("cfg1 %s iters %s ms %.1f" % (kw, res["iters"].tolist(),
+                                      res["stats"]["ms_solve"]))
+    check_replicates(cls, U, CC, res, **kw)
+
+
+def test_batch_tree_norms(cfg1):
+    # parallel (Chan-combined) norms put alpha within an ulp of the
+    # reference's; the trajectory amplifies that to ~1e-6..1e-5 on weakly
+    # identifiable abundances, so this mode is held to 1e-4, not 1e-6
+    cls, u = cfg1
+    U, CC = perturbed(cls, u, 4, 9)
+    p = H.Problem(cls)
+    res = p.solve_batch(U, CC, norm=H.NORM_TREE)
+    p.close()
+    check_replicates(cls, U, CC, res, bitwise=False, tol=1e-4)
+
+
+def test_bootstrap_counts_bitwise(cfg1):
+    cls, u = cfg1
+    seeds = np.array([11, 12, 2 ** 63 + 5], dtype=np.uint64)
+    p = H.Problem(cls)
+    uo, co = p.bootstrap_counts(u, seeds)
+    p.close()
+    w = np.concatenate([u, cls.cls_count.astype(np.uint32)])
+    N = int(w.sum(dtype=np.uint64))
+    for r, s in enumerate(seeds):
+        ref = O.resample(w, int(s))
+        np.testing.assert_array_equal(uo[r], ref[:cls.n_tx])
+        np.testing.assert_array_equal(co[r], ref[cls.n_tx:])
+        assert int(uo[r].sum(dtype=np.uint64)) + int(co[r].sum()) == N
+
+
+def test_bootstrap_end_to_end_bitwise(cfg1):
+    cls, u = cfg1
+    seeds = np.arange(5, dtype=np.uint64) * 7919 + 3
+    p = H.Problem(cls)
+    res = p.bootstrap(u, seeds)
+    uo, co = p.bootstrap_counts(u, seeds)
+    again = p.bootstrap(u, seeds)
+    p.close()
+    say("bootstrap iters %s resample %.2f ms solve %.2f ms" % (
+        res["iters"].tolist(), res["stats"]["ms_resample"],
+        res["stats"]["ms_solve"]))
+    np.testing.assert_array_equal(res["est"], again["est"])
+    check_replicates(cls, uo, co, res)
+
+
+def test_batch_rejects_untileable_and_fp32():
+    members = [[i, i + 1] for i in range(99)]
+    big = H.Classes.from_lists(100, members, [3.0] * 99, [500] * 100)
+    p = H.Problem(big)
+    assert p.tiles()["n_tiles"] == 0
+    with pytest.raises(H.HemError) as ei:
+        p.solve_batch(np.ones((2, 100), np.uint32))
+    assert ei.value.code == _lib.HEM_ERR_RANGE
+    p.close()
+    cls, u = tiny_problem(1)
+    p = H.Problem(cls)
+    with pytest.raises(H.HemError) as ei:
+        p.solve_batch(np.stack([u, u]), precision=H.PREC_FP32)
+    assert ei.value.code == _lib.HEM_ERR_INVALID
+    p.close()

@@ -7,6 +7,10 @@ start, stop at 1e-8 relative change or 1000 iterations.  One step is one
 complete solve (src/EM.c:315-387) with the class table already resident on
 the device; value = SQUAREM iterations per second over the timed steps.
 
+Also config 3 (--bootstrap-reps, default 100): the bootstrap replicates in
+one batched run, against the same replicates solved one by one on the GPU
+and against the oracle on hera's own thread schedule.
+
 Modes (--modes, first one is the headline):
   fp64-exact  order-faithful FP64 E-step and the reference's sequential
               norms: every iterate bitwise the reference's;
@@ -49,6 +53,13 @@ def main():
     ap.add_argument("--modes", default="fp64-exact,fp64-tree,fp32-tree")
     ap.add_argument("--cpu-solves", type=int, default=1,
                     help="full oracle solves timed for cpu_baseline (0: skip)")
+    ap.add_argument("--bootstrap-reps", type=int, default=100,
+                    help="bootstrap replicates of the batched run (0: skip)")
+    ap.add_argument("--seq-sample", type=int, default=3,
+                    help="replicates solved one by one for the unbatched "
+                         "comparison")
+    ap.add_argument("--cpu-boot-threads", type=int, default=16,
+                    help="threads (= replicates) of the CPU bootstrap sample")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -157,8 +168,14 @@ def main():
         "upload_s": t_upload,
     }
 
-    if args.cpu_solves > 0:
+    boot = None
+    if args.bootstrap_reps > 0:
+        boot = bench_bootstrap(args, H, np, p, cls, u, sk)
+        out["bootstrap"] = boot
+
+    if args.cpu_solves > 0 or args.cpu_boot_threads > 0:
         from oracle import oracle as O
+    if args.cpu_solves > 0:
         n = args.cpu_solves
         t0 = O.now()
         k_cpu = 0
@@ -193,8 +210,81 @@ def main():
             }
         out["parity"] = par
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    if boot is not None:
+        if args.cpu_boot_threads > 0:
+            bootstrap_cpu(args, np, O, cls, u, boot)
+        boot.pop("_est_exact", None)
     p.close()
     print(json.dumps(out), flush=True)
+
+
+def bench_bootstrap(args, H, np, p, cls, u, sk):
+    """Config 3: R bootstrap replicates (multinomial over all fragments, one
+    seed per replicate) in one batched run, against the same replicates
+    solved one after another by the plain GPU path."""
+    R = args.bootstrap_reps
+    seeds = np.arange(R, dtype=np.uint64) + 1
+    res = {"n_rep": R, "seeds": "1..%d" % R}
+    for name, nm in (("fp64-exact", H.NORM_EXACT), ("fp64-tree", H.NORM_TREE)):
+        p.bootstrap(u, seeds[:2], norm=nm, **sk)   # capture the graphs
+        t = time.perf_counter()
+        b = p.bootstrap(u, seeds, norm=nm, **sk)
+        dt = time.perf_counter() - t
+        st = b["stats"]
+        res[name] = dict(reps_per_s=R / dt, seconds=dt,
+                         ms_resample=st["ms_resample"],
+                         ms_solve=st["ms_solve"],
+                         iters_min=st["iters_min"], iters_max=st["iters_max"],
+                         iters_total=int(b["iters"].sum()),
+                         chunk=st["chunk"], n_tiles=st["n_tiles"])
+        if name == "fp64-exact":
+            res["_est_exact"] = b["est"]
+        log("bootstrap %-10s R=%d %.3fs (%.1f reps/s) resample %.1f ms "
+            "iters %d..%d" % (name, R, dt, R / dt, st["ms_resample"],
+                              st["iters_min"], st["iters_max"]))
+    # unbatched: the same resampled counts through the plain solve, one
+    # replicate after another (a sample of n_seq, scaled to R)
+    n_seq = max(1, min(args.seq_sample, R))
+    uo, co = p.bootstrap_counts(u, seeds[:n_seq])
+    for name, nm in (("fp64-exact", H.NORM_EXACT), ("fp64-tree", H.NORM_TREE)):
+        t = time.perf_counter()
+        for r in range(n_seq):
+            p.set_counts(co[r])
+            p.solve(uo[r], norm=nm, **sk)
+        dt = (time.perf_counter() - t) / n_seq
+        res[name]["sequential_plain_s_per_rep"] = dt
+        res[name]["sequential_plain_reps_per_s"] = 1.0 / dt
+        res[name]["speedup_vs_sequential_gpu"] = res[name]["reps_per_s"] * dt
+    p.set_counts(cls.cls_count)
+    res["sequential_sample"] = "%d replicates solved one by one (hem_solve " \
+                               "after hem_problem_set_counts), scaled" % n_seq
+    return res
+
+
+def bootstrap_cpu(args, np, O, cls, u, boot):
+    """The oracle on hera's own bootstrap schedule (batches of n threads,
+    joined per batch; each thread resamples and solves), on one batch."""
+    import os
+    nthr = min(args.cpu_boot_threads, os.cpu_count() or 1)
+    seeds = np.arange(nthr, dtype=np.uint64) + 1
+    t0 = O.now()
+    est, ks = O.bootstrap_resampled(cls, u, seeds, nthr,
+                                    stop_rel=args.stop_rel)
+    dt = O.now() - t0
+    boot["cpu_baseline"] = {
+        "value": nthr / dt, "unit": "bootstrap replicates/s",
+        "cores": nthr, "kind": "port",
+        "sample": "one batch of %d replicates (seeds 1..%d) on %d threads: "
+                  "resample + FP64 restatement of src/EM.c, %.1fs"
+                  % (nthr, nthr, nthr, dt)}
+    gpu = boot.pop("_est_exact")
+    same = [bool(np.array_equal(gpu[r], est[r])) for r in range(nthr)]
+    boot["parity_exact_vs_oracle"] = {"replicates": nthr,
+                                      "est_bitwise_equal": all(same)}
+    for name in ("fp64-exact", "fp64-tree"):
+        boot[name]["speedup_vs_cpu"] = boot[name]["reps_per_s"] / (nthr / dt)
+    log("bootstrap cpu: %d reps on %d threads %.1fs; exact bitwise %s" % (
+        nthr, nthr, dt, all(same)))
 
 
 if __name__ == "__main__":

@@ -135,6 +135,13 @@ class Problem:
         except Exception:
             pass
 
+    def set_counts(self, cls_count):
+        """Replaces the class counts (the structure stays on the device)."""
+        cc = as_f64(cls_count)
+        assert cc.shape == (self.classes.n_cls,)
+        check(_lib.load().hem_problem_set_counts(self._h, f64p(cc)))
+        self._counts = cc
+
     def info(self):
         T = _lib.C.c_uint32()
         Cn = _lib.C.c_uint32()

@@ -333,6 +333,39 @@ __global__ __launch_bounds__(BLOCK) void k_extrap(
 		ctl->alpha = alpha;
 }
 
+// acc + x[0] + ... + x[cnt-1] left to right, reading the next eight values
+// from LDS while the current eight are added.
+__device__ __forceinline__ double seq_chain(double acc, const double *x,
+                                            uint32_t cnt)
+{
+#pragma clang fp contract(off)
+	uint32_t j = 0;
+	if (cnt >= 8) {
+		double a[8];
+#pragma unroll
+		for (int q = 0; q < 8; ++q)
+			a[q] = x[q];
+		for (j = 8; j + 8 <= cnt; j += 8) {
+			double b[8];
+#pragma unroll
+			for (int q = 0; q < 8; ++q)
+				b[q] = x[j + q];
+#pragma unroll
+			for (int q = 0; q < 8; ++q)
+				acc += a[q];
+#pragma unroll
+			for (int q = 0; q < 8; ++q)
+				a[q] = b[q];
+		}
+#pragma unroll
+		for (int q = 0; q < 8; ++q)
+			acc += a[q];
+	}
+	for (; j < cnt; ++j)
+		acc += x[j];
+	return acc;
+}
+
 // Sequential sums on one lane per chain (see hemk_seq_norm).  Waves 2-3
 // stage chunk c (the element values, computed in parallel) in LDS while
 // lane 0 of waves 0-1 adds chunk c-1 left to right.
@@ -386,10 +419,7 @@ __global__ __launch_bounds__(BLOCK) void k_seq_norm(
 			const int slot = (c - 1) & 1;
 			const uint32_t cnt = min((uint32_t)SEQ_TB,
 			                         n - (c - 1) * SEQ_TB);
-			const double *x = buf[slot][w];
-#pragma unroll 16
-			for (uint32_t j = 0; j < cnt; ++j)
-				acc += x[j];
+			acc = seq_chain(acc, buf[slot][w], cnt);
 		}
 		__syncthreads();
 	}

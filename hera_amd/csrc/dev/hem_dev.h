@@ -85,7 +85,7 @@ typedef struct {
  * See host/tiles.c.  desc holds {tx_base, n_tx, cls_base, n_cls} per tile. */
 typedef struct {
 	uint32_t n_tiles, n_tx, n_cls;
-	uint32_t max_tx, max_cls;
+	uint32_t max_tx, max_cls, max_nnz;
 	const uint32_t *desc;
 	const uint32_t *tx;     /* tile position -> transcript id            */
 	const uint32_t *len;    /* tile position -> length                   */
@@ -103,6 +103,7 @@ typedef struct {
 	double   stop_min, stop_rel;
 	uint32_t *k, *done, *conv, *notconv;
 	double   *alpha, *mean_r, *mean_v, *lam_sum;
+	double   *q_r, *q_v;  /* second-pass sums of the sequential norms */
 	uint32_t *active;   /* [1]: replicates still iterating */
 } hemk_rctl;
 
@@ -115,17 +116,24 @@ int hemk_tile_chunk(const hemk_tiles *t, uint32_t R, int cnt_bytes);
  * (u32 when cnt_bytes == 4, FP64 when 8).
  *   map12: u1 = M(A) -> B, u2 = M(B) -> C; with part != NULL also the
  *          per-(tile, replicate) sums of r and v (Chan form) for the
- *          parallel norms, part[(tile*R + r)*4 ...].
- *   ex3:   extrapolate from A, B, C with alpha[r], map once -> A, stop test
- *          -> notconv[r]. */
+ *          parallel norms, part[(tile*R + r)*4 ...]; with xr != NULL the
+ *          r and v values in replicate-major rows xr/xv[r*T + t] for the
+ *          sequential norms.
+ *   ex3:   extrapolate from A, B, C (alpha[r] when st.alpha_given, else
+ *          from q_r/q_v), map once -> A, stop test -> notconv[r]. */
 int hemk_tile_map12(const hemk_tiles *t, hemk_rctl rc, int chunk,
                     const void *cnt, int cnt_bytes, const uint32_t *uniq,
                     const double *A, double *B, double *C, double *part,
-                    int step, void *s);
+                    double *xr, double *xv, uint32_t T, int step, void *s);
 int hemk_tile_ex3(const hemk_tiles *t, hemk_rctl rc, int chunk,
                   const void *cnt, int cnt_bytes, const uint32_t *uniq,
-                  double *A, const double *B, const double *C, hemk_step st,
-                  void *s);
+                  double *A, const double *B, const double *C, uint32_t T,
+                  hemk_step st, void *s);
+/* Sequential norms of a batch over the rows xr/xv: one chain per
+ * (replicate, r|v) block.  Phase 1 -> mean_r/mean_v; phase 2 (centred) and
+ * phase 3 (squares) -> q_r/q_v. */
+int hemk_batch_chain(hemk_rctl rc, uint32_t T, const double *xr,
+                     const double *xv, int phase, void *s);
 /* Parallel norms of a batch: fold the tile partials of each replicate in
  * tile order (Chan et al. for the sd form) -> alpha[r]. */
 int hemk_batch_alpha(const hemk_tiles *t, hemk_rctl rc, const double *part,

@@ -15,7 +15,7 @@
 #include "hem_dev.h"
 
 #define TBLOCK 256
-#define TLDS_BUDGET (64 * 1024)
+#define TLDS_BUDGET (40 * 1024)
 
 namespace {
 
@@ -33,14 +33,43 @@ __device__ __forceinline__ TileCtx load_tile(const hemk_tiles &t, uint32_t i)
 	return c;
 }
 
+// The tile's structure staged in LDS: member lists and class lists with
+// offsets relative to the tile (u16; a tile holds at most max_nnz members).
+struct TileStruct {
+	uint16_t *coff;  // nc + 1
+	uint16_t *toff;  // nt + 1
+	uint8_t *cmem;   // members
+	uint16_t *tcls;  // class-list entries
+};
+
+__device__ __forceinline__ void stage_struct(const hemk_tiles &tl,
+                                             const TileCtx &tc,
+                                             const TileStruct &ts)
+{
+	const uint32_t mb = tl.coff[tc.cls_base];
+	const uint32_t me = tl.coff[tc.cls_base + tc.nc];
+	const uint32_t jb = tl.toff[tc.tx_base];
+	const uint32_t je = tl.toff[tc.tx_base + tc.nt];
+	for (uint32_t i = threadIdx.x; i <= tc.nc; i += TBLOCK)
+		ts.coff[i] = (uint16_t)(tl.coff[tc.cls_base + i] - mb);
+	for (uint32_t i = threadIdx.x; i <= tc.nt; i += TBLOCK)
+		ts.toff[i] = (uint16_t)(tl.toff[tc.tx_base + i] - jb);
+	for (uint32_t i = threadIdx.x; i < me - mb; i += TBLOCK)
+		ts.cmem[i] = tl.cmem[mb + i];
+	for (uint32_t i = threadIdx.x; i < je - jb; i += TBLOCK)
+		ts.tcls[i] = tl.tcls[jb + i];
+}
+
 // One EM map of the tile for the block's replicate chunk, lin -> lout in
 // LDS (and to the global [T][R] vector gout when given).  With stopf the
-// stop test of src/EM.c:292-297 marks stopf[r].
+// stop test of src/EM.c:292-297 marks stopf[r].  Terms are formed four at a
+// time (independent loads and divisions in flight) and added in order.
 template <typename TC, int RC>
 __device__ __forceinline__ void tile_map(
-    const hemk_tiles &tl, const TileCtx &tc, const double *lin, double *lout,
-    double *D, const TC *n, const uint32_t *u, const int *alive, double *gout,
-    uint32_t R, uint32_t r0, int *stopf, double smin, double srel)
+    const hemk_tiles &tl, const TileCtx &tc, const TileStruct &ts,
+    const double *lin, double *lout, double *D, const TC *n,
+    const uint32_t *u, const int *alive, double *gout, uint32_t R,
+    uint32_t r0, int *stopf, double smin, double srel)
 {
 	const int r = threadIdx.x % RC;
 	const uint32_t e0 = threadIdx.x / RC;
@@ -50,11 +79,21 @@ __device__ __forceinline__ void tile_map(
 	for (uint32_t c = e0; c < tc.nc; c += ES) {
 		if (!live)
 			continue;
-		const uint32_t k0 = tl.coff[tc.cls_base + c];
-		const uint32_t k1 = tl.coff[tc.cls_base + c + 1];
+		uint32_t k = ts.coff[c];
+		const uint32_t k1 = ts.coff[c + 1];
 		double acc = 0.0;
-		for (uint32_t k = k0; k < k1; ++k)
-			acc += lin[tl.cmem[k] * RC + r];
+		for (; k + 4 <= k1; k += 4) {
+			const double x0 = lin[ts.cmem[k] * RC + r];
+			const double x1 = lin[ts.cmem[k + 1] * RC + r];
+			const double x2 = lin[ts.cmem[k + 2] * RC + r];
+			const double x3 = lin[ts.cmem[k + 3] * RC + r];
+			acc += x0;
+			acc += x1;
+			acc += x2;
+			acc += x3;
+		}
+		for (; k < k1; ++k)
+			acc += lin[ts.cmem[k] * RC + r];
 		D[c * RC + r] = acc;
 	}
 	__syncthreads();
@@ -63,14 +102,25 @@ __device__ __forceinline__ void tile_map(
 		if (!live)
 			continue;
 		const double l = lin[t * RC + r];
-		const uint32_t j0 = tl.toff[tc.tx_base + t];
-		const uint32_t j1 = tl.toff[tc.tx_base + t + 1];
+		uint32_t j = ts.toff[t];
+		const uint32_t j1 = ts.toff[t + 1];
 		double acc = 0.0;
-		for (uint32_t j = j0; j < j1; ++j) {
-			const uint32_t c = tl.tcls[j];
+		for (; j + 8 <= j1; j += 8) {
+			double x[8];
+#pragma unroll
+			for (int q = 0; q < 8; ++q) {
+				const uint32_t c = ts.tcls[j + q];
+				const double d = D[c * RC + r];
+				x[q] = d == 0.0 ? 0.0 : l * (double)n[c * RC + r] / d;
+			}
+#pragma unroll
+			for (int q = 0; q < 8; ++q)
+				acc += x[q];
+		}
+		for (; j < j1; ++j) {
+			const uint32_t c = ts.tcls[j];
 			const double d = D[c * RC + r];
-			const double x = d == 0.0 ? 0.0 : l * (double)n[c * RC + r] / d;
-			acc += x;
+			acc += d == 0.0 ? 0.0 : l * (double)n[c * RC + r] / d;
 		}
 		const double nw = ((double)u[t * RC + r] + acc) /
 		                  (double)tl.len[tc.tx_base + t];
@@ -84,17 +134,22 @@ __device__ __forceinline__ void tile_map(
 }
 
 template <typename TC, int RC>
-__device__ __forceinline__ void carve(double *smem, uint32_t mt, uint32_t mc,
+__device__ __forceinline__ void carve(void *smem, const hemk_tiles &tl,
                                       double *&lam0, double *&lam1,
                                       double *&lam2, double *&D, TC *&n,
-                                      uint32_t *&u)
+                                      uint32_t *&u, TileStruct &ts)
 {
-	lam0 = smem;
+	const uint32_t mt = tl.max_tx, mc = tl.max_cls;
+	lam0 = (double *)smem;
 	lam1 = lam0 + mt * RC;
 	lam2 = lam1 + mt * RC;
 	D = lam2 + mt * RC;
 	n = (TC *)(D + mc * RC);
 	u = (uint32_t *)(n + mc * RC);
+	ts.coff = (uint16_t *)(u + mt * RC);
+	ts.toff = ts.coff + (mc + 1);
+	ts.tcls = ts.toff + (mt + 1);
+	ts.cmem = (uint8_t *)(ts.tcls + tl.max_nnz);
 }
 
 // Marks the chunk's live replicates; returns false when none is live.
@@ -139,7 +194,7 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
     hemk_tiles tl, hemk_rctl rc, const TC *__restrict__ cnt,
     const uint32_t *__restrict__ uniq, const double *__restrict__ A,
     double *__restrict__ B, double *__restrict__ C, double *__restrict__ part,
-    int step)
+    double *__restrict__ xr, double *__restrict__ xv, uint32_t T, int step)
 {
 #pragma clang fp contract(off)
 	extern __shared__ double smem[];
@@ -152,7 +207,8 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
 	double *lam0, *lam1, *lam2, *D;
 	TC *n;
 	uint32_t *u;
-	carve<TC, RC>(smem, tl.max_tx, tl.max_cls, lam0, lam1, lam2, D, n, u);
+	TileStruct ts;
+	carve<TC, RC>(smem, tl, lam0, lam1, lam2, D, n, u, ts);
 	const int r = threadIdx.x % RC;
 	const uint32_t e0 = threadIdx.x / RC;
 	constexpr uint32_t ES = TBLOCK / RC;
@@ -161,11 +217,23 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
 			lam0[t * RC + r] =
 			    A[(size_t)tl.tx[tc.tx_base + t] * rc.R + r0 + r];
 	stage_counts<TC, RC>(tl, tc, rc, r0, alive, cnt, uniq, n, u);
+	stage_struct(tl, tc, ts);
 	__syncthreads();
-	tile_map<TC, RC>(tl, tc, lam0, lam1, D, n, u, alive, B, rc.R, r0,
+	tile_map<TC, RC>(tl, tc, ts, lam0, lam1, D, n, u, alive, B, rc.R, r0,
 	                 nullptr, 0, 0);
-	tile_map<TC, RC>(tl, tc, lam1, lam2, D, n, u, alive, C, rc.R, r0,
+	tile_map<TC, RC>(tl, tc, ts, lam1, lam2, D, n, u, alive, C, rc.R, r0,
 	                 nullptr, 0, 0);
+	if (xr && alive[r]) {
+		// r_t and v_t (src/EM.c:334-335) in replicate-major rows for the
+		// sequential norms
+		for (uint32_t t = e0; t < tc.nt; t += ES) {
+			const double x = lam1[t * RC + r] - lam0[t * RC + r];
+			const double y = (lam2[t * RC + r] - lam1[t * RC + r]) - x;
+			const size_t g = (size_t)(r0 + r) * T + tl.tx[tc.tx_base + t];
+			xr[g] = x;
+			xv[g] = y;
+		}
+	}
 	if (part && threadIdx.x < RC && alive[threadIdx.x]) {
 		// per (tile, replicate): local two-pass mean and centred sum of
 		// squares of r and v (combined across tiles by Chan's rule), or
@@ -213,56 +281,6 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
 	}
 }
 
-template <typename TC, int RC>
-__global__ __launch_bounds__(TBLOCK) void k_tile_ex3(
-    hemk_tiles tl, hemk_rctl rc, const TC *__restrict__ cnt,
-    const uint32_t *__restrict__ uniq, double *__restrict__ A,
-    const double *__restrict__ B, const double *__restrict__ C, hemk_step st)
-{
-	// no contraction: the extrapolation rounds as src/EM.c:350
-#pragma clang fp contract(off)
-	extern __shared__ double smem[];
-	__shared__ int alive[RC];
-	__shared__ int stopf[RC];
-	const uint32_t tile = blockIdx.x;
-	const uint32_t r0 = blockIdx.y * RC;
-	if (threadIdx.x < RC)
-		stopf[threadIdx.x] = 0;
-	if (!chunk_alive<RC>(rc, r0, alive))
-		return;
-	const TileCtx tc = load_tile(tl, tile);
-	double *lam0, *lam1, *lam2, *D;
-	TC *n;
-	uint32_t *u;
-	carve<TC, RC>(smem, tl.max_tx, tl.max_cls, lam0, lam1, lam2, D, n, u);
-	const int r = threadIdx.x % RC;
-	const uint32_t e0 = threadIdx.x / RC;
-	constexpr uint32_t ES = TBLOCK / RC;
-	if (alive[r]) {
-		const double alpha = rc.alpha[r0 + r];
-		for (uint32_t t = e0; t < tc.nt; t += ES) {
-			const size_t g = (size_t)tl.tx[tc.tx_base + t] * rc.R + r0 + r;
-			const double a = A[g], b = B[g], c2 = C[g];
-			const double rr = b - a;
-			const double vv = (c2 - b) - rr;
-			double x = a - 2 * alpha * rr + alpha * alpha * vv;
-			if (st.clamp == 0) {
-				if (x < 0)
-					x = 0;
-			} else if (x <= 1e-300) {
-				x = c2 / 10;
-			}
-			lam0[t * RC + r] = x;
-		}
-	}
-	stage_counts<TC, RC>(tl, tc, rc, r0, alive, cnt, uniq, n, u);
-	__syncthreads();
-	tile_map<TC, RC>(tl, tc, lam0, lam1, D, n, u, alive, A, rc.R, r0, stopf,
-	                 rc.stop_min, rc.stop_rel);
-	if (threadIdx.x < RC && alive[threadIdx.x] && stopf[threadIdx.x])
-		rc.notconv[r0 + threadIdx.x] = 1;
-}
-
 __device__ __forceinline__ double step_len(double A, double B, uint32_t n,
                                            hemk_step st)
 {
@@ -279,6 +297,70 @@ __device__ __forceinline__ double step_len(double A, double B, uint32_t n,
 	if (st.alpha_clamp && alpha > -1)
 		alpha = -1;
 	return alpha;
+}
+
+template <typename TC, int RC>
+__global__ __launch_bounds__(TBLOCK) void k_tile_ex3(
+    hemk_tiles tl, hemk_rctl rc, const TC *__restrict__ cnt,
+    const uint32_t *__restrict__ uniq, double *__restrict__ A,
+    const double *__restrict__ B, const double *__restrict__ C, uint32_t T,
+    hemk_step st)
+{
+	// no contraction: the extrapolation rounds as src/EM.c:350
+#pragma clang fp contract(off)
+	extern __shared__ double smem[];
+	__shared__ int alive[RC];
+	__shared__ int stopf[RC];
+	__shared__ double alph[RC];
+	const uint32_t tile = blockIdx.x;
+	const uint32_t r0 = blockIdx.y * RC;
+	if (threadIdx.x < RC)
+		stopf[threadIdx.x] = 0;
+	if (!chunk_alive<RC>(rc, r0, alive))
+		return;
+	if (threadIdx.x < RC && alive[threadIdx.x]) {
+		const uint32_t q = r0 + threadIdx.x;
+		// sequential norms leave the two sums; the parallel ones alpha
+		alph[threadIdx.x] = st.alpha_given
+		                        ? rc.alpha[q]
+		                        : step_len(rc.q_r[q], rc.q_v[q], T, st);
+		if (!st.alpha_given && tile == 0)
+			rc.alpha[q] = alph[threadIdx.x];
+	}
+	__syncthreads();
+	const TileCtx tc = load_tile(tl, tile);
+	double *lam0, *lam1, *lam2, *D;
+	TC *n;
+	uint32_t *u;
+	TileStruct ts;
+	carve<TC, RC>(smem, tl, lam0, lam1, lam2, D, n, u, ts);
+	const int r = threadIdx.x % RC;
+	const uint32_t e0 = threadIdx.x / RC;
+	constexpr uint32_t ES = TBLOCK / RC;
+	if (alive[r]) {
+		const double alpha = alph[r];
+		for (uint32_t t = e0; t < tc.nt; t += ES) {
+			const size_t g = (size_t)tl.tx[tc.tx_base + t] * rc.R + r0 + r;
+			const double a = A[g], b = B[g], c2 = C[g];
+			const double rr = b - a;
+			const double vv = (c2 - b) - rr;
+			double x = a - 2 * alpha * rr + alpha * alpha * vv;
+			if (st.clamp == 0) {
+				if (x < 0)
+					x = 0;
+			} else if (x <= 1e-300) {
+				x = c2 / 10;
+			}
+			lam0[t * RC + r] = x;
+		}
+	}
+	stage_counts<TC, RC>(tl, tc, rc, r0, alive, cnt, uniq, n, u);
+	stage_struct(tl, tc, ts);
+	__syncthreads();
+	tile_map<TC, RC>(tl, tc, ts, lam0, lam1, D, n, u, alive, A, rc.R, r0,
+	                 stopf, rc.stop_min, rc.stop_rel);
+	if (threadIdx.x < RC && alive[threadIdx.x] && stopf[threadIdx.x])
+		rc.notconv[r0 + threadIdx.x] = 1;
 }
 
 // (n, mean, M2) folding of Chan, Golub & LeVeque (1979).
@@ -444,6 +526,111 @@ __global__ __launch_bounds__(TBLOCK) void k_batch_seq(
 	}
 }
 
+// acc + b[0] + b[1] + ... + b[cnt-1], left to right.  The next eight values
+// are read from LDS while the current eight are added, so the chain runs at
+// the latency of a dependent FP64 add rather than of an LDS read.
+__device__ __forceinline__ double chain_add(double acc, const double *b,
+                                            uint32_t cnt)
+{
+#pragma clang fp contract(off)
+	uint32_t j = 0;
+	if (cnt >= 8) {
+		double x0 = b[0], x1 = b[1], x2 = b[2], x3 = b[3];
+		double x4 = b[4], x5 = b[5], x6 = b[6], x7 = b[7];
+		for (j = 8; j + 8 <= cnt; j += 8) {
+			const double y0 = b[j], y1 = b[j + 1], y2 = b[j + 2],
+			             y3 = b[j + 3], y4 = b[j + 4], y5 = b[j + 5],
+			             y6 = b[j + 6], y7 = b[j + 7];
+			acc += x0;
+			acc += x1;
+			acc += x2;
+			acc += x3;
+			acc += x4;
+			acc += x5;
+			acc += x6;
+			acc += x7;
+			x0 = y0;
+			x1 = y1;
+			x2 = y2;
+			x3 = y3;
+			x4 = y4;
+			x5 = y5;
+			x6 = y6;
+			x7 = y7;
+		}
+		acc += x0;
+		acc += x1;
+		acc += x2;
+		acc += x3;
+		acc += x4;
+		acc += x5;
+		acc += x6;
+		acc += x7;
+	}
+	for (; j < cnt; ++j)
+		acc += b[j];
+	return acc;
+}
+
+// Sequential norms of a batch from the replicate-major r/v rows written by
+// the map kernel: block (replicate, which) runs one left-to-right chain on
+// lane 0 of wave 0 while waves 1-3 stage the next chunk of element values.
+//   phase 1: mean = (sum x)/T      phase 2: q = sum (x - mean)^2
+//   phase 3: q = sum x^2 (L2 step)
+#define CHAIN_TB 2048
+__global__ __launch_bounds__(TBLOCK) void k_batch_chain(
+    hemk_rctl rc, uint32_t T, const double *__restrict__ xr,
+    const double *__restrict__ xv, int phase)
+{
+#pragma clang fp contract(off)
+	__shared__ double buf[2][CHAIN_TB];
+	const uint32_t r = blockIdx.x;
+	const int which = blockIdx.y;
+	if (rc.done[r])
+		return;
+	const double *x = (which ? xv : xr) + (size_t)r * T;
+	const double m = phase == 2 ? (which ? rc.mean_v[r] : rc.mean_r[r]) : 0.0;
+	const int w = threadIdx.x / 64;
+	const uint32_t nch = (T + CHAIN_TB - 1) / CHAIN_TB;
+	double acc = 0;
+	for (uint32_t ch = 0; ch <= nch; ++ch) {
+		if (w >= 1 && ch < nch) {
+			const int slot = ch & 1;
+			const uint32_t base = ch * CHAIN_TB;
+			for (int j = threadIdx.x - 64; j < CHAIN_TB; j += TBLOCK - 64) {
+				const uint32_t t = base + j;
+				double a = 0;
+				if (t < T) {
+					const double v = x[t];
+					a = phase == 1 ? v : phase == 2 ? (v - m) * (v - m)
+					                                : v * v;
+				}
+				buf[slot][j] = a;
+			}
+		}
+		if (threadIdx.x == 0 && ch >= 1) {
+			const double *b = buf[(ch - 1) & 1];
+			const uint32_t cnt = min((uint32_t)CHAIN_TB,
+			                         T - (ch - 1) * CHAIN_TB);
+			acc = chain_add(acc, b, cnt);
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		if (phase == 1) {
+			if (which)
+				rc.mean_v[r] = acc / T;
+			else
+				rc.mean_r[r] = acc / T;
+		} else {
+			if (which)
+				rc.q_v[r] = acc;
+			else
+				rc.q_r[r] = acc;
+		}
+	}
+}
+
 __global__ __launch_bounds__(TBLOCK) void k_batch_iter_end(hemk_rctl rc)
 {
 	__shared__ uint32_t cnt[TBLOCK];
@@ -565,30 +752,33 @@ inline uint32_t cdiv32(uint64_t a, uint64_t b)
 size_t tile_lds(const hemk_tiles *t, int rc, int cnt_bytes)
 {
 	return (size_t)rc * ((size_t)t->max_tx * (3 * 8 + 4) +
-	                     (size_t)t->max_cls * (8 + cnt_bytes));
+	                     (size_t)t->max_cls * (8 + cnt_bytes)) +
+	       2 * ((size_t)t->max_cls + 1) + 2 * ((size_t)t->max_tx + 1) +
+	       3 * (size_t)t->max_nnz + 16;
 }
 
 template <typename TC, int RC>
 int launch_map12(const hemk_tiles *t, hemk_rctl rc, const void *cnt,
                  const uint32_t *uniq, const double *A, double *B, double *C,
-                 double *part, int step, hipStream_t s)
+                 double *part, double *xr, double *xv, uint32_t T, int step,
+                 hipStream_t s)
 {
 	const size_t lds = tile_lds(t, RC, sizeof(TC));
 	dim3 grid(t->n_tiles, cdiv32(rc.R, RC));
 	k_tile_map12<TC, RC><<<grid, TBLOCK, lds, s>>>(
-	    *t, rc, (const TC *)cnt, uniq, A, B, C, part, step);
+	    *t, rc, (const TC *)cnt, uniq, A, B, C, part, xr, xv, T, step);
 	return (int)hipGetLastError();
 }
 
 template <typename TC, int RC>
 int launch_ex3(const hemk_tiles *t, hemk_rctl rc, const void *cnt,
                const uint32_t *uniq, double *A, const double *B,
-               const double *C, hemk_step st, hipStream_t s)
+               const double *C, uint32_t T, hemk_step st, hipStream_t s)
 {
 	const size_t lds = tile_lds(t, RC, sizeof(TC));
 	dim3 grid(t->n_tiles, cdiv32(rc.R, RC));
-	k_tile_ex3<TC, RC><<<grid, TBLOCK, lds, s>>>(*t, rc, (const TC *)cnt,
-	                                             uniq, A, B, C, st);
+	k_tile_ex3<TC, RC><<<grid, TBLOCK, lds, s>>>(
+	    *t, rc, (const TC *)cnt, uniq, A, B, C, T, st);
 	return (int)hipGetLastError();
 }
 
@@ -626,35 +816,35 @@ int hemk_tile_chunk(const hemk_tiles *t, uint32_t R, int cnt_bytes)
 int hemk_tile_map12(const hemk_tiles *t, hemk_rctl rc, int chunk,
                     const void *cnt, int cnt_bytes, const uint32_t *uniq,
                     const double *A, double *B, double *C, double *part,
-                    int step, void *s)
+                    double *xr, double *xv, uint32_t T, int step, void *s)
 {
 	hipStream_t st = (hipStream_t)s;
 	if (t->n_tiles == 0)
 		return 0;
 	if (cnt_bytes == 4) {
-#define M12U(RC) launch_map12<uint32_t, RC>(t, rc, cnt, uniq, A, B, C, part, step, st)
+#define M12U(RC) launch_map12<uint32_t, RC>(t, rc, cnt, uniq, A, B, C, part, xr, xv, T, step, st)
 		HEM_RC_SWITCH(chunk, M12U)
 #undef M12U
 	}
-#define M12D(RC) launch_map12<double, RC>(t, rc, cnt, uniq, A, B, C, part, step, st)
+#define M12D(RC) launch_map12<double, RC>(t, rc, cnt, uniq, A, B, C, part, xr, xv, T, step, st)
 	HEM_RC_SWITCH(chunk, M12D)
 #undef M12D
 }
 
 int hemk_tile_ex3(const hemk_tiles *t, hemk_rctl rc, int chunk,
                   const void *cnt, int cnt_bytes, const uint32_t *uniq,
-                  double *A, const double *B, const double *C, hemk_step st,
-                  void *s)
+                  double *A, const double *B, const double *C, uint32_t T,
+                  hemk_step st, void *s)
 {
 	hipStream_t sm = (hipStream_t)s;
 	if (t->n_tiles == 0)
 		return 0;
 	if (cnt_bytes == 4) {
-#define EXU(RC) launch_ex3<uint32_t, RC>(t, rc, cnt, uniq, A, B, C, st, sm)
+#define EXU(RC) launch_ex3<uint32_t, RC>(t, rc, cnt, uniq, A, B, C, T, st, sm)
 		HEM_RC_SWITCH(chunk, EXU)
 #undef EXU
 	}
-#define EXD(RC) launch_ex3<double, RC>(t, rc, cnt, uniq, A, B, C, st, sm)
+#define EXD(RC) launch_ex3<double, RC>(t, rc, cnt, uniq, A, B, C, T, st, sm)
 	HEM_RC_SWITCH(chunk, EXD)
 #undef EXD
 }
@@ -672,6 +862,14 @@ int hemk_batch_seq(hemk_rctl rc, uint32_t T, const double *A,
 {
 	k_batch_seq<<<cdiv32(rc.R, BSEQ_NR), TBLOCK, 0, (hipStream_t)s>>>(
 	    rc, T, A, B, C, phase, st);
+	return (int)hipGetLastError();
+}
+
+int hemk_batch_chain(hemk_rctl rc, uint32_t T, const double *xr,
+                     const double *xv, int phase, void *s)
+{
+	dim3 grid(rc.R, 2);
+	k_batch_chain<<<grid, TBLOCK, 0, (hipStream_t)s>>>(rc, T, xr, xv, phase);
 	return (int)hipGetLastError();
 }
 

@@ -52,6 +52,7 @@ int hem_tiles_upload(hem_problem *p, const hem_tiles_host *h)
 	p->dt.n_cls = h->n_cls;
 	p->dt.max_tx = h->max_tx;
 	p->dt.max_cls = h->max_cls;
+	p->dt.max_nnz = h->max_nnz;
 	p->dt.desc = p->d_tile_mem[0];
 	p->dt.tx = p->d_tile_mem[1];
 	p->dt.len = p->d_tile_mem[2];
@@ -97,6 +98,7 @@ void hem_batch_free(hem_problem *p)
 	hemd_free(w->catT);
 	hemd_free(w->cnt);
 	hemd_free(w->part);
+	hemd_free(w->X);
 	hemd_free(w->m);
 	hemd_free(w->rs_plan);
 	hemd_free(w->seeds);
@@ -143,11 +145,12 @@ static int ws_ensure(hem_problem *p, uint32_t R)
 	WALLOC(w->catT, K * R * 4);
 	WALLOC(w->cnt, NC * R * 4);
 	WALLOC(w->part, NT * R * 4 * 8);
+	WALLOC(w->X, 2 * T * R * 8);
 	WALLOC(w->m, (size_t)R * HEMK_RS_MAXB * 4);
 	WALLOC(w->rs_plan, (K + 2 * ((size_t)HEMK_RS_MAXB + 1)) * 4);
 	WALLOC(w->seeds, (size_t)R * 8);
 	WALLOC(w->u32, ((size_t)4 * R + 1) * 4);
-	WALLOC(w->f64, (size_t)4 * R * 8);
+	WALLOC(w->f64, (size_t)6 * R * 8);
 #undef WALLOC
 	if (hemd_host_alloc((void **)&w->h_active, 64))
 		return hem_fail(HEM_ERR_NOMEM, "pinned allocation");
@@ -175,6 +178,8 @@ static hemk_rctl rctl_for(hem_problem *p, uint32_t R, const hem_opts *o)
 	rc.mean_r = w->f64 + cap;
 	rc.mean_v = w->f64 + 2 * (size_t)cap;
 	rc.lam_sum = w->f64 + 3 * (size_t)cap;
+	rc.q_r = w->f64 + 4 * (size_t)cap;
+	rc.q_v = w->f64 + 5 * (size_t)cap;
 	return rc;
 }
 
@@ -186,30 +191,30 @@ static int batch_iteration(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 	hem_batch_ws *w = &p->bw;
 	void *s = p->stream;
 	const int exact = o->norm == HEM_NORM_EXACT;
-	hemk_step st = {o->step, o->clamp, o->alpha_clamp != 0, 1};
+	/* exact: alpha from the chains' sums inside ex3; tree: alpha given */
+	hemk_step st = {o->step, o->clamp, o->alpha_clamp != 0, !exact};
+	const size_t TR = (size_t)p->n_tx * rc.R;
+	double *xr = exact ? w->X : NULL, *xv = exact ? w->X + TR : NULL;
 	HEM_TRY_DEV(hemk_tile_map12(&p->dt, rc, chunk, w->cnt, 4, w->catT, w->A,
-	                            w->B, w->C, exact ? NULL : w->part, o->step,
-	                            s),
+	                            w->B, w->C, exact ? NULL : w->part, xr, xv,
+	                            p->n_tx, o->step, s),
 	            "tile map12");
 	if (exact) {
 		if (o->step == HEM_STEP_SD) {
-			HEM_TRY_DEV(hemk_batch_seq(rc, p->n_tx, w->A, w->B, w->C, 1,
-			                           st, s),
-			            "batch seq 1");
-			HEM_TRY_DEV(hemk_batch_seq(rc, p->n_tx, w->A, w->B, w->C, 2,
-			                           st, s),
-			            "batch seq 2");
+			HEM_TRY_DEV(hemk_batch_chain(rc, p->n_tx, xr, xv, 1, s),
+			            "batch chain 1");
+			HEM_TRY_DEV(hemk_batch_chain(rc, p->n_tx, xr, xv, 2, s),
+			            "batch chain 2");
 		} else {
-			HEM_TRY_DEV(hemk_batch_seq(rc, p->n_tx, w->A, w->B, w->C, 3,
-			                           st, s),
-			            "batch seq");
+			HEM_TRY_DEV(hemk_batch_chain(rc, p->n_tx, xr, xv, 3, s),
+			            "batch chain");
 		}
 	} else {
 		HEM_TRY_DEV(hemk_batch_alpha(&p->dt, rc, w->part, p->n_tx, st, s),
 		            "batch alpha");
 	}
 	HEM_TRY_DEV(hemk_tile_ex3(&p->dt, rc, chunk, w->cnt, 4, w->catT, w->A,
-	                          w->B, w->C, st, s),
+	                          w->B, w->C, p->n_tx, st, s),
 	            "tile ex3");
 	HEM_TRY_DEV(hemk_batch_iter_end(rc, s), "batch iter end");
 	return HEM_OK;

@@ -22,6 +22,7 @@ typedef struct {
 	uint32_t *catT;                /* [K][R]: unique rows, class rows   */
 	uint32_t *cnt;                 /* [tile position][R] class counts   */
 	double *part;                  /* [tile][R][4] norm partials        */
+	double *X;                     /* [2][R][T] r and v rows            */
 	uint32_t *m;                   /* [R][B] level-1 resample counts    */
 	uint32_t *rs_plan;             /* K weights + chunk starts + prefix */
 	uint64_t *seeds;               /* [R]                               */
@@ -84,9 +85,14 @@ struct hem_problem {
  * global kernels (u8 member indices bound the transcripts per tile). */
 #define HEM_TILE_TX  64
 #define HEM_TILE_CLS 320
+/* Components are packed into tiles up to these sizes (a larger component,
+ * within the limits above, gets a tile of its own): smaller tiles need less
+ * LDS per workgroup, so more workgroups share a CU. */
+#define HEM_TILE_TX_TARGET  32
+#define HEM_TILE_CLS_TARGET 160
 
 typedef struct {
-	uint32_t n_tiles, n_tx, n_cls, max_tx, max_cls;
+	uint32_t n_tiles, n_tx, n_cls, max_tx, max_cls, max_nnz;
 	uint64_t nnz;
 	uint32_t *desc, *tx, *len, *coff, *toff, *cls_orig;
 	uint8_t *cmem;

@@ -353,3 +353,52 @@ int hem_problem_info(const hem_problem *p, uint32_t *n_tx, uint32_t *n_cls,
 		*device_bytes = p->dev_bytes;
 	return HEM_OK;
 }
+
+int hem_problem_set_counts(hem_problem *p, const double *cnt)
+{
+	const uint32_t C = p ? p->n_cls : 0;
+	float *c32;
+	double *nd;
+	int rc = 0, integral = 1;
+	uint32_t i;
+	if (!p || (C && !cnt))
+		return hem_fail(HEM_ERR_INVALID, "null problem or counts");
+	for (i = 0; i < C; ++i) {
+		if (!(cnt[i] >= 0) || !isfinite(cnt[i]))
+			return hem_fail(HEM_ERR_INVALID,
+			                "class %u count is negative or not finite",
+			                i);
+		if (cnt[i] > 4294967295.0 || cnt[i] != floor(cnt[i]))
+			integral = 0;
+	}
+	if (!C)
+		return HEM_OK;
+	c32 = malloc((size_t)C * sizeof(float));
+	nd = malloc((size_t)C * 2 * sizeof(double));
+	if (!c32 || !nd) {
+		free(c32);
+		free(nd);
+		return hem_fail(HEM_ERR_NOMEM, "host allocation failed");
+	}
+	for (i = 0; i < C; ++i) {
+		c32[i] = (float)cnt[i];
+		nd[2 * (size_t)i] = cnt[i];
+		nd[2 * (size_t)i + 1] = 0;
+	}
+	pthread_mutex_lock(&p->mu);
+	rc = hemd_h2d(p->d_cnt64, cnt, (size_t)C * 8, p->stream);
+	if (!rc)
+		rc = hemd_h2d(p->d_cnt32, c32, (size_t)C * 4, p->stream);
+	if (!rc)
+		rc = hemd_h2d(p->d_nd, nd, (size_t)C * 16, p->stream);
+	if (!rc)
+		rc = hemd_stream_sync(p->stream);
+	if (!rc) {
+		memcpy(p->h_cls_count, cnt, (size_t)C * sizeof(double));
+		p->counts_integral = integral;
+	}
+	pthread_mutex_unlock(&p->mu);
+	free(c32);
+	free(nd);
+	return rc ? hem_dev_fail(rc, "count upload") : HEM_OK;
+}

@@ -128,8 +128,8 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 	{
 		uint32_t nt = 0, nc = 0;
 		for (i = 0; i < ncomp; ++i) {
-			if (nt && (nt + ctx[i] > HEM_TILE_TX ||
-			           nc + ccls[i] > HEM_TILE_CLS)) {
+			if (nt && (nt + ctx[i] > HEM_TILE_TX_TARGET ||
+			           nc + ccls[i] > HEM_TILE_CLS_TARGET)) {
 				ntile++;
 				nt = nc = 0;
 			}
@@ -238,12 +238,16 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 	h->n_tiles = ntile;
 	h->n_tx = T;
 	h->nnz = zlive;
-	h->max_tx = h->max_cls = 0;
+	h->max_tx = h->max_cls = h->max_nnz = 0;
 	for (i = 0; i < ntile; ++i) {
+		uint32_t q0 = h->desc[4 * i + 2], q1 = q0 + tile_ncls[i];
+		uint32_t z = h->coff[q1] - h->coff[q0];
 		if (tile_ntx[i] > h->max_tx)
 			h->max_tx = tile_ntx[i];
 		if (tile_ncls[i] > h->max_cls)
 			h->max_cls = tile_ncls[i];
+		if (z > h->max_nnz)
+			h->max_nnz = z;
 	}
 out:
 	if (rc)

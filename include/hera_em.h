@@ -119,6 +119,10 @@ typedef struct hem_problem hem_problem;
 int hem_problem_create(const hem_classes *cls, hem_problem **out);
 int hem_problem_destroy(hem_problem *p);
 
+/* Replaces the class counts (n_cls values, same validation as creation);
+ * the structure stays on the device. */
+int hem_problem_set_counts(hem_problem *p, const double *cls_count);
+
 /* Sizes of the device layout (for the caller's bookkeeping and bytes
  * accounting): n_tx, n_cls, nnz, device bytes held. */
 int hem_problem_info(const hem_problem *p, uint32_t *n_tx, uint32_t *n_cls,

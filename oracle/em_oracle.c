@@ -598,3 +598,76 @@ int oracle_resample(const uint32_t *w, uint32_t K, uint64_t seed,
 	free(cdf);
 	return 0;
 }
+
+/* ---- timed CPU bootstrap: hera's schedule with the multinomial resample --
+ * Batches of n_thread threads joined per batch (src/EM.c:243-255); each
+ * thread resamples its replicate (oracle_resample over unique reads then
+ * classes) and runs estimate_count on the new counts. */
+typedef struct {
+	uint32_t n_tx, n_cls;
+	const uint32_t *off, *ref, *len, *w;
+	const oracle_opts *opt;
+	uint64_t seed;
+	double *est;
+	uint32_t k;
+} o_bjob;
+
+static void *o_boot_thread(void *arg)
+{
+	o_bjob *j = (o_bjob *)arg;
+	uint32_t K = j->n_tx + j->n_cls, c;
+	uint32_t *cnt = malloc((size_t)K * 4);
+	double *cc = malloc(((size_t)j->n_cls + 1) * sizeof(double));
+	oracle_trace tr;
+	memset(&tr, 0, sizeof(tr));
+	oracle_resample(j->w, K, j->seed, cnt);
+	for (c = 0; c < j->n_cls; ++c)
+		cc[c] = cnt[j->n_tx + c];
+	oracle_estimate_count(j->n_tx, j->n_cls, j->off, j->ref, cc, j->len,
+	                      cnt, j->opt, NULL, NULL, j->est, NULL, &tr);
+	j->k = tr.k;
+	free(cnt);
+	free(cc);
+	return NULL;
+}
+
+int oracle_bootstrap_resampled(uint32_t n_tx, uint32_t n_cls,
+                               const uint32_t *off, const uint32_t *ref,
+                               const uint32_t *len, const uint32_t *w,
+                               uint32_t n_rep, const uint64_t *seeds,
+                               const oracle_opts *opt, uint32_t n_thread,
+                               double *est, uint32_t *k_out)
+{
+	pthread_t *thr = calloc(n_thread, sizeof(pthread_t));
+	o_bjob *jobs = calloc(n_rep, sizeof(o_bjob));
+	pthread_attr_t attr;
+	uint32_t i, k;
+	if (!thr || !jobs)
+		return 1;
+	pthread_attr_init(&attr);
+	pthread_attr_setstacksize(&attr, 16u << 20);
+	for (i = 0; i < n_rep;) {
+		for (k = 0; k < n_thread && i < n_rep; ++k, ++i) {
+			o_bjob *j = &jobs[i];
+			j->n_tx = n_tx;
+			j->n_cls = n_cls;
+			j->off = off;
+			j->ref = ref;
+			j->len = len;
+			j->w = w;
+			j->opt = opt;
+			j->seed = seeds[i];
+			j->est = est ? est + (size_t)i * n_tx : NULL;
+			pthread_create(&thr[k], &attr, o_boot_thread, j);
+		}
+		for (; k > 0; --k)
+			pthread_join(thr[k - 1], NULL);
+	}
+	if (k_out)
+		for (i = 0; i < n_rep; ++i)
+			k_out[i] = jobs[i].k;
+	pthread_attr_destroy(&attr);
+	free(thr);
+	free(jobs);
+	return 0;
+}

@@ -64,4 +64,10 @@ uint32_t oracle_resample_plan(const uint32_t *w, uint32_t K, uint32_t N,
                               uint32_t *start, uint32_t *cw);
 int oracle_resample(const uint32_t *w, uint32_t K, uint64_t seed,
                     uint32_t *out);
+int oracle_bootstrap_resampled(uint32_t n_tx, uint32_t n_cls,
+                               const uint32_t *off, const uint32_t *ref,
+                               const uint32_t *len, const uint32_t *w,
+                               uint32_t n_rep, const uint64_t *seeds,
+                               const oracle_opts *opt, uint32_t n_thread,
+                               double *est, uint32_t *k_out);
 #endif

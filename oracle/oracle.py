@@ -68,6 +68,11 @@ def load():
                                              _u32p, _u32p]
         lib.oracle_resample.restype = C.c_int
         lib.oracle_resample.argtypes = [_u32p, C.c_uint32, C.c_uint64, _u32p]
+        lib.oracle_bootstrap_resampled.restype = C.c_int
+        lib.oracle_bootstrap_resampled.argtypes = [
+            C.c_uint32, C.c_uint32, _u32p, _u32p, _u32p, _u32p, C.c_uint32,
+            C.POINTER(C.c_uint64), C.POINTER(OracleOpts), C.c_uint32, _f64p,
+            _u32p]
         lib.oracle_now.restype = C.c_double
         lib.oracle_opts_default.argtypes = [C.POINTER(OracleOpts)]
         _LIB = lib
@@ -213,3 +218,20 @@ def resample(w, seed):
     out = np.empty(len(w), dtype=np.uint32)
     assert load().oracle_resample(_p32(w), len(w), seed, _p32(out)) == 0
     return out
+
+
+def bootstrap_resampled(cls, unique, seeds, n_thread, **kw):
+    """hera's join-per-batch bootstrap schedule; each thread resamples all
+    fragments (as resample()) and solves.  Returns (est[R, T], k[R])."""
+    off, ref, ln = _u32(cls.cls_off), _u32(cls.cls_ref), _u32(cls.tx_len)
+    w = _u32(np.concatenate([np.asarray(unique, dtype=np.uint32),
+                             np.asarray(cls.cls_count).astype(np.uint32)]))
+    sd = np.ascontiguousarray(seeds, dtype=np.uint64)
+    R = len(sd)
+    est = np.empty((R, cls.n_tx))
+    ks = np.empty(R, dtype=np.uint32)
+    assert load().oracle_bootstrap_resampled(
+        cls.n_tx, cls.n_cls, _p32(off), _p32(ref), _p32(ln), _p32(w), R,
+        sd.ctypes.data_as(C.POINTER(C.c_uint64)), opts(**kw), n_thread,
+        _p64(est), _p32(ks)) == 0
+    return est, ks

@@ -82,7 +82,9 @@ typedef struct {
 #define HEMK_NORM_BLOCKS 512
 
 /* ---- component tiles (device view) ---------------------------------------
- * See host/tiles.c.  desc holds {tx_base, n_tx, cls_base, n_cls} per tile. */
+ * See host/tiles.c.  desc holds {tx_base, n_tx, cls_base, n_cls, member
+ * base, members, class-list base, 0} per tile (HEMK_TILE_DESC words). */
+#define HEMK_TILE_DESC 8
 typedef struct {
 	uint32_t n_tiles, n_tx, n_cls;
 	uint32_t max_tx, max_cls, max_nnz;
@@ -90,35 +92,40 @@ typedef struct {
 	const uint32_t *tx;     /* tile position -> transcript id            */
 	const uint32_t *len;    /* tile position -> length                   */
 	const uint32_t *coff;   /* class position -> member offset (+1)      */
-	const uint8_t  *cmem;   /* member -> local transcript index          */
+	const uint32_t *kmem;   /* member -> (local class << 8) | local tx    */
 	const uint32_t *toff;   /* tile position -> class-list offset (+1)   */
-	const uint16_t *tcls;   /* class list entry -> local class index     */
+	const uint16_t *tpos;   /* class-list entry -> member position in the
+	                           tile's member list                        */
+	const uint32_t *txq;    /* transcript id -> tile position            */
 } hemk_tiles;
 
 /* Per-replicate control of a batched solve (device arrays of length R). */
 typedef struct {
 	uint32_t R;
 	uint32_t min_iter, max_iter;
-	uint32_t pad;
+	uint32_t chunk;     /* replicates per tile workgroup (RC)           */
 	double   stop_min, stop_rel;
 	uint32_t *k, *done, *conv, *notconv;
 	double   *alpha, *mean_r, *mean_v, *lam_sum;
 	double   *q_r, *q_v;  /* second-pass sums of the sequential norms */
 	uint32_t *active;   /* [1]: replicates still iterating */
+	uint32_t *clist;    /* [0]: count, then the replicate chunks (of
+	                       chunk replicates) with a replicate iterating */
 } hemk_rctl;
 
 /* Largest replicate chunk (lanes per tile block) that fits the LDS budget
  * for the given tiling and count width (4 or 8 bytes). */
 int hemk_tile_chunk(const hemk_tiles *t, uint32_t R, int cnt_bytes);
 
-/* Tile map kernels.  Layouts: state A/B/C [T][R] FP64 (replicate
- * fastest), unique counts [T][R] u32, class counts [tile position][R]
- * (u32 when cnt_bytes == 4, FP64 when 8).
+/* Tile map kernels.  Layouts, rows in tile position order q (transcripts
+ * of a tile are consecutive; txq maps a transcript id to q): state A/B/C
+ * [q][R] FP64 (replicate fastest), unique counts [q][R] u32, class counts
+ * [class tile position][R] (u32 when cnt_bytes == 4, FP64 when 8).
  *   map12: u1 = M(A) -> B, u2 = M(B) -> C; with part != NULL also the
  *          per-(tile, replicate) sums of r and v (Chan form) for the
  *          parallel norms, part[(tile*R + r)*4 ...]; with xr != NULL the
- *          r and v values in replicate-major rows xr/xv[r*T + t] for the
- *          sequential norms.
+ *          r and v values in replicate-major rows xr/xv[r*T + t], t the
+ *          transcript id, for the sequential norms.
  *   ex3:   extrapolate from A, B, C (alpha[r] when st.alpha_given, else
  *          from q_r/q_v), map once -> A, stop test -> notconv[r]. */
 int hemk_tile_map12(const hemk_tiles *t, hemk_rctl rc, int chunk,
@@ -129,28 +136,28 @@ int hemk_tile_ex3(const hemk_tiles *t, hemk_rctl rc, int chunk,
                   const void *cnt, int cnt_bytes, const uint32_t *uniq,
                   double *A, const double *B, const double *C, uint32_t T,
                   hemk_step st, void *s);
-/* Sequential norms of a batch over the rows xr/xv: one chain per
- * (replicate, r|v) block.  Phase 1 -> mean_r/mean_v; phase 2 (centred) and
- * phase 3 (squares) -> q_r/q_v. */
+/* Sequential sums of a batch over replicate-major rows, one left-to-right
+ * chain per (replicate, row) block, in the reference's order over t:
+ *   phase 0: lam_sum[r] = sum of xr row r (every replicate)
+ *   phase 1: mean_r/mean_v = sums / T
+ *   phase 2: q_r/q_v = centred sums of squares    phase 3: sums of squares
+ * Phases 1-3 skip finished replicates. */
 int hemk_batch_chain(hemk_rctl rc, uint32_t T, const double *xr,
                      const double *xv, int phase, void *s);
 /* Parallel norms of a batch: fold the tile partials of each replicate in
  * tile order (Chan et al. for the sd form) -> alpha[r]. */
 int hemk_batch_alpha(const hemk_tiles *t, hemk_rctl rc, const double *part,
                      uint32_t T, hemk_step st, void *s);
-/* Sequential norms of a batch: one lane per replicate runs the reference's
- * left-to-right sums over t (phases as hemk_seq_norm; phase 0 fills
- * lam_sum from A). */
-int hemk_batch_seq(hemk_rctl rc, uint32_t T, const double *A,
-                   const double *B, const double *C, int phase, hemk_step st,
-                   void *s);
 int hemk_batch_iter_end(hemk_rctl rc, void *s);
 int hemk_batch_reset(hemk_rctl rc, uint32_t T, double *A, void *s);
-/* est[r][t] = lam[t][r] len[t]; tpm[r][t] = lam[t][r] 1e6 / lam_sum[r]
- * (zero when the sum is zero); lam_sum from hemk_batch_seq phase 0. */
-int hemk_batch_finalize(hemk_rctl rc, uint32_t T, const double *A,
-                        const uint32_t *len, double *est, double *tpm,
-                        void *s);
+/* X[r*T + t] = A[txq[t]*R + r]: the state as one row per replicate in
+ * transcript order. */
+int hemk_batch_rows(hemk_rctl rc, uint32_t T, const uint32_t *txq,
+                    const double *A, double *X, void *s);
+/* est[r][t] = X[r][t] len[t]; tpm[r][t] = X[r][t] 1e6 / lam_sum[r] (zero
+ * when the sum is zero); lam_sum from hemk_batch_chain phase 0. */
+int hemk_batch_out(hemk_rctl rc, uint32_t T, const double *X,
+                   const uint32_t *len, double *est, double *tpm, void *s);
 /* out = transpose of in (rows x cols, row-major) for u32 count blocks. */
 int hemk_transpose_u32(const uint32_t *in, uint32_t rows, uint32_t cols,
                        uint32_t *out, void *s);

@@ -1,13 +1,15 @@
 // tile_kernels.hip -- batched SQUAREM EM over component tiles (gfx950).
 //
 // A workgroup owns one tile (whole connected components) and a chunk of RC
-// replicates.  It stages the tile's state rows ([t][r], replicate fastest,
-// one coalesced row per transcript), unique counts and class counts in LDS,
-// then runs hera's class pass and transcript pass (src/EM.c:276-291) on LDS
-// only, in the reference's order: members in class order for D_c, classes
-// in ascending original id for each transcript's sum of (lam_t n_c) / D_c.
-// The structure is read once per tile and shared by the RC replicates of the
-// workgroup; replicates are independent lanes (lane r of an entity group).
+// replicates.  The batch's state, unique counts and class counts are kept
+// in tile position order, replicate fastest, so a workgroup's rows are one
+// contiguous block each and every global read it needs goes out in a single
+// round once the tile descriptor is known.  It stages them in LDS with the
+// tile's structure and runs hera's class pass and transcript pass
+// (src/EM.c:276-291) on LDS only, in the reference's order: members in
+// class order for D_c, classes in ascending original id (then member
+// order) for each transcript's sum of (lam_t n_c) / D_c.  Replicates are
+// independent lanes (lane r of an entity group).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -15,59 +17,74 @@
 #include "hem_dev.h"
 
 #define TBLOCK 256
-#define TLDS_BUDGET (40 * 1024)
+// dynamic LDS per workgroup: four workgroups share a CU's 160 KB with
+// room for the kernels' static arrays
+#define TLDS_BUDGET (40 * 1024 - 512)
 
 namespace {
 
 struct TileCtx {
-	uint32_t tx_base, nt, cls_base, nc;
+	uint32_t tx_base, nt, cls_base, nc, mb, nz, jb;
 };
 
 __device__ __forceinline__ TileCtx load_tile(const hemk_tiles &t, uint32_t i)
 {
+	const uint32_t *d = t.desc + (size_t)HEMK_TILE_DESC * i;
 	TileCtx c;
-	c.tx_base = t.desc[4 * i + 0];
-	c.nt = t.desc[4 * i + 1];
-	c.cls_base = t.desc[4 * i + 2];
-	c.nc = t.desc[4 * i + 3];
+	c.tx_base = d[0];
+	c.nt = d[1];
+	c.cls_base = d[2];
+	c.nc = d[3];
+	c.mb = d[4];
+	c.nz = d[5];
+	c.jb = d[6];
 	return c;
 }
 
-// The tile's structure staged in LDS: member lists and class lists with
-// offsets relative to the tile (u16; a tile holds at most max_nnz members).
+// The tile's structure staged in LDS, offsets relative to the tile: member
+// lists (each member as (local class << 8) | local transcript), per
+// transcript the member positions of its occurrences in class order, and
+// the transcripts' ids and lengths.
 struct TileStruct {
 	uint16_t *coff;  // nc + 1
 	uint16_t *toff;  // nt + 1
-	uint8_t *cmem;   // members
-	uint16_t *tcls;  // class-list entries
+	uint32_t *kmem;  // members
+	uint16_t *tpos;  // class-list entries
+	uint32_t *tx;    // nt
+	uint32_t *len;   // nt
 };
 
 __device__ __forceinline__ void stage_struct(const hemk_tiles &tl,
                                              const TileCtx &tc,
                                              const TileStruct &ts)
 {
-	const uint32_t mb = tl.coff[tc.cls_base];
-	const uint32_t me = tl.coff[tc.cls_base + tc.nc];
-	const uint32_t jb = tl.toff[tc.tx_base];
-	const uint32_t je = tl.toff[tc.tx_base + tc.nt];
 	for (uint32_t i = threadIdx.x; i <= tc.nc; i += TBLOCK)
-		ts.coff[i] = (uint16_t)(tl.coff[tc.cls_base + i] - mb);
+		ts.coff[i] = (uint16_t)(tl.coff[tc.cls_base + i] - tc.mb);
 	for (uint32_t i = threadIdx.x; i <= tc.nt; i += TBLOCK)
-		ts.toff[i] = (uint16_t)(tl.toff[tc.tx_base + i] - jb);
-	for (uint32_t i = threadIdx.x; i < me - mb; i += TBLOCK)
-		ts.cmem[i] = tl.cmem[mb + i];
-	for (uint32_t i = threadIdx.x; i < je - jb; i += TBLOCK)
-		ts.tcls[i] = tl.tcls[jb + i];
+		ts.toff[i] = (uint16_t)(tl.toff[tc.tx_base + i] - tc.jb);
+	for (uint32_t i = threadIdx.x; i < tc.nz; i += TBLOCK) {
+		ts.kmem[i] = tl.kmem[tc.mb + i];
+		ts.tpos[i] = tl.tpos[tc.jb + i];
+	}
+	for (uint32_t i = threadIdx.x; i < tc.nt; i += TBLOCK) {
+		ts.tx[i] = tl.tx[tc.tx_base + i];
+		ts.len[i] = tl.len[tc.tx_base + i];
+	}
 }
 
 // One EM map of the tile for the block's replicate chunk, lin -> lout in
-// LDS (and to the global [T][R] vector gout when given).  With stopf the
-// stop test of src/EM.c:292-297 marks stopf[r].  Terms are formed four at a
-// time (independent loads and divisions in flight) and added in order.
+// LDS (and to the global [T][R] vector gout when given), in three passes
+// separated by barriers:
+//   class pass       D_c = sum of lam over the members, in member order
+//   term pass        X_k = (lam_t n_c) / D_c for every member k = (c, t),
+//                    one (member, replicate) pair per lane
+//   transcript pass  lam'_t = (u_t + sum of its X_k in class order) / len_t
+// which is src/EM.c:275-291 with every sum in the reference's order.  With
+// stopf the stop test of src/EM.c:292-297 marks stopf[r].
 template <typename TC, int RC>
 __device__ __forceinline__ void tile_map(
     const hemk_tiles &tl, const TileCtx &tc, const TileStruct &ts,
-    const double *lin, double *lout, double *D, const TC *n,
+    const double *lin, double *lout, double *D, double *X, const TC *n,
     const uint32_t *u, const int *alive, double *gout, uint32_t R,
     uint32_t r0, int *stopf, double smin, double srel)
 {
@@ -75,60 +92,83 @@ __device__ __forceinline__ void tile_map(
 	const uint32_t e0 = threadIdx.x / RC;
 	constexpr uint32_t ES = TBLOCK / RC;
 	const bool live = alive[r] != 0;
-	// class pass: D_c, left to right over the members
-	for (uint32_t c = e0; c < tc.nc; c += ES) {
-		if (!live)
-			continue;
-		uint32_t k = ts.coff[c];
-		const uint32_t k1 = ts.coff[c + 1];
-		double acc = 0.0;
-		for (; k + 4 <= k1; k += 4) {
-			const double x0 = lin[ts.cmem[k] * RC + r];
-			const double x1 = lin[ts.cmem[k + 1] * RC + r];
-			const double x2 = lin[ts.cmem[k + 2] * RC + r];
-			const double x3 = lin[ts.cmem[k + 3] * RC + r];
-			acc += x0;
-			acc += x1;
-			acc += x2;
-			acc += x3;
+	(void)tl;
+	if (live) {
+		for (uint32_t c = e0; c < tc.nc; c += ES) {
+			uint32_t k = ts.coff[c];
+			const uint32_t k1 = ts.coff[c + 1];
+			double acc = 0.0;
+			for (; k + 4 <= k1; k += 4) {
+				const double x0 = lin[(ts.kmem[k] & 0xFFu) * RC + r];
+				const double x1 = lin[(ts.kmem[k + 1] & 0xFFu) * RC + r];
+				const double x2 = lin[(ts.kmem[k + 2] & 0xFFu) * RC + r];
+				const double x3 = lin[(ts.kmem[k + 3] & 0xFFu) * RC + r];
+				acc += x0;
+				acc += x1;
+				acc += x2;
+				acc += x3;
+			}
+			for (; k < k1; ++k)
+				acc += lin[(ts.kmem[k] & 0xFFu) * RC + r];
+			D[c * RC + r] = acc;
 		}
-		for (; k < k1; ++k)
-			acc += lin[ts.cmem[k] * RC + r];
-		D[c * RC + r] = acc;
 	}
 	__syncthreads();
-	// transcript pass + M-step
-	for (uint32_t t = e0; t < tc.nt; t += ES) {
-		if (!live)
-			continue;
-		const double l = lin[t * RC + r];
-		uint32_t j = ts.toff[t];
-		const uint32_t j1 = ts.toff[t + 1];
-		double acc = 0.0;
-		for (; j + 8 <= j1; j += 8) {
-			double x[8];
+	if (live) {
+		// four members per step, each stage's LDS reads issued together;
+		// the division runs on every lane (denominator 1 where D_c = 0,
+		// whose term is then 0) so no lane branches around it
+		for (uint32_t k0 = e0; k0 < tc.nz; k0 += 4 * ES) {
+			uint32_t m[4];
 #pragma unroll
-			for (int q = 0; q < 8; ++q) {
-				const uint32_t c = ts.tcls[j + q];
-				const double d = D[c * RC + r];
-				x[q] = d == 0.0 ? 0.0 : l * (double)n[c * RC + r] / d;
+			for (int q = 0; q < 4; ++q) {
+				const uint32_t k = k0 + q * ES;
+				m[q] = ts.kmem[k < tc.nz ? k : k0];
+			}
+			double dd[4], ll[4], nn[4];
+#pragma unroll
+			for (int q = 0; q < 4; ++q) {
+				const uint32_t c = m[q] >> 8, t = m[q] & 0xFFu;
+				dd[q] = D[c * RC + r];
+				ll[q] = lin[t * RC + r];
+				nn[q] = (double)n[c * RC + r];
 			}
 #pragma unroll
-			for (int q = 0; q < 8; ++q)
-				acc += x[q];
+			for (int q = 0; q < 4; ++q) {
+				const uint32_t k = k0 + q * ES;
+				const double den = dd[q] == 0.0 ? 1.0 : dd[q];
+				const double x = ll[q] * nn[q] / den;
+				if (k < tc.nz)
+					X[k * RC + r] = dd[q] == 0.0 ? 0.0 : x;
+			}
 		}
-		for (; j < j1; ++j) {
-			const uint32_t c = ts.tcls[j];
-			const double d = D[c * RC + r];
-			acc += d == 0.0 ? 0.0 : l * (double)n[c * RC + r] / d;
+	}
+	__syncthreads();
+	if (live) {
+		for (uint32_t t = e0; t < tc.nt; t += ES) {
+			const double l = lin[t * RC + r];
+			uint32_t j = ts.toff[t];
+			const uint32_t j1 = ts.toff[t + 1];
+			double acc = 0.0;
+			for (; j + 8 <= j1; j += 8) {
+				double x[8];
+#pragma unroll
+				for (int q = 0; q < 8; ++q)
+					x[q] = X[ts.tpos[j + q] * RC + r];
+#pragma unroll
+				for (int q = 0; q < 8; ++q)
+					acc += x[q];
+			}
+			for (; j < j1; ++j)
+				acc += X[ts.tpos[j] * RC + r];
+			const double nw = ((double)u[t * RC + r] + acc) /
+			                  (double)ts.len[t];
+			lout[t * RC + r] = nw;
+			if (gout)
+				gout[(size_t)(tc.tx_base + t) * R + r0 + r] = nw;
+			if (stopf && nw > smin && fabs(nw - l) / nw > srel)
+				stopf[r] = 1;
 		}
-		const double nw = ((double)u[t * RC + r] + acc) /
-		                  (double)tl.len[tc.tx_base + t];
-		lout[t * RC + r] = nw;
-		if (gout)
-			gout[(size_t)tl.tx[tc.tx_base + t] * R + r0 + r] = nw;
-		if (stopf && nw > smin && fabs(nw - l) / nw > srel)
-			stopf[r] = 1;
 	}
 	__syncthreads();
 }
@@ -136,57 +176,115 @@ __device__ __forceinline__ void tile_map(
 template <typename TC, int RC>
 __device__ __forceinline__ void carve(void *smem, const hemk_tiles &tl,
                                       double *&lam0, double *&lam1,
-                                      double *&lam2, double *&D, TC *&n,
-                                      uint32_t *&u, TileStruct &ts)
+                                      double *&lam2, double *&D, double *&X,
+                                      TC *&n, uint32_t *&u, TileStruct &ts)
 {
-	const uint32_t mt = tl.max_tx, mc = tl.max_cls;
+	const uint32_t mt = tl.max_tx, mc = tl.max_cls, mz = tl.max_nnz;
 	lam0 = (double *)smem;
 	lam1 = lam0 + mt * RC;
 	lam2 = lam1 + mt * RC;
 	D = lam2 + mt * RC;
-	n = (TC *)(D + mc * RC);
+	X = D + mc * RC;
+	n = (TC *)(X + mz * RC);
 	u = (uint32_t *)(n + mc * RC);
-	ts.coff = (uint16_t *)(u + mt * RC);
+	ts.tx = u + mt * RC;
+	ts.len = ts.tx + mt;
+	ts.kmem = ts.len + mt;
+	ts.coff = (uint16_t *)(ts.kmem + mz);
 	ts.toff = ts.coff + (mc + 1);
-	ts.tcls = ts.toff + (mt + 1);
-	ts.cmem = (uint8_t *)(ts.tcls + tl.max_nnz);
+	ts.tpos = ts.toff + (mt + 1);
 }
 
-// Marks the chunk's live replicates; returns false when none is live.
-template <int RC>
-__device__ __forceinline__ bool chunk_alive(const hemk_rctl &rc, uint32_t r0,
-                                            int *alive)
+// Rows of one replicate chunk held in registers from the prefetch, issued
+// while the previous chunk computes, to the store into LDS.  Element e of a
+// row block is (entity e / RC, replicate e % RC); thread i holds elements
+// i + TBLOCK k for k below the slot count, so its replicate is i % RC.
+// Elements beyond the slots (tiles larger than the slots cover) are read
+// at the store.
+#define PF_TX 2   // slots for transcript rows (n_tx * RC elements)
+#define PF_CLS 4  // slots for class count rows (n_cls * RC elements)
+
+template <typename TC, int NS>
+struct ChunkRows {
+	double s[NS][PF_TX];  // state rows: A, or A B C
+	uint32_t u[PF_TX];    // unique counts
+	TC n[PF_CLS];         // class counts
+	uint32_t done;        // the thread's replicate has finished (or >= R)
+	double qa, qb;        // ex3: alpha, or the two sums it comes from
+};
+
+template <typename TC, int RC, int NS>
+__device__ __forceinline__ void load_rows(ChunkRows<TC, NS> &p,
+                                          const TileCtx &tc,
+                                          const hemk_rctl &rc, uint32_t r0,
+                                          const TC *__restrict__ cnt,
+                                          const uint32_t *__restrict__ uniq,
+                                          const double *s0, const double *s1,
+                                          const double *s2)
 {
-	if (threadIdx.x < RC) {
-		const uint32_t r = r0 + threadIdx.x;
-		alive[threadIdx.x] = (r < rc.R && !rc.done[r]) ? 1 : 0;
+	const uint32_t q = r0 + threadIdx.x % RC;
+	const bool rin = q < rc.R;
+	p.done = rin ? rc.done[q] : 1u;
+#pragma unroll
+	for (int k = 0; k < PF_TX; ++k) {
+		const uint32_t t = (threadIdx.x + TBLOCK * k) / RC;
+		double v0 = 0, v1 = 0, v2 = 0;
+		uint32_t w = 0;
+		if (rin && t < tc.nt) {
+			const size_t g = (size_t)(tc.tx_base + t) * rc.R + q;
+			v0 = s0[g];
+			if (NS == 3) {
+				v1 = s1[g];
+				v2 = s2[g];
+			}
+			w = uniq[g];
+		}
+		p.s[0][k] = v0;
+		if (NS == 3) {
+			p.s[1 % NS][k] = v1;
+			p.s[2 % NS][k] = v2;
+		}
+		p.u[k] = w;
 	}
-	__syncthreads();
-	int any = 0;
-	for (int i = 0; i < RC; ++i)
-		any |= alive[i];
-	return any != 0;
+#pragma unroll
+	for (int k = 0; k < PF_CLS; ++k) {
+		const uint32_t c = (threadIdx.x + TBLOCK * k) / RC;
+		TC v = 0;
+		if (rin && c < tc.nc)
+			v = cnt[(size_t)(tc.cls_base + c) * rc.R + q];
+		p.n[k] = v;
+	}
 }
 
-template <typename TC, int RC>
-__device__ __forceinline__ void stage_counts(const hemk_tiles &tl,
+// Class and unique count rows of the chunk into LDS (n, u).
+template <typename TC, int RC, int NS>
+__device__ __forceinline__ void store_counts(const ChunkRows<TC, NS> &p,
                                              const TileCtx &tc,
                                              const hemk_rctl &rc, uint32_t r0,
-                                             const int *alive,
                                              const TC *__restrict__ cnt,
                                              const uint32_t *__restrict__ uniq,
                                              TC *n, uint32_t *u)
 {
-	const int r = threadIdx.x % RC;
-	const uint32_t e0 = threadIdx.x / RC;
-	constexpr uint32_t ES = TBLOCK / RC;
-	if (alive[r]) {
-		for (uint32_t c = e0; c < tc.nc; c += ES)
-			n[c * RC + r] = cnt[(size_t)(tc.cls_base + c) * rc.R + r0 + r];
-		for (uint32_t t = e0; t < tc.nt; t += ES)
-			u[t * RC + r] =
-			    uniq[(size_t)tl.tx[tc.tx_base + t] * rc.R + r0 + r];
+	const uint32_t q = r0 + threadIdx.x % RC;
+	const bool rin = q < rc.R;
+#pragma unroll
+	for (int k = 0; k < PF_TX; ++k) {
+		const uint32_t e = threadIdx.x + TBLOCK * k;
+		if (e / RC < tc.nt)
+			u[e] = p.u[k];
 	}
+	for (uint32_t e = threadIdx.x + TBLOCK * PF_TX; e / RC < tc.nt;
+	     e += TBLOCK)
+		u[e] = rin ? uniq[(size_t)(tc.tx_base + e / RC) * rc.R + q] : 0u;
+#pragma unroll
+	for (int k = 0; k < PF_CLS; ++k) {
+		const uint32_t e = threadIdx.x + TBLOCK * k;
+		if (e / RC < tc.nc)
+			n[e] = p.n[k];
+	}
+	for (uint32_t e = threadIdx.x + TBLOCK * PF_CLS; e / RC < tc.nc;
+	     e += TBLOCK)
+		n[e] = rin ? cnt[(size_t)(tc.cls_base + e / RC) * rc.R + q] : (TC)0;
 }
 
 template <typename TC, int RC>
@@ -200,84 +298,106 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
 	extern __shared__ double smem[];
 	__shared__ int alive[RC];
 	const uint32_t tile = blockIdx.x;
-	const uint32_t r0 = blockIdx.y * RC;
-	if (!chunk_alive<RC>(rc, r0, alive))
+	const uint32_t nact = rc.clist[0];
+	if (nact == 0)
 		return;
 	const TileCtx tc = load_tile(tl, tile);
-	double *lam0, *lam1, *lam2, *D;
+	double *lam0, *lam1, *lam2, *D, *X;
 	TC *n;
 	uint32_t *u;
 	TileStruct ts;
-	carve<TC, RC>(smem, tl, lam0, lam1, lam2, D, n, u, ts);
+	carve<TC, RC>(smem, tl, lam0, lam1, lam2, D, X, n, u, ts);
 	const int r = threadIdx.x % RC;
 	const uint32_t e0 = threadIdx.x / RC;
 	constexpr uint32_t ES = TBLOCK / RC;
-	if (alive[r])
-		for (uint32_t t = e0; t < tc.nt; t += ES)
-			lam0[t * RC + r] =
-			    A[(size_t)tl.tx[tc.tx_base + t] * rc.R + r0 + r];
-	stage_counts<TC, RC>(tl, tc, rc, r0, alive, cnt, uniq, n, u);
+	ChunkRows<TC, 1> p;
+	load_rows<TC, RC, 1>(p, tc, rc, rc.clist[1] * RC, cnt, uniq, A, nullptr,
+	                     nullptr);
 	stage_struct(tl, tc, ts);
-	__syncthreads();
-	tile_map<TC, RC>(tl, tc, ts, lam0, lam1, D, n, u, alive, B, rc.R, r0,
-	                 nullptr, 0, 0);
-	tile_map<TC, RC>(tl, tc, ts, lam1, lam2, D, n, u, alive, C, rc.R, r0,
-	                 nullptr, 0, 0);
-	if (xr && alive[r]) {
-		// r_t and v_t (src/EM.c:334-335) in replicate-major rows for the
-		// sequential norms
-		for (uint32_t t = e0; t < tc.nt; t += ES) {
-			const double x = lam1[t * RC + r] - lam0[t * RC + r];
-			const double y = (lam2[t * RC + r] - lam1[t * RC + r]) - x;
-			const size_t g = (size_t)(r0 + r) * T + tl.tx[tc.tx_base + t];
-			xr[g] = x;
-			xv[g] = y;
+	for (uint32_t i = 0; i < nact; ++i) {
+		const uint32_t r0 = rc.clist[1 + i] * RC;
+		const bool rin = r0 + r < rc.R;
+#pragma unroll
+		for (int k = 0; k < PF_TX; ++k) {
+			const uint32_t e = threadIdx.x + TBLOCK * k;
+			if (e / RC < tc.nt)
+				lam0[e] = p.s[0][k];
 		}
-	}
-	if (part && threadIdx.x < RC && alive[threadIdx.x]) {
-		// per (tile, replicate): local two-pass mean and centred sum of
-		// squares of r and v (combined across tiles by Chan's rule), or
-		// plain sums of squares for the L2 step
-		const int q = threadIdx.x;
-		double a0, a1, b0, b1;
-		if (step == 0) {
-			double sr = 0, sv = 0;
-			for (uint32_t t = 0; t < tc.nt; ++t) {
-				const double x = lam1[t * RC + q] - lam0[t * RC + q];
-				const double y = (lam2[t * RC + q] - lam1[t * RC + q]) - x;
-				sr += x;
-				sv += y;
+		for (uint32_t e = threadIdx.x + TBLOCK * PF_TX; e / RC < tc.nt;
+		     e += TBLOCK)
+			lam0[e] = rin ? A[(size_t)(tc.tx_base + e / RC) * rc.R + r0 + r]
+			              : 0.0;
+		store_counts<TC, RC, 1>(p, tc, rc, r0, cnt, uniq, n, u);
+		if (threadIdx.x < RC)
+			alive[threadIdx.x] = p.done ? 0 : 1;
+		__syncthreads();
+		if (i + 1 < nact)
+			load_rows<TC, RC, 1>(p, tc, rc, rc.clist[2 + i] * RC, cnt,
+			                     uniq, A, nullptr, nullptr);
+		tile_map<TC, RC>(tl, tc, ts, lam0, lam1, D, X, n, u, alive, B,
+		                 rc.R, r0, nullptr, 0, 0);
+		tile_map<TC, RC>(tl, tc, ts, lam1, lam2, D, X, n, u, alive, C,
+		                 rc.R, r0, nullptr, 0, 0);
+		if (xr && alive[r]) {
+			// r_t and v_t (src/EM.c:334-335) in replicate-major rows
+			// for the sequential norms
+			for (uint32_t t = e0; t < tc.nt; t += ES) {
+				const double x = lam1[t * RC + r] - lam0[t * RC + r];
+				const double y = (lam2[t * RC + r] - lam1[t * RC + r]) - x;
+				const size_t g = (size_t)(r0 + r) * T + ts.tx[t];
+				xr[g] = x;
+				xv[g] = y;
 			}
-			const double mr = sr / tc.nt, mv = sv / tc.nt;
-			double qr = 0, qv = 0;
-			for (uint32_t t = 0; t < tc.nt; ++t) {
-				const double x = lam1[t * RC + q] - lam0[t * RC + q];
-				const double y = (lam2[t * RC + q] - lam1[t * RC + q]) - x;
-				qr += (x - mr) * (x - mr);
-				qv += (y - mv) * (y - mv);
-			}
-			a0 = mr;
-			a1 = qr;
-			b0 = mv;
-			b1 = qv;
-		} else {
-			double qr = 0, qv = 0;
-			for (uint32_t t = 0; t < tc.nt; ++t) {
-				const double x = lam1[t * RC + q] - lam0[t * RC + q];
-				const double y = (lam2[t * RC + q] - lam1[t * RC + q]) - x;
-				qr += x * x;
-				qv += y * y;
-			}
-			a0 = 0;
-			a1 = qr;
-			b0 = 0;
-			b1 = qv;
 		}
-		double *pp = part + ((size_t)tile * rc.R + r0 + q) * 4;
-		pp[0] = a0;
-		pp[1] = a1;
-		pp[2] = b0;
-		pp[3] = b1;
+		if (part && threadIdx.x < RC && alive[threadIdx.x]) {
+			// per (tile, replicate): local two-pass mean and centred
+			// sum of squares of r and v (combined across tiles by
+			// Chan's rule), or plain sums of squares for the L2 step
+			const int q = threadIdx.x;
+			double a0, a1, b0, b1;
+			if (step == 0) {
+				double sr = 0, sv = 0;
+				for (uint32_t t = 0; t < tc.nt; ++t) {
+					const double x = lam1[t * RC + q] - lam0[t * RC + q];
+					const double y =
+					    (lam2[t * RC + q] - lam1[t * RC + q]) - x;
+					sr += x;
+					sv += y;
+				}
+				const double mr = sr / tc.nt, mv = sv / tc.nt;
+				double qr = 0, qv = 0;
+				for (uint32_t t = 0; t < tc.nt; ++t) {
+					const double x = lam1[t * RC + q] - lam0[t * RC + q];
+					const double y =
+					    (lam2[t * RC + q] - lam1[t * RC + q]) - x;
+					qr += (x - mr) * (x - mr);
+					qv += (y - mv) * (y - mv);
+				}
+				a0 = mr;
+				a1 = qr;
+				b0 = mv;
+				b1 = qv;
+			} else {
+				double qr = 0, qv = 0;
+				for (uint32_t t = 0; t < tc.nt; ++t) {
+					const double x = lam1[t * RC + q] - lam0[t * RC + q];
+					const double y =
+					    (lam2[t * RC + q] - lam1[t * RC + q]) - x;
+					qr += x * x;
+					qv += y * y;
+				}
+				a0 = 0;
+				a1 = qr;
+				b0 = 0;
+				b1 = qv;
+			}
+			double *pp = part + ((size_t)tile * rc.R + r0 + q) * 4;
+			pp[0] = a0;
+			pp[1] = a1;
+			pp[2] = b0;
+			pp[3] = b1;
+		}
+		__syncthreads();
 	}
 }
 
@@ -299,6 +419,42 @@ __device__ __forceinline__ double step_len(double A, double B, uint32_t n,
 	return alpha;
 }
 
+// u' = u - 2 alpha r + alpha^2 v, clamped (src/EM.c:338-360); rounds as the
+// reference (no contraction).
+__device__ __forceinline__ double extrapolate(double a, double b, double c2,
+                                              double alpha, hemk_step st)
+{
+#pragma clang fp contract(off)
+	const double rr = b - a;
+	const double vv = (c2 - b) - rr;
+	double x = a - 2 * alpha * rr + alpha * alpha * vv;
+	if (st.clamp == 0) {
+		if (x < 0)
+			x = 0;
+	} else if (x <= 1e-300) {
+		x = c2 / 10;
+	}
+	return x;
+}
+
+template <typename TC, int RC>
+__device__ __forceinline__ void load_alpha(ChunkRows<TC, 3> &p,
+                                           const hemk_rctl &rc, uint32_t r0,
+                                           hemk_step st)
+{
+	const uint32_t q = r0 + threadIdx.x % RC;
+	p.qa = 0;
+	p.qb = 0;
+	if (q < rc.R) {
+		if (st.alpha_given) {
+			p.qa = rc.alpha[q];
+		} else {
+			p.qa = rc.q_r[q];
+			p.qb = rc.q_v[q];
+		}
+	}
+}
+
 template <typename TC, int RC>
 __global__ __launch_bounds__(TBLOCK) void k_tile_ex3(
     hemk_tiles tl, hemk_rctl rc, const TC *__restrict__ cnt,
@@ -306,61 +462,68 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_ex3(
     const double *__restrict__ B, const double *__restrict__ C, uint32_t T,
     hemk_step st)
 {
-	// no contraction: the extrapolation rounds as src/EM.c:350
 #pragma clang fp contract(off)
 	extern __shared__ double smem[];
 	__shared__ int alive[RC];
 	__shared__ int stopf[RC];
-	__shared__ double alph[RC];
 	const uint32_t tile = blockIdx.x;
-	const uint32_t r0 = blockIdx.y * RC;
-	if (threadIdx.x < RC)
-		stopf[threadIdx.x] = 0;
-	if (!chunk_alive<RC>(rc, r0, alive))
+	const uint32_t nact = rc.clist[0];
+	if (nact == 0)
 		return;
-	if (threadIdx.x < RC && alive[threadIdx.x]) {
-		const uint32_t q = r0 + threadIdx.x;
-		// sequential norms leave the two sums; the parallel ones alpha
-		alph[threadIdx.x] = st.alpha_given
-		                        ? rc.alpha[q]
-		                        : step_len(rc.q_r[q], rc.q_v[q], T, st);
-		if (!st.alpha_given && tile == 0)
-			rc.alpha[q] = alph[threadIdx.x];
-	}
-	__syncthreads();
 	const TileCtx tc = load_tile(tl, tile);
-	double *lam0, *lam1, *lam2, *D;
+	double *lam0, *lam1, *lam2, *D, *X;
 	TC *n;
 	uint32_t *u;
 	TileStruct ts;
-	carve<TC, RC>(smem, tl, lam0, lam1, lam2, D, n, u, ts);
+	carve<TC, RC>(smem, tl, lam0, lam1, lam2, D, X, n, u, ts);
 	const int r = threadIdx.x % RC;
-	const uint32_t e0 = threadIdx.x / RC;
-	constexpr uint32_t ES = TBLOCK / RC;
-	if (alive[r]) {
-		const double alpha = alph[r];
-		for (uint32_t t = e0; t < tc.nt; t += ES) {
-			const size_t g = (size_t)tl.tx[tc.tx_base + t] * rc.R + r0 + r;
-			const double a = A[g], b = B[g], c2 = C[g];
-			const double rr = b - a;
-			const double vv = (c2 - b) - rr;
-			double x = a - 2 * alpha * rr + alpha * alpha * vv;
-			if (st.clamp == 0) {
-				if (x < 0)
-					x = 0;
-			} else if (x <= 1e-300) {
-				x = c2 / 10;
-			}
-			lam0[t * RC + r] = x;
-		}
-	}
-	stage_counts<TC, RC>(tl, tc, rc, r0, alive, cnt, uniq, n, u);
+	ChunkRows<TC, 3> p;
+	load_rows<TC, RC, 3>(p, tc, rc, rc.clist[1] * RC, cnt, uniq, A, B, C);
+	load_alpha<TC, RC>(p, rc, rc.clist[1] * RC, st);
 	stage_struct(tl, tc, ts);
-	__syncthreads();
-	tile_map<TC, RC>(tl, tc, ts, lam0, lam1, D, n, u, alive, A, rc.R, r0,
-	                 stopf, rc.stop_min, rc.stop_rel);
-	if (threadIdx.x < RC && alive[threadIdx.x] && stopf[threadIdx.x])
-		rc.notconv[r0 + threadIdx.x] = 1;
+	for (uint32_t i = 0; i < nact; ++i) {
+		const uint32_t r0 = rc.clist[1 + i] * RC;
+		const bool rin = r0 + r < rc.R;
+		// sequential norms leave the two sums; the parallel ones alpha
+		const double alpha = st.alpha_given
+		                         ? p.qa
+		                         : step_len(p.qa, p.qb, T, st);
+		if (!st.alpha_given && tile == 0 && threadIdx.x < RC && !p.done)
+			rc.alpha[r0 + threadIdx.x] = alpha;
+#pragma unroll
+		for (int k = 0; k < PF_TX; ++k) {
+			const uint32_t e = threadIdx.x + TBLOCK * k;
+			if (e / RC < tc.nt)
+				lam0[e] = extrapolate(p.s[0][k], p.s[1][k], p.s[2][k],
+				                      alpha, st);
+		}
+		for (uint32_t e = threadIdx.x + TBLOCK * PF_TX; e / RC < tc.nt;
+		     e += TBLOCK) {
+			double x = 0.0;
+			if (rin) {
+				const size_t g =
+				    (size_t)(tc.tx_base + e / RC) * rc.R + r0 + r;
+				x = extrapolate(A[g], B[g], C[g], alpha, st);
+			}
+			lam0[e] = x;
+		}
+		store_counts<TC, RC, 3>(p, tc, rc, r0, cnt, uniq, n, u);
+		if (threadIdx.x < RC) {
+			alive[threadIdx.x] = p.done ? 0 : 1;
+			stopf[threadIdx.x] = 0;
+		}
+		__syncthreads();
+		if (i + 1 < nact) {
+			load_rows<TC, RC, 3>(p, tc, rc, rc.clist[2 + i] * RC, cnt,
+			                     uniq, A, B, C);
+			load_alpha<TC, RC>(p, rc, rc.clist[2 + i] * RC, st);
+		}
+		tile_map<TC, RC>(tl, tc, ts, lam0, lam1, D, X, n, u, alive, A,
+		                 rc.R, r0, stopf, rc.stop_min, rc.stop_rel);
+		if (threadIdx.x < RC && alive[threadIdx.x] && stopf[threadIdx.x])
+			rc.notconv[r0 + threadIdx.x] = 1;
+		__syncthreads();
+	}
 }
 
 // (n, mean, M2) folding of Chan, Golub & LeVeque (1979).
@@ -397,7 +560,7 @@ __global__ __launch_bounds__(TBLOCK) void k_batch_alpha(
 	const uint32_t i1 = min(tl.n_tiles, i0 + per);
 	Moments a = {0, 0, 0}, b = {0, 0, 0};
 	for (uint32_t i = i0; i < i1; ++i) {
-		const double nt = (double)tl.desc[4 * i + 1];
+		const double nt = (double)tl.desc[(size_t)HEMK_TILE_DESC * i + 1];
 		const double *pp = part + ((size_t)i * rc.R + r) * 4;
 		if (st.step == 0) {
 			a = mom_merge(a, Moments{nt, pp[0], pp[1]});
@@ -428,155 +591,62 @@ __global__ __launch_bounds__(TBLOCK) void k_batch_alpha(
 		rc.alpha[r] = step_len(sr[0].q, sv[0].q, T, st);
 }
 
-// Sequential norms of a batch (see hemk_batch_seq): lanes 0..NR-1 of wave 0
-// each run two left-to-right chains for one replicate while waves 1-3 stage
-// rows [t][r0, r0+NR) of the element values in LDS.
-#define BSEQ_NR 16
-#define BSEQ_TB 64
-__global__ __launch_bounds__(TBLOCK) void k_batch_seq(
-    hemk_rctl rc, uint32_t T, const double *__restrict__ A,
-    const double *__restrict__ B, const double *__restrict__ C, int phase,
-    hemk_step st)
-{
-#pragma clang fp contract(off)
-	__shared__ double buf[2][2][BSEQ_TB][BSEQ_NR];
-	__shared__ int alive[BSEQ_NR];
-	const uint32_t r0 = blockIdx.x * BSEQ_NR;
-	const uint32_t R = rc.R;
-	if (threadIdx.x < BSEQ_NR) {
-		const uint32_t r = r0 + threadIdx.x;
-		alive[threadIdx.x] =
-		    (r < R && (phase == 0 || !rc.done[r])) ? 1 : 0;
-	}
-	__syncthreads();
-	int any = 0;
-	for (int i = 0; i < BSEQ_NR; ++i)
-		any |= alive[i];
-	if (!any)
-		return;
-	const int w = threadIdx.x / 64;
-	const uint32_t nch = (T + BSEQ_TB - 1) / BSEQ_TB;
-	double acc_a = 0, acc_b = 0, mr = 0, mv = 0;
-	if (phase == 2 && threadIdx.x < BSEQ_NR && alive[threadIdx.x]) {
-		mr = rc.mean_r[r0 + threadIdx.x];
-		mv = rc.mean_v[r0 + threadIdx.x];
-	}
-	__shared__ double smr[BSEQ_NR], smv[BSEQ_NR];
-	if (threadIdx.x < BSEQ_NR) {
-		smr[threadIdx.x] = mr;
-		smv[threadIdx.x] = mv;
-	}
-	__syncthreads();
-	for (uint32_t ch = 0; ch <= nch; ++ch) {
-		if (w >= 1 && ch < nch) {
-			const int slot = ch & 1;
-			const uint32_t t0 = ch * BSEQ_TB;
-			for (int it = threadIdx.x - 64; it < BSEQ_TB * BSEQ_NR;
-			     it += TBLOCK - 64) {
-				const int q = it % BSEQ_NR;
-				const int row = it / BSEQ_NR;
-				const uint32_t t = t0 + row;
-				double a = 0, b = 0;
-				if (t < T && alive[q]) {
-					const size_t g = (size_t)t * R + r0 + q;
-					if (phase == 0) {
-						a = A[g];
-					} else {
-						const double x = B[g] - A[g];
-						const double y = (C[g] - B[g]) - x;
-						if (phase == 1) {
-							a = x;
-							b = y;
-						} else if (phase == 2) {
-							a = (x - smr[q]) * (x - smr[q]);
-							b = (y - smv[q]) * (y - smv[q]);
-						} else {
-							a = x * x;
-							b = y * y;
-						}
-					}
-				}
-				buf[slot][0][row][q] = a;
-				buf[slot][1][row][q] = b;
-			}
-		}
-		if (w == 0 && threadIdx.x < BSEQ_NR && ch >= 1) {
-			const int slot = (ch - 1) & 1;
-			const uint32_t cnt = min((uint32_t)BSEQ_TB,
-			                         T - (ch - 1) * BSEQ_TB);
-			const int q = threadIdx.x;
-#pragma unroll 8
-			for (uint32_t j = 0; j < cnt; ++j) {
-				acc_a += buf[slot][0][j][q];
-				acc_b += buf[slot][1][j][q];
-			}
-		}
-		__syncthreads();
-	}
-	if (threadIdx.x < BSEQ_NR && alive[threadIdx.x]) {
-		const uint32_t r = r0 + threadIdx.x;
-		if (phase == 0) {
-			rc.lam_sum[r] = acc_a;
-		} else if (phase == 1) {
-			rc.mean_r[r] = acc_a / T;
-			rc.mean_v[r] = acc_b / T;
-		} else {
-			rc.alpha[r] = step_len(acc_a, acc_b, T, st);
-		}
-	}
-}
-
-// acc + b[0] + b[1] + ... + b[cnt-1], left to right.  The next eight values
-// are read from LDS while the current eight are added, so the chain runs at
-// the latency of a dependent FP64 add rather than of an LDS read.
+// acc + b[0] + b[1] + ... + b[cnt-1], left to right.  Two register banks of
+// sixteen values alternate: one is read from LDS while the other is added,
+// so the chain runs at the latency of a dependent FP64 add rather than of
+// an LDS read, with no register copies in the loop.
 __device__ __forceinline__ double chain_add(double acc, const double *b,
                                             uint32_t cnt)
 {
 #pragma clang fp contract(off)
 	uint32_t j = 0;
-	if (cnt >= 8) {
-		double x0 = b[0], x1 = b[1], x2 = b[2], x3 = b[3];
-		double x4 = b[4], x5 = b[5], x6 = b[6], x7 = b[7];
-		for (j = 8; j + 8 <= cnt; j += 8) {
-			const double y0 = b[j], y1 = b[j + 1], y2 = b[j + 2],
-			             y3 = b[j + 3], y4 = b[j + 4], y5 = b[j + 5],
-			             y6 = b[j + 6], y7 = b[j + 7];
-			acc += x0;
-			acc += x1;
-			acc += x2;
-			acc += x3;
-			acc += x4;
-			acc += x5;
-			acc += x6;
-			acc += x7;
-			x0 = y0;
-			x1 = y1;
-			x2 = y2;
-			x3 = y3;
-			x4 = y4;
-			x5 = y5;
-			x6 = y6;
-			x7 = y7;
+	if (cnt >= 16) {
+		double a[16], c[16];
+#pragma unroll
+		for (int q = 0; q < 16; ++q)
+			a[q] = b[q];
+		// a holds b[j-16, j)
+		for (j = 16; j + 32 <= cnt; j += 32) {
+#pragma unroll
+			for (int q = 0; q < 16; ++q)
+				c[q] = b[j + q];
+#pragma unroll
+			for (int q = 0; q < 16; ++q)
+				acc += a[q];
+#pragma unroll
+			for (int q = 0; q < 16; ++q)
+				a[q] = b[j + 16 + q];
+#pragma unroll
+			for (int q = 0; q < 16; ++q)
+				acc += c[q];
 		}
-		acc += x0;
-		acc += x1;
-		acc += x2;
-		acc += x3;
-		acc += x4;
-		acc += x5;
-		acc += x6;
-		acc += x7;
+		if (j + 16 <= cnt) {
+#pragma unroll
+			for (int q = 0; q < 16; ++q)
+				c[q] = b[j + q];
+#pragma unroll
+			for (int q = 0; q < 16; ++q)
+				acc += a[q];
+#pragma unroll
+			for (int q = 0; q < 16; ++q)
+				acc += c[q];
+			j += 16;
+		} else {
+#pragma unroll
+			for (int q = 0; q < 16; ++q)
+				acc += a[q];
+		}
 	}
 	for (; j < cnt; ++j)
 		acc += b[j];
 	return acc;
 }
 
-// Sequential norms of a batch from the replicate-major r/v rows written by
-// the map kernel: block (replicate, which) runs one left-to-right chain on
-// lane 0 of wave 0 while waves 1-3 stage the next chunk of element values.
-//   phase 1: mean = (sum x)/T      phase 2: q = sum (x - mean)^2
-//   phase 3: q = sum x^2 (L2 step)
+// Sequential sums of a batch over replicate-major rows: block (replicate,
+// row) runs one left-to-right chain on lane 0 of wave 0 while waves 1-3
+// stage the next chunk of element values.
+//   phase 0: lam_sum = sum x      phase 1: mean = (sum x)/T
+//   phase 2: q = sum (x - mean)^2  phase 3: q = sum x^2 (L2 step)
 #define CHAIN_TB 2048
 __global__ __launch_bounds__(TBLOCK) void k_batch_chain(
     hemk_rctl rc, uint32_t T, const double *__restrict__ xr,
@@ -586,7 +656,7 @@ __global__ __launch_bounds__(TBLOCK) void k_batch_chain(
 	__shared__ double buf[2][CHAIN_TB];
 	const uint32_t r = blockIdx.x;
 	const int which = blockIdx.y;
-	if (rc.done[r])
+	if (phase != 0 && rc.done[r])
 		return;
 	const double *x = (which ? xv : xr) + (size_t)r * T;
 	const double m = phase == 2 ? (which ? rc.mean_v[r] : rc.mean_r[r]) : 0.0;
@@ -602,7 +672,7 @@ __global__ __launch_bounds__(TBLOCK) void k_batch_chain(
 				double a = 0;
 				if (t < T) {
 					const double v = x[t];
-					a = phase == 1 ? v : phase == 2 ? (v - m) * (v - m)
+					a = phase <= 1 ? v : phase == 2 ? (v - m) * (v - m)
 					                                : v * v;
 				}
 				buf[slot][j] = a;
@@ -617,7 +687,9 @@ __global__ __launch_bounds__(TBLOCK) void k_batch_chain(
 		__syncthreads();
 	}
 	if (threadIdx.x == 0) {
-		if (phase == 1) {
+		if (phase == 0) {
+			rc.lam_sum[r] = acc;
+		} else if (phase == 1) {
 			if (which)
 				rc.mean_v[r] = acc / T;
 			else
@@ -629,6 +701,41 @@ __global__ __launch_bounds__(TBLOCK) void k_batch_chain(
 				rc.q_r[r] = acc;
 		}
 	}
+}
+
+// clist = the chunks (of rc.chunk replicates) holding a replicate that is
+// not done, in ascending order, after clist[0] = their count.  One block;
+// done[] written by this block before the call.
+__device__ void build_chunk_list(const hemk_rctl &rc)
+{
+	__shared__ uint32_t sc[TBLOCK];
+	const uint32_t RC = rc.chunk;
+	const uint32_t nch = (rc.R + RC - 1) / RC;
+	uint32_t total = 0;
+	for (uint32_t base = 0; base < nch; base += TBLOCK) {
+		const uint32_t ch = base + threadIdx.x;
+		uint32_t f = 0;
+		if (ch < nch) {
+			const uint32_t r1 = min(rc.R, (ch + 1) * RC);
+			for (uint32_t r = ch * RC; r < r1; ++r)
+				f |= rc.done[r] ? 0u : 1u;
+		}
+		sc[threadIdx.x] = f;
+		__syncthreads();
+		// inclusive scan of the flags
+		for (uint32_t w = 1; w < TBLOCK; w <<= 1) {
+			const uint32_t v = threadIdx.x >= w ? sc[threadIdx.x - w] : 0u;
+			__syncthreads();
+			sc[threadIdx.x] += v;
+			__syncthreads();
+		}
+		if (f)
+			rc.clist[1 + total + sc[threadIdx.x] - 1] = ch;
+		total += sc[TBLOCK - 1];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		rc.clist[0] = total;
 }
 
 __global__ __launch_bounds__(TBLOCK) void k_batch_iter_end(hemk_rctl rc)
@@ -658,6 +765,7 @@ __global__ __launch_bounds__(TBLOCK) void k_batch_iter_end(hemk_rctl rc)
 	}
 	if (threadIdx.x == 0)
 		rc.active[0] = cnt[0];
+	build_chunk_list(rc);
 }
 
 __global__ __launch_bounds__(TBLOCK) void k_batch_reset(hemk_rctl rc,
@@ -670,55 +778,67 @@ __global__ __launch_bounds__(TBLOCK) void k_batch_reset(hemk_rctl rc,
 	     i += (size_t)gridDim.x * TBLOCK)
 		A[i] = x;
 	if (blockIdx.x == 0) {
-		uint32_t act = 0;
+		// the loop condition of src/EM.c:328 at k = 0, stop = 0
+		const uint32_t run = (0 < rc.min_iter || 0 < rc.max_iter) ? 1u : 0u;
 		for (uint32_t r = threadIdx.x; r < rc.R; r += TBLOCK) {
 			rc.k[r] = 0;
 			rc.conv[r] = 0;
 			rc.notconv[r] = 0;
 			rc.alpha[r] = 0;
 			rc.lam_sum[r] = 0;
-			// the loop condition of src/EM.c:328 at k = 0, stop = 0
-			rc.done[r] = (0 < rc.min_iter || 0 < rc.max_iter) ? 0u : 1u;
+			rc.done[r] = run ? 0u : 1u;
 		}
-		(void)act;
 		if (threadIdx.x == 0)
-			rc.active[0] = (0 < rc.min_iter || 0 < rc.max_iter) ? rc.R : 0;
+			rc.active[0] = run ? rc.R : 0;
+		__syncthreads();
+		build_chunk_list(rc);
 	}
 }
 
-// est[r][t], tpm[r][t] from lam[t][r] through a 32x32 LDS transpose.
-__global__ __launch_bounds__(TBLOCK) void k_batch_finalize(
-    hemk_rctl rc, uint32_t T, const double *__restrict__ A,
-    const uint32_t *__restrict__ len, double *__restrict__ est,
-    double *__restrict__ tpm)
+// X[r][t] = A[txq[t]][r] through a 32x32 LDS transpose.
+__global__ __launch_bounds__(TBLOCK) void k_batch_rows(
+    uint32_t R, uint32_t T, const uint32_t *__restrict__ txq,
+    const double *__restrict__ A, double *__restrict__ X)
 {
 	__shared__ double tile[32][33];
 	const uint32_t t0 = blockIdx.x * 32, q0 = blockIdx.y * 32;
-	const uint32_t R = rc.R;
-	// load lam[t0+i][q0+j]: j fastest
+	// load A[txq[t0+i]][q0+j]: j fastest
 	for (uint32_t k = threadIdx.x; k < 32 * 32; k += TBLOCK) {
 		const uint32_t i = k / 32, j = k % 32;
 		const uint32_t t = t0 + i, q = q0 + j;
-		tile[i][j] = (t < T && q < R) ? A[(size_t)t * R + q] : 0.0;
+		tile[i][j] = (t < T && q < R) ? A[(size_t)txq[t] * R + q] : 0.0;
 	}
 	__syncthreads();
-	// store out[q0+j][t0+i]: i fastest
+	// store X[q0+j][t0+i]: i fastest
 	for (uint32_t k = threadIdx.x; k < 32 * 32; k += TBLOCK) {
 		const uint32_t j = k / 32, i = k % 32;
 		const uint32_t t = t0 + i, q = q0 + j;
-		if (t < T && q < R) {
-			const double l = tile[i][j];
-			const double sum = rc.lam_sum[q];
-			double e = 0, g = 0;
-			if (sum != 0) {
-				e = l * (double)len[t];
-				g = l * 1000000 / sum;
-			}
-			if (est)
-				est[(size_t)q * T + t] = e;
-			if (tpm)
-				tpm[(size_t)q * T + t] = g;
+		if (t < T && q < R)
+			X[(size_t)q * T + t] = tile[i][j];
+	}
+}
+
+// est and tpm of src/EM.c:381-387 for row r = blockIdx.y.
+__global__ __launch_bounds__(TBLOCK) void k_batch_out(
+    hemk_rctl rc, uint32_t T, const double *__restrict__ X,
+    const uint32_t *__restrict__ len, double *__restrict__ est,
+    double *__restrict__ tpm)
+{
+	const uint32_t r = blockIdx.y;
+	const double sum = rc.lam_sum[r];
+	for (uint32_t t = blockIdx.x * TBLOCK + threadIdx.x; t < T;
+	     t += gridDim.x * TBLOCK) {
+		const size_t i = (size_t)r * T + t;
+		const double l = X[i];
+		double e = 0, g = 0;
+		if (sum != 0) {
+			e = l * (double)len[t];
+			g = l * 1000000 / sum;
 		}
+		if (est)
+			est[i] = e;
+		if (tpm)
+			tpm[i] = g;
 	}
 }
 
@@ -752,9 +872,10 @@ inline uint32_t cdiv32(uint64_t a, uint64_t b)
 size_t tile_lds(const hemk_tiles *t, int rc, int cnt_bytes)
 {
 	return (size_t)rc * ((size_t)t->max_tx * (3 * 8 + 4) +
-	                     (size_t)t->max_cls * (8 + cnt_bytes)) +
-	       2 * ((size_t)t->max_cls + 1) + 2 * ((size_t)t->max_tx + 1) +
-	       3 * (size_t)t->max_nnz + 16;
+	                     (size_t)t->max_cls * (8 + cnt_bytes) +
+	                     (size_t)t->max_nnz * 8) +
+	       (size_t)t->max_nnz * (4 + 2) + (size_t)t->max_tx * 8 +
+	       2 * ((size_t)t->max_cls + 1) + 2 * ((size_t)t->max_tx + 1) + 16;
 }
 
 template <typename TC, int RC>
@@ -764,8 +885,9 @@ int launch_map12(const hemk_tiles *t, hemk_rctl rc, const void *cnt,
                  hipStream_t s)
 {
 	const size_t lds = tile_lds(t, RC, sizeof(TC));
-	dim3 grid(t->n_tiles, cdiv32(rc.R, RC));
-	k_tile_map12<TC, RC><<<grid, TBLOCK, lds, s>>>(
+	if (rc.chunk != (uint32_t)RC)
+		return (int)hipErrorInvalidValue;
+	k_tile_map12<TC, RC><<<t->n_tiles, TBLOCK, lds, s>>>(
 	    *t, rc, (const TC *)cnt, uniq, A, B, C, part, xr, xv, T, step);
 	return (int)hipGetLastError();
 }
@@ -776,8 +898,9 @@ int launch_ex3(const hemk_tiles *t, hemk_rctl rc, const void *cnt,
                const double *C, uint32_t T, hemk_step st, hipStream_t s)
 {
 	const size_t lds = tile_lds(t, RC, sizeof(TC));
-	dim3 grid(t->n_tiles, cdiv32(rc.R, RC));
-	k_tile_ex3<TC, RC><<<grid, TBLOCK, lds, s>>>(
+	if (rc.chunk != (uint32_t)RC)
+		return (int)hipErrorInvalidValue;
+	k_tile_ex3<TC, RC><<<t->n_tiles, TBLOCK, lds, s>>>(
 	    *t, rc, (const TC *)cnt, uniq, A, B, C, T, st);
 	return (int)hipGetLastError();
 }
@@ -856,19 +979,10 @@ int hemk_batch_alpha(const hemk_tiles *t, hemk_rctl rc, const double *part,
 	return (int)hipGetLastError();
 }
 
-int hemk_batch_seq(hemk_rctl rc, uint32_t T, const double *A,
-                   const double *B, const double *C, int phase, hemk_step st,
-                   void *s)
-{
-	k_batch_seq<<<cdiv32(rc.R, BSEQ_NR), TBLOCK, 0, (hipStream_t)s>>>(
-	    rc, T, A, B, C, phase, st);
-	return (int)hipGetLastError();
-}
-
 int hemk_batch_chain(hemk_rctl rc, uint32_t T, const double *xr,
                      const double *xv, int phase, void *s)
 {
-	dim3 grid(rc.R, 2);
+	dim3 grid(rc.R, phase == 0 ? 1 : 2);
 	k_batch_chain<<<grid, TBLOCK, 0, (hipStream_t)s>>>(rc, T, xr, xv, phase);
 	return (int)hipGetLastError();
 }
@@ -888,13 +1002,26 @@ int hemk_batch_reset(hemk_rctl rc, uint32_t T, double *A, void *s)
 	return (int)hipGetLastError();
 }
 
-int hemk_batch_finalize(hemk_rctl rc, uint32_t T, const double *A,
-                        const uint32_t *len, double *est, double *tpm,
-                        void *s)
+int hemk_batch_rows(hemk_rctl rc, uint32_t T, const uint32_t *txq,
+                    const double *A, double *X, void *s)
 {
+	if (!T || !rc.R)
+		return 0;
 	dim3 grid(cdiv32(T, 32), cdiv32(rc.R, 32));
-	k_batch_finalize<<<grid, TBLOCK, 0, (hipStream_t)s>>>(rc, T, A, len, est,
-	                                                      tpm);
+	k_batch_rows<<<grid, TBLOCK, 0, (hipStream_t)s>>>(rc.R, T, txq, A, X);
+	return (int)hipGetLastError();
+}
+
+int hemk_batch_out(hemk_rctl rc, uint32_t T, const double *X,
+                   const uint32_t *len, double *est, double *tpm, void *s)
+{
+	if (!T || !rc.R)
+		return 0;
+	uint32_t gx = cdiv32(T, TBLOCK);
+	if (gx > 1024)
+		gx = 1024;
+	dim3 grid(gx, rc.R);
+	k_batch_out<<<grid, TBLOCK, 0, (hipStream_t)s>>>(rc, T, X, len, est, tpm);
 	return (int)hipGetLastError();
 }
 

@@ -30,14 +30,15 @@ static double now_ms(void)
 int hem_tiles_upload(hem_problem *p, const hem_tiles_host *h)
 {
 	void *s = p->stream;
-	size_t sz[8] = {
-	    (size_t)h->n_tiles * 16, (size_t)h->n_tx * 4, (size_t)h->n_tx * 4,
-	    ((size_t)h->n_cls + 1) * 4, (size_t)(h->nnz ? h->nnz : 1),
-	    ((size_t)h->n_tx + 1) * 4, (size_t)(h->nnz ? h->nnz : 1) * 2,
-	    (size_t)(h->n_cls ? h->n_cls : 1) * 4};
-	const void *src[8] = {h->desc, h->tx, h->len, h->coff,
-	                      h->cmem, h->toff, h->tcls, h->cls_orig};
-	for (int i = 0; i < 8; ++i) {
+	const size_t nz = h->nnz ? h->nnz : 1;
+	size_t sz[9] = {
+	    (size_t)h->n_tiles * HEM_TILE_DESC * 4, (size_t)h->n_tx * 4,
+	    (size_t)h->n_tx * 4, ((size_t)h->n_cls + 1) * 4, nz * 4,
+	    ((size_t)h->n_tx + 1) * 4, nz * 2,
+	    (size_t)(h->n_cls ? h->n_cls : 1) * 4, (size_t)h->n_tx * 4};
+	const void *src[9] = {h->desc, h->tx,   h->len,      h->coff, h->kmem,
+	                      h->toff, h->tpos, h->cls_orig, h->txq};
+	for (int i = 0; i < 9; ++i) {
 		int e = hemd_malloc((void **)&p->d_tile_mem[i], sz[i]);
 		if (e)
 			return hem_fail(HEM_ERR_NOMEM, "tile upload: %s",
@@ -57,10 +58,11 @@ int hem_tiles_upload(hem_problem *p, const hem_tiles_host *h)
 	p->dt.tx = p->d_tile_mem[1];
 	p->dt.len = p->d_tile_mem[2];
 	p->dt.coff = p->d_tile_mem[3];
-	p->dt.cmem = (const uint8_t *)p->d_tile_mem[4];
+	p->dt.kmem = p->d_tile_mem[4];
 	p->dt.toff = p->d_tile_mem[5];
-	p->dt.tcls = (const uint16_t *)p->d_tile_mem[6];
+	p->dt.tpos = (const uint16_t *)p->d_tile_mem[6];
 	p->d_cls_orig = p->d_tile_mem[7];
+	p->dt.txq = p->d_tile_mem[8];
 	return HEM_OK;
 }
 
@@ -96,6 +98,7 @@ void hem_batch_free(hem_problem *p)
 	hemd_free(w->C);
 	hemd_free(w->cat_rm);
 	hemd_free(w->catT);
+	hemd_free(w->uq);
 	hemd_free(w->cnt);
 	hemd_free(w->part);
 	hemd_free(w->X);
@@ -106,7 +109,7 @@ void hem_batch_free(hem_problem *p)
 	hemd_free(w->f64);
 	hemd_host_free(w->h_active);
 	memset(w, 0, sizeof(*w));
-	for (int i = 0; i < 8; ++i) {
+	for (int i = 0; i < 9; ++i) {
 		hemd_free(p->d_tile_mem[i]);
 		p->d_tile_mem[i] = NULL;
 	}
@@ -122,7 +125,7 @@ static int ws_ensure(hem_problem *p, uint32_t R)
 		return HEM_OK;
 	{
 		/* keep the tile arrays; drop the per-R buffers */
-		void *keep[8];
+		void *keep[9];
 		memcpy(keep, p->d_tile_mem, sizeof(keep));
 		memset(p->d_tile_mem, 0, sizeof(keep));
 		hem_batch_free(p);
@@ -143,13 +146,14 @@ static int ws_ensure(hem_problem *p, uint32_t R)
 	WALLOC(w->C, T * R * 8);
 	WALLOC(w->cat_rm, K * R * 4);
 	WALLOC(w->catT, K * R * 4);
+	WALLOC(w->uq, T * R * 4);
 	WALLOC(w->cnt, NC * R * 4);
 	WALLOC(w->part, NT * R * 4 * 8);
 	WALLOC(w->X, 2 * T * R * 8);
 	WALLOC(w->m, (size_t)R * HEMK_RS_MAXB * 4);
 	WALLOC(w->rs_plan, (K + 2 * ((size_t)HEMK_RS_MAXB + 1)) * 4);
 	WALLOC(w->seeds, (size_t)R * 8);
-	WALLOC(w->u32, ((size_t)4 * R + 1) * 4);
+	WALLOC(w->u32, ((size_t)5 * R + 2) * 4);
 	WALLOC(w->f64, (size_t)6 * R * 8);
 #undef WALLOC
 	if (hemd_host_alloc((void **)&w->h_active, 64))
@@ -174,6 +178,7 @@ static hemk_rctl rctl_for(hem_problem *p, uint32_t R, const hem_opts *o)
 	rc.conv = w->u32 + 2 * (size_t)cap;
 	rc.notconv = w->u32 + 3 * (size_t)cap;
 	rc.active = w->u32 + 4 * (size_t)cap;
+	rc.clist = w->u32 + 4 * (size_t)cap + 1;
 	rc.alpha = w->f64;
 	rc.mean_r = w->f64 + cap;
 	rc.mean_v = w->f64 + 2 * (size_t)cap;
@@ -195,7 +200,7 @@ static int batch_iteration(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 	hemk_step st = {o->step, o->clamp, o->alpha_clamp != 0, !exact};
 	const size_t TR = (size_t)p->n_tx * rc.R;
 	double *xr = exact ? w->X : NULL, *xv = exact ? w->X + TR : NULL;
-	HEM_TRY_DEV(hemk_tile_map12(&p->dt, rc, chunk, w->cnt, 4, w->catT, w->A,
+	HEM_TRY_DEV(hemk_tile_map12(&p->dt, rc, chunk, w->cnt, 4, w->uq, w->A,
 	                            w->B, w->C, exact ? NULL : w->part, xr, xv,
 	                            p->n_tx, o->step, s),
 	            "tile map12");
@@ -213,7 +218,7 @@ static int batch_iteration(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 		HEM_TRY_DEV(hemk_batch_alpha(&p->dt, rc, w->part, p->n_tx, st, s),
 		            "batch alpha");
 	}
-	HEM_TRY_DEV(hemk_tile_ex3(&p->dt, rc, chunk, w->cnt, 4, w->catT, w->A,
+	HEM_TRY_DEV(hemk_tile_ex3(&p->dt, rc, chunk, w->cnt, 4, w->uq, w->A,
 	                          w->B, w->C, p->n_tx, st, s),
 	            "tile ex3");
 	HEM_TRY_DEV(hemk_batch_iter_end(rc, s), "batch iter end");
@@ -293,13 +298,16 @@ static int check_batch_opts(const hem_problem *p, const hem_opts *o)
 }
 
 /* Counts already in w->cat_rm ([R][K], caller order): transpose to
- * [K][R] and gather the class rows into tile order. */
+ * [K][R] and gather the unique and class rows into tile order. */
 static int stage_counts(hem_problem *p, uint32_t R)
 {
 	hem_batch_ws *w = &p->bw;
 	const uint32_t K = p->n_tx + p->n_cls;
 	HEM_TRY_DEV(hemk_transpose_u32(w->cat_rm, R, K, w->catT, p->stream),
 	            "count transpose");
+	HEM_TRY_DEV(hemk_gather_rows_u32(w->catT, p->dt.tx, p->n_tx, R, w->uq,
+	                                 p->stream),
+	            "unique row gather");
 	HEM_TRY_DEV(hemk_gather_rows_u32(w->catT + (size_t)p->n_tx * R,
 	                                 p->d_cls_orig, p->dt.n_cls, R, w->cnt,
 	                                 p->stream),
@@ -322,10 +330,10 @@ static int run_batch(hem_problem *p, const hem_opts *o, uint32_t R,
 	                                                  : o->min_iter;
 	float ms = 0;
 	int rc2;
-	hemk_step st0 = {0, 0, 0, 0};
 
 	if (chunk <= 0)
 		return hem_fail(HEM_ERR_RANGE, "tiles do not fit the LDS budget");
+	rc.chunk = (uint32_t)chunk;
 	HEM_TRY_DEV(hemk_batch_reset(rc, p->n_tx, w->A, p->stream), "reset");
 	if ((rc2 = batch_graph(p, o, rc, chunk, &graph)))
 		return rc2;
@@ -345,13 +353,16 @@ static int run_batch(hem_problem *p, const hem_opts *o, uint32_t R,
 			                "batched loop did not terminate");
 	}
 	HEM_TRY_DEV(hemd_event_record(p->ev1, p->stream), "event");
-	/* finalisation: sequential sums per replicate, then est/tpm [r][t]
+	/* finalisation: the state as one row per replicate in transcript
+	 * order, its sequential sum (src/EM.c:381-384), then est/tpm [r][t]
 	 * into B and C */
-	HEM_TRY_DEV(hemk_batch_seq(rc, p->n_tx, w->A, NULL, NULL, 0, st0,
-	                           p->stream),
+	HEM_TRY_DEV(hemk_batch_rows(rc, p->n_tx, p->dt.txq, w->A, w->X,
+	                            p->stream),
+	            "state rows");
+	HEM_TRY_DEV(hemk_batch_chain(rc, p->n_tx, w->X, NULL, 0, p->stream),
 	            "final sums");
-	HEM_TRY_DEV(hemk_batch_finalize(rc, p->n_tx, w->A, p->d_len, w->B, w->C,
-	                                p->stream),
+	HEM_TRY_DEV(hemk_batch_out(rc, p->n_tx, w->X, p->d_len, w->B, w->C,
+	                           p->stream),
 	            "finalize");
 	if (est)
 		HEM_TRY_DEV(hemd_d2h(est, w->B, (size_t)p->n_tx * R * 8,

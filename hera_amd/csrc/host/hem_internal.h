@@ -17,12 +17,14 @@
 /* Workspace of a batched solve, kept on the handle and grown on demand. */
 typedef struct {
 	uint32_t R;                    /* capacity in replicates            */
-	double *A, *B, *C;             /* [T][R] state                      */
+	double *A, *B, *C;             /* [tile position][R] state          */
 	uint32_t *cat_rm;              /* [R][K] counts, caller order       */
 	uint32_t *catT;                /* [K][R]: unique rows, class rows   */
+	uint32_t *uq;                  /* [tile position][R] unique counts  */
 	uint32_t *cnt;                 /* [tile position][R] class counts   */
 	double *part;                  /* [tile][R][4] norm partials        */
-	double *X;                     /* [2][R][T] r and v rows            */
+	double *X;                     /* [2][R][T] r and v rows (lam rows
+	                                  at the end)                       */
 	uint32_t *m;                   /* [R][B] level-1 resample counts    */
 	uint32_t *rs_plan;             /* K weights + chunk starts + prefix */
 	uint64_t *seeds;               /* [R]                               */
@@ -68,7 +70,7 @@ struct hem_problem {
 	hemk_tiles dt;
 	uint32_t *h_cls_orig;      /* tile position -> class id (host)      */
 	uint32_t *d_cls_orig;      /* same, device                          */
-	uint32_t *d_tile_mem[8];   /* device arrays behind dt               */
+	uint32_t *d_tile_mem[9];   /* device arrays behind dt               */
 	int counts_integral;       /* class counts are integers < 2^32      */
 	double *h_cls_count;       /* host copy of the class counts         */
 	hem_batch_ws bw;
@@ -88,21 +90,22 @@ struct hem_problem {
 /* Components are packed into tiles up to these sizes (a larger component,
  * within the limits above, gets a tile of its own): smaller tiles need less
  * LDS per workgroup, so more workgroups share a CU. */
-#define HEM_TILE_TX_TARGET  32
-#define HEM_TILE_CLS_TARGET 160
+#define HEM_TILE_TX_TARGET  16
+#define HEM_TILE_CLS_TARGET 80
+#define HEM_TILE_NNZ_TARGET 384
+#define HEM_TILE_DESC HEMK_TILE_DESC
 
 typedef struct {
 	uint32_t n_tiles, n_tx, n_cls, max_tx, max_cls, max_nnz;
 	uint64_t nnz;
-	uint32_t *desc, *tx, *len, *coff, *toff, *cls_orig;
-	uint8_t *cmem;
-	uint16_t *tcls;
+	uint32_t *desc, *tx, *len, *coff, *toff, *cls_orig, *txq;
+	uint32_t *kmem;
+	uint16_t *tpos;
 } hem_tiles_host;
 
 int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
                     const uint32_t *cls_ref, const uint32_t *tx_len,
-                    const uint32_t *tx_off, const uint32_t *tx_ref,
-                    hem_tiles_host *h);
+                    const uint32_t *tx_off, hem_tiles_host *h);
 void hem_tiles_free_host(hem_tiles_host *h);
 
 /* kernel kinds for per-kernel timing (hem_kernel_profile) */

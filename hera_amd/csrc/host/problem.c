@@ -244,7 +244,7 @@ int hem_problem_create(const hem_classes *c, hem_problem **out)
 	{
 		hem_tiles_host th;
 		int trc = hem_tiles_build(T, C, c->cls_off, c->cls_ref,
-		                          c->tx_len, tx_off, tx_ref, &th);
+		                          c->tx_len, tx_off, &th);
 		if (trc == HEM_OK) {
 			trc = hem_tiles_upload(p, &th);
 			p->h_cls_orig = th.cls_orig;

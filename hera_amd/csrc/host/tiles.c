@@ -7,19 +7,25 @@
  * component's transcripts and classes only, so a workgroup that holds a whole
  * component in LDS can run the class pass and the transcript pass without
  * leaving the CU.  Components are packed, in order of their smallest
- * transcript id, into tiles of at most HEM_TILE_TX transcripts and
- * HEM_TILE_CLS classes.
+ * transcript id, into tiles of about HEM_TILE_TX_TARGET transcripts,
+ * HEM_TILE_CLS_TARGET classes and HEM_TILE_NNZ_TARGET members; a component
+ * above HEM_TILE_TX transcripts or HEM_TILE_CLS classes is not tiled.
  *
  * Per tile the device holds:
- *   desc     {tx_base, n_tx, cls_base, n_cls}
+ *   desc     {tx_base, n_tx, cls_base, n_cls, member base, members,
+ *            class-list base, 0}
+ *   txq      transcript id -> tile position             (all transcripts)
  *   tx       global transcript ids, ascending          (n_tx)
  *   len      their lengths                             (n_tx)
  *   coff     member offsets of the tile's classes      (n_cls + 1 shared)
- *   cmem     member local indices (u8), hera's order    (members)
+ *   kmem     members in hera's order, each (local class << 8) | local
+ *            transcript                                 (members)
  *   toff     class-list offsets of the tile's tx        (n_tx + 1 shared)
- *   tcls     local class indices (u16) per transcript, ascending original
- *            class id, a class once per occurrence -- the order hera's class
- *            loop adds into overlap[t] (src/EM.c:276-285)
+ *   tpos     per transcript, the positions (u16, relative to the tile's
+ *            first member) of its occurrences in the member lists, in
+ *            ascending original class id and, within a class, member order
+ *            -- the order hera's class loop adds into overlap[t]
+ *            (src/EM.c:276-285)
  * Classes are renumbered in tile order ("tile position"); per-replicate
  * class counts are stored in that order so a tile reads one contiguous
  * block.  cls_orig maps a tile position back to the caller's class id.
@@ -47,23 +53,24 @@ void hem_tiles_free_host(hem_tiles_host *h)
 	free(h->tx);
 	free(h->len);
 	free(h->coff);
-	free(h->cmem);
+	free(h->kmem);
+	free(h->txq);
 	free(h->toff);
-	free(h->tcls);
+	free(h->tpos);
 	free(h->cls_orig);
 	memset(h, 0, sizeof(*h));
 }
 
-/* Builds the tiling of a CSR class table with its CSC transpose
- * (tx_off/tx_ref as built by hem_problem_create).  Returns HEM_ERR_RANGE
- * when a component exceeds the tile limits. */
+/* Builds the tiling of a CSR class table, given the transcripts' occurrence
+ * offsets tx_off (the CSC offsets built by hem_problem_create).  Returns
+ * HEM_ERR_RANGE when a component exceeds the tile limits. */
 int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
                     const uint32_t *cls_ref, const uint32_t *tx_len,
-                    const uint32_t *tx_off, const uint32_t *tx_ref,
-                    hem_tiles_host *h)
+                    const uint32_t *tx_off, hem_tiles_host *h)
 {
 	uint32_t *par = NULL, *comp = NULL, *ctx = NULL, *ccls = NULL;
-	uint32_t *ctile = NULL, *lidx = NULL, *lcidx = NULL, *pos = NULL;
+	uint32_t *ctile = NULL, *lidx = NULL, *pos = NULL;
+	uint32_t *cnz = NULL, *txpos = NULL, *mloc = NULL;
 	uint32_t *tile_ntx = NULL, *tile_ncls = NULL;
 	uint32_t ncomp = 0, ntile = 0, i, k;
 	uint64_t zlive = 0;
@@ -73,8 +80,7 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 	par = malloc((size_t)T * 4);
 	comp = malloc((size_t)T * 4);
 	lidx = malloc((size_t)T * 4);
-	lcidx = malloc((size_t)(C ? C : 1) * 4);
-	if (!par || !comp || !lidx || !lcidx) {
+	if (!par || !comp || !lidx) {
 		rc = hem_fail(HEM_ERR_NOMEM, "tiling: host allocation");
 		goto out;
 	}
@@ -104,8 +110,9 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 	}
 	ctx = calloc(ncomp, 4);
 	ccls = calloc(ncomp, 4);
+	cnz = calloc(ncomp, 4);
 	ctile = malloc((size_t)ncomp * 4);
-	if (!ctx || !ccls || !ctile) {
+	if (!ctx || !ccls || !cnz || !ctile) {
 		rc = hem_fail(HEM_ERR_NOMEM, "tiling: host allocation");
 		goto out;
 	}
@@ -114,6 +121,7 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 	for (i = 0; i < C; ++i)
 		if (cls_off[i + 1] > cls_off[i]) {
 			ccls[comp[cls_ref[cls_off[i]]]]++;
+			cnz[comp[cls_ref[cls_off[i]]]] += cls_off[i + 1] - cls_off[i];
 			zlive += cls_off[i + 1] - cls_off[i];
 		}
 	for (i = 0; i < ncomp; ++i)
@@ -126,29 +134,31 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 		}
 	/* pack components into tiles */
 	{
-		uint32_t nt = 0, nc = 0;
+		uint32_t nt = 0, nc = 0, nz = 0;
 		for (i = 0; i < ncomp; ++i) {
 			if (nt && (nt + ctx[i] > HEM_TILE_TX_TARGET ||
-			           nc + ccls[i] > HEM_TILE_CLS_TARGET)) {
+			           nc + ccls[i] > HEM_TILE_CLS_TARGET ||
+			           nz + cnz[i] > HEM_TILE_NNZ_TARGET)) {
 				ntile++;
-				nt = nc = 0;
+				nt = nc = nz = 0;
 			}
 			ctile[i] = ntile;
 			nt += ctx[i];
 			nc += ccls[i];
+			nz += cnz[i];
 		}
 		ntile++;
 	}
 	tile_ntx = calloc(ntile + 1, 4);
 	tile_ncls = calloc(ntile + 1, 4);
-	h->desc = malloc((size_t)ntile * 4 * 4);
+	h->desc = calloc((size_t)ntile * HEM_TILE_DESC, 4);
 	h->tx = malloc((size_t)T * 4);
 	h->len = malloc((size_t)T * 4);
 	h->toff = malloc(((size_t)T + 1) * 4);
-	h->tcls = malloc((zlive ? zlive : 1) * 2);
+	h->tpos = malloc((zlive ? zlive : 1) * 2);
 	pos = malloc(((size_t)ntile + 1) * 4);
 	if (!tile_ntx || !tile_ncls || !h->desc || !h->tx || !h->len ||
-	    !h->toff || !h->tcls || !pos) {
+	    !h->toff || !h->tpos || !pos) {
 		rc = hem_fail(HEM_ERR_NOMEM, "tiling: host allocation");
 		goto out;
 	}
@@ -175,13 +185,13 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 	{
 		uint32_t acc = 0;
 		for (i = 0; i < ntile; ++i) {
-			h->desc[4 * i + 0] = acc;
-			h->desc[4 * i + 1] = tile_ntx[i];
+			h->desc[HEM_TILE_DESC * i + 0] = acc;
+			h->desc[HEM_TILE_DESC * i + 1] = tile_ntx[i];
 			acc += tile_ntx[i];
 		}
 		for (i = 0; i < ntile; ++i)
 			for (k = 0; k < tile_ntx[i]; ++k)
-				lidx[h->tx[h->desc[4 * i] + k]] = k;
+				lidx[h->tx[h->desc[HEM_TILE_DESC * i] + k]] = k;
 	}
 	/* classes in tile order, ascending original id within a tile */
 	h->n_cls = 0;
@@ -189,8 +199,8 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 		h->n_cls += tile_ncls[i];
 	h->cls_orig = malloc((h->n_cls ? h->n_cls : 1) * 4);
 	h->coff = malloc(((size_t)h->n_cls + 1) * 4);
-	h->cmem = malloc((zlive ? zlive : 1));
-	if (!h->cls_orig || !h->coff || !h->cmem) {
+	h->kmem = malloc((zlive ? zlive : 1) * 4);
+	if (!h->cls_orig || !h->coff || !h->kmem) {
 		rc = hem_fail(HEM_ERR_NOMEM, "tiling: host allocation");
 		goto out;
 	}
@@ -198,49 +208,88 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 		uint32_t acc = 0;
 		for (i = 0; i < ntile; ++i) {
 			pos[i] = acc;
-			h->desc[4 * i + 2] = acc;
-			h->desc[4 * i + 3] = tile_ncls[i];
+			h->desc[HEM_TILE_DESC * i + 2] = acc;
+			h->desc[HEM_TILE_DESC * i + 3] = tile_ncls[i];
 			acc += tile_ncls[i];
 		}
 		for (i = 0; i < C; ++i) {
-			if (cls_off[i + 1] == cls_off[i]) {
-				lcidx[i] = 0xFFFFFFFFu;
+			if (cls_off[i + 1] == cls_off[i])
 				continue;
-			}
 			uint32_t tl = ctile[comp[cls_ref[cls_off[i]]]];
-			uint32_t p = pos[tl]++;
-			h->cls_orig[p] = i;
-			lcidx[i] = p - h->desc[4 * tl + 2];
+			h->cls_orig[pos[tl]++] = i;
 		}
 	}
-	/* member lists (local transcript index, hera's member order) */
+	/* member lists in hera's member order, each member as (local class
+	 * << 8) | local transcript; mloc
+	 * maps a member of the caller's table to its position here */
 	{
+		const uint64_t nnz = C ? cls_off[C] : 0;
+		mloc = malloc((nnz ? nnz : 1) * 4);
+		txpos = malloc((nnz ? nnz : 1) * 4);
+		if (!mloc || !txpos) {
+			rc = hem_fail(HEM_ERR_NOMEM, "tiling: host allocation");
+			goto out;
+		}
 		uint64_t z = 0;
-		for (uint32_t q = 0; q < h->n_cls; ++q) {
-			uint32_t c = h->cls_orig[q];
-			h->coff[q] = (uint32_t)z;
-			for (k = cls_off[c]; k < cls_off[c + 1]; ++k)
-				h->cmem[z++] = (uint8_t)lidx[cls_ref[k]];
+		for (uint32_t tl = 0; tl < ntile; ++tl) {
+			const uint32_t q0 = h->desc[HEM_TILE_DESC * tl + 2];
+			const uint32_t q1 = q0 + h->desc[HEM_TILE_DESC * tl + 3];
+			for (uint32_t q = q0; q < q1; ++q) {
+				uint32_t c = h->cls_orig[q];
+				h->coff[q] = (uint32_t)z;
+				for (k = cls_off[c]; k < cls_off[c + 1]; ++k) {
+					mloc[k] = (uint32_t)z;
+					h->kmem[z++] = ((q - q0) << 8) |
+					               lidx[cls_ref[k]];
+				}
+			}
 		}
 		h->coff[h->n_cls] = (uint32_t)z;
 	}
-	/* per-transcript class lists in ascending original class id */
+	/* each transcript's occurrences as members, in ascending class id and
+	 * member order (a counting sort of the member positions), then
+	 * relative to the first member of the transcript's tile */
 	{
+		uint32_t *fill = lidx; /* reused: next free slot per transcript */
+		for (i = 0; i < T; ++i)
+			fill[i] = tx_off[i];
+		for (i = 0; i < C; ++i)
+			for (k = cls_off[i]; k < cls_off[i + 1]; ++k)
+				txpos[fill[cls_ref[k]]++] = k;
 		uint64_t z = 0;
-		for (uint32_t q = 0; q < T; ++q) {
-			uint32_t t = h->tx[q];
-			h->toff[q] = (uint32_t)z;
-			for (k = tx_off[t]; k < tx_off[t + 1]; ++k)
-				h->tcls[z++] = (uint16_t)lcidx[tx_ref[k]];
+		for (uint32_t tl = 0; tl < ntile; ++tl) {
+			const uint32_t q0 = h->desc[HEM_TILE_DESC * tl + 0];
+			const uint32_t q1 = q0 + h->desc[HEM_TILE_DESC * tl + 1];
+			const uint32_t mb = h->coff[h->desc[HEM_TILE_DESC * tl + 2]];
+			for (uint32_t q = q0; q < q1; ++q) {
+				uint32_t t = h->tx[q];
+				h->toff[q] = (uint32_t)z;
+				for (k = tx_off[t]; k < tx_off[t + 1]; ++k)
+					h->tpos[z++] =
+					    (uint16_t)(mloc[txpos[k]] - mb);
+			}
 		}
 		h->toff[T] = (uint32_t)z;
 	}
+	for (uint32_t tl = 0; tl < ntile; ++tl) {
+		uint32_t *d = h->desc + (size_t)HEM_TILE_DESC * tl;
+		d[4] = h->coff[d[2]];
+		d[5] = h->coff[d[2] + d[3]] - d[4];
+		d[6] = h->toff[d[0]];
+	}
+	h->txq = malloc((size_t)T * 4);
+	if (!h->txq) {
+		rc = hem_fail(HEM_ERR_NOMEM, "tiling: host allocation");
+		goto out;
+	}
+	for (uint32_t q = 0; q < T; ++q)
+		h->txq[h->tx[q]] = q;
 	h->n_tiles = ntile;
 	h->n_tx = T;
 	h->nnz = zlive;
 	h->max_tx = h->max_cls = h->max_nnz = 0;
 	for (i = 0; i < ntile; ++i) {
-		uint32_t q0 = h->desc[4 * i + 2], q1 = q0 + tile_ncls[i];
+		uint32_t q0 = h->desc[HEM_TILE_DESC * i + 2], q1 = q0 + tile_ncls[i];
 		uint32_t z = h->coff[q1] - h->coff[q0];
 		if (tile_ntx[i] > h->max_tx)
 			h->max_tx = tile_ntx[i];
@@ -256,9 +305,11 @@ out:
 	free(comp);
 	free(ctx);
 	free(ccls);
+	free(cnz);
 	free(ctile);
+	free(txpos);
+	free(mloc);
 	free(lidx);
-	free(lcidx);
 	free(pos);
 	free(tile_ntx);
 	free(tile_ncls);
@@ -268,7 +319,7 @@ out:
 int hem_classes_tiling(const hem_classes *c, uint32_t *n_tiles,
                        uint32_t *max_tx, uint32_t *max_cls)
 {
-	uint32_t *tx_off, *tx_ref, *pos, i;
+	uint32_t *tx_off, i;
 	uint64_t nnz;
 	hem_tiles_host h;
 	int rc;
@@ -279,27 +330,15 @@ int hem_classes_tiling(const hem_classes *c, uint32_t *n_tiles,
 		if (c->cls_ref[k] >= c->n_tx)
 			return hem_fail(HEM_ERR_INVALID, "transcript id out of range");
 	tx_off = calloc((size_t)c->n_tx + 1, 4);
-	pos = malloc((size_t)c->n_tx * 4);
-	tx_ref = malloc((nnz ? nnz : 1) * 4);
-	if (!tx_off || !pos || !tx_ref) {
-		free(tx_off);
-		free(pos);
-		free(tx_ref);
+	if (!tx_off)
 		return hem_fail(HEM_ERR_NOMEM, "host allocation");
-	}
 	for (uint64_t k = 0; k < nnz; ++k)
 		tx_off[c->cls_ref[k] + 1]++;
 	for (i = 0; i < c->n_tx; ++i)
 		tx_off[i + 1] += tx_off[i];
-	memcpy(pos, tx_off, (size_t)c->n_tx * 4);
-	for (i = 0; i < c->n_cls; ++i)
-		for (uint32_t k = c->cls_off[i]; k < c->cls_off[i + 1]; ++k)
-			tx_ref[pos[c->cls_ref[k]]++] = i;
 	rc = hem_tiles_build(c->n_tx, c->n_cls, c->cls_off, c->cls_ref,
-	                     c->tx_len, tx_off, tx_ref, &h);
+	                     c->tx_len, tx_off, &h);
 	free(tx_off);
-	free(pos);
-	free(tx_ref);
 	if (rc == HEM_ERR_RANGE) {
 		*n_tiles = 0;
 		*max_tx = *max_cls = 0;

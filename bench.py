@@ -9,7 +9,10 @@ the device; value = SQUAREM iterations per second over the timed steps.
 
 Also config 3 (--bootstrap-reps, default 100): the bootstrap replicates in
 one batched run, against the same replicates solved one by one on the GPU
-and against the oracle on hera's own thread schedule.
+and against the oracle on hera's own thread schedule; and config 4
+(--grad-topk, default 1000): the adjoint gradient and Hessian-vector
+products at the EM solution and the 50-step CG standard errors of the top
+transcripts, with the oracle's CPU gradient timed beside it.
 
 Modes (--modes, first one is the headline):
   fp64-exact  order-faithful FP64 E-step and the reference's sequential
@@ -53,6 +56,10 @@ def main():
     ap.add_argument("--modes", default="fp64-exact,fp64-tree,fp32-tree")
     ap.add_argument("--cpu-solves", type=int, default=1,
                     help="full oracle solves timed for cpu_baseline (0: skip)")
+    ap.add_argument("--grad-topk", type=int, default=1000,
+                    help="standard errors of this many transcripts (0: skip "
+                         "the differentiation section)")
+    ap.add_argument("--cg-steps", type=int, default=50)
     ap.add_argument("--bootstrap-reps", type=int, default=100,
                     help="bootstrap replicates of the batched run (0: skip)")
     ap.add_argument("--seq-sample", type=int, default=3,
@@ -168,6 +175,9 @@ def main():
         "upload_s": t_upload,
     }
 
+    if args.grad_topk > 0:
+        out["differentiation"] = bench_grad(args, H, np, p, cls, u, sk)
+
     boot = None
     if args.bootstrap_reps > 0:
         boot = bench_bootstrap(args, H, np, p, cls, u, sk)
@@ -216,6 +226,60 @@ def main():
         boot.pop("_est_exact", None)
     p.close()
     print(json.dumps(out), flush=True)
+
+
+def bench_grad(args, H, np, p, cls, u, sk):
+    """Config 4: the adjoint at the EM solution (theta = log(lambda len),
+    log_efflen = log len), Hessian-vector products, and the CG standard
+    errors of the top transcripts; the oracle's CPU gradient is timed beside
+    it.  Accuracy against central differences is covered by the tests."""
+    from oracle import oracle as O
+    g = p.solve(u, norm=H.NORM_TREE, **sk)
+    lam = g["lam"]
+    with np.errstate(divide="ignore"):
+        theta = np.log(lam * cls.tx_len)
+    theta -= theta[np.isfinite(theta)].max()
+    ell = np.log(cls.tx_len.astype(float))
+    reps = 10
+    res = {}
+    for name, prec in (("fp64", H.PREC_FP64), ("fp32_forward", H.PREC_FP32)):
+        p.grad(u, theta, ell, precision=prec)
+        ms = []
+        for _ in range(reps):
+            ms.append(p.grad(u, theta, ell, precision=prec)["ms"])
+        res["grad_" + name + "_ms"] = float(np.median(ms))
+    gg = p.grad(u, theta, ell)
+    E = -gg["g_log_efflen"]
+    res["grad_rel_stationarity"] = float(
+        np.max(np.abs(gg["g_theta"]) / (E + 1.0)))
+    v = np.random.default_rng(0).normal(size=cls.n_tx)
+    p.hvp(u, theta, ell, v)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        p.hvp(u, theta, ell, v)
+    res["hvp_wall_ms"] = (time.perf_counter() - t0) / reps * 1e3
+    p.stderr(u, theta, ell, top_k=args.grad_topk, cg_steps=2)
+    se = p.stderr(u, theta, ell, top_k=args.grad_topk,
+                  cg_steps=args.cg_steps)
+    st = se["stats"]
+    res.update({
+        "top_k": args.grad_topk, "cg_steps": args.cg_steps,
+        "se_ms_setup": st["ms_setup"], "se_ms_cg": st["ms_cg"],
+        "se_ms_total": st["ms_total"], "se_resid_max": st["resid_max"],
+        "se_resid_median": float(np.median(se["resid"])),
+        "se_chunk": st["chunk"],
+        "se_of_top1_tpm": float(se["se"][0] * 1e6),
+    })
+    t0 = O.now()
+    O.grad(cls, u, np.where(np.isfinite(theta), theta, -745.0), ell)
+    res["cpu_grad_ms"] = (O.now() - t0) * 1e3
+    res["cpu_grad_note"] = "oracle FP64 gradient, 1 thread"
+    log("grad fp64 %.3f ms, fp32-forward %.3f ms, HVP %.2f ms (wall), "
+        "CPU grad %.1f ms; SE top %d, %d CG steps: %.1f ms (resid max %.1e)"
+        % (res["grad_fp64_ms"], res["grad_fp32_forward_ms"],
+           res["hvp_wall_ms"], res["cpu_grad_ms"], args.grad_topk,
+           args.cg_steps, st["ms_cg"], st["resid_max"]))
+    return res
 
 
 def bench_bootstrap(args, H, np, p, cls, u, sk):

@@ -248,3 +248,47 @@ class Problem:
         out = _lib.C.c_double()
         check(_lib.load().hem_loglik(self._h, u32p(u), f64p(lam), out))
         return out.value
+
+    def grad(self, unique, theta, log_efflen, precision=PREC_FP64):
+        """Log-likelihood and its gradient in the softmax logits theta and
+        the log effective lengths (include/hera_em.h, hem_grad).  Returns
+        dict(loglik, n_frag, g_theta, g_log_efflen, ms)."""
+        u = as_u32(unique)
+        th = as_f64(theta)
+        el = as_f64(log_efflen)
+        gt = np.empty(len(th))
+        ge = np.empty(len(th))
+        st = _lib.HemGradStats()
+        check(_lib.load().hem_grad(self._h, u32p(u), f64p(th), f64p(el),
+                                   precision, f64p(gt), f64p(ge), st))
+        return dict(loglik=st.loglik, n_frag=st.n_frag, g_theta=gt,
+                    g_log_efflen=ge, ms=st.ms)
+
+    def hvp(self, unique, theta, log_efflen, v):
+        """Hessian-vector product in theta (hem_hvp)."""
+        u = as_u32(unique)
+        th = as_f64(theta)
+        el = as_f64(log_efflen)
+        v = as_f64(v)
+        hv = np.empty(len(th))
+        check(_lib.load().hem_hvp(self._h, u32p(u), f64p(th), f64p(el),
+                                  f64p(v), f64p(hv)))
+        return hv
+
+    def stderr(self, unique, theta, log_efflen, top_k=1000, cg_steps=50,
+               damping=0.0):
+        """Standard errors of the top_k abundances by batched CG
+        (hem_stderr).  Returns dict(idx, se, resid, stats)."""
+        u = as_u32(unique)
+        th = as_f64(theta)
+        el = as_f64(log_efflen)
+        idx = np.empty(top_k, dtype=np.uint32)
+        se = np.empty(top_k)
+        rs = np.empty(top_k)
+        st = _lib.HemSeStats()
+        check(_lib.load().hem_stderr(self._h, u32p(u), f64p(th), f64p(el),
+                                     top_k, cg_steps, damping, u32p(idx),
+                                     f64p(se), f64p(rs), st))
+        return dict(idx=idx, se=se, resid=rs,
+                    stats={k: getattr(st, k) for k, _ in
+                           _lib.HemSeStats._fields_})

@@ -66,6 +66,18 @@ class HemBatchStats(C.Structure):
                 ("ms_total", C.c_double)]
 
 
+class HemGradStats(C.Structure):
+    _fields_ = [("loglik", C.c_double), ("n_frag", C.c_double),
+                ("ms", C.c_double)]
+
+
+class HemSeStats(C.Structure):
+    _fields_ = [("top_k", C.c_uint32), ("cg_steps", C.c_uint32),
+                ("chunk", C.c_uint32), ("resid_max", C.c_double),
+                ("ms_setup", C.c_double), ("ms_cg", C.c_double),
+                ("ms_total", C.c_double)]
+
+
 class HemGenParams(C.Structure):
     _fields_ = [("n_tx", C.c_uint32), ("n_cls_all", C.c_uint32),
                 ("n_frag", C.c_uint64), ("cluster_min", C.c_uint32),
@@ -110,6 +122,12 @@ SIGNATURES = {
                                 C.POINTER(HemBatchStats)]),
     "hem_bootstrap_counts": (C.c_int, [C.c_void_p, _u32p, C.c_uint32,
                                        C.POINTER(C.c_uint64), _u32p, _f64p]),
+    "hem_grad": (C.c_int, [C.c_void_p, _u32p, _f64p, _f64p, C.c_int, _f64p,
+                           _f64p, C.POINTER(HemGradStats)]),
+    "hem_hvp": (C.c_int, [C.c_void_p, _u32p, _f64p, _f64p, _f64p, _f64p]),
+    "hem_stderr": (C.c_int, [C.c_void_p, _u32p, _f64p, _f64p, C.c_uint32,
+                             C.c_uint32, C.c_double, _u32p, _f64p, _f64p,
+                             C.POINTER(HemSeStats)]),
     "hem_gen_default": (None, [C.POINTER(HemGenParams), C.c_uint32,
                                C.c_uint32, C.c_uint64, C.c_uint64]),
     "hem_generate": (C.c_int, [C.POINTER(HemGenParams),

@@ -179,6 +179,55 @@ int hemk_fill_u32(uint32_t *p, size_t n, uint32_t v, void *s);
 int hemk_gather_rows_u32(const uint32_t *in, const uint32_t *idx,
                          uint32_t rows, uint32_t R, uint32_t *out, void *s);
 
+/* ---- adjoint of the log-likelihood (grad_kernels.hip) --------------------
+ * part: 3 * HEMK_NORM_BLOCKS doubles; scal[0] = L, scal[1] = N. */
+int hemk_grad_softmax(const double *theta, const double *ell, uint32_t T,
+                      double *pi, double *w, double *x, float *x32,
+                      double *part, void *s);
+int hemk_grad_passes(const hemk_prob *p, int fp32, const double *x,
+                     const float *x32, const uint32_t *unique,
+                     const double *pi, double *rho, double *D, double *kappa,
+                     double *E, double *fd, double *gth, double *gell,
+                     double *a, double *part, double *scal, void *s);
+/* hv = H v (FP64); kd: n_cls doubles; part: HEMK_NORM_BLOCKS doubles. */
+int hemk_hvp(const hemk_prob *p, const double *x, const double *a,
+             const double *pi, const double *kappa, const double *scal,
+             const double *v, double *kd, double *part, double *hv, void *s);
+
+/* ---- standard errors by batched CG (se_kernels.hip) ----------------------
+ * Vectors [q][K] in tile position order; per-direction partials [rb][K]
+ * over hemk_se_rowblocks(T) row blocks, folded by hemk_cg_fold. */
+int hemk_se_chunk(const hemk_tiles *t, uint32_t K);
+uint32_t hemk_se_rowblocks(uint32_t T);
+int hemk_se_tile(const hemk_tiles *t, int chunk, uint32_t K, const double *xq,
+                 const double *aq, const double *kq, const double *P,
+                 double *Q, void *s);
+int hemk_cg_init(const double *piq, const double *mq, const uint32_t *jq,
+                 const double *pj, uint32_t T, uint32_t K, double *X,
+                 double *R, double *P, double *prz, double *prr, double *ppv,
+                 void *s);
+int hemk_cg_fp(const double *piq, const double *pv, const double *scal,
+               double delta, uint32_t T, uint32_t K, const double *P,
+               double *Q, double *ppfp, void *s);
+int hemk_cg_xr(const double *mq, const double *rz, const double *pfp,
+               const double *rr, const double *bb, uint32_t T, uint32_t K,
+               const double *P, const double *FP, double *X, double *R,
+               double *prz, double *prr, void *s);
+int hemk_cg_p(const double *piq, const double *mq, const double *rz_old,
+              const double *rz_new, uint32_t T, uint32_t K, const double *R,
+              double *P, double *ppv, void *s);
+int hemk_cg_pix(const double *piq, uint32_t T, uint32_t K, const double *X,
+                double *ppx, void *s);
+int hemk_cg_fold(const double *part, uint32_t nrb, uint32_t K, double *out,
+                 void *s);
+int hemk_cg_out(const double *X, const uint32_t *jq, const double *pj,
+                const double *pix, const double *rr, const double *bb,
+                uint32_t K, double *se, double *resid, void *s);
+int hemk_gather_f64(const double *in, const uint32_t *idx, uint32_t n,
+                    double *out, void *s);
+int hemk_precond(const double *fdq, const double *piq, const double *scal,
+                 double delta, uint32_t n, double *mq, void *s);
+
 /* ---- kernel launchers --------------------------------------------------- */
 /* lam_init: fill lam64 (and lam32 if non-NULL) with `value`, or copy from
  * init64 when non-NULL. */

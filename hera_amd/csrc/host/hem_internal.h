@@ -40,6 +40,23 @@ typedef struct {
 	int n_graph;
 } hem_batch_ws;
 
+/* Workspace of the gradient, HVP and standard-error calls. */
+typedef struct {
+	int ready;                     /* T-sized arrays allocated          */
+	double *theta, *ell, *pi, *w, *x, *E, *fd, *gth, *gell, *a, *v, *hv;
+	float *x32;
+	uint32_t *u;
+	double *rho, *D, *kappa, *kd;  /* n_cls                             */
+	double *part, *scal;           /* 3 * HEMK_NORM_BLOCKS, 2           */
+	/* standard errors: tile-order constants and K-wide CG state */
+	uint32_t K;                    /* capacity in directions            */
+	double *piq, *xq, *aq, *mq, *fdq, *kq;
+	uint32_t *jq;
+	double *pj, *X, *R, *P, *Q;
+	double *prz, *prr, *ppv, *ppfp;
+	double *rz0, *rz1, *rr, *bb, *pv, *pfp, *pix, *se, *resid;
+} hem_grad_ws;
+
 struct hem_problem {
 	pthread_mutex_t mu;
 	uint32_t n_tx, n_cls;
@@ -74,6 +91,7 @@ struct hem_problem {
 	int counts_integral;       /* class counts are integers < 2^32      */
 	double *h_cls_count;       /* host copy of the class counts         */
 	hem_batch_ws bw;
+	hem_grad_ws gw;
 	void *ev2, *ev3;
 
 	/* captured iteration graphs and the options each was captured for;
@@ -114,6 +132,8 @@ enum { HEM_PK_DENOM = 0, HEM_PK_GATHER, HEM_PK_NORM1, HEM_PK_NORM2,
 
 /* batch.c */
 void hem_batch_free(hem_problem *p);
+/* grad.c */
+void hem_grad_free(hem_problem *p);
 int hem_tiles_upload(hem_problem *p, const hem_tiles_host *h);
 
 /* error.c */

@@ -215,6 +215,53 @@ typedef struct {
 int hem_profile_solve(hem_problem *p, const uint32_t *unique,
                       const hem_opts *o, hem_kernel_profile *out);
 
+/* ---- adjoint of the log-likelihood ----------------------------------------
+ * No counterpart in the reference, which has no derivatives; the objective
+ * is the one whose EM map is src/EM.c:271-302.  With pi = softmax(theta),
+ * w_t = exp(-log_efflen_t), x_t = pi_t w_t, the table's class counts n_c and
+ * the unique counts u_t as singleton classes:
+ *   L = sum_c n_c log sum_{t in c} x_t + sum_t u_t log x_t
+ *   dL/dtheta_t = E_t - N pi_t,   dL/dlog_efflen_t = -E_t,
+ * where E_t = x_t sum_{c ni t} n_c / D_c + u_t are the expected counts of
+ * the E-step and N = sum_t E_t.  At a fixed point of hera's EM,
+ * theta = log(lambda * len) (up to a constant) and dL/dtheta = 0.
+ * precision HEM_PREC_FP64, or HEM_PREC_FP32 for x and D_c in FP32 with
+ * every sum of the adjoint in FP64.  Outputs may be NULL. */
+typedef struct {
+	double loglik;   /* L                                                 */
+	double n_frag;   /* N                                                 */
+	double ms;       /* device time                                       */
+} hem_grad_stats;
+
+int hem_grad(hem_problem *p, const uint32_t *unique, const double *theta,
+             const double *log_efflen, int precision, double *g_theta,
+             double *g_log_efflen, hem_grad_stats *st);
+
+/* Hessian-vector product in theta, FP64: hv = (d^2 L / d theta^2) v. */
+int hem_hvp(hem_problem *p, const uint32_t *unique, const double *theta,
+            const double *log_efflen, const double *v, double *hv);
+
+/* Standard errors of the top_k abundances pi_j (largest pi first, ties by
+ * id) by the delta method with the observed information F = -H:
+ * se_j = sqrt(b_j . x_j), F x_j = b_j = pi_j (e_j - pi), solved for all
+ * top_k right-hand sides at once by cg_steps of Jacobi-preconditioned
+ * conjugate gradients over the component tiles (damping adds damping * I
+ * to F).  Outputs: idx[top_k] transcript ids, se[top_k], resid[top_k] =
+ * |r| / |b| after the last step.  Needs the component tiling
+ * (hem_problem_tiles n_tiles > 0). */
+typedef struct {
+	uint32_t top_k, cg_steps, chunk;
+	double   resid_max;          /* largest relative residual             */
+	double   ms_setup;           /* gradient, constants, selection        */
+	double   ms_cg;              /* device time of the CG steps           */
+	double   ms_total;           /* host wall time of the call            */
+} hem_se_stats;
+
+int hem_stderr(hem_problem *p, const uint32_t *unique, const double *theta,
+               const double *log_efflen, uint32_t top_k, uint32_t cg_steps,
+               double damping, uint32_t *idx, double *se, double *resid,
+               hem_se_stats *st);
+
 #ifdef __cplusplus
 }
 #endif

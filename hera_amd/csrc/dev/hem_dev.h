@@ -109,8 +109,8 @@ typedef struct {
 	double   *alpha, *mean_r, *mean_v, *lam_sum;
 	double   *q_r, *q_v;  /* second-pass sums of the sequential norms */
 	uint32_t *active;   /* [1]: replicates still iterating */
-	uint32_t *clist;    /* [0]: count, then the replicate chunks (of
-	                       chunk replicates) with a replicate iterating */
+	uint32_t *alist;    /* [0]: count, then the replicates still
+	                       iterating, ascending                         */
 } hemk_rctl;
 
 /* Largest replicate chunk (lanes per tile block) that fits the LDS budget

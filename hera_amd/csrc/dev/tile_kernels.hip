@@ -73,7 +73,8 @@ __device__ __forceinline__ void stage_struct(const hemk_tiles &tl,
 }
 
 // One EM map of the tile for the block's replicate chunk, lin -> lout in
-// LDS (and to the global [T][R] vector gout when given), in three passes
+// LDS (and to column qr, the lane's replicate, of the global [q][R] vector
+// gout when given), in three passes
 // separated by barriers:
 //   class pass       D_c = sum of lam over the members, in member order
 //   term pass        X_k = (lam_t n_c) / D_c for every member k = (c, t),
@@ -86,7 +87,7 @@ __device__ __forceinline__ void tile_map(
     const hemk_tiles &tl, const TileCtx &tc, const TileStruct &ts,
     const double *lin, double *lout, double *D, double *X, const TC *n,
     const uint32_t *u, const int *alive, double *gout, uint32_t R,
-    uint32_t r0, int *stopf, double smin, double srel)
+    uint32_t qr, int *stopf, double smin, double srel)
 {
 	const int r = threadIdx.x % RC;
 	const uint32_t e0 = threadIdx.x / RC;
@@ -165,7 +166,7 @@ __device__ __forceinline__ void tile_map(
 			                  (double)ts.len[t];
 			lout[t * RC + r] = nw;
 			if (gout)
-				gout[(size_t)(tc.tx_base + t) * R + r0 + r] = nw;
+				gout[(size_t)(tc.tx_base + t) * R + qr] = nw;
 			if (stopf && nw > smin && fabs(nw - l) / nw > srel)
 				stopf[r] = 1;
 		}
@@ -209,22 +210,34 @@ struct ChunkRows {
 	double s[NS][PF_TX];  // state rows: A, or A B C
 	uint32_t u[PF_TX];    // unique counts
 	TC n[PF_CLS];         // class counts
-	uint32_t done;        // the thread's replicate has finished (or >= R)
+	uint32_t q;           // the thread's replicate, NOREP past the list
 	double qa, qb;        // ex3: alpha, or the two sums it comes from
 };
+
+#define NOREP 0xFFFFFFFFu
+
+// The replicate of this thread's lane in chunk i of the active list.
+template <int RC>
+__device__ __forceinline__ uint32_t lane_rep(const hemk_rctl &rc,
+                                             uint32_t nrep, uint32_t i)
+{
+	const uint32_t j = i * RC + threadIdx.x % RC;
+	return j < nrep ? rc.alist[1 + j] : NOREP;
+}
 
 template <typename TC, int RC, int NS>
 __device__ __forceinline__ void load_rows(ChunkRows<TC, NS> &p,
                                           const TileCtx &tc,
-                                          const hemk_rctl &rc, uint32_t r0,
+                                          const hemk_rctl &rc, uint32_t nrep,
+                                          uint32_t i,
                                           const TC *__restrict__ cnt,
                                           const uint32_t *__restrict__ uniq,
                                           const double *s0, const double *s1,
                                           const double *s2)
 {
-	const uint32_t q = r0 + threadIdx.x % RC;
-	const bool rin = q < rc.R;
-	p.done = rin ? rc.done[q] : 1u;
+	const uint32_t q = lane_rep<RC>(rc, nrep, i);
+	const bool rin = q != NOREP;
+	p.q = q;
 #pragma unroll
 	for (int k = 0; k < PF_TX; ++k) {
 		const uint32_t t = (threadIdx.x + TBLOCK * k) / RC;
@@ -256,17 +269,17 @@ __device__ __forceinline__ void load_rows(ChunkRows<TC, NS> &p,
 	}
 }
 
-// Class and unique count rows of the chunk into LDS (n, u).
+// Class and unique count rows of the chunk into LDS (n, u); q is the
+// lane's replicate.
 template <typename TC, int RC, int NS>
 __device__ __forceinline__ void store_counts(const ChunkRows<TC, NS> &p,
                                              const TileCtx &tc,
-                                             const hemk_rctl &rc, uint32_t r0,
+                                             const hemk_rctl &rc, uint32_t q,
                                              const TC *__restrict__ cnt,
                                              const uint32_t *__restrict__ uniq,
                                              TC *n, uint32_t *u)
 {
-	const uint32_t q = r0 + threadIdx.x % RC;
-	const bool rin = q < rc.R;
+	const bool rin = q != NOREP;
 #pragma unroll
 	for (int k = 0; k < PF_TX; ++k) {
 		const uint32_t e = threadIdx.x + TBLOCK * k;
@@ -298,9 +311,10 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
 	extern __shared__ double smem[];
 	__shared__ int alive[RC];
 	const uint32_t tile = blockIdx.x;
-	const uint32_t nact = rc.clist[0];
-	if (nact == 0)
+	const uint32_t nrep = rc.alist[0];
+	if (nrep == 0)
 		return;
+	const uint32_t nch = (nrep + RC - 1) / RC;
 	const TileCtx tc = load_tile(tl, tile);
 	double *lam0, *lam1, *lam2, *D, *X;
 	TC *n;
@@ -311,12 +325,11 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
 	const uint32_t e0 = threadIdx.x / RC;
 	constexpr uint32_t ES = TBLOCK / RC;
 	ChunkRows<TC, 1> p;
-	load_rows<TC, RC, 1>(p, tc, rc, rc.clist[1] * RC, cnt, uniq, A, nullptr,
-	                     nullptr);
+	load_rows<TC, RC, 1>(p, tc, rc, nrep, 0, cnt, uniq, A, nullptr, nullptr);
 	stage_struct(tl, tc, ts);
-	for (uint32_t i = 0; i < nact; ++i) {
-		const uint32_t r0 = rc.clist[1 + i] * RC;
-		const bool rin = r0 + r < rc.R;
+	for (uint32_t i = 0; i < nch; ++i) {
+		const uint32_t q = p.q;
+		const bool rin = q != NOREP;
 #pragma unroll
 		for (int k = 0; k < PF_TX; ++k) {
 			const uint32_t e = threadIdx.x + TBLOCK * k;
@@ -325,51 +338,51 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
 		}
 		for (uint32_t e = threadIdx.x + TBLOCK * PF_TX; e / RC < tc.nt;
 		     e += TBLOCK)
-			lam0[e] = rin ? A[(size_t)(tc.tx_base + e / RC) * rc.R + r0 + r]
+			lam0[e] = rin ? A[(size_t)(tc.tx_base + e / RC) * rc.R + q]
 			              : 0.0;
-		store_counts<TC, RC, 1>(p, tc, rc, r0, cnt, uniq, n, u);
+		store_counts<TC, RC, 1>(p, tc, rc, q, cnt, uniq, n, u);
 		if (threadIdx.x < RC)
-			alive[threadIdx.x] = p.done ? 0 : 1;
+			alive[threadIdx.x] = rin ? 1 : 0;
 		__syncthreads();
-		if (i + 1 < nact)
-			load_rows<TC, RC, 1>(p, tc, rc, rc.clist[2 + i] * RC, cnt,
-			                     uniq, A, nullptr, nullptr);
+		if (i + 1 < nch)
+			load_rows<TC, RC, 1>(p, tc, rc, nrep, i + 1, cnt, uniq, A,
+			                     nullptr, nullptr);
 		tile_map<TC, RC>(tl, tc, ts, lam0, lam1, D, X, n, u, alive, B,
-		                 rc.R, r0, nullptr, 0, 0);
+		                 rc.R, q, nullptr, 0, 0);
 		tile_map<TC, RC>(tl, tc, ts, lam1, lam2, D, X, n, u, alive, C,
-		                 rc.R, r0, nullptr, 0, 0);
-		if (xr && alive[r]) {
+		                 rc.R, q, nullptr, 0, 0);
+		if (xr && rin) {
 			// r_t and v_t (src/EM.c:334-335) in replicate-major rows
 			// for the sequential norms
 			for (uint32_t t = e0; t < tc.nt; t += ES) {
 				const double x = lam1[t * RC + r] - lam0[t * RC + r];
 				const double y = (lam2[t * RC + r] - lam1[t * RC + r]) - x;
-				const size_t g = (size_t)(r0 + r) * T + ts.tx[t];
+				const size_t g = (size_t)q * T + ts.tx[t];
 				xr[g] = x;
 				xv[g] = y;
 			}
 		}
-		if (part && threadIdx.x < RC && alive[threadIdx.x]) {
+		if (part && threadIdx.x < RC && rin) {
 			// per (tile, replicate): local two-pass mean and centred
 			// sum of squares of r and v (combined across tiles by
 			// Chan's rule), or plain sums of squares for the L2 step
-			const int q = threadIdx.x;
+			const int lq = threadIdx.x;
 			double a0, a1, b0, b1;
 			if (step == 0) {
 				double sr = 0, sv = 0;
 				for (uint32_t t = 0; t < tc.nt; ++t) {
-					const double x = lam1[t * RC + q] - lam0[t * RC + q];
+					const double x = lam1[t * RC + lq] - lam0[t * RC + lq];
 					const double y =
-					    (lam2[t * RC + q] - lam1[t * RC + q]) - x;
+					    (lam2[t * RC + lq] - lam1[t * RC + lq]) - x;
 					sr += x;
 					sv += y;
 				}
 				const double mr = sr / tc.nt, mv = sv / tc.nt;
 				double qr = 0, qv = 0;
 				for (uint32_t t = 0; t < tc.nt; ++t) {
-					const double x = lam1[t * RC + q] - lam0[t * RC + q];
+					const double x = lam1[t * RC + lq] - lam0[t * RC + lq];
 					const double y =
-					    (lam2[t * RC + q] - lam1[t * RC + q]) - x;
+					    (lam2[t * RC + lq] - lam1[t * RC + lq]) - x;
 					qr += (x - mr) * (x - mr);
 					qv += (y - mv) * (y - mv);
 				}
@@ -380,9 +393,9 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
 			} else {
 				double qr = 0, qv = 0;
 				for (uint32_t t = 0; t < tc.nt; ++t) {
-					const double x = lam1[t * RC + q] - lam0[t * RC + q];
+					const double x = lam1[t * RC + lq] - lam0[t * RC + lq];
 					const double y =
-					    (lam2[t * RC + q] - lam1[t * RC + q]) - x;
+					    (lam2[t * RC + lq] - lam1[t * RC + lq]) - x;
 					qr += x * x;
 					qv += y * y;
 				}
@@ -391,7 +404,7 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
 				b0 = 0;
 				b1 = qv;
 			}
-			double *pp = part + ((size_t)tile * rc.R + r0 + q) * 4;
+			double *pp = part + ((size_t)tile * rc.R + q) * 4;
 			pp[0] = a0;
 			pp[1] = a1;
 			pp[2] = b0;
@@ -437,15 +450,15 @@ __device__ __forceinline__ double extrapolate(double a, double b, double c2,
 	return x;
 }
 
-template <typename TC, int RC>
+// The step length inputs of the lane's replicate (after load_rows).
+template <typename TC>
 __device__ __forceinline__ void load_alpha(ChunkRows<TC, 3> &p,
-                                           const hemk_rctl &rc, uint32_t r0,
-                                           hemk_step st)
+                                           const hemk_rctl &rc, hemk_step st)
 {
-	const uint32_t q = r0 + threadIdx.x % RC;
+	const uint32_t q = p.q;
 	p.qa = 0;
 	p.qb = 0;
-	if (q < rc.R) {
+	if (q != NOREP) {
 		if (st.alpha_given) {
 			p.qa = rc.alpha[q];
 		} else {
@@ -467,29 +480,29 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_ex3(
 	__shared__ int alive[RC];
 	__shared__ int stopf[RC];
 	const uint32_t tile = blockIdx.x;
-	const uint32_t nact = rc.clist[0];
-	if (nact == 0)
+	const uint32_t nrep = rc.alist[0];
+	if (nrep == 0)
 		return;
+	const uint32_t nch = (nrep + RC - 1) / RC;
 	const TileCtx tc = load_tile(tl, tile);
 	double *lam0, *lam1, *lam2, *D, *X;
 	TC *n;
 	uint32_t *u;
 	TileStruct ts;
 	carve<TC, RC>(smem, tl, lam0, lam1, lam2, D, X, n, u, ts);
-	const int r = threadIdx.x % RC;
 	ChunkRows<TC, 3> p;
-	load_rows<TC, RC, 3>(p, tc, rc, rc.clist[1] * RC, cnt, uniq, A, B, C);
-	load_alpha<TC, RC>(p, rc, rc.clist[1] * RC, st);
+	load_rows<TC, RC, 3>(p, tc, rc, nrep, 0, cnt, uniq, A, B, C);
+	load_alpha<TC>(p, rc, st);
 	stage_struct(tl, tc, ts);
-	for (uint32_t i = 0; i < nact; ++i) {
-		const uint32_t r0 = rc.clist[1 + i] * RC;
-		const bool rin = r0 + r < rc.R;
+	for (uint32_t i = 0; i < nch; ++i) {
+		const uint32_t q = p.q;
+		const bool rin = q != NOREP;
 		// sequential norms leave the two sums; the parallel ones alpha
 		const double alpha = st.alpha_given
 		                         ? p.qa
 		                         : step_len(p.qa, p.qb, T, st);
-		if (!st.alpha_given && tile == 0 && threadIdx.x < RC && !p.done)
-			rc.alpha[r0 + threadIdx.x] = alpha;
+		if (!st.alpha_given && tile == 0 && threadIdx.x < RC && rin)
+			rc.alpha[q] = alpha;
 #pragma unroll
 		for (int k = 0; k < PF_TX; ++k) {
 			const uint32_t e = threadIdx.x + TBLOCK * k;
@@ -501,27 +514,26 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_ex3(
 		     e += TBLOCK) {
 			double x = 0.0;
 			if (rin) {
-				const size_t g =
-				    (size_t)(tc.tx_base + e / RC) * rc.R + r0 + r;
+				const size_t g = (size_t)(tc.tx_base + e / RC) * rc.R + q;
 				x = extrapolate(A[g], B[g], C[g], alpha, st);
 			}
 			lam0[e] = x;
 		}
-		store_counts<TC, RC, 3>(p, tc, rc, r0, cnt, uniq, n, u);
+		store_counts<TC, RC, 3>(p, tc, rc, q, cnt, uniq, n, u);
 		if (threadIdx.x < RC) {
-			alive[threadIdx.x] = p.done ? 0 : 1;
+			alive[threadIdx.x] = rin ? 1 : 0;
 			stopf[threadIdx.x] = 0;
 		}
 		__syncthreads();
-		if (i + 1 < nact) {
-			load_rows<TC, RC, 3>(p, tc, rc, rc.clist[2 + i] * RC, cnt,
-			                     uniq, A, B, C);
-			load_alpha<TC, RC>(p, rc, rc.clist[2 + i] * RC, st);
+		if (i + 1 < nch) {
+			load_rows<TC, RC, 3>(p, tc, rc, nrep, i + 1, cnt, uniq, A, B,
+			                     C);
+			load_alpha<TC>(p, rc, st);
 		}
 		tile_map<TC, RC>(tl, tc, ts, lam0, lam1, D, X, n, u, alive, A,
-		                 rc.R, r0, stopf, rc.stop_min, rc.stop_rel);
-		if (threadIdx.x < RC && alive[threadIdx.x] && stopf[threadIdx.x])
-			rc.notconv[r0 + threadIdx.x] = 1;
+		                 rc.R, q, stopf, rc.stop_min, rc.stop_rel);
+		if (threadIdx.x < RC && rin && stopf[threadIdx.x])
+			rc.notconv[q] = 1;
 		__syncthreads();
 	}
 }
@@ -703,23 +715,17 @@ __global__ __launch_bounds__(TBLOCK) void k_batch_chain(
 	}
 }
 
-// clist = the chunks (of rc.chunk replicates) holding a replicate that is
-// not done, in ascending order, after clist[0] = their count.  One block;
-// done[] written by this block before the call.
-__device__ void build_chunk_list(const hemk_rctl &rc)
+// alist = the replicates not done, ascending, after alist[0] = their count;
+// the tile kernels deal them to chunks of rc.chunk lanes, so finished
+// replicates cost nothing.  One block; done[] written by this block before
+// the call.
+__device__ void build_active_list(const hemk_rctl &rc)
 {
 	__shared__ uint32_t sc[TBLOCK];
-	const uint32_t RC = rc.chunk;
-	const uint32_t nch = (rc.R + RC - 1) / RC;
 	uint32_t total = 0;
-	for (uint32_t base = 0; base < nch; base += TBLOCK) {
-		const uint32_t ch = base + threadIdx.x;
-		uint32_t f = 0;
-		if (ch < nch) {
-			const uint32_t r1 = min(rc.R, (ch + 1) * RC);
-			for (uint32_t r = ch * RC; r < r1; ++r)
-				f |= rc.done[r] ? 0u : 1u;
-		}
+	for (uint32_t base = 0; base < rc.R; base += TBLOCK) {
+		const uint32_t r = base + threadIdx.x;
+		const uint32_t f = (r < rc.R && !rc.done[r]) ? 1u : 0u;
 		sc[threadIdx.x] = f;
 		__syncthreads();
 		// inclusive scan of the flags
@@ -730,12 +736,12 @@ __device__ void build_chunk_list(const hemk_rctl &rc)
 			__syncthreads();
 		}
 		if (f)
-			rc.clist[1 + total + sc[threadIdx.x] - 1] = ch;
+			rc.alist[1 + total + sc[threadIdx.x] - 1] = r;
 		total += sc[TBLOCK - 1];
 		__syncthreads();
 	}
 	if (threadIdx.x == 0)
-		rc.clist[0] = total;
+		rc.alist[0] = total;
 }
 
 __global__ __launch_bounds__(TBLOCK) void k_batch_iter_end(hemk_rctl rc)
@@ -765,7 +771,7 @@ __global__ __launch_bounds__(TBLOCK) void k_batch_iter_end(hemk_rctl rc)
 	}
 	if (threadIdx.x == 0)
 		rc.active[0] = cnt[0];
-	build_chunk_list(rc);
+	build_active_list(rc);
 }
 
 __global__ __launch_bounds__(TBLOCK) void k_batch_reset(hemk_rctl rc,
@@ -791,7 +797,7 @@ __global__ __launch_bounds__(TBLOCK) void k_batch_reset(hemk_rctl rc,
 		if (threadIdx.x == 0)
 			rc.active[0] = run ? rc.R : 0;
 		__syncthreads();
-		build_chunk_list(rc);
+		build_active_list(rc);
 	}
 }
 

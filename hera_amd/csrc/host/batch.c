@@ -178,7 +178,7 @@ static hemk_rctl rctl_for(hem_problem *p, uint32_t R, const hem_opts *o)
 	rc.conv = w->u32 + 2 * (size_t)cap;
 	rc.notconv = w->u32 + 3 * (size_t)cap;
 	rc.active = w->u32 + 4 * (size_t)cap;
-	rc.clist = w->u32 + 4 * (size_t)cap + 1;
+	rc.alist = w->u32 + 4 * (size_t)cap + 1;
 	rc.alpha = w->f64;
 	rc.mean_r = w->f64 + cap;
 	rc.mean_v = w->f64 + 2 * (size_t)cap;

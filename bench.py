@@ -53,6 +53,11 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--stop-rel", type=float, default=1e-8,
                     help="relative-change stop threshold (hera's is 0.01)")
+    ap.add_argument("--min-iter", type=int, default=50,
+                    help="SQUAREM iterations always run (hera: 50)")
+    ap.add_argument("--max-iter", type=int, default=1000,
+                    help="iteration cap (hera: 1000); a smaller pair bounds "
+                         "profiler runs")
     ap.add_argument("--modes", default="fp64-exact,fp64-tree,fp32-tree")
     ap.add_argument("--cpu-solves", type=int, default=1,
                     help="full oracle solves timed for cpu_baseline (0: skip)")
@@ -62,6 +67,8 @@ def main():
     ap.add_argument("--cg-steps", type=int, default=50)
     ap.add_argument("--bootstrap-reps", type=int, default=100,
                     help="bootstrap replicates of the batched run (0: skip)")
+    ap.add_argument("--boot-modes", default="fp64-exact,fp64-tree",
+                    help="bootstrap modes to run")
     ap.add_argument("--seq-sample", type=int, default=3,
                     help="replicates solved one by one for the unbatched "
                          "comparison")
@@ -96,7 +103,8 @@ def main():
     t_upload = time.time() - t0
     log("uploaded class table in %.3fs" % t_upload)
 
-    sk = dict(stop_rel_change=args.stop_rel)
+    sk = dict(stop_rel_change=args.stop_rel, min_iter=args.min_iter,
+              max_iter=args.max_iter)
     results = {}
     for name, mk in modes.items():
         steps = args.steps if name == head_name else max(args.steps // 2, 1)
@@ -165,7 +173,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic_from_profile(dom, args),
             "bytes_per_launch": d["bytes_per_launch"],
             "ms_per_launch": d["ms_avg"],
         },
@@ -174,6 +182,10 @@ def main():
                              if prof[k]["launches"]},
         "upload_s": t_upload,
     }
+
+    out["copy_bandwidth_GBps"] = H.copy_bandwidth(1 << 30, 10)
+    log("device copy (1 GiB, read+write): %.0f GB/s"
+        % out["copy_bandwidth_GBps"])
 
     if args.grad_topk > 0:
         out["differentiation"] = bench_grad(args, H, np, p, cls, u, sk)
@@ -191,7 +203,9 @@ def main():
         k_cpu = 0
         ora = None
         for _ in range(n):
-            ora = O.estimate_count(cls, u, stop_rel=args.stop_rel)
+            ora = O.estimate_count(cls, u, stop_rel=args.stop_rel,
+                                   min_round=args.min_iter,
+                                   max_round=args.max_iter)
             k_cpu += ora["k"]
         dt_cpu = O.now() - t0
         out["cpu_baseline"] = {
@@ -207,7 +221,8 @@ def main():
         ll_o = O.loglik(cls, u, ora["lam"])
         par = {"k_oracle": ora["k"], "abund_threshold": "tpm/1e6 > 1e-7"}
         for name, mk in modes.items():
-            g = p.solve(u, min_iter=ora["k"], max_iter=ora["k"], **mk, **sk)
+            g = p.solve(u, **mk, **dict(sk, min_iter=ora["k"],
+                                        max_iter=ora["k"]))
             e = np.abs(g["tpm"][m] - ora["tpm"][m]) / ora["tpm"][m]
             par[name] = {
                 "k_gpu_stop_rule": results[name]["iters_per_solve"],
@@ -226,6 +241,24 @@ def main():
         boot.pop("_est_exact", None)
     p.close()
     print(json.dumps(out), flush=True)
+
+
+def traffic_from_profile(kind, args):
+    """HBM bytes per launch of the headline kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated) of the
+    default workload, or None for any other workload."""
+    if (args.n_tx, args.n_cls, args.n_frag, args.seed) != (
+            200000, 2000000, 60_000_000, 0):
+        return None
+    kernel = {"transcript_pass": "k_gather_exact"}.get(kind)
+    path = os.path.join(ROOT, "profiles",
+                        "em_kernels_200k_tx_2M_cls_counters.md")
+    if kernel is None or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch %s:" % kernel):
+            return float(line.split(":")[1])
+    return None
 
 
 def bench_grad(args, H, np, p, cls, u, sk):
@@ -289,7 +322,10 @@ def bench_bootstrap(args, H, np, p, cls, u, sk):
     R = args.bootstrap_reps
     seeds = np.arange(R, dtype=np.uint64) + 1
     res = {"n_rep": R, "seeds": "1..%d" % R}
-    for name, nm in (("fp64-exact", H.NORM_EXACT), ("fp64-tree", H.NORM_TREE)):
+    bmodes = [(n, m) for n, m in (("fp64-exact", H.NORM_EXACT),
+                                  ("fp64-tree", H.NORM_TREE))
+              if n in args.boot_modes.split(",")]
+    for name, nm in bmodes:
         p.bootstrap(u, seeds[:2], norm=nm, **sk)   # capture the graphs
         t = time.perf_counter()
         b = p.bootstrap(u, seeds, norm=nm, **sk)
@@ -307,10 +343,12 @@ def bench_bootstrap(args, H, np, p, cls, u, sk):
             "iters %d..%d" % (name, R, dt, R / dt, st["ms_resample"],
                               st["iters_min"], st["iters_max"]))
     # unbatched: the same resampled counts through the plain solve, one
-    # replicate after another (a sample of n_seq, scaled to R)
-    n_seq = max(1, min(args.seq_sample, R))
+    # replicate after another (a sample of n_seq, scaled to R; 0 skips it)
+    if args.seq_sample <= 0:
+        return res
+    n_seq = min(args.seq_sample, R)
     uo, co = p.bootstrap_counts(u, seeds[:n_seq])
-    for name, nm in (("fp64-exact", H.NORM_EXACT), ("fp64-tree", H.NORM_TREE)):
+    for name, nm in bmodes:
         t = time.perf_counter()
         for r in range(n_seq):
             p.set_counts(co[r])
@@ -333,7 +371,9 @@ def bootstrap_cpu(args, np, O, cls, u, boot):
     seeds = np.arange(nthr, dtype=np.uint64) + 1
     t0 = O.now()
     est, ks = O.bootstrap_resampled(cls, u, seeds, nthr,
-                                    stop_rel=args.stop_rel)
+                                    stop_rel=args.stop_rel,
+                                    min_round=args.min_iter,
+                                    max_round=args.max_iter)
     dt = O.now() - t0
     boot["cpu_baseline"] = {
         "value": nthr / dt, "unit": "bootstrap replicates/s",
@@ -341,12 +381,16 @@ def bootstrap_cpu(args, np, O, cls, u, boot):
         "sample": "one batch of %d replicates (seeds 1..%d) on %d threads: "
                   "resample + FP64 restatement of src/EM.c, %.1fs"
                   % (nthr, nthr, nthr, dt)}
-    gpu = boot.pop("_est_exact")
-    same = [bool(np.array_equal(gpu[r], est[r])) for r in range(nthr)]
-    boot["parity_exact_vs_oracle"] = {"replicates": nthr,
-                                      "est_bitwise_equal": all(same)}
+    gpu = boot.pop("_est_exact", None)
+    same = [gpu is not None and bool(np.array_equal(gpu[r], est[r]))
+            for r in range(nthr)]
+    if gpu is not None:
+        boot["parity_exact_vs_oracle"] = {"replicates": nthr,
+                                          "est_bitwise_equal": all(same)}
     for name in ("fp64-exact", "fp64-tree"):
-        boot[name]["speedup_vs_cpu"] = boot[name]["reps_per_s"] / (nthr / dt)
+        if name in boot:
+            boot[name]["speedup_vs_cpu"] = (boot[name]["reps_per_s"]
+                                            / (nthr / dt))
     log("bootstrap cpu: %d reps on %d threads %.1fs; exact bitwise %s" % (
         nthr, nthr, dt, all(same)))
 

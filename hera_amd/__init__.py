@@ -103,6 +103,14 @@ def generate(n_tx, n_cls_all, n_frag, seed=0, **overrides):
     return cls, unique
 
 
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """Device-to-device copy bandwidth, read + write GB/s
+    (hem_copy_bandwidth)."""
+    out = _lib.C.c_double()
+    check(_lib.load().hem_copy_bandwidth(nbytes, reps, out))
+    return out.value
+
+
 def opts(**kw):
     o = _lib.HemOpts()
     _lib.load().hem_opts_default(o)

@@ -89,6 +89,34 @@ static void ws_free_graphs(hem_batch_ws *w)
 	w->n_graph = 0;
 }
 
+/* Takes the cached batch graphs out of use without destroying them (they
+ * hold the old buffers' addresses; destruction waits for the handle's). */
+static int ws_retire_graphs(hem_problem *p)
+{
+	hem_batch_ws *w = &p->bw;
+	if (p->n_retired + w->n_graph > p->cap_retired) {
+		int cap = 2 * (p->n_retired + w->n_graph) + 8;
+		void **nr = realloc(p->retired, (size_t)cap * sizeof(void *));
+		if (!nr)
+			return hem_fail(HEM_ERR_NOMEM, "host allocation");
+		p->retired = nr;
+		p->cap_retired = cap;
+	}
+	for (int g = 0; g < w->n_graph; ++g)
+		p->retired[p->n_retired++] = w->graph[g];
+	w->n_graph = 0;
+	return HEM_OK;
+}
+
+void hem_batch_destroy_retired(hem_problem *p)
+{
+	for (int g = 0; g < p->n_retired; ++g)
+		hemd_graph_destroy(p->retired[g]);
+	free(p->retired);
+	p->retired = NULL;
+	p->n_retired = p->cap_retired = 0;
+}
+
 void hem_batch_free(hem_problem *p)
 {
 	hem_batch_ws *w = &p->bw;
@@ -123,6 +151,8 @@ static int ws_ensure(hem_problem *p, uint32_t R)
 	const size_t NT = p->dt.n_tiles ? p->dt.n_tiles : 1;
 	if (w->R >= R)
 		return HEM_OK;
+	if (ws_retire_graphs(p))
+		return HEM_ERR_NOMEM;
 	{
 		/* keep the tile arrays; drop the per-R buffers */
 		void *keep[9];
@@ -248,8 +278,8 @@ static int batch_graph(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 			return HEM_OK;
 		}
 	}
-	if (w->n_graph == HEM_GRAPH_CACHE)
-		ws_free_graphs(w);
+	if (w->n_graph == HEM_GRAPH_CACHE && (rc2 = ws_retire_graphs(p)))
+		return rc2;
 	e = hemd_graph_begin(p->stream);
 	if (e)
 		return hem_dev_fail(e, "batch graph capture");

@@ -53,3 +53,52 @@ int hem_device_ready(void)
 		                hemd_error_string(hem_init_status));
 	return HEM_OK;
 }
+
+/* Device-to-device copy bandwidth (read + write bytes per second), the
+ * achievable reference beside the HBM peak and the known-size kernel that
+ * scales rocprof's FETCH_SIZE / WRITE_SIZE counters. */
+int hem_copy_bandwidth(size_t bytes, int reps, double *gb_per_s)
+{
+	void *a = NULL, *b = NULL, *s = NULL, *e0 = NULL, *e1 = NULL;
+	float ms = 0;
+	int rc, e;
+	if (!gb_per_s || bytes == 0 || reps <= 0)
+		return hem_fail(HEM_ERR_INVALID, "bytes and reps must be > 0");
+	if ((rc = hem_device_ready()))
+		return rc;
+	e = hemd_stream_create(&s);
+	if (!e)
+		e = hemd_malloc(&a, bytes);
+	if (!e)
+		e = hemd_malloc(&b, bytes);
+	if (!e)
+		e = hemd_event_create(&e0);
+	if (!e)
+		e = hemd_event_create(&e1);
+	if (!e)
+		e = hemd_d2d(b, a, bytes, s);
+	if (!e)
+		e = hemd_event_record(e0, s);
+	for (int i = 0; !e && i < reps; ++i)
+		e = hemd_d2d(b, a, bytes, s);
+	if (!e)
+		e = hemd_event_record(e1, s);
+	if (!e)
+		e = hemd_event_sync(e1);
+	if (!e)
+		e = hemd_event_elapsed(e0, e1, &ms);
+	rc = e ? hem_dev_fail(e, "copy bandwidth") : HEM_OK;
+	if (!rc)
+		*gb_per_s = 2.0 * (double)bytes * reps / (ms * 1e6);
+	if (s)
+		hemd_stream_sync(s);
+	hemd_free(a);
+	hemd_free(b);
+	if (e0)
+		hemd_event_destroy(e0);
+	if (e1)
+		hemd_event_destroy(e1);
+	if (s)
+		hemd_stream_destroy(s);
+	return rc;
+}

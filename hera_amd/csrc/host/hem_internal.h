@@ -94,6 +94,11 @@ struct hem_problem {
 	hem_grad_ws gw;
 	void *ev2, *ev3;
 
+	/* batch graphs superseded by a workspace reallocation or a full cache:
+	 * never launched again, destroyed with the handle */
+	void **retired;
+	int n_retired, cap_retired;
+
 	/* captured iteration graphs and the options each was captured for;
 	 * kept until the handle is destroyed */
 	void *graph[HEM_GRAPH_CACHE];
@@ -132,6 +137,7 @@ enum { HEM_PK_DENOM = 0, HEM_PK_GATHER, HEM_PK_NORM1, HEM_PK_NORM2,
 
 /* batch.c */
 void hem_batch_free(hem_problem *p);
+void hem_batch_destroy_retired(hem_problem *p);
 /* grad.c */
 void hem_grad_free(hem_problem *p);
 int hem_tiles_upload(hem_problem *p, const hem_tiles_host *h);

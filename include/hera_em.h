@@ -78,7 +78,7 @@ typedef struct {
 #define HEM_CLAMP_DIV10 1
 
 typedef struct {
-	int      precision;       /* HEM_PREC_*; default FP32                  */
+	int      precision;       /* HEM_PREC_*; default FP64                  */
 	int      step;            /* HEM_STEP_*; default SD                    */
 	int      clamp;           /* HEM_CLAMP_*; default ZERO                 */
 	int      alpha_clamp;     /* 1: alpha = min(alpha, -1) (src/EM.c:346)  */
@@ -88,9 +88,11 @@ typedef struct {
 	double   stop_min_value;  /* 1e-3 (src/EM.c:292)                       */
 	double   stop_rel_change; /* 0.01 (src/EM.c:295); e.g. 1e-8 for a
 	                             tolerance-mode run                        */
-	int      fallback;        /* 1: monotonicity fallback to the plain EM
-	                             point when the extrapolated point lowers
-	                             the log-likelihood; 0 for parity runs     */
+	int      fallback;        /* must be 0: a monotonicity fallback (plain
+	                             EM point when the extrapolated point lowers
+	                             the log-likelihood) is not in this build,
+	                             and the reference has none; 1 is rejected
+	                             with HEM_ERR_INVALID                      */
 	int      verbose;         /* 1: print hera's "Finish EM" line          */
 } hem_opts;
 
@@ -214,6 +216,11 @@ typedef struct {
 
 int hem_profile_solve(hem_problem *p, const uint32_t *unique,
                       const hem_opts *o, hem_kernel_profile *out);
+
+/* Device-to-device copy of `bytes` repeated `reps` times on a fresh stream:
+ * read + write bytes per second.  No counterpart in the reference; the
+ * achievable-bandwidth reference for the kernels' figures. */
+int hem_copy_bandwidth(size_t bytes, int reps, double *gb_per_s);
 
 /* ---- adjoint of the log-likelihood ----------------------------------------
  * No counterpart in the reference, which has no derivatives; the objective

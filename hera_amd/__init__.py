@@ -23,6 +23,7 @@ CLAMP_ZERO = 0
 CLAMP_DIV10 = 1
 NORM_EXACT = 0
 NORM_TREE = 1
+NORM_SEQ = 2
 
 
 @dataclass
@@ -109,6 +110,25 @@ def copy_bandwidth(nbytes=1 << 30, reps=10):
     out = _lib.C.c_double()
     check(_lib.load().hem_copy_bandwidth(nbytes, reps, out))
     return out.value
+
+
+SUM_PLAIN = 0
+SUM_NORM_SD = 1
+SUM_SQUARES = 2
+
+
+def exact_sums(rows, mode=SUM_PLAIN):
+    """Left-to-right sums (or Norm, or sums of squares) of each row of a
+    2-D FP64 array on the device, bitwise the sequential loop
+    (hem_exact_sums).  Returns (out, reruns)."""
+    x = as_f64(rows)
+    if x.ndim == 1:
+        x = x[None, :]
+    out = np.empty(x.shape[0])
+    rr = _lib.C.c_uint32()
+    check(_lib.load().hem_exact_sums(f64p(x), x.shape[1], x.shape[0], mode,
+                                     f64p(out), rr))
+    return out, rr.value
 
 
 def opts(**kw):

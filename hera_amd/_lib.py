@@ -43,7 +43,8 @@ class HemStats(C.Structure):
     _fields_ = [("iters", C.c_uint32), ("maps", C.c_uint32),
                 ("converged", C.c_uint32), ("fallbacks", C.c_uint32),
                 ("loglik", C.c_double), ("last_alpha", C.c_double),
-                ("ms_solve", C.c_double), ("ms_total", C.c_double)]
+                ("ms_solve", C.c_double), ("ms_total", C.c_double),
+                ("sum_reruns", C.c_uint32)]
 
 
 HEM_KERNEL_KINDS = 6
@@ -108,6 +109,8 @@ SIGNATURES = {
     "hem_solve": (C.c_int, [C.c_void_p, _u32p, C.POINTER(HemOpts), _f64p,
                             _f64p, _f64p, C.POINTER(HemStats)]),
     "hem_loglik": (C.c_int, [C.c_void_p, _u32p, _f64p, _f64p]),
+    "hem_exact_sums": (C.c_int, [_f64p, C.c_uint32, C.c_uint32, C.c_int,
+                                 _f64p, _u32p]),
     "hem_profile_solve": (C.c_int, [C.c_void_p, _u32p, C.POINTER(HemOpts),
                                     C.POINTER(HemKernelProfile)]),
     "hem_problem_tiles": (C.c_int, [C.c_void_p, _u32p, _u32p, _u32p]),

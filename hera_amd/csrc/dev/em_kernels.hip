@@ -307,8 +307,10 @@ __global__ __launch_bounds__(BLOCK) void k_extrap(
 	if (ctl->done)
 		return;
 	double alpha;
-	if (st.alpha_given) {
+	if (st.alpha_given == 1) {
 		alpha = ctl->alpha;
+	} else if (st.alpha_given == 2) {
+		alpha = step_length(part[0], part[1], n, st);
 	} else {
 		double A, B;
 		reduce_parts(part, A, B);
@@ -329,7 +331,7 @@ __global__ __launch_bounds__(BLOCK) void k_extrap(
 		if (out32)
 			out32[i] = (float)x;
 	}
-	if (blockIdx.x == 0 && threadIdx.x == 0 && !st.alpha_given)
+	if (blockIdx.x == 0 && threadIdx.x == 0 && st.alpha_given != 1)
 		ctl->alpha = alpha;
 }
 
@@ -469,6 +471,7 @@ __global__ void k_ctl_reset(hemk_ctl *ctl, uint32_t min_iter,
 	ctl->maps = 0;
 	ctl->converged = 0;
 	ctl->fallbacks = 0;
+	ctl->reruns = 0;
 	ctl->min_iter = min_iter;
 	ctl->max_iter = max_iter;
 	ctl->alpha = 0;

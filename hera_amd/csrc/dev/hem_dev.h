@@ -51,7 +51,7 @@ typedef struct {
 	uint32_t converged;   /* stop rule fired                               */
 	uint32_t min_iter, max_iter;
 	uint32_t fallbacks;
-	uint32_t pad;
+	uint32_t reruns;      /* chunks of the exact sums run again (misses)   */
 	double   alpha;       /* last step length                              */
 	double   stop_min, stop_rel;
 	double   mean_r, mean_v;  /* first pass of the sequential norms        */
@@ -76,7 +76,9 @@ typedef struct {
 	int step;        /* 0 sample sd, 1 L2                                 */
 	int clamp;       /* 0 zero, 1 u2/10                                   */
 	int alpha_clamp; /* alpha <= -1                                       */
-	int alpha_given; /* 1: alpha already in ctl (sequential norms)        */
+	int alpha_given; /* 1: alpha already in ctl (sequential norms); 2:
+	                    from the two sums in part[0], part[1] (exact
+	                    sums by speculation)                           */
 } hemk_step;
 
 #define HEMK_NORM_BLOCKS 512
@@ -271,6 +273,48 @@ int hemk_extrapolate(uint32_t n, const double *u, const double *u1,
  *                 runs even when ctl->done is set. */
 int hemk_seq_norm(uint32_t n, const double *u, const double *u1,
                   const double *u2, int phase, hemk_step st, hemk_ctl *ctl,
+                  void *s);
+
+/* ---- exact left-to-right sums in parallel (spec_sum.hip) -----------------
+ * Chains of n FP64 values; each chain's result is bitwise the sequential
+ * sum ((+0 + y_0) + y_1) + ... of the reference's loops, computed by chunks
+ * of L values: guesses of every chunk's start, HEMK_SPEC_W candidate starts
+ * per chunk run in parallel, then an in-order lookup (a start outside its
+ * window reruns that chunk).
+ * Chain slot q (grid height nchain) sums row `row` of x (rows ld apart):
+ *   pairs == 0: row = rep(q);   pairs == 1: row = (q & 1) * R + rep(q / 2)
+ *   rep(i) = alist ? alist[1 + i] (slots with i >= alist[0] do nothing)
+ *            : i.
+ * y = x (IDENT), (x - mres[row] / n)^2 (CENTRED: the second pass of the
+ * sample sd), or x * x (SQUARE).  The sum goes to res[row]; reruns are
+ * added to *miss when miss != NULL.  With ctl != NULL every kernel returns
+ * at once when ctl->done is set.  Scratch, per chain slot: csum, drift,
+ * base [nchain][P]; tab [nchain][P][HEMK_SPEC_W]. */
+#define HEMK_SPEC_W 64
+#define HEMK_SPEC_LMAX 2048
+#define HEMK_SPEC_IDENT 0
+#define HEMK_SPEC_CENTRED 1
+#define HEMK_SPEC_SQUARE 2
+typedef struct {
+	uint32_t n, L, P;      /* values per chain, chunk length, chunks      */
+	uint32_t nchain;       /* chain slots (grid height)                   */
+	uint32_t R;            /* pairs: row offset of the second row block   */
+	int pairs, xf;
+	const double *x;       /* rows; written by the pass-1 producer        */
+	size_t ld;
+	const uint32_t *alist;
+	const double *mres;    /* CENTRED: pass-1 sums by row                 */
+	double *csum, *drift, *tab;
+	long long *base;
+	double *res;           /* sums by row                                 */
+	uint32_t *miss;
+} hemk_spec;
+/* Chunk length for chains of n values. */
+uint32_t hemk_spec_chunk(uint32_t n);
+/* produce == 1 (plain solve, nchain 2, rows 0/1 of x): first writes
+ * r = u1 - u and v = (u2 - u1) - r (src/EM.c:333-336) into x. */
+int hemk_spec_sum(const hemk_spec *sp, int produce, const double *u,
+                  const double *u1, const double *u2, hemk_ctl *ctl,
                   void *s);
 
 /* End of one iteration: k++, evaluate the loop condition of src/EM.c:328

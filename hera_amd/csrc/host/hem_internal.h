@@ -79,6 +79,11 @@ struct hem_problem {
 	void *d_rho;
 	uint32_t *d_unique, *d_flags;
 	double *d_part1, *d_part2, *d_out;
+	/* exact sums by speculation: r/v rows [2][T], scratch per chain
+	 * (two chains), and the sums (pass 1 at [0..1], pass 2 at [2..3]) */
+	uint32_t spec_L, spec_P;
+	double *d_sx, *d_scsum, *d_sdrift, *d_stab, *d_sres;
+	long long *d_sbase;
 	hemk_ctl *d_ctl;
 	hemk_ctl *h_ctl; /* pinned */
 	uint32_t n_flags;

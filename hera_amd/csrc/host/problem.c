@@ -180,6 +180,14 @@ int hem_problem_create(const hem_classes *c, hem_problem **out)
 	DALLOC_H(p->d_part1, HEMK_NORM_BLOCKS * 2 * 8);
 	DALLOC_H(p->d_part2, HEMK_NORM_BLOCKS * 2 * 8);
 	DALLOC_H(p->d_out, 64);
+	p->spec_L = hemk_spec_chunk(T);
+	p->spec_P = (T + p->spec_L - 1) / p->spec_L;
+	DALLOC_H(p->d_sx, 2 * (size_t)T * 8);
+	DALLOC_H(p->d_scsum, 2 * (size_t)p->spec_P * 8);
+	DALLOC_H(p->d_sdrift, 2 * (size_t)p->spec_P * 8);
+	DALLOC_H(p->d_sbase, 2 * (size_t)p->spec_P * 8);
+	DALLOC_H(p->d_stab, 2 * (size_t)p->spec_P * HEMK_SPEC_W * 8);
+	DALLOC_H(p->d_sres, 4 * 8);
 	DALLOC_H(p->d_ctl, sizeof(hemk_ctl));
 #undef DALLOC_H
 	rc = hemd_host_alloc((void **)&p->h_ctl, sizeof(hemk_ctl));
@@ -323,6 +331,12 @@ int hem_problem_destroy(hem_problem *p)
 	hemd_free(p->d_part1);
 	hemd_free(p->d_part2);
 	hemd_free(p->d_out);
+	hemd_free(p->d_sx);
+	hemd_free(p->d_scsum);
+	hemd_free(p->d_sdrift);
+	hemd_free(p->d_sbase);
+	hemd_free(p->d_stab);
+	hemd_free(p->d_sres);
 	hemd_free(p->d_ctl);
 	hemd_host_free(p->h_ctl);
 	if (p->ev0)

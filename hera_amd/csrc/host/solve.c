@@ -9,6 +9,7 @@
  *   A = u (iterate)   B = u1 = M(A)   C = u2 = M(B)
  *   D = extrapolated, clamped point   A <- M(D)  (stop test on this map)
  */
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -52,7 +53,8 @@ static int check_opts(const hem_opts *o)
 	if (o->clamp != HEM_CLAMP_ZERO && o->clamp != HEM_CLAMP_DIV10)
 		return hem_fail(HEM_ERR_INVALID, "unknown clamp form %d",
 		                o->clamp);
-	if (o->norm != HEM_NORM_EXACT && o->norm != HEM_NORM_TREE)
+	if (o->norm != HEM_NORM_EXACT && o->norm != HEM_NORM_TREE &&
+	    o->norm != HEM_NORM_SEQ)
 		return hem_fail(HEM_ERR_INVALID, "unknown norm mode %d", o->norm);
 	if (o->fallback)
 		return hem_fail(HEM_ERR_INVALID,
@@ -65,7 +67,7 @@ static int graph_key(const hem_problem *p, const hem_opts *o)
 {
 	return o->precision | (o->step << 1) | (o->clamp << 2) |
 	       ((o->alpha_clamp != 0) << 3) | (o->norm << 4) |
-	       (p->group_cls << 5) | (p->group_tx << 13);
+	       (p->group_cls << 6) | (p->group_tx << 14);
 }
 
 /* Optional per-kernel timing: an event is recorded after every launch. */
@@ -84,6 +86,32 @@ typedef struct {
 		}                                                       \
 	} while (0)
 
+/* The exact-sum chains of the plain solve: rows of x (r/v rows, or the
+ * state for the finalisation sum), results into res[row]. */
+static hemk_spec spec_plain(const hem_problem *p, uint32_t nchain, int xf,
+                            const double *x, const double *mres,
+                            double *res)
+{
+	hemk_spec sp;
+	memset(&sp, 0, sizeof(sp));
+	sp.n = p->n_tx;
+	sp.L = p->spec_L;
+	sp.P = p->spec_P;
+	sp.nchain = nchain;
+	sp.R = 1;
+	sp.xf = xf;
+	sp.x = x;
+	sp.ld = p->n_tx;
+	sp.mres = mres;
+	sp.csum = p->d_scsum;
+	sp.drift = p->d_sdrift;
+	sp.tab = p->d_stab;
+	sp.base = p->d_sbase;
+	sp.res = res;
+	sp.miss = &p->d_ctl->reruns;
+	return sp;
+}
+
 /* Enqueue one SQUAREM iteration on the problem's stream. */
 static int enqueue_iteration(hem_problem *p, const hem_opts *o, prof_ctx *pc)
 {
@@ -94,7 +122,11 @@ static int enqueue_iteration(hem_problem *p, const hem_opts *o, prof_ctx *pc)
 	const void *gD = f32 ? (const void *)p->d_D32 : (const void *)p->d_D;
 	void *rho = f32 ? p->d_rho : (void *)p->d_nd;
 	const int exact = o->norm == HEM_NORM_EXACT;
-	hemk_step st = {o->step, o->clamp, o->alpha_clamp != 0, exact};
+	const int seq = o->norm == HEM_NORM_SEQ;
+	hemk_step st = {o->step, o->clamp, o->alpha_clamp != 0,
+	                seq ? 1 : exact ? 2 : 0};
+	const double *alpha_part = o->step == HEM_STEP_SD ? p->d_part2
+	                                                   : p->d_part1;
 	const hemk_prob *dp = &p->dp;
 	void *s = p->stream;
 	uint32_t nfl = hemk_gather_blocks(dp, p->group_tx);
@@ -121,8 +153,30 @@ static int enqueue_iteration(hem_problem *p, const hem_opts *o, prof_ctx *pc)
 	            "gather");
 	MARK(HEM_PK_GATHER);
 	if (exact) {
-		/* the reference's left-to-right sums: means then centred
-		 * sums for the sd form, one pass for the L2 form */
+		/* the reference's left-to-right sums, by speculation: r and v
+		 * rows with their sums (or sums of squares, L2 form), then
+		 * the centred sums of the sd form; the extrapolation takes
+		 * alpha from the last pair */
+		const int sd = o->step == HEM_STEP_SD;
+		hemk_spec sp = spec_plain(p, 2,
+		                          sd ? HEMK_SPEC_IDENT : HEMK_SPEC_SQUARE,
+		                          p->d_sx, NULL, p->d_sres);
+		HEM_TRY_DEV(hemk_spec_sum(&sp, 1, p->d_A, p->d_B, p->d_C,
+		                          p->d_ctl, s),
+		            "exact sums 1");
+		MARK(HEM_PK_NORM1);
+		alpha_part = p->d_sres;
+		if (sd) {
+			sp = spec_plain(p, 2, HEMK_SPEC_CENTRED, p->d_sx,
+			                p->d_sres, p->d_sres + 2);
+			HEM_TRY_DEV(hemk_spec_sum(&sp, 0, NULL, NULL, NULL,
+			                          p->d_ctl, s),
+			            "exact sums 2");
+			MARK(HEM_PK_NORM2);
+			alpha_part = p->d_sres + 2;
+		}
+	} else if (seq) {
+		/* the same sums on one lane each */
 		if (o->step == HEM_STEP_SD) {
 			HEM_TRY_DEV(hemk_seq_norm(p->n_tx, p->d_A, p->d_B,
 			                          p->d_C, 1, st, p->d_ctl, s),
@@ -152,10 +206,8 @@ static int enqueue_iteration(hem_problem *p, const hem_opts *o, prof_ctx *pc)
 	}
 	}
 	HEM_TRY_DEV(hemk_extrapolate(p->n_tx, p->d_A, p->d_B, p->d_C,
-	                             o->step == HEM_STEP_SD ? p->d_part2
-	                                                    : p->d_part1,
-	                             p->d_D, f32 ? p->d_D32 : NULL, st,
-	                             p->d_ctl, s),
+	                             alpha_part, p->d_D,
+	                             f32 ? p->d_D32 : NULL, st, p->d_ctl, s),
 	            "extrapolate");
 	MARK(HEM_PK_EXTRAP);
 	HEM_TRY_DEV(hemk_denom(dp, prec, gD, rho, gc, p->d_ctl, s),
@@ -282,6 +334,14 @@ int hem_solve(hem_problem *p, const uint32_t *unique, const hem_opts *opts,
 		goto out;
 	/* finalisation into B (est) and C (tpm); log-likelihood of A */
 	if (o.norm == HEM_NORM_EXACT) {
+		/* sum of the final state (src/EM.c:362-363) into lam_sum */
+		hemk_spec sp = spec_plain(p, 1, HEMK_SPEC_IDENT, p->d_A, NULL,
+		                          &p->d_ctl->lam_sum);
+		rc = hemk_spec_sum(&sp, 0, NULL, NULL, NULL, NULL, p->stream);
+		if (!rc)
+			rc = hemk_finalize_seq(p->n_tx, p->d_A, p->d_len, p->d_ctl,
+			                       p->d_C, p->d_B, p->stream);
+	} else if (o.norm == HEM_NORM_SEQ) {
 		hemk_step st0 = {0, 0, 0, 0};
 		rc = hemk_seq_norm(p->n_tx, p->d_A, NULL, NULL, 0, st0,
 		                   p->d_ctl, p->stream);
@@ -317,6 +377,7 @@ int hem_solve(hem_problem *p, const uint32_t *unique, const hem_opts *opts,
 		st->maps = p->h_ctl->maps;
 		st->converged = p->h_ctl->converged;
 		st->fallbacks = p->h_ctl->fallbacks;
+		st->sum_reruns = p->h_ctl->reruns;
 		st->last_alpha = p->h_ctl->alpha;
 		st->loglik = ll;
 		st->ms_solve = ms;
@@ -446,4 +507,89 @@ int hem_loglik(hem_problem *p, const uint32_t *unique, const double *lambda,
 		rc = hemd_stream_sync(p->stream);
 	pthread_mutex_unlock(&p->mu);
 	return rc ? hem_dev_fail(rc, "log-likelihood") : HEM_OK;
+}
+
+/* Exact left-to-right sums of host rows on the device (the reductions of
+ * Norm, src/hash_align.c:150-164, and src/EM.c:362-363). */
+int hem_exact_sums(const double *x, uint32_t n, uint32_t n_rows, int mode,
+                   double *out, uint32_t *reruns)
+{
+	int rc, e = 0;
+	void *s = NULL;
+	double *d_x = NULL, *d_sc = NULL, *d_tab = NULL, *d_res = NULL;
+	long long *d_base = NULL;
+	uint32_t *d_miss = NULL, miss = 0;
+	hemk_spec sp;
+	if (!x || !out || n == 0 || n_rows == 0)
+		return hem_fail(HEM_ERR_INVALID, "null rows or empty size");
+	if (mode < HEM_SUM_PLAIN || mode > HEM_SUM_SQUARES)
+		return hem_fail(HEM_ERR_INVALID, "unknown sum mode %d", mode);
+	if (n_rows > 65535 || n > 0xFFFFFFFFu - HEMK_SPEC_LMAX)
+		return hem_fail(HEM_ERR_RANGE, "at most 65535 rows of < 2^32 "
+		                "values");
+	if ((rc = hem_device_ready()))
+		return rc;
+	memset(&sp, 0, sizeof(sp));
+	sp.n = n;
+	sp.L = hemk_spec_chunk(n);
+	sp.P = (n + sp.L - 1) / sp.L;
+	sp.nchain = n_rows;
+	sp.R = 1;
+	sp.ld = n;
+	const size_t cp = (size_t)n_rows * sp.P;
+	if (!e) e = hemd_stream_create(&s);
+	if (!e) e = hemd_malloc((void **)&d_x, (size_t)n_rows * n * 8);
+	if (!e) e = hemd_malloc((void **)&d_sc, cp * 2 * 8);
+	if (!e) e = hemd_malloc((void **)&d_base, cp * 8);
+	if (!e) e = hemd_malloc((void **)&d_tab, cp * HEMK_SPEC_W * 8);
+	if (!e) e = hemd_malloc((void **)&d_res, (size_t)n_rows * 2 * 8);
+	if (!e) e = hemd_malloc((void **)&d_miss, 4);
+	if (e) {
+		rc = hem_fail(HEM_ERR_NOMEM, "device allocation: %s",
+		              hemd_error_string(e));
+		goto out;
+	}
+	sp.x = d_x;
+	sp.csum = d_sc;
+	sp.drift = d_sc + cp;
+	sp.tab = d_tab;
+	sp.base = d_base;
+	sp.miss = d_miss;
+	sp.res = d_res;
+	sp.xf = mode == HEM_SUM_SQUARES ? HEMK_SPEC_SQUARE : HEMK_SPEC_IDENT;
+	e = hemd_h2d(d_x, x, (size_t)n_rows * n * 8, s);
+	if (!e) e = hemk_fill_u32(d_miss, 1, 0, s);
+	if (!e) e = hemk_spec_sum(&sp, 0, NULL, NULL, NULL, NULL, s);
+	if (!e && mode == HEM_SUM_NORM_SD) {
+		/* second pass of Norm: centred squares about sum / n */
+		sp.xf = HEMK_SPEC_CENTRED;
+		sp.mres = d_res;
+		sp.res = d_res + n_rows;
+		e = hemk_spec_sum(&sp, 0, NULL, NULL, NULL, NULL, s);
+	}
+	if (!e) e = hemd_d2h(out, mode == HEM_SUM_NORM_SD ? d_res + n_rows
+	                                                 : d_res,
+	                     (size_t)n_rows * 8, s);
+	if (!e) e = hemd_d2h(&miss, d_miss, 4, s);
+	if (!e) e = hemd_stream_sync(s);
+	if (e) {
+		rc = hem_dev_fail(e, "exact sums");
+		goto out;
+	}
+	if (mode == HEM_SUM_NORM_SD)
+		for (uint32_t r = 0; r < n_rows; ++r)
+			out[r] = sqrt(out[r] / (n - 1));
+	if (reruns)
+		*reruns = miss;
+	rc = HEM_OK;
+out:
+	hemd_free(d_x);
+	hemd_free(d_sc);
+	hemd_free(d_base);
+	hemd_free(d_tab);
+	hemd_free(d_res);
+	hemd_free(d_miss);
+	if (s)
+		hemd_stream_destroy(s);
+	return rc;
 }

@@ -65,11 +65,15 @@ typedef struct {
 
 /* SQUAREM norm reductions.  EXACT: the left-to-right sums of Norm
  * (src/hash_align.c:150-164) and of the finalisation (src/EM.c:362-363),
- * each run by one device lane, so with HEM_PREC_FP64 every iterate is
- * bitwise the reference's.  TREE: fixed-shape parallel sums (bitwise
- * reproducible run to run, ~1 ulp from the reference's). */
+ * bitwise, computed in parallel chunks from speculated start values with
+ * an in-order lookup (a chunk whose start falls outside its window is
+ * added again from the exact start), so with HEM_PREC_FP64 every iterate
+ * is bitwise the reference's.  SEQ: the same sums, each run by one device
+ * lane (slow; kept as the plain restatement).  TREE: fixed-shape parallel
+ * sums (bitwise reproducible run to run, ~1 ulp from the reference's). */
 #define HEM_NORM_EXACT 0
 #define HEM_NORM_TREE  1
+#define HEM_NORM_SEQ   2
 
 /* Treatment of extrapolated values.  ZERO: negatives become 0
  * (src/EM.c:351).  DIV10: values <= 1e-300 become u2/10
@@ -108,6 +112,8 @@ typedef struct {
 	double   last_alpha;   /* step length of the last iteration          */
 	double   ms_solve;     /* device time of the iteration loop          */
 	double   ms_total;     /* host wall time of the call                 */
+	uint32_t sum_reruns;   /* HEM_NORM_EXACT: chunks of the exact sums
+	                          added again from their exact start          */
 } hem_stats;
 
 /* ---- device-resident problem ------------------------------------------- */
@@ -221,6 +227,21 @@ int hem_profile_solve(hem_problem *p, const uint32_t *unique,
  * read + write bytes per second.  No counterpart in the reference; the
  * achievable-bandwidth reference for the kernels' figures. */
 int hem_copy_bandwidth(size_t bytes, int reps, double *gb_per_s);
+
+/* ---- exact left-to-right sums ---------------------------------------------
+ * The reductions on the reference's path, bitwise, over n_rows rows of n
+ * values (x[r*n + i], host memory): HEM_SUM_PLAIN out[r] = the sum
+ * ((+0 + x_0) + x_1) + ... in index order (src/EM.c:362-363); HEM_SUM_NORM_SD
+ * out[r] = Norm(x_r) (src/hash_align.c:150-164: mean, centred sum of
+ * squares, / (n-1), sqrt); HEM_SUM_SQUARES out[r] = the same ordered sum of
+ * x_i * x_i.  Computed in parallel chunks (see HEM_NORM_EXACT); reruns, if
+ * not NULL, receives the number of chunks added again from their exact
+ * start.  No counterpart in the reference as an entry point. */
+#define HEM_SUM_PLAIN   0
+#define HEM_SUM_NORM_SD 1
+#define HEM_SUM_SQUARES 2
+int hem_exact_sums(const double *x, uint32_t n, uint32_t n_rows, int mode,
+                   double *out, uint32_t *reruns);
 
 /* ---- adjoint of the log-likelihood ----------------------------------------
  * No counterpart in the reference, which has no derivatives; the objective

@@ -1,0 +1,133 @@
+"""Exact left-to-right sums on the device (hem_exact_sums; the reductions of
+Norm, src/hash_align.c:150-164, and of the finalisation, src/EM.c:362-363).
+
+The device computes them in parallel chunks from speculated start values
+with an in-order lookup; the bar is bitwise equality with the sequential
+loop, (((+0 + x0) + x1) + ...), for every input, including inputs built to
+defeat the speculation (partial sums that return to zero, huge dynamic
+range, denormals, signed zeros, infinities).  The expected values come
+from numpy's sequential cumulative sum (np.add.accumulate adds strictly left
+to right) and from the oracle's restatement of Norm.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import hera_amd as H
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def seq_sum(x):
+    return np.cumsum(np.concatenate([[0.0], x]))[-1]
+
+
+def same_bits(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    both_nan = np.isnan(a) & np.isnan(b)
+    return bool(np.all(both_nan | (a.view(np.int64) == b.view(np.int64))))
+
+
+def cases():
+    rng = np.random.default_rng(7)
+    out = {}
+    for n in (1, 2, 3, 63, 64, 255, 256, 257, 1023, 1024, 1025, 5000,
+              200000):
+        out["normal_%d" % n] = rng.normal(size=(3, n))
+    out["lognormal_wide"] = rng.lognormal(0, 12, size=(2, 100000))
+    big = rng.lognormal(0, 3, size=50000)
+    canc = np.concatenate([big, -rng.permutation(big)]) + \
+        rng.normal(scale=1e-9, size=100000)
+    out["return_to_zero"] = np.stack([canc, canc[::-1]])
+    out["alternating"] = (np.where(np.arange(60000) % 2 == 0, 1.0, -1.0) *
+                          rng.lognormal(0, 1, 60000))[None, :]
+    out["denormal"] = (rng.normal(size=(2, 30000)) * 1e-310)
+    z = np.zeros((2, 5000))
+    z[0, ::3] = -0.0
+    z[1, :] = -0.0
+    out["signed_zeros"] = z
+    steps = rng.normal(size=70000)
+    steps[35000:] *= 1e-12
+    out["absorbed_tail"] = np.stack([steps, np.abs(steps)])
+    inf = rng.normal(size=(3, 4000))
+    inf[0, 1500] = np.inf
+    inf[1, 100] = np.inf
+    inf[1, 3000] = -np.inf
+    inf[2, 2500] = np.nan
+    out["nonfinite"] = inf
+    return out
+
+
+@pytest.mark.parametrize("name", sorted(cases().keys()))
+def test_exact_sums_bitwise(name):
+    x = cases()[name]
+    got, reruns = H.exact_sums(x, H.SUM_PLAIN)
+    exp = np.array([seq_sum(r) for r in x])
+    sq, _ = H.exact_sums(x, H.SUM_SQUARES)
+    exp_sq = np.array([seq_sum(r * r) for r in x])
+    say("%-16s n=%6d rows=%d reruns=%d" % (name, x.shape[1], x.shape[0],
+                                          reruns))
+    assert same_bits(got, exp), (got, exp)
+    assert same_bits(sq, exp_sq), (sq, exp_sq)
+
+
+@pytest.mark.parametrize("name", ["normal_1025", "normal_200000",
+                                  "lognormal_wide", "return_to_zero",
+                                  "alternating", "denormal"])
+def test_norm_sd_bitwise_oracle(name):
+    x = cases()[name]
+    got, _ = H.exact_sums(x, H.SUM_NORM_SD)
+    exp = np.array([O.norm_sd(r) for r in x])
+    assert same_bits(got, exp), (got, exp)
+
+
+def test_norm_sd_on_squarem_vectors(cfg1):
+    """r = u1 - u, v = (u2 - u1) - r along the reference trajectory: the
+    vectors whose norms set every step length."""
+    cls, u = cfg1
+    rows = []
+    for k in (0, 3, 20, 60):
+        lam = (np.full(cls.n_tx, 1.0 / cls.n_tx) if k == 0 else
+               O.estimate_count(cls, u, min_round=k, max_round=k)["lam"])
+        u1, _ = O.em_map(cls, u, lam)
+        u2, _ = O.em_map(cls, u, u1)
+        r = u1 - lam
+        rows += [r, (u2 - u1) - r]
+    x = np.stack(rows)
+    got, reruns = H.exact_sums(x, H.SUM_NORM_SD)
+    exp = np.array([O.norm_sd(r) for r in x])
+    say("squarem vectors: reruns=%d over %d rows" % (reruns, len(rows)))
+    assert same_bits(got, exp)
+
+
+def test_solve_exact_equals_single_lane(cfg1):
+    """The speculated sums and the single-lane chains give the same
+    solve, bit for bit, under both step forms."""
+    cls, u = cfg1
+    p = H.Problem(cls)
+    try:
+        for step in (H.STEP_SD, H.STEP_L2):
+            a = p.solve(u, norm=H.NORM_EXACT, step=step, max_iter=120)
+            b = p.solve(u, norm=H.NORM_SEQ, step=step, max_iter=120)
+            say("step=%d k=%d reruns=%d" % (step, a["stats"]["iters"],
+                                             a["stats"]["sum_reruns"]))
+            assert a["stats"]["iters"] == b["stats"]["iters"]
+            np.testing.assert_array_equal(a["lam"], b["lam"])
+            np.testing.assert_array_equal(a["tpm"], b["tpm"])
+    finally:
+        p.close()
+
+
+def test_exact_sums_rejects_bad_arguments():
+    with pytest.raises(H.HemError):
+        H.exact_sums(np.zeros((1, 4)), 7)
+    with pytest.raises(H.HemError):
+        H.exact_sums(np.zeros((1, 0)))

@@ -64,7 +64,7 @@ class HemBatchStats(C.Structure):
                 ("iters_max", C.c_uint32), ("n_converged", C.c_uint32),
                 ("chunk", C.c_uint32), ("n_tiles", C.c_uint32),
                 ("ms_resample", C.c_double), ("ms_solve", C.c_double),
-                ("ms_total", C.c_double)]
+                ("ms_total", C.c_double), ("sum_reruns", C.c_uint32)]
 
 
 class HemGradStats(C.Structure):

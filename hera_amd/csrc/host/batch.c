@@ -135,6 +135,9 @@ void hem_batch_free(hem_problem *p)
 	hemd_free(w->seeds);
 	hemd_free(w->u32);
 	hemd_free(w->f64);
+	hemd_free(w->sc);
+	hemd_free(w->sbase);
+	hemd_free(w->stab);
 	hemd_host_free(w->h_active);
 	memset(w, 0, sizeof(*w));
 	for (int i = 0; i < 9; ++i) {
@@ -183,8 +186,11 @@ static int ws_ensure(hem_problem *p, uint32_t R)
 	WALLOC(w->m, (size_t)R * HEMK_RS_MAXB * 4);
 	WALLOC(w->rs_plan, (K + 2 * ((size_t)HEMK_RS_MAXB + 1)) * 4);
 	WALLOC(w->seeds, (size_t)R * 8);
-	WALLOC(w->u32, ((size_t)5 * R + 2) * 4);
-	WALLOC(w->f64, (size_t)6 * R * 8);
+	WALLOC(w->u32, ((size_t)5 * R + 3) * 4);
+	WALLOC(w->f64, (size_t)8 * R * 8);
+	WALLOC(w->sc, (size_t)4 * R * p->spec_P * 8);
+	WALLOC(w->sbase, (size_t)2 * R * p->spec_P * 8);
+	WALLOC(w->stab, (size_t)2 * R * p->spec_P * HEMK_SPEC_W * 8);
 #undef WALLOC
 	if (hemd_host_alloc((void **)&w->h_active, 64))
 		return hem_fail(HEM_ERR_NOMEM, "pinned allocation");
@@ -213,9 +219,41 @@ static hemk_rctl rctl_for(hem_problem *p, uint32_t R, const hem_opts *o)
 	rc.mean_r = w->f64 + cap;
 	rc.mean_v = w->f64 + 2 * (size_t)cap;
 	rc.lam_sum = w->f64 + 3 * (size_t)cap;
+	/* q_v follows q_r at distance R: the exact sums write both as one
+	 * array of 2R rows */
 	rc.q_r = w->f64 + 4 * (size_t)cap;
-	rc.q_v = w->f64 + 5 * (size_t)cap;
+	rc.q_v = rc.q_r + R;
 	return rc;
+}
+
+/* Exact-sum chains over the batch's rows.  pairs: r and v rows of the
+ * replicates in alist (X[rep] and X[R + rep]); otherwise one row per
+ * replicate (every replicate, no alist). */
+static hemk_spec spec_batch(hem_problem *p, hemk_rctl rc, int pairs, int xf,
+                            const double *mres, double *res)
+{
+	hem_batch_ws *w = &p->bw;
+	const size_t cap = w->R;
+	hemk_spec sp;
+	memset(&sp, 0, sizeof(sp));
+	sp.n = p->n_tx;
+	sp.L = p->spec_L;
+	sp.P = p->spec_P;
+	sp.nchain = pairs ? 2 * rc.R : rc.R;
+	sp.R = rc.R;
+	sp.pairs = pairs;
+	sp.xf = xf;
+	sp.x = w->X;
+	sp.ld = p->n_tx;
+	sp.alist = pairs ? rc.alist : NULL;
+	sp.mres = mres;
+	sp.csum = w->sc;
+	sp.drift = w->sc + 2 * cap * p->spec_P;
+	sp.tab = w->stab;
+	sp.base = w->sbase;
+	sp.res = res;
+	sp.miss = w->u32 + 5 * cap + 2;
+	return sp;
 }
 
 /* ---- the iteration ------------------------------------------------------ */
@@ -225,8 +263,9 @@ static int batch_iteration(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 {
 	hem_batch_ws *w = &p->bw;
 	void *s = p->stream;
-	const int exact = o->norm == HEM_NORM_EXACT;
-	/* exact: alpha from the chains' sums inside ex3; tree: alpha given */
+	const int spec = o->norm == HEM_NORM_EXACT;
+	const int exact = spec || o->norm == HEM_NORM_SEQ;
+	/* exact: alpha from the sums inside ex3; tree: alpha given */
 	hemk_step st = {o->step, o->clamp, o->alpha_clamp != 0, !exact};
 	const size_t TR = (size_t)p->n_tx * rc.R;
 	double *xr = exact ? w->X : NULL, *xv = exact ? w->X + TR : NULL;
@@ -234,7 +273,25 @@ static int batch_iteration(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 	                            w->B, w->C, exact ? NULL : w->part, xr, xv,
 	                            p->n_tx, o->step, s),
 	            "tile map12");
-	if (exact) {
+	if (spec) {
+		/* the reference's left-to-right sums by speculation: pass 1
+		 * sums (sd) or sums of squares (L2, straight into q_r/q_v),
+		 * then the centred sums of the sd form into q_r/q_v */
+		const int sd = o->step == HEM_STEP_SD;
+		double *sums1 = p->bw.f64 + 6 * (size_t)p->bw.R;
+		hemk_spec sp = spec_batch(p, rc, 1,
+		                          sd ? HEMK_SPEC_IDENT : HEMK_SPEC_SQUARE,
+		                          NULL, sd ? sums1 : rc.q_r);
+		HEM_TRY_DEV(hemk_spec_sum(&sp, 0, NULL, NULL, NULL, NULL, s),
+		            "batch exact sums 1");
+		if (sd) {
+			sp = spec_batch(p, rc, 1, HEMK_SPEC_CENTRED, sums1,
+			                rc.q_r);
+			HEM_TRY_DEV(hemk_spec_sum(&sp, 0, NULL, NULL, NULL, NULL,
+			                          s),
+			            "batch exact sums 2");
+		}
+	} else if (exact) {
 		if (o->step == HEM_STEP_SD) {
 			HEM_TRY_DEV(hemk_batch_chain(rc, p->n_tx, xr, xv, 1, s),
 			            "batch chain 1");
@@ -258,7 +315,7 @@ static int batch_iteration(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 static int batch_graph_key(const hem_opts *o, int chunk)
 {
 	return o->step | (o->clamp << 1) | ((o->alpha_clamp != 0) << 2) |
-	       (o->norm << 3) | (chunk << 4);
+	       (o->norm << 3) | (chunk << 5);
 }
 
 static int batch_graph(hem_problem *p, const hem_opts *o, hemk_rctl rc,
@@ -313,7 +370,8 @@ static int check_batch_opts(const hem_problem *p, const hem_opts *o)
 	if (o->clamp != HEM_CLAMP_ZERO && o->clamp != HEM_CLAMP_DIV10)
 		return hem_fail(HEM_ERR_INVALID, "unknown clamp form %d",
 		                o->clamp);
-	if (o->norm != HEM_NORM_EXACT && o->norm != HEM_NORM_TREE)
+	if (o->norm != HEM_NORM_EXACT && o->norm != HEM_NORM_TREE &&
+	    o->norm != HEM_NORM_SEQ)
 		return hem_fail(HEM_ERR_INVALID, "unknown norm mode %d", o->norm);
 	if (o->fallback)
 		return hem_fail(HEM_ERR_INVALID, "the monotonicity fallback is "
@@ -365,6 +423,9 @@ static int run_batch(hem_problem *p, const hem_opts *o, uint32_t R,
 		return hem_fail(HEM_ERR_RANGE, "tiles do not fit the LDS budget");
 	rc.chunk = (uint32_t)chunk;
 	HEM_TRY_DEV(hemk_batch_reset(rc, p->n_tx, w->A, p->stream), "reset");
+	HEM_TRY_DEV(hemk_fill_u32(w->u32 + 5 * (size_t)w->R + 2, 1, 0,
+	                          p->stream),
+	            "rerun counter");
 	if ((rc2 = batch_graph(p, o, rc, chunk, &graph)))
 		return rc2;
 	HEM_TRY_DEV(hemd_event_record(p->ev0, p->stream), "event");
@@ -389,8 +450,17 @@ static int run_batch(hem_problem *p, const hem_opts *o, uint32_t R,
 	HEM_TRY_DEV(hemk_batch_rows(rc, p->n_tx, p->dt.txq, w->A, w->X,
 	                            p->stream),
 	            "state rows");
-	HEM_TRY_DEV(hemk_batch_chain(rc, p->n_tx, w->X, NULL, 0, p->stream),
-	            "final sums");
+	if (o->norm == HEM_NORM_SEQ) {
+		HEM_TRY_DEV(hemk_batch_chain(rc, p->n_tx, w->X, NULL, 0,
+		                             p->stream),
+		            "final sums");
+	} else {
+		hemk_spec sp = spec_batch(p, rc, 0, HEMK_SPEC_IDENT, NULL,
+		                          rc.lam_sum);
+		HEM_TRY_DEV(hemk_spec_sum(&sp, 0, NULL, NULL, NULL, NULL,
+		                          p->stream),
+		            "final sums");
+	}
 	HEM_TRY_DEV(hemk_batch_out(rc, p->n_tx, w->X, p->d_len, w->B, w->C,
 	                           p->stream),
 	            "finalize");
@@ -403,12 +473,15 @@ static int run_batch(hem_problem *p, const hem_opts *o, uint32_t R,
 		                     p->stream),
 		            "tpm readback");
 	{
-		uint32_t *hk = malloc((size_t)R * 4 * 2);
+		uint32_t *hk = malloc((size_t)R * 4 * 2 + 4);
 		if (!hk)
 			return hem_fail(HEM_ERR_NOMEM, "host allocation");
 		rc2 = hemd_d2h(hk, rc.k, (size_t)R * 4, p->stream);
 		if (!rc2)
 			rc2 = hemd_d2h(hk + R, rc.conv, (size_t)R * 4, p->stream);
+		if (!rc2)
+			rc2 = hemd_d2h(hk + 2 * (size_t)R,
+			               w->u32 + 5 * (size_t)w->R + 2, 4, p->stream);
 		if (!rc2)
 			rc2 = hemd_stream_sync(p->stream);
 		if (rc2) {
@@ -430,6 +503,7 @@ static int run_batch(hem_problem *p, const hem_opts *o, uint32_t R,
 				st->n_converged += hk[R + r] != 0;
 			}
 			st->chunk = (uint32_t)chunk;
+			st->sum_reruns = hk[2 * (size_t)R];
 			st->n_tiles = p->dt.n_tiles;
 		}
 		free(hk);

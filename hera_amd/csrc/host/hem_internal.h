@@ -28,8 +28,14 @@ typedef struct {
 	uint32_t *m;                   /* [R][B] level-1 resample counts    */
 	uint32_t *rs_plan;             /* K weights + chunk starts + prefix */
 	uint64_t *seeds;               /* [R]                               */
-	uint32_t *u32;                 /* k, done, conv, notconv, active    */
-	double *f64;                   /* alpha, mean_r, mean_v, lam_sum    */
+	uint32_t *u32;                 /* k, done, conv, notconv, active,
+	                                  alist, exact-sum reruns           */
+	double *f64;                   /* alpha, mean_r, mean_v, lam_sum,
+	                                  q_r/q_v, exact pass-1 sums        */
+	double *sc;                    /* exact sums: [2][2R][P] chunk sums
+	                                  and drifts                        */
+	long long *sbase;              /* [2R][P] window bases              */
+	double *stab;                  /* [2R][P][HEMK_SPEC_W] candidates   */
 	uint32_t *h_active;            /* pinned                            */
 	/* captured iterations; the loop bounds and stop thresholds are kernel
 	 * arguments, so a graph is reused only for identical options */

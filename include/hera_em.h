@@ -158,7 +158,7 @@ int hem_loglik(hem_problem *p, const uint32_t *unique, const double *lambda,
  * component tile ([transcript][replicate] layouts, replicate fastest);
  * each keeps its own iteration count, step length and stop test, exactly as
  * R separate hem_solve calls would (bitwise so with HEM_PREC_FP64 and
- * HEM_NORM_EXACT).  Batches compute in FP64 (precision must be FP64).
+ * HEM_NORM_EXACT or HEM_NORM_SEQ).  Batches compute in FP64 (precision must be FP64).
  * They need every connected component of the class graph to fit a tile
  * (hem_problem_tiles reports n_tiles = 0 otherwise); class counts must be
  * integers below 2^32. */
@@ -171,6 +171,8 @@ typedef struct {
 	double   ms_resample;           /* device time of the resampling     */
 	double   ms_solve;              /* device time of the iteration loop */
 	double   ms_total;              /* host wall time of the call        */
+	uint32_t sum_reruns;            /* HEM_NORM_EXACT: chunks of the exact
+	                                   sums added again (all replicates) */
 } hem_batch_stats;
 
 /* Tiling of the class graph: number of tiles (0 when a component exceeds

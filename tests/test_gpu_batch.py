@@ -100,6 +100,24 @@ def test_batch_cfg1_bitwise(cfg1, kw):
     check_replicates(cls, U, CC, res, **kw)
 
 
+@pytest.mark.parametrize("step", [H.STEP_SD, H.STEP_L2])
+def test_batch_exact_sums_equal_single_lane_chains(cfg1, step):
+    """Speculated exact sums and one-lane chains: the same batch, bit for
+    bit (replicates finish at different iterations, so the active list
+    shrinks under both)."""
+    cls, u = cfg1
+    U, CC = perturbed(cls, u, 5, 21)
+    p = H.Problem(cls)
+    a = p.solve_batch(U, CC, norm=H.NORM_EXACT, step=step)
+    b = p.solve_batch(U, CC, norm=H.NORM_SEQ, step=step)
+    p.close()
+    say("step %d iters %s reruns %d" % (step, a["iters"].tolist(),
+                                        a["stats"]["sum_reruns"]))
+    np.testing.assert_array_equal(a["iters"], b["iters"])
+    np.testing.assert_array_equal(a["est"], b["est"])
+    np.testing.assert_array_equal(a["tpm"], b["tpm"])
+
+
 def test_batch_tree_norms(cfg1):
     # parallel (Chan-combined) norms put alpha within an ulp of the
     # reference's; the trajectory amplifies that to ~1e-6..1e-5 on weakly

@@ -119,7 +119,8 @@ def main():
         dt = time.perf_counter() - t
         results[name] = dict(steps=steps, iters=it, seconds=dt,
                              iters_per_s=it / dt, ms_per_solve=dt / steps * 1e3,
-                             iters_per_solve=r["stats"]["iters"])
+                             iters_per_solve=r["stats"]["iters"],
+                             exact_sum_reruns=r["stats"]["sum_reruns"])
         log("%-10s %d solves %.4fs  %.1f it/s  k=%d" % (
             name, steps, dt, it / dt, r["stats"]["iters"]))
 
@@ -336,7 +337,8 @@ def bench_bootstrap(args, H, np, p, cls, u, sk):
                          ms_solve=st["ms_solve"],
                          iters_min=st["iters_min"], iters_max=st["iters_max"],
                          iters_total=int(b["iters"].sum()),
-                         chunk=st["chunk"], n_tiles=st["n_tiles"])
+                         chunk=st["chunk"], n_tiles=st["n_tiles"],
+                         exact_sum_reruns=st["sum_reruns"])
         if name == "fp64-exact":
             res["_est_exact"] = b["est"]
         log("bootstrap %-10s R=%d %.3fs (%.1f reps/s) resample %.1f ms "

@@ -278,8 +278,8 @@ int hemk_seq_norm(uint32_t n, const double *u, const double *u1,
 /* ---- exact left-to-right sums in parallel (spec_sum.hip) -----------------
  * Chains of n FP64 values; each chain's result is bitwise the sequential
  * sum ((+0 + y_0) + y_1) + ... of the reference's loops, computed by chunks
- * of L values: guesses of every chunk's start, HEMK_SPEC_W candidate starts
- * per chunk run in parallel, then an in-order lookup (a start outside its
+ * of L values: guesses of every chunk's start, W candidate starts per
+ * chunk run in parallel, then an in-order lookup (a start outside its
  * window reruns that chunk).
  * Chain slot q (grid height nchain) sums row `row` of x (rows ld apart):
  *   pairs == 0: row = rep(q);   pairs == 1: row = (q & 1) * R + rep(q / 2)
@@ -289,14 +289,15 @@ int hemk_seq_norm(uint32_t n, const double *u, const double *u1,
  * sample sd), or x * x (SQUARE).  The sum goes to res[row]; reruns are
  * added to *miss when miss != NULL.  With ctl != NULL every kernel returns
  * at once when ctl->done is set.  Scratch, per chain slot: csum, drift,
- * base [nchain][P]; tab [nchain][P][HEMK_SPEC_W]. */
+ * base [nchain][P]; tab [nchain][P][W] (at most HEMK_SPEC_W). */
 #define HEMK_SPEC_W 64
-#define HEMK_SPEC_LMAX 2048
+#define HEMK_SPEC_LMAX 1024   /* longest chunk the kernels accept */
 #define HEMK_SPEC_IDENT 0
 #define HEMK_SPEC_CENTRED 1
 #define HEMK_SPEC_SQUARE 2
 typedef struct {
 	uint32_t n, L, P;      /* values per chain, chunk length, chunks      */
+	uint32_t W;            /* candidates per chunk: 8, 16, 32 or 64       */
 	uint32_t nchain;       /* chain slots (grid height)                   */
 	uint32_t R;            /* pairs: row offset of the second row block   */
 	int pairs, xf;
@@ -309,8 +310,10 @@ typedef struct {
 	double *res;           /* sums by row                                 */
 	uint32_t *miss;
 } hemk_spec;
-/* Chunk length for chains of n values. */
+/* Chunk length for chains of n values, and the window width for a
+ * transform at that length. */
 uint32_t hemk_spec_chunk(uint32_t n);
+uint32_t hemk_spec_width(uint32_t L, int xf);
 /* produce == 1 (plain solve, nchain 2, rows 0/1 of x): first writes
  * r = u1 - u and v = (u2 - u1) - r (src/EM.c:333-336) into x. */
 int hemk_spec_sum(const hemk_spec *sp, int produce, const double *u,

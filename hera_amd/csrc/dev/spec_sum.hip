@@ -13,20 +13,25 @@
 // is monotone in s and cheap to evaluate, but the true start s_j is only
 // known once chunks 0..j-1 are done.  So:
 //   1. chunk sums c_j (any order; they only steer guesses);
-//   2. g0_j = sum_{i<j} c_i; the drift d_j = F_j(g0_j) - g0_j is run once
-//      per chunk, all chunks at once.  It captures the chunk's rounding
+//   2. g0_j = sum_{i<j} c_i; each chunk's drift d_j, the increments
+//      fl(s + y_i) - s summed over the chunk with s an approximate running
+//      sum from g0_j (a parallel scan), captures the chunk's rounding
 //      (small terms absorbed by a large running sum) at the right
 //      magnitude, which plain chunk sums miss by up to ~10^4 ulps;
 //   3. g1_j = sum_{i<j} d_i; every chunk runs F_j from the W consecutive
 //      doubles centred on g1_j (one candidate per lane), all at once;
-//   4. one workgroup per chain walks the chunks in order from s_0 = +0:
-//      the exact s_j is looked up among chunk j's candidates by its
-//      ordered bit pattern, which yields F_j(s_j) bitwise; a start outside
-//      the window (a miss) reruns the chunk from s_j, so the result never
-//      depends on how good the guess was.
-// Steps 2-4 are pure FP64 additions in the reference's order on values
-// already rounded as the reference rounds them; step 1's and the guesses'
-// rounding only decides hits and misses.
+//   4. one wave per chain walks the chunks in order from s_0 = +0, the
+//      running value in scalar registers: the exact s_j is looked up among
+//      chunk j's candidates by its ordered bit pattern, which yields
+//      F_j(s_j) bitwise; a start outside the window (a miss) reruns the
+//      chunk from s_j, so the result never depends on how good the guess
+//      was.
+// Steps 3-4 are pure FP64 additions in the reference's order on values
+// already rounded as the reference rounds them; the rounding of steps 1-2
+// only decides hits and misses.  Sums of squares are guessed within ~2
+// ulps, so their windows are narrow (8 candidates, eight chunks per wave);
+// the first pass of the sd form (signed values whose running sum can
+// return to zero) gets wide ones.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -34,18 +39,13 @@
 
 #define SBLOCK 256
 #define WAVE 64
-#define SPEC_W HEMK_SPEC_W
-#define SPEC_PG 32                          // chunks per LDS page
-#define SPEC_PRE (SPEC_PG * SPEC_W / SBLOCK)  // page values per thread
-
-static_assert(SPEC_W == WAVE, "one candidate per lane of one wave");
-static_assert(SPEC_PG <= SBLOCK, "page bases are loaded one per thread");
+#define WIN_LDS (32 * 1024)   // window kernel: chunk values staged per wave
 
 namespace {
 
 // Ordered integer view of a double: consecutive keys are consecutive
 // doubles (-0.0 and +0.0 are neighbours), and the map is a bijection on
-// all 2^64 bit patterns, so unkey(key(x)) has x's bits.
+// all 2^64 bit patterns, so dunkey(dkey(x)) has x's bits.
 __device__ __forceinline__ long long dkey(double x)
 {
 	const long long b = __double_as_longlong(x);
@@ -57,6 +57,43 @@ __device__ __forceinline__ double dunkey(long long k)
 	const long long b =
 	    k >= 0 ? k : ((-(k + 1)) | (long long)0x8000000000000000ULL);
 	return __longlong_as_double(b);
+}
+
+// The same key from the two 32-bit halves (scalar-unit friendly): negative
+// patterns flip every bit but the sign.
+__device__ __forceinline__ unsigned long long ukey(double x)
+{
+	const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+	const uint32_t hi = (uint32_t)(b >> 32), lo = (uint32_t)b;
+	const uint32_t t = (uint32_t)((int32_t)hi >> 31);
+	return ((unsigned long long)(hi ^ (t & 0x7FFFFFFFu)) << 32) | (lo ^ t);
+}
+
+__device__ __forceinline__ double readlane_d(double v, uint32_t l)
+{
+	const long long b = __double_as_longlong(v);
+	const int lo = __builtin_amdgcn_readlane((int)b, (int)l);
+	const int hi = __builtin_amdgcn_readlane((int)(b >> 32), (int)l);
+	return __longlong_as_double(
+	    (long long)(((unsigned long long)(unsigned)hi << 32) |
+	                (unsigned)lo));
+}
+
+// Offset of key k in the window starting at key b, as a 64-bit wrapping
+// difference.  A window is the W keys b, b+1, ... taken with the same
+// wrapping addition, so offset < W identifies the candidate whose start has
+// exactly the bits of the looked-up value, wherever the window lies.
+__device__ __forceinline__ bool in_window(unsigned long long k, long long b,
+                                          uint32_t w, uint32_t &idx)
+{
+	const unsigned long long d = k - (unsigned long long)b;
+	idx = (uint32_t)d;
+	return (uint32_t)(d >> 32) == 0 && (uint32_t)d < w;
+}
+
+__device__ __forceinline__ long long kadd(long long a, long long b)
+{
+	return (long long)((unsigned long long)a + (unsigned long long)b);
 }
 
 // Chain slot q of a launch -> the row it sums (rows of x are ld apart).
@@ -76,50 +113,59 @@ __device__ __forceinline__ bool chain_row(const hemk_spec &sp, uint32_t q,
 	return true;
 }
 
-// The value the chain adds at position i of its row: x, (x - m)^2 with
-// m = mres[row] / n (the mean of src/hash_align.c:156), or x * x.
-__device__ __forceinline__ double chain_val(int xf, double x, double m)
+// The value a chain adds at one position of its row: x, (x - m)^2 with m
+// the row's mean (src/hash_align.c:156-159), or x * x.
+template <int XF>
+__device__ __forceinline__ double xval(double x, double m)
 {
 #pragma clang fp contract(off)
-	if (xf == HEMK_SPEC_CENTRED)
+	if (XF == HEMK_SPEC_CENTRED)
 		return (x - m) * (x - m);
-	if (xf == HEMK_SPEC_SQUARE)
+	if (XF == HEMK_SPEC_SQUARE)
 		return x * x;
 	return x;
 }
 
+// m = sum / n, as `sum /= n` of src/hash_align.c:156.
+template <int XF>
 __device__ __forceinline__ double chain_mean(const hemk_spec &sp,
                                              uint32_t row)
 {
-	return sp.xf == HEMK_SPEC_CENTRED ? sp.mres[row] / (double)sp.n : 0.0;
+	return XF == HEMK_SPEC_CENTRED ? sp.mres[row] / (double)sp.n : 0.0;
 }
 
-// acc + x[0] + ... + x[cnt-1] left to right (per lane; x in LDS, read
-// eight ahead of the additions).
+__device__ __forceinline__ bool skip(const hemk_ctl *ctl)
+{
+	return ctl && ctl->done;
+}
+
+// acc + x[0] + ... + x[cnt-1] left to right (per lane, x in LDS; the next
+// sixteen values are read while the current sixteen are added, which
+// covers the LDS latency with the adds' own latency).
 __device__ __forceinline__ double run_chain(double acc, const double *x,
                                             uint32_t cnt)
 {
 #pragma clang fp contract(off)
 	uint32_t j = 0;
-	if (cnt >= 8) {
-		double a[8];
+	if (cnt >= 16) {
+		double a[16];
 #pragma unroll
-		for (int q = 0; q < 8; ++q)
+		for (int q = 0; q < 16; ++q)
 			a[q] = x[q];
-		for (j = 8; j + 8 <= cnt; j += 8) {
-			double b[8];
+		for (j = 16; j + 16 <= cnt; j += 16) {
+			double b[16];
 #pragma unroll
-			for (int q = 0; q < 8; ++q)
+			for (int q = 0; q < 16; ++q)
 				b[q] = x[j + q];
 #pragma unroll
-			for (int q = 0; q < 8; ++q)
+			for (int q = 0; q < 16; ++q)
 				acc += a[q];
 #pragma unroll
-			for (int q = 0; q < 8; ++q)
+			for (int q = 0; q < 16; ++q)
 				a[q] = b[q];
 		}
 #pragma unroll
-		for (int q = 0; q < 8; ++q)
+		for (int q = 0; q < 16; ++q)
 			acc += a[q];
 	}
 	for (; j < cnt; ++j)
@@ -127,83 +173,105 @@ __device__ __forceinline__ double run_chain(double acc, const double *x,
 	return acc;
 }
 
-// Sum of the first j values of v[] (guesses only), fixed order, the same
-// bits in every lane of the wave.
-__device__ __forceinline__ double wave_prefix(const double *v, uint32_t j)
+__device__ __forceinline__ double readfirst_d(double v)
 {
-	const int lane = threadIdx.x % WAVE;
-	double g = 0.0;
-	for (uint32_t i = lane; i < j; i += WAVE)
-		g += v[i];
+	const long long b = __double_as_longlong(v);
+	const int lo = __builtin_amdgcn_readfirstlane((int)b);
+	const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+	return __longlong_as_double(
+	    (long long)(((unsigned long long)(unsigned)hi << 32) |
+	                (unsigned)lo));
+}
+
+__device__ __forceinline__ long long readfirst_ll(long long b)
+{
+	const int lo = __builtin_amdgcn_readfirstlane((int)b);
+	const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+	return (long long)(((unsigned long long)(unsigned)hi << 32) |
+	                   (unsigned)lo);
+}
+
+// Stage cnt chain values of a row into ys with nthr threads, sixteen loads
+// in flight per thread.
+template <int XF>
+__device__ __forceinline__ void stage(const double *__restrict__ x,
+                                      uint32_t cnt, double m, double *ys,
+                                      int nthr)
+{
+	const int t = threadIdx.x % nthr;
+	uint32_t i = t;
+	for (; i + 15 * nthr < cnt; i += 16 * nthr) {
+		double v[16];
 #pragma unroll
-	for (int m = WAVE / 2; m >= 1; m >>= 1)
-		g += __shfl_xor(g, m, WAVE);
-	return g;
+		for (int q = 0; q < 16; ++q)
+			v[q] = x[i + q * nthr];
+#pragma unroll
+		for (int q = 0; q < 16; ++q)
+			ys[i + q * nthr] = xval<XF>(v[q], m);
+	}
+	double v[16];
+#pragma unroll
+	for (int q = 0; q < 16; ++q)
+		if (i + q * nthr < cnt)
+			v[q] = x[i + q * nthr];
+#pragma unroll
+	for (int q = 0; q < 16; ++q)
+		if (i + q * nthr < cnt)
+			ys[i + q * nthr] = xval<XF>(v[q], m);
 }
 
-// Stage chunk j of a row (its chain values) into LDS.
-__device__ __forceinline__ uint32_t stage_chunk(const hemk_spec &sp,
-                                                uint32_t row, uint32_t j,
-                                                double m, double *ys,
-                                                int nthreads)
+// Block sum of one double per thread (SBLOCK threads), fixed order; every
+// thread gets the result.
+__device__ __forceinline__ double block_sum(double a)
 {
-	const uint32_t beg = j * sp.L;
-	const uint32_t cnt = min(sp.L, sp.n - beg);
-	const double *x = sp.x + (size_t)row * sp.ld + beg;
-	for (uint32_t i = threadIdx.x; i < cnt; i += nthreads)
-		ys[i] = chain_val(sp.xf, x[i], m);
-	return cnt;
-}
-
-__device__ __forceinline__ bool skip(const hemk_spec &sp,
-                                     const hemk_ctl *ctl)
-{
-	return ctl && ctl->done;
+	__shared__ double red[SBLOCK / WAVE];
+#pragma unroll
+	for (int k = WAVE / 2; k >= 1; k >>= 1)
+		a += __shfl_xor(a, k, WAVE);
+	__syncthreads();
+	if (threadIdx.x % WAVE == 0)
+		red[threadIdx.x / WAVE] = a;
+	__syncthreads();
+	double t = red[0];
+#pragma unroll
+	for (int w = 1; w < SBLOCK / WAVE; ++w)
+		t += red[w];
+	return t;
 }
 
 // Step 1: chunk sums.  Grid (P, chains), SBLOCK threads.
+template <int XF>
 __global__ __launch_bounds__(SBLOCK) void k_spec_csum(hemk_spec sp,
                                                       const hemk_ctl *ctl)
 {
 #pragma clang fp contract(off)
-	__shared__ double red[SBLOCK / WAVE];
 	uint32_t row;
-	if (skip(sp, ctl) || !chain_row(sp, blockIdx.y, row))
+	if (skip(ctl) || !chain_row(sp, blockIdx.y, row))
 		return;
 	const uint32_t j = blockIdx.x;
 	const uint32_t beg = j * sp.L;
 	const uint32_t end = min(sp.n, beg + sp.L);
-	const double m = chain_mean(sp, row);
+	const double m = chain_mean<XF>(sp, row);
 	const double *x = sp.x + (size_t)row * sp.ld;
 	double a = 0.0;
 	for (uint32_t i = beg + threadIdx.x; i < end; i += SBLOCK)
-		a += chain_val(sp.xf, x[i], m);
-#pragma unroll
-	for (int k = WAVE / 2; k >= 1; k >>= 1)
-		a += __shfl_xor(a, k, WAVE);
-	if (threadIdx.x % WAVE == 0)
-		red[threadIdx.x / WAVE] = a;
-	__syncthreads();
-	if (threadIdx.x == 0) {
-		double t = red[0];
-#pragma unroll
-		for (int w = 1; w < SBLOCK / WAVE; ++w)
-			t += red[w];
-		sp.csum[(size_t)blockIdx.y * sp.P + j] = t;
-	}
+		a += xval<XF>(x[i], m);
+	a = block_sum(a);
+	if (threadIdx.x == 0)
+		sp.csum[(size_t)blockIdx.y * sp.P + j] = a;
 }
 
 // Plain solve, pass 1: r = u1 - u and v = (u2 - u1) - r (src/EM.c:333-336)
-// into rows 0 and 1 of x, and the chunk sums of their chain values (IDENT
-// or SQUARE).  Grid (P, 2).
+// into rows 0 and 1 of x, and the chunk sums of their chain values (XF
+// IDENT or SQUARE).  Grid (P, 2).
+template <int XF>
 __global__ __launch_bounds__(SBLOCK) void k_spec_rv(
     hemk_spec sp, const double *__restrict__ u, const double *__restrict__ u1,
     const double *__restrict__ u2, double *__restrict__ xw,
     const hemk_ctl *ctl)
 {
 #pragma clang fp contract(off)
-	__shared__ double red[SBLOCK / WAVE];
-	if (skip(sp, ctl))
+	if (skip(ctl))
 		return;
 	const uint32_t q = blockIdx.y;
 	const uint32_t j = blockIdx.x;
@@ -214,157 +282,241 @@ __global__ __launch_bounds__(SBLOCK) void k_spec_rv(
 		const double r = u1[i] - u[i];
 		const double val = q == 0 ? r : (u2[i] - u1[i]) - r;
 		xw[(size_t)q * sp.ld + i] = val;
-		a += chain_val(sp.xf, val, 0.0);
+		a += xval<XF>(val, 0.0);
 	}
-#pragma unroll
-	for (int k = WAVE / 2; k >= 1; k >>= 1)
-		a += __shfl_xor(a, k, WAVE);
-	if (threadIdx.x % WAVE == 0)
-		red[threadIdx.x / WAVE] = a;
-	__syncthreads();
-	if (threadIdx.x == 0) {
-		double t = red[0];
-#pragma unroll
-		for (int w = 1; w < SBLOCK / WAVE; ++w)
-			t += red[w];
-		sp.csum[(size_t)q * sp.P + j] = t;
-	}
+	a = block_sum(a);
+	if (threadIdx.x == 0)
+		sp.csum[(size_t)q * sp.P + j] = a;
 }
 
-// Step 2: drift of each chunk from the chunk-sum guess.  Grid (P, chains),
-// one wave.
-__global__ __launch_bounds__(WAVE) void k_spec_drift(hemk_spec sp,
-                                                     const hemk_ctl *ctl)
+// Step 2: drift of each chunk.  Grid (P, chains), SBLOCK threads; thread t
+// holds L/SBLOCK consecutive values.  The running sum before each value is
+// g0_j plus a block scan of the chunk (approximate, like g0 itself); the
+// drift adds up the rounded increments fl(s + y) - s.
+template <int XF>
+__global__ __launch_bounds__(SBLOCK) void k_spec_drift(hemk_spec sp,
+                                                       const hemk_ctl *ctl)
 {
 #pragma clang fp contract(off)
-	__shared__ double ys[HEMK_SPEC_LMAX];
+	__shared__ double wtot[SBLOCK / WAVE];
 	uint32_t row;
-	if (skip(sp, ctl) || !chain_row(sp, blockIdx.y, row))
+	if (skip(ctl) || !chain_row(sp, blockIdx.y, row))
 		return;
 	const uint32_t j = blockIdx.x;
 	const size_t cb = (size_t)blockIdx.y * sp.P;
-	const double g0 = wave_prefix(sp.csum + cb, j);
-	const uint32_t cnt =
-	    stage_chunk(sp, row, j, chain_mean(sp, row), ys, WAVE);
+	const int tid = threadIdx.x, lane = tid % WAVE, w = tid / WAVE;
+	double g = 0.0;
+	for (uint32_t i = tid; i < j; i += SBLOCK)
+		g += sp.csum[cb + i];
+	const double g0 = block_sum(g);
+	const double m = chain_mean<XF>(sp, row);
+	const uint32_t E = sp.L / SBLOCK;   // 1..8
+	const uint32_t beg = j * sp.L + tid * E;
+	const double *x = sp.x + (size_t)row * sp.ld;
+	double y[8], tot = 0.0;
+#pragma unroll
+	for (int e = 0; e < 8; ++e) {
+		y[e] = (e < (int)E && beg + e < sp.n) ? xval<XF>(x[beg + e], m)
+		                                       : 0.0;
+		tot += y[e];
+	}
+	// exclusive scan of the thread totals
+	double inc = tot;
+#pragma unroll
+	for (int k = 1; k < WAVE; k <<= 1) {
+		const double o = __shfl_up(inc, k, WAVE);
+		if (lane >= k)
+			inc += o;
+	}
+	if (lane == WAVE - 1)
+		wtot[w] = inc;
 	__syncthreads();
-	const double f = run_chain(g0, ys, cnt);
-	if (threadIdx.x == 0)
-		sp.drift[cb + j] = f - g0;
+	double off = inc - tot;
+	for (int k = 0; k < w; ++k)
+		off += wtot[k];
+	double sh = g0 + off, q = 0.0;
+#pragma unroll
+	for (int e = 0; e < 8; ++e) {
+		const double nx = sh + y[e];
+		q += nx - sh;
+		sh = nx;
+	}
+	q = block_sum(q);
+	if (tid == 0)
+		sp.drift[cb + j] = q;
 }
 
-// Step 3: every chunk from SPEC_W consecutive starts around the drift
-// guess.  Grid (P, chains), one wave; lane c starts at key(g1) - W/2 + c.
+// Step 3: every chunk from W consecutive starts around g1 = the sum of the
+// previous chunks' drifts.  Grid (P / (64/W), chains), one wave; lane
+// (s, c) runs chunk blockIdx.x * 64/W + s from key(g1) - W/2 + c.
+template <int W, int XF>
 __global__ __launch_bounds__(WAVE) void k_spec_window(hemk_spec sp,
                                                       const hemk_ctl *ctl)
 {
 #pragma clang fp contract(off)
-	__shared__ double ys[HEMK_SPEC_LMAX];
+	constexpr int CPV = WAVE / W;
+	__shared__ double ys[WIN_LDS / 8];
 	uint32_t row;
-	if (skip(sp, ctl) || !chain_row(sp, blockIdx.y, row))
+	if (skip(ctl) || !chain_row(sp, blockIdx.y, row))
 		return;
-	const uint32_t j = blockIdx.x;
+	const int lane = threadIdx.x, sub = lane / W, c = lane % W;
+	const uint32_t j0 = blockIdx.x * CPV;
+	const uint32_t j = j0 + sub;
 	const size_t cb = (size_t)blockIdx.y * sp.P;
-	const double g1 = wave_prefix(sp.drift + cb, j);
-	// wrapping arithmetic: a window that straddles the ends of the key
-	// range only produces misses (the lookup checks for overflow)
-	const long long base =
-	    (long long)((unsigned long long)dkey(g1) - SPEC_W / 2);
-	const uint32_t cnt =
-	    stage_chunk(sp, row, j, chain_mean(sp, row), ys, WAVE);
+	// g1: the prefix up to j0 over the wave (identical in every lane),
+	// then this lane's chunk's few predecessors in the group
+	double g = 0.0;
+	for (uint32_t i = lane; i < j0; i += WAVE)
+		g += sp.drift[cb + i];
+#pragma unroll
+	for (int k = WAVE / 2; k >= 1; k >>= 1)
+		g += __shfl_xor(g, k, WAVE);
+	for (uint32_t i = j0; i < j && i < sp.P; ++i)
+		g += sp.drift[cb + i];
+	const double m = chain_mean<XF>(sp, row);
+	const double *x = sp.x + (size_t)row * sp.ld;
+	const uint32_t nch = min((uint32_t)CPV, sp.P - j0);
+	const uint32_t tot = min(nch * sp.L, sp.n - j0 * sp.L);
+	stage<XF>(x + (size_t)j0 * sp.L, tot, m, ys, WAVE);
 	__syncthreads();
-	const double s = run_chain(
-	    dunkey((long long)((unsigned long long)base + threadIdx.x)), ys,
-	    cnt);
-	sp.tab[(cb + j) * SPEC_W + threadIdx.x] = s;
-	if (threadIdx.x == 0)
+	if (j >= sp.P)
+		return;
+	const long long base = kadd(dkey(g), -(W / 2));
+	const uint32_t cnt = min(sp.L, sp.n - j * sp.L);
+	const double s = run_chain(dunkey(kadd(base, c)), ys + sub * sp.L, cnt);
+	sp.tab[(cb + j) * W + c] = s;
+	if (c == 0)
 		sp.base[cb + j] = base;
 }
 
-// Step 4: walk the chunks in order.  One workgroup per chain; every thread
-// holds the same running value, so every branch is uniform.  Candidate
-// tables stream through two LDS pages (the next page is in registers
-// while the current one is resolved).
-__global__ __launch_bounds__(SBLOCK) void k_spec_resolve(hemk_spec sp,
-                                                         hemk_ctl *ctl)
+// Repairs after a miss.  A miss at chunk j means the guess g1_j was off by
+// delta = s_j - g1_j, and the following guesses are usually off by about
+// the same amount.  Wave 0 runs chunks j .. j+RC-1 at once: group 0 from
+// the exact s_j, group k from the RW doubles centred on g1_{j+k} + delta;
+// results go to rtl[k*RW + c], window bases to rbl[k].
+#define RW 16
+#define RC (WAVE / RW)
+static_assert(RC * HEMK_SPEC_LMAX * 8 <= WIN_LDS, "repair chunks fit LDS");
+// candidate tables held in LDS by the resolve kernel at once (bytes)
+#define RES_LDS (120 * 1024)
+
+template <int W, int XF>
+__device__ __forceinline__ void repair(const hemk_spec &sp, uint32_t row,
+                                       const long long *bsl, uint32_t jb,
+                                       uint32_t j, double m, double s,
+                                       double *ys, double *rtl,
+                                       long long *rbl)
 {
 #pragma clang fp contract(off)
-	__shared__ double tb[2][SPEC_PG * SPEC_W];
-	__shared__ long long bs[2][SPEC_PG];
-	__shared__ double ys[HEMK_SPEC_LMAX];
-	uint32_t row;
-	if (skip(sp, ctl) || !chain_row(sp, blockIdx.x, row))
-		return;
-	const size_t cb = (size_t)blockIdx.x * sp.P;
-	const double *tab = sp.tab + cb * SPEC_W;
-	const long long *bse = sp.base + cb;
-	const double m = chain_mean(sp, row);
-	const uint32_t npg = (sp.P + SPEC_PG - 1) / SPEC_PG;
-	const int tid = threadIdx.x;
-	double pre[SPEC_PRE];
-	long long preb = 0;
-
-	// page 0
-	{
-		const uint32_t lim = min(sp.P, (uint32_t)SPEC_PG) * SPEC_W;
-#pragma unroll
-		for (int i = 0; i < SPEC_PRE; ++i) {
-			const uint32_t f = tid + SBLOCK * i;
-			pre[i] = f < lim ? tab[f] : 0.0;
+	const uint32_t nch = min((uint32_t)RC, sp.P - j);
+	const uint32_t tot = min(nch * sp.L, sp.n - j * sp.L);
+	stage<XF>(sp.x + (size_t)row * sp.ld + (size_t)j * sp.L, tot, m, ys,
+	          SBLOCK);
+	__syncthreads();
+	if (threadIdx.x < WAVE) {
+		const int lane = threadIdx.x, k = lane / RW, c = lane % RW;
+		const uint32_t jj = j + k;
+		const double delta = s - dunkey(kadd(bsl[j - jb], W / 2));
+		long long b = 0;
+		double st = s;
+		if (k > 0 && jj < sp.P) {
+			const long long bj = sp.base[(size_t)blockIdx.x * sp.P + jj];
+			const double g = dunkey(kadd(bj, W / 2)) + delta;
+			b = kadd(dkey(g), -(RW / 2));
+			st = dunkey(kadd(b, c));
 		}
-		if (tid < SPEC_PG && (uint32_t)tid < sp.P)
-			preb = bse[tid];
-#pragma unroll
-		for (int i = 0; i < SPEC_PRE; ++i)
-			tb[0][tid + SBLOCK * i] = pre[i];
-		if (tid < SPEC_PG)
-			bs[0][tid] = preb;
-		__syncthreads();
+		double r = 0.0;
+		if (jj < sp.P)
+			r = run_chain(st, ys + k * sp.L,
+			              min(sp.L, sp.n - jj * sp.L));
+		rtl[lane] = r;
+		if (c == 0)
+			rbl[k] = b;
 	}
+	__syncthreads();
+}
 
+// Step 4: walk the chunks in order, one workgroup per chain.  The chain's
+// candidate tables are loaded into LDS (a segment of chunks at a time, the
+// whole chain when it fits), then every wave walks them redundantly with
+// the running value in scalar registers, so every branch is uniform and
+// all threads can stage a repair's values.  The candidate row and window
+// base of the next two chunks are read from LDS ahead of their turn; the
+// lookup itself is scalar arithmetic and one readlane.
+template <int W, int XF>
+__global__ __launch_bounds__(SBLOCK) void k_spec_resolve(hemk_spec sp,
+                                                         const hemk_ctl *ctl)
+{
+#pragma clang fp contract(off)
+	constexpr uint32_t SEG = RES_LDS / (W * 8 + 8);  // chunks per segment
+	__shared__ double tbl[SEG * W];
+	__shared__ long long bsl[SEG];
+	__shared__ double ys[WIN_LDS / 8];
+	__shared__ double rtl[WAVE];
+	__shared__ long long rbl[RC];
+	uint32_t row;
+	if (skip(ctl) || !chain_row(sp, blockIdx.x, row))
+		return;
+	const int tid = threadIdx.x;
+	const uint32_t c = (uint32_t)tid % W;
+	const size_t cb = (size_t)blockIdx.x * sp.P;
+	const double m = chain_mean<XF>(sp, row);
+	const uint32_t P = sp.P;
 	double s = 0.0;
-	uint32_t misses = 0;
-	for (uint32_t pg = 0; pg < npg; ++pg) {
-		const int buf = pg & 1;
-		const uint32_t j0 = pg * SPEC_PG;
-		const uint32_t j1 = min(sp.P, j0 + SPEC_PG);
-		const bool more = pg + 1 < npg;
-		if (more) {
-			const uint32_t n0 = j0 + SPEC_PG;
-			const uint32_t lim =
-			    (min(sp.P, n0 + SPEC_PG) - n0) * SPEC_W;
-			const double *t = tab + (size_t)n0 * SPEC_W;
+	uint32_t rj = 0, misses = 0;
+	bool rep = false;
+	for (uint32_t jb = 0; jb < P; jb += SEG) {
+		const uint32_t ns = min(SEG, P - jb);
+		__syncthreads();
+		{
+			const double *t = sp.tab + (cb + jb) * W;
+			const uint32_t cnt = ns * W;
+			uint32_t i = tid;
+			for (; i + 15 * SBLOCK < cnt; i += 16 * SBLOCK) {
+				double v[16];
 #pragma unroll
-			for (int i = 0; i < SPEC_PRE; ++i) {
-				const uint32_t f = tid + SBLOCK * i;
-				pre[i] = f < lim ? t[f] : 0.0;
-			}
-			if (tid < SPEC_PG && n0 + tid < sp.P)
-				preb = bse[n0 + tid];
-		}
-		for (uint32_t j = j0; j < j1; ++j) {
-			long long idx;
-			if (!__builtin_sub_overflow(dkey(s), bs[buf][j - j0],
-			                            &idx) &&
-			    idx >= 0 && idx < SPEC_W) {
-				s = tb[buf][(j - j0) * SPEC_W + idx];
-				continue;
-			}
-			// the exact start lies outside the window: add the
-			// chunk from it
-			const uint32_t cnt = stage_chunk(sp, row, j, m, ys, SBLOCK);
-			__syncthreads();
-			s = run_chain(s, ys, cnt);
-			__syncthreads();
-			++misses;
-		}
-		if (more) {
+				for (int q = 0; q < 16; ++q)
+					v[q] = t[i + q * SBLOCK];
 #pragma unroll
-			for (int i = 0; i < SPEC_PRE; ++i)
-				tb[buf ^ 1][tid + SBLOCK * i] = pre[i];
-			if (tid < SPEC_PG)
-				bs[buf ^ 1][tid] = preb;
+				for (int q = 0; q < 16; ++q)
+					tbl[i + q * SBLOCK] = v[q];
+			}
+			for (; i < cnt; i += SBLOCK)
+				tbl[i] = t[i];
+			for (uint32_t q = tid; q < ns; q += SBLOCK)
+				bsl[q] = sp.base[cb + jb + q];
 		}
 		__syncthreads();
+		// rows and bases of chunks q and q+1 in registers
+		double r0 = tbl[c], r1 = tbl[min(1u, ns - 1) * W + c];
+		long long b0 = readfirst_ll(bsl[0]);
+		long long b1 = bsl[min(1u, ns - 1)];
+		for (uint32_t q = 0; q < ns; ++q) {
+			const uint32_t j = jb + q;
+			const uint32_t q2 = min(q + 2, ns - 1);
+			const double r2 = tbl[q2 * W + c];
+			const long long b2 = bsl[q2];
+			const unsigned long long ks = ukey(s);
+			uint32_t idx;
+			if (in_window(ks, b0, W, idx)) {
+				s = readlane_d(r0, idx);
+			} else if (rep && j - rj < RC &&
+			           in_window(ks, readfirst_ll(rbl[j - rj]), RW,
+			                     idx)) {
+				s = readfirst_d(rtl[(j - rj) * RW + idx]);
+			} else {
+				repair<W, XF>(sp, row, bsl, jb, j, m, s, ys, rtl,
+				              rbl);
+				rj = j;
+				rep = true;
+				s = readfirst_d(rtl[0]);
+				++misses;
+			}
+			r0 = r1;
+			r1 = r2;
+			b0 = readfirst_ll(b1);
+			b1 = b2;
+		}
 	}
 	if (tid == 0) {
 		sp.res[row] = s;
@@ -373,17 +525,59 @@ __global__ __launch_bounds__(SBLOCK) void k_spec_resolve(hemk_spec sp,
 	}
 }
 
+template <int XF>
+int launch_xf(const hemk_spec &sp, int produce, const double *u,
+              const double *u1, const double *u2, const hemk_ctl *ctl,
+              hipStream_t st)
+{
+	dim3 grid(sp.P, sp.nchain);
+	if (produce)
+		k_spec_rv<XF><<<grid, SBLOCK, 0, st>>>(sp, u, u1, u2,
+		                                       (double *)sp.x, ctl);
+	else
+		k_spec_csum<XF><<<grid, SBLOCK, 0, st>>>(sp, ctl);
+	k_spec_drift<XF><<<grid, SBLOCK, 0, st>>>(sp, ctl);
+#define SPEC_W_CASE(WW)                                                    \
+	case WW:                                                           \
+		k_spec_window<WW, XF>                                      \
+		    <<<dim3((sp.P + WAVE / WW - 1) / (WAVE / WW), sp.nchain), \
+		       WAVE, 0, st>>>(sp, ctl);                            \
+		k_spec_resolve<WW, XF><<<sp.nchain, SBLOCK, 0, st>>>(sp, ctl); \
+		break;
+	switch (sp.W) {
+		SPEC_W_CASE(8)
+		SPEC_W_CASE(16)
+		SPEC_W_CASE(32)
+		SPEC_W_CASE(64)
+	default:
+		return (int)hipErrorInvalidValue;
+	}
+#undef SPEC_W_CASE
+	return (int)hipGetLastError();
+}
+
 } // namespace
 
 extern "C" {
 
 uint32_t hemk_spec_chunk(uint32_t n)
 {
-	// ~2 dependent chunk runs of L adds against ~n/L dependent lookups
-	uint32_t L = 256;
-	while (L < HEMK_SPEC_LMAX && (uint64_t)L * L < (uint64_t)n * 4)
-		L *= 2;
-	return L;
+	// ~L dependent adds per window run against ~n/L dependent lookups;
+	// long chunks also average out the guesses' errors
+	return n <= 16384 ? 256 : HEMK_SPEC_LMAX;
+}
+
+uint32_t hemk_spec_width(uint32_t L, int xf)
+{
+	// sums of squares are guessed within ~2 ulps: the narrowest window
+	// whose chunks fit the window kernel's LDS; signed sums (whose
+	// running sum can pass through zero): 64
+	if (xf == HEMK_SPEC_IDENT)
+		return 64;
+	uint32_t w = 8;
+	while (w < 64 && (uint64_t)(WAVE / w) * L * 8 > WIN_LDS)
+		w *= 2;
+	return w;
 }
 
 int hemk_spec_sum(const hemk_spec *sp, int produce, const double *u,
@@ -392,20 +586,25 @@ int hemk_spec_sum(const hemk_spec *sp, int produce, const double *u,
 {
 	hipStream_t st = (hipStream_t)s;
 	if (sp->n == 0 || sp->nchain == 0 || sp->L == 0 ||
-	    sp->L > HEMK_SPEC_LMAX || sp->L % 8 != 0 ||
+	    sp->L > HEMK_SPEC_LMAX || sp->L % SBLOCK != 0 ||
+	    sp->n > 0xFFFFFFFFu - sp->L ||
 	    sp->P != (sp->n + sp->L - 1) / sp->L ||
-	    (produce && (sp->nchain != 2 || sp->pairs || sp->alist)))
+	    (uint64_t)(WAVE / (sp->W ? sp->W : 1)) * sp->L * 8 > WIN_LDS ||
+	    (produce && (sp->nchain != 2 || sp->pairs || sp->alist ||
+	                 sp->xf == HEMK_SPEC_CENTRED)))
 		return (int)hipErrorInvalidValue;
-	dim3 grid(sp->P, sp->nchain);
-	if (produce)
-		k_spec_rv<<<grid, SBLOCK, 0, st>>>(*sp, u, u1, u2,
-		                                   (double *)sp->x, ctl);
-	else
-		k_spec_csum<<<grid, SBLOCK, 0, st>>>(*sp, ctl);
-	k_spec_drift<<<grid, WAVE, 0, st>>>(*sp, ctl);
-	k_spec_window<<<grid, WAVE, 0, st>>>(*sp, ctl);
-	k_spec_resolve<<<sp->nchain, SBLOCK, 0, st>>>(*sp, ctl);
-	return (int)hipGetLastError();
+	switch (sp->xf) {
+	case HEMK_SPEC_IDENT:
+		return launch_xf<HEMK_SPEC_IDENT>(*sp, produce, u, u1, u2, ctl,
+		                                  st);
+	case HEMK_SPEC_CENTRED:
+		return launch_xf<HEMK_SPEC_CENTRED>(*sp, produce, u, u1, u2, ctl,
+		                                    st);
+	case HEMK_SPEC_SQUARE:
+		return launch_xf<HEMK_SPEC_SQUARE>(*sp, produce, u, u1, u2, ctl,
+		                                   st);
+	}
+	return (int)hipErrorInvalidValue;
 }
 
 } // extern "C"

@@ -47,9 +47,10 @@ class HemStats(C.Structure):
                 ("sum_reruns", C.c_uint32)]
 
 
-HEM_KERNEL_KINDS = 6
+HEM_KERNEL_KINDS = 8
 KERNEL_KIND_NAMES = ["class_pass", "transcript_pass", "norm_pass1",
-                     "norm_pass2", "extrapolate", "iter_end"]
+                     "norm_pass2", "extrapolate", "iter_end", "tile_map12",
+                     "tile_ex3"]
 
 
 class HemKernelProfile(C.Structure):

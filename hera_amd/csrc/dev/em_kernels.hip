@@ -15,6 +15,7 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "em_step.h"
 #include "hem_dev.h"
 
 #define BLOCK 256
@@ -151,29 +152,15 @@ __global__ __launch_bounds__(BLOCK) void k_gather_exact(
 	}
 	if (flags) {
 		bad = __syncthreads_or(bad);
-		if (threadIdx.x == 0)
-			flags[blockIdx.x] = (uint32_t)bad;
+		if (threadIdx.x == 0 && bad)
+			em_mark_bad(flags);
 	}
 }
 
-// SQUAREM step length from the two reductions (src/EM.c:338-346): sample
-// sd (A, B = centred sums of squares) or L2 (A, B = sums of squares).
 __device__ __forceinline__ double step_length(double A, double B, uint32_t n,
                                               hemk_step st)
 {
-#pragma clang fp contract(off)
-	double n_r, n_v, alpha;
-	if (st.step == 0) {
-		n_r = sqrt(A / (double)(n - 1u));
-		n_v = sqrt(B / (double)(n - 1u));
-	} else {
-		n_r = sqrt(A);
-		n_v = B <= 1e-300 ? 0.0 : sqrt(B);
-	}
-	alpha = n_v != 0 ? -n_r / n_v : -1.0;
-	if (st.alpha_clamp && alpha > -1)
-		alpha = -1;
-	return alpha;
+	return em_step_length(A, B, n, st);
 }
 
 template <int G, typename TL>
@@ -235,8 +222,8 @@ __global__ __launch_bounds__(BLOCK) void k_gather(
 	}
 	if (flags) {
 		bad = __syncthreads_or(bad);
-		if (threadIdx.x == 0)
-			flags[blockIdx.x] = (uint32_t)bad;
+		if (threadIdx.x == 0 && bad)
+			em_mark_bad(flags);
 	}
 }
 
@@ -318,15 +305,7 @@ __global__ __launch_bounds__(BLOCK) void k_extrap(
 	}
 	for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n;
 	     i += gridDim.x * BLOCK) {
-		const double r = u1[i] - u[i];
-		const double v = (u2[i] - u1[i]) - r;
-		double x = u[i] - 2 * alpha * r + alpha * alpha * v;
-		if (st.clamp == 0) {
-			if (x < 0)
-				x = 0;
-		} else if (x <= 1e-300) {
-			x = u2[i] / 10;
-		}
+		const double x = em_extrapolate(u[i], u1[i], u2[i], alpha, st);
 		out64[i] = x;
 		if (out32)
 			out32[i] = (float)x;
@@ -441,17 +420,14 @@ __global__ __launch_bounds__(BLOCK) void k_seq_norm(
 	}
 }
 
-__global__ __launch_bounds__(BLOCK) void k_iter_end(
-    hemk_ctl *__restrict__ ctl, const uint32_t *__restrict__ flags,
-    uint32_t n_flags)
+// The third map's kernels OR a failure of the stop test into ctl->stopbad.
+__global__ void k_iter_end(hemk_ctl *__restrict__ ctl)
 {
 	if (ctl->done)
 		return;
-	uint32_t any = 0;
-	for (uint32_t i = threadIdx.x; i < n_flags; i += BLOCK)
-		any |= flags[i];
-	any = __syncthreads_or((int)any);
-	if (threadIdx.x == 0) {
+	const uint32_t any = ctl->stopbad;
+	ctl->stopbad = 0;
+	{
 		const uint32_t k = ctl->k + 1;
 		const uint32_t stop = any ? 0u : 1u;
 		ctl->k = k;
@@ -472,6 +448,7 @@ __global__ void k_ctl_reset(hemk_ctl *ctl, uint32_t min_iter,
 	ctl->converged = 0;
 	ctl->fallbacks = 0;
 	ctl->reruns = 0;
+	ctl->stopbad = 0;
 	ctl->min_iter = min_iter;
 	ctl->max_iter = max_iter;
 	ctl->alpha = 0;
@@ -736,10 +713,9 @@ int hemk_extrapolate(uint32_t n, const double *u, const double *u1,
 	return (int)hipGetLastError();
 }
 
-int hemk_iter_end(hemk_ctl *ctl, const uint32_t *stop_flags,
-                  uint32_t n_flags, void *s)
+int hemk_iter_end(hemk_ctl *ctl, void *s)
 {
-	k_iter_end<<<1, BLOCK, 0, (hipStream_t)s>>>(ctl, stop_flags, n_flags);
+	k_iter_end<<<1, 1, 0, (hipStream_t)s>>>(ctl);
 	return (int)hipGetLastError();
 }
 

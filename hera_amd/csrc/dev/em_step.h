@@ -1,0 +1,63 @@
+// em_step.h -- the SQUAREM step arithmetic shared by the plain kernels
+// (src/EM.c:333-352), included by HIP sources only.
+#ifndef EM_STEP_H_
+#define EM_STEP_H_
+
+#include <math.h>
+#include <stdint.h>
+
+#include "hem_dev.h"
+
+// Step length from the two reductions (src/EM.c:338-346): sample sd (A, B
+// = centred sums of squares, src/hash_align.c:158-163) or L2 (A, B = sums
+// of squares, hera-EM/utility.c:16-23).
+__device__ __forceinline__ double em_step_length(double A, double B,
+                                                 uint32_t n, hemk_step st)
+{
+#pragma clang fp contract(off)
+	double n_r, n_v, alpha;
+	if (st.step == 0) {
+		n_r = sqrt(A / (double)(n - 1u));
+		n_v = sqrt(B / (double)(n - 1u));
+	} else {
+		n_r = sqrt(A);
+		n_v = B <= 1e-300 ? 0.0 : sqrt(B);
+	}
+	alpha = n_v != 0 ? -n_r / n_v : -1.0;
+	if (st.alpha_clamp && alpha > -1)
+		alpha = -1;
+	return alpha;
+}
+
+// The extrapolated point of one transcript: u - 2*alpha*r + alpha*alpha*v
+// with r = u1 - u, v = (u2 - u1) - r, rounded exactly as src/EM.c:334-350
+// (no contraction), then the clamp of src/EM.c:351 (ZERO) or
+// hera-EM/utility.c:50-53 (DIV10).
+__device__ __forceinline__ double em_extrapolate(double u, double u1,
+                                                 double u2, double alpha,
+                                                 hemk_step st)
+{
+#pragma clang fp contract(off)
+	const double r = u1 - u;
+	const double v = (u2 - u1) - r;
+	double x = u - 2 * alpha * r + alpha * alpha * v;
+	if (st.clamp == 0) {
+		if (x < 0)
+			x = 0;
+	} else if (x <= 1e-300) {
+		x = u2 / 10;
+	}
+	return x;
+}
+
+// Marks the stop test as failed: one atomic OR, skipped once the flag
+// reads as set (an L2 read, so the check sees other blocks' ORs), which
+// keeps thousands of failing blocks from serialising on one word.
+__device__ __forceinline__ void em_mark_bad(uint32_t *flag)
+{
+	if (!__hip_atomic_load(flag, __ATOMIC_RELAXED,
+	                       __HIP_MEMORY_SCOPE_AGENT))
+		atomicOr(flag, 1u);
+}
+
+#endif

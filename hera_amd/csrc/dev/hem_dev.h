@@ -52,6 +52,9 @@ typedef struct {
 	uint32_t min_iter, max_iter;
 	uint32_t fallbacks;
 	uint32_t reruns;      /* chunks of the exact sums run again (misses)   */
+	uint32_t stopbad;     /* set (OR) by the third map when some
+	                         transcript fails the stop test; read and
+	                         cleared by the iteration end              */
 	double   alpha;       /* last step length                              */
 	double   stop_min, stop_rel;
 	double   mean_r, mean_v;  /* first pass of the sequential norms        */
@@ -244,11 +247,11 @@ int hemk_denom(const hemk_prob *p, int prec, const void *lam, void *rho,
                int group, const hemk_ctl *ctl, void *s);
 
 /* CSC pass + M-step: s_t = sum_{c ni t} rho_c, out_t = (u_t + lam_t s_t) /
- * len_t; FP32 shadow written when out32 != NULL; when stop_flags != NULL
- * each block stores whether any of its transcripts fails the stop test. */
+ * len_t; FP32 shadow written when out32 != NULL; when stop_flag != NULL a
+ * block with a transcript that fails the stop test ORs 1 into it. */
 int hemk_gather_mstep(const hemk_prob *p, int prec, const void *rho,
                       const uint32_t *unique, const double *lam64,
-                      double *out64, float *out32, uint32_t *stop_flags,
+                      double *out64, float *out32, uint32_t *stop_flag,
                       int group, const hemk_ctl *ctl, void *s);
 uint32_t hemk_gather_blocks(const hemk_prob *p, int group);
 
@@ -274,6 +277,26 @@ int hemk_extrapolate(uint32_t n, const double *u, const double *u1,
 int hemk_seq_norm(uint32_t n, const double *u, const double *u1,
                   const double *u2, int phase, hemk_step st, hemk_ctl *ctl,
                   void *s);
+
+/* ---- one EM instance over component tiles (ptile_kernels.hip) -----------
+ * State A, B, C in tile order; uq (unique counts) and cntq (class counts,
+ * class tile positions) likewise.  map12: B = M(A), C = M(B), and r, v
+ * rows in transcript order into xr, xv when xr != NULL.  ex3: extrapolate
+ * (alpha: st.alpha_given 1 ctl->alpha, 2 from part[0..1]), A = M(.), stop
+ * test into ctl->stopbad.  alpha: tree partials (nparts pairs) ->
+ * ctl->alpha.  rows: out[t] = A[txq[t]].  lds: dynamic LDS of one tile. */
+size_t hemk_ptile_lds(const hemk_tiles *t);
+int hemk_ptile_map12(const hemk_tiles *t, const double *A, double *B,
+                     double *C, const uint32_t *uq, const double *cntq,
+                     double *xr, double *xv, const hemk_ctl *ctl, void *s);
+int hemk_ptile_ex3(const hemk_tiles *t, double *A, const double *B,
+                   const double *C, const uint32_t *uq, const double *cntq,
+                   const double *part, uint32_t T, hemk_step st,
+                   hemk_ctl *ctl, void *s);
+int hemk_ptile_alpha(const double *part, uint32_t nparts, uint32_t T,
+                     hemk_step st, hemk_ctl *ctl, void *s);
+int hemk_ptile_rows(uint32_t T, const uint32_t *txq, const double *A,
+                    double *out, void *s);
 
 /* ---- exact left-to-right sums in parallel (spec_sum.hip) -----------------
  * Chains of n FP64 values; each chain's result is bitwise the sequential
@@ -321,9 +344,8 @@ int hemk_spec_sum(const hemk_spec *sp, int produce, const double *u,
                   void *s);
 
 /* End of one iteration: k++, evaluate the loop condition of src/EM.c:328
- * from the stop flags of the third map. */
-int hemk_iter_end(hemk_ctl *ctl, const uint32_t *stop_flags,
-                  uint32_t n_flags, void *s);
+ * from ctl->stopbad (the third map's stop test), clear it. */
+int hemk_iter_end(hemk_ctl *ctl, void *s);
 
 /* Finalisation (src/EM.c:362-373): est_t = lam_t len_t, tpm_t =
  * lam_t 1e6 / sum lam.  part: HEMK_NORM_BLOCKS doubles of scratch. */

@@ -1,0 +1,323 @@
+// ptile_kernels.hip -- one EM instance over component tiles (gfx950).
+//
+// The plain solve's maps without the two global passes: a 64-lane
+// workgroup owns one tile (whole connected components of the class graph,
+// host/tiles.c) and keeps its transcripts, classes and members in LDS.  One
+// launch applies the map twice (u -> u1 -> u2, src/EM.c:330-331), the other
+// extrapolates (src/EM.c:333-352) and applies it once more with the stop
+// test (src/EM.c:354-355).  Every sum runs in the reference's order:
+// members in class order for D_c (src/EM.c:277-278), then for each
+// transcript its terms (lam_t n_c) / D_c in ascending class order
+// (src/EM.c:282-285, the order hera's class loop adds into overlap[t]),
+// then (u_t + overlap_t) / len_t (src/EM.c:289-290) -- so every iterate is
+// bitwise the reference's.  State vectors are in tile order (a tile's
+// transcripts are consecutive); r and v are written in transcript order
+// for the exact sums.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "em_step.h"
+#include "hem_dev.h"
+
+#define PBLOCK 64
+
+namespace {
+
+struct PTile {
+	uint32_t tx_base, nt, cls_base, nc, mb, nz, jb;
+};
+
+__device__ __forceinline__ PTile ptile(const hemk_tiles &t, uint32_t i)
+{
+	const uint32_t *d = t.desc + (size_t)HEMK_TILE_DESC * i;
+	PTile c;
+	c.tx_base = d[0];
+	c.nt = d[1];
+	c.cls_base = d[2];
+	c.nc = d[3];
+	c.mb = d[4];
+	c.nz = d[5];
+	c.jb = d[6];
+	return c;
+}
+
+// The tile in LDS, sized by the tiling's largest tile.
+struct PLds {
+	double *l0, *l1, *l2;  // densities in, after one map, after two
+	double *D, *X, *n;     // class sums, member terms, class counts
+	uint32_t *u, *len, *tx, *kmem;
+	uint16_t *coff, *toff, *tpos;
+};
+
+__device__ __forceinline__ PLds pcarve(void *smem, const hemk_tiles &tl)
+{
+	const uint32_t mt = tl.max_tx, mc = tl.max_cls, mz = tl.max_nnz;
+	PLds L;
+	L.l0 = (double *)smem;
+	L.l1 = L.l0 + mt;
+	L.l2 = L.l1 + mt;
+	L.D = L.l2 + mt;
+	L.X = L.D + mc;
+	L.n = L.X + mz;
+	L.u = (uint32_t *)(L.n + mc);
+	L.len = L.u + mt;
+	L.tx = L.len + mt;
+	L.kmem = L.tx + mt;
+	L.coff = (uint16_t *)(L.kmem + mz);
+	L.toff = L.coff + (mc + 1);
+	L.tpos = L.toff + (mt + 1);
+	return L;
+}
+
+// Structure, counts and lengths of the tile into LDS.
+__device__ __forceinline__ void pstage(const hemk_tiles &tl, const PTile &tc,
+                                       const PLds &L,
+                                       const uint32_t *__restrict__ uq,
+                                       const double *__restrict__ cntq)
+{
+	const int tid = threadIdx.x;
+	for (uint32_t i = tid; i <= tc.nc; i += PBLOCK)
+		L.coff[i] = (uint16_t)(tl.coff[tc.cls_base + i] - tc.mb);
+	for (uint32_t i = tid; i < tc.nc; i += PBLOCK)
+		L.n[i] = cntq[tc.cls_base + i];
+	for (uint32_t i = tid; i <= tc.nt; i += PBLOCK)
+		L.toff[i] = (uint16_t)(tl.toff[tc.tx_base + i] - tc.jb);
+	for (uint32_t i = tid; i < tc.nz; i += PBLOCK) {
+		L.kmem[i] = tl.kmem[tc.mb + i];
+		L.tpos[i] = tl.tpos[tc.jb + i];
+	}
+	for (uint32_t i = tid; i < tc.nt; i += PBLOCK) {
+		L.tx[i] = tl.tx[tc.tx_base + i];
+		L.len[i] = tl.len[tc.tx_base + i];
+		L.u[i] = uq[tc.tx_base + i];
+	}
+}
+
+// One map lin -> lout of the tile (src/EM.c:275-291) in three passes;
+// with stop the test of src/EM.c:292-297 against lin sets bad.
+__device__ __forceinline__ void pmap(const PLds &L, const PTile &tc,
+                                     const double *lin, double *lout,
+                                     bool stop, double smin, double srel,
+                                     int &bad)
+{
+#pragma clang fp contract(off)
+	const int tid = threadIdx.x;
+	for (uint32_t c = tid; c < tc.nc; c += PBLOCK) {
+		uint32_t k = L.coff[c];
+		const uint32_t k1 = L.coff[c + 1];
+		double acc = 0.0;
+		for (; k + 4 <= k1; k += 4) {
+			const double x0 = lin[L.kmem[k] & 0xFFu];
+			const double x1 = lin[L.kmem[k + 1] & 0xFFu];
+			const double x2 = lin[L.kmem[k + 2] & 0xFFu];
+			const double x3 = lin[L.kmem[k + 3] & 0xFFu];
+			acc += x0;
+			acc += x1;
+			acc += x2;
+			acc += x3;
+		}
+		for (; k < k1; ++k)
+			acc += lin[L.kmem[k] & 0xFFu];
+		L.D[c] = acc;
+	}
+	__syncthreads();
+	// (lam_t n_c) / D_c per member; a class with D_c == 0 is skipped by
+	// the reference, which a zero term reproduces (sums start at +0 and
+	// every term is >= 0)
+	for (uint32_t k = tid; k < tc.nz; k += PBLOCK) {
+		const uint32_t m = L.kmem[k];
+		const uint32_t c = m >> 8, t = m & 0xFFu;
+		const double d = L.D[c];
+		const double x = lin[t] * L.n[c] / (d == 0.0 ? 1.0 : d);
+		L.X[k] = d == 0.0 ? 0.0 : x;
+	}
+	__syncthreads();
+	for (uint32_t t = tid; t < tc.nt; t += PBLOCK) {
+		uint32_t j = L.toff[t];
+		const uint32_t j1 = L.toff[t + 1];
+		double acc = 0.0;
+		for (; j + 8 <= j1; j += 8) {
+			double x[8];
+#pragma unroll
+			for (int q = 0; q < 8; ++q)
+				x[q] = L.X[L.tpos[j + q]];
+#pragma unroll
+			for (int q = 0; q < 8; ++q)
+				acc += x[q];
+		}
+		for (; j < j1; ++j)
+			acc += L.X[L.tpos[j]];
+		const double nw = ((double)L.u[t] + acc) / (double)L.len[t];
+		lout[t] = nw;
+		if (stop && nw > smin && fabs(nw - lin[t]) / nw > srel)
+			bad = 1;
+	}
+	__syncthreads();
+}
+
+// u1 = M(A) -> B, u2 = M(u1) -> C; r = u1 - u, v = (u2 - u1) - r into the
+// rows xr, xv (transcript order) when xr != NULL.
+__global__ __launch_bounds__(PBLOCK) void k_ptile_map12(
+    hemk_tiles tl, const double *__restrict__ A, double *__restrict__ B,
+    double *__restrict__ C, const uint32_t *__restrict__ uq,
+    const double *__restrict__ cntq, double *__restrict__ xr,
+    double *__restrict__ xv, const hemk_ctl *__restrict__ ctl)
+{
+#pragma clang fp contract(off)
+	extern __shared__ double smem[];
+	if (ctl->done)
+		return;
+	const PTile tc = ptile(tl, blockIdx.x);
+	const PLds L = pcarve(smem, tl);
+	pstage(tl, tc, L, uq, cntq);
+	for (uint32_t i = threadIdx.x; i < tc.nt; i += PBLOCK)
+		L.l0[i] = A[tc.tx_base + i];
+	__syncthreads();
+	int bad = 0;
+	pmap(L, tc, L.l0, L.l1, false, 0.0, 0.0, bad);
+	pmap(L, tc, L.l1, L.l2, false, 0.0, 0.0, bad);
+	for (uint32_t i = threadIdx.x; i < tc.nt; i += PBLOCK) {
+		const double a = L.l0[i], b = L.l1[i], c = L.l2[i];
+		B[tc.tx_base + i] = b;
+		C[tc.tx_base + i] = c;
+		if (xr) {
+			const double r = b - a;
+			xr[L.tx[i]] = r;
+			xv[L.tx[i]] = (c - b) - r;
+		}
+	}
+}
+
+// Extrapolate from A, B, C with alpha (st.alpha_given 1: ctl->alpha; 2:
+// from the two sums part[0], part[1]), map once -> A with the stop test
+// (ORed into ctl->stopbad).
+__global__ __launch_bounds__(PBLOCK) void k_ptile_ex3(
+    hemk_tiles tl, double *__restrict__ A, const double *__restrict__ B,
+    const double *__restrict__ C, const uint32_t *__restrict__ uq,
+    const double *__restrict__ cntq, const double *__restrict__ part,
+    uint32_t T, hemk_step st, hemk_ctl *__restrict__ ctl)
+{
+#pragma clang fp contract(off)
+	extern __shared__ double smem[];
+	if (ctl->done)
+		return;
+	const double alpha = st.alpha_given == 2
+	                         ? em_step_length(part[0], part[1], T, st)
+	                         : ctl->alpha;
+	const double smin = ctl->stop_min, srel = ctl->stop_rel;
+	const PTile tc = ptile(tl, blockIdx.x);
+	const PLds L = pcarve(smem, tl);
+	pstage(tl, tc, L, uq, cntq);
+	for (uint32_t i = threadIdx.x; i < tc.nt; i += PBLOCK) {
+		const size_t g = tc.tx_base + i;
+		L.l0[i] = em_extrapolate(A[g], B[g], C[g], alpha, st);
+	}
+	__syncthreads();
+	int bad = 0;
+	pmap(L, tc, L.l0, L.l1, true, smin, srel, bad);
+	for (uint32_t i = threadIdx.x; i < tc.nt; i += PBLOCK)
+		A[tc.tx_base + i] = L.l1[i];
+	bad = __syncthreads_or(bad);
+	if (threadIdx.x == 0) {
+		if (bad)
+			em_mark_bad(&ctl->stopbad);
+		if (blockIdx.x == 0 && st.alpha_given == 2)
+			ctl->alpha = alpha;
+	}
+}
+
+// Tree norms: alpha from the block partials of the norm passes.
+__global__ __launch_bounds__(256) void k_ptile_alpha(
+    const double *__restrict__ part, uint32_t nparts, uint32_t T,
+    hemk_step st, hemk_ctl *__restrict__ ctl)
+{
+	__shared__ double sa[4], sb[4];
+	if (ctl->done)
+		return;
+	double a = 0.0, b = 0.0;
+	for (uint32_t i = threadIdx.x; i < nparts; i += 256) {
+		a += part[2 * i];
+		b += part[2 * i + 1];
+	}
+#pragma unroll
+	for (int m = 32; m >= 1; m >>= 1) {
+		a += __shfl_xor(a, m, 64);
+		b += __shfl_xor(b, m, 64);
+	}
+	if (threadIdx.x % 64 == 0) {
+		sa[threadIdx.x / 64] = a;
+		sb[threadIdx.x / 64] = b;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		a = sa[0];
+		b = sb[0];
+		for (int w = 1; w < 4; ++w) {
+			a += sa[w];
+			b += sb[w];
+		}
+		ctl->alpha = em_step_length(a, b, T, st);
+	}
+}
+
+// out[t] = A[txq[t]]: the state in transcript order.
+__global__ __launch_bounds__(256) void k_ptile_rows(
+    uint32_t T, const uint32_t *__restrict__ txq, const double *__restrict__ A,
+    double *__restrict__ out)
+{
+	for (uint32_t t = blockIdx.x * 256 + threadIdx.x; t < T;
+	     t += gridDim.x * 256)
+		out[t] = A[txq[t]];
+}
+
+} // namespace
+
+extern "C" {
+
+size_t hemk_ptile_lds(const hemk_tiles *t)
+{
+	const size_t mt = t->max_tx, mc = t->max_cls, mz = t->max_nnz;
+	return 8 * (3 * mt + 2 * mc + mz) + 4 * (3 * mt + mz) +
+	       2 * ((mc + 1) + (mt + 1) + mz);
+}
+
+int hemk_ptile_map12(const hemk_tiles *t, const double *A, double *B,
+                     double *C, const uint32_t *uq, const double *cntq,
+                     double *xr, double *xv, const hemk_ctl *ctl, void *s)
+{
+	if (!t->n_tiles || t->max_tx > 256)
+		return (int)hipErrorInvalidValue;
+	k_ptile_map12<<<t->n_tiles, PBLOCK, hemk_ptile_lds(t),
+	                (hipStream_t)s>>>(*t, A, B, C, uq, cntq, xr, xv, ctl);
+	return (int)hipGetLastError();
+}
+
+int hemk_ptile_ex3(const hemk_tiles *t, double *A, const double *B,
+                   const double *C, const uint32_t *uq, const double *cntq,
+                   const double *part, uint32_t T, hemk_step st,
+                   hemk_ctl *ctl, void *s)
+{
+	if (!t->n_tiles || t->max_tx > 256)
+		return (int)hipErrorInvalidValue;
+	k_ptile_ex3<<<t->n_tiles, PBLOCK, hemk_ptile_lds(t), (hipStream_t)s>>>(
+	    *t, A, B, C, uq, cntq, part, T, st, ctl);
+	return (int)hipGetLastError();
+}
+
+int hemk_ptile_alpha(const double *part, uint32_t nparts, uint32_t T,
+                     hemk_step st, hemk_ctl *ctl, void *s)
+{
+	k_ptile_alpha<<<1, 256, 0, (hipStream_t)s>>>(part, nparts, T, st, ctl);
+	return (int)hipGetLastError();
+}
+
+int hemk_ptile_rows(uint32_t T, const uint32_t *txq, const double *A,
+                    double *out, void *s)
+{
+	k_ptile_rows<<<(T + 255) / 256 < 2048 ? (T + 255) / 256 : 2048, 256, 0,
+	               (hipStream_t)s>>>(T, txq, A, out);
+	return (int)hipGetLastError();
+}
+
+} // extern "C"

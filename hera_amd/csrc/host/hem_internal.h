@@ -83,7 +83,11 @@ struct hem_problem {
 	double *d_A, *d_B, *d_C, *d_D;
 	float *d_A32, *d_B32, *d_D32;
 	void *d_rho;
-	uint32_t *d_unique, *d_flags;
+	uint32_t *d_unique;
+	/* plain solve over component tiles: unique counts and class counts
+	 * in tile order (state vectors A, B, C are then in tile order too) */
+	uint32_t *d_uq;
+	double *d_cntq;
 	double *d_part1, *d_part2, *d_out;
 	/* exact sums by speculation: r/v rows [2][T], scratch per chain
 	 * (two chains), and the sums (pass 1 at [0..1], pass 2 at [2..3]) */
@@ -92,7 +96,6 @@ struct hem_problem {
 	long long *d_sbase;
 	hemk_ctl *d_ctl;
 	hemk_ctl *h_ctl; /* pinned */
-	uint32_t n_flags;
 
 	/* component tiles (n_tiles == 0: some component is too large) */
 	hemk_tiles dt;
@@ -144,7 +147,10 @@ void hem_tiles_free_host(hem_tiles_host *h);
 
 /* kernel kinds for per-kernel timing (hem_kernel_profile) */
 enum { HEM_PK_DENOM = 0, HEM_PK_GATHER, HEM_PK_NORM1, HEM_PK_NORM2,
-       HEM_PK_EXTRAP, HEM_PK_ITER_END };
+       HEM_PK_EXTRAP, HEM_PK_ITER_END, HEM_PK_TMAP12, HEM_PK_TEX3 };
+/* Largest dynamic LDS of the plain tile kernels; larger tiles keep to the
+ * two-pass kernels. */
+#define HEM_PTILE_LDS_MAX (64 * 1024)
 
 /* batch.c */
 void hem_batch_free(hem_problem *p);

@@ -173,10 +173,7 @@ int hem_problem_create(const hem_classes *c, hem_problem **out)
 	DALLOC_H(p->d_D32, (size_t)T * 4);
 	DALLOC_H(p->d_rho, (C ? C : 1) * 8);
 	DALLOC_H(p->d_unique, (size_t)T * 4);
-	/* one stop flag per block of the transcript pass; the widest lane
-	 * group (64 lanes per transcript) launches the most blocks */
-	p->n_flags = hemk_gather_blocks(&(hemk_prob){.n_tx = T}, 64);
-	DALLOC_H(p->d_flags, (size_t)p->n_flags * 4);
+	DALLOC_H(p->d_uq, (size_t)T * 4);
 	DALLOC_H(p->d_part1, HEMK_NORM_BLOCKS * 2 * 8);
 	DALLOC_H(p->d_part2, HEMK_NORM_BLOCKS * 2 * 8);
 	DALLOC_H(p->d_out, 64);
@@ -255,6 +252,10 @@ int hem_problem_create(const hem_classes *c, hem_problem **out)
 		                          c->tx_len, tx_off, &th);
 		if (trc == HEM_OK) {
 			trc = hem_tiles_upload(p, &th);
+			if (trc == HEM_OK)
+				trc = dalloc(p, (void **)&p->d_cntq,
+				             (size_t)(th.n_cls ? th.n_cls : 1) *
+				                 8);
 			p->h_cls_orig = th.cls_orig;
 			th.cls_orig = NULL;
 			hem_tiles_free_host(&th);
@@ -327,7 +328,8 @@ int hem_problem_destroy(hem_problem *p)
 	hemd_free(p->d_D32);
 	hemd_free(p->d_rho);
 	hemd_free(p->d_unique);
-	hemd_free(p->d_flags);
+	hemd_free(p->d_uq);
+	hemd_free(p->d_cntq);
 	hemd_free(p->d_part1);
 	hemd_free(p->d_part2);
 	hemd_free(p->d_out);

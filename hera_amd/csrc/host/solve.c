@@ -113,9 +113,81 @@ static hemk_spec spec_plain(const hem_problem *p, uint32_t nchain, int xf,
 	return sp;
 }
 
+/* FP64 solves run over the component tiles when every component fits a
+ * tile (one kernel for u -> u1 -> u2, one for the extrapolated map). */
+static int use_tiles(const hem_problem *p, const hem_opts *o)
+{
+	return p->dt.n_tiles && o->precision == HEM_PREC_FP64 &&
+	       (o->norm == HEM_NORM_EXACT || o->norm == HEM_NORM_TREE) &&
+	       hemk_ptile_lds(&p->dt) <= HEM_PTILE_LDS_MAX;
+}
+
+/* One SQUAREM iteration over the tiles. */
+static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
+                                   prof_ctx *pc)
+{
+	const int exact = o->norm == HEM_NORM_EXACT;
+	const int sd = o->step == HEM_STEP_SD;
+	const uint32_t T = p->n_tx;
+	hemk_step st = {o->step, o->clamp, o->alpha_clamp != 0, exact ? 2 : 1};
+	const double *alpha_part = NULL;
+	void *s = p->stream;
+
+	HEM_TRY_DEV(hemk_ptile_map12(&p->dt, p->d_A, p->d_B, p->d_C, p->d_uq,
+	                             p->d_cntq, exact ? p->d_sx : NULL,
+	                             exact ? p->d_sx + T : NULL, p->d_ctl, s),
+	            "tile maps");
+	MARK(HEM_PK_TMAP12);
+	if (exact) {
+		/* the reference's left-to-right sums over the r and v rows */
+		hemk_spec sp = spec_plain(p, 2,
+		                          sd ? HEMK_SPEC_IDENT : HEMK_SPEC_SQUARE,
+		                          p->d_sx, NULL, p->d_sres);
+		HEM_TRY_DEV(hemk_spec_sum(&sp, 0, NULL, NULL, NULL, p->d_ctl, s),
+		            "exact sums 1");
+		MARK(HEM_PK_NORM1);
+		alpha_part = p->d_sres;
+		if (sd) {
+			sp = spec_plain(p, 2, HEMK_SPEC_CENTRED, p->d_sx,
+			                p->d_sres, p->d_sres + 2);
+			HEM_TRY_DEV(hemk_spec_sum(&sp, 0, NULL, NULL, NULL,
+			                          p->d_ctl, s),
+			            "exact sums 2");
+			MARK(HEM_PK_NORM2);
+			alpha_part = p->d_sres + 2;
+		}
+	} else {
+		HEM_TRY_DEV(hemk_norm_pass1(T, p->d_A, p->d_B, p->d_C,
+		                            p->d_part1, o->step, p->d_ctl, s),
+		            "norm1");
+		if (sd) {
+			MARK(HEM_PK_NORM1);
+			HEM_TRY_DEV(hemk_norm_pass2(T, p->d_A, p->d_B, p->d_C,
+			                            p->d_part1, p->d_part2,
+			                            o->step, p->d_ctl, s),
+			            "norm2");
+		}
+		HEM_TRY_DEV(hemk_ptile_alpha(sd ? p->d_part2 : p->d_part1,
+		                             HEMK_NORM_BLOCKS, T, st, p->d_ctl,
+		                             s),
+		            "alpha");
+		MARK(sd ? HEM_PK_NORM2 : HEM_PK_NORM1);
+	}
+	HEM_TRY_DEV(hemk_ptile_ex3(&p->dt, p->d_A, p->d_B, p->d_C, p->d_uq,
+	                           p->d_cntq, alpha_part, T, st, p->d_ctl, s),
+	            "tile extrapolated map");
+	MARK(HEM_PK_TEX3);
+	HEM_TRY_DEV(hemk_iter_end(p->d_ctl, s), "iter_end");
+	MARK(HEM_PK_ITER_END);
+	return HEM_OK;
+}
+
 /* Enqueue one SQUAREM iteration on the problem's stream. */
 static int enqueue_iteration(hem_problem *p, const hem_opts *o, prof_ctx *pc)
 {
+	if (use_tiles(p, o))
+		return enqueue_iteration_tiles(p, o, pc);
+	{
 	const int f32 = o->precision == HEM_PREC_FP32;
 	const int prec = f32 ? 0 : 1;
 	const void *gA = f32 ? (const void *)p->d_A32 : (const void *)p->d_A;
@@ -130,13 +202,7 @@ static int enqueue_iteration(hem_problem *p, const hem_opts *o, prof_ctx *pc)
 	                                                   : p->d_part1;
 	const hemk_prob *dp = &p->dp;
 	void *s = p->stream;
-	uint32_t nfl = hemk_gather_blocks(dp, p->group_tx);
 	int gc = p->group_cls, gt = p->group_tx;
-
-	if (nfl > p->n_flags)
-		return hem_fail(HEM_ERR_RANGE,
-		                "transcript pass grid %u exceeds %u stop flags",
-		                nfl, p->n_flags);
 
 	HEM_TRY_DEV(hemk_denom(dp, prec, gA, rho, gc, p->d_ctl, s),
 	            "denom");
@@ -216,12 +282,13 @@ static int enqueue_iteration(hem_problem *p, const hem_opts *o, prof_ctx *pc)
 	MARK(HEM_PK_DENOM);
 	HEM_TRY_DEV(hemk_gather_mstep(dp, prec, rho, p->d_unique, p->d_D,
 	                              p->d_A, f32 ? p->d_A32 : NULL,
-	                              p->d_flags, gt, p->d_ctl, s),
+	                              &p->d_ctl->stopbad, gt, p->d_ctl, s),
 	            "gather");
 	MARK(HEM_PK_GATHER);
-	HEM_TRY_DEV(hemk_iter_end(p->d_ctl, p->d_flags, nfl, s), "iter_end");
+	HEM_TRY_DEV(hemk_iter_end(p->d_ctl, s), "iter_end");
 	MARK(HEM_PK_ITER_END);
 	return HEM_OK;
+	}
 }
 
 /* Returns the iteration graph for these options, capturing it on first
@@ -297,6 +364,14 @@ static int solve_setup(hem_problem *p, const uint32_t *unique,
                        const hem_opts *o)
 {
 	int rc = hemd_h2d(p->d_unique, unique, (size_t)p->n_tx * 4, p->stream);
+	/* tile order: unique counts by tile position, class counts by class
+	 * tile position (the uniform start is the same in any order) */
+	if (!rc && use_tiles(p, o))
+		rc = hemk_gather_rows_u32(p->d_unique, p->dt.tx, p->n_tx, 1,
+		                          p->d_uq, p->stream);
+	if (!rc && use_tiles(p, o))
+		rc = hemk_gather_f64(p->d_cnt64, p->d_cls_orig, p->dt.n_cls,
+		                     p->d_cntq, p->stream);
 	if (!rc)
 		rc = hemk_init_state(p->n_tx, p->d_A, p->d_A32,
 		                     (double)1 / p->n_tx, NULL, p->stream);
@@ -323,6 +398,7 @@ int hem_solve(hem_problem *p, const uint32_t *unique, const hem_opts *opts,
 	int rc;
 	float ms = 0;
 	double t0 = now_ms(), ll = 0;
+	const double *lam;
 	if (!p || !unique)
 		return hem_fail(HEM_ERR_INVALID, "null problem or counts");
 	if ((rc = load_opts(opts, &o)))
@@ -333,14 +409,23 @@ int hem_solve(hem_problem *p, const uint32_t *unique, const hem_opts *opts,
 		goto out;
 	if ((rc = run_loop(p, &o, &ms)))
 		goto out;
-	/* finalisation into B (est) and C (tpm); log-likelihood of A */
-	if (o.norm == HEM_NORM_EXACT) {
+	/* finalisation into B (est) and C (tpm); log-likelihood of the final
+	 * state lam (A, or its transcript-order copy in D after a tiled
+	 * solve) */
+	lam = p->d_A;
+	if (use_tiles(p, &o)) {
+		rc = hemk_ptile_rows(p->n_tx, p->dt.txq, p->d_A, p->d_D,
+		                     p->stream);
+		lam = p->d_D;
+	}
+	if (rc) {
+	} else if (o.norm == HEM_NORM_EXACT) {
 		/* sum of the final state (src/EM.c:362-363) into lam_sum */
-		hemk_spec sp = spec_plain(p, 1, HEMK_SPEC_IDENT, p->d_A, NULL,
+		hemk_spec sp = spec_plain(p, 1, HEMK_SPEC_IDENT, lam, NULL,
 		                          &p->d_ctl->lam_sum);
 		rc = hemk_spec_sum(&sp, 0, NULL, NULL, NULL, NULL, p->stream);
 		if (!rc)
-			rc = hemk_finalize_seq(p->n_tx, p->d_A, p->d_len, p->d_ctl,
+			rc = hemk_finalize_seq(p->n_tx, lam, p->d_len, p->d_ctl,
 			                       p->d_C, p->d_B, p->stream);
 	} else if (o.norm == HEM_NORM_SEQ) {
 		hemk_step st0 = {0, 0, 0, 0};
@@ -350,18 +435,18 @@ int hem_solve(hem_problem *p, const uint32_t *unique, const hem_opts *opts,
 			rc = hemk_finalize_seq(p->n_tx, p->d_A, p->d_len, p->d_ctl,
 			                       p->d_C, p->d_B, p->stream);
 	} else {
-		rc = hemk_finalize(p->n_tx, p->d_A, p->d_len, p->d_part1,
+		rc = hemk_finalize(p->n_tx, lam, p->d_len, p->d_part1,
 		                   p->d_C, p->d_B, p->stream);
 	}
 	if (!rc)
-		rc = hemk_loglik(&p->dp, p->d_unique, p->d_A, p->d_part2,
+		rc = hemk_loglik(&p->dp, p->d_unique, lam, p->d_part2,
 		                 p->d_out, p->stream);
 	if (!rc && tpm)
 		rc = hemd_d2h(tpm, p->d_C, (size_t)p->n_tx * 8, p->stream);
 	if (!rc && est)
 		rc = hemd_d2h(est, p->d_B, (size_t)p->n_tx * 8, p->stream);
 	if (!rc && lambda)
-		rc = hemd_d2h(lambda, p->d_A, (size_t)p->n_tx * 8, p->stream);
+		rc = hemd_d2h(lambda, lam, (size_t)p->n_tx * 8, p->stream);
 	if (!rc)
 		rc = hemd_d2h(p->h_ctl, p->d_ctl, sizeof(hemk_ctl), p->stream);
 	if (!rc)
@@ -412,7 +497,17 @@ static void kernel_bytes(const hem_problem *p, int f32, double *b)
 	b[HEM_PK_NORM1] = 24 * T;   /* u, u1, u2 */
 	b[HEM_PK_NORM2] = 24 * T;
 	b[HEM_PK_EXTRAP] = 24 * T + 8 * T + (f32 ? 4 * T : 0);
-	b[HEM_PK_ITER_END] = 4.0 * hemk_gather_blocks(&p->dp, p->group_tx);
+	b[HEM_PK_ITER_END] = 8;
+	{
+		/* tile kernels: each tile's structure once (descriptor, class
+		 * and transcript offsets, members, occurrence positions, ids,
+		 * lengths, unique and class counts), then the state */
+		const double NT = p->dt.n_tiles, TC = p->dt.n_cls;
+		const double S = 32 * NT + 4 * (TC + NT) + 4 * (T + NT) +
+		                 6 * Z + 12 * T + 8 * TC;
+		b[HEM_PK_TMAP12] = S + 8 * T + 16 * T + 16 * T;
+		b[HEM_PK_TEX3] = S + 24 * T + 8 * T;
+	}
 }
 
 int hem_profile_solve(hem_problem *p, const uint32_t *unique,

@@ -212,9 +212,10 @@ int hem_bootstrap_counts(hem_problem *p, const uint32_t *unique,
  * graph, with a HIP event after every launch on the solve's stream, and
  * reports the device time and algorithmic bytes of each kernel kind.
  * Kinds: 0 class pass (denominators), 1 transcript pass (gather + M-step),
- * 2 norm pass 1, 3 norm pass 2 (tree partial sums, or the two sequential
- * passes), 4 extrapolation, 5 iteration end. */
-#define HEM_KERNEL_KINDS 6
+ * 2 norm pass 1, 3 norm pass 2 (tree partial sums, or the exact sums),
+ * 4 extrapolation, 5 iteration end, 6 tile maps u -> u1 -> u2 (FP64 solves
+ * over component tiles), 7 tile extrapolation + map + stop test. */
+#define HEM_KERNEL_KINDS 8
 typedef struct {
 	uint32_t iters;
 	uint32_t launches[HEM_KERNEL_KINDS];

@@ -151,6 +151,37 @@ class Problem:
         self._h = _lib.C.c_void_p()
         check(lib.hem_problem_create(self._c, self._h))
         self.n_tx = classes.n_tx
+        self.n_samples = 1
+
+    @classmethod
+    def from_samples(cls, samples):
+        """Ragged multi-sample handle (hem_samples_create): the samples'
+        tables side by side; batched solves and bootstraps run every
+        sample's replicates together.  Count and output rows span all
+        samples' transcripts (sample q at columns tx0[q]:tx0[q+1]); seeds and
+        iteration counts are per (sample, replicate)."""
+        self = cls.__new__(cls)
+        lib = _lib.load()
+        arr = (_lib.HemClasses * len(samples))(*[c.to_c() for c in samples])
+        self._samples = samples
+        self._h = _lib.C.c_void_p()
+        check(lib.hem_samples_create(arr, len(samples), self._h))
+        self.n_samples = len(samples)
+        self.tx0 = np.concatenate([[0], np.cumsum([c.n_tx for c in samples])])
+        self.cls0 = np.concatenate([[0], np.cumsum([c.n_cls for c in samples])])
+        self.n_tx = int(self.tx0[-1])
+        off = [np.zeros(1, np.uint32)]
+        base = 0
+        for c in samples:
+            off.append(np.asarray(c.cls_off[1:], np.uint64) + base)
+            base += int(c.cls_off[-1]) if c.n_cls else 0
+        self.classes = Classes(
+            self.n_tx, np.concatenate(off).astype(np.uint32),
+            np.concatenate([np.asarray(c.cls_ref, np.uint32) + np.uint32(t)
+                            for c, t in zip(samples, self.tx0[:-1])]),
+            np.concatenate([c.cls_count for c in samples]),
+            np.concatenate([c.tx_len for c in samples]))
+        return self
 
     def close(self):
         if self._h:
@@ -234,11 +265,13 @@ class Problem:
         o = opts(**kw)
         est = np.empty((R, self.n_tx))
         tpm = np.empty((R, self.n_tx))
-        it = np.empty(R, dtype=np.uint32)
+        it = np.empty(self.n_samples * R, dtype=np.uint32)
         st = _lib.HemBatchStats()
         check(_lib.load().hem_solve_batch(
             self._h, R, u32p(u), f64p(cc) if cc is not None else None, o,
             f64p(est), f64p(tpm), u32p(it), st))
+        if self.n_samples > 1:
+            it = it.reshape(self.n_samples, R)
         return dict(est=est, tpm=tpm, iters=it, stats=self._bstats(st))
 
     def bootstrap(self, unique, seeds, **kw):
@@ -246,22 +279,26 @@ class Problem:
         replicate.  Returns dict(est, tpm, iters, stats)."""
         u = as_u32(unique)
         sd = np.ascontiguousarray(seeds, dtype=np.uint64)
-        R = len(sd)
+        R = sd.shape[-1]
+        assert sd.size == self.n_samples * R
         o = opts(**kw)
         est = np.empty((R, self.n_tx))
         tpm = np.empty((R, self.n_tx))
-        it = np.empty(R, dtype=np.uint32)
+        it = np.empty(self.n_samples * R, dtype=np.uint32)
         st = _lib.HemBatchStats()
         check(_lib.load().hem_bootstrap(
             self._h, u32p(u), R, sd.ctypes.data_as(_lib.C.POINTER(_lib.C.c_uint64)),
             o, f64p(est), f64p(tpm), u32p(it), st))
+        if self.n_samples > 1:
+            it = it.reshape(self.n_samples, R)
         return dict(est=est, tpm=tpm, iters=it, stats=self._bstats(st))
 
     def bootstrap_counts(self, unique, seeds):
         """The resampled (unique, class) counts hem_bootstrap would solve."""
         u = as_u32(unique)
         sd = np.ascontiguousarray(seeds, dtype=np.uint64)
-        R = len(sd)
+        R = sd.shape[-1]
+        assert sd.size == self.n_samples * R
         uo = np.empty((R, self.n_tx), dtype=np.uint32)
         co = np.empty((R, self.classes.n_cls))
         check(_lib.load().hem_bootstrap_counts(

@@ -104,6 +104,8 @@ SIGNATURES = {
     "hem_opts_default": (None, [C.POINTER(HemOpts)]),
     "hem_problem_create": (C.c_int, [C.POINTER(HemClasses), C.POINTER(C.c_void_p)]),
     "hem_problem_destroy": (C.c_int, [C.c_void_p]),
+    "hem_samples_create": (C.c_int, [C.POINTER(HemClasses), C.c_uint32,
+                                     C.POINTER(C.c_void_p)]),
     "hem_problem_set_counts": (C.c_int, [C.c_void_p, _f64p]),
     "hem_problem_info": (C.c_int, [C.c_void_p, _u32p, _u32p,
                                    C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),

@@ -104,7 +104,11 @@ typedef struct {
 	const uint32_t *txq;    /* transcript id -> tile position            */
 } hemk_tiles;
 
-/* Per-replicate control of a batched solve (device arrays of length R). */
+/* Per-instance control of a batched solve.  A handle holds ns samples
+ * (independent class tables side by side: sample s owns transcript ids and
+ * tile positions [tx0_s, tx0_s + n_tx_s) and tiles [tile0_s, tile0_s +
+ * n_tiles_s)); each runs R replicates, and instance i = s * R + r has its
+ * own EM (control arrays of length ns * R, indexed by i). */
 typedef struct {
 	uint32_t R;
 	uint32_t min_iter, max_iter;
@@ -113,9 +117,16 @@ typedef struct {
 	uint32_t *k, *done, *conv, *notconv;
 	double   *alpha, *mean_r, *mean_v, *lam_sum;
 	double   *q_r, *q_v;  /* second-pass sums of the sequential norms */
-	uint32_t *active;   /* [1]: replicates still iterating */
-	uint32_t *alist;    /* [0]: count, then the replicates still
-	                       iterating, ascending                         */
+	uint32_t *active;   /* [ns]: replicates of each sample still
+	                       iterating */
+	uint32_t *alist;    /* per sample, R + 1 words: count, then the
+	                       replicates still iterating, ascending        */
+	uint32_t ns;              /* samples (1: a single class table)    */
+	const uint32_t *tsamp;    /* tile -> sample (NULL when ns == 1)   */
+	const uint32_t *txsamp;   /* transcript id or tile position ->
+	                             sample (NULL when ns == 1)           */
+	const uint32_t *sinfo;    /* per sample {tx0, n_tx, tile0, n_tiles}
+	                             (NULL when ns == 1)                  */
 } hemk_rctl;
 
 /* Largest replicate chunk (lanes per tile block) that fits the LDS budget
@@ -142,11 +153,12 @@ int hemk_tile_ex3(const hemk_tiles *t, hemk_rctl rc, int chunk,
                   double *A, const double *B, const double *C, uint32_t T,
                   hemk_step st, void *s);
 /* Sequential sums of a batch over replicate-major rows, one left-to-right
- * chain per (replicate, row) block, in the reference's order over t:
- *   phase 0: lam_sum[r] = sum of xr row r (every replicate)
- *   phase 1: mean_r/mean_v = sums / T
+ * chain per (instance, row) block over the instance's sample's transcripts
+ * (row r, ids tx0_s ..), in the reference's order over t:
+ *   phase 0: lam_sum[i] = sum of xr row (every instance)
+ *   phase 1: mean_r/mean_v = sums / n_tx_s
  *   phase 2: q_r/q_v = centred sums of squares    phase 3: sums of squares
- * Phases 1-3 skip finished replicates. */
+ * Phases 1-3 skip finished instances. */
 int hemk_batch_chain(hemk_rctl rc, uint32_t T, const double *xr,
                      const double *xv, int phase, void *s);
 /* Parallel norms of a batch: fold the tile partials of each replicate in
@@ -180,6 +192,11 @@ int hemk_resample(const uint32_t *w, uint32_t K, const uint32_t *chunk_start,
                   const uint64_t *seeds, uint32_t R, uint32_t *m,
                   uint32_t *out, void *s);
 int hemk_fill_u32(uint32_t *p, size_t n, uint32_t v, void *s);
+/* One sample's resampled rows in[R][Tq + Cq] into out[R][K]: unique counts
+ * at off_u.., class counts at off_c.. */
+int hemk_scatter_counts(const uint32_t *in, uint32_t R, uint32_t Tq,
+                        uint32_t Cq, uint32_t *out, uint32_t K,
+                        uint32_t off_u, uint32_t off_c, void *s);
 /* out[q*R + r] = in[idx[q]*R + r] */
 int hemk_gather_rows_u32(const uint32_t *in, const uint32_t *idx,
                          uint32_t rows, uint32_t R, uint32_t *out, void *s);

@@ -151,6 +151,37 @@ inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b
 
 extern "C" {
 
+// out[r][off_u + k] = in[r][k] (k < Tq) and out[r][off_c + k] =
+// in[r][Tq + k] (k < Cq): one sample's resampled rows into the handle's
+// unique and class blocks (rows K words long).
+__global__ __launch_bounds__(RBLOCK) void k_scatter_counts(
+    const uint32_t *__restrict__ in, uint32_t R, uint32_t Tq, uint32_t Cq,
+    uint32_t *__restrict__ out, uint32_t K, uint32_t off_u, uint32_t off_c)
+{
+	const uint32_t Kq = Tq + Cq;
+	const uint64_t n = (uint64_t)R * Kq;
+	for (uint64_t i = (uint64_t)blockIdx.x * RBLOCK + threadIdx.x; i < n;
+	     i += (uint64_t)gridDim.x * RBLOCK) {
+		const uint32_t r = (uint32_t)(i / Kq), k = (uint32_t)(i % Kq);
+		const uint32_t o = k < Tq ? off_u + k : off_c + (k - Tq);
+		out[(size_t)r * K + o] = in[i];
+	}
+}
+
+int hemk_scatter_counts(const uint32_t *in, uint32_t R, uint32_t Tq,
+                        uint32_t Cq, uint32_t *out, uint32_t K,
+                        uint32_t off_u, uint32_t off_c, void *s)
+{
+	uint32_t grid = cdiv((uint64_t)R * (Tq + Cq), RBLOCK);
+	if (!grid)
+		return 0;
+	if (grid > 16384)
+		grid = 16384;
+	k_scatter_counts<<<grid, RBLOCK, 0, (hipStream_t)s>>>(in, R, Tq, Cq, out,
+	                                                      K, off_u, off_c);
+	return (int)hipGetLastError();
+}
+
 int hemk_fill_u32(uint32_t *p, size_t n, uint32_t v, void *s)
 {
 	if (!n)

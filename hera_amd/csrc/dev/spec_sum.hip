@@ -366,13 +366,28 @@ __global__ __launch_bounds__(WAVE) void k_spec_window(hemk_spec sp,
 	// g1: the prefix up to j0 over the wave (identical in every lane),
 	// then this lane's chunk's few predecessors in the group
 	double g = 0.0;
-	for (uint32_t i = lane; i < j0; i += WAVE)
-		g += sp.drift[cb + i];
+	for (uint32_t i = lane; i < j0; i += 8 * WAVE) {
+		double d[8];
+#pragma unroll
+		for (int q = 0; q < 8; ++q)
+			d[q] = i + q * WAVE < j0 ? sp.drift[cb + i + q * WAVE] : 0.0;
+#pragma unroll
+		for (int q = 0; q < 8; ++q)
+			g += d[q];
+	}
 #pragma unroll
 	for (int k = WAVE / 2; k >= 1; k >>= 1)
 		g += __shfl_xor(g, k, WAVE);
-	for (uint32_t i = j0; i < j && i < sp.P; ++i)
-		g += sp.drift[cb + i];
+	{
+		double d[CPV];
+#pragma unroll
+		for (int q = 0; q < CPV; ++q)
+			d[q] = j0 + q < j && j0 + q < sp.P ? sp.drift[cb + j0 + q]
+			                                   : 0.0;
+#pragma unroll
+		for (int q = 0; q < CPV; ++q)
+			g += d[q];
+	}
 	const double m = chain_mean<XF>(sp, row);
 	const double *x = sp.x + (size_t)row * sp.ld;
 	const uint32_t nch = min((uint32_t)CPV, sp.P - j0);
@@ -436,6 +451,36 @@ __device__ __forceinline__ void repair(const hemk_spec &sp, uint32_t row,
 	__syncthreads();
 }
 
+// One chunk of the walk: look the running value up in the chunk's window
+// (row r, base b), else in the current repair's, else repair; then load r
+// and b with the chunk two ahead.
+template <int W, int XF>
+__device__ __forceinline__ void walk_step(
+    const hemk_spec &sp, uint32_t row, double m, uint32_t jb, uint32_t q,
+    uint32_t ns, uint32_t c, const double *tbl, const long long *bsl,
+    double *ys, double *rtl, long long *rbl, double &r, long long &b,
+    double &s, uint32_t &rj, bool &rep, uint32_t &misses)
+{
+	const uint32_t j = jb + q;
+	const unsigned long long ks = ukey(s);
+	uint32_t idx;
+	if (in_window(ks, readfirst_ll(b), W, idx)) {
+		s = readlane_d(r, idx);
+	} else if (rep && j - rj < RC &&
+	           in_window(ks, readfirst_ll(rbl[j - rj]), RW, idx)) {
+		s = readfirst_d(rtl[(j - rj) * RW + idx]);
+	} else {
+		repair<W, XF>(sp, row, bsl, jb, j, m, s, ys, rtl, rbl);
+		rj = j;
+		rep = true;
+		s = readfirst_d(rtl[0]);
+		++misses;
+	}
+	const uint32_t qn = min(q + 2, ns - 1);
+	r = tbl[qn * W + c];
+	b = bsl[qn];
+}
+
 // Step 4: walk the chunks in order, one workgroup per chain.  The chain's
 // candidate tables are loaded into LDS (a segment of chunks at a time, the
 // whole chain when it fits), then every wave walks them redundantly with
@@ -469,53 +514,43 @@ __global__ __launch_bounds__(SBLOCK) void k_spec_resolve(hemk_spec sp,
 		const uint32_t ns = min(SEG, P - jb);
 		__syncthreads();
 		{
+			// sixteen loads in flight per thread, the bases with them
 			const double *t = sp.tab + (cb + jb) * W;
 			const uint32_t cnt = ns * W;
-			uint32_t i = tid;
-			for (; i + 15 * SBLOCK < cnt; i += 16 * SBLOCK) {
+			long long bv[(SEG + SBLOCK - 1) / SBLOCK];
+#pragma unroll
+			for (int q = 0; q < (int)((SEG + SBLOCK - 1) / SBLOCK); ++q)
+				if (tid + q * SBLOCK < (int)ns)
+					bv[q] = sp.base[cb + jb + tid + q * SBLOCK];
+			for (uint32_t i = tid; i < cnt; i += 16 * SBLOCK) {
 				double v[16];
 #pragma unroll
 				for (int q = 0; q < 16; ++q)
-					v[q] = t[i + q * SBLOCK];
+					if (i + q * SBLOCK < cnt)
+						v[q] = t[i + q * SBLOCK];
 #pragma unroll
 				for (int q = 0; q < 16; ++q)
-					tbl[i + q * SBLOCK] = v[q];
+					if (i + q * SBLOCK < cnt)
+						tbl[i + q * SBLOCK] = v[q];
 			}
-			for (; i < cnt; i += SBLOCK)
-				tbl[i] = t[i];
-			for (uint32_t q = tid; q < ns; q += SBLOCK)
-				bsl[q] = sp.base[cb + jb + q];
+#pragma unroll
+			for (int q = 0; q < (int)((SEG + SBLOCK - 1) / SBLOCK); ++q)
+				if (tid + q * SBLOCK < (int)ns)
+					bsl[tid + q * SBLOCK] = bv[q];
 		}
 		__syncthreads();
-		// rows and bases of chunks q and q+1 in registers
-		double r0 = tbl[c], r1 = tbl[min(1u, ns - 1) * W + c];
-		long long b0 = readfirst_ll(bsl[0]);
-		long long b1 = bsl[min(1u, ns - 1)];
-		for (uint32_t q = 0; q < ns; ++q) {
-			const uint32_t j = jb + q;
-			const uint32_t q2 = min(q + 2, ns - 1);
-			const double r2 = tbl[q2 * W + c];
-			const long long b2 = bsl[q2];
-			const unsigned long long ks = ukey(s);
-			uint32_t idx;
-			if (in_window(ks, b0, W, idx)) {
-				s = readlane_d(r0, idx);
-			} else if (rep && j - rj < RC &&
-			           in_window(ks, readfirst_ll(rbl[j - rj]), RW,
-			                     idx)) {
-				s = readfirst_d(rtl[(j - rj) * RW + idx]);
-			} else {
-				repair<W, XF>(sp, row, bsl, jb, j, m, s, ys, rtl,
-				              rbl);
-				rj = j;
-				rep = true;
-				s = readfirst_d(rtl[0]);
-				++misses;
-			}
-			r0 = r1;
-			r1 = r2;
-			b0 = readfirst_ll(b1);
-			b1 = b2;
+		// two register pairs (row, base), each reloaded right after its
+		// chunk's lookup with the chunk two ahead, so a lookup waits only
+		// for reads issued two steps earlier
+		double ra = tbl[c], rb = tbl[min(1u, ns - 1) * W + c];
+		long long ba = bsl[0], bb = bsl[min(1u, ns - 1)];
+		for (uint32_t q = 0; q < ns; q += 2) {
+			walk_step<W, XF>(sp, row, m, jb, q, ns, c, tbl, bsl, ys, rtl,
+			                 rbl, ra, ba, s, rj, rep, misses);
+			if (q + 1 < ns)
+				walk_step<W, XF>(sp, row, m, jb, q + 1, ns, c, tbl, bsl,
+				                 ys, rtl, rbl, rb, bb, s, rj, rep,
+				                 misses);
 		}
 	}
 	if (tid == 0) {

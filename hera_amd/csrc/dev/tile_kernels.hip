@@ -216,26 +216,47 @@ struct ChunkRows {
 
 #define NOREP 0xFFFFFFFFu
 
-// The replicate of this thread's lane in chunk i of the active list.
+// The sample of a tile, its active list, its transcript count, its first
+// tile (a single table is sample 0 of ns == 1).
+__device__ __forceinline__ uint32_t tile_sample(const hemk_rctl &rc,
+                                                uint32_t tile)
+{
+	return rc.tsamp ? rc.tsamp[tile] : 0u;
+}
+
+__device__ __forceinline__ const uint32_t *sample_alist(const hemk_rctl &rc,
+                                                        uint32_t smp)
+{
+	return rc.alist + (size_t)smp * (rc.R + 1);
+}
+
+__device__ __forceinline__ uint32_t sample_T(const hemk_rctl &rc,
+                                             uint32_t smp, uint32_t T)
+{
+	return rc.sinfo ? rc.sinfo[4 * smp + 1] : T;
+}
+
+// The replicate of this thread's lane in chunk i of an active list.
 template <int RC>
-__device__ __forceinline__ uint32_t lane_rep(const hemk_rctl &rc,
+__device__ __forceinline__ uint32_t lane_rep(const uint32_t *al,
                                              uint32_t nrep, uint32_t i)
 {
 	const uint32_t j = i * RC + threadIdx.x % RC;
-	return j < nrep ? rc.alist[1 + j] : NOREP;
+	return j < nrep ? al[1 + j] : NOREP;
 }
 
 template <typename TC, int RC, int NS>
 __device__ __forceinline__ void load_rows(ChunkRows<TC, NS> &p,
                                           const TileCtx &tc,
-                                          const hemk_rctl &rc, uint32_t nrep,
+                                          const hemk_rctl &rc,
+                                          const uint32_t *al, uint32_t nrep,
                                           uint32_t i,
                                           const TC *__restrict__ cnt,
                                           const uint32_t *__restrict__ uniq,
                                           const double *s0, const double *s1,
                                           const double *s2)
 {
-	const uint32_t q = lane_rep<RC>(rc, nrep, i);
+	const uint32_t q = lane_rep<RC>(al, nrep, i);
 	const bool rin = q != NOREP;
 	p.q = q;
 #pragma unroll
@@ -311,7 +332,8 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
 	extern __shared__ double smem[];
 	__shared__ int alive[RC];
 	const uint32_t tile = blockIdx.x;
-	const uint32_t nrep = rc.alist[0];
+	const uint32_t *al = sample_alist(rc, tile_sample(rc, tile));
+	const uint32_t nrep = al[0];
 	if (nrep == 0)
 		return;
 	const uint32_t nch = (nrep + RC - 1) / RC;
@@ -325,7 +347,8 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
 	const uint32_t e0 = threadIdx.x / RC;
 	constexpr uint32_t ES = TBLOCK / RC;
 	ChunkRows<TC, 1> p;
-	load_rows<TC, RC, 1>(p, tc, rc, nrep, 0, cnt, uniq, A, nullptr, nullptr);
+	load_rows<TC, RC, 1>(p, tc, rc, al, nrep, 0, cnt, uniq, A, nullptr,
+	                     nullptr);
 	stage_struct(tl, tc, ts);
 	for (uint32_t i = 0; i < nch; ++i) {
 		const uint32_t q = p.q;
@@ -345,8 +368,8 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
 			alive[threadIdx.x] = rin ? 1 : 0;
 		__syncthreads();
 		if (i + 1 < nch)
-			load_rows<TC, RC, 1>(p, tc, rc, nrep, i + 1, cnt, uniq, A,
-			                     nullptr, nullptr);
+			load_rows<TC, RC, 1>(p, tc, rc, al, nrep, i + 1, cnt, uniq,
+			                     A, nullptr, nullptr);
 		tile_map<TC, RC>(tl, tc, ts, lam0, lam1, D, X, n, u, alive, B,
 		                 rc.R, q, nullptr, 0, 0);
 		tile_map<TC, RC>(tl, tc, ts, lam1, lam2, D, X, n, u, alive, C,
@@ -453,17 +476,19 @@ __device__ __forceinline__ double extrapolate(double a, double b, double c2,
 // The step length inputs of the lane's replicate (after load_rows).
 template <typename TC>
 __device__ __forceinline__ void load_alpha(ChunkRows<TC, 3> &p,
-                                           const hemk_rctl &rc, hemk_step st)
+                                           const hemk_rctl &rc, uint32_t smp,
+                                           hemk_step st)
 {
 	const uint32_t q = p.q;
 	p.qa = 0;
 	p.qb = 0;
 	if (q != NOREP) {
+		const uint32_t in = smp * rc.R + q;
 		if (st.alpha_given) {
-			p.qa = rc.alpha[q];
+			p.qa = rc.alpha[in];
 		} else {
-			p.qa = rc.q_r[q];
-			p.qb = rc.q_v[q];
+			p.qa = rc.q_r[in];
+			p.qb = rc.q_v[in];
 		}
 	}
 }
@@ -480,9 +505,13 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_ex3(
 	__shared__ int alive[RC];
 	__shared__ int stopf[RC];
 	const uint32_t tile = blockIdx.x;
-	const uint32_t nrep = rc.alist[0];
+	const uint32_t smp = tile_sample(rc, tile);
+	const uint32_t *al = sample_alist(rc, smp);
+	const uint32_t nrep = al[0];
 	if (nrep == 0)
 		return;
+	const uint32_t Ts = sample_T(rc, smp, T);
+	const uint32_t tile0 = rc.sinfo ? rc.sinfo[4 * smp + 2] : 0u;
 	const uint32_t nch = (nrep + RC - 1) / RC;
 	const TileCtx tc = load_tile(tl, tile);
 	double *lam0, *lam1, *lam2, *D, *X;
@@ -491,8 +520,8 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_ex3(
 	TileStruct ts;
 	carve<TC, RC>(smem, tl, lam0, lam1, lam2, D, X, n, u, ts);
 	ChunkRows<TC, 3> p;
-	load_rows<TC, RC, 3>(p, tc, rc, nrep, 0, cnt, uniq, A, B, C);
-	load_alpha<TC>(p, rc, st);
+	load_rows<TC, RC, 3>(p, tc, rc, al, nrep, 0, cnt, uniq, A, B, C);
+	load_alpha<TC>(p, rc, smp, st);
 	stage_struct(tl, tc, ts);
 	for (uint32_t i = 0; i < nch; ++i) {
 		const uint32_t q = p.q;
@@ -500,9 +529,9 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_ex3(
 		// sequential norms leave the two sums; the parallel ones alpha
 		const double alpha = st.alpha_given
 		                         ? p.qa
-		                         : step_len(p.qa, p.qb, T, st);
-		if (!st.alpha_given && tile == 0 && threadIdx.x < RC && rin)
-			rc.alpha[q] = alpha;
+		                         : step_len(p.qa, p.qb, Ts, st);
+		if (!st.alpha_given && tile == tile0 && threadIdx.x < RC && rin)
+			rc.alpha[smp * rc.R + q] = alpha;
 #pragma unroll
 		for (int k = 0; k < PF_TX; ++k) {
 			const uint32_t e = threadIdx.x + TBLOCK * k;
@@ -526,14 +555,14 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_ex3(
 		}
 		__syncthreads();
 		if (i + 1 < nch) {
-			load_rows<TC, RC, 3>(p, tc, rc, nrep, i + 1, cnt, uniq, A, B,
-			                     C);
-			load_alpha<TC>(p, rc, st);
+			load_rows<TC, RC, 3>(p, tc, rc, al, nrep, i + 1, cnt, uniq,
+			                     A, B, C);
+			load_alpha<TC>(p, rc, smp, st);
 		}
 		tile_map<TC, RC>(tl, tc, ts, lam0, lam1, D, X, n, u, alive, A,
 		                 rc.R, q, stopf, rc.stop_min, rc.stop_rel);
 		if (threadIdx.x < RC && rin && stopf[threadIdx.x])
-			rc.notconv[q] = 1;
+			rc.notconv[smp * rc.R + q] = 1;
 		__syncthreads();
 	}
 }
@@ -558,18 +587,22 @@ __device__ __forceinline__ Moments mom_merge(Moments a, Moments b)
 	return o;
 }
 
-// One block per replicate: fold the tile partials in tile order.
+// One block per instance: fold the tile partials of its sample in tile
+// order.
 __global__ __launch_bounds__(TBLOCK) void k_batch_alpha(
     hemk_tiles tl, hemk_rctl rc, const double *__restrict__ part, uint32_t T,
     hemk_step st)
 {
-	const uint32_t r = blockIdx.x;
-	if (rc.done[r])
+	const uint32_t in = blockIdx.x;
+	const uint32_t smp = in / rc.R, r = in % rc.R;
+	if (rc.done[in])
 		return;
 	__shared__ Moments sr[TBLOCK], sv[TBLOCK];
-	const uint32_t per = (tl.n_tiles + TBLOCK - 1) / TBLOCK;
-	const uint32_t i0 = threadIdx.x * per;
-	const uint32_t i1 = min(tl.n_tiles, i0 + per);
+	const uint32_t tb = rc.sinfo ? rc.sinfo[4 * smp + 2] : 0u;
+	const uint32_t nt = rc.sinfo ? rc.sinfo[4 * smp + 3] : tl.n_tiles;
+	const uint32_t per = (nt + TBLOCK - 1) / TBLOCK;
+	const uint32_t i0 = tb + threadIdx.x * per;
+	const uint32_t i1 = min(tb + nt, i0 + per);
 	Moments a = {0, 0, 0}, b = {0, 0, 0};
 	for (uint32_t i = i0; i < i1; ++i) {
 		const double nt = (double)tl.desc[(size_t)HEMK_TILE_DESC * i + 1];
@@ -600,7 +633,7 @@ __global__ __launch_bounds__(TBLOCK) void k_batch_alpha(
 		__syncthreads();
 	}
 	if (threadIdx.x == 0)
-		rc.alpha[r] = step_len(sr[0].q, sv[0].q, T, st);
+		rc.alpha[in] = step_len(sr[0].q, sv[0].q, sample_T(rc, smp, T), st);
 }
 
 // acc + b[0] + b[1] + ... + b[cnt-1], left to right.  Two register banks of
@@ -666,11 +699,16 @@ __global__ __launch_bounds__(TBLOCK) void k_batch_chain(
 {
 #pragma clang fp contract(off)
 	__shared__ double buf[2][CHAIN_TB];
-	const uint32_t r = blockIdx.x;
+	const uint32_t in = blockIdx.x;
+	const uint32_t smp = in / rc.R, rr = in % rc.R;
 	const int which = blockIdx.y;
-	if (phase != 0 && rc.done[r])
+	if (phase != 0 && rc.done[in])
 		return;
-	const double *x = (which ? xv : xr) + (size_t)r * T;
+	const uint32_t t0 = rc.sinfo ? rc.sinfo[4 * smp] : 0u;
+	const uint32_t Tr = T;
+	T = sample_T(rc, smp, T);
+	const double *x = (which ? xv : xr) + (size_t)rr * Tr + t0;
+	const uint32_t r = in;
 	const double m = phase == 2 ? (which ? rc.mean_v[r] : rc.mean_r[r]) : 0.0;
 	const int w = threadIdx.x / 64;
 	const uint32_t nch = (T + CHAIN_TB - 1) / CHAIN_TB;
@@ -719,13 +757,15 @@ __global__ __launch_bounds__(TBLOCK) void k_batch_chain(
 // the tile kernels deal them to chunks of rc.chunk lanes, so finished
 // replicates cost nothing.  One block; done[] written by this block before
 // the call.
-__device__ void build_active_list(const hemk_rctl &rc)
+__device__ void build_active_list(const hemk_rctl &rc, uint32_t smp)
 {
 	__shared__ uint32_t sc[TBLOCK];
+	uint32_t *al = rc.alist + (size_t)smp * (rc.R + 1);
+	const uint32_t *done = rc.done + (size_t)smp * rc.R;
 	uint32_t total = 0;
 	for (uint32_t base = 0; base < rc.R; base += TBLOCK) {
 		const uint32_t r = base + threadIdx.x;
-		const uint32_t f = (r < rc.R && !rc.done[r]) ? 1u : 0u;
+		const uint32_t f = (r < rc.R && !done[r]) ? 1u : 0u;
 		sc[threadIdx.x] = f;
 		__syncthreads();
 		// inclusive scan of the flags
@@ -736,19 +776,22 @@ __device__ void build_active_list(const hemk_rctl &rc)
 			__syncthreads();
 		}
 		if (f)
-			rc.alist[1 + total + sc[threadIdx.x] - 1] = r;
+			al[1 + total + sc[threadIdx.x] - 1] = r;
 		total += sc[TBLOCK - 1];
 		__syncthreads();
 	}
 	if (threadIdx.x == 0)
-		rc.alist[0] = total;
+		al[0] = total;
 }
 
+// One block per sample: end the iteration of its instances.
 __global__ __launch_bounds__(TBLOCK) void k_batch_iter_end(hemk_rctl rc)
 {
 	__shared__ uint32_t cnt[TBLOCK];
+	const uint32_t smp = blockIdx.x;
 	uint32_t act = 0;
-	for (uint32_t r = threadIdx.x; r < rc.R; r += TBLOCK) {
+	for (uint32_t rr = threadIdx.x; rr < rc.R; rr += TBLOCK) {
+		const uint32_t r = smp * rc.R + rr;
 		if (!rc.done[r]) {
 			const uint32_t k = rc.k[r] + 1;
 			const uint32_t stop = rc.notconv[r] ? 0u : 1u;
@@ -770,8 +813,8 @@ __global__ __launch_bounds__(TBLOCK) void k_batch_iter_end(hemk_rctl rc)
 		__syncthreads();
 	}
 	if (threadIdx.x == 0)
-		rc.active[0] = cnt[0];
-	build_active_list(rc);
+		rc.active[smp] = cnt[0];
+	build_active_list(rc, smp);
 }
 
 __global__ __launch_bounds__(TBLOCK) void k_batch_reset(hemk_rctl rc,
@@ -779,14 +822,18 @@ __global__ __launch_bounds__(TBLOCK) void k_batch_reset(hemk_rctl rc,
                                                         double *__restrict__ A)
 {
 	const size_t n = (size_t)T * rc.R;
-	const double x = (double)1 / T;
+	// the uniform start 1 / n_tx of each sample (src/EM.c:311-312)
 	for (size_t i = (size_t)blockIdx.x * TBLOCK + threadIdx.x; i < n;
-	     i += (size_t)gridDim.x * TBLOCK)
-		A[i] = x;
+	     i += (size_t)gridDim.x * TBLOCK) {
+		const uint32_t q = (uint32_t)(i / rc.R);
+		const uint32_t smp = rc.txsamp ? rc.txsamp[q] : 0u;
+		A[i] = (double)1 / sample_T(rc, smp, T);
+	}
 	if (blockIdx.x == 0) {
 		// the loop condition of src/EM.c:328 at k = 0, stop = 0
 		const uint32_t run = (0 < rc.min_iter || 0 < rc.max_iter) ? 1u : 0u;
-		for (uint32_t r = threadIdx.x; r < rc.R; r += TBLOCK) {
+		const uint32_t ni = rc.ns * rc.R;
+		for (uint32_t r = threadIdx.x; r < ni; r += TBLOCK) {
 			rc.k[r] = 0;
 			rc.conv[r] = 0;
 			rc.notconv[r] = 0;
@@ -794,10 +841,11 @@ __global__ __launch_bounds__(TBLOCK) void k_batch_reset(hemk_rctl rc,
 			rc.lam_sum[r] = 0;
 			rc.done[r] = run ? 0u : 1u;
 		}
-		if (threadIdx.x == 0)
-			rc.active[0] = run ? rc.R : 0;
+		for (uint32_t smp = threadIdx.x; smp < rc.ns; smp += TBLOCK)
+			rc.active[smp] = run ? rc.R : 0;
 		__syncthreads();
-		build_active_list(rc);
+		for (uint32_t smp = 0; smp < rc.ns; ++smp)
+			build_active_list(rc, smp);
 	}
 }
 
@@ -831,10 +879,11 @@ __global__ __launch_bounds__(TBLOCK) void k_batch_out(
     double *__restrict__ tpm)
 {
 	const uint32_t r = blockIdx.y;
-	const double sum = rc.lam_sum[r];
 	for (uint32_t t = blockIdx.x * TBLOCK + threadIdx.x; t < T;
 	     t += gridDim.x * TBLOCK) {
 		const size_t i = (size_t)r * T + t;
+		const uint32_t smp = rc.txsamp ? rc.txsamp[t] : 0u;
+		const double sum = rc.lam_sum[smp * rc.R + r];
 		const double l = X[i];
 		double e = 0, g = 0;
 		if (sum != 0) {
@@ -981,21 +1030,22 @@ int hemk_tile_ex3(const hemk_tiles *t, hemk_rctl rc, int chunk,
 int hemk_batch_alpha(const hemk_tiles *t, hemk_rctl rc, const double *part,
                      uint32_t T, hemk_step st, void *s)
 {
-	k_batch_alpha<<<rc.R, TBLOCK, 0, (hipStream_t)s>>>(*t, rc, part, T, st);
+	k_batch_alpha<<<rc.ns * rc.R, TBLOCK, 0, (hipStream_t)s>>>(*t, rc, part,
+	                                                           T, st);
 	return (int)hipGetLastError();
 }
 
 int hemk_batch_chain(hemk_rctl rc, uint32_t T, const double *xr,
                      const double *xv, int phase, void *s)
 {
-	dim3 grid(rc.R, phase == 0 ? 1 : 2);
+	dim3 grid(rc.ns * rc.R, phase == 0 ? 1 : 2);
 	k_batch_chain<<<grid, TBLOCK, 0, (hipStream_t)s>>>(rc, T, xr, xv, phase);
 	return (int)hipGetLastError();
 }
 
 int hemk_batch_iter_end(hemk_rctl rc, void *s)
 {
-	k_batch_iter_end<<<1, TBLOCK, 0, (hipStream_t)s>>>(rc);
+	k_batch_iter_end<<<rc.ns, TBLOCK, 0, (hipStream_t)s>>>(rc);
 	return (int)hipGetLastError();
 }
 

@@ -186,22 +186,42 @@ static int ws_ensure(hem_problem *p, uint32_t R)
 	WALLOC(w->m, (size_t)R * HEMK_RS_MAXB * 4);
 	WALLOC(w->rs_plan, (K + 2 * ((size_t)HEMK_RS_MAXB + 1)) * 4);
 	WALLOC(w->seeds, (size_t)R * 8);
-	WALLOC(w->u32, ((size_t)5 * R + 3) * 4);
-	WALLOC(w->f64, (size_t)8 * R * 8);
+	{
+		const size_t ns = p->ns, NI = ns * R;
+		WALLOC(w->u32, (4 * NI + ns + ns * ((size_t)R + 1) + 1) * 4);
+		WALLOC(w->f64, 8 * NI * 8);
+	}
 	WALLOC(w->sc, (size_t)4 * R * p->spec_P * 8);
 	WALLOC(w->sbase, (size_t)2 * R * p->spec_P * 8);
 	WALLOC(w->stab, (size_t)2 * R * p->spec_P * HEMK_SPEC_W * 8);
 #undef WALLOC
-	if (hemd_host_alloc((void **)&w->h_active, 64))
+	if (hemd_host_alloc((void **)&w->h_active,
+	                    (size_t)(p->ns > 16 ? p->ns : 16) * 4))
 		return hem_fail(HEM_ERR_NOMEM, "pinned allocation");
 	w->R = R;
 	return HEM_OK;
 }
 
+/* Control arrays of the workspace, for NIc = ns * capacity instances:
+ *   u32: k, done, conv, notconv [NIc each], active [ns],
+ *        alist [ns][capacity + 1], exact-sum reruns [1]
+ *   f64: alpha, mean_r, mean_v, lam_sum [NIc each], q_r/q_v [2 NIc],
+ *        exact pass-1 sums [2 NIc] */
+static uint32_t *reruns_word(const hem_problem *p)
+{
+	const size_t ns = p->ns, NIc = ns * p->bw.R;
+	return p->bw.u32 + 4 * NIc + ns + ns * ((size_t)p->bw.R + 1);
+}
+
+static double *pass1_sums(const hem_problem *p)
+{
+	return p->bw.f64 + 6 * (size_t)p->ns * p->bw.R;
+}
+
 static hemk_rctl rctl_for(hem_problem *p, uint32_t R, const hem_opts *o)
 {
 	hem_batch_ws *w = &p->bw;
-	const uint32_t cap = w->R;
+	const size_t NIc = (size_t)p->ns * w->R;
 	hemk_rctl rc;
 	memset(&rc, 0, sizeof(rc));
 	rc.R = R;
@@ -210,19 +230,25 @@ static hemk_rctl rctl_for(hem_problem *p, uint32_t R, const hem_opts *o)
 	rc.stop_min = o->stop_min_value;
 	rc.stop_rel = o->stop_rel_change;
 	rc.k = w->u32;
-	rc.done = w->u32 + cap;
-	rc.conv = w->u32 + 2 * (size_t)cap;
-	rc.notconv = w->u32 + 3 * (size_t)cap;
-	rc.active = w->u32 + 4 * (size_t)cap;
-	rc.alist = w->u32 + 4 * (size_t)cap + 1;
+	rc.done = w->u32 + NIc;
+	rc.conv = w->u32 + 2 * NIc;
+	rc.notconv = w->u32 + 3 * NIc;
+	rc.active = w->u32 + 4 * NIc;
+	rc.alist = w->u32 + 4 * NIc + p->ns;
 	rc.alpha = w->f64;
-	rc.mean_r = w->f64 + cap;
-	rc.mean_v = w->f64 + 2 * (size_t)cap;
-	rc.lam_sum = w->f64 + 3 * (size_t)cap;
-	/* q_v follows q_r at distance R: the exact sums write both as one
-	 * array of 2R rows */
-	rc.q_r = w->f64 + 4 * (size_t)cap;
-	rc.q_v = rc.q_r + R;
+	rc.mean_r = w->f64 + NIc;
+	rc.mean_v = w->f64 + 2 * NIc;
+	rc.lam_sum = w->f64 + 3 * NIc;
+	/* q_v follows q_r at distance ns * R: the exact sums write both as
+	 * one array of 2R rows (one sample) */
+	rc.q_r = w->f64 + 4 * NIc;
+	rc.q_v = rc.q_r + (size_t)p->ns * R;
+	rc.ns = p->ns;
+	if (p->ns > 1) {
+		rc.tsamp = p->d_tsamp;
+		rc.txsamp = p->d_txsamp;
+		rc.sinfo = p->d_sinfo;
+	}
 	return rc;
 }
 
@@ -253,7 +279,7 @@ static hemk_spec spec_batch(hem_problem *p, hemk_rctl rc, int pairs, int xf,
 	sp.tab = w->stab;
 	sp.base = w->sbase;
 	sp.res = res;
-	sp.miss = w->u32 + 5 * cap + 2;
+	sp.miss = reruns_word(p);
 	return sp;
 }
 
@@ -264,8 +290,11 @@ static int batch_iteration(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 {
 	hem_batch_ws *w = &p->bw;
 	void *s = p->stream;
-	const int spec = o->norm == HEM_NORM_EXACT;
-	const int exact = spec || o->norm == HEM_NORM_SEQ;
+	/* exact sums by speculation for one sample; several samples (rows of
+	 * different lengths) use the single-lane chains, with the same
+	 * results */
+	const int spec = o->norm == HEM_NORM_EXACT && p->ns == 1;
+	const int exact = o->norm != HEM_NORM_TREE;
 	/* exact: alpha from the sums inside ex3; tree: alpha given */
 	hemk_step st = {o->step, o->clamp, o->alpha_clamp != 0, !exact};
 	const size_t TR = (size_t)p->n_tx * rc.R;
@@ -279,7 +308,7 @@ static int batch_iteration(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 		 * sums (sd) or sums of squares (L2, straight into q_r/q_v),
 		 * then the centred sums of the sd form into q_r/q_v */
 		const int sd = o->step == HEM_STEP_SD;
-		double *sums1 = p->bw.f64 + 6 * (size_t)p->bw.R;
+		double *sums1 = pass1_sums(p);
 		hemk_spec sp = spec_batch(p, rc, 1,
 		                          sd ? HEMK_SPEC_IDENT : HEMK_SPEC_SQUARE,
 		                          NULL, sd ? sums1 : rc.q_r);
@@ -424,8 +453,7 @@ static int run_batch(hem_problem *p, const hem_opts *o, uint32_t R,
 		return hem_fail(HEM_ERR_RANGE, "tiles do not fit the LDS budget");
 	rc.chunk = (uint32_t)chunk;
 	HEM_TRY_DEV(hemk_batch_reset(rc, p->n_tx, w->A, p->stream), "reset");
-	HEM_TRY_DEV(hemk_fill_u32(w->u32 + 5 * (size_t)w->R + 2, 1, 0,
-	                          p->stream),
+	HEM_TRY_DEV(hemk_fill_u32(reruns_word(p), 1, 0, p->stream),
 	            "rerun counter");
 	if ((rc2 = batch_graph(p, o, rc, chunk, &graph)))
 		return rc2;
@@ -435,11 +463,17 @@ static int run_batch(hem_problem *p, const hem_opts *o, uint32_t R,
 		launched += HEM_ITERS_PER_GRAPH;
 		if (launched < o->min_iter && launched < limit)
 			continue;
-		HEM_TRY_DEV(hemd_d2h(w->h_active, rc.active, 4, p->stream),
+		HEM_TRY_DEV(hemd_d2h(w->h_active, rc.active,
+		                     (size_t)p->ns * 4, p->stream),
 		            "active readback");
 		HEM_TRY_DEV(hemd_stream_sync(p->stream), "stream sync");
-		if (w->h_active[0] == 0)
-			break;
+		{
+			uint32_t act = 0;
+			for (uint32_t q = 0; q < p->ns; ++q)
+				act |= w->h_active[q];
+			if (act == 0)
+				break;
+		}
 		if (launched > limit + HEM_ITERS_PER_GRAPH)
 			return hem_fail(HEM_ERR_DEVICE,
 			                "batched loop did not terminate");
@@ -451,7 +485,7 @@ static int run_batch(hem_problem *p, const hem_opts *o, uint32_t R,
 	HEM_TRY_DEV(hemk_batch_rows(rc, p->n_tx, p->dt.txq, w->A, w->X,
 	                            p->stream),
 	            "state rows");
-	if (o->norm == HEM_NORM_SEQ) {
+	if (o->norm == HEM_NORM_SEQ || p->ns > 1) {
 		HEM_TRY_DEV(hemk_batch_chain(rc, p->n_tx, w->X, NULL, 0,
 		                             p->stream),
 		            "final sums");
@@ -474,15 +508,15 @@ static int run_batch(hem_problem *p, const hem_opts *o, uint32_t R,
 		                     p->stream),
 		            "tpm readback");
 	{
-		uint32_t *hk = malloc((size_t)R * 4 * 2 + 4);
+		const size_t NI = (size_t)p->ns * R;
+		uint32_t *hk = malloc(NI * 4 * 2 + 4);
 		if (!hk)
 			return hem_fail(HEM_ERR_NOMEM, "host allocation");
-		rc2 = hemd_d2h(hk, rc.k, (size_t)R * 4, p->stream);
+		rc2 = hemd_d2h(hk, rc.k, NI * 4, p->stream);
 		if (!rc2)
-			rc2 = hemd_d2h(hk + R, rc.conv, (size_t)R * 4, p->stream);
+			rc2 = hemd_d2h(hk + NI, rc.conv, NI * 4, p->stream);
 		if (!rc2)
-			rc2 = hemd_d2h(hk + 2 * (size_t)R,
-			               w->u32 + 5 * (size_t)w->R + 2, 4, p->stream);
+			rc2 = hemd_d2h(hk + 2 * NI, reruns_word(p), 4, p->stream);
 		if (!rc2)
 			rc2 = hemd_stream_sync(p->stream);
 		if (rc2) {
@@ -490,21 +524,21 @@ static int run_batch(hem_problem *p, const hem_opts *o, uint32_t R,
 			return hem_dev_fail(rc2, "iteration readback");
 		}
 		if (iters)
-			memcpy(iters, hk, (size_t)R * 4);
+			memcpy(iters, hk, NI * 4);
 		if (st) {
 			st->n_rep = R;
 			st->iters_min = 0xFFFFFFFFu;
 			st->iters_max = 0;
 			st->n_converged = 0;
-			for (uint32_t r = 0; r < R; ++r) {
+			for (size_t r = 0; r < NI; ++r) {
 				if (hk[r] < st->iters_min)
 					st->iters_min = hk[r];
 				if (hk[r] > st->iters_max)
 					st->iters_max = hk[r];
-				st->n_converged += hk[R + r] != 0;
+				st->n_converged += hk[NI + r] != 0;
 			}
 			st->chunk = (uint32_t)chunk;
-			st->sum_reruns = hk[2 * (size_t)R];
+			st->sum_reruns = hk[2 * NI];
 			st->n_tiles = p->dt.n_tiles;
 		}
 		free(hk);
@@ -647,12 +681,39 @@ static int plan_chunks(const uint32_t *w, uint32_t K, uint32_t N,
 	}
 }
 
-/* Resamples n_rep count vectors into w->cat_rm. */
+/* Resamples n_rep count vectors of every sample into w->cat_rm ([R][K],
+ * K = n_tx + n_cls of the handle: unique counts, then class counts);
+ * sample q uses seeds[q * n_rep + r]. */
+static int resample_one(hem_problem *p, const uint32_t *unique,
+                        uint32_t n_rep, const uint64_t *seeds, uint32_t q,
+                        float *ms);
+
 static int resample(hem_problem *p, const uint32_t *unique, uint32_t n_rep,
                     const uint64_t *seeds, float *ms)
 {
+	float tot = 0;
+	for (uint32_t q = 0; q < p->ns; ++q) {
+		float m = 0;
+		int rc = resample_one(p, unique, n_rep, seeds + (size_t)q * n_rep,
+		                      q, ms ? &m : NULL);
+		if (rc)
+			return rc;
+		tot += m;
+	}
+	if (ms)
+		*ms = tot;
+	return HEM_OK;
+}
+
+static int resample_one(hem_problem *p, const uint32_t *unique,
+                        uint32_t n_rep, const uint64_t *seeds, uint32_t q,
+                        float *ms)
+{
 	hem_batch_ws *wk = &p->bw;
-	const uint32_t T = p->n_tx, C = p->n_cls;
+	const uint32_t t0 = p->ns > 1 ? p->h_tx0[q] : 0;
+	const uint32_t c0 = p->ns > 1 ? p->h_cls0[q] : 0;
+	const uint32_t T = p->ns > 1 ? p->h_tx0[q + 1] - t0 : p->n_tx;
+	const uint32_t C = p->ns > 1 ? p->h_cls0[q + 1] - c0 : p->n_cls;
 	const uint32_t K = T + C;
 	uint32_t *hw = malloc(((size_t)K + 2 * (HEMK_RS_MAXB + 1)) * 4);
 	uint32_t *start = hw + K, *cw = start + HEMK_RS_MAXB + 1, B = 0;
@@ -667,11 +728,11 @@ static int resample(hem_problem *p, const uint32_t *unique, uint32_t n_rep,
 		                "bootstrap needs integer class counts below 2^32");
 	}
 	for (uint32_t t = 0; t < T; ++t) {
-		hw[t] = unique[t];
-		N += unique[t];
+		hw[t] = unique[t0 + t];
+		N += unique[t0 + t];
 	}
 	for (uint32_t c = 0; c < C; ++c) {
-		hw[T + c] = (uint32_t)p->h_cls_count[c];
+		hw[T + c] = (uint32_t)p->h_cls_count[c0 + c];
 		N += hw[T + c];
 	}
 	if (N > 0xFFFFFFFFull) {
@@ -690,8 +751,14 @@ static int resample(hem_problem *p, const uint32_t *unique, uint32_t n_rep,
 		rc = hemd_event_record(p->ev2, p->stream);
 	if (!rc)
 		rc = hemk_resample(d, K, d + K, d + K + HEMK_RS_MAXB + 1, B,
-		                   (uint32_t)N, wk->seeds, n_rep, wk->m, wk->cat_rm,
-		                   p->stream);
+		                   (uint32_t)N, wk->seeds, n_rep, wk->m,
+		                   p->ns > 1 ? wk->catT : wk->cat_rm, p->stream);
+	/* several samples: this sample's rows into its ranges of the
+	 * handle's unique and class blocks */
+	if (!rc && p->ns > 1)
+		rc = hemk_scatter_counts(wk->catT, n_rep, T, C, wk->cat_rm,
+		                         p->n_tx + p->n_cls, t0, p->n_tx + c0,
+		                         p->stream);
 	if (!rc)
 		rc = hemd_event_record(p->ev3, p->stream);
 	if (!rc)

@@ -150,6 +150,8 @@ static int check_args(hem_problem *p, const uint32_t *unique,
 {
 	if (!p || !unique || !theta || !ell)
 		return hem_fail(HEM_ERR_INVALID, "null argument");
+	if (p->ns > 1)
+		return hem_fail(HEM_ERR_INVALID, "one-sample handles only");
 	for (uint32_t t = 0; t < p->n_tx; ++t)
 		if (!(theta[t] == theta[t]) || theta[t] == 1.0 / 0.0 ||
 		    !(ell[t] == ell[t]) || ell[t] == 1.0 / 0.0 ||

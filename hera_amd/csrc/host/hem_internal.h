@@ -113,6 +113,14 @@ struct hem_problem {
 	void **retired;
 	int n_retired, cap_retired;
 
+	/* samples: ns independent tables side by side (ns == 1: one table);
+	 * h_tx0/h_cls0 [ns+1] first transcript / class of each sample;
+	 * device: sinfo [ns][4] {tx0, n_tx, tile0, n_tiles}, tile -> sample,
+	 * transcript -> sample */
+	uint32_t ns;
+	uint32_t *h_tx0, *h_cls0;
+	uint32_t *d_sinfo, *d_tsamp, *d_txsamp;
+
 	/* captured iteration graphs and the options each was captured for;
 	 * kept until the handle is destroyed */
 	void *graph[HEM_GRAPH_CACHE];
@@ -138,11 +146,15 @@ typedef struct {
 	uint32_t *desc, *tx, *len, *coff, *toff, *cls_orig, *txq;
 	uint32_t *kmem;
 	uint16_t *tpos;
+	uint32_t *tsamp;   /* tile -> sample (ns > 1 only)               */
 } hem_tiles_host;
 
+/* tx0[0..ns]: first transcript of each sample (tx0[ns] = T), or NULL with
+ * ns = 1. */
 int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
                     const uint32_t *cls_ref, const uint32_t *tx_len,
-                    const uint32_t *tx_off, hem_tiles_host *h);
+                    const uint32_t *tx_off, const uint32_t *tx0,
+                    uint32_t ns, hem_tiles_host *h);
 void hem_tiles_free_host(hem_tiles_host *h);
 
 /* kernel kinds for per-kernel timing (hem_kernel_profile) */

@@ -61,7 +61,61 @@ static int dalloc(hem_problem *p, void **ptr, size_t n)
 	return HEM_OK;
 }
 
-int hem_problem_create(const hem_classes *c, hem_problem **out)
+/* Sample layout on the device: {tx0, n_tx, tile0, n_tiles} per sample,
+ * tile -> sample, transcript -> sample; host copies of the first
+ * transcript and class of each sample. */
+static int samples_upload(hem_problem *p, const hem_tiles_host *th,
+                          const uint32_t *tx0, const uint32_t *cls0)
+{
+	const uint32_t ns = p->ns, T = p->n_tx;
+	uint32_t *si = calloc((size_t)ns * 4, 4);
+	uint32_t *ts = malloc((size_t)T * 4);
+	int rc = HEM_OK, e = 0;
+	p->h_tx0 = malloc(((size_t)ns + 1) * 4);
+	p->h_cls0 = malloc(((size_t)ns + 1) * 4);
+	if (!si || !ts || !p->h_tx0 || !p->h_cls0) {
+		free(si);
+		free(ts);
+		return hem_fail(HEM_ERR_NOMEM, "host allocation failed");
+	}
+	memcpy(p->h_tx0, tx0, ((size_t)ns + 1) * 4);
+	memcpy(p->h_cls0, cls0, ((size_t)ns + 1) * 4);
+	for (uint32_t q = 0; q < ns; ++q) {
+		si[4 * q] = tx0[q];
+		si[4 * q + 1] = tx0[q + 1] - tx0[q];
+		for (uint32_t t = tx0[q]; t < tx0[q + 1]; ++t)
+			ts[t] = q;
+	}
+	for (uint32_t i = th->n_tiles; i-- > 0;) {
+		const uint32_t q = th->tsamp[i];
+		si[4 * q + 2] = i;
+		si[4 * q + 3]++;
+	}
+	if ((rc = dalloc(p, (void **)&p->d_sinfo, (size_t)ns * 16)) ||
+	    (rc = dalloc(p, (void **)&p->d_tsamp, (size_t)th->n_tiles * 4)) ||
+	    (rc = dalloc(p, (void **)&p->d_txsamp, (size_t)T * 4))) {
+		free(si);
+		free(ts);
+		return rc;
+	}
+	e = hemd_h2d(p->d_sinfo, si, (size_t)ns * 16, p->stream);
+	if (!e)
+		e = hemd_h2d(p->d_tsamp, th->tsamp, (size_t)th->n_tiles * 4,
+		             p->stream);
+	if (!e)
+		e = hemd_h2d(p->d_txsamp, ts, (size_t)T * 4, p->stream);
+	if (!e)
+		e = hemd_stream_sync(p->stream);
+	free(si);
+	free(ts);
+	return e ? hem_dev_fail(e, "sample layout upload") : HEM_OK;
+}
+
+/* Device problem for a class table holding ns samples side by side
+ * (tx0/cls0: first transcript / class of each sample, ns + 1 entries each;
+ * NULL with ns == 1). */
+static int create_impl(const hem_classes *c, const uint32_t *tx0,
+                       const uint32_t *cls0, uint32_t ns, hem_problem **out)
 {
 	int rc;
 	hem_problem *p;
@@ -123,6 +177,7 @@ int hem_problem_create(const hem_classes *c, hem_problem **out)
 	p->n_tx = T;
 	p->n_cls = C;
 	p->nnz = nnz;
+	p->ns = ns;
 	p->group_cls = 4;
 	p->group_tx = 16;
 
@@ -249,13 +304,15 @@ int hem_problem_create(const hem_classes *c, hem_problem **out)
 	{
 		hem_tiles_host th;
 		int trc = hem_tiles_build(T, C, c->cls_off, c->cls_ref,
-		                          c->tx_len, tx_off, &th);
+		                          c->tx_len, tx_off, tx0, ns, &th);
 		if (trc == HEM_OK) {
 			trc = hem_tiles_upload(p, &th);
 			if (trc == HEM_OK)
 				trc = dalloc(p, (void **)&p->d_cntq,
 				             (size_t)(th.n_cls ? th.n_cls : 1) *
 				                 8);
+			if (trc == HEM_OK && ns > 1)
+				trc = samples_upload(p, &th, tx0, cls0);
 			p->h_cls_orig = th.cls_orig;
 			th.cls_orig = NULL;
 			hem_tiles_free_host(&th);
@@ -270,7 +327,7 @@ int hem_problem_create(const hem_classes *c, hem_problem **out)
 				hem_problem_destroy(p);
 				return trc;
 			}
-		} else if (trc != HEM_ERR_RANGE) {
+		} else if (trc != HEM_ERR_RANGE || ns > 1) {
 			free(tx_off);
 			free(tx_ref);
 			free(cnt32);
@@ -296,6 +353,83 @@ int hem_problem_create(const hem_classes *c, hem_problem **out)
 	p->dp.len = p->d_len;
 	*out = p;
 	return HEM_OK;
+}
+
+int hem_problem_create(const hem_classes *c, hem_problem **out)
+{
+	return create_impl(c, NULL, NULL, 1, out);
+}
+
+int hem_samples_create(const hem_classes *samples, uint32_t n_samples,
+                       hem_problem **out)
+{
+	uint32_t *tx0, *cls0, *off = NULL, *ref = NULL, *len = NULL;
+	double *cnt = NULL;
+	uint64_t T = 0, C = 0, Z = 0;
+	hem_classes all;
+	int rc;
+	if (!out)
+		return hem_fail(HEM_ERR_INVALID, "null output handle");
+	*out = NULL;
+	if (!samples || n_samples == 0)
+		return hem_fail(HEM_ERR_INVALID, "no samples");
+	for (uint32_t q = 0; q < n_samples; ++q) {
+		if ((rc = validate(&samples[q])))
+			return rc;
+		T += samples[q].n_tx;
+		C += samples[q].n_cls;
+		Z += samples[q].n_cls ? samples[q].cls_off[samples[q].n_cls] : 0;
+	}
+	if (T > 0xFFFFFFFFull || C > 0xFFFFFFFEull || Z > 0xFFFFFFFFull)
+		return hem_fail(HEM_ERR_RANGE, "samples exceed 2^32 transcripts, "
+		                               "classes or members together");
+	tx0 = malloc(((size_t)n_samples + 1) * 4);
+	cls0 = malloc(((size_t)n_samples + 1) * 4);
+	off = malloc((C + 1) * 4);
+	ref = malloc((Z ? Z : 1) * 4);
+	len = malloc(T * 4);
+	cnt = malloc((C ? C : 1) * 8);
+	if (!tx0 || !cls0 || !off || !ref || !len || !cnt) {
+		rc = hem_fail(HEM_ERR_NOMEM, "host allocation failed");
+		goto out;
+	}
+	/* sample q's transcript ids shift by tx0[q], its classes by cls0[q]:
+	 * the table is block diagonal */
+	{
+		uint32_t t = 0, c = 0, z = 0;
+		off[0] = 0;
+		for (uint32_t q = 0; q < n_samples; ++q) {
+			const hem_classes *sq = &samples[q];
+			tx0[q] = t;
+			cls0[q] = c;
+			memcpy(len + t, sq->tx_len, (size_t)sq->n_tx * 4);
+			for (uint32_t i = 0; i < sq->n_cls; ++i) {
+				for (uint32_t k = sq->cls_off[i]; k < sq->cls_off[i + 1];
+				     ++k)
+					ref[z++] = sq->cls_ref[k] + t;
+				cnt[c] = sq->cls_count[i];
+				off[++c] = z;
+			}
+			t += sq->n_tx;
+		}
+		tx0[n_samples] = t;
+		cls0[n_samples] = c;
+	}
+	all.n_tx = (uint32_t)T;
+	all.n_cls = (uint32_t)C;
+	all.cls_off = off;
+	all.cls_ref = ref;
+	all.cls_count = cnt;
+	all.tx_len = len;
+	rc = create_impl(&all, tx0, cls0, n_samples, out);
+out:
+	free(tx0);
+	free(cls0);
+	free(off);
+	free(ref);
+	free(len);
+	free(cnt);
+	return rc;
 }
 
 int hem_problem_destroy(hem_problem *p)
@@ -330,6 +464,11 @@ int hem_problem_destroy(hem_problem *p)
 	hemd_free(p->d_unique);
 	hemd_free(p->d_uq);
 	hemd_free(p->d_cntq);
+	hemd_free(p->d_sinfo);
+	hemd_free(p->d_tsamp);
+	hemd_free(p->d_txsamp);
+	free(p->h_tx0);
+	free(p->h_cls0);
 	hemd_free(p->d_part1);
 	hemd_free(p->d_part2);
 	hemd_free(p->d_out);

@@ -401,6 +401,9 @@ int hem_solve(hem_problem *p, const uint32_t *unique, const hem_opts *opts,
 	const double *lam;
 	if (!p || !unique)
 		return hem_fail(HEM_ERR_INVALID, "null problem or counts");
+	if (p->ns > 1)
+		return hem_fail(HEM_ERR_INVALID, "hem_solve needs a one-sample "
+		                                 "handle (see hem_solve_batch)");
 	if ((rc = load_opts(opts, &o)))
 		return rc;
 
@@ -519,6 +522,8 @@ int hem_profile_solve(hem_problem *p, const uint32_t *unique,
 	int rc, i;
 	if (!p || !unique || !out)
 		return hem_fail(HEM_ERR_INVALID, "null argument");
+	if (p->ns > 1)
+		return hem_fail(HEM_ERR_INVALID, "one-sample handles only");
 	if ((rc = load_opts(opts, &o)))
 		return rc;
 	memset(out, 0, sizeof(*out));
@@ -590,6 +595,8 @@ int hem_loglik(hem_problem *p, const uint32_t *unique, const double *lambda,
 	int rc;
 	if (!p || !unique || !lambda || !loglik)
 		return hem_fail(HEM_ERR_INVALID, "null argument");
+	if (p->ns > 1)
+		return hem_fail(HEM_ERR_INVALID, "one-sample handles only");
 	pthread_mutex_lock(&p->mu);
 	rc = hemd_h2d(p->d_unique, unique, (size_t)p->n_tx * 4, p->stream);
 	if (!rc)

@@ -58,6 +58,7 @@ void hem_tiles_free_host(hem_tiles_host *h)
 	free(h->toff);
 	free(h->tpos);
 	free(h->cls_orig);
+	free(h->tsamp);
 	memset(h, 0, sizeof(*h));
 }
 
@@ -66,7 +67,8 @@ void hem_tiles_free_host(hem_tiles_host *h)
  * HEM_ERR_RANGE when a component exceeds the tile limits. */
 int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
                     const uint32_t *cls_ref, const uint32_t *tx_len,
-                    const uint32_t *tx_off, hem_tiles_host *h)
+                    const uint32_t *tx_off, const uint32_t *tx0,
+                    uint32_t ns, hem_tiles_host *h)
 {
 	uint32_t *par = NULL, *comp = NULL, *ctx = NULL, *ccls = NULL;
 	uint32_t *ctile = NULL, *lidx = NULL, *pos = NULL;
@@ -132,16 +134,26 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 			              ccls[i], HEM_TILE_TX, HEM_TILE_CLS);
 			goto out;
 		}
-	/* pack components into tiles */
+	/* pack components into tiles; with samples, a tile never holds two
+	 * (components are in order of their smallest transcript, so a
+	 * sample's components are consecutive and its packing is the one
+	 * its own table would get) */
 	{
-		uint32_t nt = 0, nc = 0, nz = 0;
+		uint32_t nt = 0, nc = 0, nz = 0, smp = 0, cur = 0;
+		uint32_t *root = lidx;   /* smallest transcript of each component */
+		for (i = T; i-- > 0;)
+			root[comp[i]] = i;
 		for (i = 0; i < ncomp; ++i) {
-			if (nt && (nt + ctx[i] > HEM_TILE_TX_TARGET ||
+			if (ns > 1)
+				while (smp + 1 < ns && root[i] >= tx0[smp + 1])
+					++smp;
+			if (nt && (smp != cur || nt + ctx[i] > HEM_TILE_TX_TARGET ||
 			           nc + ccls[i] > HEM_TILE_CLS_TARGET ||
 			           nz + cnz[i] > HEM_TILE_NNZ_TARGET)) {
 				ntile++;
 				nt = nc = nz = 0;
 			}
+			cur = smp;
 			ctile[i] = ntile;
 			nt += ctx[i];
 			nc += ccls[i];
@@ -287,6 +299,21 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 	h->n_tiles = ntile;
 	h->n_tx = T;
 	h->nnz = zlive;
+	if (ns > 1) {
+		/* each tile's sample, from its first transcript */
+		uint32_t smp = 0;
+		h->tsamp = malloc((size_t)ntile * 4);
+		if (!h->tsamp) {
+			rc = hem_fail(HEM_ERR_NOMEM, "tiling: host allocation");
+			goto out;
+		}
+		for (i = 0; i < ntile; ++i) {
+			const uint32_t t = h->tx[h->desc[HEM_TILE_DESC * i]];
+			while (smp + 1 < ns && t >= tx0[smp + 1])
+				++smp;
+			h->tsamp[i] = smp;
+		}
+	}
 	h->max_tx = h->max_cls = h->max_nnz = 0;
 	for (i = 0; i < ntile; ++i) {
 		uint32_t q0 = h->desc[HEM_TILE_DESC * i + 2], q1 = q0 + tile_ncls[i];
@@ -337,7 +364,7 @@ int hem_classes_tiling(const hem_classes *c, uint32_t *n_tiles,
 	for (i = 0; i < c->n_tx; ++i)
 		tx_off[i + 1] += tx_off[i];
 	rc = hem_tiles_build(c->n_tx, c->n_cls, c->cls_off, c->cls_ref,
-	                     c->tx_len, tx_off, &h);
+	                     c->tx_len, tx_off, NULL, 1, &h);
 	free(tx_off);
 	if (rc == HEM_ERR_RANGE) {
 		*n_tiles = 0;

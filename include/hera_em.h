@@ -127,6 +127,24 @@ typedef struct hem_problem hem_problem;
 int hem_problem_create(const hem_classes *cls, hem_problem **out);
 int hem_problem_destroy(hem_problem *p);
 
+/* Ragged multi-sample handle (no counterpart in the reference, which
+ * quantifies one sample per run): n_samples independent class tables side
+ * by side in one device problem -- sample q's transcripts are ids
+ * tx0_q .. tx0_q + n_tx_q - 1 of the handle (tx0_q = the sum of the earlier
+ * samples' n_tx), its classes likewise -- tiled so that no tile holds two
+ * samples.  On such a handle hem_solve_batch, hem_bootstrap and
+ * hem_bootstrap_counts run n_samples x n_rep EM instances together, each
+ * its own EM over its sample's transcripts (uniform start 1/n_tx_q, norms,
+ * step, stop test and iteration count per instance), exactly as each
+ * sample's own handle would; seeds and iteration counts are per instance,
+ * index q * n_rep + r; count and output rows span all samples' transcripts
+ * (tpm sums to 1e6 within each sample).  Every component must fit a tile;
+ * HEM_NORM_EXACT runs the single-lane sums on such handles (same results).
+ * The single-problem entry points (hem_solve, hem_loglik,
+ * hem_profile_solve, hem_grad, hem_hvp, hem_stderr) reject it. */
+int hem_samples_create(const hem_classes *samples, uint32_t n_samples,
+                       hem_problem **out);
+
 /* Replaces the class counts (n_cls values, same validation as creation);
  * the structure stays on the device. */
 int hem_problem_set_counts(hem_problem *p, const double *cls_count);

@@ -74,6 +74,12 @@ def main():
                          "comparison")
     ap.add_argument("--cpu-boot-threads", type=int, default=16,
                     help="threads (= replicates) of the CPU bootstrap sample")
+    ap.add_argument("--ragged-samples", type=int, default=64,
+                    help="samples of the ragged multi-sample batch (0: skip)")
+    ap.add_argument("--ragged-reps", type=int, default=20)
+    ap.add_argument("--ragged-seq-sample", type=int, default=8,
+                    help="samples bootstrapped one handle at a time for the "
+                         "comparison (scaled to all samples)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -190,6 +196,9 @@ def main():
 
     if args.grad_topk > 0:
         out["differentiation"] = bench_grad(args, H, np, p, cls, u, sk)
+
+    if args.ragged_samples > 0:
+        out["ragged"] = bench_ragged(args, H, np)
 
     boot = None
     if args.bootstrap_reps > 0:
@@ -313,6 +322,98 @@ def bench_grad(args, H, np, p, cls, u, sk):
         % (res["grad_fp64_ms"], res["grad_fp32_forward_ms"],
            res["hvp_wall_ms"], res["cpu_grad_ms"], args.grad_topk,
            args.cg_steps, st["ms_cg"], st["resid_max"]))
+    return res
+
+
+def bench_ragged(args, H, np):
+    """Config 5: samples with transcript counts log-uniform in
+    5,000-200,000, 10x as many drawn classes, fragments uniform in 5M-80M,
+    each with R bootstrap replicates, all in one multi-sample handle (every
+    (sample, replicate) EM instance in the same launches, per-instance stop
+    tests); against the same samples bootstrapped one handle at a time, and
+    a gradient pass per sample at its EM solution."""
+    S, R = args.ragged_samples, args.ragged_reps
+    rng = np.random.default_rng(12345)
+    ntx = np.exp(rng.uniform(np.log(5000), np.log(200000), S)).astype(int)
+    nfr = rng.integers(5_000_000, 80_000_001, S)
+    t0 = time.time()
+    gens = [H.generate(int(t), int(10 * t), int(f), seed=1000 + q)
+            for q, (t, f) in enumerate(zip(ntx, nfr))]
+    cls = [g[0] for g in gens]
+    us = [g[1] for g in gens]
+    log("ragged: %d samples, %d transcripts, %d classes, %d members "
+        "generated in %.1fs" % (S, sum(c.n_tx for c in cls),
+                                sum(c.n_cls for c in cls),
+                                sum(c.nnz for c in cls), time.time() - t0))
+    seeds = (np.arange(S * R, dtype=np.uint64) + 1).reshape(S, R)
+    sk = dict(stop_rel_change=args.stop_rel, min_iter=args.min_iter,
+              max_iter=args.max_iter)
+    t0 = time.time()
+    mp = H.Problem.from_samples(cls)
+    res = {"n_samples": S, "reps_per_sample": R,
+           "n_tx_total": int(mp.n_tx), "n_cls_total": int(mp.cls0[-1]),
+           "nnz_total": int(sum(c.nnz for c in cls)),
+           "upload_s": time.time() - t0, "seeds": "1..%d" % (S * R)}
+    u_all = np.concatenate(us)
+    out_multi = {}
+    for name, nm in (("fp64-tree", H.NORM_TREE),
+                     ("fp64-exact", H.NORM_EXACT)):
+        mp.bootstrap(u_all, seeds[:, :2], norm=nm, **sk)   # graphs
+        t = time.perf_counter()
+        b = mp.bootstrap(u_all, seeds, norm=nm, **sk)
+        dt = time.perf_counter() - t
+        st = b["stats"]
+        res[name] = dict(seconds=dt, instances_per_s=S * R / dt,
+                         ms_resample=st["ms_resample"],
+                         ms_solve=st["ms_solve"],
+                         iters_min=int(b["iters"].min()),
+                         iters_max=int(b["iters"].max()),
+                         iters_total=int(b["iters"].sum()))
+        out_multi[name] = b
+        log("ragged %-10s %d samples x %d reps: %.3fs (%.0f instances/s) "
+            "iters %d..%d" % (name, S, R, dt, S * R / dt,
+                              b["iters"].min(), b["iters"].max()))
+    mp.close()
+    # the same samples one handle at a time (a sample of them, scaled)
+    nseq = min(args.ragged_seq_sample, S)
+    pick = np.linspace(0, S - 1, nseq).astype(int)
+    for name, nm in (("fp64-tree", H.NORM_TREE),):
+        tsum = 0.0
+        same = True
+        for q in pick:
+            p = H.Problem(cls[q])
+            p.bootstrap(us[q], seeds[q, :2], norm=nm, **sk)
+            t = time.perf_counter()
+            b = p.bootstrap(us[q], seeds[q], norm=nm, **sk)
+            tsum += time.perf_counter() - t
+            p.close()
+            t0q = int(sum(c.n_tx for c in cls[:q]))
+            same &= bool(np.array_equal(
+                out_multi[name]["est"][:, t0q:t0q + cls[q].n_tx], b["est"]))
+        frac = sum(cls[q].n_tx for q in pick) / sum(c.n_tx for c in cls)
+        est_all = tsum / frac
+        res[name]["one_handle_at_a_time_s"] = est_all
+        res[name]["speedup_vs_one_at_a_time"] = est_all / res[name]["seconds"]
+        res[name]["bitwise_equal_own_handles"] = same
+        log("ragged %s one handle at a time: %.3fs for %d samples (%.1f%% of "
+            "transcripts) -> %.2fs for all; x%.2f; bitwise %s" % (
+                name, tsum, nseq, 100 * frac, est_all,
+                res[name]["speedup_vs_one_at_a_time"], same))
+    # a gradient pass per sample at its EM solution
+    tg = 0.0
+    for q in range(S):
+        p = H.Problem(cls[q])
+        g = p.solve(us[q], norm=H.NORM_TREE, **sk)
+        with np.errstate(divide="ignore"):
+            th = np.log(g["lam"] * cls[q].tx_len)
+        th -= th[np.isfinite(th)].max()
+        ell = np.log(cls[q].tx_len.astype(float))
+        p.grad(us[q], th, ell)
+        tg += p.grad(us[q], th, ell)["ms"]
+        p.close()
+    res["grad_pass_ms_total"] = tg
+    res["grad_pass_note"] = "hem_grad at each sample's EM solution, device ms"
+    log("ragged gradient passes: %.2f ms for %d samples" % (tg, S))
     return res
 
 

@@ -74,6 +74,9 @@ def main():
                          "comparison")
     ap.add_argument("--cpu-boot-threads", type=int, default=16,
                     help="threads (= replicates) of the CPU bootstrap sample")
+    ap.add_argument("--skip-plain", action="store_true",
+                    help="skip the plain solves (profiling runs of the other "
+                         "sections); the JSON line then has no headline")
     ap.add_argument("--ragged-samples", type=int, default=64,
                     help="samples of the ragged multi-sample batch (0: skip)")
     ap.add_argument("--ragged-reps", type=int, default=20)
@@ -111,6 +114,17 @@ def main():
 
     sk = dict(stop_rel_change=args.stop_rel, min_iter=args.min_iter,
               max_iter=args.max_iter)
+    if args.skip_plain:
+        out = {"metric": METRIC, "skipped": "plain solves (--skip-plain)"}
+        if args.bootstrap_reps > 0:
+            boot = bench_bootstrap(args, H, np, p, cls, u, sk)
+            boot.pop("_est_exact", None)
+            out["bootstrap"] = boot
+        if args.ragged_samples > 0:
+            out["ragged"] = bench_ragged(args, H, np)
+        p.close()
+        print(json.dumps(out), flush=True)
+        return
     results = {}
     for name, mk in modes.items():
         steps = args.steps if name == head_name else max(args.steps // 2, 1)
@@ -260,7 +274,8 @@ def traffic_from_profile(kind, args):
     if (args.n_tx, args.n_cls, args.n_frag, args.seed) != (
             200000, 2000000, 60_000_000, 0):
         return None
-    kernel = {"transcript_pass": "k_gather_exact"}.get(kind)
+    kernel = {"tile_map12": "k_ptile_map12",
+              "tile_ex3": "k_ptile_ex3"}.get(kind)
     path = os.path.join(ROOT, "profiles",
                         "em_kernels_200k_tx_2M_cls_counters.md")
     if kernel is None or not os.path.exists(path):

@@ -160,6 +160,32 @@ def test_bootstrap_end_to_end_bitwise(cfg1):
     check_replicates(cls, uo, co, res)
 
 
+def test_bootstrap_after_plain_solves_on_same_handle(cfg1):
+    """Plain solves (tile kernels, exact sums, graphs) and then bootstraps
+    on one handle give what a fresh handle gives, bit for bit."""
+    cls, u = cfg1
+    seeds = np.arange(3, dtype=np.uint64) + 5
+    p = H.Problem(cls)
+    for norm in (H.NORM_EXACT, H.NORM_TREE, H.NORM_SEQ):
+        p.solve(u, norm=norm, stop_rel_change=1e-8)
+    p.solve(u, precision=H.PREC_FP32, norm=H.NORM_TREE, max_iter=60)
+    a = p.bootstrap(u, seeds[:2])
+    a = p.bootstrap(u, seeds)
+    b = p.bootstrap(u, seeds, norm=H.NORM_TREE)
+    g = p.solve(u)
+    p.close()
+    q = H.Problem(cls)
+    a2 = q.bootstrap(u, seeds)
+    b2 = q.bootstrap(u, seeds, norm=H.NORM_TREE)
+    g2 = q.solve(u)
+    q.close()
+    for x, y in ((a, a2), (b, b2)):
+        np.testing.assert_array_equal(x["iters"], y["iters"])
+        np.testing.assert_array_equal(x["est"], y["est"])
+        np.testing.assert_array_equal(x["tpm"], y["tpm"])
+    np.testing.assert_array_equal(g["lam"], g2["lam"])
+
+
 def test_batch_rejects_untileable_and_fp32():
     members = [[i, i + 1] for i in range(99)]
     big = H.Classes.from_lists(100, members, [3.0] * 99, [500] * 100)

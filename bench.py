@@ -86,9 +86,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU "
-              "path (replicates and problems would shard with no "
-              "communication; see DESIGN.md).  Run with --gpus 1.",
+        print("bench.py measures one GPU only: there is no multi-GPU "
+              "path (bootstrap replicates and samples would shard across "
+              "GPUs with no communication).  Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 
@@ -131,15 +131,21 @@ def main():
         for _ in range(args.warmup):
             p.solve(u, **mk, **sk)
         it = 0
+        loop_ms = 0.0
         r = None
         t = time.perf_counter()
         for _ in range(steps):
             r = p.solve(u, **mk, **sk)
             it += r["stats"]["iters"]
+            loop_ms += r["stats"]["ms_solve"]
         dt = time.perf_counter() - t
+        # value: whole calls (unique counts up, tpm/est/lambda down, setup
+        # and finalisation included); loop: HIP events around the
+        # iteration graphs alone, table and counts resident
         results[name] = dict(steps=steps, iters=it, seconds=dt,
                              iters_per_s=it / dt, ms_per_solve=dt / steps * 1e3,
                              iters_per_solve=r["stats"]["iters"],
+                             loop_iters_per_s=it / (loop_ms * 1e-3),
                              exact_sum_reruns=r["stats"]["sum_reruns"])
         log("%-10s %d solves %.4fs  %.1f it/s  k=%d" % (
             name, steps, dt, it / dt, r["stats"]["iters"]))
@@ -147,8 +153,8 @@ def main():
     head = results[head_name]
     prof = p.profile(u, **modes[head_name], **sk)
     kinds = [k for k in prof if k != "iters"]
-    # the roofline names the dominant memory-streaming kernel; the
-    # sequential norm chains are latency-bound by construction
+    # the roofline names the dominant memory-streaming kernel; the exact
+    # sums' in-order chunk walk is latency-bound by construction
     stream_kinds = [k for k in kinds if k not in ("norm_pass1", "norm_pass2")
                     or modes[head_name]["norm"] != H.NORM_EXACT]
     dom = max(stream_kinds, key=lambda k: prof[k]["ms_total"])

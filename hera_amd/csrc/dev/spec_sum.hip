@@ -29,9 +29,10 @@
 // Steps 3-4 are pure FP64 additions in the reference's order on values
 // already rounded as the reference rounds them; the rounding of steps 1-2
 // only decides hits and misses.  Sums of squares are guessed within ~2
-// ulps, so their windows are narrow (8 candidates, eight chunks per wave);
-// the first pass of the sd form (signed values whose running sum can
-// return to zero) gets wide ones.
+// ulps, so their windows are narrow (8 or 16 candidates: the narrowest
+// whose 64/W chunks per wave fit the window kernel's LDS); the first pass
+// of the sd form (signed values whose running sum can return to zero) gets
+// 64.  A miss re-speculates the next few chunks from the exact value.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 

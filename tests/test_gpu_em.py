@@ -137,6 +137,47 @@ def test_cfg2_exact_fp64_bitwise():
     assert kg == ko
 
 
+def untiled_problem(seed):
+    # a generated table plus a chain of classes that joins 151 transcripts
+    # into one component, more than a tile holds, so the solve runs the
+    # two global passes
+    cls, u = H.generate(2000, 20000, 1_000_000, seed=seed)
+    members = [cls.cls_ref[cls.cls_off[i]:cls.cls_off[i + 1]].tolist()
+               for i in range(cls.n_cls)]
+    members += [[i, i + 1] for i in range(150)]
+    counts = np.concatenate([cls.cls_count, np.full(150, 3.0)])
+    return H.Classes.from_lists(cls.n_tx, members, counts.tolist(),
+                                cls.tx_len.tolist()), u
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(min_round=200, max_round=200)])
+def test_untiled_table_exact_bitwise(kw):
+    cls, u = untiled_problem(3)
+    p = H.Problem(cls)
+    assert p.tiles()["n_tiles"] == 0
+    p.close()
+    kg, ko = check_parity(cls, u, H.PREC_FP64, 0.0, 1e-12, "untiled",
+                          bitwise=True, **kw)
+    assert kg == ko
+    check_parity(cls, u, H.PREC_FP64, 1e-4, 1e-9, "untiled",
+                 norm=H.NORM_TREE, **kw)
+
+
+@pytest.mark.parametrize("kw", [dict(step_form=1), dict(clamp_form=1)])
+def test_untiled_table_option_forms_bitwise(kw):
+    cls, u = untiled_problem(4)
+    kw = dict(kw, min_round=200, max_round=200)
+    ora = O.estimate_count(cls, u, **kw)
+    gkw = {dict(step_form="step", clamp_form="clamp", min_round="min_iter",
+                max_round="max_iter")[k]: v for k, v in kw.items()}
+    p = H.Problem(cls)
+    g = p.solve(u, **gkw)
+    p.close()
+    assert g["stats"]["iters"] == ora["k"] == 200
+    np.testing.assert_array_equal(g["lam"], ora["lam"])
+    np.testing.assert_array_equal(g["tpm"], ora["tpm"])
+
+
 @pytest.mark.parametrize("kw", [dict(step_form=1), dict(clamp_form=1),
                                 dict(alpha_clamp=0),
                                 dict(stop_rel=1e-8, min_round=1,

@@ -329,12 +329,14 @@ int hemk_ptile_rows(uint32_t T, const uint32_t *txq, const double *A,
  * sample sd), or x * x (SQUARE).  The sum goes to res[row]; reruns are
  * added to *miss when miss != NULL.  With ctl != NULL every kernel returns
  * at once when ctl->done is set.  Scratch, per chain slot: csum, drift,
- * base [nchain][P]; tab [nchain][P][W] (at most HEMK_SPEC_W). */
+ * base [nchain][P]; tab [nchain][P][W] doubles and nx [nchain][P][W] bytes
+ * (W at most HEMK_SPEC_W). */
 #define HEMK_SPEC_W 64
 #define HEMK_SPEC_LMAX 1024   /* longest chunk the kernels accept */
 #define HEMK_SPEC_IDENT 0
 #define HEMK_SPEC_CENTRED 1
 #define HEMK_SPEC_SQUARE 2
+#define HEMK_SPEC_NOLINK 0xFFu   /* nx entry: result outside next window */
 typedef struct {
 	uint32_t n, L, P;      /* values per chain, chunk length, chunks      */
 	uint32_t W;            /* candidates per chunk: 8, 16, 32 or 64       */
@@ -347,6 +349,7 @@ typedef struct {
 	const double *mres;    /* CENTRED: pass-1 sums by row                 */
 	double *csum, *drift, *tab;
 	long long *base;
+	uint8_t *nx;           /* [nchain][P][W] links to the next window     */
 	double *res;           /* sums by row                                 */
 	uint32_t *miss;
 } hemk_spec;

@@ -347,9 +347,94 @@ __global__ __launch_bounds__(SBLOCK) void k_spec_drift(hemk_spec sp,
 		sp.drift[cb + j] = q;
 }
 
-// Step 3: every chunk from W consecutive starts around g1 = the sum of the
-// previous chunks' drifts.  Grid (P / (64/W), chains), one wave; lane
-// (s, c) runs chunk blockIdx.x * 64/W + s from key(g1) - W/2 + c.
+// Step 2 for signed chains: each chunk's drift run exactly, d_j = F_j(g0_j)
+// - g0_j, from the sum g0_j of the earlier chunk sums.  A chunk's rounding
+// depends on where its start lies; a start within a few ulps of the true
+// one reproduces most of it, so these drifts leave about 5x fewer misses
+// than the block-scan drift on signed rows whose running sum returns close
+// to zero.  Grid (ceil(P / 4), chains), SBLOCK threads, one wave per chunk
+// (every lane runs the same chain; its values are read by broadcast).
+template <int XF>
+__global__ __launch_bounds__(SBLOCK) void k_spec_seqdrift(hemk_spec sp,
+                                                          const hemk_ctl *ctl)
+{
+#pragma clang fp contract(off)
+	constexpr int NW = SBLOCK / WAVE;
+	__shared__ double ys[NW * HEMK_SPEC_LMAX];
+	uint32_t row;
+	if (skip(ctl) || !chain_row(sp, blockIdx.y, row))
+		return;
+	const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+	const uint32_t j = blockIdx.x * NW + w;
+	const size_t cb = (size_t)blockIdx.y * sp.P;
+	const bool live = j < sp.P;
+	double g = 0.0;
+	for (uint32_t i = lane; live && i < j; i += WAVE)
+		g += sp.csum[cb + i];
+#pragma unroll
+	for (int k = WAVE / 2; k >= 1; k >>= 1)
+		g += __shfl_xor(g, k, WAVE);
+	const double m = chain_mean<XF>(sp, row);
+	const uint32_t cnt = live ? min(sp.L, sp.n - j * sp.L) : 0;
+	double *yw = ys + w * HEMK_SPEC_LMAX;
+	if (live)
+		stage<XF>(sp.x + (size_t)row * sp.ld + (size_t)j * sp.L, cnt, m, yw,
+		          WAVE);
+	__syncthreads();
+	if (!live)
+		return;
+	const double s = run_chain(g, yw, cnt);
+	if (lane == 0)
+		sp.drift[cb + j] = s - g;
+}
+
+// Guess of chunk j's start from the drifts d:
+//   g(j) = ((B_k + d_64k) + d_64k+1) + ... + d_j-1,   k = j / 64,
+//   B_k  = ((bs_0 + bs_1) + ...) + bs_k-1,  bs_i = xor-tree sum of block i.
+// Whoever needs the guess of a chunk evaluates this same expression, so the
+// window base that chunk j computes for chunk j + 1 is bitwise the one
+// chunk j + 1 runs from.  The wave's chunks j0 .. j0 + CPV - 1 share block
+// k = j0 / 64 (CPV divides 64); every lane gets g(j) and g(j + 1) for its j.
+__device__ __forceinline__ double tree_sum(double v)
+{
+#pragma clang fp contract(off)
+#pragma unroll
+	for (int k = WAVE / 2; k >= 1; k >>= 1)
+		v += __shfl_xor(v, k, WAVE);
+	return v;
+}
+
+template <int CPV>
+__device__ __forceinline__ void chunk_guess(const double *d, uint32_t P,
+                                            uint32_t j0, uint32_t j,
+                                            double &gj, double &gn)
+{
+#pragma clang fp contract(off)
+	const int lane = threadIdx.x % WAVE;
+	const uint32_t k = j0 / WAVE;
+	double B = 0.0;
+	for (uint32_t b = 0; b < k; ++b)
+		B += tree_sum(d[b * WAVE + lane]);
+	const double dv = k * WAVE + lane < P ? d[k * WAVE + lane] : 0.0;
+	const double bs = tree_sum(dv);
+	double g = B;
+	const uint32_t top = j0 + CPV - 1 - k * WAVE;   // largest j - 64k
+	for (uint32_t t = 0; t < top; ++t) {
+		const double y = readlane_d(dv, t);
+		if (k * WAVE + t < j)
+			g += y;
+	}
+	gj = g;
+	const uint32_t lj = j - k * WAVE;
+	const double dj = __shfl(dv, (int)(lj < WAVE ? lj : 0), WAVE);
+	gn = lj + 1 == WAVE ? B + bs : g + dj;
+}
+
+// Step 3: every chunk from W consecutive starts around its guess.  Grid
+// (P / (64/W), chains), one wave; lane (s, c) runs chunk blockIdx.x * 64/W
+// + s from key(g) - W/2 + c.  Besides the results (tab) and the window base,
+// each candidate's result is looked up in the next chunk's window: nx holds
+// its offset there, or HEMK_SPEC_NOLINK.
 template <int W, int XF>
 __global__ __launch_bounds__(WAVE) void k_spec_window(hemk_spec sp,
                                                       const hemk_ctl *ctl)
@@ -364,31 +449,8 @@ __global__ __launch_bounds__(WAVE) void k_spec_window(hemk_spec sp,
 	const uint32_t j0 = blockIdx.x * CPV;
 	const uint32_t j = j0 + sub;
 	const size_t cb = (size_t)blockIdx.y * sp.P;
-	// g1: the prefix up to j0 over the wave (identical in every lane),
-	// then this lane's chunk's few predecessors in the group
-	double g = 0.0;
-	for (uint32_t i = lane; i < j0; i += 8 * WAVE) {
-		double d[8];
-#pragma unroll
-		for (int q = 0; q < 8; ++q)
-			d[q] = i + q * WAVE < j0 ? sp.drift[cb + i + q * WAVE] : 0.0;
-#pragma unroll
-		for (int q = 0; q < 8; ++q)
-			g += d[q];
-	}
-#pragma unroll
-	for (int k = WAVE / 2; k >= 1; k >>= 1)
-		g += __shfl_xor(g, k, WAVE);
-	{
-		double d[CPV];
-#pragma unroll
-		for (int q = 0; q < CPV; ++q)
-			d[q] = j0 + q < j && j0 + q < sp.P ? sp.drift[cb + j0 + q]
-			                                   : 0.0;
-#pragma unroll
-		for (int q = 0; q < CPV; ++q)
-			g += d[q];
-	}
+	double g, gn;
+	chunk_guess<CPV>(sp.drift + cb, sp.P, j0, j, g, gn);
 	const double m = chain_mean<XF>(sp, row);
 	const double *x = sp.x + (size_t)row * sp.ld;
 	const uint32_t nch = min((uint32_t)CPV, sp.P - j0);
@@ -401,26 +463,30 @@ __global__ __launch_bounds__(WAVE) void k_spec_window(hemk_spec sp,
 	const uint32_t cnt = min(sp.L, sp.n - j * sp.L);
 	const double s = run_chain(dunkey(kadd(base, c)), ys + sub * sp.L, cnt);
 	sp.tab[(cb + j) * W + c] = s;
+	uint32_t link = HEMK_SPEC_NOLINK, idx;
+	if (j + 1 < sp.P &&
+	    in_window(ukey(s), kadd(dkey(gn), -(W / 2)), W, idx))
+		link = idx;
+	sp.nx[(cb + j) * W + c] = (uint8_t)link;
 	if (c == 0)
 		sp.base[cb + j] = base;
 }
 
-// Repairs after a miss.  A miss at chunk j means the guess g1_j was off by
-// delta = s_j - g1_j, and the following guesses are usually off by about
-// the same amount.  Wave 0 runs chunks j .. j+RC-1 at once: group 0 from
-// the exact s_j, group k from the RW doubles centred on g1_{j+k} + delta;
+// Repairs after a miss.  A miss at chunk j means the guess g_j was off by
+// delta = s_j - g_j, and the following guesses are usually off by about the
+// same amount.  Wave 0 runs chunks j .. j+RC-1 at once: group 0 from the
+// exact s_j, group k from the RW doubles centred on g_{j+k} + delta;
 // results go to rtl[k*RW + c], window bases to rbl[k].
 #define RW 16
 #define RC (WAVE / RW)
 static_assert(RC * HEMK_SPEC_LMAX * 8 <= WIN_LDS, "repair chunks fit LDS");
-// candidate tables held in LDS by the resolve kernel at once (bytes)
-#define RES_LDS (120 * 1024)
+// chunks whose links the resolve kernel holds in LDS at once
+#define SEGP 1024
 
 template <int W, int XF>
 __device__ __forceinline__ void repair(const hemk_spec &sp, uint32_t row,
-                                       const long long *bsl, uint32_t jb,
-                                       uint32_t j, double m, double s,
-                                       double *ys, double *rtl,
+                                       size_t cb, uint32_t j, double m,
+                                       double s, double *ys, double *rtl,
                                        long long *rbl)
 {
 #pragma clang fp contract(off)
@@ -432,12 +498,11 @@ __device__ __forceinline__ void repair(const hemk_spec &sp, uint32_t row,
 	if (threadIdx.x < WAVE) {
 		const int lane = threadIdx.x, k = lane / RW, c = lane % RW;
 		const uint32_t jj = j + k;
-		const double delta = s - dunkey(kadd(bsl[j - jb], W / 2));
+		const double delta = s - dunkey(kadd(sp.base[cb + j], W / 2));
 		long long b = 0;
 		double st = s;
 		if (k > 0 && jj < sp.P) {
-			const long long bj = sp.base[(size_t)blockIdx.x * sp.P + jj];
-			const double g = dunkey(kadd(bj, W / 2)) + delta;
+			const double g = dunkey(kadd(sp.base[cb + jj], W / 2)) + delta;
 			b = kadd(dkey(g), -(RW / 2));
 			st = dunkey(kadd(b, c));
 		}
@@ -452,51 +517,29 @@ __device__ __forceinline__ void repair(const hemk_spec &sp, uint32_t row,
 	__syncthreads();
 }
 
-// One chunk of the walk: look the running value up in the chunk's window
-// (row r, base b), else in the current repair's, else repair; then load r
-// and b with the chunk two ahead.
-template <int W, int XF>
-__device__ __forceinline__ void walk_step(
-    const hemk_spec &sp, uint32_t row, double m, uint32_t jb, uint32_t q,
-    uint32_t ns, uint32_t c, const double *tbl, const long long *bsl,
-    double *ys, double *rtl, long long *rbl, double &r, long long &b,
-    double &s, uint32_t &rj, bool &rep, uint32_t &misses)
+__device__ __forceinline__ uint32_t ufirst(uint32_t v)
 {
-	const uint32_t j = jb + q;
-	const unsigned long long ks = ukey(s);
-	uint32_t idx;
-	if (in_window(ks, readfirst_ll(b), W, idx)) {
-		s = readlane_d(r, idx);
-	} else if (rep && j - rj < RC &&
-	           in_window(ks, readfirst_ll(rbl[j - rj]), RW, idx)) {
-		s = readfirst_d(rtl[(j - rj) * RW + idx]);
-	} else {
-		repair<W, XF>(sp, row, bsl, jb, j, m, s, ys, rtl, rbl);
-		rj = j;
-		rep = true;
-		s = readfirst_d(rtl[0]);
-		++misses;
-	}
-	const uint32_t qn = min(q + 2, ns - 1);
-	r = tbl[qn * W + c];
-	b = bsl[qn];
+	return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
 
-// Step 4: walk the chunks in order, one workgroup per chain.  The chain's
-// candidate tables are loaded into LDS (a segment of chunks at a time, the
-// whole chain when it fits), then every wave walks them redundantly with
-// the running value in scalar registers, so every branch is uniform and
-// all threads can stage a repair's values.  The candidate row and window
-// base of the next two chunks are read from LDS ahead of their turn; the
-// lookup itself is scalar arithmetic and one readlane.
+// Step 4: follow the chain through the windows, one workgroup per chain.
+// The walk holds either the exact start s of chunk j or the offset e of
+// that start in chunk j's window.  With an offset, the next chunk's offset
+// is nx[j][e]; composed links over 16 and 256 chunks (built in LDS per
+// segment of SEGP chunks) let it skip ahead, and the result of the last
+// chunk is tab[P-1][e].  A missing link means chunk j+1's exact start
+// tab[j][e] lies outside its window: the walk takes that value, and a start
+// that is in neither its window nor the current repair's starts a repair.
+// Every wave walks redundantly (all branches uniform), so all threads can
+// stage a repair's values.
 template <int W, int XF>
 __global__ __launch_bounds__(SBLOCK) void k_spec_resolve(hemk_spec sp,
                                                          const hemk_ctl *ctl)
 {
 #pragma clang fp contract(off)
-	constexpr uint32_t SEG = RES_LDS / (W * 8 + 8);  // chunks per segment
-	__shared__ double tbl[SEG * W];
-	__shared__ long long bsl[SEG];
+	__shared__ uint8_t nxl[SEGP * W];
+	__shared__ uint8_t c1[SEGP / 16 * W];
+	__shared__ uint8_t c2[SEGP / 256 * W];
 	__shared__ double ys[WIN_LDS / 8];
 	__shared__ double rtl[WAVE];
 	__shared__ long long rbl[RC];
@@ -504,56 +547,110 @@ __global__ __launch_bounds__(SBLOCK) void k_spec_resolve(hemk_spec sp,
 	if (skip(ctl) || !chain_row(sp, blockIdx.x, row))
 		return;
 	const int tid = threadIdx.x;
-	const uint32_t c = (uint32_t)tid % W;
 	const size_t cb = (size_t)blockIdx.x * sp.P;
 	const double m = chain_mean<XF>(sp, row);
 	const uint32_t P = sp.P;
 	double s = 0.0;
-	uint32_t rj = 0, misses = 0;
-	bool rep = false;
-	for (uint32_t jb = 0; jb < P; jb += SEG) {
-		const uint32_t ns = min(SEG, P - jb);
+	uint32_t j = 0, e = 0, rj = 0, misses = 0;
+	bool have = false, rep = false;
+	for (uint32_t jb = 0; jb < P; jb += SEGP) {
+		const uint32_t ns = min((uint32_t)SEGP, P - jb);
+		const uint32_t last = P - 1 - jb;
 		__syncthreads();
 		{
-			// sixteen loads in flight per thread, the bases with them
-			const double *t = sp.tab + (cb + jb) * W;
-			const uint32_t cnt = ns * W;
-			long long bv[(SEG + SBLOCK - 1) / SBLOCK];
+			// W is a multiple of 8: 8-byte pieces, eight in flight
+			const unsigned long long *src =
+			    (const unsigned long long *)(sp.nx + (cb + jb) * W);
+			unsigned long long *dst = (unsigned long long *)nxl;
+			const uint32_t cnt = ns * W / 8;
+			for (uint32_t i = tid; i < cnt; i += 8 * SBLOCK) {
+				unsigned long long v[8];
 #pragma unroll
-			for (int q = 0; q < (int)((SEG + SBLOCK - 1) / SBLOCK); ++q)
-				if (tid + q * SBLOCK < (int)ns)
-					bv[q] = sp.base[cb + jb + tid + q * SBLOCK];
-			for (uint32_t i = tid; i < cnt; i += 16 * SBLOCK) {
-				double v[16];
-#pragma unroll
-				for (int q = 0; q < 16; ++q)
+				for (int q = 0; q < 8; ++q)
 					if (i + q * SBLOCK < cnt)
-						v[q] = t[i + q * SBLOCK];
+						v[q] = src[i + q * SBLOCK];
 #pragma unroll
-				for (int q = 0; q < 16; ++q)
+				for (int q = 0; q < 8; ++q)
 					if (i + q * SBLOCK < cnt)
-						tbl[i + q * SBLOCK] = v[q];
+						dst[i + q * SBLOCK] = v[q];
 			}
-#pragma unroll
-			for (int q = 0; q < (int)((SEG + SBLOCK - 1) / SBLOCK); ++q)
-				if (tid + q * SBLOCK < (int)ns)
-					bsl[tid + q * SBLOCK] = bv[q];
 		}
 		__syncthreads();
-		// two register pairs (row, base), each reloaded right after its
-		// chunk's lookup with the chunk two ahead, so a lookup waits only
-		// for reads issued two steps earlier
-		double ra = tbl[c], rb = tbl[min(1u, ns - 1) * W + c];
-		long long ba = bsl[0], bb = bsl[min(1u, ns - 1)];
-		for (uint32_t q = 0; q < ns; q += 2) {
-			walk_step<W, XF>(sp, row, m, jb, q, ns, c, tbl, bsl, ys, rtl,
-			                 rbl, ra, ba, s, rj, rep, misses);
-			if (q + 1 < ns)
-				walk_step<W, XF>(sp, row, m, jb, q + 1, ns, c, tbl, bsl,
-				                 ys, rtl, rbl, rb, bb, s, rj, rep,
-				                 misses);
+		// links over 16 chunks: block b maps an offset at chunk jb + 16b
+		// to one at jb + 16b + 16 (that chunk exists, its links in LDS)
+		const uint32_t nb1 = min(ns / 16, last / 16);
+		for (uint32_t it = tid; it < nb1 * W; it += SBLOCK) {
+			const uint32_t b = it / W;
+			uint32_t x = it % W;
+			for (int t = 0; t < 16 && x != HEMK_SPEC_NOLINK; ++t)
+				x = nxl[(16 * b + t) * W + x];
+			c1[it] = (uint8_t)x;
+		}
+		__syncthreads();
+		const uint32_t nb2 = min(ns / 256, last / 256);
+		for (uint32_t it = tid; it < nb2 * W; it += SBLOCK) {
+			const uint32_t b = it / W;
+			uint32_t x = it % W;
+			for (int t = 0; t < 16 && x != HEMK_SPEC_NOLINK; ++t)
+				x = c1[(16 * b + t) * W + x];
+			c2[it] = (uint8_t)x;
+		}
+		__syncthreads();
+		while (j < jb + ns) {
+			if (!have) {
+				const unsigned long long ks = ukey(s);
+				uint32_t idx;
+				if (in_window(ks, readfirst_ll(sp.base[cb + j]), W,
+				              idx)) {
+					e = ufirst(idx);
+					have = true;
+				} else if (rep && j - rj < RC &&
+				           in_window(ks, readfirst_ll(rbl[j - rj]), RW,
+				                     idx)) {
+					s = readfirst_d(rtl[(j - rj) * RW + ufirst(idx)]);
+					++j;
+					continue;
+				} else {
+					repair<W, XF>(sp, row, cb, j, m, s, ys, rtl, rbl);
+					rj = j;
+					rep = true;
+					s = readfirst_d(rtl[0]);
+					++misses;
+					++j;
+					continue;
+				}
+			}
+			if (j == P - 1)
+				break;
+			const uint32_t q = j - jb;
+			uint32_t x = HEMK_SPEC_NOLINK;
+			if ((q & 255) == 0 && (q >> 8) < nb2)
+				x = ufirst(c2[(q >> 8) * W + e]);
+			if (x != HEMK_SPEC_NOLINK) {
+				e = x;
+				j += 256;
+				continue;
+			}
+			if ((q & 15) == 0 && (q >> 4) < nb1)
+				x = ufirst(c1[(q >> 4) * W + e]);
+			if (x != HEMK_SPEC_NOLINK) {
+				e = x;
+				j += 16;
+				continue;
+			}
+			x = ufirst(nxl[q * W + e]);
+			if (x != HEMK_SPEC_NOLINK) {
+				e = x;
+				++j;
+				continue;
+			}
+			s = readfirst_d(sp.tab[(cb + j) * W + e]);
+			have = false;
+			++j;
 		}
 	}
+	if (have)
+		s = readfirst_d(sp.tab[(cb + P - 1) * W + e]);
 	if (tid == 0) {
 		sp.res[row] = s;
 		if (misses && sp.miss)
@@ -572,7 +669,13 @@ int launch_xf(const hemk_spec &sp, int produce, const double *u,
 		                                       (double *)sp.x, ctl);
 	else
 		k_spec_csum<XF><<<grid, SBLOCK, 0, st>>>(sp, ctl);
-	k_spec_drift<XF><<<grid, SBLOCK, 0, st>>>(sp, ctl);
+	if (XF == HEMK_SPEC_IDENT)
+		k_spec_seqdrift<XF>
+		    <<<dim3((sp.P + SBLOCK / WAVE - 1) / (SBLOCK / WAVE),
+		            sp.nchain),
+		       SBLOCK, 0, st>>>(sp, ctl);
+	else
+		k_spec_drift<XF><<<grid, SBLOCK, 0, st>>>(sp, ctl);
 #define SPEC_W_CASE(WW)                                                    \
 	case WW:                                                           \
 		k_spec_window<WW, XF>                                      \
@@ -626,6 +729,7 @@ int hemk_spec_sum(const hemk_spec *sp, int produce, const double *u,
 	    sp->n > 0xFFFFFFFFu - sp->L ||
 	    sp->P != (sp->n + sp->L - 1) / sp->L ||
 	    (uint64_t)(WAVE / (sp->W ? sp->W : 1)) * sp->L * 8 > WIN_LDS ||
+	    !sp->nx || !sp->tab || !sp->base ||
 	    (produce && (sp->nchain != 2 || sp->pairs || sp->alist ||
 	                 sp->xf == HEMK_SPEC_CENTRED)))
 		return (int)hipErrorInvalidValue;

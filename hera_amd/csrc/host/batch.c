@@ -138,6 +138,7 @@ void hem_batch_free(hem_problem *p)
 	hemd_free(w->sc);
 	hemd_free(w->sbase);
 	hemd_free(w->stab);
+	hemd_free(w->snx);
 	hemd_host_free(w->h_active);
 	memset(w, 0, sizeof(*w));
 	for (int i = 0; i < 9; ++i) {
@@ -194,6 +195,7 @@ static int ws_ensure(hem_problem *p, uint32_t R)
 	WALLOC(w->sc, (size_t)4 * R * p->spec_P * 8);
 	WALLOC(w->sbase, (size_t)2 * R * p->spec_P * 8);
 	WALLOC(w->stab, (size_t)2 * R * p->spec_P * HEMK_SPEC_W * 8);
+	WALLOC(w->snx, (size_t)2 * R * p->spec_P * HEMK_SPEC_W);
 #undef WALLOC
 	if (hemd_host_alloc((void **)&w->h_active,
 	                    (size_t)(p->ns > 16 ? p->ns : 16) * 4))
@@ -278,6 +280,7 @@ static hemk_spec spec_batch(hem_problem *p, hemk_rctl rc, int pairs, int xf,
 	sp.drift = w->sc + 2 * cap * p->spec_P;
 	sp.tab = w->stab;
 	sp.base = w->sbase;
+	sp.nx = w->snx;
 	sp.res = res;
 	sp.miss = reruns_word(p);
 	return sp;

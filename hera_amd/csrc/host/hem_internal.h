@@ -36,6 +36,7 @@ typedef struct {
 	                                  and drifts                        */
 	long long *sbase;              /* [2R][P] window bases              */
 	double *stab;                  /* [2R][P][HEMK_SPEC_W] candidates   */
+	uint8_t *snx;                  /* [2R][P][HEMK_SPEC_W] window links */
 	uint32_t *h_active;            /* pinned                            */
 	/* captured iterations; the loop bounds and stop thresholds are kernel
 	 * arguments, so a graph is reused only for identical options */
@@ -94,6 +95,7 @@ struct hem_problem {
 	uint32_t spec_L, spec_P;
 	double *d_sx, *d_scsum, *d_sdrift, *d_stab, *d_sres;
 	long long *d_sbase;
+	uint8_t *d_snx;
 	hemk_ctl *d_ctl;
 	hemk_ctl *h_ctl; /* pinned */
 

@@ -239,6 +239,7 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 	DALLOC_H(p->d_sdrift, 2 * (size_t)p->spec_P * 8);
 	DALLOC_H(p->d_sbase, 2 * (size_t)p->spec_P * 8);
 	DALLOC_H(p->d_stab, 2 * (size_t)p->spec_P * HEMK_SPEC_W * 8);
+	DALLOC_H(p->d_snx, 2 * (size_t)p->spec_P * HEMK_SPEC_W);
 	DALLOC_H(p->d_sres, 4 * 8);
 	DALLOC_H(p->d_ctl, sizeof(hemk_ctl));
 #undef DALLOC_H
@@ -477,6 +478,7 @@ int hem_problem_destroy(hem_problem *p)
 	hemd_free(p->d_sdrift);
 	hemd_free(p->d_sbase);
 	hemd_free(p->d_stab);
+	hemd_free(p->d_snx);
 	hemd_free(p->d_sres);
 	hemd_free(p->d_ctl);
 	hemd_host_free(p->h_ctl);

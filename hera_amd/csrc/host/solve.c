@@ -108,6 +108,7 @@ static hemk_spec spec_plain(const hem_problem *p, uint32_t nchain, int xf,
 	sp.drift = p->d_sdrift;
 	sp.tab = p->d_stab;
 	sp.base = p->d_sbase;
+	sp.nx = p->d_snx;
 	sp.res = res;
 	sp.miss = &p->d_ctl->reruns;
 	return sp;
@@ -620,6 +621,7 @@ int hem_exact_sums(const double *x, uint32_t n, uint32_t n_rows, int mode,
 	int rc, e = 0;
 	void *s = NULL;
 	double *d_x = NULL, *d_sc = NULL, *d_tab = NULL, *d_res = NULL;
+	uint8_t *d_nx = NULL;
 	long long *d_base = NULL;
 	uint32_t *d_miss = NULL, miss = 0;
 	hemk_spec sp;
@@ -645,6 +647,7 @@ int hem_exact_sums(const double *x, uint32_t n, uint32_t n_rows, int mode,
 	if (!e) e = hemd_malloc((void **)&d_sc, cp * 2 * 8);
 	if (!e) e = hemd_malloc((void **)&d_base, cp * 8);
 	if (!e) e = hemd_malloc((void **)&d_tab, cp * HEMK_SPEC_W * 8);
+	if (!e) e = hemd_malloc((void **)&d_nx, cp * HEMK_SPEC_W);
 	if (!e) e = hemd_malloc((void **)&d_res, (size_t)n_rows * 2 * 8);
 	if (!e) e = hemd_malloc((void **)&d_miss, 4);
 	if (e) {
@@ -657,6 +660,7 @@ int hem_exact_sums(const double *x, uint32_t n, uint32_t n_rows, int mode,
 	sp.drift = d_sc + cp;
 	sp.tab = d_tab;
 	sp.base = d_base;
+	sp.nx = d_nx;
 	sp.miss = d_miss;
 	sp.res = d_res;
 	sp.xf = mode == HEM_SUM_SQUARES ? HEMK_SPEC_SQUARE : HEMK_SPEC_IDENT;
@@ -692,6 +696,7 @@ out:
 	hemd_free(d_sc);
 	hemd_free(d_base);
 	hemd_free(d_tab);
+	hemd_free(d_nx);
 	hemd_free(d_res);
 	hemd_free(d_miss);
 	if (s)

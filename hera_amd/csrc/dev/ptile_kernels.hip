@@ -70,32 +70,87 @@ __device__ __forceinline__ PLds pcarve(void *smem, const hemk_tiles &tl)
 	return L;
 }
 
-// Structure, counts and lengths of the tile into LDS.
+// Structure, counts and lengths of the tile into LDS.  The loads of a
+// typical tile (up to 128 classes, 512 members, 64 transcripts) are all
+// issued before the first LDS store, so the tile arrives in one round of
+// memory latency; larger tiles finish in plain loops.
+#define PK_Z 8   // members per lane in the first round
 __device__ __forceinline__ void pstage(const hemk_tiles &tl, const PTile &tc,
                                        const PLds &L,
                                        const uint32_t *__restrict__ uq,
                                        const double *__restrict__ cntq)
 {
-	const int tid = threadIdx.x;
-	for (uint32_t i = tid; i <= tc.nc; i += PBLOCK)
-		L.coff[i] = (uint16_t)(tl.coff[tc.cls_base + i] - tc.mb);
-	for (uint32_t i = tid; i < tc.nc; i += PBLOCK)
-		L.n[i] = cntq[tc.cls_base + i];
-	for (uint32_t i = tid; i <= tc.nt; i += PBLOCK)
-		L.toff[i] = (uint16_t)(tl.toff[tc.tx_base + i] - tc.jb);
-	for (uint32_t i = tid; i < tc.nz; i += PBLOCK) {
-		L.kmem[i] = tl.kmem[tc.mb + i];
-		L.tpos[i] = tl.tpos[tc.jb + i];
+	const uint32_t tid = threadIdx.x;
+	uint32_t km[PK_Z], co[2], to, tx, ln, u;
+	uint16_t tp[PK_Z];
+	double nn[2];
+#pragma unroll
+	for (int q = 0; q < PK_Z; ++q) {
+		const uint32_t i = tid + q * PBLOCK;
+		if (i < tc.nz) {
+			km[q] = tl.kmem[tc.mb + i];
+			tp[q] = tl.tpos[tc.jb + i];
+		}
 	}
-	for (uint32_t i = tid; i < tc.nt; i += PBLOCK) {
+#pragma unroll
+	for (int q = 0; q < 2; ++q) {
+		const uint32_t i = tid + q * PBLOCK;
+		if (i <= tc.nc)
+			co[q] = tl.coff[tc.cls_base + i];
+		if (i < tc.nc)
+			nn[q] = cntq[tc.cls_base + i];
+	}
+	if (tid <= tc.nt)
+		to = tl.toff[tc.tx_base + tid];
+	if (tid < tc.nt) {
+		tx = tl.tx[tc.tx_base + tid];
+		ln = tl.len[tc.tx_base + tid];
+		u = uq[tc.tx_base + tid];
+	}
+#pragma unroll
+	for (int q = 0; q < PK_Z; ++q) {
+		const uint32_t i = tid + q * PBLOCK;
+		if (i < tc.nz) {
+			L.kmem[i] = km[q];
+			L.tpos[i] = tp[q];
+		}
+	}
+#pragma unroll
+	for (int q = 0; q < 2; ++q) {
+		const uint32_t i = tid + q * PBLOCK;
+		if (i <= tc.nc)
+			L.coff[i] = (uint16_t)(co[q] - tc.mb);
+		if (i < tc.nc)
+			L.n[i] = nn[q];
+	}
+	if (tid <= tc.nt)
+		L.toff[tid] = (uint16_t)(to - tc.jb);
+	if (tid < tc.nt) {
+		L.tx[tid] = tx;
+		L.len[tid] = ln;
+		L.u[tid] = u;
+	}
+	for (uint32_t i = tid + 2 * PBLOCK; i <= tc.nc; i += PBLOCK)
+		L.coff[i] = (uint16_t)(tl.coff[tc.cls_base + i] - tc.mb);
+	for (uint32_t i = tid + 2 * PBLOCK; i < tc.nc; i += PBLOCK)
+		L.n[i] = cntq[tc.cls_base + i];
+	for (uint32_t i = tid + PBLOCK; i <= tc.nt; i += PBLOCK)
+		L.toff[i] = (uint16_t)(tl.toff[tc.tx_base + i] - tc.jb);
+	for (uint32_t i = tid + PBLOCK; i < tc.nt; i += PBLOCK) {
 		L.tx[i] = tl.tx[tc.tx_base + i];
 		L.len[i] = tl.len[tc.tx_base + i];
 		L.u[i] = uq[tc.tx_base + i];
 	}
+	for (uint32_t i = tid + PK_Z * PBLOCK; i < tc.nz; i += PBLOCK) {
+		L.kmem[i] = tl.kmem[tc.mb + i];
+		L.tpos[i] = tl.tpos[tc.jb + i];
+	}
 }
 
 // One map lin -> lout of the tile (src/EM.c:275-291) in three passes;
-// with stop the test of src/EM.c:292-297 against lin sets bad.
+// with stop the test of src/EM.c:292-297 against lin sets bad.  The term
+// pass writes each transcript's terms contiguously in the order its sum
+// takes them, so the ordered sums read LDS without an indirection.
 __device__ __forceinline__ void pmap(const PLds &L, const PTile &tc,
                                      const double *lin, double *lout,
                                      bool stop, double smin, double srel,
@@ -122,15 +177,15 @@ __device__ __forceinline__ void pmap(const PLds &L, const PTile &tc,
 		L.D[c] = acc;
 	}
 	__syncthreads();
-	// (lam_t n_c) / D_c per member; a class with D_c == 0 is skipped by
-	// the reference, which a zero term reproduces (sums start at +0 and
-	// every term is >= 0)
-	for (uint32_t k = tid; k < tc.nz; k += PBLOCK) {
-		const uint32_t m = L.kmem[k];
+	// (lam_t n_c) / D_c for occurrence j of transcript t (member tpos[j]);
+	// a class with D_c == 0 is skipped by the reference, which a zero term
+	// reproduces (sums start at +0 and every term is >= 0)
+	for (uint32_t j = tid; j < tc.nz; j += PBLOCK) {
+		const uint32_t m = L.kmem[L.tpos[j]];
 		const uint32_t c = m >> 8, t = m & 0xFFu;
 		const double d = L.D[c];
 		const double x = lin[t] * L.n[c] / (d == 0.0 ? 1.0 : d);
-		L.X[k] = d == 0.0 ? 0.0 : x;
+		L.X[j] = d == 0.0 ? 0.0 : x;
 	}
 	__syncthreads();
 	for (uint32_t t = tid; t < tc.nt; t += PBLOCK) {
@@ -141,13 +196,13 @@ __device__ __forceinline__ void pmap(const PLds &L, const PTile &tc,
 			double x[8];
 #pragma unroll
 			for (int q = 0; q < 8; ++q)
-				x[q] = L.X[L.tpos[j + q]];
+				x[q] = L.X[j + q];
 #pragma unroll
 			for (int q = 0; q < 8; ++q)
 				acc += x[q];
 		}
 		for (; j < j1; ++j)
-			acc += L.X[L.tpos[j]];
+			acc += L.X[j];
 		const double nw = ((double)L.u[t] + acc) / (double)L.len[t];
 		lout[t] = nw;
 		if (stop && nw > smin && fabs(nw - lin[t]) / nw > srel)

@@ -140,37 +140,48 @@ __device__ __forceinline__ bool skip(const hemk_ctl *ctl)
 	return ctl && ctl->done;
 }
 
-// acc + x[0] + ... + x[cnt-1] left to right (per lane, x in LDS; the next
-// sixteen values are read while the current sixteen are added, which
-// covers the LDS latency with the adds' own latency).
+// acc = acc + y in one FP64 add (round to nearest, as the compiler's own
+// v_add_f64), kept in place: with the running sum pinned to one register
+// the loop below needs no register copies between its buffers.
+__device__ __forceinline__ void add_in_place(double &acc, double y)
+{
+	asm volatile("v_add_f64 %0, %0, %1" : "+v"(acc) : "v"(y));
+}
+
+// acc + x[0] + ... + x[cnt-1] left to right (per lane, x in LDS).  Two
+// register buffers of sixteen values alternate: one is read from LDS while
+// the other is added, so the reads' latency hides behind the adds.
 __device__ __forceinline__ double run_chain(double acc, const double *x,
                                             uint32_t cnt)
 {
 #pragma clang fp contract(off)
 	uint32_t j = 0;
 	if (cnt >= 16) {
-		double a[16];
+		double a[16], b[16];
 #pragma unroll
 		for (int q = 0; q < 16; ++q)
 			a[q] = x[q];
-		for (j = 16; j + 16 <= cnt; j += 16) {
-			double b[16];
+		// a holds x[j-16 .. j-1], not yet added
+		for (j = 16; j + 32 <= cnt; j += 32) {
 #pragma unroll
 			for (int q = 0; q < 16; ++q)
 				b[q] = x[j + q];
 #pragma unroll
 			for (int q = 0; q < 16; ++q)
-				acc += a[q];
+				add_in_place(acc, a[q]);
 #pragma unroll
 			for (int q = 0; q < 16; ++q)
-				a[q] = b[q];
+				a[q] = x[j + 16 + q];
+#pragma unroll
+			for (int q = 0; q < 16; ++q)
+				add_in_place(acc, b[q]);
 		}
 #pragma unroll
 		for (int q = 0; q < 16; ++q)
-			acc += a[q];
+			add_in_place(acc, a[q]);
 	}
 	for (; j < cnt; ++j)
-		acc += x[j];
+		add_in_place(acc, x[j]);
 	return acc;
 }
 

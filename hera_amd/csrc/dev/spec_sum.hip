@@ -13,26 +13,23 @@
 // is monotone in s and cheap to evaluate, but the true start s_j is only
 // known once chunks 0..j-1 are done.  So:
 //   1. chunk sums c_j (any order; they only steer guesses);
-//   2. g0_j = sum_{i<j} c_i; each chunk's drift d_j, the increments
-//      fl(s + y_i) - s summed over the chunk with s an approximate running
-//      sum from g0_j (a parallel scan), captures the chunk's rounding
-//      (small terms absorbed by a large running sum) at the right
-//      magnitude, which plain chunk sums miss by up to ~10^4 ulps;
-//   3. g1_j = sum_{i<j} d_i; every chunk runs F_j from the W consecutive
-//      doubles centred on g1_j (one candidate per lane), all at once;
-//   4. one wave per chain walks the chunks in order from s_0 = +0, the
-//      running value in scalar registers: the exact s_j is looked up among
-//      chunk j's candidates by its ordered bit pattern, which yields
-//      F_j(s_j) bitwise; a start outside the window (a miss) reruns the
-//      chunk from s_j, so the result never depends on how good the guess
-//      was.
-// Steps 3-4 are pure FP64 additions in the reference's order on values
-// already rounded as the reference rounds them; the rounding of steps 1-2
-// only decides hits and misses.  Sums of squares are guessed within ~2
-// ulps, so their windows are narrow (8 or 16 candidates: the narrowest
-// whose 64/W chunks per wave fit the window kernel's LDS); the first pass
-// of the sd form (signed values whose running sum can return to zero) gets
-// 64.  A miss re-speculates the next few chunks from the exact value.
+//   2. each chunk's drift d_j, its end minus its start when run from the
+//      guess g0_j = sum_{i<j} c_i: exactly F_j(g0_j) - g0_j for signed
+//      rows (whose running sum can return close to zero, where a start a
+//      few ulps off changes the rounding), and for sums of squares the
+//      rounded increments fl(s + y) - s over a block scan from g0_j;
+//      plain chunk sums miss the rounding by up to ~10^4 ulps;
+//   3. g_j = sum_{i<j} d_i (a fixed expression, so every kernel forms the
+//      same bits); every chunk runs F_j from the 64 consecutive doubles
+//      centred on g_j (one candidate per lane), all at once, and looks each
+//      result up in the next chunk's window (a link, or none);
+//   4. one workgroup per chain follows the links from s_0 = +0, skipping
+//      ahead over links composed in LDS; where the exact start of a chunk
+//      lies outside its window (a miss), that chunk is run again from it
+//      and the next ones from windows shifted by the guess's error, so the
+//      result never depends on how good the guesses were.
+// Steps 3-4 are FP64 additions in the reference's order on values rounded
+// as the reference rounds them; steps 1-2 only decide hits and misses.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -232,6 +229,32 @@ __device__ __forceinline__ void stage(const double *__restrict__ x,
 			ys[i + q * nthr] = xval<XF>(v[q], m);
 }
 
+// One wave's chunk (cnt <= HEMK_SPEC_LMAX values) through registers:
+// chunk_load only issues the loads (lane l takes values l, l + 64, ...), so
+// whatever the caller loads next is in flight with them; chunk_store
+// applies the transform and writes the values to LDS.
+#define CH_REG (HEMK_SPEC_LMAX / WAVE)
+__device__ __forceinline__ void chunk_load(const double *__restrict__ x,
+                                           uint32_t cnt, double *v)
+{
+	const uint32_t lane = threadIdx.x % WAVE;
+#pragma unroll
+	for (int q = 0; q < CH_REG; ++q)
+		if (lane + q * WAVE < cnt)
+			v[q] = x[lane + q * WAVE];
+}
+
+template <int XF>
+__device__ __forceinline__ void chunk_store(const double *v, uint32_t cnt,
+                                            double m, double *ys)
+{
+	const uint32_t lane = threadIdx.x % WAVE;
+#pragma unroll
+	for (int q = 0; q < CH_REG; ++q)
+		if (lane + q * WAVE < cnt)
+			ys[lane + q * WAVE] = xval<XF>(v[q], m);
+}
+
 // Block sum of one double per thread (SBLOCK threads), fixed order; every
 // thread gets the result.
 __device__ __forceinline__ double block_sum(double a)
@@ -379,18 +402,26 @@ __global__ __launch_bounds__(SBLOCK) void k_spec_seqdrift(hemk_spec sp,
 	const uint32_t j = blockIdx.x * NW + w;
 	const size_t cb = (size_t)blockIdx.y * sp.P;
 	const bool live = j < sp.P;
+	const uint32_t cnt = live ? min(sp.L, sp.n - j * sp.L) : 0;
+	double v[CH_REG];
+	chunk_load(sp.x + (size_t)row * sp.ld + (size_t)j * sp.L, cnt, v);
+	// g0 = the earlier chunk sums (any order: it only steers the guess),
+	// eight loads per lane in flight with the chunk's
 	double g = 0.0;
-	for (uint32_t i = lane; live && i < j; i += WAVE)
-		g += sp.csum[cb + i];
+	for (uint32_t i = lane; live && i < j; i += 8 * WAVE) {
+		double c[8];
+#pragma unroll
+		for (int q = 0; q < 8; ++q)
+			c[q] = i + q * WAVE < j ? sp.csum[cb + i + q * WAVE] : 0.0;
+#pragma unroll
+		for (int q = 0; q < 8; ++q)
+			g += c[q];
+	}
 #pragma unroll
 	for (int k = WAVE / 2; k >= 1; k >>= 1)
 		g += __shfl_xor(g, k, WAVE);
-	const double m = chain_mean<XF>(sp, row);
-	const uint32_t cnt = live ? min(sp.L, sp.n - j * sp.L) : 0;
 	double *yw = ys + w * HEMK_SPEC_LMAX;
-	if (live)
-		stage<XF>(sp.x + (size_t)row * sp.ld + (size_t)j * sp.L, cnt, m, yw,
-		          WAVE);
+	chunk_store<XF>(v, cnt, chain_mean<XF>(sp, row), yw);
 	__syncthreads();
 	if (!live)
 		return;
@@ -424,9 +455,31 @@ __device__ __forceinline__ void chunk_guess(const double *d, uint32_t P,
 	const int lane = threadIdx.x % WAVE;
 	const uint32_t k = j0 / WAVE;
 	double B = 0.0;
-	for (uint32_t b = 0; b < k; ++b)
-		B += tree_sum(d[b * WAVE + lane]);
-	const double dv = k * WAVE + lane < P ? d[k * WAVE + lane] : 0.0;
+	uint32_t b = 0;
+	// full blocks before k, eight at a time (loads first, then the trees,
+	// then B in block order); block k's drifts load with the last group
+	double dv = 0.0;
+	for (;;) {
+		double t[8];
+#pragma unroll
+		for (int q = 0; q < 8; ++q)
+			if (b + q < k)
+				t[q] = d[(b + q) * WAVE + lane];
+		const bool lastg = b + 8 >= k;
+		if (lastg && k * WAVE + lane < P)
+			dv = d[k * WAVE + lane];
+#pragma unroll
+		for (int q = 0; q < 8; ++q)
+			if (b + q < k)
+				t[q] = tree_sum(t[q]);
+#pragma unroll
+		for (int q = 0; q < 8; ++q)
+			if (b + q < k)
+				B += t[q];
+		if (lastg)
+			break;
+		b += 8;
+	}
 	const double bs = tree_sum(dv);
 	double g = B;
 	const uint32_t top = j0 + CPV - 1 - k * WAVE;   // largest j - 64k
@@ -437,42 +490,39 @@ __device__ __forceinline__ void chunk_guess(const double *d, uint32_t P,
 	}
 	gj = g;
 	const uint32_t lj = j - k * WAVE;
-	const double dj = __shfl(dv, (int)(lj < WAVE ? lj : 0), WAVE);
+	const double dj = __shfl(dv, (int)lj, WAVE);
 	gn = lj + 1 == WAVE ? B + bs : g + dj;
 }
 
 // Step 3: every chunk from W consecutive starts around its guess.  Grid
-// (P / (64/W), chains), one wave; lane (s, c) runs chunk blockIdx.x * 64/W
-// + s from key(g) - W/2 + c.  Besides the results (tab) and the window base,
-// each candidate's result is looked up in the next chunk's window: nx holds
-// its offset there, or HEMK_SPEC_NOLINK.
+// (P, chains), one wave per chunk (its values staged in one round of
+// loads); lane c < W runs the chunk from key(g) - W/2 + c.  Besides the
+// results (tab) and the window base, each candidate's result is looked up
+// in the next chunk's window: nx holds its offset there, or
+// HEMK_SPEC_NOLINK.
 template <int W, int XF>
 __global__ __launch_bounds__(WAVE) void k_spec_window(hemk_spec sp,
                                                       const hemk_ctl *ctl)
 {
 #pragma clang fp contract(off)
-	constexpr int CPV = WAVE / W;
-	__shared__ double ys[WIN_LDS / 8];
+	__shared__ double ys[HEMK_SPEC_LMAX];
 	uint32_t row;
 	if (skip(ctl) || !chain_row(sp, blockIdx.y, row))
 		return;
-	const int lane = threadIdx.x, sub = lane / W, c = lane % W;
-	const uint32_t j0 = blockIdx.x * CPV;
-	const uint32_t j = j0 + sub;
+	const int c = threadIdx.x;
+	const uint32_t j = blockIdx.x;
 	const size_t cb = (size_t)blockIdx.y * sp.P;
+	const uint32_t cnt = min(sp.L, sp.n - j * sp.L);
+	double v[CH_REG];
+	chunk_load(sp.x + (size_t)row * sp.ld + (size_t)j * sp.L, cnt, v);
 	double g, gn;
-	chunk_guess<CPV>(sp.drift + cb, sp.P, j0, j, g, gn);
-	const double m = chain_mean<XF>(sp, row);
-	const double *x = sp.x + (size_t)row * sp.ld;
-	const uint32_t nch = min((uint32_t)CPV, sp.P - j0);
-	const uint32_t tot = min(nch * sp.L, sp.n - j0 * sp.L);
-	stage<XF>(x + (size_t)j0 * sp.L, tot, m, ys, WAVE);
+	chunk_guess<1>(sp.drift + cb, sp.P, j, j, g, gn);
+	chunk_store<XF>(v, cnt, chain_mean<XF>(sp, row), ys);
 	__syncthreads();
-	if (j >= sp.P)
+	if (c >= W)
 		return;
 	const long long base = kadd(dkey(g), -(W / 2));
-	const uint32_t cnt = min(sp.L, sp.n - j * sp.L);
-	const double s = run_chain(dunkey(kadd(base, c)), ys + sub * sp.L, cnt);
+	const double s = run_chain(dunkey(kadd(base, c)), ys, cnt);
 	sp.tab[(cb + j) * W + c] = s;
 	uint32_t link = HEMK_SPEC_NOLINK, idx;
 	if (j + 1 < sp.P &&
@@ -690,8 +740,7 @@ int launch_xf(const hemk_spec &sp, int produce, const double *u,
 #define SPEC_W_CASE(WW)                                                    \
 	case WW:                                                           \
 		k_spec_window<WW, XF>                                      \
-		    <<<dim3((sp.P + WAVE / WW - 1) / (WAVE / WW), sp.nchain), \
-		       WAVE, 0, st>>>(sp, ctl);                            \
+		    <<<dim3(sp.P, sp.nchain), WAVE, 0, st>>>(sp, ctl);      \
 		k_spec_resolve<WW, XF><<<sp.nchain, SBLOCK, 0, st>>>(sp, ctl); \
 		break;
 	switch (sp.W) {
@@ -719,15 +768,10 @@ uint32_t hemk_spec_chunk(uint32_t n)
 
 uint32_t hemk_spec_width(uint32_t L, int xf)
 {
-	// sums of squares are guessed within ~2 ulps: the narrowest window
-	// whose chunks fit the window kernel's LDS; signed sums (whose
-	// running sum can pass through zero): 64
-	if (xf == HEMK_SPEC_IDENT)
-		return 64;
-	uint32_t w = 8;
-	while (w < 64 && (uint64_t)(WAVE / w) * L * 8 > WIN_LDS)
-		w *= 2;
-	return w;
+	// signed sums (whose running sum can pass close to zero): 64
+	// candidates; sums of squares are guessed within ~2 ulps: 16
+	(void)L;
+	return xf == HEMK_SPEC_IDENT ? 64 : 16;
 }
 
 int hemk_spec_sum(const hemk_spec *sp, int produce, const double *u,
@@ -739,7 +783,7 @@ int hemk_spec_sum(const hemk_spec *sp, int produce, const double *u,
 	    sp->L > HEMK_SPEC_LMAX || sp->L % SBLOCK != 0 ||
 	    sp->n > 0xFFFFFFFFu - sp->L ||
 	    sp->P != (sp->n + sp->L - 1) / sp->L ||
-	    (uint64_t)(WAVE / (sp->W ? sp->W : 1)) * sp->L * 8 > WIN_LDS ||
+	    sp->W < 8 || sp->W > HEMK_SPEC_W || sp->W % 8 != 0 ||
 	    !sp->nx || !sp->tab || !sp->base ||
 	    (produce && (sp->nchain != 2 || sp->pairs || sp->alist ||
 	                 sp->xf == HEMK_SPEC_CENTRED)))

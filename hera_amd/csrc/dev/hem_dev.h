@@ -88,7 +88,8 @@ typedef struct {
 
 /* ---- component tiles (device view) ---------------------------------------
  * See host/tiles.c.  desc holds {tx_base, n_tx, cls_base, n_cls, member
- * base, members, class-list base, 0} per tile (HEMK_TILE_DESC words). */
+ * base, members, class-list base, record offset} per tile (HEMK_TILE_DESC
+ * words). */
 #define HEMK_TILE_DESC 8
 typedef struct {
 	uint32_t n_tiles, n_tx, n_cls;
@@ -102,6 +103,9 @@ typedef struct {
 	const uint16_t *tpos;   /* class-list entry -> member position in the
 	                           tile's member list                        */
 	const uint32_t *txq;    /* transcript id -> tile position            */
+	const uint32_t *rec;    /* per-tile records (host/tiles.c), 16-byte
+	                           units; tile i's at desc[8i + 7]          */
+	uint32_t max_rec;       /* bytes of the largest record               */
 } hemk_tiles;
 
 /* Per-instance control of a batched solve.  A handle holds ns samples

@@ -25,7 +25,7 @@
 namespace {
 
 struct PTile {
-	uint32_t tx_base, nt, cls_base, nc, mb, nz, jb;
+	uint32_t tx_base, nt, cls_base, nc, nz, rec;
 };
 
 __device__ __forceinline__ PTile ptile(const hemk_tiles &t, uint32_t i)
@@ -36,115 +36,105 @@ __device__ __forceinline__ PTile ptile(const hemk_tiles &t, uint32_t i)
 	c.nt = d[1];
 	c.cls_base = d[2];
 	c.nc = d[3];
-	c.mb = d[4];
 	c.nz = d[5];
-	c.jb = d[6];
+	c.rec = d[7];
 	return c;
 }
 
-// The tile in LDS, sized by the tiling's largest tile.
+// Bytes of a tile's record: u16 class offsets (nc + 1), class-list offsets
+// (nt + 1), members (c << 6 | t) and class-list positions (nz each), then
+// 4-byte aligned u32 lengths and transcript ids (nt each); 16-byte units.
+__device__ __forceinline__ uint32_t rec_bytes(uint32_t nt, uint32_t nc,
+                                              uint32_t nz)
+{
+	uint32_t u16s = (nc + 1) + (nt + 1) + 2 * nz;
+	u16s += u16s & 1;
+	return (2 * u16s + 8 * nt + 15) / 16 * 16;
+}
+
+// The tile in LDS: its record (room for the largest), then the FP64 work
+// arrays and unique counts sized by the tiling's largest tile.
 struct PLds {
 	double *l0, *l1, *l2;  // densities in, after one map, after two
 	double *D, *X, *n;     // class sums, member terms, class counts
-	uint32_t *u, *len, *tx, *kmem;
-	uint16_t *coff, *toff, *tpos;
+	uint32_t *u;
+	uint4 *rec;
+	// record fields of the current tile
+	const uint16_t *coff, *toff, *kmem, *tpos;
+	const uint32_t *len, *tx;
 };
 
 __device__ __forceinline__ PLds pcarve(void *smem, const hemk_tiles &tl)
 {
 	const uint32_t mt = tl.max_tx, mc = tl.max_cls, mz = tl.max_nnz;
 	PLds L;
-	L.l0 = (double *)smem;
+	L.rec = (uint4 *)smem;
+	L.l0 = (double *)((char *)smem + tl.max_rec);
 	L.l1 = L.l0 + mt;
 	L.l2 = L.l1 + mt;
 	L.D = L.l2 + mt;
 	L.X = L.D + mc;
 	L.n = L.X + mz;
 	L.u = (uint32_t *)(L.n + mc);
-	L.len = L.u + mt;
-	L.tx = L.len + mt;
-	L.kmem = L.tx + mt;
-	L.coff = (uint16_t *)(L.kmem + mz);
-	L.toff = L.coff + (mc + 1);
-	L.tpos = L.toff + (mt + 1);
 	return L;
 }
 
-// Structure, counts and lengths of the tile into LDS.  The loads of a
-// typical tile (up to 128 classes, 512 members, 64 transcripts) are all
-// issued before the first LDS store, so the tile arrives in one round of
-// memory latency; larger tiles finish in plain loops.
-#define PK_Z 8   // members per lane in the first round
+__device__ __forceinline__ void pfields(PLds &L, const PTile &tc)
+{
+	const uint16_t *r = (const uint16_t *)L.rec;
+	L.coff = r;
+	L.toff = L.coff + tc.nc + 1;
+	L.kmem = L.toff + tc.nt + 1;
+	L.tpos = L.kmem + tc.nz;
+	uint32_t k = (tc.nc + 1) + (tc.nt + 1) + 2 * tc.nz;
+	k += k & 1;
+	L.len = (const uint32_t *)(r + k);
+	L.tx = L.len + tc.nt;
+}
+
+// The tile's record, class counts and unique counts into LDS.  The loads
+// of a typical tile (a record up to 4 KB, 128 classes, 64 transcripts)
+// are all issued before the first LDS store, so the tile arrives in one
+// round of memory latency; larger tiles finish in plain loops.
+#define PK_R 4   // 16-byte record pieces per lane in the first round
 __device__ __forceinline__ void pstage(const hemk_tiles &tl, const PTile &tc,
                                        const PLds &L,
                                        const uint32_t *__restrict__ uq,
                                        const double *__restrict__ cntq)
 {
 	const uint32_t tid = threadIdx.x;
-	uint32_t km[PK_Z], co[2], to, tx, ln, u;
-	uint16_t tp[PK_Z];
+	const uint32_t n16 = rec_bytes(tc.nt, tc.nc, tc.nz) / 16;
+	const uint4 *src = (const uint4 *)tl.rec + tc.rec;
+	uint4 rv[PK_R];
 	double nn[2];
+	uint32_t u = 0;
 #pragma unroll
-	for (int q = 0; q < PK_Z; ++q) {
-		const uint32_t i = tid + q * PBLOCK;
-		if (i < tc.nz) {
-			km[q] = tl.kmem[tc.mb + i];
-			tp[q] = tl.tpos[tc.jb + i];
-		}
-	}
+	for (int q = 0; q < PK_R; ++q)
+		rv[q] = tid + q * PBLOCK < n16 ? src[tid + q * PBLOCK]
+		                               : make_uint4(0, 0, 0, 0);
 #pragma unroll
-	for (int q = 0; q < 2; ++q) {
-		const uint32_t i = tid + q * PBLOCK;
-		if (i <= tc.nc)
-			co[q] = tl.coff[tc.cls_base + i];
-		if (i < tc.nc)
-			nn[q] = cntq[tc.cls_base + i];
-	}
-	if (tid <= tc.nt)
-		to = tl.toff[tc.tx_base + tid];
-	if (tid < tc.nt) {
-		tx = tl.tx[tc.tx_base + tid];
-		ln = tl.len[tc.tx_base + tid];
+	for (int q = 0; q < 2; ++q)
+		nn[q] = tid + q * PBLOCK < tc.nc
+		            ? cntq[tc.cls_base + tid + q * PBLOCK]
+		            : 0.0;
+	if (tid < tc.nt)
 		u = uq[tc.tx_base + tid];
-	}
 #pragma unroll
-	for (int q = 0; q < PK_Z; ++q) {
-		const uint32_t i = tid + q * PBLOCK;
-		if (i < tc.nz) {
-			L.kmem[i] = km[q];
-			L.tpos[i] = tp[q];
-		}
-	}
+	for (int q = 0; q < PK_R; ++q)
+		if (tid + q * PBLOCK < n16)
+			L.rec[tid + q * PBLOCK] = rv[q];
 #pragma unroll
-	for (int q = 0; q < 2; ++q) {
-		const uint32_t i = tid + q * PBLOCK;
-		if (i <= tc.nc)
-			L.coff[i] = (uint16_t)(co[q] - tc.mb);
-		if (i < tc.nc)
-			L.n[i] = nn[q];
-	}
-	if (tid <= tc.nt)
-		L.toff[tid] = (uint16_t)(to - tc.jb);
-	if (tid < tc.nt) {
-		L.tx[tid] = tx;
-		L.len[tid] = ln;
+	for (int q = 0; q < 2; ++q)
+		if (tid + q * PBLOCK < tc.nc)
+			L.n[tid + q * PBLOCK] = nn[q];
+	if (tid < tc.nt)
 		L.u[tid] = u;
-	}
-	for (uint32_t i = tid + 2 * PBLOCK; i <= tc.nc; i += PBLOCK)
-		L.coff[i] = (uint16_t)(tl.coff[tc.cls_base + i] - tc.mb);
+	for (uint32_t i = tid + PK_R * PBLOCK; i < n16; i += PBLOCK)
+		L.rec[i] = src[i];
 	for (uint32_t i = tid + 2 * PBLOCK; i < tc.nc; i += PBLOCK)
 		L.n[i] = cntq[tc.cls_base + i];
-	for (uint32_t i = tid + PBLOCK; i <= tc.nt; i += PBLOCK)
-		L.toff[i] = (uint16_t)(tl.toff[tc.tx_base + i] - tc.jb);
-	for (uint32_t i = tid + PBLOCK; i < tc.nt; i += PBLOCK) {
-		L.tx[i] = tl.tx[tc.tx_base + i];
-		L.len[i] = tl.len[tc.tx_base + i];
+	for (uint32_t i = tid + PBLOCK; i < tc.nt; i += PBLOCK)
 		L.u[i] = uq[tc.tx_base + i];
-	}
-	for (uint32_t i = tid + PK_Z * PBLOCK; i < tc.nz; i += PBLOCK) {
-		L.kmem[i] = tl.kmem[tc.mb + i];
-		L.tpos[i] = tl.tpos[tc.jb + i];
-	}
 }
 
 // One map lin -> lout of the tile (src/EM.c:275-291) in three passes;
@@ -163,17 +153,17 @@ __device__ __forceinline__ void pmap(const PLds &L, const PTile &tc,
 		const uint32_t k1 = L.coff[c + 1];
 		double acc = 0.0;
 		for (; k + 4 <= k1; k += 4) {
-			const double x0 = lin[L.kmem[k] & 0xFFu];
-			const double x1 = lin[L.kmem[k + 1] & 0xFFu];
-			const double x2 = lin[L.kmem[k + 2] & 0xFFu];
-			const double x3 = lin[L.kmem[k + 3] & 0xFFu];
+			const double x0 = lin[L.kmem[k] & 0x3Fu];
+			const double x1 = lin[L.kmem[k + 1] & 0x3Fu];
+			const double x2 = lin[L.kmem[k + 2] & 0x3Fu];
+			const double x3 = lin[L.kmem[k + 3] & 0x3Fu];
 			acc += x0;
 			acc += x1;
 			acc += x2;
 			acc += x3;
 		}
 		for (; k < k1; ++k)
-			acc += lin[L.kmem[k] & 0xFFu];
+			acc += lin[L.kmem[k] & 0x3Fu];
 		L.D[c] = acc;
 	}
 	__syncthreads();
@@ -182,7 +172,7 @@ __device__ __forceinline__ void pmap(const PLds &L, const PTile &tc,
 	// reproduces (sums start at +0 and every term is >= 0)
 	for (uint32_t j = tid; j < tc.nz; j += PBLOCK) {
 		const uint32_t m = L.kmem[L.tpos[j]];
-		const uint32_t c = m >> 8, t = m & 0xFFu;
+		const uint32_t c = m >> 6, t = m & 0x3Fu;
 		const double d = L.D[c];
 		const double x = lin[t] * L.n[c] / (d == 0.0 ? 1.0 : d);
 		L.X[j] = d == 0.0 ? 0.0 : x;
@@ -224,7 +214,8 @@ __global__ __launch_bounds__(PBLOCK) void k_ptile_map12(
 	if (ctl->done)
 		return;
 	const PTile tc = ptile(tl, blockIdx.x);
-	const PLds L = pcarve(smem, tl);
+	PLds L = pcarve(smem, tl);
+	pfields(L, tc);
 	pstage(tl, tc, L, uq, cntq);
 	for (uint32_t i = threadIdx.x; i < tc.nt; i += PBLOCK)
 		L.l0[i] = A[tc.tx_base + i];
@@ -262,7 +253,8 @@ __global__ __launch_bounds__(PBLOCK) void k_ptile_ex3(
 	                         : ctl->alpha;
 	const double smin = ctl->stop_min, srel = ctl->stop_rel;
 	const PTile tc = ptile(tl, blockIdx.x);
-	const PLds L = pcarve(smem, tl);
+	PLds L = pcarve(smem, tl);
+	pfields(L, tc);
 	pstage(tl, tc, L, uq, cntq);
 	for (uint32_t i = threadIdx.x; i < tc.nt; i += PBLOCK) {
 		const size_t g = tc.tx_base + i;
@@ -333,8 +325,7 @@ extern "C" {
 size_t hemk_ptile_lds(const hemk_tiles *t)
 {
 	const size_t mt = t->max_tx, mc = t->max_cls, mz = t->max_nnz;
-	return 8 * (3 * mt + 2 * mc + mz) + 4 * (3 * mt + mz) +
-	       2 * ((mc + 1) + (mt + 1) + mz);
+	return (size_t)t->max_rec + 8 * (3 * mt + 2 * mc + mz) + 4 * mt;
 }
 
 int hemk_ptile_map12(const hemk_tiles *t, const double *A, double *B,

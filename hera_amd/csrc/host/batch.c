@@ -31,14 +31,16 @@ int hem_tiles_upload(hem_problem *p, const hem_tiles_host *h)
 {
 	void *s = p->stream;
 	const size_t nz = h->nnz ? h->nnz : 1;
-	size_t sz[9] = {
+	size_t sz[HEM_TILE_ARRAYS] = {
 	    (size_t)h->n_tiles * HEM_TILE_DESC * 4, (size_t)h->n_tx * 4,
 	    (size_t)h->n_tx * 4, ((size_t)h->n_cls + 1) * 4, nz * 4,
 	    ((size_t)h->n_tx + 1) * 4, nz * 2,
-	    (size_t)(h->n_cls ? h->n_cls : 1) * 4, (size_t)h->n_tx * 4};
-	const void *src[9] = {h->desc, h->tx,   h->len,      h->coff, h->kmem,
-	                      h->toff, h->tpos, h->cls_orig, h->txq};
-	for (int i = 0; i < 9; ++i) {
+	    (size_t)(h->n_cls ? h->n_cls : 1) * 4, (size_t)h->n_tx * 4,
+	    (size_t)(h->rec_units ? h->rec_units : 1) * 16};
+	const void *src[HEM_TILE_ARRAYS] = {
+	    h->desc, h->tx,   h->len,      h->coff, h->kmem,
+	    h->toff, h->tpos, h->cls_orig, h->txq,  h->rec};
+	for (int i = 0; i < HEM_TILE_ARRAYS; ++i) {
 		int e = hemd_malloc((void **)&p->d_tile_mem[i], sz[i]);
 		if (e)
 			return hem_fail(HEM_ERR_NOMEM, "tile upload: %s",
@@ -63,6 +65,9 @@ int hem_tiles_upload(hem_problem *p, const hem_tiles_host *h)
 	p->dt.tpos = (const uint16_t *)p->d_tile_mem[6];
 	p->d_cls_orig = p->d_tile_mem[7];
 	p->dt.txq = p->d_tile_mem[8];
+	p->dt.rec = p->d_tile_mem[9];
+	p->dt.max_rec = h->max_rec;
+	p->tile_rec_bytes = h->rec_units * 16;
 	return HEM_OK;
 }
 
@@ -141,7 +146,7 @@ void hem_batch_free(hem_problem *p)
 	hemd_free(w->snx);
 	hemd_host_free(w->h_active);
 	memset(w, 0, sizeof(*w));
-	for (int i = 0; i < 9; ++i) {
+	for (int i = 0; i < HEM_TILE_ARRAYS; ++i) {
 		hemd_free(p->d_tile_mem[i]);
 		p->d_tile_mem[i] = NULL;
 	}
@@ -159,7 +164,7 @@ static int ws_ensure(hem_problem *p, uint32_t R)
 		return HEM_ERR_NOMEM;
 	{
 		/* keep the tile arrays; drop the per-R buffers */
-		void *keep[9];
+		void *keep[HEM_TILE_ARRAYS];
 		memcpy(keep, p->d_tile_mem, sizeof(keep));
 		memset(p->d_tile_mem, 0, sizeof(keep));
 		hem_batch_free(p);

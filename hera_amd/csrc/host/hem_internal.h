@@ -13,6 +13,8 @@
 #define HEM_ITERS_PER_GRAPH 4
 /* Distinct option sets whose graphs a handle keeps. */
 #define HEM_GRAPH_CACHE 8
+/* device arrays of a tiling (hem_tiles_upload) */
+#define HEM_TILE_ARRAYS 10
 
 /* Workspace of a batched solve, kept on the handle and grown on demand. */
 typedef struct {
@@ -103,7 +105,8 @@ struct hem_problem {
 	hemk_tiles dt;
 	uint32_t *h_cls_orig;      /* tile position -> class id (host)      */
 	uint32_t *d_cls_orig;      /* same, device                          */
-	uint32_t *d_tile_mem[9];   /* device arrays behind dt               */
+	uint32_t *d_tile_mem[HEM_TILE_ARRAYS]; /* device arrays behind dt   */
+	uint64_t tile_rec_bytes;   /* all tile records (algorithmic bytes)  */
 	int counts_integral;       /* class counts are integers < 2^32      */
 	double *h_cls_count;       /* host copy of the class counts         */
 	hem_batch_ws bw;
@@ -149,6 +152,9 @@ typedef struct {
 	uint32_t *kmem;
 	uint16_t *tpos;
 	uint32_t *tsamp;   /* tile -> sample (ns > 1 only)               */
+	uint32_t *rec;     /* per-tile records, 16-byte units (desc[7])  */
+	uint64_t rec_units;
+	uint32_t max_rec;  /* bytes of the largest record                */
 } hem_tiles_host;
 
 /* tx0[0..ns]: first transcript of each sample (tx0[ns] = T), or NULL with
@@ -158,6 +164,14 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
                     const uint32_t *tx_off, const uint32_t *tx0,
                     uint32_t ns, hem_tiles_host *h);
 void hem_tiles_free_host(hem_tiles_host *h);
+/* Bytes of a tile's record (see tiles.c), a multiple of 16. */
+static inline uint64_t hem_tile_rec_bytes(uint32_t nt, uint32_t nc,
+                                          uint32_t nz)
+{
+	uint64_t u16s = (uint64_t)(nc + 1) + (nt + 1) + 2 * (uint64_t)nz;
+	u16s += u16s & 1;
+	return (2 * u16s + 8 * (uint64_t)nt + 15) / 16 * 16;
+}
 
 /* kernel kinds for per-kernel timing (hem_kernel_profile) */
 enum { HEM_PK_DENOM = 0, HEM_PK_GATHER, HEM_PK_NORM1, HEM_PK_NORM2,

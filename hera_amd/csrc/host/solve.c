@@ -503,12 +503,12 @@ static void kernel_bytes(const hem_problem *p, int f32, double *b)
 	b[HEM_PK_EXTRAP] = 24 * T + 8 * T + (f32 ? 4 * T : 0);
 	b[HEM_PK_ITER_END] = 8;
 	{
-		/* tile kernels: each tile's structure once (descriptor, class
-		 * and transcript offsets, members, occurrence positions, ids,
-		 * lengths, unique and class counts), then the state */
+		/* tile kernels: each tile's descriptor and record (offsets,
+		 * members, occurrence positions, lengths, ids) once, its
+		 * unique and class counts, then the state */
 		const double NT = p->dt.n_tiles, TC = p->dt.n_cls;
-		const double S = 32 * NT + 4 * (TC + NT) + 4 * (T + NT) +
-		                 6 * Z + 12 * T + 8 * TC;
+		const double S = 32 * NT + (double)p->tile_rec_bytes + 4 * T +
+		                 8 * TC;
 		b[HEM_PK_TMAP12] = S + 8 * T + 16 * T + 16 * T;
 		b[HEM_PK_TEX3] = S + 24 * T + 8 * T;
 	}

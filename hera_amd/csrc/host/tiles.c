@@ -45,6 +45,57 @@ static uint32_t uf_find(uint32_t *par, uint32_t x)
 	return x;
 }
 
+/* One contiguous record per tile for the single-problem tile kernels
+ * (ptile_kernels.hip), 16-byte aligned, its offset in 16-byte units in
+ * desc[7]: u16 class offsets (n_cls + 1) and class-list offsets
+ * (n_tx + 1), both relative to the tile; u16 members
+ * (local class << 6) | local transcript; u16 class-list positions; then,
+ * 4-byte aligned, u32 lengths and u32 transcript ids (n_tx each). */
+static int tiles_records(hem_tiles_host *h)
+{
+	uint64_t units = 0;
+	h->max_rec = 0;
+	for (uint32_t i = 0; i < h->n_tiles; ++i) {
+		const uint32_t *d = h->desc + (size_t)HEM_TILE_DESC * i;
+		const uint64_t b = hem_tile_rec_bytes(d[1], d[3], d[5]);
+		if (b > h->max_rec)
+			h->max_rec = (uint32_t)b;
+		units += b / 16;
+	}
+	if (units > 0xFFFFFFFFu)
+		return hem_fail(HEM_ERR_RANGE, "tile records exceed 64 GiB");
+	h->rec = calloc(units ? units * 4 : 4, 4);
+	if (!h->rec)
+		return hem_fail(HEM_ERR_NOMEM, "tiling: host allocation");
+	h->rec_units = units;
+	units = 0;
+	for (uint32_t i = 0; i < h->n_tiles; ++i) {
+		uint32_t *d = h->desc + (size_t)HEM_TILE_DESC * i;
+		const uint32_t nt = d[1], nc = d[3], nz = d[5];
+		uint16_t *r16 = (uint16_t *)(h->rec + units * 4);
+		uint32_t k = 0;
+		for (uint32_t c = 0; c <= nc; ++c)
+			r16[k++] = (uint16_t)(h->coff[d[2] + c] - d[4]);
+		for (uint32_t t = 0; t <= nt; ++t)
+			r16[k++] = (uint16_t)(h->toff[d[0] + t] - d[6]);
+		for (uint32_t z = 0; z < nz; ++z) {
+			const uint32_t m = h->kmem[d[4] + z];
+			r16[k++] = (uint16_t)(((m >> 8) << 6) | (m & 0x3Fu));
+		}
+		for (uint32_t z = 0; z < nz; ++z)
+			r16[k++] = h->tpos[d[6] + z];
+		k += k & 1;
+		uint32_t *r32 = (uint32_t *)r16 + k / 2;
+		for (uint32_t t = 0; t < nt; ++t) {
+			r32[t] = h->len[d[0] + t];
+			r32[nt + t] = h->tx[d[0] + t];
+		}
+		d[7] = (uint32_t)units;
+		units += hem_tile_rec_bytes(nt, nc, nz) / 16;
+	}
+	return HEM_OK;
+}
+
 void hem_tiles_free_host(hem_tiles_host *h)
 {
 	if (!h)
@@ -57,6 +108,7 @@ void hem_tiles_free_host(hem_tiles_host *h)
 	free(h->txq);
 	free(h->toff);
 	free(h->tpos);
+	free(h->rec);
 	free(h->cls_orig);
 	free(h->tsamp);
 	memset(h, 0, sizeof(*h));
@@ -325,6 +377,8 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 		if (z > h->max_nnz)
 			h->max_nnz = z;
 	}
+	if ((rc = tiles_records(h)))
+		goto out;
 out:
 	if (rc)
 		hem_tiles_free_host(h);

@@ -126,6 +126,7 @@ def main():
         print(json.dumps(out), flush=True)
         return
     results = {}
+    head_result = None
     for name, mk in modes.items():
         steps = args.steps if name == head_name else max(args.steps // 2, 1)
         for _ in range(args.warmup):
@@ -137,6 +138,8 @@ def main():
         for _ in range(steps):
             r = p.solve(u, **mk, **sk)
             it += r["stats"]["iters"]
+            if name == head_name:
+                head_result = r
             loop_ms += r["stats"]["ms_solve"]
         dt = time.perf_counter() - t
         # value: whole calls (unique counts up, tpm/est/lambda down, setup
@@ -213,6 +216,8 @@ def main():
     out["copy_bandwidth_GBps"] = H.copy_bandwidth(1 << 30, 10)
     log("device copy (1 GiB, read+write): %.0f GB/s"
         % out["copy_bandwidth_GBps"])
+
+    out["writer"] = bench_writer(H, np, cls, u, head_result)
 
     if args.grad_topk > 0:
         out["differentiation"] = bench_grad(args, H, np, p, cls, u, sk)
@@ -344,6 +349,29 @@ def bench_grad(args, H, np, p, cls, u, sk):
            res["hvp_wall_ms"], res["cpu_grad_ms"], args.grad_topk,
            args.cg_steps, st["ms_cg"], st["resid_max"]))
     return res
+
+
+def bench_writer(H, np, cls, u, res):
+    """hera's abundance.tsv and abundance.gene.tsv for this solve's result
+    (host formatting and file writes; names and genes synthetic)."""
+    import tempfile
+    T = cls.n_tx
+    names = ["SYNT%011d.1|tx%d" % (i, i) for i in range(T)]
+    gene = (np.arange(T) // 4).astype(np.uint32)
+    gnames = ["SYNG%011d.1|g%d" % (g, g) for g in range(int(gene[-1]) + 1)]
+    eff = cls.tx_len.astype(np.float64)
+    with tempfile.TemporaryDirectory() as d:
+        t = time.perf_counter()
+        H.write_result(d, "bench", names, u, cls.tx_len, eff, res["est"],
+                       res["tpm"], gene=gene, gene_names=gnames)
+        dt = time.perf_counter() - t
+        size = sum(os.path.getsize(os.path.join(d, f))
+                   for f in os.listdir(d))
+    log("writer: %d rows, %.1f ms, %.1f MB" % (T, dt * 1e3, size / 1e6))
+    return {"rows": T, "ms": dt * 1e3, "bytes": size,
+            "note": "hem_write_result: abundance.tsv + abundance.gene.tsv "
+                    "(4 transcripts per gene), including packing the name "
+                    "blocks in Python"}
 
 
 def bench_ragged(args, H, np):

@@ -5,6 +5,7 @@ drop-in ``estimate_count`` of include/hera_dropin.h).  This package is a thin
 ctypes view of it for tests and benchmarks.  Nothing here computes on the
 CPU: every call goes to the HIP path and raises if the library is missing.
 """
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -129,6 +130,70 @@ def exact_sums(rows, mode=SUM_PLAIN):
     check(_lib.load().hem_exact_sums(f64p(x), x.shape[1], x.shape[0], mode,
                                      f64p(out), rr))
     return out, rr.value
+
+
+def name_blocks(names, width=None):
+    """Names as hera stores them (REF_INF->ref_name, GENE_MAP->gene_name):
+    fixed-width blocks, each name NUL-terminated inside its block.
+    Returns (bytes, width)."""
+    enc = [n.encode() if isinstance(n, str) else bytes(n) for n in names]
+    w = width or (max((len(x) for x in enc), default=0) + 1)
+    buf = bytearray(w * len(enc))
+    for i, x in enumerate(enc):
+        if len(x) >= w or b"\0" in x:
+            raise ValueError("name %r does not fit a %d-byte block" % (x, w))
+        buf[i * w:i * w + len(x)] = x
+    return bytes(buf), w
+
+
+def write_abundance(path, names, unique, length, eff_len, est, tpm,
+                    name_len=None):
+    """hera's abundance.tsv (hem_write_abundance)."""
+    blk, w = name_blocks(names, name_len)
+    n = len(names)
+    u, ln = as_u32(unique), as_u32(length)
+    e, es, tp = as_f64(eff_len), as_f64(est), as_f64(tpm)
+    for a in (u, ln, e, es, tp):
+        if a.shape != (n,):
+            raise ValueError("arrays must have one entry per name")
+    check(_lib.load().hem_write_abundance(
+        os.fsencode(path), n, blk, w, u32p(u), u32p(ln), f64p(e),
+        f64p(es), f64p(tp)))
+
+
+def write_gene_abundance(path, names, gene, gene_names, est, tpm,
+                         name_len=None, gene_len=None):
+    """hera's abundance.gene.tsv (hem_write_gene_abundance): names and gene
+    names are read from their 17th byte, as the reference does."""
+    blk, w = name_blocks(names, name_len)
+    gblk, gw = name_blocks(gene_names, gene_len)
+    n = len(names)
+    g, es, tp = as_u32(gene), as_f64(est), as_f64(tpm)
+    for a in (g, es, tp):
+        if a.shape != (n,):
+            raise ValueError("arrays must have one entry per name")
+    check(_lib.load().hem_write_gene_abundance(
+        os.fsencode(path), n, blk, w, u32p(g), gblk, gw, len(gene_names),
+        f64p(es), f64p(tp)))
+
+
+def write_result(out_dir, prefix, names, unique, length, eff_len, est, tpm,
+                 gene=None, gene_names=None, name_len=None, gene_len=None):
+    """Both tables under hera's file names (hem_write_result)."""
+    blk, w = name_blocks(names, name_len)
+    n = len(names)
+    u, ln = as_u32(unique), as_u32(length)
+    e, es, tp = as_f64(eff_len), as_f64(est), as_f64(tpm)
+    g = gblk = None
+    gw = ng = 0
+    if gene is not None:
+        g = as_u32(gene)
+        gblk, gw = name_blocks(gene_names, gene_len)
+        ng = len(gene_names)
+    check(_lib.load().hem_write_result(
+        os.fsencode(out_dir), prefix.encode(), n, blk, w, u32p(u),
+        u32p(ln), f64p(e), u32p(g) if g is not None else None, gblk, gw,
+        ng, f64p(es), f64p(tp)))
 
 
 def opts(**kw):

@@ -18,6 +18,7 @@ HEM_ERR_NOMEM = 2
 HEM_ERR_DEVICE = 3
 HEM_ERR_NODEVICE = 4
 HEM_ERR_RANGE = 5
+HEM_ERR_IO = 6
 
 _u32p = C.POINTER(C.c_uint32)
 _f64p = C.POINTER(C.c_double)
@@ -112,6 +113,17 @@ SIGNATURES = {
     "hem_solve": (C.c_int, [C.c_void_p, _u32p, C.POINTER(HemOpts), _f64p,
                             _f64p, _f64p, C.POINTER(HemStats)]),
     "hem_loglik": (C.c_int, [C.c_void_p, _u32p, _f64p, _f64p]),
+    "hem_write_abundance": (C.c_int, [C.c_char_p, C.c_uint32, C.c_char_p,
+                                      C.c_uint32, _u32p, _u32p, _f64p,
+                                      _f64p, _f64p]),
+    "hem_write_gene_abundance": (C.c_int, [C.c_char_p, C.c_uint32,
+                                           C.c_char_p, C.c_uint32, _u32p,
+                                           C.c_char_p, C.c_uint32,
+                                           C.c_uint32, _f64p, _f64p]),
+    "hem_write_result": (C.c_int, [C.c_char_p, C.c_char_p, C.c_uint32,
+                                   C.c_char_p, C.c_uint32, _u32p, _u32p,
+                                   _f64p, _u32p, C.c_char_p, C.c_uint32,
+                                   C.c_uint32, _f64p, _f64p]),
     "hem_exact_sums": (C.c_int, [_f64p, C.c_uint32, C.c_uint32, C.c_int,
                                  _f64p, _u32p]),
     "hem_profile_solve": (C.c_int, [C.c_void_p, _u32p, C.POINTER(HemOpts),

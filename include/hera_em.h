@@ -29,6 +29,7 @@ extern "C" {
 #define HEM_ERR_DEVICE     3  /* HIP runtime error                          */
 #define HEM_ERR_NODEVICE   4  /* no usable gfx950 device                    */
 #define HEM_ERR_RANGE      5  /* size beyond a documented limit             */
+#define HEM_ERR_IO         6  /* a file could not be opened or written      */
 
 /* Last error message of the calling thread (static storage, never NULL). */
 const char *hem_last_error(void);
@@ -263,6 +264,43 @@ int hem_copy_bandwidth(size_t bytes, int reps, double *gb_per_s);
 #define HEM_SUM_SQUARES 2
 int hem_exact_sums(const double *x, uint32_t n, uint32_t n_rows, int mode,
                    double *out, uint32_t *reruns);
+
+/* ---- abundance tables ------------------------------------------------------
+ * hera's text outputs of write_result (src/EM.c:152-205), from host arrays
+ * (no device work).  names: n fixed-width blocks of name_len bytes (a
+ * block's name ends at its first NUL), as REF_INF->ref_name; est and tpm
+ * as EM_val.overlap and .grama.  Rows are "%s\t%u\t%u\t%f\t%f\t%f\n" of
+ * name, unique count, length, effective length, est_counts, tpm after the
+ * header "#target_id\tunique_map\tlength\teff_length\test_counts\ttpm".
+ * Replaces the abundance.tsv part of write_result (src/EM.c:176-185). */
+int hem_write_abundance(const char *path, uint32_t n, const char *names,
+                        uint32_t name_len, const uint32_t *unique,
+                        const uint32_t *len, const double *eff_len,
+                        const double *est, const double *tpm);
+/* One row per run of consecutive transcripts with the same gene id
+ * (gene[i] < n_gene, as GENE_MAP->gene): est_counts and tpm summed in
+ * float, named by the run's last transcript's name block and the gene's
+ * name block (l_gene bytes each, as GENE_MAP->gene_name), both from their
+ * 17th byte; header "#gene_id\tgene_name\test_counts\ttpm".  Replaces the
+ * abundance.gene.tsv part (src/EM.c:187-205); the reference's read of
+ * gene[n] past the array at the last transcript is not reproduced (the
+ * array's end ends the run).  name_len and l_gene must exceed 16. */
+int hem_write_gene_abundance(const char *path, uint32_t n, const char *names,
+                             uint32_t name_len, const uint32_t *gene,
+                             const char *gene_names, uint32_t l_gene,
+                             uint32_t n_gene, const double *est,
+                             const double *tpm);
+/* Both files under the reference's names: out_dir, '/' when it does not
+ * end in one, prefix, '_' when prefix is not empty, then "abundance.tsv"
+ * and (when gene != NULL) "abundance.gene.tsv" (src/EM.c:161-187).  The
+ * HDF5 file of src/EM.c:208-231 is not written.  Errors: HEM_ERR_INVALID,
+ * HEM_ERR_IO (file), HEM_ERR_NOMEM. */
+int hem_write_result(const char *out_dir, const char *prefix, uint32_t n,
+                     const char *names, uint32_t name_len,
+                     const uint32_t *unique, const uint32_t *len,
+                     const double *eff_len, const uint32_t *gene,
+                     const char *gene_names, uint32_t l_gene,
+                     uint32_t n_gene, const double *est, const double *tpm);
 
 /* ---- adjoint of the log-likelihood ----------------------------------------
  * No counterpart in the reference, which has no derivatives; the objective

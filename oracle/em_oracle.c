@@ -28,6 +28,7 @@
 #include <math.h>
 #include <pthread.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -670,4 +671,54 @@ int oracle_bootstrap_resampled(uint32_t n_tx, uint32_t n_cls,
 	free(thr);
 	free(jobs);
 	return 0;
+}
+
+/* ---- abundance tables: src/EM.c:176-205 restated ------------------------
+ * abundance.tsv: the header of :178, then per transcript the fprintf of
+ * :180-184 with the name block as a C string.  abundance.gene.tsv: the
+ * header of :188, then the loop of :189-204 -- float sums, the row named by
+ * the last transcript of the run (name block + 16) and the gene's name
+ * block + 16 -- except that the run also ends at the last transcript
+ * (the reference compares gene[n], one past its array, there). */
+int oracle_write_abundance(const char *path, uint32_t n, const char *names,
+                           uint32_t name_len, const uint32_t *count,
+                           const uint32_t *len, const double *eff_len,
+                           const double *overlap, const double *grama)
+{
+	FILE *out = fopen(path, "w");
+	uint32_t i;
+	if (!out)
+		return 1;
+	fprintf(out, "#target_id\tunique_map\tlength\teff_length\test_counts\ttpm\n");
+	for (i = 0; i < n; ++i)
+		fprintf(out, "%s\t%u\t%u\t%f\t%f\t%f\n", names + i * name_len,
+		        count[i], len[i], eff_len[i], overlap[i], grama[i]);
+	return fclose(out) != 0;
+}
+
+int oracle_write_gene_abundance(const char *path, uint32_t n,
+                                const char *names, uint32_t name_len,
+                                const uint32_t *gene, const char *gene_name,
+                                uint32_t l_gene, const double *overlap,
+                                const double *grama)
+{
+	FILE *out = fopen(path, "w");
+	uint32_t i, g;
+	float sum[2];
+	if (!out)
+		return 1;
+	fprintf(out, "#gene_id\tgene_name\test_counts\ttpm\n");
+	for (i = 0; i < n; ++i) {
+		g = gene[i];
+		sum[0] = overlap[i];
+		sum[1] = grama[i];
+		while (i + 1 < n && g == gene[i + 1]) {
+			++i;
+			sum[0] += overlap[i];
+			sum[1] += grama[i];
+		}
+		fprintf(out, "%s\t%s\t%f\t%f\n", names + i * name_len + 16,
+		        gene_name + g * l_gene + 16, sum[0], sum[1]);
+	}
+	return fclose(out) != 0;
 }

@@ -70,4 +70,14 @@ int oracle_bootstrap_resampled(uint32_t n_tx, uint32_t n_cls,
                                uint32_t n_rep, const uint64_t *seeds,
                                const oracle_opts *opt, uint32_t n_thread,
                                double *est, uint32_t *k_out);
+int oracle_write_abundance(const char *path, uint32_t n, const char *names,
+                           uint32_t name_len, const uint32_t *count,
+                           const uint32_t *len, const double *eff_len,
+                           const double *overlap, const double *grama);
+int oracle_write_gene_abundance(const char *path, uint32_t n,
+                                const char *names, uint32_t name_len,
+                                const uint32_t *gene, const char *gene_name,
+                                uint32_t l_gene, const double *overlap,
+                                const double *grama);
+
 #endif

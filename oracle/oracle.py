@@ -44,6 +44,14 @@ def load():
             C.c_uint32, C.c_uint32, _u32p, _u32p, _f64p, _u32p, _u32p,
             C.POINTER(OracleOpts), _f64p, _f64p, _f64p, _f64p,
             C.POINTER(OracleTrace)]
+        lib.oracle_write_abundance.restype = C.c_int
+        lib.oracle_write_abundance.argtypes = [
+            C.c_char_p, C.c_uint32, C.c_char_p, C.c_uint32, _u32p, _u32p,
+            _f64p, _f64p, _f64p]
+        lib.oracle_write_gene_abundance.restype = C.c_int
+        lib.oracle_write_gene_abundance.argtypes = [
+            C.c_char_p, C.c_uint32, C.c_char_p, C.c_uint32, _u32p,
+            C.c_char_p, C.c_uint32, _f64p, _f64p]
         lib.oracle_em_map.restype = C.c_int
         lib.oracle_em_map.argtypes = [C.c_uint32, C.c_uint32, _u32p, _u32p,
                                       _f64p, _u32p, _u32p, _f64p, _f64p,
@@ -235,3 +243,29 @@ def bootstrap_resampled(cls, unique, seeds, n_thread, **kw):
         sd.ctypes.data_as(C.POINTER(C.c_uint64)), opts(**kw), n_thread,
         _p64(est), _p32(ks)) == 0
     return est, ks
+
+
+def write_abundance(path, blocks, width, unique, length, eff_len, est, tpm):
+    """src/EM.c:176-185 restated (names as fixed-width NUL-padded blocks)."""
+    u = np.ascontiguousarray(unique, np.uint32)
+    ln = np.ascontiguousarray(length, np.uint32)
+    e, es, tp = (np.ascontiguousarray(a, np.float64)
+                 for a in (eff_len, est, tpm))
+    if load().oracle_write_abundance(
+            os.fsencode(path), len(u), blocks, width,
+            u.ctypes.data_as(_u32p), ln.ctypes.data_as(_u32p),
+            e.ctypes.data_as(_f64p), es.ctypes.data_as(_f64p),
+            tp.ctypes.data_as(_f64p)):
+        raise OSError("oracle write failed: %s" % path)
+
+
+def write_gene_abundance(path, blocks, width, gene, gblocks, gwidth, est,
+                         tpm):
+    """src/EM.c:187-205 restated."""
+    g = np.ascontiguousarray(gene, np.uint32)
+    es, tp = (np.ascontiguousarray(a, np.float64) for a in (est, tpm))
+    if load().oracle_write_gene_abundance(
+            os.fsencode(path), len(g), blocks, width,
+            g.ctypes.data_as(_u32p), gblocks, gwidth,
+            es.ctypes.data_as(_f64p), tp.ctypes.data_as(_f64p)):
+        raise OSError("oracle write failed: %s" % path)

@@ -67,7 +67,8 @@ def main():
     ap.add_argument("--cg-steps", type=int, default=50)
     ap.add_argument("--bootstrap-reps", type=int, default=100,
                     help="bootstrap replicates of the batched run (0: skip)")
-    ap.add_argument("--boot-modes", default="fp64-exact,fp64-tree",
+    ap.add_argument("--boot-modes",
+                    default="fp64-exact,fp64-tree,hera-sampler",
                     help="bootstrap modes to run")
     ap.add_argument("--seq-sample", type=int, default=3,
                     help="replicates solved one by one for the unbatched "
@@ -494,6 +495,37 @@ def bench_bootstrap(args, H, np, p, cls, u, sk):
         log("bootstrap %-10s R=%d %.3fs (%.1f reps/s) resample %.1f ms "
             "iters %d..%d" % (name, R, dt, R / dt, st["ms_resample"],
                               st["iters_min"], st["iters_max"]))
+    if "hera-sampler" in args.boot_modes.split(","):
+        # hera's own sampler (unique reads redrawn with rand_r, class
+        # counts kept), the replicates of a one-thread hera run (seed 0)
+        p.bootstrap_hera(u, 2, 0, **sk)
+        t = time.perf_counter()
+        b = p.bootstrap_hera(u, R, 0, **sk)
+        dt = time.perf_counter() - t
+        st = b["stats"]
+        res["fp64-exact-hera-sampler"] = dict(
+            reps_per_s=R / dt, seconds=dt, ms_resample=st["ms_resample"],
+            ms_solve=st["ms_solve"], iters_min=st["iters_min"],
+            iters_max=st["iters_max"], iters_total=int(b["iters"].sum()),
+            unique_reads=int(u.sum()), seed=0)
+        log("bootstrap hera-sampler R=%d %.3fs (%.1f reps/s) resample "
+            "%.1f ms iters %d..%d" % (R, dt, R / dt, st["ms_resample"],
+                                      st["iters_min"], st["iters_max"]))
+        if args.cpu_boot_threads > 0:
+            from oracle import oracle as O
+            t = time.perf_counter()
+            cu, _ = O.hera_sample(u, 1, 0)
+            dtc = time.perf_counter() - t
+            ok = bool(np.array_equal(cu[0],
+                                     p.bootstrap_hera_counts(u, 1, 0)[0]))
+            res["fp64-exact-hera-sampler"]["cpu_sampler_ms_per_rep"] = \
+                dtc * 1e3
+            res["fp64-exact-hera-sampler"]["gpu_sampler_ms_per_rep"] = \
+                st["ms_resample"] / R
+            res["fp64-exact-hera-sampler"]["counts_bitwise_vs_oracle"] = ok
+            log("hera sampler: CPU %.1f ms per replicate (1 thread), GPU "
+                "%.2f ms; counts bitwise %s" % (dtc * 1e3,
+                                                st["ms_resample"] / R, ok))
     # unbatched: the same resampled counts through the plain solve, one
     # replicate after another (a sample of n_seq, scaled to R; 0 skips it)
     if args.seq_sample <= 0:

@@ -372,6 +372,30 @@ class Problem:
             f64p(co)))
         return uo, co
 
+    def bootstrap_hera(self, unique, n_rep, seed=0, **kw):
+        """Bootstrap with hera's own sampler (unique reads redrawn with
+        glibc rand_r, class counts kept); replicate r continues the rand_r
+        state of replicate r - 1 from seed, as hera does on one thread.
+        Returns dict(est, tpm, iters, stats)."""
+        u = as_u32(unique)
+        o = opts(**kw)
+        est = np.empty((n_rep, self.n_tx))
+        tpm = np.empty((n_rep, self.n_tx))
+        it = np.empty(n_rep, dtype=np.uint32)
+        st = _lib.HemBatchStats()
+        check(_lib.load().hem_bootstrap_hera(
+            self._h, u32p(u), n_rep, seed, o, f64p(est), f64p(tpm),
+            u32p(it), st))
+        return dict(est=est, tpm=tpm, iters=it, stats=self._bstats(st))
+
+    def bootstrap_hera_counts(self, unique, n_rep, seed=0):
+        """The unique counts bootstrap_hera would solve, [n_rep, n_tx]."""
+        u = as_u32(unique)
+        uo = np.empty((n_rep, self.n_tx), dtype=np.uint32)
+        check(_lib.load().hem_bootstrap_hera_counts(
+            self._h, u32p(u), n_rep, seed, u32p(uo)))
+        return uo
+
     def loglik(self, unique, lam):
         u = as_u32(unique)
         lam = as_f64(lam)

@@ -140,6 +140,12 @@ SIGNATURES = {
                                 C.POINTER(HemBatchStats)]),
     "hem_bootstrap_counts": (C.c_int, [C.c_void_p, _u32p, C.c_uint32,
                                        C.POINTER(C.c_uint64), _u32p, _f64p]),
+    "hem_bootstrap_hera": (C.c_int, [C.c_void_p, _u32p, C.c_uint32,
+                                     C.c_uint32, C.POINTER(HemOpts), _f64p,
+                                     _f64p, _u32p,
+                                     C.POINTER(HemBatchStats)]),
+    "hem_bootstrap_hera_counts": (C.c_int, [C.c_void_p, _u32p, C.c_uint32,
+                                            C.c_uint32, _u32p]),
     "hem_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_int,
                                      C.POINTER(C.c_double)]),
     "hem_grad": (C.c_int, [C.c_void_p, _u32p, _f64p, _f64p, C.c_int, _f64p,

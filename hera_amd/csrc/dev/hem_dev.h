@@ -195,6 +195,15 @@ int hemk_resample(const uint32_t *w, uint32_t K, const uint32_t *chunk_start,
                   const uint32_t *chunk_cw, uint32_t B, uint32_t N,
                   const uint64_t *seeds, uint32_t R, uint32_t *m,
                   uint32_t *out, void *s);
+/* hera's sampler (src/EM.c:98-150): rows out[R][T + C] get the unique
+ * counts of N - 1 draws each (cum: running sums of the unique counts, N =
+ * cum[T-1]) and the class counts cls; replicate r's rand_r state starts
+ * 3 (N-1) r LCG steps past seed.  bt: ((N-1) >> sh) + 1 words of
+ * scratch. */
+int hemk_hera_resample(const uint32_t *cum, uint32_t T, uint32_t N,
+                       uint32_t *bt, uint32_t sh, const uint32_t *cls,
+                       uint32_t C, uint32_t seed, uint32_t R, uint32_t *out,
+                       void *s);
 int hemk_fill_u32(uint32_t *p, size_t n, uint32_t v, void *s);
 /* One sample's resampled rows in[R][Tq + Cq] into out[R][K]: unique counts
  * at off_u.., class counts at off_c.. */

@@ -147,6 +147,111 @@ __global__ __launch_bounds__(RBLOCK) void k_gather_rows_u32(
 
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// ---- hera's sampler (src/EM.c:98-150) ----------------------------------
+// The reference lays the unique reads out as N slots (create_distribution,
+// src/EM.c:98-113: slot k belongs to the transcript t with
+// cum[t-1] <= k < cum[t], cum the running sum of the unique counts) and
+// draws, for n = N-1 down to 1, slot rand_r() % n (src/EM.c:134-135):
+// N - 1 draws from glibc's rand_r, which advances its 32-bit state by three
+// steps of the LCG x -> 1103515245 x + 12345 (mod 2^32) per call and
+// returns 11 + 10 + 10 bits taken from bits 16.. of the three states.  The
+// LCG has full period, so the state before draw i is 3i steps past the
+// replicate's first state: every draw is computed on its own.
+#define HERA_A 1103515245u
+#define HERA_C 12345u
+
+// (A, C) with x_k = A x_0 + C after k steps.
+__device__ __forceinline__ void lcg_jump(uint32_t k, uint32_t &A, uint32_t &C)
+{
+	uint32_t a = HERA_A, c = HERA_C;   // the 2^b-step map
+	A = 1u;
+	C = 0u;
+	while (k) {
+		if (k & 1u) {
+			C = a * C + c;
+			A = a * A;
+		}
+		c = a * c + c;
+		a = a * a;
+		k >>= 1;
+	}
+}
+
+__device__ __forceinline__ uint32_t hera_rand(uint32_t &x)
+{
+	uint32_t r;
+	x = HERA_A * x + HERA_C;
+	r = (x >> 16) % 2048u;
+	x = HERA_A * x + HERA_C;
+	r = (r << 10) ^ ((x >> 16) % 1024u);
+	x = HERA_A * x + HERA_C;
+	r = (r << 10) ^ ((x >> 16) % 1024u);
+	return r;
+}
+
+// bt[b] = the transcript of slot b << sh (first t with cum[t] > b << sh).
+__global__ __launch_bounds__(RBLOCK) void k_hera_buckets(
+    const uint32_t *__restrict__ cum, uint32_t T, uint32_t sh, uint32_t nb,
+    uint32_t *__restrict__ bt)
+{
+	for (uint32_t b = blockIdx.x * RBLOCK + threadIdx.x; b < nb;
+	     b += gridDim.x * RBLOCK) {
+		const uint64_t k = (uint64_t)b << sh;
+		uint32_t lo = 0, hi = T;
+		while (lo < hi) {
+			const uint32_t mid = lo + (hi - lo) / 2;
+			if (cum[mid] > k)
+				hi = mid;
+			else
+				lo = mid + 1;
+		}
+		bt[b] = lo;
+	}
+}
+
+// Rows out[r][K]: zero unique counts, the table's class counts.
+__global__ __launch_bounds__(RBLOCK) void k_hera_rows(
+    const uint32_t *__restrict__ cls, uint32_t T, uint32_t C, uint32_t R,
+    uint32_t *__restrict__ out)
+{
+	const uint32_t K = T + C;
+	const uint64_t n = (uint64_t)R * K;
+	for (uint64_t i = (uint64_t)blockIdx.x * RBLOCK + threadIdx.x; i < n;
+	     i += (uint64_t)gridDim.x * RBLOCK) {
+		const uint32_t k = (uint32_t)(i % K);
+		out[i] = k < T ? 0u : cls[k - T];
+	}
+}
+
+// Draws: thread handles HERA_DPT consecutive draws of replicate
+// blockIdx.y; replicate r's first state is 3 (N-1) r steps past seed.
+#define HERA_DPT 32
+__global__ __launch_bounds__(RBLOCK) void k_hera_draw(
+    const uint32_t *__restrict__ cum, uint32_t T, uint32_t N,
+    const uint32_t *__restrict__ bt, uint32_t sh, uint32_t seed,
+    uint32_t *__restrict__ out, uint32_t K)
+{
+	const uint32_t M = N - 1;   // draws per replicate (N >= 2)
+	const uint32_t r = blockIdx.y;
+	const uint32_t i0 =
+	    (blockIdx.x * RBLOCK + threadIdx.x) * (uint32_t)HERA_DPT;
+	if (i0 >= M)
+		return;
+	uint32_t A, C;
+	// steps mod 2^32 (the period): 3 (M r + i0)
+	lcg_jump((uint32_t)(3ull * ((uint64_t)M * r + i0)), A, C);
+	uint32_t x = A * seed + C;
+	uint32_t *row = out + (size_t)r * K;
+	const uint32_t i1 = min(M, i0 + (uint32_t)HERA_DPT);
+	for (uint32_t i = i0; i < i1; ++i) {
+		const uint32_t slot = hera_rand(x) % (M - i);
+		uint32_t t = bt[slot >> sh];
+		while (cum[t] <= slot)
+			++t;
+		atomicAdd(row + t, 1u);
+	}
+}
+
 } // namespace
 
 extern "C" {
@@ -179,6 +284,29 @@ int hemk_scatter_counts(const uint32_t *in, uint32_t R, uint32_t Tq,
 		grid = 16384;
 	k_scatter_counts<<<grid, RBLOCK, 0, (hipStream_t)s>>>(in, R, Tq, Cq, out,
 	                                                      K, off_u, off_c);
+	return (int)hipGetLastError();
+}
+
+int hemk_hera_resample(const uint32_t *cum, uint32_t T, uint32_t N,
+                       uint32_t *bt, uint32_t sh, const uint32_t *cls,
+                       uint32_t C, uint32_t seed, uint32_t R, uint32_t *out,
+                       void *s)
+{
+	hipStream_t st = (hipStream_t)s;
+	const uint32_t K = T + C;
+	if (T == 0 || R == 0 || R > 65535 || sh > 31)
+		return (int)hipErrorInvalidValue;
+	uint32_t grid = cdiv((uint64_t)R * K, RBLOCK);
+	k_hera_rows<<<grid > 16384 ? 16384 : grid, RBLOCK, 0, st>>>(cls, T, C,
+	                                                            R, out);
+	if (N >= 2) {
+		const uint32_t nb = (uint32_t)(((uint64_t)(N - 1) >> sh) + 1);
+		grid = cdiv(nb, RBLOCK);
+		k_hera_buckets<<<grid > 8192 ? 8192 : grid, RBLOCK, 0, st>>>(
+		    cum, T, sh, nb, bt);
+		k_hera_draw<<<dim3(cdiv(cdiv(N - 1, HERA_DPT), RBLOCK), R),
+		              RBLOCK, 0, st>>>(cum, T, N, bt, sh, seed, out, K);
+	}
 	return (int)hipGetLastError();
 }
 

@@ -807,6 +807,134 @@ out:
 	return rc;
 }
 
+/* hera's sampler into w->cat_rm (one-sample handles): unique counts
+ * redrawn, class counts the table's (src/EM.c:98-150). */
+static int resample_hera(hem_problem *p, const uint32_t *unique,
+                         uint32_t n_rep, uint32_t seed, float *ms)
+{
+	const uint32_t T = p->n_tx, C = p->n_cls;
+	uint32_t *h = NULL, *d = NULL, sh = 0;
+	uint64_t N = 0;
+	int rc = HEM_OK, e = 0;
+	if (p->ns > 1)
+		return hem_fail(HEM_ERR_INVALID, "hera's sampler is for "
+		                                 "one-sample handles");
+	if (!p->counts_integral)
+		return hem_fail(HEM_ERR_INVALID,
+		                "bootstrap needs integer class counts below 2^32");
+	if (n_rep > 65535)
+		return hem_fail(HEM_ERR_RANGE, "at most 65535 replicates");
+	h = malloc(((size_t)T + C) * 4);
+	if (!h)
+		return hem_fail(HEM_ERR_NOMEM, "host allocation");
+	for (uint32_t t = 0; t < T; ++t) {
+		N += unique[t];
+		h[t] = (uint32_t)N;
+		if (N > 0xFFFFFFFFull) {
+			free(h);
+			return hem_fail(HEM_ERR_RANGE, "%llu unique reads: hera's "
+			                "sampler needs fewer than 2^32",
+			                (unsigned long long)N);
+		}
+	}
+	for (uint32_t c = 0; c < C; ++c)
+		h[T + c] = (uint32_t)p->h_cls_count[c];
+	/* about four slot buckets per transcript */
+	while (N > 1 && ((N - 1) >> sh) + 1 > 4 * (uint64_t)T)
+		++sh;
+	{
+		const size_t nb = N > 1 ? (size_t)(((N - 1) >> sh) + 1) : 1;
+		e = hemd_malloc((void **)&d, ((size_t)T + C + nb) * 4);
+		if (e) {
+			free(h);
+			return hem_fail(HEM_ERR_NOMEM, "device allocation: %s",
+			                hemd_error_string(e));
+		}
+		e = hemd_h2d(d, h, ((size_t)T + C) * 4, p->stream);
+		if (!e)
+			e = hemd_event_record(p->ev2, p->stream);
+		if (!e)
+			e = hemk_hera_resample(d, T, (uint32_t)N, d + T + C, sh,
+			                       d + T, C, seed, n_rep, p->bw.cat_rm,
+			                       p->stream);
+		if (!e)
+			e = hemd_event_record(p->ev3, p->stream);
+		if (!e)
+			e = hemd_event_sync(p->ev3);
+		if (!e && ms)
+			e = hemd_event_elapsed(p->ev2, p->ev3, ms);
+	}
+	if (e)
+		rc = hem_dev_fail(e, "hera's sampler");
+	hemd_free(d);
+	free(h);
+	return rc;
+}
+
+int hem_bootstrap_hera(hem_problem *p, const uint32_t *unique,
+                       uint32_t n_rep, uint32_t seed, const hem_opts *opts,
+                       double *est, double *tpm, uint32_t *iters,
+                       hem_batch_stats *st)
+{
+	hem_opts o;
+	int rc;
+	float ms_rs = 0;
+	const double t0 = now_ms();
+	if (!p || !unique || n_rep == 0)
+		return hem_fail(HEM_ERR_INVALID, "null argument or n_rep 0");
+	load_batch_opts(opts, &o);
+	if ((rc = check_batch_opts(p, &o)))
+		return rc;
+	pthread_mutex_lock(&p->mu);
+	if ((rc = ws_ensure(p, n_rep)))
+		goto out;
+	if ((rc = resample_hera(p, unique, n_rep, seed, &ms_rs)))
+		goto out;
+	if ((rc = stage_counts(p, n_rep)))
+		goto out;
+	rc = run_batch(p, &o, n_rep, est, tpm, iters, st);
+	if (!rc && st) {
+		st->ms_resample = ms_rs;
+		st->ms_total = now_ms() - t0;
+	}
+out:
+	pthread_mutex_unlock(&p->mu);
+	return rc;
+}
+
+int hem_bootstrap_hera_counts(hem_problem *p, const uint32_t *unique,
+                              uint32_t n_rep, uint32_t seed,
+                              uint32_t *unique_out)
+{
+	int rc;
+	const uint32_t T = p ? p->n_tx : 0, C = p ? p->n_cls : 0;
+	const size_t K = (size_t)T + C;
+	uint32_t *h;
+	if (!p || !unique || !unique_out || n_rep == 0)
+		return hem_fail(HEM_ERR_INVALID, "null argument or n_rep 0");
+	h = malloc(K * n_rep * 4);
+	if (!h)
+		return hem_fail(HEM_ERR_NOMEM, "host allocation");
+	pthread_mutex_lock(&p->mu);
+	rc = ws_ensure(p, n_rep);
+	if (!rc)
+		rc = resample_hera(p, unique, n_rep, seed, NULL);
+	if (!rc) {
+		int e = hemd_d2h(h, p->bw.cat_rm, K * n_rep * 4, p->stream);
+		if (!e)
+			e = hemd_stream_sync(p->stream);
+		if (e)
+			rc = hem_dev_fail(e, "count readback");
+	}
+	pthread_mutex_unlock(&p->mu);
+	if (!rc)
+		for (uint32_t r = 0; r < n_rep; ++r)
+			memcpy(unique_out + (size_t)r * T, h + (size_t)r * K,
+			       (size_t)T * 4);
+	free(h);
+	return rc;
+}
+
 int hem_bootstrap_counts(hem_problem *p, const uint32_t *unique,
                          uint32_t n_rep, const uint64_t *seeds,
                          uint32_t *unique_out, double *cls_out)

@@ -226,6 +226,28 @@ int hem_bootstrap_counts(hem_problem *p, const uint32_t *unique,
                          uint32_t n_rep, const uint64_t *seeds,
                          uint32_t *unique_out, double *cls_out);
 
+/* Bootstrap with hera's own sampler (create_distribution and
+ * bootstrap_sample, src/EM.c:98-150), replacing the replicate loop of
+ * write_result (src/EM.c:233-255): each replicate redraws the N unique
+ * reads as N - 1 draws of slot rand_r() % n for n = N-1 down to 1 (a read
+ * slot belongs to its transcript through the running sums of unique);
+ * class counts are not resampled.  rand_r is glibc's (three steps of the
+ * LCG x -> 1103515245 x + 12345 per call); replicate r (0-based) starts
+ * from the state 3 (N-1) r steps past seed, so seed 0 gives, bit for bit,
+ * the counts of hera's bootstrap run on one thread (its global seed starts
+ * at 0, src/hash_align.c:33, and its replicates follow each other), and
+ * any seed gives replicates that do not depend on scheduling.  Then every
+ * replicate is solved as by hem_bootstrap.  One-sample handles; N < 2^32;
+ * n_rep <= 65535. */
+int hem_bootstrap_hera(hem_problem *p, const uint32_t *unique,
+                       uint32_t n_rep, uint32_t seed, const hem_opts *opts,
+                       double *est, double *tpm, uint32_t *iters,
+                       hem_batch_stats *st);
+/* The unique counts hem_bootstrap_hera would solve: unique_out[r*n_tx + t]. */
+int hem_bootstrap_hera_counts(hem_problem *p, const uint32_t *unique,
+                              uint32_t n_rep, uint32_t seed,
+                              uint32_t *unique_out);
+
 /* ---- per-kernel timing --------------------------------------------------
  * Runs the same solve as hem_solve, one iteration at a time outside the
  * graph, with a HIP event after every launch on the solve's stream, and

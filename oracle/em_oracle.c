@@ -722,3 +722,53 @@ int oracle_write_gene_abundance(const char *path, uint32_t n,
 	}
 	return fclose(out) != 0;
 }
+
+/* ---- hera's bootstrap sampler: src/EM.c:98-150 restated -----------------
+ * oracle_rand_r is glibc's rand_r (three steps of the LCG
+ * x -> 1103515245 x + 12345 mod 2^32; 11 bits of bits 16.. of the first
+ * state, then 10 and 10 of the next two, shifted in).  oracle_hera_sample
+ * builds DIST->table as create_distribution does (one entry per unique
+ * read, in transcript order) and runs bootstrap_sample's loop
+ * `n = N; while (--n > 0) ++count[table[rand_r(&seed) % n]];` from *seed,
+ * leaving *seed where the loop leaves it (hera's global seed). */
+uint32_t oracle_rand_r(uint32_t *seed)
+{
+	uint32_t next = *seed, result;
+	next = next * 1103515245u + 12345u;
+	result = (next / 65536u) % 2048u;
+	next = next * 1103515245u + 12345u;
+	result <<= 10;
+	result ^= (next / 65536u) % 1024u;
+	next = next * 1103515245u + 12345u;
+	result <<= 10;
+	result ^= (next / 65536u) % 1024u;
+	*seed = next;
+	return result;
+}
+
+int oracle_hera_sample(const uint32_t *unique, uint32_t n_tx, uint32_t *seed,
+                       uint32_t *count)
+{
+	uint64_t N = 0;
+	uint32_t i, k, n, *table;
+	for (i = 0; i < n_tx; ++i)
+		N += unique[i];
+	memset(count, 0, (size_t)n_tx * sizeof(uint32_t));
+	if (N == 0)
+		return 0;
+	if (N > 0xFFFFFFFFull)
+		return 1;
+	table = malloc(N * sizeof(uint32_t));
+	if (!table)
+		return 1;
+	for (i = 0, k = 0; i < n_tx; ++i) {
+		uint32_t c;
+		for (c = 0; c < unique[i]; ++c)
+			table[k++] = i;
+	}
+	n = (uint32_t)N;
+	while (--n > 0)
+		++count[table[oracle_rand_r(seed) % n]];
+	free(table);
+	return 0;
+}

@@ -80,4 +80,8 @@ int oracle_write_gene_abundance(const char *path, uint32_t n,
                                 uint32_t l_gene, const double *overlap,
                                 const double *grama);
 
+uint32_t oracle_rand_r(uint32_t *seed);
+int oracle_hera_sample(const uint32_t *unique, uint32_t n_tx, uint32_t *seed,
+                       uint32_t *count);
+
 #endif

@@ -52,6 +52,10 @@ def load():
         lib.oracle_write_gene_abundance.argtypes = [
             C.c_char_p, C.c_uint32, C.c_char_p, C.c_uint32, _u32p,
             C.c_char_p, C.c_uint32, _f64p, _f64p]
+        lib.oracle_rand_r.restype = C.c_uint32
+        lib.oracle_rand_r.argtypes = [_u32p]
+        lib.oracle_hera_sample.restype = C.c_int
+        lib.oracle_hera_sample.argtypes = [_u32p, C.c_uint32, _u32p, _u32p]
         lib.oracle_em_map.restype = C.c_int
         lib.oracle_em_map.argtypes = [C.c_uint32, C.c_uint32, _u32p, _u32p,
                                       _f64p, _u32p, _u32p, _f64p, _f64p,
@@ -269,3 +273,28 @@ def write_gene_abundance(path, blocks, width, gene, gblocks, gwidth, est,
             g.ctypes.data_as(_u32p), gblocks, gwidth,
             es.ctypes.data_as(_f64p), tp.ctypes.data_as(_f64p)):
         raise OSError("oracle write failed: %s" % path)
+
+
+def rand_r_seq(seed, n):
+    """n successive values of glibc's rand_r from seed (restated);
+    returns (values, final seed)."""
+    st = C.c_uint32(seed)
+    lib = load()
+    out = np.empty(n, np.uint32)
+    for i in range(n):
+        out[i] = lib.oracle_rand_r(C.byref(st))
+    return out, st.value
+
+
+def hera_sample(unique, n_rep, seed=0):
+    """hera's bootstrap_sample (src/EM.c:123-150) for n_rep replicates run
+    one after another on one global seed, as hera with one thread."""
+    u = np.ascontiguousarray(unique, np.uint32)
+    st = C.c_uint32(seed)
+    out = np.empty((n_rep, len(u)), np.uint32)
+    for r in range(n_rep):
+        row = out[r]
+        if load().oracle_hera_sample(u.ctypes.data_as(_u32p), len(u),
+                                     C.byref(st), row.ctypes.data_as(_u32p)):
+            raise MemoryError("oracle sampler")
+    return out, st.value

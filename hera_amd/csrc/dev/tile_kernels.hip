@@ -74,18 +74,17 @@ __device__ __forceinline__ void stage_struct(const hemk_tiles &tl,
 
 // One EM map of the tile for the block's replicate chunk, lin -> lout in
 // LDS (and to column qr, the lane's replicate, of the global [q][R] vector
-// gout when given), in three passes
+// gout when given), in two passes
 // separated by barriers:
 //   class pass       D_c = sum of lam over the members, in member order
-//   term pass        X_k = (lam_t n_c) / D_c for every member k = (c, t),
-//                    one (member, replicate) pair per lane
-//   transcript pass  lam'_t = (u_t + sum of its X_k in class order) / len_t
+//   transcript pass  lam'_t = (u_t + sum of (lam_t n_c) / D_c over its
+//                    occurrences in class order) / len_t
 // which is src/EM.c:275-291 with every sum in the reference's order.  With
 // stopf the stop test of src/EM.c:292-297 marks stopf[r].
 template <typename TC, int RC>
 __device__ __forceinline__ void tile_map(
     const hemk_tiles &tl, const TileCtx &tc, const TileStruct &ts,
-    const double *lin, double *lout, double *D, double *X, const TC *n,
+    const double *lin, double *lout, double *D, const TC *n,
     const uint32_t *u, const int *alive, double *gout, uint32_t R,
     uint32_t qr, int *stopf, double smin, double srel)
 {
@@ -116,52 +115,34 @@ __device__ __forceinline__ void tile_map(
 	}
 	__syncthreads();
 	if (live) {
-		// four members per step, each stage's LDS reads issued together;
-		// the division runs on every lane (denominator 1 where D_c = 0,
-		// whose term is then 0) so no lane branches around it
-		for (uint32_t k0 = e0; k0 < tc.nz; k0 += 4 * ES) {
-			uint32_t m[4];
-#pragma unroll
-			for (int q = 0; q < 4; ++q) {
-				const uint32_t k = k0 + q * ES;
-				m[q] = ts.kmem[k < tc.nz ? k : k0];
-			}
-			double dd[4], ll[4], nn[4];
-#pragma unroll
-			for (int q = 0; q < 4; ++q) {
-				const uint32_t c = m[q] >> 8, t = m[q] & 0xFFu;
-				dd[q] = D[c * RC + r];
-				ll[q] = lin[t * RC + r];
-				nn[q] = (double)n[c * RC + r];
-			}
-#pragma unroll
-			for (int q = 0; q < 4; ++q) {
-				const uint32_t k = k0 + q * ES;
-				const double den = dd[q] == 0.0 ? 1.0 : dd[q];
-				const double x = ll[q] * nn[q] / den;
-				if (k < tc.nz)
-					X[k * RC + r] = dd[q] == 0.0 ? 0.0 : x;
-			}
-		}
-	}
-	__syncthreads();
-	if (live) {
+		// each occurrence's term (lam_t n_c) / D_c computed where the
+		// transcript's ordered sum takes it (a class with D_c == 0 adds
+		// an exact 0, as the reference skips it); four occurrences' LDS
+		// reads issued together, the division on every lane
 		for (uint32_t t = e0; t < tc.nt; t += ES) {
 			const double l = lin[t * RC + r];
 			uint32_t j = ts.toff[t];
 			const uint32_t j1 = ts.toff[t + 1];
 			double acc = 0.0;
-			for (; j + 8 <= j1; j += 8) {
-				double x[8];
+			for (; j < j1; j += 4) {
+				uint32_t c[4];
+				double dd[4], nn[4];
 #pragma unroll
-				for (int q = 0; q < 8; ++q)
-					x[q] = X[ts.tpos[j + q] * RC + r];
+				for (int q = 0; q < 4; ++q)
+					c[q] = ts.kmem[ts.tpos[j + q < j1 ? j + q : j]] >> 8;
 #pragma unroll
-				for (int q = 0; q < 8; ++q)
-					acc += x[q];
+				for (int q = 0; q < 4; ++q) {
+					dd[q] = D[c[q] * RC + r];
+					nn[q] = (double)n[c[q] * RC + r];
+				}
+#pragma unroll
+				for (int q = 0; q < 4; ++q) {
+					const double den = dd[q] == 0.0 ? 1.0 : dd[q];
+					const double x = l * nn[q] / den;
+					if (j + q < j1)
+						acc += dd[q] == 0.0 ? 0.0 : x;
+				}
 			}
-			for (; j < j1; ++j)
-				acc += X[ts.tpos[j] * RC + r];
 			const double nw = ((double)u[t * RC + r] + acc) /
 			                  (double)ts.len[t];
 			lout[t * RC + r] = nw;
@@ -177,16 +158,15 @@ __device__ __forceinline__ void tile_map(
 template <typename TC, int RC>
 __device__ __forceinline__ void carve(void *smem, const hemk_tiles &tl,
                                       double *&lam0, double *&lam1,
-                                      double *&lam2, double *&D, double *&X,
-                                      TC *&n, uint32_t *&u, TileStruct &ts)
+                                      double *&lam2, double *&D, TC *&n,
+                                      uint32_t *&u, TileStruct &ts)
 {
 	const uint32_t mt = tl.max_tx, mc = tl.max_cls, mz = tl.max_nnz;
 	lam0 = (double *)smem;
 	lam1 = lam0 + mt * RC;
 	lam2 = lam1 + mt * RC;
 	D = lam2 + mt * RC;
-	X = D + mc * RC;
-	n = (TC *)(X + mz * RC);
+	n = (TC *)(D + mc * RC);
 	u = (uint32_t *)(n + mc * RC);
 	ts.tx = u + mt * RC;
 	ts.len = ts.tx + mt;
@@ -338,11 +318,11 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
 		return;
 	const uint32_t nch = (nrep + RC - 1) / RC;
 	const TileCtx tc = load_tile(tl, tile);
-	double *lam0, *lam1, *lam2, *D, *X;
+	double *lam0, *lam1, *lam2, *D;
 	TC *n;
 	uint32_t *u;
 	TileStruct ts;
-	carve<TC, RC>(smem, tl, lam0, lam1, lam2, D, X, n, u, ts);
+	carve<TC, RC>(smem, tl, lam0, lam1, lam2, D, n, u, ts);
 	const int r = threadIdx.x % RC;
 	const uint32_t e0 = threadIdx.x / RC;
 	constexpr uint32_t ES = TBLOCK / RC;
@@ -370,9 +350,9 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
 		if (i + 1 < nch)
 			load_rows<TC, RC, 1>(p, tc, rc, al, nrep, i + 1, cnt, uniq,
 			                     A, nullptr, nullptr);
-		tile_map<TC, RC>(tl, tc, ts, lam0, lam1, D, X, n, u, alive, B,
+		tile_map<TC, RC>(tl, tc, ts, lam0, lam1, D, n, u, alive, B,
 		                 rc.R, q, nullptr, 0, 0);
-		tile_map<TC, RC>(tl, tc, ts, lam1, lam2, D, X, n, u, alive, C,
+		tile_map<TC, RC>(tl, tc, ts, lam1, lam2, D, n, u, alive, C,
 		                 rc.R, q, nullptr, 0, 0);
 		if (xr && rin) {
 			// r_t and v_t (src/EM.c:334-335) in replicate-major rows
@@ -514,11 +494,11 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_ex3(
 	const uint32_t tile0 = rc.sinfo ? rc.sinfo[4 * smp + 2] : 0u;
 	const uint32_t nch = (nrep + RC - 1) / RC;
 	const TileCtx tc = load_tile(tl, tile);
-	double *lam0, *lam1, *lam2, *D, *X;
+	double *lam0, *lam1, *lam2, *D;
 	TC *n;
 	uint32_t *u;
 	TileStruct ts;
-	carve<TC, RC>(smem, tl, lam0, lam1, lam2, D, X, n, u, ts);
+	carve<TC, RC>(smem, tl, lam0, lam1, lam2, D, n, u, ts);
 	ChunkRows<TC, 3> p;
 	load_rows<TC, RC, 3>(p, tc, rc, al, nrep, 0, cnt, uniq, A, B, C);
 	load_alpha<TC>(p, rc, smp, st);
@@ -559,7 +539,7 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_ex3(
 			                     A, B, C);
 			load_alpha<TC>(p, rc, smp, st);
 		}
-		tile_map<TC, RC>(tl, tc, ts, lam0, lam1, D, X, n, u, alive, A,
+		tile_map<TC, RC>(tl, tc, ts, lam0, lam1, D, n, u, alive, A,
 		                 rc.R, q, stopf, rc.stop_min, rc.stop_rel);
 		if (threadIdx.x < RC && rin && stopf[threadIdx.x])
 			rc.notconv[smp * rc.R + q] = 1;
@@ -927,8 +907,7 @@ inline uint32_t cdiv32(uint64_t a, uint64_t b)
 size_t tile_lds(const hemk_tiles *t, int rc, int cnt_bytes)
 {
 	return (size_t)rc * ((size_t)t->max_tx * (3 * 8 + 4) +
-	                     (size_t)t->max_cls * (8 + cnt_bytes) +
-	                     (size_t)t->max_nnz * 8) +
+	                     (size_t)t->max_cls * (8 + cnt_bytes)) +
 	       (size_t)t->max_nnz * (4 + 2) + (size_t)t->max_tx * 8 +
 	       2 * ((size_t)t->max_cls + 1) + 2 * ((size_t)t->max_tx + 1) + 16;
 }

@@ -81,6 +81,9 @@ def main():
     ap.add_argument("--ragged-samples", type=int, default=64,
                     help="samples of the ragged multi-sample batch (0: skip)")
     ap.add_argument("--ragged-reps", type=int, default=20)
+    ap.add_argument("--weighted-solves", type=int, default=3,
+                    help="solves of the workload with member weights "
+                         "(hera-EM's per-alignment probabilities); 0 skips")
     ap.add_argument("--ragged-seq-sample", type=int, default=8,
                     help="samples bootstrapped one handle at a time for the "
                          "comparison (scaled to all samples)")
@@ -225,6 +228,9 @@ def main():
 
     if args.ragged_samples > 0:
         out["ragged"] = bench_ragged(args, H, np)
+
+    if args.weighted_solves > 0:
+        out["weighted"] = bench_weighted(args, H, np, cls, u, sk)
 
     boot = None
     if args.bootstrap_reps > 0:
@@ -373,6 +379,42 @@ def bench_writer(H, np, cls, u, res):
             "note": "hem_write_result: abundance.tsv + abundance.gene.tsv "
                     "(4 transcripts per gene), including packing the name "
                     "blocks in Python"}
+
+
+def bench_weighted(args, H, np, cls, u, sk):
+    """The same table with a weight per member (uniform in [0.05, 1),
+    seeded): FP64 exact solves on the order-faithful class and transcript
+    passes, whole calls timed, plus the per-kernel profile of one solve."""
+    import dataclasses
+    rng = np.random.default_rng(args.seed + 1)
+    wc = dataclasses.replace(cls, member_weight=rng.uniform(0.05, 1.0,
+                                                            cls.nnz))
+    pw = H.Problem(wc)
+    mk = dict(precision=H.PREC_FP64, norm=H.NORM_EXACT)
+    pw.solve(u, **mk, **sk)
+    it = 0
+    t = time.perf_counter()
+    for _ in range(args.weighted_solves):
+        r = pw.solve(u, **mk, **sk)
+        it += r["stats"]["iters"]
+    dt = time.perf_counter() - t
+    prof = pw.profile(u, **mk, **sk)
+    pw.close()
+    kern = {}
+    for k, d in prof.items():
+        if k == "iters" or not d["launches"]:
+            continue
+        kern[k] = dict(launches=d["launches"], ms_avg=d["ms_avg"],
+                       bytes_per_launch=d["bytes_per_launch"],
+                       GBps=d["bytes_per_launch"] / (d["ms_avg"] * 1e-3)
+                       / 1e9)
+    res = dict(solves=args.weighted_solves, iters_per_solve=r["stats"]["iters"],
+               iters_per_s=it / dt, ms_per_solve=dt / args.weighted_solves
+               * 1e3, kernels=kern,
+               weights="uniform [0.05, 1) per member, seed %d" % (args.seed + 1))
+    log("weighted  %d solves %.4fs  %.1f it/s  k=%d" % (
+        args.weighted_solves, dt, it / dt, r["stats"]["iters"]))
+    return res
 
 
 def bench_ragged(args, H, np):

@@ -32,7 +32,10 @@ class Classes:
     """hera's class table in CSR form (Gclass + Ref_inf->len).
 
     Class c has members cls_ref[cls_off[c]:cls_off[c+1]] and cls_count[c]
-    fragments; tx_len[t] is the u32 transcript length.
+    fragments; tx_len[t] is the u32 transcript length.  member_weight
+    (optional, one float per member, >= 0) weighs each member's share of its
+    class as hera-EM's alignment probabilities do (hera-EM/smodel.h:15-20);
+    None is hera's model.
     """
     n_tx: int
     cls_off: np.ndarray
@@ -40,6 +43,7 @@ class Classes:
     cls_count: np.ndarray
     tx_len: np.ndarray
     cluster_off: np.ndarray = field(default=None)
+    member_weight: np.ndarray = field(default=None)
 
     @property
     def n_cls(self):
@@ -72,9 +76,13 @@ class Classes:
         self.cls_ref = as_u32(self.cls_ref)
         self.cls_count = as_f64(self.cls_count)
         self.tx_len = as_u32(self.tx_len)
+        w = None
+        if self.member_weight is not None:
+            self.member_weight = as_f64(self.member_weight)
+            w = f64p(self.member_weight)
         return _lib.HemClasses(self.n_tx, self.n_cls, u32p(self.cls_off),
                                u32p(self.cls_ref), f64p(self.cls_count),
-                               u32p(self.tx_len))
+                               u32p(self.tx_len), w)
 
 
 def generate(n_tx, n_cls_all, n_frag, seed=0, **overrides):

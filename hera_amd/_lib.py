@@ -27,7 +27,8 @@ _f64p = C.POINTER(C.c_double)
 class HemClasses(C.Structure):
     _fields_ = [("n_tx", C.c_uint32), ("n_cls", C.c_uint32),
                 ("cls_off", _u32p), ("cls_ref", _u32p),
-                ("cls_count", _f64p), ("tx_len", _u32p)]
+                ("cls_count", _f64p), ("tx_len", _u32p),
+                ("member_weight", _f64p)]
 
 
 class HemOpts(C.Structure):

@@ -82,11 +82,14 @@ __device__ __forceinline__ void ordered_add(double &acc, double x,
 }
 
 // FP64 class pass, order-faithful: D_c is the member-order sequential sum
-// of src/EM.c:277-278.  nd[c] = {n_c, D_c}; only D_c is written.
+// of src/EM.c:277-278.  nd[c] = {n_c, D_c}; only D_c is written.  With
+// member weights w (CSR order) the summand is lam * w, so w == 1 gives the
+// unweighted sum bit for bit.
 template <int G>
 __global__ __launch_bounds__(BLOCK) void k_denom_exact(
     const uint32_t *__restrict__ off, const uint32_t *__restrict__ ref,
-    const double *__restrict__ lam, double2 *__restrict__ nd, uint32_t n_cls,
+    const double *__restrict__ w, const double *__restrict__ lam,
+    double2 *__restrict__ nd, uint32_t n_cls,
     const hemk_ctl *__restrict__ ctl)
 {
 	if (ctl->done)
@@ -101,7 +104,12 @@ __global__ __launch_bounds__(BLOCK) void k_denom_exact(
 	double acc = 0.0;
 	for (uint32_t base = beg; base < end; base += G) {
 		const uint32_t k = base + lane;
-		const double x = k < end ? lam[ref[k]] : 0.0;
+		double x = 0.0;
+		if (k < end) {
+			x = lam[ref[k]];
+			if (w)
+				x *= w[k];
+		}
 		ordered_add<G>(acc, x, min((uint32_t)G, end - base));
 	}
 	if (c < n_cls && lane == 0)
@@ -112,11 +120,12 @@ __global__ __launch_bounds__(BLOCK) void k_denom_exact(
 // (lam_t * n_c) / D_c over the classes of t in ascending class order (the
 // order src/EM.c:276-285 visits them), skipping D_c == 0; then
 // lam'_t = ((double)u_t + overlap_t) / len_t (src/EM.c:289-290) and the
-// stop test of src/EM.c:292-297 when flags != NULL.
+// stop test of src/EM.c:292-297 when flags != NULL.  With member weights w
+// (CSC order) the term is ((lam_t * w) * n_c) / D_c.
 template <int G>
 __global__ __launch_bounds__(BLOCK) void k_gather_exact(
     const uint32_t *__restrict__ off, const uint32_t *__restrict__ ref,
-    const double2 *__restrict__ nd, const uint32_t *__restrict__ unique,
+    const double *__restrict__ w, const double2 *__restrict__ nd, const uint32_t *__restrict__ unique,
     const uint32_t *__restrict__ len, const double *__restrict__ lam,
     double *__restrict__ out64, uint32_t *__restrict__ flags, uint32_t n_tx,
     const hemk_ctl *__restrict__ ctl)
@@ -138,7 +147,8 @@ __global__ __launch_bounds__(BLOCK) void k_gather_exact(
 		double x = 0.0;
 		if (k < end) {
 			const double2 v = nd[ref[k]];
-			x = v.y == 0.0 ? 0.0 : l * v.x / v.y;
+			const double lw = w ? l * w[k] : l;
+			x = v.y == 0.0 ? 0.0 : lw * v.x / v.y;
 		}
 		ordered_add<G>(acc, x, min((uint32_t)G, end - base));
 	}
@@ -607,7 +617,7 @@ int launch_denom(const hemk_prob *p, int prec, const void *lam, void *rho,
 #define HEM_DENOM64(G)                                                      \
 	case G:                                                             \
 		k_denom_exact<G><<<grid, BLOCK, 0, s>>>(                    \
-		    p->cls_off, p->cls_ref, (const double *)lam,              \
+		    p->cls_off, p->cls_ref, p->cls_w, (const double *)lam,    \
 		    (double2 *)rho, p->n_cls, ctl);                           \
 		break;
 		HEM_GROUP_SWITCH(group, HEM_DENOM64)
@@ -635,7 +645,8 @@ int launch_gather(const hemk_prob *p, int prec, const void *rho,
 #define HEM_GATHER64(G)                                                     \
 	case G:                                                             \
 		k_gather_exact<G><<<grid, BLOCK, 0, s>>>(                   \
-		    p->tx_off, p->tx_ref, (const double2 *)rho, unique,       \
+		    p->tx_off, p->tx_ref, p->tx_w, (const double2 *)rho,      \
+		    unique,                                                   \
 		    p->len, lam, out64, flags, p->n_tx, ctl);                 \
 		break;
 		HEM_GROUP_SWITCH(group, HEM_GATHER64)

@@ -71,6 +71,8 @@ typedef struct {
 	const double   *cnt64;     /* n_cls class counts                       */
 	const uint32_t *tx_off;    /* n_tx+1, CSC over transcripts             */
 	const uint32_t *tx_ref;    /* nnz class ids, ascending per transcript  */
+	const double *cls_w;       /* nnz member weights (CSR), or NULL        */
+	const double *tx_w;        /* the same weights in CSC order, or NULL   */
 	const uint32_t *len;       /* n_tx transcript lengths                  */
 } hemk_prob;
 

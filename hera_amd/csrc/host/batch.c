@@ -399,6 +399,9 @@ static int batch_graph(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 
 static int check_batch_opts(const hem_problem *p, const hem_opts *o)
 {
+	if (p->weighted)
+		return hem_fail(HEM_ERR_INVALID, "batched solves take tables "
+		                                 "without member weights");
 	if (o->precision != HEM_PREC_FP64)
 		return hem_fail(HEM_ERR_INVALID,
 		                "batched solves compute in FP64 (precision %d)",

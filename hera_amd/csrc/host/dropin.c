@@ -83,6 +83,7 @@ static hem_problem *cached_problem(void)
 		hc.cls_ref = ref;
 		hc.cls_count = cnt;
 		hc.tx_len = REF_INF->len;
+		hc.member_weight = NULL;
 		rc = hem_problem_create(&hc, &dropin_prob);
 		free(off);
 		free(ref);

@@ -152,6 +152,9 @@ static int check_args(hem_problem *p, const uint32_t *unique,
 		return hem_fail(HEM_ERR_INVALID, "null argument");
 	if (p->ns > 1)
 		return hem_fail(HEM_ERR_INVALID, "one-sample handles only");
+	if (p->weighted)
+		return hem_fail(HEM_ERR_INVALID, "no likelihood derivatives for "
+		                                 "a table with member weights");
 	for (uint32_t t = 0; t < p->n_tx; ++t)
 		if (!(theta[t] == theta[t]) || theta[t] == 1.0 / 0.0 ||
 		    !(ell[t] == ell[t]) || ell[t] == 1.0 / 0.0 ||

@@ -98,6 +98,8 @@ struct hem_problem {
 	double *d_sx, *d_scsum, *d_sdrift, *d_stab, *d_sres;
 	long long *d_sbase;
 	uint8_t *d_snx;
+	double *d_cls_w, *d_tx_w;  /* member weights (weighted tables)   */
+	int weighted;
 	hemk_ctl *d_ctl;
 	hemk_ctl *h_ctl; /* pinned */
 

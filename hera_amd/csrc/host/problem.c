@@ -46,6 +46,14 @@ static int validate(const hem_classes *c)
 		if (c->tx_len[i] == 0)
 			return hem_fail(HEM_ERR_INVALID,
 			                "transcript %u has length 0", i);
+	if (c->member_weight)
+		for (uint64_t k = 0; k < nnz; ++k)
+			if (!(c->member_weight[k] >= 0) ||
+			    !isfinite(c->member_weight[k]))
+				return hem_fail(HEM_ERR_INVALID,
+				                "member %llu weight is negative or "
+				                "not finite",
+				                (unsigned long long)k);
 	return HEM_OK;
 }
 
@@ -120,6 +128,7 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 	int rc;
 	hem_problem *p;
 	uint32_t *tx_off = NULL, *tx_ref = NULL, *pos = NULL;
+	double *tx_w = NULL;
 	float *cnt32 = NULL;
 	double *nd = NULL;
 	uint64_t nnz;
@@ -143,10 +152,14 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 	tx_ref = malloc((nnz ? nnz : 1) * sizeof(uint32_t));
 	cnt32 = malloc((C ? C : 1) * sizeof(float));
 	nd = calloc((size_t)(C ? C : 1) * 2, sizeof(double));
-	if (!tx_off || !pos || !tx_ref || !cnt32 || !nd) {
+	if (c->member_weight)
+		tx_w = malloc((nnz ? nnz : 1) * sizeof(double));
+	if (!tx_off || !pos || !tx_ref || !cnt32 || !nd ||
+	    (c->member_weight && !tx_w)) {
 		free(tx_off);
 		free(pos);
 		free(tx_ref);
+		free(tx_w);
 		free(cnt32);
 		free(nd);
 		return hem_fail(HEM_ERR_NOMEM, "host allocation failed");
@@ -157,8 +170,11 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 		tx_off[i + 1] += tx_off[i];
 	memcpy(pos, tx_off, (size_t)T * sizeof(uint32_t));
 	for (i = 0; i < C; ++i)
-		for (uint32_t k = c->cls_off[i]; k < c->cls_off[i + 1]; ++k)
+		for (uint32_t k = c->cls_off[i]; k < c->cls_off[i + 1]; ++k) {
+			if (tx_w)
+				tx_w[pos[c->cls_ref[k]]] = c->member_weight[k];
 			tx_ref[pos[c->cls_ref[k]]++] = i;
+		}
 	for (i = 0; i < C; ++i) {
 		cnt32[i] = (float)c->cls_count[i];
 		nd[2 * (size_t)i] = c->cls_count[i];
@@ -169,6 +185,7 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 	if (!p) {
 		free(tx_off);
 		free(tx_ref);
+		free(tx_w);
 		free(cnt32);
 		free(nd);
 		return hem_fail(HEM_ERR_NOMEM, "host allocation failed");
@@ -193,6 +210,7 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 	if (rc) {
 		free(tx_off);
 		free(tx_ref);
+		free(tx_w);
 		free(cnt32);
 		free(nd);
 		hem_problem_destroy(p);
@@ -205,6 +223,7 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 		if (r_) {                                          \
 			free(tx_off);                              \
 			free(tx_ref);                              \
+			free(tx_w);                                \
 			free(cnt32);                               \
 			free(nd);                                  \
 			hem_problem_destroy(p);                    \
@@ -219,6 +238,10 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 	DALLOC_H(p->d_tx_off, ((size_t)T + 1) * 4);
 	DALLOC_H(p->d_tx_ref, (nnz ? nnz : 1) * 4);
 	DALLOC_H(p->d_len, (size_t)T * 4);
+	if (tx_w) {
+		DALLOC_H(p->d_cls_w, (nnz ? nnz : 1) * 8);
+		DALLOC_H(p->d_tx_w, (nnz ? nnz : 1) * 8);
+	}
 	DALLOC_H(p->d_A, (size_t)T * 8);
 	DALLOC_H(p->d_B, (size_t)T * 8);
 	DALLOC_H(p->d_C, (size_t)T * 8);
@@ -247,6 +270,7 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 	if (rc) {
 		free(tx_off);
 		free(tx_ref);
+		free(tx_w);
 		free(cnt32);
 		free(nd);
 		hem_problem_destroy(p);
@@ -267,6 +291,11 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 			rc = hemd_h2d(p->d_nd, nd, (size_t)C * 16, p->stream);
 		if (!rc)
 			rc = hemd_h2d(p->d_tx_ref, tx_ref, nnz * 4, p->stream);
+		if (!rc && tx_w && nnz)
+			rc = hemd_h2d(p->d_cls_w, c->member_weight, nnz * 8,
+			              p->stream);
+		if (!rc && tx_w && nnz)
+			rc = hemd_h2d(p->d_tx_w, tx_w, nnz * 8, p->stream);
 	}
 	if (!rc)
 		rc = hemd_h2d(p->d_tx_off, tx_off, ((size_t)T + 1) * 4,
@@ -278,6 +307,7 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 	if (rc) {
 		free(tx_off);
 		free(tx_ref);
+		free(tx_w);
 		free(cnt32);
 		free(nd);
 		hem_problem_destroy(p);
@@ -288,6 +318,7 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 	if (!p->h_cls_count) {
 		free(tx_off);
 		free(tx_ref);
+		free(tx_w);
 		free(cnt32);
 		free(nd);
 		hem_problem_destroy(p);
@@ -300,9 +331,10 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 		if (c->cls_count[i] > 4294967295.0 ||
 		    c->cls_count[i] != floor(c->cls_count[i]))
 			p->counts_integral = 0;
-	/* component tiles for the batched kernels; a table with a component
-	 * over the tile limits keeps only the two-pass kernels */
-	{
+	/* component tiles for the tile kernels; a table with a component
+	 * over the tile limits, or with member weights, keeps only the
+	 * two-pass kernels */
+	if (!c->member_weight) {
 		hem_tiles_host th;
 		int trc = hem_tiles_build(T, C, c->cls_off, c->cls_ref,
 		                          c->tx_len, tx_off, tx0, ns, &th);
@@ -323,6 +355,7 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 			if (trc) {
 				free(tx_off);
 				free(tx_ref);
+				free(tx_w);
 				free(cnt32);
 				free(nd);
 				hem_problem_destroy(p);
@@ -331,6 +364,7 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 		} else if (trc != HEM_ERR_RANGE || ns > 1) {
 			free(tx_off);
 			free(tx_ref);
+			free(tx_w);
 			free(cnt32);
 			free(nd);
 			hem_problem_destroy(p);
@@ -339,6 +373,7 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 	}
 	free(tx_off);
 	free(tx_ref);
+	free(tx_w);
 	free(cnt32);
 	free(nd);
 
@@ -352,6 +387,9 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 	p->dp.tx_off = p->d_tx_off;
 	p->dp.tx_ref = p->d_tx_ref;
 	p->dp.len = p->d_len;
+	p->dp.cls_w = p->d_cls_w;
+	p->dp.tx_w = p->d_tx_w;
+	p->weighted = c->member_weight != NULL;
 	*out = p;
 	return HEM_OK;
 }
@@ -377,6 +415,9 @@ int hem_samples_create(const hem_classes *samples, uint32_t n_samples,
 	for (uint32_t q = 0; q < n_samples; ++q) {
 		if ((rc = validate(&samples[q])))
 			return rc;
+		if (samples[q].member_weight)
+			return hem_fail(HEM_ERR_INVALID, "member weights are for "
+			                                 "one-sample handles");
 		T += samples[q].n_tx;
 		C += samples[q].n_cls;
 		Z += samples[q].n_cls ? samples[q].cls_off[samples[q].n_cls] : 0;
@@ -422,6 +463,7 @@ int hem_samples_create(const hem_classes *samples, uint32_t n_samples,
 	all.cls_ref = ref;
 	all.cls_count = cnt;
 	all.tx_len = len;
+	all.member_weight = NULL;
 	rc = create_impl(&all, tx0, cls0, n_samples, out);
 out:
 	free(tx0);
@@ -480,6 +522,8 @@ int hem_problem_destroy(hem_problem *p)
 	hemd_free(p->d_stab);
 	hemd_free(p->d_snx);
 	hemd_free(p->d_sres);
+	hemd_free(p->d_cls_w);
+	hemd_free(p->d_tx_w);
 	hemd_free(p->d_ctl);
 	hemd_host_free(p->h_ctl);
 	if (p->ev0)

@@ -392,6 +392,16 @@ static int load_opts(const hem_opts *opts, hem_opts *o)
 	return check_opts(o);
 }
 
+/* Member weights run on the FP64 order-faithful class and transcript
+ * kernels only. */
+static int check_weighted(const hem_problem *p, const hem_opts *o)
+{
+	if (p->weighted && o->precision != HEM_PREC_FP64)
+		return hem_fail(HEM_ERR_INVALID, "a table with member weights "
+		                                 "solves in FP64 only");
+	return HEM_OK;
+}
+
 int hem_solve(hem_problem *p, const uint32_t *unique, const hem_opts *opts,
               double *tpm, double *est, double *lambda, hem_stats *st)
 {
@@ -406,6 +416,8 @@ int hem_solve(hem_problem *p, const uint32_t *unique, const hem_opts *opts,
 		return hem_fail(HEM_ERR_INVALID, "hem_solve needs a one-sample "
 		                                 "handle (see hem_solve_batch)");
 	if ((rc = load_opts(opts, &o)))
+		return rc;
+	if ((rc = check_weighted(p, &o)))
 		return rc;
 
 	pthread_mutex_lock(&p->mu);
@@ -442,7 +454,7 @@ int hem_solve(hem_problem *p, const uint32_t *unique, const hem_opts *opts,
 		rc = hemk_finalize(p->n_tx, lam, p->d_len, p->d_part1,
 		                   p->d_C, p->d_B, p->stream);
 	}
-	if (!rc)
+	if (!rc && !p->weighted)
 		rc = hemk_loglik(&p->dp, p->d_unique, lam, p->d_part2,
 		                 p->d_out, p->stream);
 	if (!rc && tpm)
@@ -453,8 +465,10 @@ int hem_solve(hem_problem *p, const uint32_t *unique, const hem_opts *opts,
 		rc = hemd_d2h(lambda, lam, (size_t)p->n_tx * 8, p->stream);
 	if (!rc)
 		rc = hemd_d2h(p->h_ctl, p->d_ctl, sizeof(hemk_ctl), p->stream);
-	if (!rc)
+	if (!rc && !p->weighted)
 		rc = hemd_d2h(&ll, p->d_out, sizeof(double), p->stream);
+	if (p->weighted)
+		ll = NAN;
 	if (!rc)
 		rc = hemd_stream_sync(p->stream);
 	if (rc) {
@@ -498,6 +512,11 @@ static void kernel_bytes(const hem_problem *p, int f32, double *b)
 		b[HEM_PK_GATHER] = 4 * (T + 1) + 4 * Z + 16 * C +
 		                   4 * T + 4 * T + 8 * T + 8 * T;
 	}
+	if (p->weighted) {
+		/* one FP64 weight per member in each pass */
+		b[HEM_PK_DENOM] += 8 * Z;
+		b[HEM_PK_GATHER] += 8 * Z;
+	}
 	b[HEM_PK_NORM1] = 24 * T;   /* u, u1, u2 */
 	b[HEM_PK_NORM2] = 24 * T;
 	b[HEM_PK_EXTRAP] = 24 * T + 8 * T + (f32 ? 4 * T : 0);
@@ -526,6 +545,8 @@ int hem_profile_solve(hem_problem *p, const uint32_t *unique,
 	if (p->ns > 1)
 		return hem_fail(HEM_ERR_INVALID, "one-sample handles only");
 	if ((rc = load_opts(opts, &o)))
+		return rc;
+	if ((rc = check_weighted(p, &o)))
 		return rc;
 	memset(out, 0, sizeof(*out));
 	memset(&pc, 0, sizeof(pc));
@@ -598,6 +619,9 @@ int hem_loglik(hem_problem *p, const uint32_t *unique, const double *lambda,
 		return hem_fail(HEM_ERR_INVALID, "null argument");
 	if (p->ns > 1)
 		return hem_fail(HEM_ERR_INVALID, "one-sample handles only");
+	if (p->weighted)
+		return hem_fail(HEM_ERR_INVALID, "no log-likelihood for a "
+		                                 "table with member weights");
 	pthread_mutex_lock(&p->mu);
 	rc = hemd_h2d(p->d_unique, unique, (size_t)p->n_tx * 4, p->stream);
 	if (!rc)

@@ -39,7 +39,17 @@ const char *hem_last_error(void);
 /* The class table in CSR form.  Equivalent to hera's Gclass + Ref_inf:
  * class c has members cls_ref[cls_off[c] .. cls_off[c+1]) (u32 transcript
  * ids < n_tx, duplicates allowed, any order) and cls_count[c] fragments.
- * tx_len[t] is REF_INF->len[t] (must be > 0). */
+ * tx_len[t] is REF_INF->len[t] (must be > 0).
+ *
+ * member_weight (NULL in hera's model): a weight w_k >= 0 per member, as
+ * hera-EM's alignment probabilities (c_prob, hera-EM/smodel.h:15-20).  The
+ * map becomes D_c = sum_k lam_{t_k} w_k (member order) and, for each
+ * member, ((lam_t w_k) n_c) / D_c into overlap_t; w = 1 is hera's map bit
+ * for bit.  With unit lengths and the noise as one more transcript that is
+ * a member of every class with weight 4^-read_length, this is hera-EM's
+ * E-step and renormalisation (hera-EM/em.c:75-157, transdis.c:54-61) up
+ * to the scale of lam; locking and filtering are not part of it.
+ * Weighted tables run plain FP64 solves (hem_solve) only. */
 typedef struct {
 	uint32_t        n_tx;
 	uint32_t        n_cls;
@@ -47,6 +57,7 @@ typedef struct {
 	const uint32_t *cls_ref;    /* cls_off[n_cls] */
 	const double   *cls_count;  /* n_cls */
 	const uint32_t *tx_len;     /* n_tx */
+	const double   *member_weight; /* cls_off[n_cls], or NULL */
 } hem_classes;
 
 /* Bulk E-step precision.  FP64 (default): the class and transcript passes
@@ -109,7 +120,8 @@ typedef struct {
 	uint32_t maps;         /* EM map applications (3 per iteration + fallbacks) */
 	uint32_t converged;    /* 1 if the stop rule ended the loop          */
 	uint32_t fallbacks;    /* iterations where the fallback fired        */
-	double   loglik;       /* FP64 log-likelihood of the final densities */
+	double   loglik;       /* FP64 log-likelihood of the final densities
+	                          (NaN for a table with member weights)   */
 	double   last_alpha;   /* step length of the last iteration          */
 	double   ms_solve;     /* device time of the iteration loop          */
 	double   ms_total;     /* host wall time of the call                 */

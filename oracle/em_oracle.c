@@ -43,6 +43,7 @@ typedef struct {
 	const uint32_t *off, *ref;
 	const double *cnt;
 	const uint32_t *len;
+	const double *w;      /* member weights (CSR order), or NULL */
 } o_prob;
 
 /* Norm, src/hash_align.c:150-164: mean, centred sum of squares / (n-1),
@@ -63,7 +64,11 @@ double oracle_norm_sd(const double *arr, uint32_t n)
 
 /* One EM map, src/EM.c:271-302.  grama is read, overlap is the scratch the
  * class loop accumulates into and is left zeroed.  Returns a calloc'd
- * vector like the reference (the allocation is part of the timed cost). */
+ * vector like the reference (the allocation is part of the timed cost).
+ * With member weights the member's share is grama * w in both sums, the
+ * responsibility of hera-EM's simple model (s_joint_prob, hera-EM/smodel.h:
+ * 15-20, over the read's total, hera-EM/em.c:106-133); w == NULL is the
+ * reference's map unchanged. */
 static double *o_mstep(const o_prob *P, const double *grama, double *overlap,
                        const uint32_t *count, uint32_t *stop,
                        double stop_min, double stop_rel)
@@ -75,12 +80,22 @@ static double *o_mstep(const o_prob *P, const double *grama, double *overlap,
 	for (i = 0; i < P->n_cls; ++i) {
 		const uint32_t *r = P->ref + P->off[i];
 		uint32_t n = P->off[i + 1] - P->off[i];
+		const double *w = P->w ? P->w + P->off[i] : NULL;
+		if (!w) {
+			for (k = 0, total = 0.0; k < n; ++k)
+				total += grama[r[k]];
+			if (total == 0)
+				continue;
+			for (k = 0; k < n; ++k)
+				overlap[r[k]] += grama[r[k]] * P->cnt[i] / total;
+			continue;
+		}
 		for (k = 0, total = 0.0; k < n; ++k)
-			total += grama[r[k]];
+			total += grama[r[k]] * w[k];
 		if (total == 0)
 			continue;
 		for (k = 0; k < n; ++k)
-			overlap[r[k]] += grama[r[k]] * P->cnt[i] / total;
+			overlap[r[k]] += grama[r[k]] * w[k] * P->cnt[i] / total;
 	}
 
 	for (i = 0, max_err = 0.0, *stop = 1; i < P->n_tx; i++) {
@@ -115,7 +130,19 @@ int oracle_estimate_count(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
                           double *tpm, double *est, double *lambda,
                           oracle_trace *tr)
 {
-	o_prob P = {n_tx, n_cls, off, ref, cnt, len};
+	return oracle_estimate_count_w(n_tx, n_cls, off, ref, cnt, NULL, len,
+	                               count, opt, init, tpm, est, lambda, tr);
+}
+
+int oracle_estimate_count_w(uint32_t n_tx, uint32_t n_cls,
+                            const uint32_t *off, const uint32_t *ref,
+                            const double *cnt, const double *w,
+                            const uint32_t *len, const uint32_t *count,
+                            const oracle_opts *opt, const double *init,
+                            double *tpm, double *est, double *lambda,
+                            oracle_trace *tr)
+{
+	o_prob P = {n_tx, n_cls, off, ref, cnt, len, w};
 	uint32_t i, k, stop;
 	double *u, *u1, *u2, *grama, *overlap, sum, alpha, n_v, n_r;
 	double *r = malloc(n_tx * sizeof(double));
@@ -221,7 +248,16 @@ int oracle_em_map(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
                   const uint32_t *count, const double *lam_in,
                   double *lam_out, uint32_t *stop)
 {
-	o_prob P = {n_tx, n_cls, off, ref, cnt, len};
+	return oracle_em_map_w(n_tx, n_cls, off, ref, cnt, NULL, len, count,
+	                       lam_in, lam_out, stop);
+}
+
+int oracle_em_map_w(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
+                    const uint32_t *ref, const double *cnt, const double *w,
+                    const uint32_t *len, const uint32_t *count,
+                    const double *lam_in, double *lam_out, uint32_t *stop)
+{
+	o_prob P = {n_tx, n_cls, off, ref, cnt, len, w};
 	double *ov = calloc(n_tx, sizeof(double));
 	double *nw;
 	if (!ov)

@@ -38,6 +38,19 @@ int oracle_em_map(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
                   const uint32_t *ref, const double *cnt, const uint32_t *len,
                   const uint32_t *count, const double *lam_in,
                   double *lam_out, uint32_t *stop);
+/* The same two with per-member weights w (CSR order; NULL = unweighted):
+ * D_c = sum_k lam_{t_k} w_k and each member adds lam_t w_k n_c / D_c. */
+int oracle_estimate_count_w(uint32_t n_tx, uint32_t n_cls,
+                            const uint32_t *off, const uint32_t *ref,
+                            const double *cnt, const double *w,
+                            const uint32_t *len, const uint32_t *count,
+                            const oracle_opts *opt, const double *init,
+                            double *tpm, double *est, double *lambda,
+                            oracle_trace *tr);
+int oracle_em_map_w(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
+                    const uint32_t *ref, const double *cnt, const double *w,
+                    const uint32_t *len, const uint32_t *count,
+                    const double *lam_in, double *lam_out, uint32_t *stop);
 double oracle_loglik(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
                      const uint32_t *ref, const double *cnt,
                      const uint32_t *len, const uint32_t *count,

@@ -44,6 +44,15 @@ def load():
             C.c_uint32, C.c_uint32, _u32p, _u32p, _f64p, _u32p, _u32p,
             C.POINTER(OracleOpts), _f64p, _f64p, _f64p, _f64p,
             C.POINTER(OracleTrace)]
+        lib.oracle_estimate_count_w.restype = C.c_int
+        lib.oracle_estimate_count_w.argtypes = [
+            C.c_uint32, C.c_uint32, _u32p, _u32p, _f64p, _f64p, _u32p, _u32p,
+            C.POINTER(OracleOpts), _f64p, _f64p, _f64p, _f64p,
+            C.POINTER(OracleTrace)]
+        lib.oracle_em_map_w.restype = C.c_int
+        lib.oracle_em_map_w.argtypes = [C.c_uint32, C.c_uint32, _u32p, _u32p,
+                                        _f64p, _f64p, _u32p, _u32p, _f64p,
+                                        _f64p, _u32p]
         lib.oracle_write_abundance.restype = C.c_int
         lib.oracle_write_abundance.argtypes = [
             C.c_char_p, C.c_uint32, C.c_char_p, C.c_uint32, _u32p, _u32p,
@@ -135,10 +144,13 @@ def estimate_count(cls, unique, init=None, trace_cap=1000, **kw):
     last_u = np.zeros(T)
     tr = OracleTrace(trace_cap, 0, _p64(alpha), _p32(stop), _p64(last_u))
     ini = _f64(init) if init is not None else None
-    rc = lib.oracle_estimate_count(T, Cn, _p32(off), _p32(ref), _p64(cnt),
-                                   _p32(ln), _p32(u), opts(**kw),
-                                   _p64(ini) if ini is not None else None,
-                                   _p64(tpm), _p64(est), _p64(lam), tr)
+    w = getattr(cls, "member_weight", None)
+    w = _f64(w) if w is not None else None
+    rc = lib.oracle_estimate_count_w(T, Cn, _p32(off), _p32(ref), _p64(cnt),
+                                     _p64(w) if w is not None else None,
+                                     _p32(ln), _p32(u), opts(**kw),
+                                     _p64(ini) if ini is not None else None,
+                                     _p64(tpm), _p64(est), _p64(lam), tr)
     assert rc == 0
     k = tr.k
     return dict(tpm=tpm, est=est, lam=lam, k=k,
@@ -154,9 +166,12 @@ def em_map(cls, unique, lam):
                        _f64(lam))
     out = np.empty(T)
     stop = C.c_uint32()
-    assert lib.oracle_em_map(T, cls.n_cls, _p32(off), _p32(ref), _p64(cnt),
-                             _p32(ln), _p32(u), _p64(lam), _p64(out),
-                             C.byref(stop)) == 0
+    w = getattr(cls, "member_weight", None)
+    w = _f64(w) if w is not None else None
+    assert lib.oracle_em_map_w(T, cls.n_cls, _p32(off), _p32(ref), _p64(cnt),
+                               _p64(w) if w is not None else None,
+                               _p32(ln), _p32(u), _p64(lam), _p64(out),
+                               C.byref(stop)) == 0
     return out, stop.value
 
 

@@ -45,3 +45,35 @@ def cfg1():
     drawn (singletons folded into unique counts), 10M fragments, seed 0."""
     import hera_amd as H
     return H.generate(20000, 200000, 10_000_000, seed=0)
+
+
+def hera_em_reads(seed, n_tx=30, n_pat=200, max_aln=4):
+    """A hera-EM simple-model read set: n_pat distinct read patterns, each a
+    list of (transcript, c_prob) alignments, a read length and a count of
+    identical reads."""
+    rng = np.random.default_rng(seed)
+    pats = []
+    for _ in range(n_pat):
+        k = int(rng.integers(1, max_aln + 1))
+        ts = rng.choice(n_tx, size=k, replace=False)
+        cp = rng.uniform(1e-3, 1.0, size=k)
+        pats.append(([(int(t), float(c)) for t, c in zip(ts, cp)],
+                     int(rng.integers(3, 9)), int(rng.integers(1, 60))))
+    return pats
+
+
+def hera_em_table(n_tx, pats):
+    """The class table whose weighted map is hera-EM's simple-model E-step:
+    one class per read pattern (count = identical reads), its alignments as
+    members weighted by c_prob, and the noise as transcript n_tx, a member
+    of every class with weight 4^-read_length (hera-EM/smodel.h:15-26,
+    utility.c:10-12).  Lengths 1, no unique reads."""
+    from hera_amd import Classes
+    members, weights, counts = [], [], []
+    for aln, rl, n in pats:
+        members.append([t for t, _ in aln] + [n_tx])
+        weights += [c for _, c in aln] + [0.25 ** rl]
+        counts.append(float(n))
+    c = Classes.from_lists(n_tx + 1, members, counts, [1] * (n_tx + 1))
+    c.member_weight = np.array(weights)
+    return c, np.zeros(n_tx + 1, np.uint32)

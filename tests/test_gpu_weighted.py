@@ -1,0 +1,118 @@
+"""Class tables with per-member weights on the GPU (hem_classes
+member_weight): the FP64 exact solve bit for bit the oracle's weighted
+estimate_count, unit weights bit for bit the unweighted solve, and the
+entry points that do not take weights refusing them.  The oracle's weighted
+map is pinned to hera-EM's simple-model E-step in tests/test_weighted.py.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import hera_amd as H
+from hera_amd import _lib
+from conftest import hera_em_reads, hera_em_table
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# hera's SQUAREM forms, and hera-EM's (L2 step, u2/10 clamp, no step clamp)
+FORMS = [dict(), dict(step_form=1, clamp_form=1, alpha_clamp=0)]
+GKEY = dict(step_form="step", clamp_form="clamp", alpha_clamp="alpha_clamp",
+            min_round="min_iter", max_round="max_iter", stop_min="stop_min_value",
+            stop_rel="stop_rel_change")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def solve_both(cls, u, **kw):
+    ora = O.estimate_count(cls, u, **kw)
+    p = H.Problem(cls)
+    assert p.tiles()["n_tiles"] == 0      # weighted tables run untiled
+    g = p.solve(u, **{GKEY[k]: v for k, v in kw.items()})
+    p.close()
+    say("T = %d, nnz = %d, k = %d / %d" % (cls.n_tx, cls.nnz,
+                                           g["stats"]["iters"], ora["k"]))
+    return g, ora
+
+
+def weighted_cfg(seed):
+    cls, u = H.generate(20000, 200000, 10_000_000, seed=seed)
+    rng = np.random.default_rng(seed)
+    cls.member_weight = rng.uniform(0.05, 1.0, cls.nnz)
+    return cls, u
+
+
+@pytest.mark.parametrize("kw", FORMS)
+def test_weighted_generated_bitwise(kw):
+    cls, u = weighted_cfg(7)
+    g, ora = solve_both(cls, u, **kw)
+    assert g["stats"]["iters"] == ora["k"]
+    np.testing.assert_array_equal(g["lam"], ora["lam"])
+    np.testing.assert_array_equal(g["tpm"], ora["tpm"])
+    np.testing.assert_array_equal(g["est"], ora["est"])
+    assert np.isnan(g["stats"]["loglik"])
+
+
+@pytest.mark.parametrize("kw", FORMS)
+def test_weighted_hera_em_table_bitwise(kw):
+    pats = hera_em_reads(9, 500, n_pat=4000, max_aln=6)
+    cls, u = hera_em_table(500, pats)
+    kw = dict(kw, stop_min=0.0, stop_rel=1e-10)
+    g, ora = solve_both(cls, u, **kw)
+    assert g["stats"]["iters"] == ora["k"]
+    np.testing.assert_array_equal(g["lam"], ora["lam"])
+    np.testing.assert_array_equal(g["tpm"], ora["tpm"])
+
+
+def test_unit_weights_bitwise_unweighted():
+    cls, u = H.generate(20000, 200000, 10_000_000, seed=0)
+    p = H.Problem(cls)
+    a = p.solve(u)
+    p.close()
+    cls.member_weight = np.ones(cls.nnz)
+    p = H.Problem(cls)
+    b = p.solve(u)
+    p.close()
+    assert a["stats"]["iters"] == b["stats"]["iters"]
+    np.testing.assert_array_equal(a["lam"], b["lam"])
+    np.testing.assert_array_equal(a["tpm"], b["tpm"])
+
+
+def test_weighted_tree_norm_close():
+    cls, u = weighted_cfg(8)
+    ora = O.estimate_count(cls, u)
+    p = H.Problem(cls)
+    g = p.solve(u, norm=H.NORM_TREE)
+    p.close()
+    big = ora["tpm"] > 1e-2
+    np.testing.assert_allclose(g["tpm"][big], ora["tpm"][big], rtol=1e-4)
+
+
+def test_weighted_rejects():
+    cls, u = weighted_cfg(9)
+    p = H.Problem(cls)
+    for call in (lambda: p.solve(u, precision=H.PREC_FP32),
+                 lambda: p.loglik(u, np.ones(cls.n_tx)),
+                 lambda: p.grad(u, np.zeros(cls.n_tx), np.zeros(cls.n_tx)),
+                 lambda: p.solve_batch(np.stack([u, u])),
+                 lambda: p.bootstrap_hera(u, 2, 0)):
+        with pytest.raises(H.HemError) as ei:
+            call()
+        assert ei.value.code == _lib.HEM_ERR_INVALID
+    p.close()
+    with pytest.raises(H.HemError) as ei:
+        H.Problem.from_samples([cls, cls])
+    assert ei.value.code == _lib.HEM_ERR_INVALID
+    bad = weighted_cfg(9)[0]
+    bad.member_weight[5] = -1.0
+    with pytest.raises(H.HemError) as ei:
+        H.Problem(bad)
+    assert ei.value.code == _lib.HEM_ERR_INVALID
+    bad.member_weight[5] = np.nan
+    with pytest.raises(H.HemError) as ei:
+        H.Problem(bad)
+    assert ei.value.code == _lib.HEM_ERR_INVALID

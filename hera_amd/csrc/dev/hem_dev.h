@@ -102,8 +102,7 @@ typedef struct {
 	const uint32_t *coff;   /* class position -> member offset (+1)      */
 	const uint32_t *kmem;   /* member -> (local class << 8) | local tx    */
 	const uint32_t *toff;   /* tile position -> class-list offset (+1)   */
-	const uint16_t *tpos;   /* class-list entry -> member position in the
-	                           tile's member list                        */
+	const uint16_t *tcls;   /* class-list entry -> local class           */
 	const uint32_t *txq;    /* transcript id -> tile position            */
 	const uint32_t *rec;    /* per-tile records (host/tiles.c), 16-byte
 	                           units; tile i's at desc[8i + 7]          */

@@ -42,7 +42,8 @@ __device__ __forceinline__ PTile ptile(const hemk_tiles &t, uint32_t i)
 }
 
 // Bytes of a tile's record: u16 class offsets (nc + 1), class-list offsets
-// (nt + 1), members (c << 6 | t) and class-list positions (nz each), then
+// (nt + 1), members (c << 6 | t) in class order and in class-list order
+// (nz each), then
 // 4-byte aligned u32 lengths and transcript ids (nt each); 16-byte units.
 __device__ __forceinline__ uint32_t rec_bytes(uint32_t nt, uint32_t nc,
                                               uint32_t nz)
@@ -60,7 +61,7 @@ struct PLds {
 	uint32_t *u;
 	uint4 *rec;
 	// record fields of the current tile
-	const uint16_t *coff, *toff, *kmem, *tpos;
+	const uint16_t *coff, *toff, *kmem, *kocc;
 	const uint32_t *len, *tx;
 };
 
@@ -85,7 +86,7 @@ __device__ __forceinline__ void pfields(PLds &L, const PTile &tc)
 	L.coff = r;
 	L.toff = L.coff + tc.nc + 1;
 	L.kmem = L.toff + tc.nt + 1;
-	L.tpos = L.kmem + tc.nz;
+	L.kocc = L.kmem + tc.nz;
 	uint32_t k = (tc.nc + 1) + (tc.nt + 1) + 2 * tc.nz;
 	k += k & 1;
 	L.len = (const uint32_t *)(r + k);
@@ -167,11 +168,11 @@ __device__ __forceinline__ void pmap(const PLds &L, const PTile &tc,
 		L.D[c] = acc;
 	}
 	__syncthreads();
-	// (lam_t n_c) / D_c for occurrence j of transcript t (member tpos[j]);
+	// (lam_t n_c) / D_c for occurrence j, class c of transcript t;
 	// a class with D_c == 0 is skipped by the reference, which a zero term
 	// reproduces (sums start at +0 and every term is >= 0)
 	for (uint32_t j = tid; j < tc.nz; j += PBLOCK) {
-		const uint32_t m = L.kmem[L.tpos[j]];
+		const uint32_t m = L.kocc[j];
 		const uint32_t c = m >> 6, t = m & 0x3Fu;
 		const double d = L.D[c];
 		const double x = lin[t] * L.n[c] / (d == 0.0 ? 1.0 : d);

@@ -68,14 +68,14 @@ __global__ __launch_bounds__(SB) void k_se_tile(
 	uint32_t *kmem = (uint32_t *)(ks + mc);
 	uint16_t *coff = (uint16_t *)(kmem + mz);
 	uint16_t *toff = coff + (mc + 1);
-	uint16_t *tpos = toff + (mt + 1);
+	uint16_t *tcls = toff + (mt + 1);
 	for (uint32_t i = threadIdx.x; i <= tc.nc; i += SB)
 		coff[i] = (uint16_t)(tl.coff[tc.cls_base + i] - tc.mb);
 	for (uint32_t i = threadIdx.x; i <= tc.nt; i += SB)
 		toff[i] = (uint16_t)(tl.toff[tc.tx_base + i] - tc.jb);
 	for (uint32_t i = threadIdx.x; i < tc.nz; i += SB) {
 		kmem[i] = tl.kmem[tc.mb + i];
-		tpos[i] = tl.tpos[tc.jb + i];
+		tcls[i] = tl.tcls[tc.jb + i];
 	}
 	for (uint32_t i = threadIdx.x; i < tc.nt; i += SB) {
 		xs[i] = xq[tc.tx_base + i];
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(SB) void k_se_tile(
 		for (uint32_t t = e0; t < tc.nt; t += ES) {
 			double acc = 0;
 			for (uint32_t j = toff[t]; j < toff[t + 1]; ++j) {
-				const uint32_t c = kmem[tpos[j]] >> 8;
+				const uint32_t c = tcls[j];
 				acc += Dd[c * RC + r];
 			}
 			if (kin)

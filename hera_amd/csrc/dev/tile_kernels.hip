@@ -43,13 +43,13 @@ __device__ __forceinline__ TileCtx load_tile(const hemk_tiles &t, uint32_t i)
 
 // The tile's structure staged in LDS, offsets relative to the tile: member
 // lists (each member as (local class << 8) | local transcript), per
-// transcript the member positions of its occurrences in class order, and
+// transcript the local classes of its occurrences in class order, and
 // the transcripts' ids and lengths.
 struct TileStruct {
 	uint16_t *coff;  // nc + 1
 	uint16_t *toff;  // nt + 1
 	uint32_t *kmem;  // members
-	uint16_t *tpos;  // class-list entries
+	uint16_t *tcls;  // class-list entries: local class
 	uint32_t *tx;    // nt
 	uint32_t *len;   // nt
 };
@@ -64,7 +64,7 @@ __device__ __forceinline__ void stage_struct(const hemk_tiles &tl,
 		ts.toff[i] = (uint16_t)(tl.toff[tc.tx_base + i] - tc.jb);
 	for (uint32_t i = threadIdx.x; i < tc.nz; i += TBLOCK) {
 		ts.kmem[i] = tl.kmem[tc.mb + i];
-		ts.tpos[i] = tl.tpos[tc.jb + i];
+		ts.tcls[i] = tl.tcls[tc.jb + i];
 	}
 	for (uint32_t i = threadIdx.x; i < tc.nt; i += TBLOCK) {
 		ts.tx[i] = tl.tx[tc.tx_base + i];
@@ -129,7 +129,7 @@ __device__ __forceinline__ void tile_map(
 				double dd[4], nn[4];
 #pragma unroll
 				for (int q = 0; q < 4; ++q)
-					c[q] = ts.kmem[ts.tpos[j + q < j1 ? j + q : j]] >> 8;
+					c[q] = ts.tcls[j + q < j1 ? j + q : j];
 #pragma unroll
 				for (int q = 0; q < 4; ++q) {
 					dd[q] = D[c[q] * RC + r];
@@ -173,7 +173,7 @@ __device__ __forceinline__ void carve(void *smem, const hemk_tiles &tl,
 	ts.kmem = ts.len + mt;
 	ts.coff = (uint16_t *)(ts.kmem + mz);
 	ts.toff = ts.coff + (mc + 1);
-	ts.tpos = ts.toff + (mt + 1);
+	ts.tcls = ts.toff + (mt + 1);
 }
 
 // Rows of one replicate chunk held in registers from the prefetch, issued

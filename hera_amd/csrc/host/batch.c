@@ -39,7 +39,7 @@ int hem_tiles_upload(hem_problem *p, const hem_tiles_host *h)
 	    (size_t)(h->rec_units ? h->rec_units : 1) * 16};
 	const void *src[HEM_TILE_ARRAYS] = {
 	    h->desc, h->tx,   h->len,      h->coff, h->kmem,
-	    h->toff, h->tpos, h->cls_orig, h->txq,  h->rec};
+	    h->toff, h->tcls, h->cls_orig, h->txq,  h->rec};
 	for (int i = 0; i < HEM_TILE_ARRAYS; ++i) {
 		int e = hemd_malloc((void **)&p->d_tile_mem[i], sz[i]);
 		if (e)
@@ -62,7 +62,7 @@ int hem_tiles_upload(hem_problem *p, const hem_tiles_host *h)
 	p->dt.coff = p->d_tile_mem[3];
 	p->dt.kmem = p->d_tile_mem[4];
 	p->dt.toff = p->d_tile_mem[5];
-	p->dt.tpos = (const uint16_t *)p->d_tile_mem[6];
+	p->dt.tcls = (const uint16_t *)p->d_tile_mem[6];
 	p->d_cls_orig = p->d_tile_mem[7];
 	p->dt.txq = p->d_tile_mem[8];
 	p->dt.rec = p->d_tile_mem[9];

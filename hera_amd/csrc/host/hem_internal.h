@@ -152,7 +152,7 @@ typedef struct {
 	uint64_t nnz;
 	uint32_t *desc, *tx, *len, *coff, *toff, *cls_orig, *txq;
 	uint32_t *kmem;
-	uint16_t *tpos;
+	uint16_t *tcls;    /* class-list entry -> local class            */
 	uint32_t *tsamp;   /* tile -> sample (ns > 1 only)               */
 	uint32_t *rec;     /* per-tile records, 16-byte units (desc[7])  */
 	uint64_t rec_units;

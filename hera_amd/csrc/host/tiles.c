@@ -21,11 +21,10 @@
  *   kmem     members in hera's order, each (local class << 8) | local
  *            transcript                                 (members)
  *   toff     class-list offsets of the tile's tx        (n_tx + 1 shared)
- *   tpos     per transcript, the positions (u16, relative to the tile's
- *            first member) of its occurrences in the member lists, in
- *            ascending original class id and, within a class, member order
- *            -- the order hera's class loop adds into overlap[t]
- *            (src/EM.c:276-285)
+ *   tcls     per transcript, the local class (u16) of each of its
+ *            occurrences, in ascending original class id and, within a
+ *            class, member order -- the order hera's class loop adds into
+ *            overlap[t] (src/EM.c:276-285)
  * Classes are renumbered in tile order ("tile position"); per-replicate
  * class counts are stored in that order so a tile reads one contiguous
  * block.  cls_orig maps a tile position back to the caller's class id.
@@ -49,8 +48,10 @@ static uint32_t uf_find(uint32_t *par, uint32_t x)
  * (ptile_kernels.hip), 16-byte aligned, its offset in 16-byte units in
  * desc[7]: u16 class offsets (n_cls + 1) and class-list offsets
  * (n_tx + 1), both relative to the tile; u16 members
- * (local class << 6) | local transcript; u16 class-list positions; then,
- * 4-byte aligned, u32 lengths and u32 transcript ids (n_tx each). */
+ * (local class << 6) | local transcript in class order, then the same u16
+ * pairs in class-list order (each transcript's occurrences in the order its
+ * sum takes them); then, 4-byte aligned, u32 lengths and u32 transcript
+ * ids (n_tx each).  Runs while tcls still holds member positions. */
 static int tiles_records(hem_tiles_host *h)
 {
 	uint64_t units = 0;
@@ -82,8 +83,10 @@ static int tiles_records(hem_tiles_host *h)
 			const uint32_t m = h->kmem[d[4] + z];
 			r16[k++] = (uint16_t)(((m >> 8) << 6) | (m & 0x3Fu));
 		}
-		for (uint32_t z = 0; z < nz; ++z)
-			r16[k++] = h->tpos[d[6] + z];
+		for (uint32_t z = 0; z < nz; ++z) {
+			const uint32_t m = h->kmem[d[4] + h->tcls[d[6] + z]];
+			r16[k++] = (uint16_t)(((m >> 8) << 6) | (m & 0x3Fu));
+		}
 		k += k & 1;
 		uint32_t *r32 = (uint32_t *)r16 + k / 2;
 		for (uint32_t t = 0; t < nt; ++t) {
@@ -107,7 +110,7 @@ void hem_tiles_free_host(hem_tiles_host *h)
 	free(h->kmem);
 	free(h->txq);
 	free(h->toff);
-	free(h->tpos);
+	free(h->tcls);
 	free(h->rec);
 	free(h->cls_orig);
 	free(h->tsamp);
@@ -219,10 +222,10 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 	h->tx = malloc((size_t)T * 4);
 	h->len = malloc((size_t)T * 4);
 	h->toff = malloc(((size_t)T + 1) * 4);
-	h->tpos = malloc((zlive ? zlive : 1) * 2);
+	h->tcls = malloc((zlive ? zlive : 1) * 2);
 	pos = malloc(((size_t)ntile + 1) * 4);
 	if (!tile_ntx || !tile_ncls || !h->desc || !h->tx || !h->len ||
-	    !h->toff || !h->tpos || !pos) {
+	    !h->toff || !h->tcls || !pos) {
 		rc = hem_fail(HEM_ERR_NOMEM, "tiling: host allocation");
 		goto out;
 	}
@@ -312,7 +315,8 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 	}
 	/* each transcript's occurrences as members, in ascending class id and
 	 * member order (a counting sort of the member positions), then
-	 * relative to the first member of the transcript's tile */
+	 * relative to the first member of the transcript's tile; the records
+	 * read these positions, then they become local classes */
 	{
 		uint32_t *fill = lidx; /* reused: next free slot per transcript */
 		for (i = 0; i < T; ++i)
@@ -329,7 +333,7 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 				uint32_t t = h->tx[q];
 				h->toff[q] = (uint32_t)z;
 				for (k = tx_off[t]; k < tx_off[t + 1]; ++k)
-					h->tpos[z++] =
+					h->tcls[z++] =
 					    (uint16_t)(mloc[txpos[k]] - mb);
 			}
 		}
@@ -379,6 +383,12 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 	}
 	if ((rc = tiles_records(h)))
 		goto out;
+	for (i = 0; i < ntile; ++i) {
+		const uint32_t *d = h->desc + (size_t)HEM_TILE_DESC * i;
+		const uint32_t nz = h->toff[d[0] + d[1]] - d[6];
+		for (uint32_t z = d[6]; z < d[6] + nz; ++z)
+			h->tcls[z] = (uint16_t)(h->kmem[d[4] + h->tcls[z]] >> 8);
+	}
 out:
 	if (rc)
 		hem_tiles_free_host(h);

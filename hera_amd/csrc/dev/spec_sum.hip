@@ -37,7 +37,6 @@
 
 #define SBLOCK 256
 #define WAVE 64
-#define WIN_LDS (32 * 1024)   // window kernel: chunk values staged per wave
 
 namespace {
 
@@ -535,12 +534,15 @@ __global__ __launch_bounds__(WAVE) void k_spec_window(hemk_spec sp,
 
 // Repairs after a miss.  A miss at chunk j means the guess g_j was off by
 // delta = s_j - g_j, and the following guesses are usually off by about the
-// same amount.  Wave 0 runs chunks j .. j+RC-1 at once: group 0 from the
-// exact s_j, group k from the RW doubles centred on g_{j+k} + delta;
-// results go to rtl[k*RW + c], window bases to rbl[k].
+// same amount.  Waves 0 and 1 run chunks j .. j+RC-1 at once: group 0 from
+// the exact s_j, group k from the RW doubles centred on g_{j+k} + delta;
+// results go to rtl[k*RW + c], window bases to rbl[k].  Eight chunks per
+// repair (rather than four) roughly halve the longest run of repairs in a
+// row, which sets the resolve time when many rows resolve at once.
 #define RW 16
-#define RC (WAVE / RW)
-static_assert(RC * HEMK_SPEC_LMAX * 8 <= WIN_LDS, "repair chunks fit LDS");
+#define RC 8
+#define REP_LDS (RC * HEMK_SPEC_LMAX * 8)   // a repair's chunk values
+static_assert(RC * RW <= SBLOCK, "repair lanes fit the workgroup");
 // chunks whose links the resolve kernel holds in LDS at once
 #define SEGP 1024
 
@@ -556,7 +558,7 @@ __device__ __forceinline__ void repair(const hemk_spec &sp, uint32_t row,
 	stage<XF>(sp.x + (size_t)row * sp.ld + (size_t)j * sp.L, tot, m, ys,
 	          SBLOCK);
 	__syncthreads();
-	if (threadIdx.x < WAVE) {
+	if (threadIdx.x < RC * RW) {
 		const int lane = threadIdx.x, k = lane / RW, c = lane % RW;
 		const uint32_t jj = j + k;
 		const double delta = s - dunkey(kadd(sp.base[cb + j], W / 2));
@@ -601,8 +603,8 @@ __global__ __launch_bounds__(SBLOCK) void k_spec_resolve(hemk_spec sp,
 	__shared__ uint8_t nxl[SEGP * W];
 	__shared__ uint8_t c1[SEGP / 16 * W];
 	__shared__ uint8_t c2[SEGP / 256 * W];
-	__shared__ double ys[WIN_LDS / 8];
-	__shared__ double rtl[WAVE];
+	__shared__ double ys[REP_LDS / 8];
+	__shared__ double rtl[RC * RW];
 	__shared__ long long rbl[RC];
 	uint32_t row;
 	if (skip(ctl) || !chain_row(sp, blockIdx.x, row))

@@ -407,7 +407,7 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
 				b0 = 0;
 				b1 = qv;
 			}
-			double *pp = part + ((size_t)tile * rc.R + q) * 4;
+			double *pp = part + ((size_t)q * tl.n_tiles + tile) * 4;
 			pp[0] = a0;
 			pp[1] = a1;
 			pp[2] = b0;
@@ -568,7 +568,8 @@ __device__ __forceinline__ Moments mom_merge(Moments a, Moments b)
 }
 
 // One block per instance: fold the tile partials of its sample in tile
-// order.
+// order.  Partials are replicate-major (part[(q * n_tiles + tile) * 4]), so
+// each thread reads its run of tiles as one contiguous stream.
 __global__ __launch_bounds__(TBLOCK) void k_batch_alpha(
     hemk_tiles tl, hemk_rctl rc, const double *__restrict__ part, uint32_t T,
     hemk_step st)
@@ -586,7 +587,7 @@ __global__ __launch_bounds__(TBLOCK) void k_batch_alpha(
 	Moments a = {0, 0, 0}, b = {0, 0, 0};
 	for (uint32_t i = i0; i < i1; ++i) {
 		const double nt = (double)tl.desc[(size_t)HEMK_TILE_DESC * i + 1];
-		const double *pp = part + ((size_t)i * rc.R + r) * 4;
+		const double *pp = part + ((size_t)r * tl.n_tiles + i) * 4;
 		if (st.step == 0) {
 			a = mom_merge(a, Moments{nt, pp[0], pp[1]});
 			b = mom_merge(b, Moments{nt, pp[2], pp[3]});

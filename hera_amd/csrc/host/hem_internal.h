@@ -145,6 +145,12 @@ struct hem_problem {
 #define HEM_TILE_TX_TARGET  16
 #define HEM_TILE_CLS_TARGET 80
 #define HEM_TILE_NNZ_TARGET 384
+/* a tile holds at most HEM_TILE_CLS transcripts or classes (tiles.c sorts
+ * them in buffers of that size) */
+_Static_assert(HEM_TILE_TX <= HEM_TILE_CLS &&
+               HEM_TILE_TX_TARGET <= HEM_TILE_CLS &&
+               HEM_TILE_CLS_TARGET <= HEM_TILE_CLS,
+               "tile sort buffers hold a tile");
 #define HEM_TILE_DESC HEMK_TILE_DESC
 
 typedef struct {

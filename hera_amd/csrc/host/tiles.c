@@ -15,7 +15,8 @@
  *   desc     {tx_base, n_tx, cls_base, n_cls, member base, members,
  *            class-list base, 0}
  *   txq      transcript id -> tile position             (all transcripts)
- *   tx       global transcript ids, ascending          (n_tx)
+ *   tx       global transcript ids, within a tile by descending
+ *            occurrence count (then ascending id)     (n_tx)
  *   len      their lengths                             (n_tx)
  *   coff     member offsets of the tile's classes      (n_cls + 1 shared)
  *   kmem     members in hera's order, each (local class << 8) | local
@@ -25,15 +26,38 @@
  *            occurrences, in ascending original class id and, within a
  *            class, member order -- the order hera's class loop adds into
  *            overlap[t] (src/EM.c:276-285)
- * Classes are renumbered in tile order ("tile position"); per-replicate
- * class counts are stored in that order so a tile reads one contiguous
- * block.  cls_orig maps a tile position back to the caller's class id.
+ * Classes are renumbered in tile order ("tile position"), within a tile by
+ * descending member count (then ascending id); per-replicate class counts
+ * are stored in that order so a tile reads one contiguous block.  cls_orig
+ * maps a tile position back to the caller's class id.  Local indices are
+ * labels only: every sum runs in the order its list gives (members in
+ * hera's order, each transcript's class list in ascending original class
+ * id).  Ordering by length puts entities of similar length in the same
+ * wave of the batched kernels, whose waves loop to their longest entity.
  * Classes without members never contribute (D_c = 0) and are dropped.
  */
 #include <stdlib.h>
 #include <string.h>
 
 #include "hem_internal.h"
+
+static int cmp_u64(const void *a, const void *b)
+{
+	const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+	return x < y ? -1 : x > y;
+}
+
+/* Orders ids[0..n) by descending len(id), then ascending id. */
+static void order_by_length(uint32_t *ids, uint32_t n, const uint32_t *off,
+                            uint64_t *key)
+{
+	for (uint32_t i = 0; i < n; ++i)
+		key[i] = ((uint64_t)(0xFFFFFFFFu - (off[ids[i] + 1] -
+		                                    off[ids[i]])) << 32) | ids[i];
+	qsort(key, n, sizeof(uint64_t), cmp_u64);
+	for (uint32_t i = 0; i < n; ++i)
+		ids[i] = (uint32_t)key[i];
+}
 
 static uint32_t uf_find(uint32_t *par, uint32_t x)
 {
@@ -234,19 +258,23 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 	for (i = 0; i < C; ++i)
 		if (cls_off[i + 1] > cls_off[i])
 			tile_ncls[ctile[comp[cls_ref[cls_off[i]]]]]++;
-	/* transcripts in tile order, ascending within a tile */
+	/* transcripts in tile order, by descending occurrence count within a
+	 * tile */
 	{
 		uint32_t acc = 0;
+		uint64_t key[HEM_TILE_CLS];
 		for (i = 0; i < ntile; ++i) {
 			pos[i] = acc;
 			acc += tile_ntx[i];
 		}
 		for (i = 0; i < T; ++i) {
 			uint32_t tl = ctile[comp[i]];
-			uint32_t p = pos[tl]++;
-			h->tx[p] = i;
-			h->len[p] = tx_len[i];
+			h->tx[pos[tl]++] = i;
 		}
+		for (i = 0, acc = 0; i < ntile; acc += tile_ntx[i++])
+			order_by_length(h->tx + acc, tile_ntx[i], tx_off, key);
+		for (i = 0; i < T; ++i)
+			h->len[i] = tx_len[h->tx[i]];
 	}
 	/* local index = position minus the tile's first position */
 	{
@@ -260,7 +288,7 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 			for (k = 0; k < tile_ntx[i]; ++k)
 				lidx[h->tx[h->desc[HEM_TILE_DESC * i] + k]] = k;
 	}
-	/* classes in tile order, ascending original id within a tile */
+	/* classes in tile order, by descending member count within a tile */
 	h->n_cls = 0;
 	for (i = 0; i < ntile; ++i)
 		h->n_cls += tile_ncls[i];
@@ -285,6 +313,10 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 			uint32_t tl = ctile[comp[cls_ref[cls_off[i]]]];
 			h->cls_orig[pos[tl]++] = i;
 		}
+		uint64_t key[HEM_TILE_CLS];
+		for (i = 0, acc = 0; i < ntile; acc += tile_ncls[i++])
+			order_by_length(h->cls_orig + acc, tile_ncls[i], cls_off,
+			                key);
 	}
 	/* member lists in hera's member order, each member as (local class
 	 * << 8) | local transcript; mloc

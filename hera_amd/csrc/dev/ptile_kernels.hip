@@ -165,18 +165,17 @@ __device__ __forceinline__ void pmap(const PLds &L, const PTile &tc,
 		}
 		for (; k < k1; ++k)
 			acc += lin[L.kmem[k] & 0x3Fu];
-		L.D[c] = acc;
+		// 1 in place of 0: densities are >= 0, so D_c == 0 means every
+		// member has lam == 0 and the term below is the exact 0 that the
+		// reference's skip of the class leaves (src/EM.c:279-280)
+		L.D[c] = acc == 0.0 ? 1.0 : acc;
 	}
 	__syncthreads();
-	// (lam_t n_c) / D_c for occurrence j, class c of transcript t;
-	// a class with D_c == 0 is skipped by the reference, which a zero term
-	// reproduces (sums start at +0 and every term is >= 0)
+	// (lam_t n_c) / D_c for occurrence j, class c of transcript t
 	for (uint32_t j = tid; j < tc.nz; j += PBLOCK) {
 		const uint32_t m = L.kocc[j];
 		const uint32_t c = m >> 6, t = m & 0x3Fu;
-		const double d = L.D[c];
-		const double x = lin[t] * L.n[c] / (d == 0.0 ? 1.0 : d);
-		L.X[j] = d == 0.0 ? 0.0 : x;
+		L.X[j] = lin[t] * L.n[c] / L.D[c];
 	}
 	__syncthreads();
 	for (uint32_t t = tid; t < tc.nt; t += PBLOCK) {

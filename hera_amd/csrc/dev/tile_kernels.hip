@@ -81,6 +81,11 @@ __device__ __forceinline__ void stage_struct(const hemk_tiles &tl,
 //                    occurrences in class order) / len_t
 // which is src/EM.c:275-291 with every sum in the reference's order.  With
 // stopf the stop test of src/EM.c:292-297 marks stopf[r].
+//
+// The class pass stores 1 in place of a D_c that is 0: densities are >= 0,
+// so D_c == 0 means every member, the transcript whose term it is
+// included, has lam == 0, and (0 n_c) / 1 adds the exact 0 that the
+// reference's skip of that class leaves (src/EM.c:279-280).
 template <typename TC, int RC>
 __device__ __forceinline__ void tile_map(
     const hemk_tiles &tl, const TileCtx &tc, const TileStruct &ts,
@@ -110,15 +115,16 @@ __device__ __forceinline__ void tile_map(
 			}
 			for (; k < k1; ++k)
 				acc += lin[(ts.kmem[k] & 0xFFu) * RC + r];
-			D[c * RC + r] = acc;
+			D[c * RC + r] = acc == 0.0 ? 1.0 : acc;
 		}
 	}
 	__syncthreads();
 	if (live) {
 		// each occurrence's term (lam_t n_c) / D_c computed where the
-		// transcript's ordered sum takes it (a class with D_c == 0 adds
-		// an exact 0, as the reference skips it); four occurrences' LDS
-		// reads issued together, the division on every lane
+		// transcript's ordered sum takes it; four occurrences at a time,
+		// their LDS reads and divisions straight-line (a slot past the
+		// list computes a term it adds as +0, so the four divisions
+		// overlap instead of waiting on branches)
 		for (uint32_t t = e0; t < tc.nt; t += ES) {
 			const double l = lin[t * RC + r];
 			uint32_t j = ts.toff[t];
@@ -126,22 +132,17 @@ __device__ __forceinline__ void tile_map(
 			double acc = 0.0;
 			for (; j < j1; j += 4) {
 				uint32_t c[4];
-				double dd[4], nn[4];
+				double x[4];
 #pragma unroll
 				for (int q = 0; q < 4; ++q)
 					c[q] = ts.tcls[j + q < j1 ? j + q : j];
 #pragma unroll
-				for (int q = 0; q < 4; ++q) {
-					dd[q] = D[c[q] * RC + r];
-					nn[q] = (double)n[c[q] * RC + r];
-				}
+				for (int q = 0; q < 4; ++q)
+					x[q] = l * (double)n[c[q] * RC + r] /
+					       D[c[q] * RC + r];
 #pragma unroll
-				for (int q = 0; q < 4; ++q) {
-					const double den = dd[q] == 0.0 ? 1.0 : dd[q];
-					const double x = l * nn[q] / den;
-					if (j + q < j1)
-						acc += dd[q] == 0.0 ? 0.0 : x;
-				}
+				for (int q = 0; q < 4; ++q)
+					acc += j + q < j1 ? x[q] : 0.0;
 			}
 			const double nw = ((double)u[t * RC + r] + acc) /
 			                  (double)ts.len[t];

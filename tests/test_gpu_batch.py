@@ -201,3 +201,36 @@ def test_batch_rejects_untileable_and_fp32():
         p.solve_batch(np.stack([u, u]), precision=H.PREC_FP32)
     assert ei.value.code == _lib.HEM_ERR_INVALID
     p.close()
+
+
+def zero_component_problem(seed):
+    """A generated table plus a component of four transcripts whose unique
+    and class counts are all 0: after the first map its densities are exact
+    zeros and its classes have D_c == 0, which the reference skips
+    (src/EM.c:279-280), beside components that carry reads."""
+    cls, u = tiny_problem(seed, n_tx=60, n_cls=240, dup=True)
+    T = cls.n_tx
+    members = [cls.cls_ref[cls.cls_off[i]:cls.cls_off[i + 1]].tolist()
+               for i in range(cls.n_cls)]
+    members += [[T, T + 1], [T + 1, T + 2, T + 3], [T + 3, T]]
+    counts = np.concatenate([cls.cls_count, [0.0, 0.0, 0.0]])
+    lens = np.concatenate([cls.tx_len, [300, 400, 500, 600]])
+    big = H.Classes.from_lists(T + 4, members, counts.tolist(), lens.tolist())
+    return big, np.concatenate([u, np.zeros(4, np.uint32)])
+
+
+def test_zero_density_classes_bitwise():
+    cls, u = zero_component_problem(4)
+    p = H.Problem(cls)
+    assert p.tiles()["n_tiles"] > 0
+    g = p.solve(u)
+    ora = O.estimate_count(cls, u)
+    assert g["stats"]["iters"] == ora["k"]
+    np.testing.assert_array_equal(g["lam"], ora["lam"])
+    assert (g["lam"][-4:] == 0).all()
+    U, CC = perturbed(cls, u, 5, 77)
+    CC[:, -3:] = 0
+    U[:, -4:] = 0
+    res = p.solve_batch(U, CC)
+    p.close()
+    check_replicates(cls, U, CC, res)

@@ -20,7 +20,7 @@
 #include "em_step.h"
 #include "hem_dev.h"
 
-#define PBLOCK 64
+#define PBLOCK 256
 
 namespace {
 

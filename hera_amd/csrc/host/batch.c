@@ -27,7 +27,9 @@ static double now_ms(void)
 
 /* ---- tiles on the device ------------------------------------------------ */
 
-int hem_tiles_upload(hem_problem *p, const hem_tiles_host *h)
+int hem_tiles_upload(hem_problem *p, const hem_tiles_host *h, hemk_tiles *dt,
+                     uint32_t **mem, uint32_t **cls_orig,
+                     uint64_t *rec_bytes)
 {
 	void *s = p->stream;
 	const size_t nz = h->nnz ? h->nnz : 1;
@@ -41,33 +43,34 @@ int hem_tiles_upload(hem_problem *p, const hem_tiles_host *h)
 	    h->desc, h->tx,   h->len,      h->coff, h->kmem,
 	    h->toff, h->tcls, h->cls_orig, h->txq,  h->rec};
 	for (int i = 0; i < HEM_TILE_ARRAYS; ++i) {
-		int e = hemd_malloc((void **)&p->d_tile_mem[i], sz[i]);
+		int e = hemd_malloc((void **)&mem[i], sz[i]);
 		if (e)
 			return hem_fail(HEM_ERR_NOMEM, "tile upload: %s",
 			                hemd_error_string(e));
 		p->dev_bytes += sz[i];
-		e = hemd_h2d(p->d_tile_mem[i], src[i], sz[i], s);
+		e = hemd_h2d(mem[i], src[i], sz[i], s);
 		if (e)
 			return hem_dev_fail(e, "tile upload");
 	}
-	p->dt.n_tiles = h->n_tiles;
-	p->dt.n_tx = h->n_tx;
-	p->dt.n_cls = h->n_cls;
-	p->dt.max_tx = h->max_tx;
-	p->dt.max_cls = h->max_cls;
-	p->dt.max_nnz = h->max_nnz;
-	p->dt.desc = p->d_tile_mem[0];
-	p->dt.tx = p->d_tile_mem[1];
-	p->dt.len = p->d_tile_mem[2];
-	p->dt.coff = p->d_tile_mem[3];
-	p->dt.kmem = p->d_tile_mem[4];
-	p->dt.toff = p->d_tile_mem[5];
-	p->dt.tcls = (const uint16_t *)p->d_tile_mem[6];
-	p->d_cls_orig = p->d_tile_mem[7];
-	p->dt.txq = p->d_tile_mem[8];
-	p->dt.rec = p->d_tile_mem[9];
-	p->dt.max_rec = h->max_rec;
-	p->tile_rec_bytes = h->rec_units * 16;
+	dt->n_tiles = h->n_tiles;
+	dt->n_tx = h->n_tx;
+	dt->n_cls = h->n_cls;
+	dt->max_tx = h->max_tx;
+	dt->max_cls = h->max_cls;
+	dt->max_nnz = h->max_nnz;
+	dt->desc = mem[0];
+	dt->tx = mem[1];
+	dt->len = mem[2];
+	dt->coff = mem[3];
+	dt->kmem = mem[4];
+	dt->toff = mem[5];
+	dt->tcls = (const uint16_t *)mem[6];
+	*cls_orig = mem[7];
+	dt->txq = mem[8];
+	dt->rec = mem[9];
+	dt->max_rec = h->max_rec;
+	if (rec_bytes)
+		*rec_bytes = h->rec_units * 16;
 	return HEM_OK;
 }
 

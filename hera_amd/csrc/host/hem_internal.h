@@ -103,12 +103,18 @@ struct hem_problem {
 	hemk_ctl *d_ctl;
 	hemk_ctl *h_ctl; /* pinned */
 
-	/* component tiles (n_tiles == 0: some component is too large) */
+	/* component tiles (n_tiles == 0: some component is too large) for
+	 * the batched and standard-error kernels */
 	hemk_tiles dt;
 	uint32_t *h_cls_orig;      /* tile position -> class id (host)      */
 	uint32_t *d_cls_orig;      /* same, device                          */
 	uint32_t *d_tile_mem[HEM_TILE_ARRAYS]; /* device arrays behind dt   */
-	uint64_t tile_rec_bytes;   /* all tile records (algorithmic bytes)  */
+	/* the single-problem tile kernels' tiling (larger tiles; one-sample
+	 * handles only; n_tiles == 0 otherwise) */
+	hemk_tiles dtp;
+	uint32_t *d_cls_orig_p;    /* its tile position -> class id         */
+	uint32_t *d_ptile_mem[HEM_TILE_ARRAYS];
+	uint64_t ptile_rec_bytes;  /* its tile records (algorithmic bytes)  */
 	int counts_integral;       /* class counts are integers < 2^32      */
 	double *h_cls_count;       /* host copy of the class counts         */
 	hem_batch_ws bw;
@@ -140,16 +146,25 @@ struct hem_problem {
 #define HEM_TILE_TX  64
 #define HEM_TILE_CLS 320
 /* Components are packed into tiles up to these sizes (a larger component,
- * within the limits above, gets a tile of its own): smaller tiles need less
- * LDS per workgroup, so more workgroups share a CU. */
+ * within the limits above, gets a tile of its own).  The batched and
+ * standard-error kernels hold a chunk of replicates per tile in LDS, so
+ * their tiles are small and many workgroups share a CU; the single-problem
+ * tile kernels (256 threads per tile) run fastest on tiles about four times
+ * larger, whose ordered-sum loops are shared by more transcripts and
+ * classes. */
 #define HEM_TILE_TX_TARGET  16
 #define HEM_TILE_CLS_TARGET 80
 #define HEM_TILE_NNZ_TARGET 384
+#define HEM_PTILE_TX_TARGET  64
+#define HEM_PTILE_CLS_TARGET 320
+#define HEM_PTILE_NNZ_TARGET 1024
 /* a tile holds at most HEM_TILE_CLS transcripts or classes (tiles.c sorts
  * them in buffers of that size) */
 _Static_assert(HEM_TILE_TX <= HEM_TILE_CLS &&
                HEM_TILE_TX_TARGET <= HEM_TILE_CLS &&
-               HEM_TILE_CLS_TARGET <= HEM_TILE_CLS,
+               HEM_TILE_CLS_TARGET <= HEM_TILE_CLS &&
+               HEM_PTILE_TX_TARGET <= HEM_TILE_CLS &&
+               HEM_PTILE_CLS_TARGET <= HEM_TILE_CLS,
                "tile sort buffers hold a tile");
 #define HEM_TILE_DESC HEMK_TILE_DESC
 
@@ -166,11 +181,13 @@ typedef struct {
 } hem_tiles_host;
 
 /* tx0[0..ns]: first transcript of each sample (tx0[ns] = T), or NULL with
- * ns = 1. */
+ * ns = 1.  Components are packed up to the given targets (HEM_TILE_* or
+ * HEM_PTILE_*). */
 int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
                     const uint32_t *cls_ref, const uint32_t *tx_len,
                     const uint32_t *tx_off, const uint32_t *tx0,
-                    uint32_t ns, hem_tiles_host *h);
+                    uint32_t ns, uint32_t tx_target, uint32_t cls_target,
+                    uint32_t nnz_target, hem_tiles_host *h);
 void hem_tiles_free_host(hem_tiles_host *h);
 /* Bytes of a tile's record (see tiles.c), a multiple of 16. */
 static inline uint64_t hem_tile_rec_bytes(uint32_t nt, uint32_t nc,
@@ -193,7 +210,11 @@ void hem_batch_free(hem_problem *p);
 void hem_batch_destroy_retired(hem_problem *p);
 /* grad.c */
 void hem_grad_free(hem_problem *p);
-int hem_tiles_upload(hem_problem *p, const hem_tiles_host *h);
+/* Uploads a tiling into dt (device arrays in mem[], its class map in
+ * *cls_orig, the records' bytes in *rec_bytes when given). */
+int hem_tiles_upload(hem_problem *p, const hem_tiles_host *h, hemk_tiles *dt,
+                     uint32_t **mem, uint32_t **cls_orig,
+                     uint64_t *rec_bytes);
 
 /* error.c */
 int hem_fail(int code, const char *fmt, ...);

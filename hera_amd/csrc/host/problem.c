@@ -122,6 +122,27 @@ static int samples_upload(hem_problem *p, const hem_tiles_host *th,
 /* Device problem for a class table holding ns samples side by side
  * (tx0/cls0: first transcript / class of each sample, ns + 1 entries each;
  * NULL with ns == 1). */
+/* The single-problem tile kernels' own tiling: the same components packed
+ * into larger tiles, and room for the class counts in its tile order
+ * (gathered at each solve). */
+static int plain_tiles(hem_problem *p, const hem_classes *c,
+                       const uint32_t *tx_off)
+{
+	hem_tiles_host th;
+	int rc = hem_tiles_build(c->n_tx, c->n_cls, c->cls_off, c->cls_ref,
+	                         c->tx_len, tx_off, NULL, 1,
+	                         HEM_PTILE_TX_TARGET, HEM_PTILE_CLS_TARGET,
+	                         HEM_PTILE_NNZ_TARGET, &th);
+	if (rc == HEM_OK)
+		rc = hem_tiles_upload(p, &th, &p->dtp, p->d_ptile_mem,
+		                      &p->d_cls_orig_p, &p->ptile_rec_bytes);
+	if (rc == HEM_OK)
+		rc = dalloc(p, (void **)&p->d_cntq,
+		            (size_t)(th.n_cls ? th.n_cls : 1) * 8);
+	hem_tiles_free_host(&th);
+	return rc;
+}
+
 static int create_impl(const hem_classes *c, const uint32_t *tx0,
                        const uint32_t *cls0, uint32_t ns, hem_problem **out)
 {
@@ -337,15 +358,16 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 	if (!c->member_weight) {
 		hem_tiles_host th;
 		int trc = hem_tiles_build(T, C, c->cls_off, c->cls_ref,
-		                          c->tx_len, tx_off, tx0, ns, &th);
+		                          c->tx_len, tx_off, tx0, ns,
+		                          HEM_TILE_TX_TARGET, HEM_TILE_CLS_TARGET,
+		                          HEM_TILE_NNZ_TARGET, &th);
 		if (trc == HEM_OK) {
-			trc = hem_tiles_upload(p, &th);
-			if (trc == HEM_OK)
-				trc = dalloc(p, (void **)&p->d_cntq,
-				             (size_t)(th.n_cls ? th.n_cls : 1) *
-				                 8);
+			trc = hem_tiles_upload(p, &th, &p->dt, p->d_tile_mem,
+			                       &p->d_cls_orig, NULL);
 			if (trc == HEM_OK && ns > 1)
 				trc = samples_upload(p, &th, tx0, cls0);
+			if (trc == HEM_OK && ns == 1)
+				trc = plain_tiles(p, c, tx_off);
 			p->h_cls_orig = th.cls_orig;
 			th.cls_orig = NULL;
 			hem_tiles_free_host(&th);
@@ -524,6 +546,8 @@ int hem_problem_destroy(hem_problem *p)
 	hemd_free(p->d_sres);
 	hemd_free(p->d_cls_w);
 	hemd_free(p->d_tx_w);
+	for (int i = 0; i < HEM_TILE_ARRAYS; ++i)
+		hemd_free(p->d_ptile_mem[i]);
 	hemd_free(p->d_ctl);
 	hemd_host_free(p->h_ctl);
 	if (p->ev0)

@@ -118,9 +118,9 @@ static hemk_spec spec_plain(const hem_problem *p, uint32_t nchain, int xf,
  * tile (one kernel for u -> u1 -> u2, one for the extrapolated map). */
 static int use_tiles(const hem_problem *p, const hem_opts *o)
 {
-	return p->dt.n_tiles && o->precision == HEM_PREC_FP64 &&
+	return p->dtp.n_tiles && o->precision == HEM_PREC_FP64 &&
 	       (o->norm == HEM_NORM_EXACT || o->norm == HEM_NORM_TREE) &&
-	       hemk_ptile_lds(&p->dt) <= HEM_PTILE_LDS_MAX;
+	       hemk_ptile_lds(&p->dtp) <= HEM_PTILE_LDS_MAX;
 }
 
 /* One SQUAREM iteration over the tiles. */
@@ -134,7 +134,7 @@ static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
 	const double *alpha_part = NULL;
 	void *s = p->stream;
 
-	HEM_TRY_DEV(hemk_ptile_map12(&p->dt, p->d_A, p->d_B, p->d_C, p->d_uq,
+	HEM_TRY_DEV(hemk_ptile_map12(&p->dtp, p->d_A, p->d_B, p->d_C, p->d_uq,
 	                             p->d_cntq, exact ? p->d_sx : NULL,
 	                             exact ? p->d_sx + T : NULL, p->d_ctl, s),
 	            "tile maps");
@@ -174,7 +174,7 @@ static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
 		            "alpha");
 		MARK(sd ? HEM_PK_NORM2 : HEM_PK_NORM1);
 	}
-	HEM_TRY_DEV(hemk_ptile_ex3(&p->dt, p->d_A, p->d_B, p->d_C, p->d_uq,
+	HEM_TRY_DEV(hemk_ptile_ex3(&p->dtp, p->d_A, p->d_B, p->d_C, p->d_uq,
 	                           p->d_cntq, alpha_part, T, st, p->d_ctl, s),
 	            "tile extrapolated map");
 	MARK(HEM_PK_TEX3);
@@ -368,10 +368,10 @@ static int solve_setup(hem_problem *p, const uint32_t *unique,
 	/* tile order: unique counts by tile position, class counts by class
 	 * tile position (the uniform start is the same in any order) */
 	if (!rc && use_tiles(p, o))
-		rc = hemk_gather_rows_u32(p->d_unique, p->dt.tx, p->n_tx, 1,
+		rc = hemk_gather_rows_u32(p->d_unique, p->dtp.tx, p->n_tx, 1,
 		                          p->d_uq, p->stream);
 	if (!rc && use_tiles(p, o))
-		rc = hemk_gather_f64(p->d_cnt64, p->d_cls_orig, p->dt.n_cls,
+		rc = hemk_gather_f64(p->d_cnt64, p->d_cls_orig_p, p->dtp.n_cls,
 		                     p->d_cntq, p->stream);
 	if (!rc)
 		rc = hemk_init_state(p->n_tx, p->d_A, p->d_A32,
@@ -430,7 +430,7 @@ int hem_solve(hem_problem *p, const uint32_t *unique, const hem_opts *opts,
 	 * solve) */
 	lam = p->d_A;
 	if (use_tiles(p, &o)) {
-		rc = hemk_ptile_rows(p->n_tx, p->dt.txq, p->d_A, p->d_D,
+		rc = hemk_ptile_rows(p->n_tx, p->dtp.txq, p->d_A, p->d_D,
 		                     p->stream);
 		lam = p->d_D;
 	}
@@ -525,8 +525,8 @@ static void kernel_bytes(const hem_problem *p, int f32, double *b)
 		/* tile kernels: each tile's descriptor and record (offsets,
 		 * members, occurrence positions, lengths, ids) once, its
 		 * unique and class counts, then the state */
-		const double NT = p->dt.n_tiles, TC = p->dt.n_cls;
-		const double S = 32 * NT + (double)p->tile_rec_bytes + 4 * T +
+		const double NT = p->dtp.n_tiles, TC = p->dtp.n_cls;
+		const double S = 32 * NT + (double)p->ptile_rec_bytes + 4 * T +
 		                 8 * TC;
 		b[HEM_PK_TMAP12] = S + 8 * T + 16 * T + 16 * T;
 		b[HEM_PK_TEX3] = S + 24 * T + 8 * T;

@@ -147,7 +147,8 @@ void hem_tiles_free_host(hem_tiles_host *h)
 int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
                     const uint32_t *cls_ref, const uint32_t *tx_len,
                     const uint32_t *tx_off, const uint32_t *tx0,
-                    uint32_t ns, hem_tiles_host *h)
+                    uint32_t ns, uint32_t tx_target, uint32_t cls_target,
+                    uint32_t nnz_target, hem_tiles_host *h)
 {
 	uint32_t *par = NULL, *comp = NULL, *ctx = NULL, *ccls = NULL;
 	uint32_t *ctile = NULL, *lidx = NULL, *pos = NULL;
@@ -226,9 +227,9 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 			if (ns > 1)
 				while (smp + 1 < ns && root[i] >= tx0[smp + 1])
 					++smp;
-			if (nt && (smp != cur || nt + ctx[i] > HEM_TILE_TX_TARGET ||
-			           nc + ccls[i] > HEM_TILE_CLS_TARGET ||
-			           nz + cnz[i] > HEM_TILE_NNZ_TARGET)) {
+			if (nt && (smp != cur || nt + ctx[i] > tx_target ||
+			           nc + ccls[i] > cls_target ||
+			           nz + cnz[i] > nnz_target)) {
 				ntile++;
 				nt = nc = nz = 0;
 			}
@@ -460,7 +461,8 @@ int hem_classes_tiling(const hem_classes *c, uint32_t *n_tiles,
 	for (i = 0; i < c->n_tx; ++i)
 		tx_off[i + 1] += tx_off[i];
 	rc = hem_tiles_build(c->n_tx, c->n_cls, c->cls_off, c->cls_ref,
-	                     c->tx_len, tx_off, NULL, 1, &h);
+	                     c->tx_len, tx_off, NULL, 1, HEM_TILE_TX_TARGET,
+	                     HEM_TILE_CLS_TARGET, HEM_TILE_NNZ_TARGET, &h);
 	free(tx_off);
 	if (rc == HEM_ERR_RANGE) {
 		*n_tiles = 0;

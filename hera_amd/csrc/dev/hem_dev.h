@@ -361,12 +361,28 @@ typedef struct {
 	size_t ld;
 	const uint32_t *alist;
 	const double *mres;    /* CENTRED: pass-1 sums by row                 */
+	double *csq;           /* IDENT, chain slot = row: when set, the chunk
+	                          sums step also writes each chunk's sum of
+	                          x * x here ([row][P])                    */
+	const double *gsum, *gsq; /* CENTRED, chain slot = row: pass-1 chunk
+	                          sums of x and x * x ([row][P]); when set,
+	                          the drifts take each chunk's sum of
+	                          (x - m)^2 as gsq - 2 m gsum + L m^2 (it only
+	                          steers the guesses) and the chunk sums step
+	                          is not needed                             */
 	double *csum, *drift, *tab;
 	long long *base;
 	uint8_t *nx;           /* [nchain][P][W] links to the next window     */
 	double *res;           /* sums by row                                 */
 	uint32_t *miss;
 } hemk_spec;
+/* The steps of hemk_spec_sum by themselves, in this order: chunk sums,
+ * drifts, then windows and resolve (produce == 0). */
+#define HEMK_SPEC_PH_CSUM   1
+#define HEMK_SPEC_PH_DRIFT  2
+#define HEMK_SPEC_PH_FINISH 4
+int hemk_spec_phases(const hemk_spec *sp, int phases, hemk_ctl *ctl,
+                     void *s);
 /* Chunk length for chains of n values, and the window width for a
  * transform at that length. */
 uint32_t hemk_spec_chunk(uint32_t n);

@@ -287,12 +287,19 @@ __global__ __launch_bounds__(SBLOCK) void k_spec_csum(hemk_spec sp,
 	const uint32_t end = min(sp.n, beg + sp.L);
 	const double m = chain_mean<XF>(sp, row);
 	const double *x = sp.x + (size_t)row * sp.ld;
-	double a = 0.0;
-	for (uint32_t i = beg + threadIdx.x; i < end; i += SBLOCK)
+	double a = 0.0, b = 0.0;
+	for (uint32_t i = beg + threadIdx.x; i < end; i += SBLOCK) {
 		a += xval<XF>(x[i], m);
+		b += x[i] * x[i];
+	}
 	a = block_sum(a);
-	if (threadIdx.x == 0)
+	if (XF == HEMK_SPEC_IDENT && sp.csq)
+		b = block_sum(b);
+	if (threadIdx.x == 0) {
 		sp.csum[(size_t)blockIdx.y * sp.P + j] = a;
+		if (XF == HEMK_SPEC_IDENT && sp.csq)
+			sp.csq[(size_t)blockIdx.y * sp.P + j] = b;
+	}
 }
 
 // Plain solve, pass 1: r = u1 - u and v = (u2 - u1) - r (src/EM.c:333-336)
@@ -339,11 +346,19 @@ __global__ __launch_bounds__(SBLOCK) void k_spec_drift(hemk_spec sp,
 	const uint32_t j = blockIdx.x;
 	const size_t cb = (size_t)blockIdx.y * sp.P;
 	const int tid = threadIdx.x, lane = tid % WAVE, w = tid / WAVE;
-	double g = 0.0;
-	for (uint32_t i = tid; i < j; i += SBLOCK)
-		g += sp.csum[cb + i];
-	const double g0 = block_sum(g);
 	const double m = chain_mean<XF>(sp, row);
+	double g = 0.0;
+	if (XF == HEMK_SPEC_CENTRED && sp.gsq) {
+		// chunks before j are full: sum of (x - m)^2 over one is
+		// sq - 2 m s + L m^2
+		const double lm2 = (double)sp.L * m * m;
+		for (uint32_t i = tid; i < j; i += SBLOCK)
+			g += (sp.gsq[cb + i] - 2.0 * m * sp.gsum[cb + i]) + lm2;
+	} else {
+		for (uint32_t i = tid; i < j; i += SBLOCK)
+			g += sp.csum[cb + i];
+	}
+	const double g0 = block_sum(g);
 	const uint32_t E = sp.L / SBLOCK;   // 1..8
 	const uint32_t beg = j * sp.L + tid * E;
 	const double *x = sp.x + (size_t)row * sp.ld;
@@ -722,7 +737,7 @@ __global__ __launch_bounds__(SBLOCK) void k_spec_resolve(hemk_spec sp,
 }
 
 template <int XF>
-int launch_xf(const hemk_spec &sp, int produce, const double *u,
+int launch_xf(const hemk_spec &sp, int produce, int phases, const double *u,
               const double *u1, const double *u2, const hemk_ctl *ctl,
               hipStream_t st)
 {
@@ -730,15 +745,19 @@ int launch_xf(const hemk_spec &sp, int produce, const double *u,
 	if (produce)
 		k_spec_rv<XF><<<grid, SBLOCK, 0, st>>>(sp, u, u1, u2,
 		                                       (double *)sp.x, ctl);
-	else
+	else if (phases & HEMK_SPEC_PH_CSUM)
 		k_spec_csum<XF><<<grid, SBLOCK, 0, st>>>(sp, ctl);
-	if (XF == HEMK_SPEC_IDENT)
-		k_spec_seqdrift<XF>
-		    <<<dim3((sp.P + SBLOCK / WAVE - 1) / (SBLOCK / WAVE),
-		            sp.nchain),
-		       SBLOCK, 0, st>>>(sp, ctl);
-	else
-		k_spec_drift<XF><<<grid, SBLOCK, 0, st>>>(sp, ctl);
+	if (phases & HEMK_SPEC_PH_DRIFT) {
+		if (XF == HEMK_SPEC_IDENT)
+			k_spec_seqdrift<XF>
+			    <<<dim3((sp.P + SBLOCK / WAVE - 1) / (SBLOCK / WAVE),
+			            sp.nchain),
+			       SBLOCK, 0, st>>>(sp, ctl);
+		else
+			k_spec_drift<XF><<<grid, SBLOCK, 0, st>>>(sp, ctl);
+	}
+	if (!(phases & HEMK_SPEC_PH_FINISH))
+		return (int)hipGetLastError();
 #define SPEC_W_CASE(WW)                                                    \
 	case WW:                                                           \
 		k_spec_window<WW, XF>                                      \
@@ -776,11 +795,12 @@ uint32_t hemk_spec_width(uint32_t L, int xf)
 	return xf == HEMK_SPEC_IDENT ? 64 : 16;
 }
 
-int hemk_spec_sum(const hemk_spec *sp, int produce, const double *u,
-                  const double *u1, const double *u2, hemk_ctl *ctl,
-                  void *s)
+static int spec_launch(const hemk_spec *sp, int produce, int phases,
+                       const double *u, const double *u1, const double *u2,
+                       hemk_ctl *ctl, void *s)
 {
 	hipStream_t st = (hipStream_t)s;
+	const bool by_row = !sp->pairs && !sp->alist;
 	if (sp->n == 0 || sp->nchain == 0 || sp->L == 0 ||
 	    sp->L > HEMK_SPEC_LMAX || sp->L % SBLOCK != 0 ||
 	    sp->n > 0xFFFFFFFFu - sp->L ||
@@ -788,20 +808,38 @@ int hemk_spec_sum(const hemk_spec *sp, int produce, const double *u,
 	    sp->W < 8 || sp->W > HEMK_SPEC_W || sp->W % 8 != 0 ||
 	    !sp->nx || !sp->tab || !sp->base ||
 	    (produce && (sp->nchain != 2 || sp->pairs || sp->alist ||
-	                 sp->xf == HEMK_SPEC_CENTRED)))
+	                 sp->xf == HEMK_SPEC_CENTRED)) ||
+	    (sp->csq && (sp->xf != HEMK_SPEC_IDENT || !by_row || produce)) ||
+	    (!sp->gsq != !sp->gsum) ||
+	    (sp->gsq && (sp->xf != HEMK_SPEC_CENTRED || !by_row)))
 		return (int)hipErrorInvalidValue;
 	switch (sp->xf) {
 	case HEMK_SPEC_IDENT:
-		return launch_xf<HEMK_SPEC_IDENT>(*sp, produce, u, u1, u2, ctl,
-		                                  st);
+		return launch_xf<HEMK_SPEC_IDENT>(*sp, produce, phases, u, u1,
+		                                  u2, ctl, st);
 	case HEMK_SPEC_CENTRED:
-		return launch_xf<HEMK_SPEC_CENTRED>(*sp, produce, u, u1, u2, ctl,
-		                                    st);
+		return launch_xf<HEMK_SPEC_CENTRED>(*sp, produce, phases, u, u1,
+		                                    u2, ctl, st);
 	case HEMK_SPEC_SQUARE:
-		return launch_xf<HEMK_SPEC_SQUARE>(*sp, produce, u, u1, u2, ctl,
-		                                   st);
+		return launch_xf<HEMK_SPEC_SQUARE>(*sp, produce, phases, u, u1,
+		                                   u2, ctl, st);
 	}
 	return (int)hipErrorInvalidValue;
+}
+
+int hemk_spec_sum(const hemk_spec *sp, int produce, const double *u,
+                  const double *u1, const double *u2, hemk_ctl *ctl,
+                  void *s)
+{
+	return spec_launch(sp, produce,
+	                   HEMK_SPEC_PH_CSUM | HEMK_SPEC_PH_DRIFT |
+	                       HEMK_SPEC_PH_FINISH,
+	                   u, u1, u2, ctl, s);
+}
+
+int hemk_spec_phases(const hemk_spec *sp, int phases, hemk_ctl *ctl, void *s)
+{
+	return spec_launch(sp, 0, phases, NULL, NULL, NULL, ctl, s);
 }
 
 } // extern "C"

@@ -96,6 +96,7 @@ struct hem_problem {
 	 * (two chains), and the sums (pass 1 at [0..1], pass 2 at [2..3]) */
 	uint32_t spec_L, spec_P;
 	double *d_sx, *d_scsum, *d_sdrift, *d_stab, *d_sres;
+	double *d_scsq;             /* pass-1 chunk sums of x * x         */
 	long long *d_sbase;
 	uint8_t *d_snx;
 	double *d_cls_w, *d_tx_w;  /* member weights (weighted tables)   */

@@ -281,6 +281,7 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 	DALLOC_H(p->d_sx, 2 * (size_t)T * 8);
 	DALLOC_H(p->d_scsum, 2 * (size_t)p->spec_P * 8);
 	DALLOC_H(p->d_sdrift, 2 * (size_t)p->spec_P * 8);
+	DALLOC_H(p->d_scsq, 2 * (size_t)p->spec_P * 8);
 	DALLOC_H(p->d_sbase, 2 * (size_t)p->spec_P * 8);
 	DALLOC_H(p->d_stab, 2 * (size_t)p->spec_P * HEMK_SPEC_W * 8);
 	DALLOC_H(p->d_snx, 2 * (size_t)p->spec_P * HEMK_SPEC_W);
@@ -540,6 +541,7 @@ int hem_problem_destroy(hem_problem *p)
 	hemd_free(p->d_sx);
 	hemd_free(p->d_scsum);
 	hemd_free(p->d_sdrift);
+	hemd_free(p->d_scsq);
 	hemd_free(p->d_sbase);
 	hemd_free(p->d_stab);
 	hemd_free(p->d_snx);

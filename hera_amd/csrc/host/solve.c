@@ -140,10 +140,15 @@ static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
 	            "tile maps");
 	MARK(HEM_PK_TMAP12);
 	if (exact) {
-		/* the reference's left-to-right sums over the r and v rows */
+		/* the reference's left-to-right sums over the r and v rows;
+		 * with the sd form pass 1 also keeps each chunk's sum of
+		 * squares, from which pass 2 (centred sums of squares) seeds
+		 * its guesses without a chunk-sums step of its own */
 		hemk_spec sp = spec_plain(p, 2,
 		                          sd ? HEMK_SPEC_IDENT : HEMK_SPEC_SQUARE,
 		                          p->d_sx, NULL, p->d_sres);
+		if (sd)
+			sp.csq = p->d_scsq;
 		HEM_TRY_DEV(hemk_spec_sum(&sp, 0, NULL, NULL, NULL, p->d_ctl, s),
 		            "exact sums 1");
 		MARK(HEM_PK_NORM1);
@@ -151,8 +156,12 @@ static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
 		if (sd) {
 			sp = spec_plain(p, 2, HEMK_SPEC_CENTRED, p->d_sx,
 			                p->d_sres, p->d_sres + 2);
-			HEM_TRY_DEV(hemk_spec_sum(&sp, 0, NULL, NULL, NULL,
-			                          p->d_ctl, s),
+			sp.gsum = p->d_scsum;
+			sp.gsq = p->d_scsq;
+			HEM_TRY_DEV(hemk_spec_phases(&sp,
+			                             HEMK_SPEC_PH_DRIFT |
+			                                 HEMK_SPEC_PH_FINISH,
+			                             p->d_ctl, s),
 			            "exact sums 2");
 			MARK(HEM_PK_NORM2);
 			alpha_part = p->d_sres + 2;

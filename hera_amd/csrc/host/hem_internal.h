@@ -153,9 +153,9 @@ struct hem_problem {
  * tile kernels (256 threads per tile) run fastest on tiles about four times
  * larger, whose ordered-sum loops are shared by more transcripts and
  * classes. */
-#define HEM_TILE_TX_TARGET  16
-#define HEM_TILE_CLS_TARGET 80
-#define HEM_TILE_NNZ_TARGET 384
+#define HEM_TILE_TX_TARGET  24
+#define HEM_TILE_CLS_TARGET 120
+#define HEM_TILE_NNZ_TARGET 576
 #define HEM_PTILE_TX_TARGET  64
 #define HEM_PTILE_CLS_TARGET 320
 #define HEM_PTILE_NNZ_TARGET 1024

@@ -234,3 +234,42 @@ def test_zero_density_classes_bitwise():
     res = p.solve_batch(U, CC)
     p.close()
     check_replicates(cls, U, CC, res)
+
+
+def limit_component_problem(seed):
+    """A generated table plus one component at the tile limits: 64
+    transcripts and 320 classes (a chain through all 64 transcripts, then
+    random classes inside the component), so one tile of each tiling holds
+    it alone and sets the kernels' LDS allocation."""
+    cls, u = H.generate(400, 4000, 200_000, seed=seed)
+    rng = np.random.default_rng(seed)
+    T = cls.n_tx
+    members = [cls.cls_ref[cls.cls_off[i]:cls.cls_off[i + 1]].tolist()
+               for i in range(cls.n_cls)]
+    comp = [[T + i, T + i + 1] for i in range(63)]
+    while len(comp) < 320:
+        k = int(rng.integers(2, 9))
+        comp.append((T + rng.choice(64, size=k, replace=False)).tolist())
+    members += comp
+    counts = np.concatenate([cls.cls_count,
+                             rng.integers(1, 400, size=320).astype(float)])
+    lens = np.concatenate([cls.tx_len, rng.integers(200, 5000, size=64)])
+    u2 = np.concatenate([u, rng.integers(0, 300, size=64).astype(np.uint32)])
+    return H.Classes.from_lists(T + 64, members, counts.tolist(),
+                                lens.tolist()), u2
+
+
+def test_tile_limit_component_bitwise():
+    cls, u = limit_component_problem(6)
+    t = cls.tiling()
+    assert t["n_tiles"] > 0 and t["max_tx"] == 64 and t["max_cls"] == 320
+    p = H.Problem(cls)
+    g = p.solve(u)
+    ora = O.estimate_count(cls, u)
+    assert g["stats"]["iters"] == ora["k"]
+    np.testing.assert_array_equal(g["lam"], ora["lam"])
+    U, CC = perturbed(cls, u, 3, 91)
+    res = p.solve_batch(U, CC)
+    p.close()
+    say("limit tile: chunk %d" % res["stats"]["chunk"])
+    check_replicates(cls, U, CC, res)

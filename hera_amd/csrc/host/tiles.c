@@ -449,6 +449,8 @@ int hem_classes_tiling(const hem_classes *c, uint32_t *n_tiles,
 	int rc;
 	if (!c || !c->tx_len || !c->n_tx || (c->n_cls && !c->cls_off))
 		return hem_fail(HEM_ERR_INVALID, "null class table");
+	if (!n_tiles || !max_tx || !max_cls)
+		return hem_fail(HEM_ERR_INVALID, "null output");
 	nnz = c->n_cls ? c->cls_off[c->n_cls] : 0;
 	for (uint64_t k = 0; k < nnz; ++k)
 		if (c->cls_ref[k] >= c->n_tx)

@@ -206,13 +206,19 @@ typedef struct {
 	                                   sums added again (all replicates) */
 } hem_batch_stats;
 
-/* Tiling of the class graph: number of tiles (0 when a component exceeds
- * the tile limits), largest tile in transcripts and classes. */
+/* Tiling of the class graph used by the batched and standard-error kernels
+ * (connected components packed into tiles of about 24 transcripts):
+ * number of tiles, largest tile in transcripts and classes.  n_tiles is 0
+ * when a component exceeds the tile limits (64 transcripts or 320
+ * classes) and for tables with member weights; batches then refuse the
+ * table and hem_solve uses its two global passes.  (hem_solve's tile
+ * kernels use a second packing of the same components into larger
+ * tiles.)  Any output pointer may be NULL. */
 int hem_problem_tiles(const hem_problem *p, uint32_t *n_tiles,
                       uint32_t *max_tx, uint32_t *max_cls);
-/* The same for a class table on the host, without a device: also the
- * number of connected components and the largest one.  Returns HEM_OK
- * with *n_tiles = 0 when a component exceeds the tile limits. */
+/* The same tiling for a class table on the host, without a device (all
+ * three outputs required).  Returns HEM_OK with *n_tiles = 0 when a
+ * component exceeds the tile limits. */
 int hem_classes_tiling(const hem_classes *cls, uint32_t *n_tiles,
                        uint32_t *max_tx, uint32_t *max_cls);
 

@@ -107,6 +107,7 @@ typedef struct {
 	const uint32_t *rec;    /* per-tile records (host/tiles.c), 16-byte
 	                           units; tile i's at desc[8i + 7]          */
 	uint32_t max_rec;       /* bytes of the largest record               */
+	uint32_t weighted;      /* records carry member weights             */
 } hemk_tiles;
 
 /* Per-instance control of a batched solve.  A handle holds ns samples

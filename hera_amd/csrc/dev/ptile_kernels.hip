@@ -43,14 +43,19 @@ __device__ __forceinline__ PTile ptile(const hemk_tiles &t, uint32_t i)
 
 // Bytes of a tile's record: u16 class offsets (nc + 1), class-list offsets
 // (nt + 1), members (c << 6 | t) in class order and in class-list order
-// (nz each), then
-// 4-byte aligned u32 lengths and transcript ids (nt each); 16-byte units.
+// (nz each), then 4-byte aligned u32 lengths and transcript ids (nt each),
+// then for a weighted table, 8-byte aligned, FP64 member weights in class
+// order and in class-list order (nz each); 16-byte units.
+template <bool W>
 __device__ __forceinline__ uint32_t rec_bytes(uint32_t nt, uint32_t nc,
                                               uint32_t nz)
 {
 	uint32_t u16s = (nc + 1) + (nt + 1) + 2 * nz;
 	u16s += u16s & 1;
-	return (2 * u16s + 8 * nt + 15) / 16 * 16;
+	uint32_t b = 2 * u16s + 8 * nt;
+	if (W)
+		b = (b + 7) / 8 * 8 + 16 * nz;
+	return (b + 15) / 16 * 16;
 }
 
 // The tile in LDS: its record (room for the largest), then the FP64 work
@@ -63,6 +68,7 @@ struct PLds {
 	// record fields of the current tile
 	const uint16_t *coff, *toff, *kmem, *kocc;
 	const uint32_t *len, *tx;
+	const double *wc, *wo;  // weighted tables: weights by member, by entry
 };
 
 __device__ __forceinline__ PLds pcarve(void *smem, const hemk_tiles &tl)
@@ -80,6 +86,7 @@ __device__ __forceinline__ PLds pcarve(void *smem, const hemk_tiles &tl)
 	return L;
 }
 
+template <bool W>
 __device__ __forceinline__ void pfields(PLds &L, const PTile &tc)
 {
 	const uint16_t *r = (const uint16_t *)L.rec;
@@ -91,6 +98,12 @@ __device__ __forceinline__ void pfields(PLds &L, const PTile &tc)
 	k += k & 1;
 	L.len = (const uint32_t *)(r + k);
 	L.tx = L.len + tc.nt;
+	L.wc = L.wo = nullptr;
+	if (W) {
+		const uint32_t b = (2 * k + 8 * tc.nt + 7) / 8 * 8;
+		L.wc = (const double *)((const char *)r + b);
+		L.wo = L.wc + tc.nz;
+	}
 }
 
 // The tile's record, class counts and unique counts into LDS.  The loads
@@ -98,13 +111,14 @@ __device__ __forceinline__ void pfields(PLds &L, const PTile &tc)
 // are all issued before the first LDS store, so the tile arrives in one
 // round of memory latency; larger tiles finish in plain loops.
 #define PK_R 4   // 16-byte record pieces per lane in the first round
+template <bool W>
 __device__ __forceinline__ void pstage(const hemk_tiles &tl, const PTile &tc,
                                        const PLds &L,
                                        const uint32_t *__restrict__ uq,
                                        const double *__restrict__ cntq)
 {
 	const uint32_t tid = threadIdx.x;
-	const uint32_t n16 = rec_bytes(tc.nt, tc.nc, tc.nz) / 16;
+	const uint32_t n16 = rec_bytes<W>(tc.nt, tc.nc, tc.nz) / 16;
 	const uint4 *src = (const uint4 *)tl.rec + tc.rec;
 	uint4 rv[PK_R];
 	double nn[2];
@@ -141,7 +155,10 @@ __device__ __forceinline__ void pstage(const hemk_tiles &tl, const PTile &tc,
 // One map lin -> lout of the tile (src/EM.c:275-291) in three passes;
 // with stop the test of src/EM.c:292-297 against lin sets bad.  The term
 // pass writes each transcript's terms contiguously in the order its sum
-// takes them, so the ordered sums read LDS without an indirection.
+// takes them, so the ordered sums read LDS without an indirection.  With
+// member weights (W) a member's share is lam_t w_k: D_c sums lam_t w_k and
+// the term is ((lam_t w_k) n_c) / D_c.
+template <bool W>
 __device__ __forceinline__ void pmap(const PLds &L, const PTile &tc,
                                      const double *lin, double *lout,
                                      bool stop, double smin, double srel,
@@ -154,20 +171,31 @@ __device__ __forceinline__ void pmap(const PLds &L, const PTile &tc,
 		const uint32_t k1 = L.coff[c + 1];
 		double acc = 0.0;
 		for (; k + 4 <= k1; k += 4) {
-			const double x0 = lin[L.kmem[k] & 0x3Fu];
-			const double x1 = lin[L.kmem[k + 1] & 0x3Fu];
-			const double x2 = lin[L.kmem[k + 2] & 0x3Fu];
-			const double x3 = lin[L.kmem[k + 3] & 0x3Fu];
+			double x0 = lin[L.kmem[k] & 0x3Fu];
+			double x1 = lin[L.kmem[k + 1] & 0x3Fu];
+			double x2 = lin[L.kmem[k + 2] & 0x3Fu];
+			double x3 = lin[L.kmem[k + 3] & 0x3Fu];
+			if constexpr (W) {
+				x0 *= L.wc[k];
+				x1 *= L.wc[k + 1];
+				x2 *= L.wc[k + 2];
+				x3 *= L.wc[k + 3];
+			}
 			acc += x0;
 			acc += x1;
 			acc += x2;
 			acc += x3;
 		}
-		for (; k < k1; ++k)
-			acc += lin[L.kmem[k] & 0x3Fu];
-		// 1 in place of 0: densities are >= 0, so D_c == 0 means every
-		// member has lam == 0 and the term below is the exact 0 that the
-		// reference's skip of the class leaves (src/EM.c:279-280)
+		for (; k < k1; ++k) {
+			double x = lin[L.kmem[k] & 0x3Fu];
+			if constexpr (W)
+				x *= L.wc[k];
+			acc += x;
+		}
+		// 1 in place of 0: densities (and weights) are >= 0, so D_c == 0
+		// means every member's lam (times its weight) is 0 and the term
+		// below is the exact 0 that the reference's skip of the class
+		// leaves (src/EM.c:279-280)
 		L.D[c] = acc == 0.0 ? 1.0 : acc;
 	}
 	__syncthreads();
@@ -175,7 +203,10 @@ __device__ __forceinline__ void pmap(const PLds &L, const PTile &tc,
 	for (uint32_t j = tid; j < tc.nz; j += PBLOCK) {
 		const uint32_t m = L.kocc[j];
 		const uint32_t c = m >> 6, t = m & 0x3Fu;
-		L.X[j] = lin[t] * L.n[c] / L.D[c];
+		double lt = lin[t];
+		if constexpr (W)
+			lt *= L.wo[j];
+		L.X[j] = lt * L.n[c] / L.D[c];
 	}
 	__syncthreads();
 	for (uint32_t t = tid; t < tc.nt; t += PBLOCK) {
@@ -203,6 +234,7 @@ __device__ __forceinline__ void pmap(const PLds &L, const PTile &tc,
 
 // u1 = M(A) -> B, u2 = M(u1) -> C; r = u1 - u, v = (u2 - u1) - r into the
 // rows xr, xv (transcript order) when xr != NULL.
+template <bool W>
 __global__ __launch_bounds__(PBLOCK) void k_ptile_map12(
     hemk_tiles tl, const double *__restrict__ A, double *__restrict__ B,
     double *__restrict__ C, const uint32_t *__restrict__ uq,
@@ -215,14 +247,14 @@ __global__ __launch_bounds__(PBLOCK) void k_ptile_map12(
 		return;
 	const PTile tc = ptile(tl, blockIdx.x);
 	PLds L = pcarve(smem, tl);
-	pfields(L, tc);
-	pstage(tl, tc, L, uq, cntq);
+	pfields<W>(L, tc);
+	pstage<W>(tl, tc, L, uq, cntq);
 	for (uint32_t i = threadIdx.x; i < tc.nt; i += PBLOCK)
 		L.l0[i] = A[tc.tx_base + i];
 	__syncthreads();
 	int bad = 0;
-	pmap(L, tc, L.l0, L.l1, false, 0.0, 0.0, bad);
-	pmap(L, tc, L.l1, L.l2, false, 0.0, 0.0, bad);
+	pmap<W>(L, tc, L.l0, L.l1, false, 0.0, 0.0, bad);
+	pmap<W>(L, tc, L.l1, L.l2, false, 0.0, 0.0, bad);
 	for (uint32_t i = threadIdx.x; i < tc.nt; i += PBLOCK) {
 		const double a = L.l0[i], b = L.l1[i], c = L.l2[i];
 		B[tc.tx_base + i] = b;
@@ -238,6 +270,7 @@ __global__ __launch_bounds__(PBLOCK) void k_ptile_map12(
 // Extrapolate from A, B, C with alpha (st.alpha_given 1: ctl->alpha; 2:
 // from the two sums part[0], part[1]), map once -> A with the stop test
 // (ORed into ctl->stopbad).
+template <bool W>
 __global__ __launch_bounds__(PBLOCK) void k_ptile_ex3(
     hemk_tiles tl, double *__restrict__ A, const double *__restrict__ B,
     const double *__restrict__ C, const uint32_t *__restrict__ uq,
@@ -254,15 +287,15 @@ __global__ __launch_bounds__(PBLOCK) void k_ptile_ex3(
 	const double smin = ctl->stop_min, srel = ctl->stop_rel;
 	const PTile tc = ptile(tl, blockIdx.x);
 	PLds L = pcarve(smem, tl);
-	pfields(L, tc);
-	pstage(tl, tc, L, uq, cntq);
+	pfields<W>(L, tc);
+	pstage<W>(tl, tc, L, uq, cntq);
 	for (uint32_t i = threadIdx.x; i < tc.nt; i += PBLOCK) {
 		const size_t g = tc.tx_base + i;
 		L.l0[i] = em_extrapolate(A[g], B[g], C[g], alpha, st);
 	}
 	__syncthreads();
 	int bad = 0;
-	pmap(L, tc, L.l0, L.l1, true, smin, srel, bad);
+	pmap<W>(L, tc, L.l0, L.l1, true, smin, srel, bad);
 	for (uint32_t i = threadIdx.x; i < tc.nt; i += PBLOCK)
 		A[tc.tx_base + i] = L.l1[i];
 	bad = __syncthreads_or(bad);
@@ -334,8 +367,14 @@ int hemk_ptile_map12(const hemk_tiles *t, const double *A, double *B,
 {
 	if (!t->n_tiles || t->max_tx > 256)
 		return (int)hipErrorInvalidValue;
-	k_ptile_map12<<<t->n_tiles, PBLOCK, hemk_ptile_lds(t),
-	                (hipStream_t)s>>>(*t, A, B, C, uq, cntq, xr, xv, ctl);
+	if (t->weighted)
+		k_ptile_map12<true><<<t->n_tiles, PBLOCK, hemk_ptile_lds(t),
+		                      (hipStream_t)s>>>(*t, A, B, C, uq, cntq, xr,
+		                                        xv, ctl);
+	else
+		k_ptile_map12<false><<<t->n_tiles, PBLOCK, hemk_ptile_lds(t),
+		                       (hipStream_t)s>>>(*t, A, B, C, uq, cntq, xr,
+		                                         xv, ctl);
 	return (int)hipGetLastError();
 }
 
@@ -346,8 +385,14 @@ int hemk_ptile_ex3(const hemk_tiles *t, double *A, const double *B,
 {
 	if (!t->n_tiles || t->max_tx > 256)
 		return (int)hipErrorInvalidValue;
-	k_ptile_ex3<<<t->n_tiles, PBLOCK, hemk_ptile_lds(t), (hipStream_t)s>>>(
-	    *t, A, B, C, uq, cntq, part, T, st, ctl);
+	if (t->weighted)
+		k_ptile_ex3<true><<<t->n_tiles, PBLOCK, hemk_ptile_lds(t),
+		                    (hipStream_t)s>>>(*t, A, B, C, uq, cntq, part,
+		                                      T, st, ctl);
+	else
+		k_ptile_ex3<false><<<t->n_tiles, PBLOCK, hemk_ptile_lds(t),
+		                     (hipStream_t)s>>>(*t, A, B, C, uq, cntq, part,
+		                                       T, st, ctl);
 	return (int)hipGetLastError();
 }
 

@@ -69,6 +69,7 @@ int hem_tiles_upload(hem_problem *p, const hem_tiles_host *h, hemk_tiles *dt,
 	dt->txq = mem[8];
 	dt->rec = mem[9];
 	dt->max_rec = h->max_rec;
+	dt->weighted = h->wcls != NULL;
 	if (rec_bytes)
 		*rec_bytes = h->rec_units * 16;
 	return HEM_OK;

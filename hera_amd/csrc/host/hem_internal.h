@@ -175,6 +175,8 @@ typedef struct {
 	uint32_t *desc, *tx, *len, *coff, *toff, *cls_orig, *txq;
 	uint32_t *kmem;
 	uint16_t *tcls;    /* class-list entry -> local class            */
+	double *wcls, *wocc; /* member weights in member order and in
+	                        class-list order (weighted tables), or NULL */
 	uint32_t *tsamp;   /* tile -> sample (ns > 1 only)               */
 	uint32_t *rec;     /* per-tile records, 16-byte units (desc[7])  */
 	uint64_t rec_units;
@@ -187,16 +189,20 @@ typedef struct {
 int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
                     const uint32_t *cls_ref, const uint32_t *tx_len,
                     const uint32_t *tx_off, const uint32_t *tx0,
-                    uint32_t ns, uint32_t tx_target, uint32_t cls_target,
+                    uint32_t ns, const double *member_weight,
+                    uint32_t tx_target, uint32_t cls_target,
                     uint32_t nnz_target, hem_tiles_host *h);
 void hem_tiles_free_host(hem_tiles_host *h);
 /* Bytes of a tile's record (see tiles.c), a multiple of 16. */
 static inline uint64_t hem_tile_rec_bytes(uint32_t nt, uint32_t nc,
-                                          uint32_t nz)
+                                          uint32_t nz, int weighted)
 {
 	uint64_t u16s = (uint64_t)(nc + 1) + (nt + 1) + 2 * (uint64_t)nz;
 	u16s += u16s & 1;
-	return (2 * u16s + 8 * (uint64_t)nt + 15) / 16 * 16;
+	uint64_t b = 2 * u16s + 8 * (uint64_t)nt;
+	if (weighted)
+		b = (b + 7) / 8 * 8 + 16 * (uint64_t)nz;
+	return (b + 15) / 16 * 16;
 }
 
 /* kernel kinds for per-kernel timing (hem_kernel_profile) */

@@ -130,7 +130,7 @@ static int plain_tiles(hem_problem *p, const hem_classes *c,
 {
 	hem_tiles_host th;
 	int rc = hem_tiles_build(c->n_tx, c->n_cls, c->cls_off, c->cls_ref,
-	                         c->tx_len, tx_off, NULL, 1,
+	                         c->tx_len, tx_off, NULL, 1, c->member_weight,
 	                         HEM_PTILE_TX_TARGET, HEM_PTILE_CLS_TARGET,
 	                         HEM_PTILE_NNZ_TARGET, &th);
 	if (rc == HEM_OK)
@@ -354,12 +354,26 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 		    c->cls_count[i] != floor(c->cls_count[i]))
 			p->counts_integral = 0;
 	/* component tiles for the tile kernels; a table with a component
-	 * over the tile limits, or with member weights, keeps only the
-	 * two-pass kernels */
-	if (!c->member_weight) {
+	 * over the tile limits keeps only the two-pass kernels, and one with
+	 * member weights has only the single-problem tiling (weights in its
+	 * records) */
+	if (c->member_weight) {
+		int trc = plain_tiles(p, c, tx_off);
+		if (trc == HEM_OK)
+			trc = hemd_stream_sync(p->stream) ? HEM_ERR_DEVICE : HEM_OK;
+		if (trc && trc != HEM_ERR_RANGE) {
+			free(tx_off);
+			free(tx_ref);
+			free(tx_w);
+			free(cnt32);
+			free(nd);
+			hem_problem_destroy(p);
+			return trc;
+		}
+	} else {
 		hem_tiles_host th;
 		int trc = hem_tiles_build(T, C, c->cls_off, c->cls_ref,
-		                          c->tx_len, tx_off, tx0, ns,
+		                          c->tx_len, tx_off, tx0, ns, NULL,
 		                          HEM_TILE_TX_TARGET, HEM_TILE_CLS_TARGET,
 		                          HEM_TILE_NNZ_TARGET, &th);
 		if (trc == HEM_OK) {

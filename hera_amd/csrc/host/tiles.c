@@ -75,14 +75,17 @@ static uint32_t uf_find(uint32_t *par, uint32_t x)
  * (local class << 6) | local transcript in class order, then the same u16
  * pairs in class-list order (each transcript's occurrences in the order its
  * sum takes them); then, 4-byte aligned, u32 lengths and u32 transcript
- * ids (n_tx each).  Runs while tcls still holds member positions. */
+ * ids (n_tx each); for a weighted table then, 8-byte aligned, the FP64
+ * member weights in class order and in class-list order.  Runs while tcls
+ * still holds member positions. */
 static int tiles_records(hem_tiles_host *h)
 {
 	uint64_t units = 0;
 	h->max_rec = 0;
 	for (uint32_t i = 0; i < h->n_tiles; ++i) {
 		const uint32_t *d = h->desc + (size_t)HEM_TILE_DESC * i;
-		const uint64_t b = hem_tile_rec_bytes(d[1], d[3], d[5]);
+		const uint64_t b = hem_tile_rec_bytes(d[1], d[3], d[5],
+		                                      h->wcls != NULL);
 		if (b > h->max_rec)
 			h->max_rec = (uint32_t)b;
 		units += b / 16;
@@ -117,8 +120,14 @@ static int tiles_records(hem_tiles_host *h)
 			r32[t] = h->len[d[0] + t];
 			r32[nt + t] = h->tx[d[0] + t];
 		}
+		if (h->wcls) {
+			const size_t b = ((size_t)2 * k + 8 * (size_t)nt + 7) / 8 * 8;
+			double *w = (double *)((char *)r16 + b);
+			memcpy(w, h->wcls + d[4], (size_t)nz * 8);
+			memcpy(w + nz, h->wocc + d[6], (size_t)nz * 8);
+		}
 		d[7] = (uint32_t)units;
-		units += hem_tile_rec_bytes(nt, nc, nz) / 16;
+		units += hem_tile_rec_bytes(nt, nc, nz, h->wcls != NULL) / 16;
 	}
 	return HEM_OK;
 }
@@ -135,6 +144,8 @@ void hem_tiles_free_host(hem_tiles_host *h)
 	free(h->txq);
 	free(h->toff);
 	free(h->tcls);
+	free(h->wcls);
+	free(h->wocc);
 	free(h->rec);
 	free(h->cls_orig);
 	free(h->tsamp);
@@ -147,7 +158,8 @@ void hem_tiles_free_host(hem_tiles_host *h)
 int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
                     const uint32_t *cls_ref, const uint32_t *tx_len,
                     const uint32_t *tx_off, const uint32_t *tx0,
-                    uint32_t ns, uint32_t tx_target, uint32_t cls_target,
+                    uint32_t ns, const double *member_weight,
+                    uint32_t tx_target, uint32_t cls_target,
                     uint32_t nnz_target, hem_tiles_host *h)
 {
 	uint32_t *par = NULL, *comp = NULL, *ctx = NULL, *ccls = NULL;
@@ -296,7 +308,12 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 	h->cls_orig = malloc((h->n_cls ? h->n_cls : 1) * 4);
 	h->coff = malloc(((size_t)h->n_cls + 1) * 4);
 	h->kmem = malloc((zlive ? zlive : 1) * 4);
-	if (!h->cls_orig || !h->coff || !h->kmem) {
+	if (member_weight) {
+		h->wcls = malloc((zlive ? zlive : 1) * 8);
+		h->wocc = malloc((zlive ? zlive : 1) * 8);
+	}
+	if (!h->cls_orig || !h->coff || !h->kmem ||
+	    (member_weight && (!h->wcls || !h->wocc))) {
 		rc = hem_fail(HEM_ERR_NOMEM, "tiling: host allocation");
 		goto out;
 	}
@@ -339,6 +356,8 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 				h->coff[q] = (uint32_t)z;
 				for (k = cls_off[c]; k < cls_off[c + 1]; ++k) {
 					mloc[k] = (uint32_t)z;
+					if (member_weight)
+						h->wcls[z] = member_weight[k];
 					h->kmem[z++] = ((q - q0) << 8) |
 					               lidx[cls_ref[k]];
 				}
@@ -365,9 +384,13 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
 			for (uint32_t q = q0; q < q1; ++q) {
 				uint32_t t = h->tx[q];
 				h->toff[q] = (uint32_t)z;
-				for (k = tx_off[t]; k < tx_off[t + 1]; ++k)
+				for (k = tx_off[t]; k < tx_off[t + 1]; ++k) {
+					if (member_weight)
+						h->wocc[z] =
+						    member_weight[txpos[k]];
 					h->tcls[z++] =
 					    (uint16_t)(mloc[txpos[k]] - mb);
+				}
 			}
 		}
 		h->toff[T] = (uint32_t)z;
@@ -463,8 +486,9 @@ int hem_classes_tiling(const hem_classes *c, uint32_t *n_tiles,
 	for (i = 0; i < c->n_tx; ++i)
 		tx_off[i + 1] += tx_off[i];
 	rc = hem_tiles_build(c->n_tx, c->n_cls, c->cls_off, c->cls_ref,
-	                     c->tx_len, tx_off, NULL, 1, HEM_TILE_TX_TARGET,
-	                     HEM_TILE_CLS_TARGET, HEM_TILE_NNZ_TARGET, &h);
+	                     c->tx_len, tx_off, NULL, 1, NULL,
+	                     HEM_TILE_TX_TARGET, HEM_TILE_CLS_TARGET,
+	                     HEM_TILE_NNZ_TARGET, &h);
 	free(tx_off);
 	if (rc == HEM_ERR_RANGE) {
 		*n_tiles = 0;

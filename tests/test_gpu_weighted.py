@@ -28,14 +28,20 @@ def say(*a):
     sys.stdout.flush()
 
 
-def solve_both(cls, u, **kw):
+def solve_both(cls, u, tiled=True, **kw):
     ora = O.estimate_count(cls, u, **kw)
     p = H.Problem(cls)
-    assert p.tiles()["n_tiles"] == 0      # weighted tables run untiled
-    g = p.solve(u, **{GKEY[k]: v for k, v in kw.items()})
+    assert p.tiles()["n_tiles"] == 0      # no batched tiling for weights
+    gkw = {GKEY[k]: v for k, v in kw.items()}
+    g = p.solve(u, **gkw)
+    # which kernels the solve ran: the tile kernels (weights in the tile
+    # records) or the two global passes
+    prof = p.profile(u, **gkw)
     p.close()
-    say("T = %d, nnz = %d, k = %d / %d" % (cls.n_tx, cls.nnz,
-                                           g["stats"]["iters"], ora["k"]))
+    assert (prof["tile_map12"]["launches"] > 0) == tiled
+    assert (prof["class_pass"]["launches"] > 0) == (not tiled)
+    say("T = %d, nnz = %d, k = %d / %d, tiled %s" % (
+        cls.n_tx, cls.nnz, g["stats"]["iters"], ora["k"], tiled))
     return g, ora
 
 
@@ -62,10 +68,28 @@ def test_weighted_hera_em_table_bitwise(kw):
     pats = hera_em_reads(9, 500, n_pat=4000, max_aln=6)
     cls, u = hera_em_table(500, pats)
     kw = dict(kw, stop_min=0.0, stop_rel=1e-10)
-    g, ora = solve_both(cls, u, **kw)
+    # the noise transcript is a member of every class: one component,
+    # over the tile limits, so the global passes run
+    g, ora = solve_both(cls, u, tiled=False, **kw)
     assert g["stats"]["iters"] == ora["k"]
     np.testing.assert_array_equal(g["lam"], ora["lam"])
     np.testing.assert_array_equal(g["tpm"], ora["tpm"])
+
+
+def test_weighted_untiled_bitwise():
+    """A component over the tile limits keeps the table on the global
+    passes, whose class and transcript kernels carry the weights too."""
+    cls, u = H.generate(2000, 20000, 1_000_000, seed=3)
+    members = [cls.cls_ref[cls.cls_off[i]:cls.cls_off[i + 1]].tolist()
+               for i in range(cls.n_cls)]
+    members += [[i, i + 1] for i in range(150)]
+    counts = np.concatenate([cls.cls_count, np.full(150, 3.0)])
+    big = H.Classes.from_lists(cls.n_tx, members, counts.tolist(),
+                               cls.tx_len.tolist())
+    big.member_weight = np.random.default_rng(3).uniform(0.05, 1.0, big.nnz)
+    g, ora = solve_both(big, u, tiled=False, min_round=200, max_round=200)
+    assert g["stats"]["iters"] == ora["k"] == 200
+    np.testing.assert_array_equal(g["lam"], ora["lam"])
 
 
 def test_unit_weights_bitwise_unweighted():

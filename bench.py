@@ -293,7 +293,7 @@ def traffic_from_profile(kind, args):
             200000, 2000000, 60_000_000, 0):
         return None
     kernel = {"tile_map12": "k_ptile_map12",
-              "tile_ex3": "k_ptile_ex3"}.get(kind)
+              "tile_ex3_map12": "k_ptile_ex3_map12"}.get(kind)
     path = os.path.join(ROOT, "profiles",
                         "em_kernels_200k_tx_2M_cls_counters.md")
     if kernel is None or not os.path.exists(path):

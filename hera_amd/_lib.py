@@ -52,7 +52,7 @@ class HemStats(C.Structure):
 HEM_KERNEL_KINDS = 8
 KERNEL_KIND_NAMES = ["class_pass", "transcript_pass", "norm_pass1",
                      "norm_pass2", "extrapolate", "iter_end", "tile_map12",
-                     "tile_ex3"]
+                     "tile_ex3_map12"]
 
 
 class HemKernelProfile(C.Structure):

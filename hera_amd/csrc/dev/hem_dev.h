@@ -313,18 +313,19 @@ int hemk_seq_norm(uint32_t n, const double *u, const double *u1,
 /* ---- one EM instance over component tiles (ptile_kernels.hip) -----------
  * State A, B, C in tile order; uq (unique counts) and cntq (class counts,
  * class tile positions) likewise.  map12: B = M(A), C = M(B), and r, v
- * rows in transcript order into xr, xv when xr != NULL.  ex3: extrapolate
- * (alpha: st.alpha_given 1 ctl->alpha, 2 from part[0..1]), A = M(.), stop
- * test into ctl->stopbad.  alpha: tree partials (nparts pairs) ->
+ * rows in transcript order into xr, xv when xr != NULL.  ex3_map12:
+ * extrapolate (alpha: st.alpha_given 1 ctl->alpha, 2 from part[0..1]),
+ * A = M(.) with the stop test into ctl->stopbad, then B = M(A), C = M(B)
+ * and the r, v rows as map12 does.  alpha: tree partials (nparts pairs) ->
  * ctl->alpha.  rows: out[t] = A[txq[t]].  lds: dynamic LDS of one tile. */
 size_t hemk_ptile_lds(const hemk_tiles *t);
 int hemk_ptile_map12(const hemk_tiles *t, const double *A, double *B,
                      double *C, const uint32_t *uq, const double *cntq,
                      double *xr, double *xv, const hemk_ctl *ctl, void *s);
-int hemk_ptile_ex3(const hemk_tiles *t, double *A, const double *B,
-                   const double *C, const uint32_t *uq, const double *cntq,
-                   const double *part, uint32_t T, hemk_step st,
-                   hemk_ctl *ctl, void *s);
+int hemk_ptile_ex3_map12(const hemk_tiles *t, double *A, double *B,
+                         double *C, const uint32_t *uq, const double *cntq,
+                         const double *part, uint32_t T, hemk_step st,
+                         double *xr, double *xv, hemk_ctl *ctl, void *s);
 int hemk_ptile_alpha(const double *part, uint32_t nparts, uint32_t T,
                      hemk_step st, hemk_ctl *ctl, void *s);
 int hemk_ptile_rows(uint32_t T, const uint32_t *txq, const double *A,

@@ -61,7 +61,7 @@ __device__ __forceinline__ uint32_t rec_bytes(uint32_t nt, uint32_t nc,
 // The tile in LDS: its record (room for the largest), then the FP64 work
 // arrays and unique counts sized by the tiling's largest tile.
 struct PLds {
-	double *l0, *l1, *l2;  // densities in, after one map, after two
+	double *l0, *l1, *l2, *l3;  // densities in, after one, two, three maps
 	double *D, *X, *n;     // class sums, member terms, class counts
 	uint32_t *u;
 	uint4 *rec;
@@ -79,7 +79,8 @@ __device__ __forceinline__ PLds pcarve(void *smem, const hemk_tiles &tl)
 	L.l0 = (double *)((char *)smem + tl.max_rec);
 	L.l1 = L.l0 + mt;
 	L.l2 = L.l1 + mt;
-	L.D = L.l2 + mt;
+	L.l3 = L.l2 + mt;
+	L.D = L.l3 + mt;
 	L.X = L.D + mc;
 	L.n = L.X + mz;
 	L.u = (uint32_t *)(L.n + mc);
@@ -267,15 +268,20 @@ __global__ __launch_bounds__(PBLOCK) void k_ptile_map12(
 	}
 }
 
-// Extrapolate from A, B, C with alpha (st.alpha_given 1: ctl->alpha; 2:
-// from the two sums part[0], part[1]), map once -> A with the stop test
-// (ORed into ctl->stopbad).
+// The end of one SQUAREM iteration and the start of the next in one pass
+// over each tile: extrapolate u' from A, B, C with alpha (st.alpha_given 1:
+// ctl->alpha; 2: from the two sums part[0], part[1]; src/EM.c:338-352),
+// lam = M(u') -> A with the stop test (src/EM.c:354-355, ORed into
+// ctl->stopbad), then the next iteration's u1 = M(lam) -> B, u2 = M(u1) -> C
+// and, when xr != NULL, its r and v rows (src/EM.c:330-336).  If the stop
+// rule ends the loop here, the extra u1, u2, r, v are never read.
 template <bool W>
-__global__ __launch_bounds__(PBLOCK) void k_ptile_ex3(
-    hemk_tiles tl, double *__restrict__ A, const double *__restrict__ B,
-    const double *__restrict__ C, const uint32_t *__restrict__ uq,
+__global__ __launch_bounds__(PBLOCK) void k_ptile_ex3_map12(
+    hemk_tiles tl, double *__restrict__ A, double *__restrict__ B,
+    double *__restrict__ C, const uint32_t *__restrict__ uq,
     const double *__restrict__ cntq, const double *__restrict__ part,
-    uint32_t T, hemk_step st, hemk_ctl *__restrict__ ctl)
+    uint32_t T, hemk_step st, double *__restrict__ xr,
+    double *__restrict__ xv, hemk_ctl *__restrict__ ctl)
 {
 #pragma clang fp contract(off)
 	extern __shared__ double smem[];
@@ -304,6 +310,18 @@ __global__ __launch_bounds__(PBLOCK) void k_ptile_ex3(
 			em_mark_bad(&ctl->stopbad);
 		if (blockIdx.x == 0 && st.alpha_given == 2)
 			ctl->alpha = alpha;
+	}
+	pmap<W>(L, tc, L.l1, L.l2, false, 0.0, 0.0, bad);
+	pmap<W>(L, tc, L.l2, L.l3, false, 0.0, 0.0, bad);
+	for (uint32_t i = threadIdx.x; i < tc.nt; i += PBLOCK) {
+		const double a = L.l1[i], b = L.l2[i], c = L.l3[i];
+		B[tc.tx_base + i] = b;
+		C[tc.tx_base + i] = c;
+		if (xr) {
+			const double r = b - a;
+			xr[L.tx[i]] = r;
+			xv[L.tx[i]] = (c - b) - r;
+		}
 	}
 }
 
@@ -358,7 +376,7 @@ extern "C" {
 size_t hemk_ptile_lds(const hemk_tiles *t)
 {
 	const size_t mt = t->max_tx, mc = t->max_cls, mz = t->max_nnz;
-	return (size_t)t->max_rec + 8 * (3 * mt + 2 * mc + mz) + 4 * mt;
+	return (size_t)t->max_rec + 8 * (4 * mt + 2 * mc + mz) + 4 * mt;
 }
 
 int hemk_ptile_map12(const hemk_tiles *t, const double *A, double *B,
@@ -378,21 +396,22 @@ int hemk_ptile_map12(const hemk_tiles *t, const double *A, double *B,
 	return (int)hipGetLastError();
 }
 
-int hemk_ptile_ex3(const hemk_tiles *t, double *A, const double *B,
-                   const double *C, const uint32_t *uq, const double *cntq,
-                   const double *part, uint32_t T, hemk_step st,
-                   hemk_ctl *ctl, void *s)
+int hemk_ptile_ex3_map12(const hemk_tiles *t, double *A, double *B,
+                         double *C, const uint32_t *uq, const double *cntq,
+                         const double *part, uint32_t T, hemk_step st,
+                         double *xr, double *xv, hemk_ctl *ctl, void *s)
 {
 	if (!t->n_tiles || t->max_tx > 256)
 		return (int)hipErrorInvalidValue;
 	if (t->weighted)
-		k_ptile_ex3<true><<<t->n_tiles, PBLOCK, hemk_ptile_lds(t),
-		                    (hipStream_t)s>>>(*t, A, B, C, uq, cntq, part,
-		                                      T, st, ctl);
+		k_ptile_ex3_map12<true><<<t->n_tiles, PBLOCK, hemk_ptile_lds(t),
+		                          (hipStream_t)s>>>(*t, A, B, C, uq, cntq,
+		                                            part, T, st, xr, xv,
+		                                            ctl);
 	else
-		k_ptile_ex3<false><<<t->n_tiles, PBLOCK, hemk_ptile_lds(t),
-		                     (hipStream_t)s>>>(*t, A, B, C, uq, cntq, part,
-		                                       T, st, ctl);
+		k_ptile_ex3_map12<false><<<t->n_tiles, PBLOCK,
+		                           hemk_ptile_lds(t), (hipStream_t)s>>>(
+		    *t, A, B, C, uq, cntq, part, T, st, xr, xv, ctl);
 	return (int)hipGetLastError();
 }
 

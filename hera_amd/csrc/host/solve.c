@@ -123,7 +123,24 @@ static int use_tiles(const hem_problem *p, const hem_opts *o)
 	       hemk_ptile_lds(&p->dtp) <= HEM_PTILE_LDS_MAX;
 }
 
-/* One SQUAREM iteration over the tiles. */
+/* The first iteration's u1 = M(u), u2 = M(u1) over the tiles (and its r,
+ * v rows for the exact sums); every later iteration's come from the
+ * kernel that ends the iteration before it. */
+static int enqueue_tiles_prologue(hem_problem *p, const hem_opts *o)
+{
+	const int exact = o->norm == HEM_NORM_EXACT;
+	const uint32_t T = p->n_tx;
+	HEM_TRY_DEV(hemk_ptile_map12(&p->dtp, p->d_A, p->d_B, p->d_C, p->d_uq,
+	                             p->d_cntq, exact ? p->d_sx : NULL,
+	                             exact ? p->d_sx + T : NULL, p->d_ctl,
+	                             p->stream),
+	            "tile maps");
+	return HEM_OK;
+}
+
+/* One SQUAREM iteration over the tiles, its u1 and u2 already in B and C:
+ * the norms, then one kernel that extrapolates, maps with the stop test
+ * and maps twice more for the next iteration. */
 static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
                                    prof_ctx *pc)
 {
@@ -134,11 +151,6 @@ static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
 	const double *alpha_part = NULL;
 	void *s = p->stream;
 
-	HEM_TRY_DEV(hemk_ptile_map12(&p->dtp, p->d_A, p->d_B, p->d_C, p->d_uq,
-	                             p->d_cntq, exact ? p->d_sx : NULL,
-	                             exact ? p->d_sx + T : NULL, p->d_ctl, s),
-	            "tile maps");
-	MARK(HEM_PK_TMAP12);
 	if (exact) {
 		/* the reference's left-to-right sums over the r and v rows;
 		 * with the sd form pass 1 also keeps each chunk's sum of
@@ -183,8 +195,11 @@ static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
 		            "alpha");
 		MARK(sd ? HEM_PK_NORM2 : HEM_PK_NORM1);
 	}
-	HEM_TRY_DEV(hemk_ptile_ex3(&p->dtp, p->d_A, p->d_B, p->d_C, p->d_uq,
-	                           p->d_cntq, alpha_part, T, st, p->d_ctl, s),
+	HEM_TRY_DEV(hemk_ptile_ex3_map12(&p->dtp, p->d_A, p->d_B, p->d_C,
+	                                 p->d_uq, p->d_cntq, alpha_part, T, st,
+	                                 exact ? p->d_sx : NULL,
+	                                 exact ? p->d_sx + T : NULL, p->d_ctl,
+	                                 s),
 	            "tile extrapolated map");
 	MARK(HEM_PK_TEX3);
 	HEM_TRY_DEV(hemk_iter_end(p->d_ctl, s), "iter_end");
@@ -389,7 +404,9 @@ static int solve_setup(hem_problem *p, const uint32_t *unique,
 		rc = hemk_ctl_reset(p->d_ctl, o->min_iter, o->max_iter,
 		                    o->stop_min_value, o->stop_rel_change,
 		                    p->stream);
-	return rc ? hem_dev_fail(rc, "solve setup") : HEM_OK;
+	if (rc)
+		return hem_dev_fail(rc, "solve setup");
+	return use_tiles(p, o) ? enqueue_tiles_prologue(p, o) : HEM_OK;
 }
 
 static int load_opts(const hem_opts *opts, hem_opts *o)
@@ -505,7 +522,7 @@ out:
 
 /* Algorithmic bytes per launch of each kernel kind: compulsory traffic with
  * every array element touched once (gathered vectors counted once). */
-static void kernel_bytes(const hem_problem *p, int f32, double *b)
+static void kernel_bytes(const hem_problem *p, int f32, int rv, double *b)
 {
 	const double T = p->n_tx, C = p->n_cls, Z = (double)p->nnz;
 	if (f32) {
@@ -538,7 +555,8 @@ static void kernel_bytes(const hem_problem *p, int f32, double *b)
 		const double S = 32 * NT + (double)p->ptile_rec_bytes + 4 * T +
 		                 8 * TC;
 		b[HEM_PK_TMAP12] = S + 8 * T + 16 * T + 16 * T;
-		b[HEM_PK_TEX3] = S + 24 * T + 8 * T;
+		/* A, B, C in; lam, u1, u2 out; r, v rows with the exact sums */
+		b[HEM_PK_TEX3] = S + 24 * T + 24 * T + (rv ? 16 * T : 0);
 	}
 }
 
@@ -559,7 +577,8 @@ int hem_profile_solve(hem_problem *p, const uint32_t *unique,
 		return rc;
 	memset(out, 0, sizeof(*out));
 	memset(&pc, 0, sizeof(pc));
-	kernel_bytes(p, o.precision == HEM_PREC_FP32, out->bytes);
+	kernel_bytes(p, o.precision == HEM_PREC_FP32,
+	             o.norm == HEM_NORM_EXACT, out->bytes);
 	for (i = 0; i < 16 && !rc; ++i)
 		rc = hemd_event_create(&pc.ev[i]);
 	if (!rc)

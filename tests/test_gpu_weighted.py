@@ -38,7 +38,7 @@ def solve_both(cls, u, tiled=True, **kw):
     # records) or the two global passes
     prof = p.profile(u, **gkw)
     p.close()
-    assert (prof["tile_map12"]["launches"] > 0) == tiled
+    assert (prof["tile_ex3_map12"]["launches"] > 0) == tiled
     assert (prof["class_pass"]["launches"] > 0) == (not tiled)
     say("T = %d, nnz = %d, k = %d / %d, tiled %s" % (
         cls.n_tx, cls.nnz, g["stats"]["iters"], ora["k"], tiled))

@@ -145,7 +145,7 @@ int hemk_tile_chunk(const hemk_tiles *t, uint32_t R, int cnt_bytes);
  * [class tile position][R] (u32 when cnt_bytes == 4, FP64 when 8).
  *   map12: u1 = M(A) -> B, u2 = M(B) -> C; with part != NULL also the
  *          per-(tile, replicate) sums of r and v (Chan form) for the
- *          parallel norms, part[(tile*R + r)*4 ...]; with xr != NULL the
+ *          parallel norms, part[(r*n_tiles + tile)*4 ...]; with xr the
  *          r and v values in replicate-major rows xr/xv[r*T + t], t the
  *          transcript id, for the sequential norms.
  *   ex3:   extrapolate from A, B, C (alpha[r] when st.alpha_given, else
@@ -154,10 +154,14 @@ int hemk_tile_map12(const hemk_tiles *t, hemk_rctl rc, int chunk,
                     const void *cnt, int cnt_bytes, const uint32_t *uniq,
                     const double *A, double *B, double *C, double *part,
                     double *xr, double *xv, uint32_t T, int step, void *s);
+/* ex3 with next != 0 also maps twice more for the next iteration:
+ * B = M(A), C = M(B) and the r, v rows (xr, xv) or tree partials (part),
+ * as map12 does. */
 int hemk_tile_ex3(const hemk_tiles *t, hemk_rctl rc, int chunk,
                   const void *cnt, int cnt_bytes, const uint32_t *uniq,
-                  double *A, const double *B, const double *C, uint32_t T,
-                  hemk_step st, void *s);
+                  double *A, double *B, double *C, uint32_t T,
+                  hemk_step st, int next, double *part, double *xr,
+                  double *xv, void *s);
 /* Sequential sums of a batch over replicate-major rows, one left-to-right
  * chain per (instance, row) block over the instance's sample's transcripts
  * (row r, ids tx0_s ..), in the reference's order over t:

@@ -302,6 +302,88 @@ __device__ __forceinline__ void store_counts(const ChunkRows<TC, NS> &p,
 		n[e] = rin ? cnt[(size_t)(tc.cls_base + e / RC) * rc.R + q] : (TC)0;
 }
 
+// The r and v of one replicate chunk from u, u1, u2 in LDS (src/EM.c:
+// 334-335): as replicate-major rows for the exact or sequential norms
+// (xr, xv), or, with part, as per (tile, replicate) partials for the tree
+// norms: the local two-pass mean and centred sum of squares of r and v
+// (combined across tiles by Chan's rule), or plain sums of squares for the
+// L2 step.
+template <typename TC, int RC>
+__device__ __forceinline__ void rv_parts(const hemk_tiles &tl,
+                                         const TileCtx &tc,
+                                         const TileStruct &ts, uint32_t tile,
+                                         const double *lam0,
+                                         const double *lam1,
+                                         const double *lam2, uint32_t q,
+                                         double *xr, double *xv, double *part,
+                                         uint32_t T, int step)
+{
+#pragma clang fp contract(off)
+	const int r = threadIdx.x % RC;
+	const uint32_t e0 = threadIdx.x / RC;
+	constexpr uint32_t ES = TBLOCK / RC;
+	const bool rin = q != NOREP;
+	if (xr && rin) {
+		// r_t and v_t (src/EM.c:334-335) in replicate-major rows
+		// for the sequential norms
+		for (uint32_t t = e0; t < tc.nt; t += ES) {
+			const double x = lam1[t * RC + r] - lam0[t * RC + r];
+			const double y = (lam2[t * RC + r] - lam1[t * RC + r]) - x;
+			const size_t g = (size_t)q * T + ts.tx[t];
+			xr[g] = x;
+			xv[g] = y;
+		}
+	}
+	if (part && threadIdx.x < RC && rin) {
+		// per (tile, replicate): local two-pass mean and centred
+		// sum of squares of r and v (combined across tiles by
+		// Chan's rule), or plain sums of squares for the L2 step
+		const int lq = threadIdx.x;
+		double a0, a1, b0, b1;
+		if (step == 0) {
+			double sr = 0, sv = 0;
+			for (uint32_t t = 0; t < tc.nt; ++t) {
+				const double x = lam1[t * RC + lq] - lam0[t * RC + lq];
+				const double y =
+				    (lam2[t * RC + lq] - lam1[t * RC + lq]) - x;
+				sr += x;
+				sv += y;
+			}
+			const double mr = sr / tc.nt, mv = sv / tc.nt;
+			double qr = 0, qv = 0;
+			for (uint32_t t = 0; t < tc.nt; ++t) {
+				const double x = lam1[t * RC + lq] - lam0[t * RC + lq];
+				const double y =
+				    (lam2[t * RC + lq] - lam1[t * RC + lq]) - x;
+				qr += (x - mr) * (x - mr);
+				qv += (y - mv) * (y - mv);
+			}
+			a0 = mr;
+			a1 = qr;
+			b0 = mv;
+			b1 = qv;
+		} else {
+			double qr = 0, qv = 0;
+			for (uint32_t t = 0; t < tc.nt; ++t) {
+				const double x = lam1[t * RC + lq] - lam0[t * RC + lq];
+				const double y =
+				    (lam2[t * RC + lq] - lam1[t * RC + lq]) - x;
+				qr += x * x;
+				qv += y * y;
+			}
+			a0 = 0;
+			a1 = qr;
+			b0 = 0;
+			b1 = qv;
+		}
+		double *pp = part + ((size_t)q * tl.n_tiles + tile) * 4;
+		pp[0] = a0;
+		pp[1] = a1;
+		pp[2] = b0;
+		pp[3] = b1;
+	}
+}
+
 template <typename TC, int RC>
 __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
     hemk_tiles tl, hemk_rctl rc, const TC *__restrict__ cnt,
@@ -355,65 +437,8 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
 		                 rc.R, q, nullptr, 0, 0);
 		tile_map<TC, RC>(tl, tc, ts, lam1, lam2, D, n, u, alive, C,
 		                 rc.R, q, nullptr, 0, 0);
-		if (xr && rin) {
-			// r_t and v_t (src/EM.c:334-335) in replicate-major rows
-			// for the sequential norms
-			for (uint32_t t = e0; t < tc.nt; t += ES) {
-				const double x = lam1[t * RC + r] - lam0[t * RC + r];
-				const double y = (lam2[t * RC + r] - lam1[t * RC + r]) - x;
-				const size_t g = (size_t)q * T + ts.tx[t];
-				xr[g] = x;
-				xv[g] = y;
-			}
-		}
-		if (part && threadIdx.x < RC && rin) {
-			// per (tile, replicate): local two-pass mean and centred
-			// sum of squares of r and v (combined across tiles by
-			// Chan's rule), or plain sums of squares for the L2 step
-			const int lq = threadIdx.x;
-			double a0, a1, b0, b1;
-			if (step == 0) {
-				double sr = 0, sv = 0;
-				for (uint32_t t = 0; t < tc.nt; ++t) {
-					const double x = lam1[t * RC + lq] - lam0[t * RC + lq];
-					const double y =
-					    (lam2[t * RC + lq] - lam1[t * RC + lq]) - x;
-					sr += x;
-					sv += y;
-				}
-				const double mr = sr / tc.nt, mv = sv / tc.nt;
-				double qr = 0, qv = 0;
-				for (uint32_t t = 0; t < tc.nt; ++t) {
-					const double x = lam1[t * RC + lq] - lam0[t * RC + lq];
-					const double y =
-					    (lam2[t * RC + lq] - lam1[t * RC + lq]) - x;
-					qr += (x - mr) * (x - mr);
-					qv += (y - mv) * (y - mv);
-				}
-				a0 = mr;
-				a1 = qr;
-				b0 = mv;
-				b1 = qv;
-			} else {
-				double qr = 0, qv = 0;
-				for (uint32_t t = 0; t < tc.nt; ++t) {
-					const double x = lam1[t * RC + lq] - lam0[t * RC + lq];
-					const double y =
-					    (lam2[t * RC + lq] - lam1[t * RC + lq]) - x;
-					qr += x * x;
-					qv += y * y;
-				}
-				a0 = 0;
-				a1 = qr;
-				b0 = 0;
-				b1 = qv;
-			}
-			double *pp = part + ((size_t)q * tl.n_tiles + tile) * 4;
-			pp[0] = a0;
-			pp[1] = a1;
-			pp[2] = b0;
-			pp[3] = b1;
-		}
+		rv_parts<TC, RC>(tl, tc, ts, tile, lam0, lam1, lam2, q, xr, xv,
+		                 part, T, step);
 		__syncthreads();
 	}
 }
@@ -475,11 +500,13 @@ __device__ __forceinline__ void load_alpha(ChunkRows<TC, 3> &p,
 }
 
 template <typename TC, int RC>
-__global__ __launch_bounds__(TBLOCK) void k_tile_ex3(
+// four waves per SIMD (at most 128 VGPRs), as many as the LDS budget allows
+__global__ __launch_bounds__(TBLOCK, 4) void k_tile_ex3(
     hemk_tiles tl, hemk_rctl rc, const TC *__restrict__ cnt,
     const uint32_t *__restrict__ uniq, double *__restrict__ A,
-    const double *__restrict__ B, const double *__restrict__ C, uint32_t T,
-    hemk_step st)
+    double *__restrict__ B, double *__restrict__ C, uint32_t T,
+    hemk_step st, int next, double *__restrict__ part,
+    double *__restrict__ xr, double *__restrict__ xv)
 {
 #pragma clang fp contract(off)
 	extern __shared__ double smem[];
@@ -544,6 +571,16 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_ex3(
 		                 rc.R, q, stopf, rc.stop_min, rc.stop_rel);
 		if (threadIdx.x < RC && rin && stopf[threadIdx.x])
 			rc.notconv[smp * rc.R + q] = 1;
+		if (next) {
+			// the next iteration's u1 -> B, u2 -> C (into lam0, whose
+			// extrapolated values are spent) and its r, v
+			tile_map<TC, RC>(tl, tc, ts, lam1, lam2, D, n, u, alive, B,
+			                 rc.R, q, nullptr, 0, 0);
+			tile_map<TC, RC>(tl, tc, ts, lam2, lam0, D, n, u, alive, C,
+			                 rc.R, q, nullptr, 0, 0);
+			rv_parts<TC, RC>(tl, tc, ts, tile, lam1, lam2, lam0, q, xr,
+			                 xv, part, T, st.step);
+		}
 		__syncthreads();
 	}
 }
@@ -930,14 +967,15 @@ int launch_map12(const hemk_tiles *t, hemk_rctl rc, const void *cnt,
 
 template <typename TC, int RC>
 int launch_ex3(const hemk_tiles *t, hemk_rctl rc, const void *cnt,
-               const uint32_t *uniq, double *A, const double *B,
-               const double *C, uint32_t T, hemk_step st, hipStream_t s)
+               const uint32_t *uniq, double *A, double *B, double *C,
+               uint32_t T, hemk_step st, int next, double *part, double *xr,
+               double *xv, hipStream_t s)
 {
 	const size_t lds = tile_lds(t, RC, sizeof(TC));
 	if (rc.chunk != (uint32_t)RC)
 		return (int)hipErrorInvalidValue;
 	k_tile_ex3<TC, RC><<<t->n_tiles, TBLOCK, lds, s>>>(
-	    *t, rc, (const TC *)cnt, uniq, A, B, C, T, st);
+	    *t, rc, (const TC *)cnt, uniq, A, B, C, T, st, next, part, xr, xv);
 	return (int)hipGetLastError();
 }
 
@@ -992,18 +1030,19 @@ int hemk_tile_map12(const hemk_tiles *t, hemk_rctl rc, int chunk,
 
 int hemk_tile_ex3(const hemk_tiles *t, hemk_rctl rc, int chunk,
                   const void *cnt, int cnt_bytes, const uint32_t *uniq,
-                  double *A, const double *B, const double *C, uint32_t T,
-                  hemk_step st, void *s)
+                  double *A, double *B, double *C, uint32_t T,
+                  hemk_step st, int next, double *part, double *xr,
+                  double *xv, void *s)
 {
 	hipStream_t sm = (hipStream_t)s;
 	if (t->n_tiles == 0)
 		return 0;
 	if (cnt_bytes == 4) {
-#define EXU(RC) launch_ex3<uint32_t, RC>(t, rc, cnt, uniq, A, B, C, T, st, sm)
+#define EXU(RC) launch_ex3<uint32_t, RC>(t, rc, cnt, uniq, A, B, C, T, st, next, part, xr, xv, sm)
 		HEM_RC_SWITCH(chunk, EXU)
 #undef EXU
 	}
-#define EXD(RC) launch_ex3<double, RC>(t, rc, cnt, uniq, A, B, C, T, st, sm)
+#define EXD(RC) launch_ex3<double, RC>(t, rc, cnt, uniq, A, B, C, T, st, next, part, xr, xv, sm)
 	HEM_RC_SWITCH(chunk, EXD)
 #undef EXD
 }

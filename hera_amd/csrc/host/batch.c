@@ -297,6 +297,25 @@ static hemk_spec spec_batch(hem_problem *p, hemk_rctl rc, int pairs, int xf,
 
 /* ---- the iteration ------------------------------------------------------ */
 
+/* The first iteration's u1, u2 and r/v rows (or tree partials); every
+ * later iteration's come from the kernel that ends the one before it. */
+static int batch_prologue(hem_problem *p, const hem_opts *o, hemk_rctl rc,
+                          int chunk)
+{
+	hem_batch_ws *w = &p->bw;
+	const int exact = o->norm != HEM_NORM_TREE;
+	const size_t TR = (size_t)p->n_tx * rc.R;
+	double *xr = exact ? w->X : NULL, *xv = exact ? w->X + TR : NULL;
+	HEM_TRY_DEV(hemk_tile_map12(&p->dt, rc, chunk, w->cnt, 4, w->uq, w->A,
+	                            w->B, w->C, exact ? NULL : w->part, xr, xv,
+	                            p->n_tx, o->step, p->stream),
+	            "tile map12");
+	return HEM_OK;
+}
+
+/* One SQUAREM iteration of every active instance, its u1 and u2 already
+ * in B and C: the norms, then one kernel that extrapolates, maps with the
+ * stop test and maps twice more for the next iteration. */
 static int batch_iteration(hem_problem *p, const hem_opts *o, hemk_rctl rc,
                            int chunk)
 {
@@ -311,10 +330,6 @@ static int batch_iteration(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 	hemk_step st = {o->step, o->clamp, o->alpha_clamp != 0, !exact};
 	const size_t TR = (size_t)p->n_tx * rc.R;
 	double *xr = exact ? w->X : NULL, *xv = exact ? w->X + TR : NULL;
-	HEM_TRY_DEV(hemk_tile_map12(&p->dt, rc, chunk, w->cnt, 4, w->uq, w->A,
-	                            w->B, w->C, exact ? NULL : w->part, xr, xv,
-	                            p->n_tx, o->step, s),
-	            "tile map12");
 	if (spec) {
 		/* the reference's left-to-right sums by speculation: pass 1
 		 * sums (sd) or sums of squares (L2, straight into q_r/q_v),
@@ -348,7 +363,8 @@ static int batch_iteration(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 		            "batch alpha");
 	}
 	HEM_TRY_DEV(hemk_tile_ex3(&p->dt, rc, chunk, w->cnt, 4, w->uq, w->A,
-	                          w->B, w->C, p->n_tx, st, s),
+	                          w->B, w->C, p->n_tx, st, 1,
+	                          exact ? NULL : w->part, xr, xv, s),
 	            "tile ex3");
 	HEM_TRY_DEV(hemk_batch_iter_end(rc, s), "batch iter end");
 	return HEM_OK;
@@ -470,6 +486,8 @@ static int run_batch(hem_problem *p, const hem_opts *o, uint32_t R,
 	HEM_TRY_DEV(hemk_batch_reset(rc, p->n_tx, w->A, p->stream), "reset");
 	HEM_TRY_DEV(hemk_fill_u32(reruns_word(p), 1, 0, p->stream),
 	            "rerun counter");
+	if ((rc2 = batch_prologue(p, o, rc, chunk)))
+		return rc2;
 	if ((rc2 = batch_graph(p, o, rc, chunk, &graph)))
 		return rc2;
 	HEM_TRY_DEV(hemd_event_record(p->ev0, p->stream), "event");

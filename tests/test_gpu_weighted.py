@@ -140,3 +140,19 @@ def test_weighted_rejects():
     with pytest.raises(H.HemError) as ei:
         H.Problem(bad)
     assert ei.value.code == _lib.HEM_ERR_INVALID
+
+
+def test_zero_weights_bitwise():
+    """Weights may be 0: a class whose members all weigh 0 has D_c == 0
+    while its transcripts' densities are positive; the reference's map
+    skips it (src/EM.c:279-280), which the tile kernels reproduce with
+    an exact zero term."""
+    cls, u = weighted_cfg(12)
+    rng = np.random.default_rng(12)
+    w = cls.member_weight
+    w[rng.random(cls.nnz) < 0.1] = 0.0
+    c0 = int(np.argmax(np.diff(cls.cls_off) >= 3))
+    w[cls.cls_off[c0]:cls.cls_off[c0 + 1]] = 0.0
+    g, ora = solve_both(cls, u)
+    assert g["stats"]["iters"] == ora["k"]
+    np.testing.assert_array_equal(g["lam"], ora["lam"])

@@ -351,7 +351,7 @@ int hemk_ptile_rows(uint32_t T, const uint32_t *txq, const double *A,
  * at once when ctl->done is set.  Scratch, per chain slot: csum, drift,
  * base [nchain][P]; tab [nchain][P][W] doubles and nx [nchain][P][W] bytes
  * (W at most HEMK_SPEC_W). */
-#define HEMK_SPEC_W 64
+#define HEMK_SPEC_W 128
 #define HEMK_SPEC_LMAX 1024   /* longest chunk the kernels accept */
 #define HEMK_SPEC_IDENT 0
 #define HEMK_SPEC_CENTRED 1
@@ -359,7 +359,7 @@ int hemk_ptile_rows(uint32_t T, const uint32_t *txq, const double *A,
 #define HEMK_SPEC_NOLINK 0xFFu   /* nx entry: result outside next window */
 typedef struct {
 	uint32_t n, L, P;      /* values per chain, chunk length, chunks      */
-	uint32_t W;            /* candidates per chunk: 8, 16, 32 or 64       */
+	uint32_t W;            /* candidates per chunk: 8, 16, 32, 64 or 128  */
 	uint32_t nchain;       /* chain slots (grid height)                   */
 	uint32_t R;            /* pairs: row offset of the second row block   */
 	int pairs, xf;
@@ -390,9 +390,10 @@ typedef struct {
 int hemk_spec_phases(const hemk_spec *sp, int phases, hemk_ctl *ctl,
                      void *s);
 /* Chunk length for chains of n values, and the window width for a
- * transform at that length. */
+ * transform at that length when a launch holds `chunks` chunks (chains x
+ * chunks per chain). */
 uint32_t hemk_spec_chunk(uint32_t n);
-uint32_t hemk_spec_width(uint32_t L, int xf);
+uint32_t hemk_spec_width(uint32_t L, int xf, uint32_t chunks);
 /* produce == 1 (plain solve, nchain 2, rows 0/1 of x): first writes
  * r = u1 - u and v = (u2 - u1) - r (src/EM.c:333-336) into x. */
 int hemk_spec_sum(const hemk_spec *sp, int produce, const double *u,

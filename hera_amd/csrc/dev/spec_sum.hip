@@ -20,9 +20,10 @@
 //      rounded increments fl(s + y) - s over a block scan from g0_j;
 //      plain chunk sums miss the rounding by up to ~10^4 ulps;
 //   3. g_j = sum_{i<j} d_i (a fixed expression, so every kernel forms the
-//      same bits); every chunk runs F_j from the 64 consecutive doubles
-//      centred on g_j (one candidate per lane), all at once, and looks each
-//      result up in the next chunk's window (a link, or none);
+//      same bits); every chunk runs F_j from the W consecutive doubles
+//      centred on g_j (one candidate per lane; W = 64 or 128 for signed
+//      rows, 16 for squares), all at once, and looks each result up in
+//      the next chunk's window (a link, or none);
 //   4. one workgroup per chain follows the links from s_0 = +0, skipping
 //      ahead over links composed in LDS; where the exact start of a chunk
 //      lies outside its window (a miss), that chunk is run again from it
@@ -509,14 +510,15 @@ __device__ __forceinline__ void chunk_guess(const double *d, uint32_t P,
 }
 
 // Step 3: every chunk from W consecutive starts around its guess.  Grid
-// (P, chains), one wave per chunk (its values staged in one round of
-// loads); lane c < W runs the chunk from key(g) - W/2 + c.  Besides the
+// (P, chains), one workgroup of max(W, 64) threads per chunk (its values
+// staged by the first wave in one round of loads); thread c < W runs the
+// chunk from key(g) - W/2 + c.  Besides the
 // results (tab) and the window base, each candidate's result is looked up
 // in the next chunk's window: nx holds its offset there, or
 // HEMK_SPEC_NOLINK.
 template <int W, int XF>
-__global__ __launch_bounds__(WAVE) void k_spec_window(hemk_spec sp,
-                                                      const hemk_ctl *ctl)
+__global__ __launch_bounds__(W > WAVE ? W : WAVE) void k_spec_window(
+    hemk_spec sp, const hemk_ctl *ctl)
 {
 #pragma clang fp contract(off)
 	__shared__ double ys[HEMK_SPEC_LMAX];
@@ -527,11 +529,15 @@ __global__ __launch_bounds__(WAVE) void k_spec_window(hemk_spec sp,
 	const uint32_t j = blockIdx.x;
 	const size_t cb = (size_t)blockIdx.y * sp.P;
 	const uint32_t cnt = min(sp.L, sp.n - j * sp.L);
+	const bool stager = c < WAVE;
 	double v[CH_REG];
-	chunk_load(sp.x + (size_t)row * sp.ld + (size_t)j * sp.L, cnt, v);
+	if (stager)
+		chunk_load(sp.x + (size_t)row * sp.ld + (size_t)j * sp.L, cnt,
+		           v);
 	double g, gn;
 	chunk_guess<1>(sp.drift + cb, sp.P, j, j, g, gn);
-	chunk_store<XF>(v, cnt, chain_mean<XF>(sp, row), ys);
+	if (stager)
+		chunk_store<XF>(v, cnt, chain_mean<XF>(sp, row), ys);
 	__syncthreads();
 	if (c >= W)
 		return;
@@ -558,8 +564,9 @@ __global__ __launch_bounds__(WAVE) void k_spec_window(hemk_spec sp,
 #define RC 8
 #define REP_LDS (RC * HEMK_SPEC_LMAX * 8)   // a repair's chunk values
 static_assert(RC * RW <= SBLOCK, "repair lanes fit the workgroup");
-// chunks whose links the resolve kernel holds in LDS at once
-#define SEGP 1024
+// chunks whose links the resolve kernel holds in LDS at once: 64 KB of
+// links for W = 64 and W = 128
+#define SEGP(W) ((W) > 64 ? 512 : 1024)
 
 template <int W, int XF>
 __device__ __forceinline__ void repair(const hemk_spec &sp, uint32_t row,
@@ -604,7 +611,7 @@ __device__ __forceinline__ uint32_t ufirst(uint32_t v)
 // The walk holds either the exact start s of chunk j or the offset e of
 // that start in chunk j's window.  With an offset, the next chunk's offset
 // is nx[j][e]; composed links over 16 and 256 chunks (built in LDS per
-// segment of SEGP chunks) let it skip ahead, and the result of the last
+// segment of SEGP(W) chunks) let it skip ahead, and the result of the last
 // chunk is tab[P-1][e].  A missing link means chunk j+1's exact start
 // tab[j][e] lies outside its window: the walk takes that value, and a start
 // that is in neither its window nor the current repair's starts a repair.
@@ -615,9 +622,10 @@ __global__ __launch_bounds__(SBLOCK) void k_spec_resolve(hemk_spec sp,
                                                          const hemk_ctl *ctl)
 {
 #pragma clang fp contract(off)
-	__shared__ uint8_t nxl[SEGP * W];
-	__shared__ uint8_t c1[SEGP / 16 * W];
-	__shared__ uint8_t c2[SEGP / 256 * W];
+	constexpr uint32_t SEG = SEGP(W);
+	__shared__ uint8_t nxl[SEG * W];
+	__shared__ uint8_t c1[SEG / 16 * W];
+	__shared__ uint8_t c2[SEG / 256 * W];
 	__shared__ double ys[REP_LDS / 8];
 	__shared__ double rtl[RC * RW];
 	__shared__ long long rbl[RC];
@@ -631,8 +639,8 @@ __global__ __launch_bounds__(SBLOCK) void k_spec_resolve(hemk_spec sp,
 	double s = 0.0;
 	uint32_t j = 0, e = 0, rj = 0, misses = 0;
 	bool have = false, rep = false;
-	for (uint32_t jb = 0; jb < P; jb += SEGP) {
-		const uint32_t ns = min((uint32_t)SEGP, P - jb);
+	for (uint32_t jb = 0; jb < P; jb += SEG) {
+		const uint32_t ns = min(SEG, P - jb);
 		const uint32_t last = P - 1 - jb;
 		__syncthreads();
 		{
@@ -656,13 +664,29 @@ __global__ __launch_bounds__(SBLOCK) void k_spec_resolve(hemk_spec sp,
 		__syncthreads();
 		// links over 16 chunks: block b maps an offset at chunk jb + 16b
 		// to one at jb + 16b + 16 (that chunk exists, its links in LDS)
+		// (with wide windows four entries per thread step together, so
+		// their dependent LDS reads overlap)
+		constexpr int IL = W >= 64 ? 4 : 1;
 		const uint32_t nb1 = min(ns / 16, last / 16);
-		for (uint32_t it = tid; it < nb1 * W; it += SBLOCK) {
-			const uint32_t b = it / W;
-			uint32_t x = it % W;
-			for (int t = 0; t < 16 && x != HEMK_SPEC_NOLINK; ++t)
-				x = nxl[(16 * b + t) * W + x];
-			c1[it] = (uint8_t)x;
+		const uint32_t n1 = nb1 * W;
+		for (uint32_t it0 = tid; it0 < n1; it0 += IL * SBLOCK) {
+			uint32_t x[IL], b[IL];
+#pragma unroll
+			for (int q = 0; q < IL; ++q) {
+				const uint32_t it = it0 + q * SBLOCK;
+				b[q] = it / W;
+				x[q] = it < n1 ? it % W : HEMK_SPEC_NOLINK;
+			}
+#pragma unroll
+			for (int t = 0; t < 16; ++t)
+#pragma unroll
+				for (int q = 0; q < IL; ++q)
+					if (x[q] != HEMK_SPEC_NOLINK)
+						x[q] = nxl[(16 * b[q] + t) * W + x[q]];
+#pragma unroll
+			for (int q = 0; q < IL; ++q)
+				if (it0 + q * SBLOCK < n1)
+					c1[it0 + q * SBLOCK] = (uint8_t)x[q];
 		}
 		__syncthreads();
 		const uint32_t nb2 = min(ns / 256, last / 256);
@@ -760,8 +784,8 @@ int launch_xf(const hemk_spec &sp, int produce, int phases, const double *u,
 		return (int)hipGetLastError();
 #define SPEC_W_CASE(WW)                                                    \
 	case WW:                                                           \
-		k_spec_window<WW, XF>                                      \
-		    <<<dim3(sp.P, sp.nchain), WAVE, 0, st>>>(sp, ctl);      \
+		k_spec_window<WW, XF><<<dim3(sp.P, sp.nchain),               \
+		                        WW > WAVE ? WW : WAVE, 0, st>>>(sp, ctl); \
 		k_spec_resolve<WW, XF><<<sp.nchain, SBLOCK, 0, st>>>(sp, ctl); \
 		break;
 	switch (sp.W) {
@@ -769,6 +793,7 @@ int launch_xf(const hemk_spec &sp, int produce, int phases, const double *u,
 		SPEC_W_CASE(16)
 		SPEC_W_CASE(32)
 		SPEC_W_CASE(64)
+		SPEC_W_CASE(128)
 	default:
 		return (int)hipErrorInvalidValue;
 	}
@@ -787,12 +812,18 @@ uint32_t hemk_spec_chunk(uint32_t n)
 	return n <= 16384 ? 256 : HEMK_SPEC_LMAX;
 }
 
-uint32_t hemk_spec_width(uint32_t L, int xf)
+uint32_t hemk_spec_width(uint32_t L, int xf, uint32_t chunks)
 {
-	// signed sums (whose running sum can pass close to zero): 64
-	// candidates; sums of squares are guessed within ~2 ulps: 16
+	// Sums of squares are guessed within ~2 ulps: 16 candidates.  Signed
+	// sums (whose running sum can pass close to zero) miss more: 64
+	// candidates, or 128 (two waves per chunk) when the window launch
+	// has few enough chunks that the second wave finds an idle SIMD;
+	// on SQUAREM's r and v rows of 200k values that turns about a third
+	// of the resolve's reruns into hits.
 	(void)L;
-	return xf == HEMK_SPEC_IDENT ? 64 : 16;
+	if (xf != HEMK_SPEC_IDENT)
+		return 16;
+	return chunks <= 512 ? 128 : 64;
 }
 
 static int spec_launch(const hemk_spec *sp, int produce, int phases,

@@ -276,8 +276,8 @@ static hemk_spec spec_batch(hem_problem *p, hemk_rctl rc, int pairs, int xf,
 	sp.n = p->n_tx;
 	sp.L = p->spec_L;
 	sp.P = p->spec_P;
-	sp.W = hemk_spec_width(sp.L, xf);
 	sp.nchain = pairs ? 2 * rc.R : rc.R;
+	sp.W = hemk_spec_width(sp.L, xf, sp.nchain * sp.P);
 	sp.R = rc.R;
 	sp.pairs = pairs;
 	sp.xf = xf;

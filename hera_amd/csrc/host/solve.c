@@ -97,8 +97,8 @@ static hemk_spec spec_plain(const hem_problem *p, uint32_t nchain, int xf,
 	sp.n = p->n_tx;
 	sp.L = p->spec_L;
 	sp.P = p->spec_P;
-	sp.W = hemk_spec_width(sp.L, xf);
 	sp.nchain = nchain;
+	sp.W = hemk_spec_width(sp.L, xf, nchain * sp.P);
 	sp.R = 1;
 	sp.xf = xf;
 	sp.x = x;
@@ -716,14 +716,14 @@ int hem_exact_sums(const double *x, uint32_t n, uint32_t n_rows, int mode,
 	sp.miss = d_miss;
 	sp.res = d_res;
 	sp.xf = mode == HEM_SUM_SQUARES ? HEMK_SPEC_SQUARE : HEMK_SPEC_IDENT;
-	sp.W = hemk_spec_width(sp.L, sp.xf);
+	sp.W = hemk_spec_width(sp.L, sp.xf, sp.nchain * sp.P);
 	e = hemd_h2d(d_x, x, (size_t)n_rows * n * 8, s);
 	if (!e) e = hemk_fill_u32(d_miss, 1, 0, s);
 	if (!e) e = hemk_spec_sum(&sp, 0, NULL, NULL, NULL, NULL, s);
 	if (!e && mode == HEM_SUM_NORM_SD) {
 		/* second pass of Norm: centred squares about sum / n */
 		sp.xf = HEMK_SPEC_CENTRED;
-		sp.W = hemk_spec_width(sp.L, sp.xf);
+		sp.W = hemk_spec_width(sp.L, sp.xf, sp.nchain * sp.P);
 		sp.mres = d_res;
 		sp.res = d_res + n_rows;
 		e = hemk_spec_sum(&sp, 0, NULL, NULL, NULL, NULL, s);

@@ -394,6 +394,9 @@ int hemk_spec_phases(const hemk_spec *sp, int phases, hemk_ctl *ctl,
  * chunks per chain). */
 uint32_t hemk_spec_chunk(uint32_t n);
 uint32_t hemk_spec_width(uint32_t L, int xf, uint32_t chunks);
+/* Candidate-table entries (tab doubles, nx bytes) that launches of up to
+ * max_rows chains of P chunks each can use, at the width each picks. */
+size_t hemk_spec_tab_entries(uint32_t L, uint32_t max_rows, uint32_t P);
 /* produce == 1 (plain solve, nchain 2, rows 0/1 of x): first writes
  * r = u1 - u and v = (u2 - u1) - r (src/EM.c:333-336) into x. */
 int hemk_spec_sum(const hemk_spec *sp, int produce, const double *u,

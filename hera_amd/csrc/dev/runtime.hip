@@ -55,19 +55,40 @@ int hemd_stream_create(void **s)
 int hemd_stream_destroy(void *s) { return (int)hipStreamDestroy((hipStream_t)s); }
 int hemd_stream_sync(void *s) { return (int)hipStreamSynchronize((hipStream_t)s); }
 
+// Host transfers go in pieces of at most 16 MiB.  For a pageable host
+// buffer above its pinned-transfer threshold (128 MiB by default) the HIP
+// runtime locks the caller's pages for the copy's duration, the
+// registration of existing host memory that a locked-down host may refuse;
+// below it the runtime copies through its own pinned staging buffers.  The
+// batched solves read back R x T doubles (160 MB at 100 x 200k) into
+// caller memory, so every large transfer is split here.
+static const size_t kHostPiece = (size_t)16 << 20;
+
 int hemd_h2d(void *dst, const void *src, size_t n, void *s)
 {
-	if (!n)
-		return 0;
-	return (int)hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice,
-	                           (hipStream_t)s);
+	for (size_t o = 0; o < n; o += kHostPiece) {
+		const size_t m = n - o < kHostPiece ? n - o : kHostPiece;
+		hipError_t e = hipMemcpyAsync((char *)dst + o,
+		                              (const char *)src + o, m,
+		                              hipMemcpyHostToDevice,
+		                              (hipStream_t)s);
+		if (e != hipSuccess)
+			return (int)e;
+	}
+	return 0;
 }
 int hemd_d2h(void *dst, const void *src, size_t n, void *s)
 {
-	if (!n)
-		return 0;
-	return (int)hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost,
-	                           (hipStream_t)s);
+	for (size_t o = 0; o < n; o += kHostPiece) {
+		const size_t m = n - o < kHostPiece ? n - o : kHostPiece;
+		hipError_t e = hipMemcpyAsync((char *)dst + o,
+		                              (const char *)src + o, m,
+		                              hipMemcpyDeviceToHost,
+		                              (hipStream_t)s);
+		if (e != hipSuccess)
+			return (int)e;
+	}
+	return 0;
 }
 int hemd_d2d(void *dst, const void *src, size_t n, void *s)
 {

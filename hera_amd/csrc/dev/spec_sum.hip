@@ -826,6 +826,22 @@ uint32_t hemk_spec_width(uint32_t L, int xf, uint32_t chunks)
 	return chunks <= 512 ? 128 : 64;
 }
 
+size_t hemk_spec_tab_entries(uint32_t L, uint32_t max_rows, uint32_t P)
+{
+	// the width only narrows as the launch grows, so a smaller launch
+	// can need more entries than the largest one
+	size_t best = 0;
+	for (uint32_t rows = 1; rows <= max_rows; ++rows) {
+		const size_t chunks = (size_t)rows * P;
+		const size_t w = hemk_spec_width(
+		    L, HEMK_SPEC_IDENT,
+		    chunks > 0xFFFFFFFFu ? 0xFFFFFFFFu : (uint32_t)chunks);
+		if (chunks * w > best)
+			best = chunks * w;
+	}
+	return best;
+}
+
 static int spec_launch(const hemk_spec *sp, int produce, int phases,
                        const double *u, const double *u1, const double *u2,
                        hemk_ctl *ctl, void *s)

@@ -203,8 +203,14 @@ static int ws_ensure(hem_problem *p, uint32_t R)
 	}
 	WALLOC(w->sc, (size_t)4 * R * p->spec_P * 8);
 	WALLOC(w->sbase, (size_t)2 * R * p->spec_P * 8);
-	WALLOC(w->stab, (size_t)2 * R * p->spec_P * HEMK_SPEC_W * 8);
-	WALLOC(w->snx, (size_t)2 * R * p->spec_P * HEMK_SPEC_W);
+	{
+		/* r/v pairs (2R rows) and the final sums (R rows), each at
+		 * the window width its launch picks */
+		const size_t ent = hemk_spec_tab_entries(p->spec_L, 2 * R,
+		                                         p->spec_P);
+		WALLOC(w->stab, ent * 8);
+		WALLOC(w->snx, ent);
+	}
 #undef WALLOC
 	if (hemd_host_alloc((void **)&w->h_active,
 	                    (size_t)(p->ns > 16 ? p->ns : 16) * 4))

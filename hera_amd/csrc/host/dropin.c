@@ -4,7 +4,8 @@
  * The class table behind CLASS is flattened into CSR once and kept on the
  * device; it is rebuilt when CLASS, its class array, its size or REF_INF
  * change.  Concurrent callers (hera's bootstrap threads, src/EM.c:243-255)
- * are serialised by the problem handle.
+ * share the handle, which serialises their solves; each holds a reference
+ * to it for the length of its call.
  */
 #include <pthread.h>
 #include <stdarg.h>
@@ -24,8 +25,17 @@ __attribute__((weak)) Gclass *CLASS = NULL;
 /* hera's logger (src/log.c:25), called when hera provides it. */
 extern void _log(const char *fmt, ...) __attribute__((weak));
 
+/* The cached handle and its users.  The cache holds one reference and
+ * every estimate_count call in flight holds one more, so a key change or
+ * hem_dropin_release while other threads are solving only detaches the
+ * handle: the last holder destroys it. */
+typedef struct {
+	hem_problem *p;
+	unsigned refs;
+} dropin_ent;
+
 static pthread_mutex_t dropin_mu = PTHREAD_MUTEX_INITIALIZER;
-static hem_problem *dropin_prob;
+static dropin_ent *dropin_cur;
 static const void *dropin_key[4];
 static uint32_t dropin_n[2];
 static uint32_t bootstrap_cnt;
@@ -36,77 +46,105 @@ static void die(const char *what)
 	exit(EXIT_FAILURE);
 }
 
-static hem_problem *cached_problem(void)
+/* Drops one reference; dropin_mu held. */
+static void ent_put_locked(dropin_ent *e)
 {
-	hem_problem *p;
+	if (e && --e->refs == 0) {
+		hem_problem_destroy(e->p);
+		free(e);
+	}
+}
+
+static hem_problem *table_upload(void)
+{
+	uint32_t C = CLASS->n, i;
+	uint64_t nnz = 0;
+	uint32_t *off = malloc(((size_t)C + 1) * 4);
+	double *cnt = malloc((C ? C : 1) * sizeof(double));
+	uint32_t *ref;
+	hem_classes hc;
+	hem_problem *p = NULL;
+	int rc;
+	if (!off || !cnt)
+		die("estimate_count: host allocation");
+	off[0] = 0;
+	for (i = 0; i < C; ++i) {
+		nnz += CLASS->cls[i].n;
+		if (nnz > 0xFFFFFFFFull) {
+			fprintf(stderr, "ERROR: estimate_count: more than "
+			                "2^32-1 class members\n");
+			exit(EXIT_FAILURE);
+		}
+		off[i + 1] = (uint32_t)nnz;
+		cnt[i] = CLASS->cls[i].count;
+	}
+	ref = malloc((nnz ? nnz : 1) * 4);
+	if (!ref)
+		die("estimate_count: host allocation");
+	for (i = 0; i < C; ++i)
+		memcpy(ref + off[i], CLASS->cls[i].ref,
+		       (size_t)CLASS->cls[i].n * 4);
+	hc.n_tx = REF_INF->n;
+	hc.n_cls = C;
+	hc.cls_off = off;
+	hc.cls_ref = ref;
+	hc.cls_count = cnt;
+	hc.tx_len = REF_INF->len;
+	hc.member_weight = NULL;
+	rc = hem_problem_create(&hc, &p);
+	free(off);
+	free(ref);
+	free(cnt);
+	if (rc)
+		die("estimate_count: class table upload");
+	return p;
+}
+
+/* The handle for the current CLASS / REF_INF, with a reference taken for
+ * the caller (released by dropin_put). */
+static dropin_ent *dropin_get(void)
+{
+	dropin_ent *e;
 	pthread_mutex_lock(&dropin_mu);
-	if (dropin_prob && dropin_key[0] == CLASS &&
+	if (dropin_cur && dropin_key[0] == CLASS &&
 	    dropin_key[1] == CLASS->cls && dropin_key[2] == REF_INF &&
 	    dropin_key[3] == REF_INF->len && dropin_n[0] == CLASS->n &&
 	    dropin_n[1] == REF_INF->n) {
-		p = dropin_prob;
+		e = dropin_cur;
+		++e->refs;
 		pthread_mutex_unlock(&dropin_mu);
-		return p;
+		return e;
 	}
-	hem_problem_destroy(dropin_prob);
-	dropin_prob = NULL;
-	{
-		uint32_t C = CLASS->n, i;
-		uint64_t nnz = 0;
-		uint32_t *off = malloc(((size_t)C + 1) * 4);
-		double *cnt = malloc((C ? C : 1) * sizeof(double));
-		uint32_t *ref;
-		hem_classes hc;
-		int rc;
-		if (!off || !cnt)
-			die("estimate_count: host allocation");
-		off[0] = 0;
-		for (i = 0; i < C; ++i) {
-			nnz += CLASS->cls[i].n;
-			if (nnz > 0xFFFFFFFFull) {
-				fprintf(stderr, "ERROR: estimate_count: more than "
-				                "2^32-1 class members\n");
-				exit(EXIT_FAILURE);
-			}
-			off[i + 1] = (uint32_t)nnz;
-			cnt[i] = CLASS->cls[i].count;
-		}
-		ref = malloc((nnz ? nnz : 1) * 4);
-		if (!ref)
-			die("estimate_count: host allocation");
-		for (i = 0; i < C; ++i)
-			memcpy(ref + off[i], CLASS->cls[i].ref,
-			       (size_t)CLASS->cls[i].n * 4);
-		hc.n_tx = REF_INF->n;
-		hc.n_cls = C;
-		hc.cls_off = off;
-		hc.cls_ref = ref;
-		hc.cls_count = cnt;
-		hc.tx_len = REF_INF->len;
-		hc.member_weight = NULL;
-		rc = hem_problem_create(&hc, &dropin_prob);
-		free(off);
-		free(ref);
-		free(cnt);
-		if (rc)
-			die("estimate_count: class table upload");
-	}
+	ent_put_locked(dropin_cur);
+	dropin_cur = NULL;
+	e = malloc(sizeof(*e));
+	if (!e)
+		die("estimate_count: host allocation");
+	e->p = table_upload();
+	e->refs = 2;
+	dropin_cur = e;
 	dropin_key[0] = CLASS;
 	dropin_key[1] = CLASS->cls;
 	dropin_key[2] = REF_INF;
 	dropin_key[3] = REF_INF->len;
 	dropin_n[0] = CLASS->n;
 	dropin_n[1] = REF_INF->n;
-	p = dropin_prob;
 	pthread_mutex_unlock(&dropin_mu);
-	return p;
+	return e;
+}
+
+static void dropin_put(dropin_ent *e)
+{
+	pthread_mutex_lock(&dropin_mu);
+	ent_put_locked(e);
+	pthread_mutex_unlock(&dropin_mu);
 }
 
 EM_val *estimate_count(unsigned int *count, unsigned int order)
 {
 	hem_opts o;
 	hem_stats st;
-	hem_problem *p;
+	dropin_ent *ent;
 	EM_val *em_val;
 	uint32_t T;
 
@@ -116,7 +154,7 @@ EM_val *estimate_count(unsigned int *count, unsigned int order)
 		exit(EXIT_FAILURE);
 	}
 	T = REF_INF->n;
-	p = cached_problem();
+	ent = dropin_get();
 	em_val = malloc(sizeof(EM_val));
 	if (!em_val)
 		die("estimate_count: host allocation");
@@ -125,8 +163,10 @@ EM_val *estimate_count(unsigned int *count, unsigned int order)
 	if (!em_val->grama || !em_val->overlap)
 		die("estimate_count: host allocation");
 	hem_opts_default(&o);
-	if (hem_solve(p, count, &o, em_val->grama, em_val->overlap, NULL, &st))
+	if (hem_solve(ent->p, count, &o, em_val->grama, em_val->overlap, NULL,
+	              &st))
 		die("estimate_count");
+	dropin_put(ent);
 
 	/* src/EM.c:375-384 */
 	if (order == 0) {
@@ -157,8 +197,8 @@ __attribute__((weak)) void destroy_emVal(EM_val *em_val)
 int hem_dropin_release(void)
 {
 	pthread_mutex_lock(&dropin_mu);
-	hem_problem_destroy(dropin_prob);
-	dropin_prob = NULL;
+	ent_put_locked(dropin_cur);
+	dropin_cur = NULL;
 	memset(dropin_key, 0, sizeof(dropin_key));
 	pthread_mutex_unlock(&dropin_mu);
 	return HEM_OK;

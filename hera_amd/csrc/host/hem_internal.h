@@ -37,8 +37,8 @@ typedef struct {
 	double *sc;                    /* exact sums: [2][2R][P] chunk sums
 	                                  and drifts                        */
 	long long *sbase;              /* [2R][P] window bases              */
-	double *stab;                  /* [2R][P][HEMK_SPEC_W] candidates   */
-	uint8_t *snx;                  /* [2R][P][HEMK_SPEC_W] window links */
+	double *stab;                  /* [rows][P][W] candidates           */
+	uint8_t *snx;                  /* [rows][P][W] window links         */
 	uint32_t *h_active;            /* pinned                            */
 	/* captured iterations; the loop bounds and stop thresholds are kernel
 	 * arguments, so a graph is reused only for identical options */

@@ -263,41 +263,102 @@ class CEMVal(C.Structure):
                 ("overlap", C.POINTER(C.c_double))]
 
 
+class HeraGlobals:
+    """hera's REF_INF and CLASS (src/hash_align.h:99-121) over a class
+    table, pointed at by the library's weak globals while in use."""
+
+    def __init__(self, lib, cls, u):
+        self.lib = lib
+        self.keep = []
+        carr = (CClass * cls.n_cls)()
+        for c in range(cls.n_cls):
+            m = np.ascontiguousarray(
+                cls.cls_ref[cls.cls_off[c]:cls.cls_off[c + 1]], dtype=np.uint32)
+            self.keep.append(m)
+            carr[c].n = len(m)
+            carr[c].ref = m.ctypes.data_as(C.POINTER(C.c_uint))
+            carr[c].count = cls.cls_count[c]
+            carr[c].id = c
+        self.carr = carr
+        self.g = CGclass(cls.n_cls, None, carr)
+        self.lens = np.ascontiguousarray(cls.tx_len, dtype=np.uint32)
+        self.cnt = np.ascontiguousarray(u, dtype=np.uint32)
+        self.ri = CRefInf(cls.n_tx, 0, None,
+                          self.lens.ctypes.data_as(C.POINTER(C.c_uint)),
+                          None, None, None,
+                          self.cnt.ctypes.data_as(C.POINTER(C.c_uint)))
+
+    def __enter__(self):
+        C.c_void_p.in_dll(self.lib, "REF_INF").value = C.addressof(self.ri)
+        C.c_void_p.in_dll(self.lib, "CLASS").value = C.addressof(self.g)
+        return self
+
+    def __exit__(self, *exc):
+        self.lib.hem_dropin_release()
+        C.c_void_p.in_dll(self.lib, "REF_INF").value = None
+        C.c_void_p.in_dll(self.lib, "CLASS").value = None
+
+    def estimate(self, counts, order):
+        """estimate_count(counts, order) -> (tpm, est), freed after."""
+        T = self.ri.n
+        c = np.ascontiguousarray(counts, dtype=np.uint32)
+        ptr = self.lib.estimate_count(c.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                      order)
+        ev = C.cast(ptr, C.POINTER(CEMVal)).contents
+        tpm = np.ctypeslib.as_array(ev.grama, (T,)).copy()
+        est = np.ctypeslib.as_array(ev.overlap, (T,)).copy()
+        self.lib.destroy_emVal(ptr)
+        return tpm, est
+
+
 def test_dropin_estimate_count_reads_hera_globals():
     from hera_amd import _lib
     lib = _lib.load()
     cls, u = tiny_problem(5, n_tx=80, n_cls=400, dup=True)
-    T = cls.n_tx
-    keep = []
-    carr = (CClass * cls.n_cls)()
-    for c in range(cls.n_cls):
-        m = np.ascontiguousarray(cls.cls_ref[cls.cls_off[c]:cls.cls_off[c + 1]],
-                                 dtype=np.uint32)
-        keep.append(m)
-        carr[c].n = len(m)
-        carr[c].ref = m.ctypes.data_as(C.POINTER(C.c_uint))
-        carr[c].count = cls.cls_count[c]
-        carr[c].id = c
-    g = CGclass(cls.n_cls, None, carr)
-    lens = np.ascontiguousarray(cls.tx_len, dtype=np.uint32)
-    cnt = np.ascontiguousarray(u, dtype=np.uint32)
-    ri = CRefInf(T, 0, None, lens.ctypes.data_as(C.POINTER(C.c_uint)), None,
-                 None, None, cnt.ctypes.data_as(C.POINTER(C.c_uint)))
-    C.c_void_p.in_dll(lib, "REF_INF").value = C.addressof(ri)
-    C.c_void_p.in_dll(lib, "CLASS").value = C.addressof(g)
-    try:
-        ptr = lib.estimate_count(cnt.ctypes.data_as(C.POINTER(C.c_uint32)), 0)
-        ev = C.cast(ptr, C.POINTER(CEMVal)).contents
-        tpm = np.ctypeslib.as_array(ev.grama, (T,)).copy()
-        est = np.ctypeslib.as_array(ev.overlap, (T,)).copy()
-        lib.destroy_emVal(ptr)
-    finally:
-        lib.hem_dropin_release()
-        C.c_void_p.in_dll(lib, "REF_INF").value = None
-        C.c_void_p.in_dll(lib, "CLASS").value = None
+    with HeraGlobals(lib, cls, u) as hg:
+        tpm, est = hg.estimate(u, 0)
     ora = O.estimate_count(cls, u)
     assert tpm.sum() == pytest.approx(1e6, rel=1e-12)
     # default mode (FP64, exact norms) with the reference's own stop rule:
     # bitwise the reference's result
     np.testing.assert_array_equal(tpm, ora["tpm"])
     np.testing.assert_array_equal(est, ora["est"])
+
+
+def test_dropin_concurrent_bootstrap_threads():
+    # hera calls estimate_count from nThread bootstrap threads at once
+    # (src/EM.c:243-255), each with its own count vector.  Eight threads,
+    # four calls each, while the main thread releases the cached handle
+    # twice (each release forces the next call to rebuild it): every
+    # result is bitwise the oracle's on its own counts.
+    import threading
+    from hera_amd import _lib
+    lib = _lib.load()
+    cls, u = tiny_problem(9, n_tx=120, n_cls=600, dup=True)
+    rng = np.random.default_rng(4)
+    n_thr, per = 8, 4
+    counts = [rng.poisson(np.maximum(u, 1)).astype(np.uint32)
+              for _ in range(n_thr * per)]
+    out = [None] * len(counts)
+    errs = []
+    with HeraGlobals(lib, cls, u) as hg:
+        def work(i):
+            try:
+                for j in range(per):
+                    q = i * per + j
+                    out[q] = hg.estimate(counts[q], q + 1)
+            except Exception as e:      # pragma: no cover
+                errs.append(e)
+        th = [threading.Thread(target=work, args=(i,)) for i in range(n_thr)]
+        for t in th:
+            t.start()
+        for _ in range(2):
+            lib.hem_dropin_release()
+        for t in th:
+            t.join(timeout=120)
+        assert not any(t.is_alive() for t in th)
+    assert not errs, errs
+    for q, c in enumerate(counts):
+        ora = O.estimate_count(cls, c)
+        np.testing.assert_array_equal(out[q][0], ora["tpm"])
+        np.testing.assert_array_equal(out[q][1], ora["est"])

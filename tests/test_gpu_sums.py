@@ -79,6 +79,40 @@ def test_exact_sums_bitwise(name):
     assert same_bits(sq, exp_sq), (sq, exp_sq)
 
 
+def launch_chunks(rows, n):
+    """Chunks of one launch of the signed sums: chains of 256 values up to
+    16,384 values per row, of 1,024 above."""
+    L = 256 if n <= 16384 else 1024
+    return rows * ((n + L - 1) // L)
+
+
+ADVERSARIAL = ["return_to_zero", "alternating", "denormal", "absorbed_tail",
+               "nonfinite", "signed_zeros", "lognormal_wide"]
+
+
+@pytest.mark.parametrize("name", ADVERSARIAL)
+def test_exact_sums_bitwise_narrow_windows(name):
+    # Signed sums take 128-candidate windows when a launch holds at most
+    # 512 chunks and 64-candidate windows above that: the adversarial rows
+    # stacked (each copy scaled by a power of two) until the launch passes
+    # 512 chunks run the 64-wide path.
+    x = cases()[name]
+    reps = 1
+    while launch_chunks(reps * x.shape[0], x.shape[1]) <= 512:
+        reps += 1
+    scale = 2.0 ** (np.arange(reps) % 5 - 2)
+    xs = np.concatenate([x * s for s in scale])
+    assert launch_chunks(xs.shape[0], xs.shape[1]) > 512
+    got, reruns = H.exact_sums(xs, H.SUM_PLAIN)
+    exp = np.array([seq_sum(r) for r in xs])
+    sq, _ = H.exact_sums(xs, H.SUM_SQUARES)
+    exp_sq = np.array([seq_sum(r * r) for r in xs])
+    say("%-16s n=%6d rows=%d reruns=%d (64-wide)" % (
+        name, xs.shape[1], xs.shape[0], reruns))
+    assert same_bits(got, exp)
+    assert same_bits(sq, exp_sq)
+
+
 @pytest.mark.parametrize("name", ["normal_1025", "normal_200000",
                                   "lognormal_wide", "return_to_zero",
                                   "alternating", "denormal"])

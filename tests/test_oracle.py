@@ -164,6 +164,43 @@ def test_alpha_minus_one_when_em_is_exact():
                                rtol=4e-16)
 
 
+def test_squarem_step_uses_sample_sd():
+    # One SQUAREM iteration from the uniform start (src/EM.c:330-352):
+    # r = u1 - u, v = (u2 - u1) - r from two plain maps, the step length
+    # -Norm(r)/Norm(v) with Norm the sample sd of src/hash_align.c:150-164
+    # (mean-centred, n-1), clamped to <= -1, then u - 2 a r + a^2 v >= 0.
+    # The table is chosen so that r and v have means far from zero: there
+    # the L2 form -||r||/||v|| (hera-EM/utility.c:16-41) gives another
+    # step, so an oracle that used it by default would fail here.
+    cls, u = tiny_problem(0)
+    T = cls.n_tx
+    u0 = np.full(T, 1.0 / T)
+    u1, _ = O.em_map(cls, u, u0)
+    u2, _ = O.em_map(cls, u, u1)
+    r = u1 - u0
+    v = (u2 - u1) - r
+    a_sd = -np.std(r, ddof=1) / np.std(v, ddof=1)
+    a_l2 = -np.sqrt(np.sum(r * r)) / np.sqrt(np.sum(v * v))
+    # preconditions: both forms escape the clamp and differ by far more
+    # than rounding
+    assert a_sd < -1 and a_l2 < -1
+    assert abs(a_sd - a_l2) > 0.1 * abs(a_sd)
+    res = O.estimate_count(cls, u, min_round=1, max_round=1)
+    assert res["k"] == 1
+    assert res["alpha"][0] == pytest.approx(min(-1.0, a_sd), rel=1e-12)
+    a = float(res["alpha"][0])
+    # the extrapolated point (the third map's input), in the reference's
+    # expression order: no FMA on x86-64 without -mfma, so bit for bit
+    ext = np.maximum(u0 - 2 * a * r + a * a * v, 0.0)
+    np.testing.assert_array_equal(res["last_u"], ext)
+    lam, _ = O.em_map(cls, u, ext)
+    np.testing.assert_array_equal(res["lam"], lam)
+    # the L2 option is hera-EM's form, and is not the default
+    res2 = O.estimate_count(cls, u, min_round=1, max_round=1, step_form=1)
+    assert res2["alpha"][0] == pytest.approx(min(-1.0, a_l2), rel=1e-12)
+    assert abs(res2["alpha"][0] - res["alpha"][0]) > 0.1 * abs(a_sd)
+
+
 def test_alpha_clamp_and_extrapolation_clamp():
     cls, u = tiny_problem(11, n_tx=60, n_cls=300, zero_frac=0.6)
     r = O.estimate_count(cls, u, min_round=30, max_round=30)

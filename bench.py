@@ -70,9 +70,9 @@ def main():
     ap.add_argument("--boot-modes",
                     default="fp64-exact,fp64-tree,hera-sampler",
                     help="bootstrap modes to run")
-    ap.add_argument("--seq-sample", type=int, default=3,
+    ap.add_argument("--seq-sample", type=int, default=-1,
                     help="replicates solved one by one for the unbatched "
-                         "comparison")
+                         "comparison (-1: all of them; 0: skip)")
     ap.add_argument("--cpu-boot-threads", type=int, default=16,
                     help="threads (= replicates) of the CPU bootstrap sample")
     ap.add_argument("--skip-plain", action="store_true",
@@ -120,9 +120,14 @@ def main():
               max_iter=args.max_iter)
     if args.skip_plain:
         out = {"metric": METRIC, "skipped": "plain solves (--skip-plain)"}
+        # the copy of known size that calibrates profiler byte counters
+        out["copy_bandwidth_GBps"] = H.copy_bandwidth(1 << 30, 10)
         if args.bootstrap_reps > 0:
             boot = bench_bootstrap(args, H, np, p, cls, u, sk)
             boot.pop("_est_exact", None)
+            for v in boot.values():
+                if isinstance(v, dict):
+                    v.pop("_iters", None)
             out["bootstrap"] = boot
         if args.ragged_samples > 0:
             out["ragged"] = bench_ragged(args, H, np)
@@ -166,7 +171,13 @@ def main():
                     or modes[head_name]["norm"] != H.NORM_EXACT]
     dom = max(stream_kinds, key=lambda k: prof[k]["ms_total"])
     d = prof[dom]
-    achieved = d["bytes_per_launch"] / (d["ms_avg"] * 1e-3) / 1e9
+    # algorithmic bytes: 8Z + 16C + 24T per EM map (u32 indices, FP32
+    # values: the compulsory traffic of a class pass and a transcript pass
+    # with the M-step), times the maps one launch applies
+    b_map = map_bytes(cls)
+    maps = MAPS_PER_LAUNCH.get(dom)
+    alg = b_map * maps if maps else d["bytes_per_launch"]
+    achieved = alg / (d["ms_avg"] * 1e-3) / 1e9
     tot_ms = sum(prof[k]["ms_total"] for k in kinds)
     for k in kinds:
         if prof[k]["launches"]:
@@ -174,6 +185,7 @@ def main():
                 k, prof[k]["launches"], prof[k]["ms_avg"],
                 100 * prof[k]["ms_total"] / tot_ms,
                 prof[k]["bytes_per_launch"] / (prof[k]["ms_avg"] * 1e-3) / 1e9))
+    it_ms = tot_ms / max(prof["iters"], 1)
 
     out = {
         "metric": METRIC,
@@ -208,8 +220,18 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_from_profile(dom, args),
-            "bytes_per_launch": d["bytes_per_launch"],
+            "algorithmic_bytes_per_launch": alg,
+            "bytes_rule": "8Z + 16C + 24T per EM map (u32 indices, FP32 "
+                          "values) x %s maps per launch" % maps,
+            "kernel_own_bytes_per_launch": d["bytes_per_launch"],
             "ms_per_launch": d["ms_avg"],
+            "iteration": {
+                "ms": it_ms,
+                "algorithmic_bytes": 3 * b_map + 80 * cls.n_tx,
+                "achieved_GBps": (3 * b_map + 80 * cls.n_tx)
+                                 / (it_ms * 1e-3) / 1e9,
+                "note": "one SQUAREM iteration, all its kernels, by HIP "
+                        "events: 3 maps + 80T of FP64 vector traffic"},
         },
         "modes": results,
         "kernels_headline": {k: dict(prof[k]) for k in kinds
@@ -250,7 +272,7 @@ def main():
                                    max_round=args.max_iter)
             k_cpu += ora["k"]
         dt_cpu = O.now() - t0
-        out["cpu_baseline"] = {
+        out["cpu_baseline"] = dict({
             "value": k_cpu / dt_cpu,
             "unit": "EM iterations/s",
             "cores": 1,
@@ -258,7 +280,7 @@ def main():
             "sample": "%d full plain solve(s) of this workload (%d iterations, "
                       "%.1fs) by the FP64 C restatement of src/EM.c, "
                       "gcc -O2, 1 thread" % (n, k_cpu, dt_cpu),
-        }
+        }, **cpu_info())
         m = ora["tpm"] / 1e6 > 1e-7
         ll_o = O.loglik(cls, u, ora["lam"])
         par = {"k_oracle": ora["k"], "abund_threshold": "tpm/1e6 > 1e-7"}
@@ -281,8 +303,49 @@ def main():
         if args.cpu_boot_threads > 0:
             bootstrap_cpu(args, np, O, cls, u, boot)
         boot.pop("_est_exact", None)
+        for v in boot.values():
+            if isinstance(v, dict):
+                v.pop("_iters", None)
     p.close()
     print(json.dumps(out), flush=True)
+
+
+MAPS_PER_LAUNCH = {"tile_map12": 2, "tile_ex3_map12": 3, "class_pass": None,
+                   "transcript_pass": None}
+
+
+def map_bytes(cls):
+    """Algorithmic bytes of one EM map: 8Z + 16C + 24T (row_ptr 4C, col 4Z,
+    n 4C, rho write 4C, lambda 4T; col_ptr 4T, row 4Z, rho read 4C, lambda
+    4T; M-step u 4T, w 4T, lambda' 4T)."""
+    return 8.0 * cls.nnz + 16.0 * cls.n_cls + 24.0 * cls.n_tx
+
+
+def batch_map_bytes(cls, reps):
+    """The same for one map of `reps` replicates sharing the structure:
+    8Z + 4C + 8T once, 12C + 16T per replicate."""
+    return (8.0 * cls.nnz + 4.0 * cls.n_cls + 8.0 * cls.n_tx
+            + reps * (12.0 * cls.n_cls + 16.0 * cls.n_tx))
+
+
+def cpu_info():
+    """The host's processor: model, clock, logical CPUs and the CPUs this
+    process may run on."""
+    model, mhz = None, None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if model is None and line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+            if mhz is None and line.startswith("cpu MHz"):
+                mhz = float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count()
+    return {"cpu_model": model, "cpu_mhz": mhz, "nproc": os.cpu_count(),
+            "cpus_available": aff}
 
 
 def traffic_from_profile(kind, args):
@@ -532,6 +595,7 @@ def bench_bootstrap(args, H, np, p, cls, u, sk):
                          iters_total=int(b["iters"].sum()),
                          chunk=st["chunk"], n_tiles=st["n_tiles"],
                          exact_sum_reruns=st["sum_reruns"])
+        res[name]["_iters"] = b["iters"]
         if name == "fp64-exact":
             res["_est_exact"] = b["est"]
         log("bootstrap %-10s R=%d %.3fs (%.1f reps/s) resample %.1f ms "
@@ -569,31 +633,76 @@ def bench_bootstrap(args, H, np, p, cls, u, sk):
                 "%.2f ms; counts bitwise %s" % (dtc * 1e3,
                                                 st["ms_resample"] / R, ok))
     # unbatched: the same resampled counts through the plain solve, one
-    # replicate after another (a sample of n_seq, scaled to R; 0 skips it)
-    if args.seq_sample <= 0:
+    # replicate after another (all R unless --seq-sample asks for fewer)
+    if args.seq_sample == 0:
         return res
-    n_seq = min(args.seq_sample, R)
-    uo, co = p.bootstrap_counts(u, seeds[:n_seq])
+    n_seq = R if args.seq_sample < 0 else min(args.seq_sample, R)
+    uo, co = p.bootstrap_counts(u, seeds)
     for name, nm in bmodes:
+        its = []
         t = time.perf_counter()
         for r in range(n_seq):
             p.set_counts(co[r])
-            p.solve(uo[r], norm=nm, **sk)
-        dt = (time.perf_counter() - t) / n_seq
-        res[name]["sequential_plain_s_per_rep"] = dt
-        res[name]["sequential_plain_reps_per_s"] = 1.0 / dt
-        res[name]["speedup_vs_sequential_gpu"] = res[name]["reps_per_s"] * dt
+            its.append(p.solve(uo[r], norm=nm, **sk)["stats"]["iters"])
+        dt = (time.perf_counter() - t) * R / n_seq
+        its = np.array(its)
+        res[name]["sequential_plain_s"] = dt
+        res[name]["sequential_plain_reps_per_s"] = R / dt
+        res[name]["speedup_vs_sequential_gpu"] = dt / res[name]["seconds"]
+        res[name]["sequential_iters"] = {
+            "min": int(its.min()), "max": int(its.max()),
+            "total": int(its.sum()),
+            "same_as_batched": bool(np.array_equal(
+                its, res[name]["_iters"][:n_seq]))}
+        log("sequential %-10s %d replicates one by one: %.3fs (x%.2f); "
+            "iters %d..%d total %d" % (name, n_seq, dt,
+                                       res[name]["speedup_vs_sequential_gpu"],
+                                       its.min(), its.max(), its.sum()))
     p.set_counts(cls.cls_count)
-    res["sequential_sample"] = "%d replicates solved one by one (hem_solve " \
-                               "after hem_problem_set_counts), scaled" % n_seq
+    res["sequential_sample"] = (
+        "%d of %d replicates solved one by one (hem_problem_set_counts, "
+        "then hem_solve), whole calls timed%s"
+        % (n_seq, R, "" if n_seq == R else ", scaled to all"))
+    # the batched kernels launch by launch (HIP events) on the same counts
+    for name, nm in bmodes:
+        pr = p.profile_batch(uo, co, norm=nm, **sk)
+        kern = {}
+        for k in ("tile_map12", "norm_pass1", "norm_pass2", "tile_ex3",
+                  "iter_end"):
+            d = pr[k]
+            if d["launches"]:
+                kern[k] = dict(d, GBps=d["bytes_total"]
+                               / (d["ms_total"] * 1e-3) / 1e9)
+        ex = pr["tile_ex3"]
+        ach = ex["bytes_total"] / (ex["ms_total"] * 1e-3) / 1e9
+        res[name]["kernels"] = kern
+        res[name]["roofline"] = {
+            "bound": "hbm", "kernel": "k_tile_ex3",
+            "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_rule": "per launch 3 maps x (8Z + 4C + 8T + R_active "
+                          "(12C + 16T)), u32 indices and FP32 values, "
+                          "summed over launches",
+            "ms_per_replicate_iteration": ex["ms_total"] / pr["rep_iters"],
+            "launches": ex["launches"], "rep_iters": pr["rep_iters"]}
+        log("batched %-10s k_tile_ex3 %d launches %.1f ms: %.0f GB/s by "
+            "algorithmic bytes (%.1f%% of peak), %.1f us per "
+            "replicate-iteration" % (
+                name, ex["launches"], ex["ms_total"], ach,
+                100 * ach / HBM_PEAK_GBS,
+                1e3 * ex["ms_total"] / pr["rep_iters"]))
     return res
 
 
 def bootstrap_cpu(args, np, O, cls, u, boot):
     """The oracle on hera's own bootstrap schedule (batches of n threads,
-    joined per batch; each thread resamples and solves), on one batch."""
-    import os
-    nthr = min(args.cpu_boot_threads, os.cpu_count() or 1)
+    joined per batch; each thread resamples and solves), on one batch of
+    as many replicates as threads.  Threads: the CPUs this process may use,
+    at most --cpu-boot-threads (16, the host share of one GPU on the
+    benchmark machines)."""
+    info = cpu_info()
+    nthr = max(1, min(args.cpu_boot_threads, info["cpus_available"] or 1))
     seeds = np.arange(nthr, dtype=np.uint64) + 1
     t0 = O.now()
     est, ks = O.bootstrap_resampled(cls, u, seeds, nthr,
@@ -601,12 +710,13 @@ def bootstrap_cpu(args, np, O, cls, u, boot):
                                     min_round=args.min_iter,
                                     max_round=args.max_iter)
     dt = O.now() - t0
-    boot["cpu_baseline"] = {
+    boot["cpu_baseline"] = dict({
         "value": nthr / dt, "unit": "bootstrap replicates/s",
         "cores": nthr, "kind": "port",
-        "sample": "one batch of %d replicates (seeds 1..%d) on %d threads: "
-                  "resample + FP64 restatement of src/EM.c, %.1fs"
-                  % (nthr, nthr, nthr, dt)}
+        "sample": "one batch of %d replicates (seeds 1..%d, iterations "
+                  "%d..%d) on %d threads: resample + FP64 restatement of "
+                  "src/EM.c, %.1fs" % (nthr, nthr, int(ks.min()),
+                                       int(ks.max()), nthr, dt)}, **info)
     gpu = boot.pop("_est_exact", None)
     same = [gpu is not None and bool(np.array_equal(gpu[r], est[r]))
             for r in range(nthr)]

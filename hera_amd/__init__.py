@@ -347,6 +347,30 @@ class Problem:
             it = it.reshape(self.n_samples, R)
         return dict(est=est, tpm=tpm, iters=it, stats=self._bstats(st))
 
+    def profile_batch(self, unique, cls_count=None, **kw):
+        """Per-kernel device time and algorithmic bytes of the batched
+        solves of solve_batch (hem_profile_batch).  Returns {kind:
+        dict(launches, ms_total, bytes_total)} plus iters, rep_iters,
+        chunk, n_tiles, sum_reruns."""
+        u = as_u32(unique)
+        R = u.shape[0]
+        assert u.shape == (R, self.n_tx)
+        cc = None
+        if cls_count is not None:
+            cc = as_f64(cls_count)
+            assert cc.shape == (R, self.classes.n_cls)
+        o = opts(**kw)
+        pr = _lib.HemBatchProfile()
+        check(_lib.load().hem_profile_batch(
+            self._h, R, u32p(u), f64p(cc) if cc is not None else None, o,
+            pr))
+        out = dict(iters=pr.iters, rep_iters=pr.rep_iters, chunk=pr.chunk,
+                   n_tiles=pr.n_tiles, sum_reruns=pr.sum_reruns)
+        for i, name in enumerate(_lib.BATCH_KIND_NAMES):
+            out[name] = dict(launches=pr.launches[i], ms_total=pr.ms[i],
+                             bytes_total=pr.bytes[i])
+        return out
+
     def bootstrap(self, unique, seeds, **kw):
         """hera's -b bootstrap, multinomial over all fragments; one seed per
         replicate.  Returns dict(est, tpm, iters, stats)."""

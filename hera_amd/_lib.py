@@ -62,6 +62,20 @@ class HemKernelProfile(C.Structure):
                 ("bytes", C.c_double * HEM_KERNEL_KINDS)]
 
 
+HEM_BATCH_KINDS = 5
+BATCH_KIND_NAMES = ["tile_map12", "norm_pass1", "norm_pass2",
+                    "tile_ex3", "iter_end"]
+
+
+class HemBatchProfile(C.Structure):
+    _fields_ = [("iters", C.c_uint32), ("chunk", C.c_uint32),
+                ("n_tiles", C.c_uint32), ("sum_reruns", C.c_uint32),
+                ("rep_iters", C.c_uint64),
+                ("launches", C.c_uint32 * HEM_BATCH_KINDS),
+                ("ms", C.c_double * HEM_BATCH_KINDS),
+                ("bytes", C.c_double * HEM_BATCH_KINDS)]
+
+
 class HemBatchStats(C.Structure):
     _fields_ = [("n_rep", C.c_uint32), ("iters_min", C.c_uint32),
                 ("iters_max", C.c_uint32), ("n_converged", C.c_uint32),
@@ -129,6 +143,9 @@ SIGNATURES = {
                                  _f64p, _u32p]),
     "hem_profile_solve": (C.c_int, [C.c_void_p, _u32p, C.POINTER(HemOpts),
                                     C.POINTER(HemKernelProfile)]),
+    "hem_profile_batch": (C.c_int, [C.c_void_p, C.c_uint32, _u32p, _f64p,
+                                    C.POINTER(HemOpts),
+                                    C.POINTER(HemBatchProfile)]),
     "hem_problem_tiles": (C.c_int, [C.c_void_p, _u32p, _u32p, _u32p]),
     "hem_classes_tiling": (C.c_int, [C.POINTER(HemClasses), _u32p, _u32p,
                                      _u32p]),

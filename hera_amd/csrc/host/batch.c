@@ -303,10 +303,28 @@ static hemk_spec spec_batch(hem_problem *p, hemk_rctl rc, int pairs, int xf,
 
 /* ---- the iteration ------------------------------------------------------ */
 
+/* Optional per-launch timing of the batched loop: an event after every
+ * launch (hem_profile_batch). */
+typedef struct {
+	void *ev[16];
+	int kind[16];
+	int n;
+} bprof;
+
+#define BMARK(kd)                                                       \
+	do {                                                            \
+		if (pc) {                                               \
+			HEM_TRY_DEV(hemd_event_record(pc->ev[pc->n],    \
+			                              p->stream),       \
+			            "event");                           \
+			pc->kind[pc->n++] = (kd);                       \
+		}                                                       \
+	} while (0)
+
 /* The first iteration's u1, u2 and r/v rows (or tree partials); every
  * later iteration's come from the kernel that ends the one before it. */
 static int batch_prologue(hem_problem *p, const hem_opts *o, hemk_rctl rc,
-                          int chunk)
+                          int chunk, bprof *pc)
 {
 	hem_batch_ws *w = &p->bw;
 	const int exact = o->norm != HEM_NORM_TREE;
@@ -316,6 +334,7 @@ static int batch_prologue(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 	                            w->B, w->C, exact ? NULL : w->part, xr, xv,
 	                            p->n_tx, o->step, p->stream),
 	            "tile map12");
+	BMARK(HEM_BK_MAP12);
 	return HEM_OK;
 }
 
@@ -323,7 +342,7 @@ static int batch_prologue(hem_problem *p, const hem_opts *o, hemk_rctl rc,
  * in B and C: the norms, then one kernel that extrapolates, maps with the
  * stop test and maps twice more for the next iteration. */
 static int batch_iteration(hem_problem *p, const hem_opts *o, hemk_rctl rc,
-                           int chunk)
+                           int chunk, bprof *pc)
 {
 	hem_batch_ws *w = &p->bw;
 	void *s = p->stream;
@@ -347,32 +366,40 @@ static int batch_iteration(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 		                          NULL, sd ? sums1 : rc.q_r);
 		HEM_TRY_DEV(hemk_spec_sum(&sp, 0, NULL, NULL, NULL, NULL, s),
 		            "batch exact sums 1");
+		BMARK(HEM_BK_NORM1);
 		if (sd) {
 			sp = spec_batch(p, rc, 1, HEMK_SPEC_CENTRED, sums1,
 			                rc.q_r);
 			HEM_TRY_DEV(hemk_spec_sum(&sp, 0, NULL, NULL, NULL, NULL,
 			                          s),
 			            "batch exact sums 2");
+			BMARK(HEM_BK_NORM2);
 		}
 	} else if (exact) {
 		if (o->step == HEM_STEP_SD) {
 			HEM_TRY_DEV(hemk_batch_chain(rc, p->n_tx, xr, xv, 1, s),
 			            "batch chain 1");
+			BMARK(HEM_BK_NORM1);
 			HEM_TRY_DEV(hemk_batch_chain(rc, p->n_tx, xr, xv, 2, s),
 			            "batch chain 2");
+			BMARK(HEM_BK_NORM2);
 		} else {
 			HEM_TRY_DEV(hemk_batch_chain(rc, p->n_tx, xr, xv, 3, s),
 			            "batch chain");
+			BMARK(HEM_BK_NORM1);
 		}
 	} else {
 		HEM_TRY_DEV(hemk_batch_alpha(&p->dt, rc, w->part, p->n_tx, st, s),
 		            "batch alpha");
+		BMARK(HEM_BK_NORM1);
 	}
 	HEM_TRY_DEV(hemk_tile_ex3(&p->dt, rc, chunk, w->cnt, 4, w->uq, w->A,
 	                          w->B, w->C, p->n_tx, st, 1,
 	                          exact ? NULL : w->part, xr, xv, s),
 	            "tile ex3");
+	BMARK(HEM_BK_EX3);
 	HEM_TRY_DEV(hemk_batch_iter_end(rc, s), "batch iter end");
+	BMARK(HEM_BK_ITER_END);
 	return HEM_OK;
 }
 
@@ -405,7 +432,7 @@ static int batch_graph(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 	if (e)
 		return hem_dev_fail(e, "batch graph capture");
 	for (int i = 0; i < HEM_ITERS_PER_GRAPH; ++i) {
-		rc2 = batch_iteration(p, o, rc, chunk);
+		rc2 = batch_iteration(p, o, rc, chunk, NULL);
 		if (rc2) {
 			hemd_graph_end(p->stream, &exec);
 			hemd_graph_destroy(exec);
@@ -492,7 +519,7 @@ static int run_batch(hem_problem *p, const hem_opts *o, uint32_t R,
 	HEM_TRY_DEV(hemk_batch_reset(rc, p->n_tx, w->A, p->stream), "reset");
 	HEM_TRY_DEV(hemk_fill_u32(reruns_word(p), 1, 0, p->stream),
 	            "rerun counter");
-	if ((rc2 = batch_prologue(p, o, rc, chunk)))
+	if ((rc2 = batch_prologue(p, o, rc, chunk, NULL)))
 		return rc2;
 	if ((rc2 = batch_graph(p, o, rc, chunk, &graph)))
 		return rc2;
@@ -597,27 +624,24 @@ static int load_batch_opts(const hem_opts *opts, hem_opts *o)
 	return HEM_OK;
 }
 
-int hem_solve_batch(hem_problem *p, uint32_t n_rep, const uint32_t *unique,
-                    const double *cls_count, const hem_opts *opts, double *est,
-                    double *tpm, uint32_t *iters, hem_batch_stats *st)
+/* The counts of n_rep solves as rows [r][T + C] (unique counts, then class
+ * counts as u32), checked; NULL with the error set on failure. */
+static uint32_t *pack_counts(const hem_problem *p, uint32_t n_rep,
+                             const uint32_t *unique, const double *cls_count)
 {
-	hem_opts o;
-	int rc;
-	const double t0 = now_ms();
-	const uint32_t T = p ? p->n_tx : 0, C = p ? p->n_cls : 0;
+	const uint32_t T = p->n_tx, C = p->n_cls;
 	const size_t K = (size_t)T + C;
-	uint32_t *h = NULL;
-	if (!p || !unique || n_rep == 0)
-		return hem_fail(HEM_ERR_INVALID, "null problem/counts or n_rep 0");
-	load_batch_opts(opts, &o);
-	if ((rc = check_batch_opts(p, &o)))
-		return rc;
-	if (!cls_count && !p->counts_integral)
-		return hem_fail(HEM_ERR_INVALID,
-		                "class counts must be integers below 2^32");
+	uint32_t *h;
+	if (!cls_count && !p->counts_integral) {
+		hem_fail(HEM_ERR_INVALID,
+		         "class counts must be integers below 2^32");
+		return NULL;
+	}
 	h = malloc(K * n_rep * 4);
-	if (!h)
-		return hem_fail(HEM_ERR_NOMEM, "host allocation");
+	if (!h) {
+		hem_fail(HEM_ERR_NOMEM, "host allocation");
+		return NULL;
+	}
 	for (uint32_t r = 0; r < n_rep; ++r) {
 		uint32_t *row = h + (size_t)r * K;
 		memcpy(row, unique + (size_t)r * T, (size_t)T * 4);
@@ -626,29 +650,198 @@ int hem_solve_batch(hem_problem *p, uint32_t n_rep, const uint32_t *unique,
 			                           : p->h_cls_count[c];
 			if (!(x >= 0) || x > 4294967295.0 || x != floor(x)) {
 				free(h);
-				return hem_fail(HEM_ERR_INVALID,
-				                "class count %g (replicate %u, class "
-				                "%u) is not an integer below 2^32",
-				                x, r, c);
+				hem_fail(HEM_ERR_INVALID,
+				         "class count %g (replicate %u, class "
+				         "%u) is not an integer below 2^32",
+				         x, r, c);
+				return NULL;
 			}
 			row[T + c] = (uint32_t)x;
 		}
 	}
-	pthread_mutex_lock(&p->mu);
+	return h;
+}
+
+/* Uploads packed counts into the workspace (sized for n_rep) and stages
+ * them in tile order; p->mu held. */
+static int upload_counts(hem_problem *p, uint32_t n_rep, const uint32_t *h)
+{
+	const size_t K = (size_t)p->n_tx + p->n_cls;
+	int rc;
 	if ((rc = ws_ensure(p, n_rep)))
-		goto out;
+		return rc;
 	rc = hemd_h2d(p->bw.cat_rm, h, K * n_rep * 4, p->stream);
-	if (rc) {
-		rc = hem_dev_fail(rc, "count upload");
-		goto out;
-	}
-	if ((rc = stage_counts(p, n_rep)))
+	if (rc)
+		return hem_dev_fail(rc, "count upload");
+	return stage_counts(p, n_rep);
+}
+
+int hem_solve_batch(hem_problem *p, uint32_t n_rep, const uint32_t *unique,
+                    const double *cls_count, const hem_opts *opts, double *est,
+                    double *tpm, uint32_t *iters, hem_batch_stats *st)
+{
+	hem_opts o;
+	int rc;
+	const double t0 = now_ms();
+	uint32_t *h = NULL;
+	if (!p || !unique || n_rep == 0)
+		return hem_fail(HEM_ERR_INVALID, "null problem/counts or n_rep 0");
+	load_batch_opts(opts, &o);
+	if ((rc = check_batch_opts(p, &o)))
+		return rc;
+	if (!(h = pack_counts(p, n_rep, unique, cls_count)))
+		return hem_last_code();
+	pthread_mutex_lock(&p->mu);
+	if ((rc = upload_counts(p, n_rep, h)))
 		goto out;
 	rc = run_batch(p, &o, n_rep, est, tpm, iters, st);
 	if (!rc && st)
 		st->ms_total = now_ms() - t0;
 out:
 	pthread_mutex_unlock(&p->mu);
+	free(h);
+	return rc;
+}
+
+/* Algorithmic bytes of one launch over `act` active replicates, with u32
+ * indices and FP32 values: per EM map the structure (8Z + 4C + 8T) once
+ * and each replicate's counts, densities and outputs (12C + 16T); each
+ * exact-sum pass reads the r and v rows of every active replicate. */
+static double batch_launch_bytes(const hem_problem *p, int kind, double act)
+{
+	const double T = p->n_tx, C = p->n_cls, Z = (double)p->nnz;
+	const double shared = 8 * Z + 4 * C + 8 * T, per = 12 * C + 16 * T;
+	switch (kind) {
+	case HEM_BK_MAP12:
+		return 2 * (shared + act * per);
+	case HEM_BK_EX3:
+		return 3 * (shared + act * per);
+	case HEM_BK_NORM1:
+	case HEM_BK_NORM2:
+		return 16 * T * act;
+	default:
+		return 8 * act;
+	}
+}
+
+int hem_profile_batch(hem_problem *p, uint32_t n_rep, const uint32_t *unique,
+                      const double *cls_count, const hem_opts *opts,
+                      hem_batch_profile *out)
+{
+	hem_opts o;
+	bprof pc;
+	void *ev_start = NULL;
+	uint32_t *h = NULL, act_h[1];
+	int rc = HEM_OK, i, e;
+	if (!p || !unique || n_rep == 0 || !out)
+		return hem_fail(HEM_ERR_INVALID, "null argument or n_rep 0");
+	load_batch_opts(opts, &o);
+	if ((rc = check_batch_opts(p, &o)))
+		return rc;
+	if (!(h = pack_counts(p, n_rep, unique, cls_count)))
+		return hem_last_code();
+	memset(out, 0, sizeof(*out));
+	memset(&pc, 0, sizeof(pc));
+	for (i = 0; i < 16 && !rc; ++i)
+		rc = hemd_event_create(&pc.ev[i]);
+	if (!rc)
+		rc = hemd_event_create(&ev_start);
+	if (rc) {
+		rc = hem_dev_fail(rc, "event creation");
+		goto free_ev;
+	}
+	pthread_mutex_lock(&p->mu);
+	if ((rc = upload_counts(p, n_rep, h)))
+		goto out;
+	{
+		hem_batch_ws *w = &p->bw;
+		hemk_rctl bc = rctl_for(p, n_rep, &o);
+		const int chunk = hemk_tile_chunk(&p->dt, n_rep, 4);
+		const uint32_t limit = o.max_iter > o.min_iter ? o.max_iter
+		                                                : o.min_iter;
+		uint32_t act = n_rep * p->ns;
+		if (chunk <= 0) {
+			rc = hem_fail(HEM_ERR_RANGE,
+			              "tiles do not fit the LDS budget");
+			goto out;
+		}
+		bc.chunk = (uint32_t)chunk;
+		e = hemk_batch_reset(bc, p->n_tx, w->A, p->stream);
+		if (!e)
+			e = hemk_fill_u32(reruns_word(p), 1, 0, p->stream);
+		if (e) {
+			rc = hem_dev_fail(e, "reset");
+			goto out;
+		}
+		for (int it = -1;; ++it) {
+			pc.n = 0;
+			e = hemd_event_record(ev_start, p->stream);
+			if (e) {
+				rc = hem_dev_fail(e, "event");
+				goto out;
+			}
+			rc = it < 0 ? batch_prologue(p, &o, bc, chunk, &pc)
+			            : batch_iteration(p, &o, bc, chunk, &pc);
+			if (rc)
+				goto out;
+			e = hemd_d2h(w->h_active, bc.active,
+			             (size_t)p->ns * 4, p->stream);
+			if (!e)
+				e = hemd_stream_sync(p->stream);
+			if (e) {
+				rc = hem_dev_fail(e, "profiled iteration");
+				goto out;
+			}
+			for (i = 0; i < pc.n; ++i) {
+				float ms = 0;
+				e = hemd_event_elapsed(i ? pc.ev[i - 1] : ev_start,
+				                       pc.ev[i], &ms);
+				if (e) {
+					rc = hem_dev_fail(e, "event time");
+					goto out;
+				}
+				out->ms[pc.kind[i]] += ms;
+				out->launches[pc.kind[i]]++;
+				out->bytes[pc.kind[i]] +=
+				    batch_launch_bytes(p, pc.kind[i], act);
+			}
+			if (it < 0)
+				continue;
+			out->iters++;
+			out->rep_iters += act;
+			act = 0;
+			for (uint32_t q = 0; q < p->ns; ++q)
+				act += w->h_active[q];
+			if (act == 0)
+				break;
+			if (out->iters > limit + 1) {
+				rc = hem_fail(HEM_ERR_DEVICE,
+				              "profiled loop did not terminate");
+				goto out;
+			}
+		}
+		out->chunk = (uint32_t)chunk;
+		out->n_tiles = p->dt.n_tiles;
+		{
+			/* the exact sums' reruns, as run_batch reports them */
+			e = hemd_d2h(act_h, reruns_word(p), 4, p->stream);
+			if (!e)
+				e = hemd_stream_sync(p->stream);
+			if (e) {
+				rc = hem_dev_fail(e, "rerun readback");
+				goto out;
+			}
+			out->sum_reruns = act_h[0];
+		}
+	}
+out:
+	pthread_mutex_unlock(&p->mu);
+free_ev:
+	for (i = 0; i < 16; ++i)
+		if (pc.ev[i])
+			hemd_event_destroy(pc.ev[i]);
+	if (ev_start)
+		hemd_event_destroy(ev_start);
 	free(h);
 	return rc;
 }

@@ -7,12 +7,16 @@
 #include "hem_internal.h"
 
 static __thread char hem_msg[512];
+static __thread int hem_code;
 
 const char *hem_last_error(void) { return hem_msg; }
+
+int hem_last_code(void) { return hem_code; }
 
 int hem_fail(int code, const char *fmt, ...)
 {
 	va_list ap;
+	hem_code = code;
 	va_start(ap, fmt);
 	vsnprintf(hem_msg, sizeof(hem_msg), fmt, ap);
 	va_end(ap);

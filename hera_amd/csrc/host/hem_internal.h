@@ -208,6 +208,9 @@ static inline uint64_t hem_tile_rec_bytes(uint32_t nt, uint32_t nc,
 /* kernel kinds for per-kernel timing (hem_kernel_profile) */
 enum { HEM_PK_DENOM = 0, HEM_PK_GATHER, HEM_PK_NORM1, HEM_PK_NORM2,
        HEM_PK_EXTRAP, HEM_PK_ITER_END, HEM_PK_TMAP12, HEM_PK_TEX3 };
+/* kernel kinds of hem_profile_batch */
+enum { HEM_BK_MAP12 = 0, HEM_BK_NORM1, HEM_BK_NORM2, HEM_BK_EX3,
+       HEM_BK_ITER_END };
 /* Largest dynamic LDS of the plain tile kernels; larger tiles keep to the
  * two-pass kernels. */
 #define HEM_PTILE_LDS_MAX (64 * 1024)
@@ -225,6 +228,8 @@ int hem_tiles_upload(hem_problem *p, const hem_tiles_host *h, hemk_tiles *dt,
 
 /* error.c */
 int hem_fail(int code, const char *fmt, ...);
+/* The code of the calling thread's last hem_fail. */
+int hem_last_code(void);
 int hem_dev_fail(int hip_code, const char *what);
 int hem_device_ready(void);
 

@@ -285,6 +285,30 @@ typedef struct {
 int hem_profile_solve(hem_problem *p, const uint32_t *unique,
                       const hem_opts *o, hem_kernel_profile *out);
 
+/* The same for a batch: the solves of hem_solve_batch (same arguments),
+ * one iteration at a time outside the graph with a HIP event after every
+ * launch, reporting per kernel kind the launches, device time and
+ * algorithmic bytes summed over launches.  Bytes count u32 indices and
+ * FP32 values: per EM map the class structure once, 8 nnz + 4 n_cls +
+ * 8 n_tx, plus 12 n_cls + 16 n_tx for each replicate the launch advances;
+ * an exact-sum pass 16 n_tx per replicate.  Kinds: 0 first iteration's
+ * maps u -> u1 -> u2 (2 maps), 1 norm pass 1, 2 norm pass 2 (exact sums,
+ * chains or tree alpha), 3 extrapolated map with the stop test and the
+ * next iteration's two maps (3 maps), 4 iteration end. */
+#define HEM_BATCH_KINDS 5
+typedef struct {
+	uint32_t iters;        /* iterations run (the slowest replicate's k)  */
+	uint32_t chunk, n_tiles, sum_reruns;
+	uint64_t rep_iters;    /* replicate-iterations (sum of every k)       */
+	uint32_t launches[HEM_BATCH_KINDS];
+	double   ms[HEM_BATCH_KINDS];     /* summed device time per kind      */
+	double   bytes[HEM_BATCH_KINDS];  /* algorithmic bytes, all launches  */
+} hem_batch_profile;
+
+int hem_profile_batch(hem_problem *p, uint32_t n_rep, const uint32_t *unique,
+                      const double *cls_count, const hem_opts *o,
+                      hem_batch_profile *out);
+
 /* Device-to-device copy of `bytes` repeated `reps` times on a fresh stream:
  * read + write bytes per second.  No counterpart in the reference; the
  * achievable-bandwidth reference for the kernels' figures. */

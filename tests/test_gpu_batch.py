@@ -273,3 +273,38 @@ def test_tile_limit_component_bitwise():
     p.close()
     say("limit tile: chunk %d" % res["stats"]["chunk"])
     check_replicates(cls, U, CC, res)
+
+
+def test_profile_batch_follows_the_solve(cfg1):
+    """hem_profile_batch runs the solves of hem_solve_batch launch by
+    launch: as many iterations as the slowest replicate's k, as many
+    replicate-iterations as all k together, one extrapolated-map launch
+    per iteration and the first maps once."""
+    cls, u = cfg1
+    U, CC = perturbed(cls, u, 6, 77)
+    p = H.Problem(cls)
+    try:
+        res = p.solve_batch(U, CC, stop_rel_change=1e-6)
+        pr = p.profile_batch(U, CC, stop_rel_change=1e-6)
+        pt = p.profile_batch(U, CC, stop_rel_change=1e-6, norm=H.NORM_TREE)
+        rt = p.solve_batch(U, CC, stop_rel_change=1e-6, norm=H.NORM_TREE)
+    finally:
+        p.close()
+    it = res["iters"]
+    say("iters %s; profile %d iterations, %d replicate-iterations" % (
+        it.tolist(), pr["iters"], pr["rep_iters"]))
+    for prof, its in ((pr, it), (pt, rt["iters"])):
+        assert prof["iters"] == its.max()
+        assert prof["rep_iters"] == its.sum()
+        assert prof["tile_map12"]["launches"] == 1
+        assert prof["tile_ex3"]["launches"] == prof["iters"]
+        assert prof["iter_end"]["launches"] == prof["iters"]
+        assert prof["tile_ex3"]["ms_total"] > 0
+        # 3 maps per launch: 3 (8Z + 4C + 8T) per launch plus
+        # 3 (12C + 16T) per replicate-iteration
+        T, C, Z = cls.n_tx, cls.n_cls, cls.nnz
+        exp = (3 * (8 * Z + 4 * C + 8 * T) * prof["iters"]
+               + 3 * (12 * C + 16 * T) * prof["rep_iters"])
+        assert prof["tile_ex3"]["bytes_total"] == pytest.approx(exp,
+                                                                rel=1e-12)
+    assert pr["norm_pass2"]["launches"] == pr["iters"]

@@ -367,6 +367,24 @@ def traffic_from_profile(kind, args):
     return None
 
 
+def batch_traffic_from_profile(args, R, mode):
+    """Mean HBM bytes per launch of the batched tile kernel over the
+    bootstrap run, from the committed copy-calibrated FETCH_SIZE /
+    WRITE_SIZE summary of the default workload (exact mode, 100
+    replicates), or None for any other run."""
+    if (args.n_tx, args.n_cls, args.n_frag, args.seed, R, mode) != (
+            200000, 2000000, 60_000_000, 0, 100, "fp64-exact"):
+        return None
+    path = os.path.join(ROOT, "profiles",
+                        "bootstrap_tile_ex3_200k_tx_2M_cls_R100_counters.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch k_tile_ex3:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def bench_grad(args, H, np, p, cls, u, sk):
     """Config 4: the adjoint at the EM solution (theta = log(lambda len),
     log_efflen = log len), Hessian-vector products, and the CG standard
@@ -680,7 +698,7 @@ def bench_bootstrap(args, H, np, p, cls, u, sk):
             "bound": "hbm", "kernel": "k_tile_ex3",
             "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": batch_traffic_from_profile(args, R, name),
             "bytes_rule": "per launch 3 maps x (8Z + 4C + 8T + R_active "
                           "(12C + 16T)), u32 indices and FP32 values, "
                           "summed over launches",

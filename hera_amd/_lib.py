@@ -141,6 +141,9 @@ SIGNATURES = {
                                    C.c_uint32, _f64p, _f64p]),
     "hem_exact_sums": (C.c_int, [_f64p, C.c_uint32, C.c_uint32, C.c_int,
                                  _f64p, _u32p]),
+    "hem_exact_sums_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32,
+                                         C.c_int, C.c_void_p, C.c_void_p,
+                                         C.c_void_p]),
     "hem_profile_solve": (C.c_int, [C.c_void_p, _u32p, C.POINTER(HemOpts),
                                     C.POINTER(HemKernelProfile)]),
     "hem_profile_batch": (C.c_int, [C.c_void_p, C.c_uint32, _u32p, _f64p,

@@ -397,6 +397,20 @@ uint32_t hemk_spec_width(uint32_t L, int xf, uint32_t chunks);
 /* Candidate-table entries (tab doubles, nx bytes) that launches of up to
  * max_rows chains of P chunks each can use, at the width each picks. */
 size_t hemk_spec_tab_entries(uint32_t L, uint32_t max_rows, uint32_t P);
+/* The allocation-free sums' fixed device scratch: chunk sums and drifts
+ * (2 doubles), window base (8 bytes), candidates (HEMK_SPEC_W doubles) and
+ * links (HEMK_SPEC_W bytes) per chunk, and two sums per row, for up to
+ * HEMK_NOALLOC_CHUNKS chunks (rows x chunks per row), which also bounds the
+ * rows.  hemk_noalloc_scratch: its address on the current device (looked
+ * up on the first call on each device) and size. */
+#define HEMK_NOALLOC_CHUNKS 4096
+#define HEMK_NOALLOC_BYTES                                                   \
+	((size_t)HEMK_NOALLOC_CHUNKS * (3 * 8 + HEMK_SPEC_W * 9 + 2 * 8))
+#define HEMK_NOALLOC_DEVICES 64
+int hemk_noalloc_scratch(void **ptr, size_t *bytes);
+/* out[r] = sqrt(q[r] / (n - 1)) */
+int hemk_sd_finish(const double *q, uint32_t rows, uint32_t n, double *out,
+                   void *s);
 /* produce == 1 (plain solve, nchain 2, rows 0/1 of x): first writes
  * r = u1 - u and v = (u2 - u1) - r (src/EM.c:333-336) into x. */
 int hemk_spec_sum(const hemk_spec *sp, int produce, const double *u,

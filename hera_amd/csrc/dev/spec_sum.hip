@@ -34,6 +34,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "hem_dev.h"
 
 #define SBLOCK 256
@@ -801,9 +803,60 @@ int launch_xf(const hemk_spec &sp, int produce, int phases, const double *u,
 	return (int)hipGetLastError();
 }
 
+// out[r] = sqrt(q[r] / (n - 1)): the end of Norm (src/hash_align.c:161-163)
+// on the device, for the allocation-free sums.
+__global__ void k_sd_finish(const double *__restrict__ q, uint32_t rows,
+                            uint32_t n, double *__restrict__ out)
+{
+	const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+	if (r < rows)
+		out[r] = sqrt(q[r] / (double)(n - 1u));
+}
+
 } // namespace
 
+// Fixed device scratch of hem_exact_sums_noalloc: chunk sums, drifts and
+// window bases of up to HEMK_NOALLOC_CHUNKS chunks, their candidate tables
+// and links at the widest window, and two sums per row.  Every call on a
+// device uses that device's copy; the host reaches it only through the
+// address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[HEMK_NOALLOC_BYTES];
+
+static std::mutex scratch_mu;
+static void *scratch_dev[HEMK_NOALLOC_DEVICES];
+
 extern "C" {
+
+int hemk_noalloc_scratch(void **ptr, size_t *bytes)
+{
+	int dev = 0;
+	hipError_t e = hipGetDevice(&dev);
+	if (e != hipSuccess)
+		return (int)e;
+	if (dev < 0 || dev >= HEMK_NOALLOC_DEVICES)
+		return (int)hipErrorInvalidDevice;
+	std::lock_guard<std::mutex> g(scratch_mu);
+	if (!scratch_dev[dev]) {
+		void *p = nullptr;
+		e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess)
+			return (int)e;
+		scratch_dev[dev] = p;
+	}
+	*ptr = scratch_dev[dev];
+	*bytes = sizeof(scratch);
+	return 0;
+}
+
+int hemk_sd_finish(const double *q, uint32_t rows, uint32_t n, double *out,
+                   void *s)
+{
+	if (!rows)
+		return 0;
+	k_sd_finish<<<(rows + 255) / 256, 256, 0, (hipStream_t)s>>>(q, rows, n,
+	                                                           out);
+	return (int)hipGetLastError();
+}
 
 uint32_t hemk_spec_chunk(uint32_t n)
 {

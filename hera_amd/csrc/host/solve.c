@@ -755,3 +755,80 @@ out:
 		hemd_stream_destroy(s);
 	return rc;
 }
+
+/* The sums of hem_exact_sums over device rows, on the caller's stream, with
+ * the library's fixed device scratch instead of allocations. */
+int hem_exact_sums_noalloc(const double *x, uint32_t n, uint32_t n_rows,
+                           int mode, double *out, uint32_t *reruns,
+                           void *stream)
+{
+	hemk_spec sp;
+	void *mem = NULL;
+	size_t bytes = 0, cp;
+	uint32_t L, P;
+	double *csum, *drift, *res, *tab;
+	long long *base;
+	uint8_t *nx;
+	int rc, e;
+	if (!x || !out || n == 0 || n_rows == 0)
+		return hem_fail(HEM_ERR_INVALID, "null rows or empty size");
+	if (mode < HEM_SUM_PLAIN || mode > HEM_SUM_SQUARES)
+		return hem_fail(HEM_ERR_INVALID, "unknown sum mode %d", mode);
+	if (n > 0xFFFFFFFFu - HEMK_SPEC_LMAX)
+		return hem_fail(HEM_ERR_RANGE, "rows of < 2^32 values");
+	L = hemk_spec_chunk(n);
+	P = (n + L - 1) / L;
+	cp = (size_t)n_rows * P;
+	if (cp > HEMK_NOALLOC_CHUNKS)
+		return hem_fail(HEM_ERR_RANGE,
+		                "%u rows of %u values are %zu chunks of %u; the "
+		                "allocation-free sums take at most %d chunks",
+		                n_rows, n, cp, L, HEMK_NOALLOC_CHUNKS);
+	if ((rc = hem_device_ready()))
+		return rc;
+	e = hemk_noalloc_scratch(&mem, &bytes);
+	if (e)
+		return hem_dev_fail(e, "allocation-free scratch");
+	if (bytes < HEMK_NOALLOC_BYTES)
+		return hem_fail(HEM_ERR_DEVICE, "scratch of %zu bytes", bytes);
+	/* [C] chunk sums, [C] drifts, [C] bases, [2C] sums by row, [C][W]
+	 * candidates, [C][W] links; C = HEMK_NOALLOC_CHUNKS */
+	csum = (double *)mem;
+	drift = csum + HEMK_NOALLOC_CHUNKS;
+	base = (long long *)(drift + HEMK_NOALLOC_CHUNKS);
+	res = (double *)(base + HEMK_NOALLOC_CHUNKS);
+	tab = res + 2 * (size_t)HEMK_NOALLOC_CHUNKS;
+	nx = (uint8_t *)(tab + (size_t)HEMK_NOALLOC_CHUNKS * HEMK_SPEC_W);
+	memset(&sp, 0, sizeof(sp));
+	sp.n = n;
+	sp.L = L;
+	sp.P = P;
+	sp.nchain = n_rows;
+	sp.R = 1;
+	sp.ld = n;
+	sp.x = x;
+	sp.csum = csum;
+	sp.drift = drift;
+	sp.tab = tab;
+	sp.base = base;
+	sp.nx = nx;
+	sp.miss = reruns;
+	sp.xf = mode == HEM_SUM_SQUARES ? HEMK_SPEC_SQUARE : HEMK_SPEC_IDENT;
+	sp.W = hemk_spec_width(L, sp.xf, (uint32_t)cp);
+	sp.res = mode == HEM_SUM_NORM_SD ? res : out;
+	e = reruns ? hemk_fill_u32(reruns, 1, 0, stream) : 0;
+	if (!e)
+		e = hemk_spec_sum(&sp, 0, NULL, NULL, NULL, NULL, stream);
+	if (!e && mode == HEM_SUM_NORM_SD) {
+		/* second pass of Norm: centred squares about sum / n, then
+		 * sqrt(q / (n - 1)) */
+		sp.xf = HEMK_SPEC_CENTRED;
+		sp.W = hemk_spec_width(L, sp.xf, (uint32_t)cp);
+		sp.mres = res;
+		sp.res = res + n_rows;
+		e = hemk_spec_sum(&sp, 0, NULL, NULL, NULL, NULL, stream);
+		if (!e)
+			e = hemk_sd_finish(res + n_rows, n_rows, n, out, stream);
+	}
+	return e ? hem_dev_fail(e, "allocation-free exact sums") : HEM_OK;
+}

@@ -329,6 +329,22 @@ int hem_copy_bandwidth(size_t bytes, int reps, double *gb_per_s);
 int hem_exact_sums(const double *x, uint32_t n, uint32_t n_rows, int mode,
                    double *out, uint32_t *reruns);
 
+/* The same sums without any allocation, for callers that cannot allocate
+ * device memory: code under HIP graph capture, tight serving loops.  No
+ * counterpart in the reference.  x (n_rows * n values), out (n_rows) and
+ * reruns (one word, or NULL) are device pointers; the work is enqueued on
+ * `stream` (a hipStream_t; NULL: the default stream) and the call returns
+ * without waiting for it.  The scratch is a fixed array in device memory
+ * (4.9 MB, held on each device the library runs on until the process
+ * ends): n_rows * ceil(n / L) chunks, L = 256 for n <= 16384 and 1024
+ * above, must not exceed 4096 (HEM_ERR_RANGE otherwise).  Calls on one
+ * device share that array, so calls that overlap on a device race: the
+ * caller orders them (one stream, or events).  Results are bitwise those
+ * of hem_exact_sums. */
+int hem_exact_sums_noalloc(const double *x, uint32_t n, uint32_t n_rows,
+                           int mode, double *out, uint32_t *reruns,
+                           void *stream);
+
 /* ---- abundance tables ------------------------------------------------------
  * hera's text outputs of write_result (src/EM.c:152-205), from host arrays
  * (no device work).  names: n fixed-width blocks of name_len bytes (a

@@ -459,6 +459,8 @@ __global__ void k_ctl_reset(hemk_ctl *ctl, uint32_t min_iter,
 	ctl->fallbacks = 0;
 	ctl->reruns = 0;
 	ctl->stopbad = 0;
+	ctl->stopbad2 = 0;
+	ctl->fb = 0;
 	ctl->min_iter = min_iter;
 	ctl->max_iter = max_iter;
 	ctl->alpha = 0;

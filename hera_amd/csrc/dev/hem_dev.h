@@ -55,6 +55,10 @@ typedef struct {
 	uint32_t stopbad;     /* set (OR) by the third map when some
 	                         transcript fails the stop test; read and
 	                         cleared by the iteration end              */
+	uint32_t stopbad2;    /* monotonicity fallback: the same for the
+	                         second map (u2 against u1)                */
+	uint32_t fb;          /* monotonicity fallback: this iteration keeps
+	                         u2                                        */
 	double   alpha;       /* last step length                              */
 	double   stop_min, stop_rel;
 	double   mean_r, mean_v;  /* first pass of the sequential norms        */
@@ -332,6 +336,26 @@ int hemk_ptile_ex3_map12(const hemk_tiles *t, double *A, double *B,
                          double *xr, double *xv, hemk_ctl *ctl, void *s);
 int hemk_ptile_alpha(const double *part, uint32_t nparts, uint32_t T,
                      hemk_step st, hemk_ctl *ctl, void *s);
+/* Monotonicity fallback (no counterpart in the reference).  map12_fb: as
+ * map12, and the second map's stop test into ctl->stopbad2.  ex3_fb: from
+ * A, B, C extrapolate, lam = M(u') -> A with the stop test (ctl->stopbad),
+ * and per tile the terms of ell(lam) - ell(u2) (u2 = C) as
+ * log1p-differences: dpart[2 tile] = the tile's difference, dpart[2 tile
+ * + 1] = the sum of its terms' magnitudes.  fb_decide: totals over tiles;
+ * when the difference is below -tol times the magnitudes the iteration
+ * keeps u2 (ctl->fb, ctl->fallbacks, stop test of the second map).
+ * fb_apply: A = C when ctl->fb. */
+int hemk_ptile_map12_fb(const hemk_tiles *t, const double *A, double *B,
+                        double *C, const uint32_t *uq, const double *cntq,
+                        double *xr, double *xv, hemk_ctl *ctl, void *s);
+int hemk_ptile_ex3_fb(const hemk_tiles *t, double *A, const double *B,
+                      const double *C, const uint32_t *uq,
+                      const double *cntq, const double *part, uint32_t T,
+                      hemk_step st, double *dpart, hemk_ctl *ctl, void *s);
+int hemk_fb_decide(const double *dpart, uint32_t n_tiles, double tol,
+                   hemk_ctl *ctl, void *s);
+int hemk_fb_apply(uint32_t T, const double *C, double *A,
+                  const hemk_ctl *ctl, void *s);
 int hemk_ptile_rows(uint32_t T, const uint32_t *txq, const double *A,
                     double *out, void *s);
 

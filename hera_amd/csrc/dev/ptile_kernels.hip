@@ -234,13 +234,14 @@ __device__ __forceinline__ void pmap(const PLds &L, const PTile &tc,
 }
 
 // u1 = M(A) -> B, u2 = M(u1) -> C; r = u1 - u, v = (u2 - u1) - r into the
-// rows xr, xv (transcript order) when xr != NULL.
-template <bool W>
+// rows xr, xv (transcript order) when xr != NULL.  FB (monotonicity
+// fallback): the second map's stop test is ORed into ctl->stopbad2.
+template <bool W, bool FB>
 __global__ __launch_bounds__(PBLOCK) void k_ptile_map12(
     hemk_tiles tl, const double *__restrict__ A, double *__restrict__ B,
     double *__restrict__ C, const uint32_t *__restrict__ uq,
     const double *__restrict__ cntq, double *__restrict__ xr,
-    double *__restrict__ xv, const hemk_ctl *__restrict__ ctl)
+    double *__restrict__ xv, hemk_ctl *__restrict__ ctl)
 {
 #pragma clang fp contract(off)
 	extern __shared__ double smem[];
@@ -255,7 +256,13 @@ __global__ __launch_bounds__(PBLOCK) void k_ptile_map12(
 	__syncthreads();
 	int bad = 0;
 	pmap<W>(L, tc, L.l0, L.l1, false, 0.0, 0.0, bad);
-	pmap<W>(L, tc, L.l1, L.l2, false, 0.0, 0.0, bad);
+	pmap<W>(L, tc, L.l1, L.l2, FB, FB ? ctl->stop_min : 0.0,
+	        FB ? ctl->stop_rel : 0.0, bad);
+	if (FB) {
+		bad = __syncthreads_or(bad);
+		if (threadIdx.x == 0 && bad)
+			em_mark_bad(&ctl->stopbad2);
+	}
 	for (uint32_t i = threadIdx.x; i < tc.nt; i += PBLOCK) {
 		const double a = L.l0[i], b = L.l1[i], c = L.l2[i];
 		B[tc.tx_base + i] = b;
@@ -325,6 +332,167 @@ __global__ __launch_bounds__(PBLOCK) void k_ptile_ex3_map12(
 	}
 }
 
+// Class sums of lin in member order (weights applied), stored as they are
+// (a zero stays zero), for the fallback's log-likelihood terms.
+template <bool W>
+__device__ __forceinline__ void pclass_sums(const PLds &L, const PTile &tc,
+                                            const double *lin, double *D)
+{
+#pragma clang fp contract(off)
+	for (uint32_t c = threadIdx.x; c < tc.nc; c += PBLOCK) {
+		double acc = 0.0;
+		for (uint32_t k = L.coff[c]; k < L.coff[c + 1]; ++k) {
+			double x = lin[L.kmem[k] & 0x3Fu];
+			if constexpr (W)
+				x *= L.wc[k];
+			acc += x;
+		}
+		D[c] = acc;
+	}
+}
+
+// Sum over the block's 256 threads in a fixed order (wave butterflies,
+// then the four waves in turn); the result is valid in thread 0.
+__device__ __forceinline__ double pblock_sum(double x, double *red)
+{
+#pragma unroll
+	for (int m = 32; m >= 1; m >>= 1)
+		x += __shfl_xor(x, m, 64);
+	if (threadIdx.x % 64 == 0)
+		red[threadIdx.x / 64] = x;
+	__syncthreads();
+	double t = 0.0;
+	if (threadIdx.x == 0)
+		t = ((red[0] + red[1]) + red[2]) + red[3];
+	__syncthreads();
+	return t;
+}
+
+// Monotonicity fallback, first half (no counterpart in the reference): the
+// extrapolation and stabilising map of k_ptile_ex3_map12 without the next
+// iteration's maps, plus this tile's terms of ell(lam) - ell(u2): per class
+// n_c log1p((D_c(lam) - D_c(u2)) / D_c(u2)) (a zero sum at either point:
+// that point's n_c log D_c as it stands), per transcript u_t log1p((lam_t -
+// u2_t) / u2_t) and -len_t (lam_t - u2_t); dpart[2 tile] their sum and
+// dpart[2 tile + 1] the sum of their magnitudes.  D2 (class sums at u2)
+// lives d2 doubles into the dynamic LDS.
+template <bool W>
+__global__ __launch_bounds__(PBLOCK) void k_ptile_ex3_fb(
+    hemk_tiles tl, double *__restrict__ A, const double *__restrict__ B,
+    const double *__restrict__ C, const uint32_t *__restrict__ uq,
+    const double *__restrict__ cntq, const double *__restrict__ part,
+    uint32_t T, hemk_step st, double *__restrict__ dpart, size_t d2,
+    hemk_ctl *__restrict__ ctl)
+{
+#pragma clang fp contract(off)
+	extern __shared__ double smem[];
+	__shared__ double red[4];
+	if (ctl->done)
+		return;
+	const double alpha = st.alpha_given == 2
+	                         ? em_step_length(part[0], part[1], T, st)
+	                         : ctl->alpha;
+	const double smin = ctl->stop_min, srel = ctl->stop_rel;
+	const PTile tc = ptile(tl, blockIdx.x);
+	PLds L = pcarve(smem, tl);
+	double *D2 = smem + d2;
+	pfields<W>(L, tc);
+	pstage<W>(tl, tc, L, uq, cntq);
+	for (uint32_t i = threadIdx.x; i < tc.nt; i += PBLOCK) {
+		const size_t g = tc.tx_base + i;
+		const double c = C[g];
+		L.l2[i] = c;
+		L.l0[i] = em_extrapolate(A[g], B[g], c, alpha, st);
+	}
+	__syncthreads();
+	pclass_sums<W>(L, tc, L.l2, D2);
+	int bad = 0;
+	pmap<W>(L, tc, L.l0, L.l1, true, smin, srel, bad);
+	for (uint32_t i = threadIdx.x; i < tc.nt; i += PBLOCK)
+		A[tc.tx_base + i] = L.l1[i];
+	bad = __syncthreads_or(bad);
+	if (threadIdx.x == 0) {
+		if (bad)
+			em_mark_bad(&ctl->stopbad);
+		if (blockIdx.x == 0 && st.alpha_given == 2)
+			ctl->alpha = alpha;
+	}
+	pclass_sums<W>(L, tc, L.l1, L.D);
+	__syncthreads();
+	double d = 0.0, m = 0.0;
+	for (uint32_t c = threadIdx.x; c < tc.nc; c += PBLOCK) {
+		const double n = L.n[c], dl = L.D[c], dm = D2[c];
+		double x = 0.0;
+		if (n == 0.0)
+			continue;
+		if (dl > 0 && dm > 0)
+			x = n * log1p((dl - dm) / dm);
+		else if (dl > 0)
+			x = n * log(dl);
+		else if (dm > 0)
+			x = -(n * log(dm));
+		d += x;
+		m += fabs(x);
+	}
+	for (uint32_t t = threadIdx.x; t < tc.nt; t += PBLOCK) {
+		const double lam = L.l1[t], mu = L.l2[t];
+		if (L.u[t] != 0) {
+			const double x = (double)L.u[t] * log1p((lam - mu) / mu);
+			d += x;
+			m += fabs(x);
+		}
+		const double y = (double)L.len[t] * (lam - mu);
+		d -= y;
+		m += fabs(y);
+	}
+	d = pblock_sum(d, red);
+	m = pblock_sum(m, red);
+	if (threadIdx.x == 0) {
+		dpart[2 * (size_t)blockIdx.x] = d;
+		dpart[2 * (size_t)blockIdx.x + 1] = m;
+	}
+}
+
+// Monotonicity fallback, the decision: totals of the tiles' differences and
+// magnitudes (fixed order); the iteration keeps u2 when the difference is
+// below -tol times the magnitudes, with the second map's stop test.
+__global__ __launch_bounds__(256) void k_fb_decide(
+    const double *__restrict__ dpart, uint32_t n_tiles, double tol,
+    hemk_ctl *__restrict__ ctl)
+{
+	__shared__ double red[4];
+	if (ctl->done)
+		return;
+	double d = 0.0, m = 0.0;
+	for (uint32_t i = threadIdx.x; i < n_tiles; i += 256) {
+		d += dpart[2 * (size_t)i];
+		m += dpart[2 * (size_t)i + 1];
+	}
+	d = pblock_sum(d, red);
+	m = pblock_sum(m, red);
+	if (threadIdx.x == 0) {
+		const uint32_t fb = d < -tol * m ? 1u : 0u;
+		ctl->fb = fb;
+		if (fb) {
+			ctl->fallbacks += 1;
+			ctl->stopbad = ctl->stopbad2;
+		}
+		ctl->stopbad2 = 0;
+	}
+}
+
+// A = C (u2) when the decision kept u2.
+__global__ __launch_bounds__(256) void k_fb_apply(
+    uint32_t T, const double *__restrict__ C, double *__restrict__ A,
+    const hemk_ctl *__restrict__ ctl)
+{
+	if (ctl->done || !ctl->fb)
+		return;
+	for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < T;
+	     i += gridDim.x * 256)
+		A[i] = C[i];
+}
+
 // Tree norms: alpha from the block partials of the norm passes.
 __global__ __launch_bounds__(256) void k_ptile_alpha(
     const double *__restrict__ part, uint32_t nparts, uint32_t T,
@@ -385,14 +553,76 @@ int hemk_ptile_map12(const hemk_tiles *t, const double *A, double *B,
 {
 	if (!t->n_tiles || t->max_tx > 256)
 		return (int)hipErrorInvalidValue;
+	// the kernel writes no control word without FB
+	hemk_ctl *c = (hemk_ctl *)ctl;
 	if (t->weighted)
-		k_ptile_map12<true><<<t->n_tiles, PBLOCK, hemk_ptile_lds(t),
-		                      (hipStream_t)s>>>(*t, A, B, C, uq, cntq, xr,
-		                                        xv, ctl);
+		k_ptile_map12<true, false><<<t->n_tiles, PBLOCK,
+		                             hemk_ptile_lds(t), (hipStream_t)s>>>(
+		    *t, A, B, C, uq, cntq, xr, xv, c);
 	else
-		k_ptile_map12<false><<<t->n_tiles, PBLOCK, hemk_ptile_lds(t),
-		                       (hipStream_t)s>>>(*t, A, B, C, uq, cntq, xr,
-		                                         xv, ctl);
+		k_ptile_map12<false, false><<<t->n_tiles, PBLOCK,
+		                              hemk_ptile_lds(t), (hipStream_t)s>>>(
+		    *t, A, B, C, uq, cntq, xr, xv, c);
+	return (int)hipGetLastError();
+}
+
+int hemk_ptile_map12_fb(const hemk_tiles *t, const double *A, double *B,
+                        double *C, const uint32_t *uq, const double *cntq,
+                        double *xr, double *xv, hemk_ctl *ctl, void *s)
+{
+	if (!t->n_tiles || t->max_tx > 256)
+		return (int)hipErrorInvalidValue;
+	if (t->weighted)
+		k_ptile_map12<true, true><<<t->n_tiles, PBLOCK, hemk_ptile_lds(t),
+		                            (hipStream_t)s>>>(*t, A, B, C, uq, cntq,
+		                                              xr, xv, ctl);
+	else
+		k_ptile_map12<false, true><<<t->n_tiles, PBLOCK,
+		                             hemk_ptile_lds(t), (hipStream_t)s>>>(
+		    *t, A, B, C, uq, cntq, xr, xv, ctl);
+	return (int)hipGetLastError();
+}
+
+// LDS of the fallback's extrapolated map: the tile kernels' layout, then
+// the class sums at u2 (8-byte aligned).
+static size_t ptile_fb_lds(const hemk_tiles *t)
+{
+	return (hemk_ptile_lds(t) + 7) / 8 * 8 + 8 * (size_t)t->max_cls;
+}
+
+int hemk_ptile_ex3_fb(const hemk_tiles *t, double *A, const double *B,
+                      const double *C, const uint32_t *uq,
+                      const double *cntq, const double *part, uint32_t T,
+                      hemk_step st, double *dpart, hemk_ctl *ctl, void *s)
+{
+	if (!t->n_tiles || t->max_tx > 256)
+		return (int)hipErrorInvalidValue;
+	const size_t d2 = (hemk_ptile_lds(t) + 7) / 8;   // in doubles
+	if (t->weighted)
+		k_ptile_ex3_fb<true><<<t->n_tiles, PBLOCK, ptile_fb_lds(t),
+		                       (hipStream_t)s>>>(*t, A, B, C, uq, cntq,
+		                                         part, T, st, dpart, d2,
+		                                         ctl);
+	else
+		k_ptile_ex3_fb<false><<<t->n_tiles, PBLOCK, ptile_fb_lds(t),
+		                        (hipStream_t)s>>>(*t, A, B, C, uq, cntq,
+		                                          part, T, st, dpart, d2,
+		                                          ctl);
+	return (int)hipGetLastError();
+}
+
+int hemk_fb_decide(const double *dpart, uint32_t n_tiles, double tol,
+                   hemk_ctl *ctl, void *s)
+{
+	k_fb_decide<<<1, 256, 0, (hipStream_t)s>>>(dpart, n_tiles, tol, ctl);
+	return (int)hipGetLastError();
+}
+
+int hemk_fb_apply(uint32_t T, const double *C, double *A,
+                  const hemk_ctl *ctl, void *s)
+{
+	const uint32_t g = (T + 255) / 256 < 2048 ? (T + 255) / 256 : 2048;
+	k_fb_apply<<<g ? g : 1, 256, 0, (hipStream_t)s>>>(T, C, A, ctl);
 	return (int)hipGetLastError();
 }
 

@@ -468,8 +468,8 @@ static int check_batch_opts(const hem_problem *p, const hem_opts *o)
 	    o->norm != HEM_NORM_SEQ)
 		return hem_fail(HEM_ERR_INVALID, "unknown norm mode %d", o->norm);
 	if (o->fallback)
-		return hem_fail(HEM_ERR_INVALID, "the monotonicity fallback is "
-		                                 "not available in this build");
+		return hem_fail(HEM_ERR_INVALID, "batched solves do not take the "
+		                                 "monotonicity fallback");
 	if (!p->dt.n_tiles)
 		return hem_fail(HEM_ERR_RANGE,
 		                "a connected component of this class table exceeds "

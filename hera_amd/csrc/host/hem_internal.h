@@ -11,6 +11,9 @@
 /* Iterations per captured graph.  Iterations past convergence are no-ops
  * (every kernel reads the control block's done flag first). */
 #define HEM_ITERS_PER_GRAPH 4
+/* Monotonicity fallback: an iteration keeps u2 when ell(lam) - ell(u2) <
+ * -HEM_FB_TOL times the sum of the difference's terms' magnitudes. */
+#define HEM_FB_TOL 1e-10
 /* Distinct option sets whose graphs a handle keeps. */
 #define HEM_GRAPH_CACHE 8
 /* device arrays of a tiling (hem_tiles_upload) */
@@ -91,6 +94,7 @@ struct hem_problem {
 	 * in tile order (state vectors A, B, C are then in tile order too) */
 	uint32_t *d_uq;
 	double *d_cntq;
+	double *d_fbpart;  /* fallback: [tile] {ell difference, magnitudes} */
 	double *d_part1, *d_part2, *d_out;
 	/* exact sums by speculation: r/v rows [2][T], scratch per chain
 	 * (two chains), and the sums (pass 1 at [0..1], pass 2 at [2..3]) */

@@ -56,10 +56,9 @@ static int check_opts(const hem_opts *o)
 	if (o->norm != HEM_NORM_EXACT && o->norm != HEM_NORM_TREE &&
 	    o->norm != HEM_NORM_SEQ)
 		return hem_fail(HEM_ERR_INVALID, "unknown norm mode %d", o->norm);
-	if (o->fallback)
-		return hem_fail(HEM_ERR_INVALID,
-		                "the monotonicity fallback is not available "
-		                "in this build");
+	if (o->fallback != 0 && o->fallback != 1)
+		return hem_fail(HEM_ERR_INVALID, "fallback must be 0 or 1 (%d)",
+		                o->fallback);
 	return HEM_OK;
 }
 
@@ -67,7 +66,8 @@ static int graph_key(const hem_problem *p, const hem_opts *o)
 {
 	return o->precision | (o->step << 1) | (o->clamp << 2) |
 	       ((o->alpha_clamp != 0) << 3) | (o->norm << 4) |
-	       (p->group_cls << 6) | (p->group_tx << 14);
+	       (p->group_cls << 6) | (p->group_tx << 14) |
+	       ((o->fallback != 0) << 24);
 }
 
 /* Optional per-kernel timing: an event is recorded after every launch. */
@@ -118,9 +118,13 @@ static hemk_spec spec_plain(const hem_problem *p, uint32_t nchain, int xf,
  * tile (one kernel for u -> u1 -> u2, one for the extrapolated map). */
 static int use_tiles(const hem_problem *p, const hem_opts *o)
 {
+	size_t lds = hemk_ptile_lds(&p->dtp);
+	/* the fallback's kernel also keeps the class sums at u2 */
+	if (o->fallback)
+		lds = (lds + 7) / 8 * 8 + 8 * (size_t)p->dtp.max_cls;
 	return p->dtp.n_tiles && o->precision == HEM_PREC_FP64 &&
 	       (o->norm == HEM_NORM_EXACT || o->norm == HEM_NORM_TREE) &&
-	       hemk_ptile_lds(&p->dtp) <= HEM_PTILE_LDS_MAX;
+	       lds <= HEM_PTILE_LDS_MAX;
 }
 
 /* The first iteration's u1 = M(u), u2 = M(u1) over the tiles (and its r,
@@ -151,6 +155,18 @@ static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
 	const double *alpha_part = NULL;
 	void *s = p->stream;
 
+	if (o->fallback) {
+		/* u1, u2 (and the r/v rows) at the start of every iteration,
+		 * with the second map's stop test for an iteration that keeps
+		 * u2 */
+		HEM_TRY_DEV(hemk_ptile_map12_fb(&p->dtp, p->d_A, p->d_B, p->d_C,
+		                                p->d_uq, p->d_cntq,
+		                                exact ? p->d_sx : NULL,
+		                                exact ? p->d_sx + T : NULL,
+		                                p->d_ctl, s),
+		            "tile maps");
+		MARK(HEM_PK_TMAP12);
+	}
 	if (exact) {
 		/* the reference's left-to-right sums over the r and v rows;
 		 * with the sd form pass 1 also keeps each chunk's sum of
@@ -195,13 +211,30 @@ static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
 		            "alpha");
 		MARK(sd ? HEM_PK_NORM2 : HEM_PK_NORM1);
 	}
-	HEM_TRY_DEV(hemk_ptile_ex3_map12(&p->dtp, p->d_A, p->d_B, p->d_C,
-	                                 p->d_uq, p->d_cntq, alpha_part, T, st,
-	                                 exact ? p->d_sx : NULL,
-	                                 exact ? p->d_sx + T : NULL, p->d_ctl,
-	                                 s),
-	            "tile extrapolated map");
-	MARK(HEM_PK_TEX3);
+	if (o->fallback) {
+		/* extrapolate and stabilise; keep u2 instead when that lowers
+		 * the log-likelihood */
+		HEM_TRY_DEV(hemk_ptile_ex3_fb(&p->dtp, p->d_A, p->d_B, p->d_C,
+		                              p->d_uq, p->d_cntq, alpha_part, T,
+		                              st, p->d_fbpart, p->d_ctl, s),
+		            "tile extrapolated map");
+		MARK(HEM_PK_TEX3);
+		HEM_TRY_DEV(hemk_fb_decide(p->d_fbpart, p->dtp.n_tiles,
+		                           HEM_FB_TOL, p->d_ctl, s),
+		            "fallback decision");
+		HEM_TRY_DEV(hemk_fb_apply(T, p->d_C, p->d_A, p->d_ctl, s),
+		            "fallback");
+		MARK(HEM_PK_ITER_END);
+	} else {
+		HEM_TRY_DEV(hemk_ptile_ex3_map12(&p->dtp, p->d_A, p->d_B,
+		                                 p->d_C, p->d_uq, p->d_cntq,
+		                                 alpha_part, T, st,
+		                                 exact ? p->d_sx : NULL,
+		                                 exact ? p->d_sx + T : NULL,
+		                                 p->d_ctl, s),
+		            "tile extrapolated map");
+		MARK(HEM_PK_TEX3);
+	}
 	HEM_TRY_DEV(hemk_iter_end(p->d_ctl, s), "iter_end");
 	MARK(HEM_PK_ITER_END);
 	return HEM_OK;
@@ -406,7 +439,8 @@ static int solve_setup(hem_problem *p, const uint32_t *unique,
 		                    p->stream);
 	if (rc)
 		return hem_dev_fail(rc, "solve setup");
-	return use_tiles(p, o) ? enqueue_tiles_prologue(p, o) : HEM_OK;
+	return use_tiles(p, o) && !o->fallback ? enqueue_tiles_prologue(p, o)
+	                                       : HEM_OK;
 }
 
 static int load_opts(const hem_opts *opts, hem_opts *o)
@@ -419,9 +453,14 @@ static int load_opts(const hem_opts *opts, hem_opts *o)
 }
 
 /* Member weights run on the FP64 order-faithful class and transcript
- * kernels only. */
+ * kernels only; the monotonicity fallback runs on the component tiles. */
 static int check_weighted(const hem_problem *p, const hem_opts *o)
 {
+	if (o->fallback && !use_tiles(p, o))
+		return hem_fail(HEM_ERR_RANGE,
+		                "the monotonicity fallback runs on the component "
+		                "tiles: FP64 with exact or tree norms, on a table "
+		                "whose components fit a tile");
 	if (p->weighted && o->precision != HEM_PREC_FP64)
 		return hem_fail(HEM_ERR_INVALID, "a table with member weights "
 		                                 "solves in FP64 only");

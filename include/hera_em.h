@@ -104,20 +104,32 @@ typedef struct {
 	double   stop_min_value;  /* 1e-3 (src/EM.c:292)                       */
 	double   stop_rel_change; /* 0.01 (src/EM.c:295); e.g. 1e-8 for a
 	                             tolerance-mode run                        */
-	int      fallback;        /* must be 0: a monotonicity fallback (plain
-	                             EM point when the extrapolated point lowers
-	                             the log-likelihood) is not in this build,
-	                             and the reference has none; 1 is rejected
-	                             with HEM_ERR_INVALID                      */
+	int      fallback;        /* 0 (default, the reference's loop) or 1:
+	                             monotonicity fallback, see below         */
 	int      verbose;         /* 1: print hera's "Finish EM" line          */
 } hem_opts;
+
+/* Monotonicity fallback (fallback = 1; neither reference has one, nor a
+ * likelihood).  Each SQUAREM iteration computes u1, u2, the extrapolated
+ * point and lam = M(u') as src/EM.c:329-355 does, then compares the
+ * log-likelihood of hem_loglik at lam and at u2: when ell(lam) - ell(u2) <
+ * -1e-10 S, with the difference summed as per-term differences (n_c
+ * log1p((D_c(lam) - D_c(u2)) / D_c(u2)) per class, u_t log1p((lam_t -
+ * u2_t) / u2_t) and -len_t (lam_t - u2_t) per transcript) and S the sum of
+ * those terms' magnitudes, the iteration ends at u2 (two plain maps, which
+ * never lower the likelihood) and its stop test is the second map's (u2
+ * against u1).  The likelihood thus never decreases from one iteration to
+ * the next beyond rounding.  hem_stats.fallbacks counts such iterations.
+ * With fallback = 0 nothing changes.  hem_solve runs it on the component
+ * tiles only (FP64, exact or tree norms, every component within the tile
+ * limits; HEM_ERR_RANGE otherwise); member weights enter D_c. */
 
 /* Fills the reference defaults (src/EM.c:324-346, 292-295). */
 void hem_opts_default(hem_opts *o);
 
 typedef struct {
 	uint32_t iters;        /* SQUAREM iterations k (hera's "rounds")      */
-	uint32_t maps;         /* EM map applications (3 per iteration + fallbacks) */
+	uint32_t maps;         /* EM map applications (3 per iteration)      */
 	uint32_t converged;    /* 1 if the stop rule ended the loop          */
 	uint32_t fallbacks;    /* iterations where the fallback fired        */
 	double   loglik;       /* FP64 log-likelihood of the final densities

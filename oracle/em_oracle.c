@@ -15,6 +15,10 @@
  * and, behind options that default off:
  *   L2 step length  hera-EM/utility.c:16-41
  *   u2/10 clamp     hera-EM/utility.c:50-53
+ *   monotonicity fallback (no counterpart in either reference, which never
+ *                   evaluate the likelihood): an iteration whose stabilised
+ *                   point lowers the log-likelihood below that of its second
+ *                   plain map u2 ends at u2 instead
  * plus three functions the reference lacks (documented in em_oracle.h):
  * the Poisson log-likelihood, its adjoint and its Hessian-vector product.
  *
@@ -111,6 +115,77 @@ static double *o_mstep(const o_prob *P, const double *grama, double *overlap,
 	return new_grama;
 }
 
+/* ell(lam) - ell(mu) for the log-likelihood of oracle_loglik (member
+ * weights in D_c when the table has them), as sums of per-term differences
+ * so that near-equal points do not cancel two large totals: n_c
+ * log1p((D_c(lam) - D_c(mu)) / D_c(mu)) over classes (a class with a zero
+ * sum at either point contributes its terms as they are, the zero one
+ * nothing), u_t log1p((lam_t - mu_t) / mu_t) over transcripts with unique
+ * reads, minus len_t (lam_t - mu_t).  Each of the three sums runs in index
+ * order; the result is (classes + uniques) - lengths, and *scale receives
+ * the same sums of the terms' magnitudes. */
+static double o_dloglik(const o_prob *P, const uint32_t *count,
+                        const double *lam, const double *mu, double *scale)
+{
+	uint32_t i, k;
+	double a = 0, b = 0, c = 0, sa = 0, sb = 0, sc = 0;
+	for (i = 0; i < P->n_cls; ++i) {
+		double dl = 0, dm = 0, x = 0;
+		if (P->cnt[i] == 0)
+			continue;
+		for (k = P->off[i]; k < P->off[i + 1]; ++k) {
+			const double wk = P->w ? P->w[k] : 1.0;
+			dl += lam[P->ref[k]] * wk;
+			dm += mu[P->ref[k]] * wk;
+		}
+		if (dl > 0 && dm > 0)
+			x = P->cnt[i] * log1p((dl - dm) / dm);
+		else if (dl > 0)
+			x = P->cnt[i] * log(dl);
+		else if (dm > 0)
+			x = -(P->cnt[i] * log(dm));
+		a += x;
+		sa += fabs(x);
+	}
+	for (i = 0; i < P->n_tx; ++i) {
+		if (count[i] != 0) {
+			const double x = count[i] * log1p((lam[i] - mu[i]) / mu[i]);
+			b += x;
+			sb += fabs(x);
+		}
+		{
+			const double y = P->len[i] * (lam[i] - mu[i]);
+			c += y;
+			sc += fabs(y);
+		}
+	}
+	if (scale)
+		*scale = (sa + sb) + sc;
+	return (a + b) - c;
+}
+
+/* The fallback's test: the stabilised point lam is kept unless
+ * ell(lam) - ell(u2) < -ORACLE_FB_TOL * scale, that is unless it lowers the
+ * log-likelihood by more than rounding can explain. */
+#define ORACLE_FB_TOL 1e-10
+
+static int o_fallback(const o_prob *P, const uint32_t *count,
+                      const double *lam, const double *u2)
+{
+	double scale = 0;
+	const double d = o_dloglik(P, count, lam, u2, &scale);
+	return d < -ORACLE_FB_TOL * scale;
+}
+
+double oracle_dloglik(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
+                      const uint32_t *ref, const double *cnt, const double *w,
+                      const uint32_t *len, const uint32_t *count,
+                      const double *lam, const double *mu, double *scale)
+{
+	o_prob P = {n_tx, n_cls, off, ref, cnt, len, w};
+	return o_dloglik(&P, count, lam, mu, scale);
+}
+
 void oracle_opts_default(oracle_opts *o)
 {
 	o->min_round = 50;
@@ -120,6 +195,7 @@ void oracle_opts_default(oracle_opts *o)
 	o->step_form = 0;
 	o->clamp_form = 0;
 	o->alpha_clamp = 1;
+	o->fallback = 0;
 }
 
 /* estimate_count, src/EM.c:315-387, on a CSR class table. */
@@ -143,7 +219,7 @@ int oracle_estimate_count_w(uint32_t n_tx, uint32_t n_cls,
                             oracle_trace *tr)
 {
 	o_prob P = {n_tx, n_cls, off, ref, cnt, len, w};
-	uint32_t i, k, stop;
+	uint32_t i, k, stop, stop2, fb;
 	double *u, *u1, *u2, *grama, *overlap, sum, alpha, n_v, n_r;
 	double *r = malloc(n_tx * sizeof(double));
 	double *v = malloc(n_tx * sizeof(double));
@@ -164,6 +240,7 @@ int oracle_estimate_count_w(uint32_t n_tx, uint32_t n_cls,
 		             opt->stop_rel);
 		u2 = o_mstep(&P, u1, overlap, count, &stop, opt->stop_min,
 		             opt->stop_rel);
+		stop2 = stop;
 
 		for (i = 0; i < n_tx; ++i) {
 			r[i] = u1[i] - u[i];
@@ -202,10 +279,20 @@ int oracle_estimate_count_w(uint32_t n_tx, uint32_t n_cls,
 			}
 			u1[i] = x;
 		}
-		free(u2);
 
 		grama = o_mstep(&P, u1, overlap, count, &stop, opt->stop_min,
 		                opt->stop_rel);
+		fb = 0;
+		if (opt->fallback && o_fallback(&P, count, grama, u2)) {
+			/* the stabilised point is worse than two plain maps:
+			 * keep u2, with the stop test of the map that made it */
+			memcpy(grama, u2, n_tx * sizeof(double));
+			stop = stop2;
+			fb = 1;
+		}
+		free(u2);
+		if (tr && tr->fallback && k < tr->cap)
+			tr->fallback[k] = fb;
 		if (tr && k < tr->cap) {
 			tr->alpha[k] = alpha;
 			tr->stop[k] = stop;
@@ -806,5 +893,235 @@ int oracle_hera_sample(const uint32_t *unique, uint32_t n_tx, uint32_t *seed,
 	while (--n > 0)
 		++count[table[oracle_rand_r(seed) % n]];
 	free(table);
+	return 0;
+}
+
+/* ---- hera-EM's simplified-model SQUAREM --------------------------------
+ * s_squarem (hera-EM/em.c:418-426) over squarem_loop (em.c:384-406) with
+ * the simple model's E-step (the e_step macro, em.c:75-157, with
+ * s_joint_prob / s_noise_prob / s_model_update / s_model_update_noise,
+ * smodel.h:15-45), M-step (trans_master_init + lock_normalize,
+ * transdis.c:20-30, 54-61, utility.h:104-122), locked extrapolation
+ * (lock_accelerate + the locking rule, utility.c:64-107, transdis.c:
+ * 97-115) and stop test (is_stop / arr_diff, transdis.c:123-130,
+ * utility.c:109-128), on the class table: a class is a set of identical
+ * reads (count n_c) whose alignments are its members (transcript t_k,
+ * weight w_k = c_prob); the noise component is transcript noise_tx, a
+ * member of every class with weight 4^-read_length.
+ *
+ * Restated per class, in class order (hera-EM's per-thread joins have no
+ * fixed order):
+ *   E-step (p -> acc), flag filt (the second EM of each loop):
+ *     acc[noise] starts at the banked noise count B (sum over removed
+ *     classes of their banks, class order) when B >= thr and the noise is
+ *     not locked (n_update(new_m, NULL, noise_cnt), em.c:85);
+ *     per live class: total = P0_c + sum over live members of p_t w_k
+ *     (member order); below 1e-300 the class is skipped (and, filtering,
+ *     removed with bank n_c, em.c:112-115);
+ *     per live member: post = (p_t w_k) / total; the noise member adds
+ *     n_c post to acc[noise] unless post < thr or the noise is locked;
+ *     another member adds n_c post to acc[t] unless post < thr (filtering:
+ *     the member is removed) or t is locked (filtering: removed, and
+ *     P0_c += p_t w_k, member order);
+ *     filtering, a class left without a contributing member is removed
+ *     and banks n_c times its noise posterior (em.c:139-145).
+ *   M-step: locked entries keep p; the others are acc scaled by
+ *     (1 / sum of unlocked acc) * (1 - sum of locked p), sums in index
+ *     order.
+ *   Loop: p2 = EM(p1), p3 = EM(p2, filtering); over unlocked entries r =
+ *     p2 - p1, v = (p3 - p2) - r, g2 = |r|^2 / |v|^2 (1 when |v|^2 <=
+ *     1e-300), g = -sqrt(g2), p3 = p1 - 2 g r + g2 v, or p3 / 10 when that
+ *     is <= 1e-300; then (locking) an unlocked entry locks when p3 <
+ *     1e-300 or |p3 - p1| < lock_thr p3; p1 = EM(p3); converged when no
+ *     entry has p3 >= stop_min and |p3 - p1| / p3 >= stop_rel. */
+typedef struct {
+	uint32_t T, C, z;
+	const uint32_t *off, *ref;
+	const double *cnt, *w;
+	double thr;
+	uint8_t *lock, *alive, *dead;
+	double *P0, *bank;
+	uint32_t removed_members, removed_classes;
+} o_he;
+
+static void he_estep(o_he *S, const double *p, double *acc, int filt)
+{
+	uint32_t c, k, t;
+	const int has_z = S->z < S->T;
+	double B = 0;
+	for (c = 0; c < S->C; ++c)
+		if (S->dead[c])
+			B += S->bank[c];
+	memset(acc, 0, S->T * sizeof(double));
+	if (has_z && !S->lock[S->z] && B >= S->thr)
+		acc[S->z] = B;
+	for (c = 0; c < S->C; ++c) {
+		double total, zpost = 0;
+		uint32_t live = 0;
+		if (S->dead[c])
+			continue;
+		total = S->P0[c];
+		for (k = S->off[c]; k < S->off[c + 1]; ++k)
+			if (S->alive[k])
+				total += p[S->ref[k]] * S->w[k];
+		if (total < 1e-300) {
+			if (filt) {
+				S->dead[c] = 1;
+				S->bank[c] = S->cnt[c];
+				S->removed_classes++;
+			}
+			continue;
+		}
+		for (k = S->off[c]; k < S->off[c + 1]; ++k) {
+			double post;
+			if (!S->alive[k])
+				continue;
+			t = S->ref[k];
+			post = (p[t] * S->w[k]) / total;
+			if (t == S->z) {
+				zpost = post;
+				if (post >= S->thr && !S->lock[t])
+					acc[t] += S->cnt[c] * post;
+				continue;
+			}
+			if (post < S->thr) {
+				if (filt) {
+					S->alive[k] = 0;
+					S->removed_members++;
+				}
+			} else if (S->lock[t]) {
+				if (filt) {
+					S->alive[k] = 0;
+					S->removed_members++;
+					S->P0[c] += p[t] * S->w[k];
+				}
+			} else {
+				acc[t] += S->cnt[c] * post;
+				++live;
+			}
+		}
+		if (filt && live == 0) {
+			S->dead[c] = 1;
+			S->bank[c] = S->cnt[c] * zpost;
+			S->removed_classes++;
+		}
+	}
+}
+
+static void he_mstep(const o_he *S, const double *p, const double *acc,
+                     double *out)
+{
+	uint32_t t;
+	double sum = 0, l_sum = 0, f;
+	for (t = 0; t < S->T; ++t) {
+		if (S->lock[t])
+			l_sum += p[t];
+		else
+			sum += acc[t];
+	}
+	f = 1 / sum * (1 - l_sum);
+	for (t = 0; t < S->T; ++t)
+		out[t] = S->lock[t] ? p[t] : acc[t] * f;
+}
+
+int oracle_hera_em(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
+                   const uint32_t *ref, const double *cnt, const double *w,
+                   const oracle_he_opts *o, const double *p0, double *p_out,
+                   oracle_he_stats *st)
+{
+	o_he S;
+	uint32_t t, k = 0, conv = 0, nz = n_cls ? off[n_cls] : 0;
+	double *p1 = malloc(n_tx * sizeof(double));
+	double *p2 = malloc(n_tx * sizeof(double));
+	double *p3 = malloc(n_tx * sizeof(double));
+	double *acc = malloc(n_tx * sizeof(double));
+	memset(&S, 0, sizeof(S));
+	S.T = n_tx;
+	S.C = n_cls;
+	S.z = o->noise_tx;
+	S.off = off;
+	S.ref = ref;
+	S.cnt = cnt;
+	S.w = w;
+	S.thr = o->filter_thr;
+	S.lock = calloc(n_tx ? n_tx : 1, 1);
+	S.alive = malloc(nz ? nz : 1);
+	S.dead = calloc(n_cls ? n_cls : 1, 1);
+	S.P0 = calloc(n_cls ? n_cls : 1, sizeof(double));
+	S.bank = calloc(n_cls ? n_cls : 1, sizeof(double));
+	if (!p1 || !p2 || !p3 || !acc || !S.lock || !S.alive || !S.dead ||
+	    !S.P0 || !S.bank || n_tx == 0)
+		return 1;
+	memset(S.alive, 1, nz ? nz : 1);
+	/* trans_uniform_init (transdis.c:32-40): 1 / (trans_cnt + 1) over the
+	 * transcripts and the noise */
+	for (t = 0; t < n_tx; ++t)
+		p1[t] = p0 ? p0[t] : 1.0 / n_tx;
+	while (k < o->max_iter) {
+		double r2 = 0, v2 = 0, g2, g;
+		uint32_t bad = 0;
+		he_estep(&S, p1, acc, 0);
+		he_mstep(&S, p1, acc, p2);
+		he_estep(&S, p2, acc, o->filtering);
+		he_mstep(&S, p2, acc, p3);
+		for (t = 0; t < n_tx; ++t)
+			if (!S.lock[t]) {
+				double r = p2[t] - p1[t];
+				double v = p3[t] - p2[t] - r;
+				r2 += r * r;
+				v2 += v * v;
+			}
+		g2 = v2 <= 1e-300 ? 1 : r2 / v2;
+		g = -sqrt(g2);
+		for (t = 0; t < n_tx; ++t)
+			if (!S.lock[t]) {
+				double r = p2[t] - p1[t];
+				double v = p3[t] - p2[t] - r;
+				double nv = p1[t] - 2 * g * r + g2 * v;
+				if (nv <= 1e-300)
+					nv = p3[t] / 10;
+				p3[t] = nv;
+			}
+		if (o->locking)
+			for (t = 0; t < n_tx; ++t)
+				if (!S.lock[t]) {
+					double diff = fabs(p3[t] - p1[t]);
+					S.lock[t] = p3[t] < 1e-300 ||
+					            diff < o->lock_thr * p3[t];
+				}
+		he_estep(&S, p3, acc, 0);
+		he_mstep(&S, p3, acc, p1);
+		++k;
+		for (t = 0; t < n_tx; ++t) {
+			double diff = fabs(p3[t] - p1[t]);
+			if (p3[t] >= o->stop_min) {
+				diff /= p3[t];
+				bad += diff >= o->stop_rel;
+			}
+		}
+		if (bad == 0) {
+			conv = 1;
+			break;
+		}
+	}
+	memcpy(p_out, p1, n_tx * sizeof(double));
+	if (st) {
+		st->iters = k;
+		st->converged = conv;
+		st->n_locked = 0;
+		for (t = 0; t < n_tx; ++t)
+			st->n_locked += S.lock[t];
+		st->removed_members = S.removed_members;
+		st->removed_classes = S.removed_classes;
+	}
+	free(p1);
+	free(p2);
+	free(p3);
+	free(acc);
+	free(S.lock);
+	free(S.alive);
+	free(S.dead);
+	free(S.P0);
+	free(S.bank);
 	return 0;
 }

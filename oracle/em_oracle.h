@@ -15,6 +15,7 @@ typedef struct {
 	int step_form;        /* 0 sample sd (src/EM.c:338), 1 L2 (hera-EM) */
 	int clamp_form;       /* 0 zero (src/EM.c:351), 1 u2/10 (hera-EM)   */
 	int alpha_clamp;      /* 1: alpha <= -1 (src/EM.c:346)               */
+	int fallback;         /* 1: monotonicity fallback (no reference)     */
 } oracle_opts;
 
 typedef struct {
@@ -24,6 +25,8 @@ typedef struct {
 	uint32_t *stop;       /* stop flag after each iteration's 3rd map    */
 	double *last_u;       /* n_tx: extrapolated point of the last
 	                         iteration (the input of its 3rd map), or NULL */
+	uint32_t *fallback;   /* per iteration: 1 if the fallback kept u2, or
+	                         NULL                                        */
 } oracle_trace;
 
 void oracle_opts_default(oracle_opts *o);
@@ -69,6 +72,14 @@ int oracle_bootstrap_batch(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
                            const double *cls_counts, const oracle_opts *opt,
                            uint32_t n_thread, double *est, uint32_t *k_out);
 double oracle_now(void);
+/* ell(lam) - ell(mu) of oracle_loglik's objective (w: member weights or
+ * NULL), summed as per-term differences (log1p of relative changes);
+ * *scale (if not NULL) = the sum of the terms' magnitudes.  The fallback
+ * keeps u2 when ell(lam) - ell(u2) < -1e-10 * scale. */
+double oracle_dloglik(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
+                      const uint32_t *ref, const double *cnt, const double *w,
+                      const uint32_t *len, const uint32_t *count,
+                      const double *lam, const double *mu, double *scale);
 
 
 /* bootstrap resampling restatement */
@@ -92,6 +103,25 @@ int oracle_write_gene_abundance(const char *path, uint32_t n,
                                 const uint32_t *gene, const char *gene_name,
                                 uint32_t l_gene, const double *overlap,
                                 const double *grama);
+
+/* hera-EM's simplified-model SQUAREM (restated in em_oracle.c) */
+typedef struct {
+	uint32_t noise_tx;    /* the noise component's transcript, or >= n_tx */
+	uint32_t max_iter;
+	int locking;          /* lock entries after each extrapolation        */
+	int filtering;        /* filter on the second EM of each loop         */
+	double lock_thr;      /* 1e-5  LOCK_THRESHOLD (hera-EM/utility.h:13)  */
+	double filter_thr;    /* 1e-6  FILTERING_THRESHOLD (utility.h:14)     */
+	double stop_rel;      /* 1e-3  THRESHOLD1 (utility.h:10)              */
+	double stop_min;      /* 1e-7  THRESHOLD2 (utility.h:11)              */
+} oracle_he_opts;
+typedef struct {
+	uint32_t iters, converged, n_locked, removed_members, removed_classes;
+} oracle_he_stats;
+int oracle_hera_em(uint32_t n_tx, uint32_t n_cls, const uint32_t *off,
+                   const uint32_t *ref, const double *cnt, const double *w,
+                   const oracle_he_opts *o, const double *p0, double *p_out,
+                   oracle_he_stats *st);
 
 uint32_t oracle_rand_r(uint32_t *seed);
 int oracle_hera_sample(const uint32_t *unique, uint32_t n_tx, uint32_t *seed,

@@ -20,12 +20,26 @@ class OracleOpts(C.Structure):
     _fields_ = [("min_round", C.c_uint32), ("max_round", C.c_uint32),
                 ("stop_min", C.c_double), ("stop_rel", C.c_double),
                 ("step_form", C.c_int), ("clamp_form", C.c_int),
-                ("alpha_clamp", C.c_int)]
+                ("alpha_clamp", C.c_int), ("fallback", C.c_int)]
+
+
+class OracleHeOpts(C.Structure):
+    _fields_ = [("noise_tx", C.c_uint32), ("max_iter", C.c_uint32),
+                ("locking", C.c_int), ("filtering", C.c_int),
+                ("lock_thr", C.c_double), ("filter_thr", C.c_double),
+                ("stop_rel", C.c_double), ("stop_min", C.c_double)]
+
+
+class OracleHeStats(C.Structure):
+    _fields_ = [("iters", C.c_uint32), ("converged", C.c_uint32),
+                ("n_locked", C.c_uint32), ("removed_members", C.c_uint32),
+                ("removed_classes", C.c_uint32)]
 
 
 class OracleTrace(C.Structure):
     _fields_ = [("cap", C.c_uint32), ("k", C.c_uint32),
-                ("alpha", _f64p), ("stop", _u32p), ("last_u", _f64p)]
+                ("alpha", _f64p), ("stop", _u32p), ("last_u", _f64p),
+                ("fallback", _u32p)]
 
 
 _LIB = None
@@ -95,6 +109,14 @@ def load():
             C.POINTER(C.c_uint64), C.POINTER(OracleOpts), C.c_uint32, _f64p,
             _u32p]
         lib.oracle_now.restype = C.c_double
+        lib.oracle_hera_em.restype = C.c_int
+        lib.oracle_hera_em.argtypes = [
+            C.c_uint32, C.c_uint32, _u32p, _u32p, _f64p, _f64p,
+            C.POINTER(OracleHeOpts), _f64p, _f64p, C.POINTER(OracleHeStats)]
+        lib.oracle_dloglik.restype = C.c_double
+        lib.oracle_dloglik.argtypes = [C.c_uint32, C.c_uint32, _u32p, _u32p,
+                                       _f64p, _f64p, _u32p, _u32p, _f64p,
+                                       _f64p, _f64p]
         lib.oracle_opts_default.argtypes = [C.POINTER(OracleOpts)]
         _LIB = lib
     return _LIB
@@ -142,7 +164,9 @@ def estimate_count(cls, unique, init=None, trace_cap=1000, **kw):
     alpha = np.zeros(max(trace_cap, 1))
     stop = np.zeros(max(trace_cap, 1), dtype=np.uint32)
     last_u = np.zeros(T)
-    tr = OracleTrace(trace_cap, 0, _p64(alpha), _p32(stop), _p64(last_u))
+    fb = np.zeros(max(trace_cap, 1), dtype=np.uint32)
+    tr = OracleTrace(trace_cap, 0, _p64(alpha), _p32(stop), _p64(last_u),
+                     _p32(fb))
     ini = _f64(init) if init is not None else None
     w = getattr(cls, "member_weight", None)
     w = _f64(w) if w is not None else None
@@ -155,7 +179,25 @@ def estimate_count(cls, unique, init=None, trace_cap=1000, **kw):
     k = tr.k
     return dict(tpm=tpm, est=est, lam=lam, k=k,
                 alpha=alpha[:min(k, trace_cap)].copy(),
-                stop=stop[:min(k, trace_cap)].copy(), last_u=last_u)
+                stop=stop[:min(k, trace_cap)].copy(), last_u=last_u,
+                fallback=fb[:min(k, trace_cap)].copy())
+
+
+def dloglik(cls, unique, lam, mu):
+    """(loglik(lam) - loglik(mu), scale): member weights in D_c when the
+    table has them, summed as per-term differences; scale is the sum of the
+    terms' magnitudes."""
+    off, ref = _u32(cls.cls_off), _u32(cls.cls_ref)
+    cnt, ln, u = _f64(cls.cls_count), _u32(cls.tx_len), _u32(unique)
+    lam, mu = _f64(lam), _f64(mu)
+    w = getattr(cls, "member_weight", None)
+    w = _f64(w) if w is not None else None
+    sc = C.c_double()
+    d = load().oracle_dloglik(cls.n_tx, cls.n_cls, _p32(off), _p32(ref),
+                              _p64(cnt), _p64(w) if w is not None else None,
+                              _p32(ln), _p32(u), _p64(lam), _p64(mu),
+                              C.byref(sc))
+    return d, sc.value
 
 
 def em_map(cls, unique, lam):
@@ -313,3 +355,27 @@ def hera_sample(unique, n_rep, seed=0):
                                      C.byref(st), row.ctypes.data_as(_u32p)):
             raise MemoryError("oracle sampler")
     return out, st.value
+
+
+HE_DEFAULTS = dict(max_iter=1000, locking=1, filtering=1, lock_thr=1e-5,
+                   filter_thr=1e-6, stop_rel=1e-3, stop_min=1e-7)
+
+
+def hera_em(cls, noise_tx, p0=None, **kw):
+    """hera-EM's simplified-model SQUAREM on a weighted class table (the
+    noise component is transcript noise_tx; >= n_tx for none).  Returns
+    (p, stats dict)."""
+    o = dict(HE_DEFAULTS, **kw)
+    opt = OracleHeOpts(noise_tx, o["max_iter"], o["locking"],
+                       o["filtering"], o["lock_thr"], o["filter_thr"],
+                       o["stop_rel"], o["stop_min"])
+    off, ref = _u32(cls.cls_off), _u32(cls.cls_ref)
+    cnt, w = _f64(cls.cls_count), _f64(cls.member_weight)
+    out = np.empty(cls.n_tx)
+    st = OracleHeStats()
+    ini = _f64(p0) if p0 is not None else None
+    assert load().oracle_hera_em(cls.n_tx, cls.n_cls, _p32(off), _p32(ref),
+                                 _p64(cnt), _p64(w), opt,
+                                 _p64(ini) if ini is not None else None,
+                                 _p64(out), st) == 0
+    return out, {k: getattr(st, k) for k, _ in OracleHeStats._fields_}

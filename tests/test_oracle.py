@@ -292,3 +292,100 @@ def test_hvp_matches_gradient_differences():
     w = rng.standard_normal(cls.n_tx)
     hw = O.hvp(cls, u, theta, ell, w)
     assert np.dot(w, hv) == pytest.approx(np.dot(v, hw), rel=1e-10)
+
+
+# ---- monotonicity fallback (an option; neither reference has one) --------
+
+FB_TABLES = (13, 23, 30)   # tables where hera's SQUAREM lowers ell
+
+
+def fb_table(seed):
+    return tiny_problem(seed, n_tx=60, n_cls=300, zero_frac=0.3)
+
+
+def ll_path(cls, u, K, **kw):
+    """ell after each of the first K iterations, and the iterates."""
+    lam = np.full(cls.n_tx, 1.0 / cls.n_tx)
+    out, lams, fbs = [], [], []
+    for _ in range(K):
+        r = O.estimate_count(cls, u, init=lam, min_round=1, max_round=1,
+                             **kw)
+        lam = r["lam"]
+        lams.append(lam)
+        fbs.append(int(r["fallback"][0]))
+        out.append(O.loglik(cls, u, lam))
+    return np.array(out), lams, fbs
+
+
+@pytest.mark.parametrize("seed", FB_TABLES)
+def test_fallback_keeps_loglik_monotone(seed):
+    cls, u = fb_table(seed)
+    ll0, _, _ = ll_path(cls, u, 30)
+    # without the fallback hera's step lowers ell at some iteration
+    assert np.any(np.diff(ll0) < -1e-9 * np.abs(ll0[1:]))
+    ll1, lams, fbs = ll_path(cls, u, 30, fallback=1)
+    assert sum(fbs) > 0
+    assert np.all(np.diff(ll1) >= -1e-12 * np.abs(ll1[1:]))
+    # a run of 30 iterations in one call takes the same decisions
+    r = O.estimate_count(cls, u, min_round=30, max_round=30, fallback=1)
+    np.testing.assert_array_equal(r["fallback"], fbs)
+    np.testing.assert_array_equal(r["lam"], lams[-1])
+
+
+@pytest.mark.parametrize("seed", FB_TABLES)
+def test_fallback_iteration_ends_at_two_plain_maps(seed):
+    # an iteration that falls back ends exactly at u2 = M(M(lam)), with
+    # the stop test of the second map; one that does not is the plain
+    # SQUAREM iteration
+    cls, u = fb_table(seed)
+    _, lams, fbs = ll_path(cls, u, 30, fallback=1)
+    lam = np.full(cls.n_tx, 1.0 / cls.n_tx)
+    seen = 0
+    for k in range(30):
+        u1, _ = O.em_map(cls, u, lam)
+        u2, st2 = O.em_map(cls, u, u1)
+        plain = O.estimate_count(cls, u, init=lam, min_round=1, max_round=1)
+        fbr = O.estimate_count(cls, u, init=lam, min_round=1, max_round=1,
+                               fallback=1)
+        if fbs[k]:
+            np.testing.assert_array_equal(lams[k], u2)
+            assert fbr["stop"][0] == st2
+            d, sc = O.dloglik(cls, u, plain["lam"], u2)
+            assert d < -1e-10 * sc
+            seen += 1
+        else:
+            np.testing.assert_array_equal(lams[k], plain["lam"])
+            assert fbr["stop"][0] == plain["stop"][0]
+        lam = lams[k]
+    assert seen == sum(fbs) > 0
+
+
+def test_fallback_off_is_the_reference_loop(cfg1):
+    cls, u = cfg1
+    a = O.estimate_count(cls, u)
+    b = O.estimate_count(cls, u, fallback=0)
+    np.testing.assert_array_equal(a["lam"], b["lam"])
+    assert a["k"] == b["k"] and not b["fallback"].any()
+
+
+def test_dloglik_is_the_loglik_difference():
+    for seed in range(4):
+        cls, u = tiny_problem(seed, zero_frac=0.2 if seed % 2 else 0.0)
+        rng = np.random.default_rng(seed)
+        lam = rng.uniform(1e-3, 1.0, cls.n_tx)
+        mu = lam * rng.uniform(0.5, 2.0, cls.n_tx)
+        d, sc = O.dloglik(cls, u, lam, mu)
+        exact = O.loglik(cls, u, lam) - O.loglik(cls, u, mu)
+        assert d == pytest.approx(exact, rel=1e-9, abs=1e-9 * sc)
+        d2, sc2 = O.dloglik(cls, u, mu, lam)
+        assert d2 == pytest.approx(-d, rel=1e-9, abs=1e-12 * sc)
+        assert O.dloglik(cls, u, lam, lam)[0] == 0.0
+        # a class whose members are all zero at one point only
+        mu2 = mu.copy()
+        m = cls.cls_ref[cls.cls_off[0]:cls.cls_off[1]]
+        mu2[m] = 0.0
+        u2 = u.copy()
+        u2[m] = 0
+        exact = O.loglik(cls, u2, lam) - O.loglik(cls, u2, mu2)
+        assert O.dloglik(cls, u2, lam, mu2)[0] == pytest.approx(exact,
+                                                               rel=1e-9)

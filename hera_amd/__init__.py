@@ -428,6 +428,27 @@ class Problem:
             self._h, u32p(u), n_rep, seed, u32p(uo)))
         return uo
 
+    def solve_hera_em(self, noise_tx, p0=None, **kw):
+        """hera-EM's simplified-model SQUAREM with locking and filtering
+        (hem_solve_hera_em); noise_tx is the noise component's transcript
+        (>= n_tx: none).  Returns (p, stats dict)."""
+        lib = _lib.load()
+        o = _lib.HemHeraEmOpts()
+        lib.hem_hera_em_opts_default(o)
+        o.noise_tx = noise_tx
+        for k, v in kw.items():
+            if not hasattr(o, k):
+                raise KeyError(k)
+            setattr(o, k, v)
+        ini = as_f64(p0) if p0 is not None else None
+        out = np.empty(self.n_tx)
+        st = _lib.HemHeraEmStats()
+        check(lib.hem_solve_hera_em(self._h,
+                                    f64p(ini) if ini is not None else None,
+                                    o, f64p(out), st))
+        return out, {k: getattr(st, k) for k, _ in
+                     _lib.HemHeraEmStats._fields_}
+
     def loglik(self, unique, lam):
         u = as_u32(unique)
         lam = as_f64(lam)

@@ -84,6 +84,21 @@ class HemBatchStats(C.Structure):
                 ("ms_total", C.c_double), ("sum_reruns", C.c_uint32)]
 
 
+class HemHeraEmOpts(C.Structure):
+    _fields_ = [("noise_tx", C.c_uint32), ("max_iter", C.c_uint32),
+                ("locking", C.c_int), ("filtering", C.c_int),
+                ("lock_threshold", C.c_double),
+                ("filter_threshold", C.c_double),
+                ("stop_rel_change", C.c_double),
+                ("stop_min_value", C.c_double)]
+
+
+class HemHeraEmStats(C.Structure):
+    _fields_ = [("iters", C.c_uint32), ("converged", C.c_uint32),
+                ("n_locked", C.c_uint32), ("removed_members", C.c_uint32),
+                ("removed_classes", C.c_uint32), ("ms", C.c_double)]
+
+
 class HemGradStats(C.Structure):
     _fields_ = [("loglik", C.c_double), ("n_frag", C.c_double),
                 ("ms", C.c_double)]
@@ -169,6 +184,10 @@ SIGNATURES = {
                                             C.c_uint32, _u32p]),
     "hem_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_int,
                                      C.POINTER(C.c_double)]),
+    "hem_hera_em_opts_default": (None, [C.POINTER(HemHeraEmOpts)]),
+    "hem_solve_hera_em": (C.c_int, [C.c_void_p, _f64p,
+                                    C.POINTER(HemHeraEmOpts), _f64p,
+                                    C.POINTER(HemHeraEmStats)]),
     "hem_grad": (C.c_int, [C.c_void_p, _u32p, _f64p, _f64p, C.c_int, _f64p,
                            _f64p, C.POINTER(HemGradStats)]),
     "hem_hvp": (C.c_int, [C.c_void_p, _u32p, _f64p, _f64p, _f64p, _f64p]),

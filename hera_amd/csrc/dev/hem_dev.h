@@ -359,6 +359,42 @@ int hemk_fb_apply(uint32_t T, const double *C, double *A,
 int hemk_ptile_rows(uint32_t T, const uint32_t *txq, const double *A,
                     double *out, void *s);
 
+/* ---- hera-EM's simplified model (hera_em_kernels.hip) ---------------------
+ * State of one solve on the device: the table (CSR, CSC with the member
+ * index of every occurrence), the noise transcript z (>= T: none), the
+ * filtering threshold, and the persistent lock / live / removed flags,
+ * locked masses P0 and noise banks; rows and sums are scratch (sums:
+ * B, unlocked sum, locked sum, |r|^2, |v|^2; counts: removed members,
+ * removed classes, failed stop tests). */
+typedef struct {
+	uint32_t T, C, z;
+	double thr;
+	const uint32_t *off, *ref, *tx_off, *tx_ref, *occ2mem;
+	const double *cnt, *w, *tx_w;
+	uint8_t *lock, *alive, *dead;
+	double *P0, *bank, *total;
+	double *brow, *zrow, *urow, *lrow;
+	double *sums;
+	uint32_t *counts;
+} hemk_he;
+int hemk_he_occ2mem(uint32_t C, const uint32_t *off, const uint32_t *ref,
+                    const uint32_t *tx_off, const uint32_t *tx_ref,
+                    uint32_t *occ2mem, void *s);
+int hemk_he_fill(double *x, uint32_t n, double v, void *s);
+int hemk_he_rowsum(const double *x, uint32_t n, double *out, void *s);
+/* E-step p -> acc (filt: the second EM of a loop, which filters) */
+int hemk_he_estep(const hemk_he *h, const double *p, int filt, double *acc,
+                  void *s);
+/* M-step: out = locked ? p : acc (1 / sum unlocked acc) (1 - sum locked p) */
+int hemk_he_mstep(const hemk_he *h, const double *p, const double *acc,
+                  double *out, void *s);
+/* p3 = the locked extrapolation from p1, p2, p3; then the locking rule */
+int hemk_he_accelerate(const hemk_he *h, const double *p1, const double *p2,
+                       double *p3, int locking, double lock_thr, void *s);
+/* counts[2] += entries failing the stop test between p3 and its image p1 */
+int hemk_he_stop(const hemk_he *h, const double *p3, const double *p1,
+                 double stop_min, double stop_rel, void *s);
+
 /* ---- exact left-to-right sums in parallel (spec_sum.hip) -----------------
  * Chains of n FP64 values; each chain's result is bitwise the sequential
  * sum ((+0 + y_0) + y_1) + ... of the reference's loops, computed by chunks

@@ -124,6 +124,15 @@ struct hem_problem {
 	double *h_cls_count;       /* host copy of the class counts         */
 	hem_batch_ws bw;
 	hem_grad_ws gw;
+	/* hera-EM's simplified model (hem_solve_hera_em): device state,
+	 * allocated on the first call */
+	struct {
+		int ready;
+		uint32_t *occ2mem, *counts;
+		uint8_t *lock, *alive, *dead;
+		double *P0, *bank, *total, *brow, *zrow, *urow, *lrow, *sums;
+		double *p1, *p2, *p3, *acc;
+	} he;
 	void *ev2, *ev3;
 
 	/* batch graphs superseded by a workspace reallocation or a full cache:
@@ -232,6 +241,8 @@ int hem_tiles_upload(hem_problem *p, const hem_tiles_host *h, hemk_tiles *dt,
 
 /* error.c */
 int hem_fail(int code, const char *fmt, ...);
+/* Frees the hera-EM workspace of a handle (hera_em.c). */
+void hem_hera_em_free(hem_problem *p);
 /* The code of the calling thread's last hem_fail. */
 int hem_last_code(void);
 int hem_dev_fail(int hip_code, const char *what);

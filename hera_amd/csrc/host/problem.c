@@ -526,6 +526,7 @@ int hem_problem_destroy(hem_problem *p)
 		hemd_graph_destroy(p->graph[g]);
 	hem_batch_free(p);
 	hem_batch_destroy_retired(p);
+	hem_hera_em_free(p);
 	hem_grad_free(p);
 	free(p->h_cls_orig);
 	free(p->h_cls_count);

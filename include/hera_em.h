@@ -394,6 +394,58 @@ int hem_write_result(const char *out_dir, const char *prefix, uint32_t n,
                      const char *gene_names, uint32_t l_gene,
                      uint32_t n_gene, const double *est, const double *tpm);
 
+/* ---- hera-EM's simplified-model SQUAREM -------------------------------------
+ * hera-EM's EM after its full-model loops (s_squarem, hera-EM/em.c:418-426
+ * and 438-452) on a class table whose classes are groups of identical reads:
+ * count n_c = the group's size, members = the reads' alignments (transcript,
+ * member_weight = c_prob; NULL = 1), and the noise component as transcript
+ * noise_tx, a member of every class with weight 4^-read_length.  The state
+ * p is a distribution over all n_tx entries (the noise included); lengths
+ * and unique counts of the table are not used.  Each loop runs three EMs of
+ * the simple model (hera-EM/em.c:75-157, smodel.h:15-45): per class, total
+ * = locked mass P0 + the live members' p_t w_k, member posteriors post =
+ * (p_t w_k) / total; a member adds n_c post to its transcript unless post <
+ * filter_threshold or its transcript is locked; the second EM of each loop
+ * filters (removes such members for good, moving a locked member's p_t w_k
+ * into P0; removes a class left with no contributing member, banking n_c
+ * times its noise posterior, added to the noise in every later E-step).
+ * M-step (transdis.c:20-30, 54-61): locked entries keep their value, the
+ * others are rescaled to sum to 1 minus the locked mass.  Between the
+ * second and third EM the unlocked entries are extrapolated (L2 step
+ * gamma^2 = |r|^2 / |v|^2, values <= 1e-300 replaced by u2 / 10,
+ * utility.c:64-107), then an unlocked entry locks when it is < 1e-300 or
+ * moved by less than lock_threshold times itself (transdis.c:97-115).  The
+ * loop ends when no entry with p3 >= stop_min_value changed by at least
+ * stop_rel_change relative to p3 (is_stop, transdis.c:123-130) or at
+ * max_iter (hera-EM has no cap).  Sums run in index / class order on the
+ * device (hera-EM's per-thread joins have no fixed order).  p0: start
+ * (NULL: uniform 1 / n_tx, trans_uniform_init).  One-sample handles, FP64. */
+typedef struct {
+	uint32_t noise_tx;          /* noise component, or >= n_tx for none     */
+	uint32_t max_iter;          /* 1000                                      */
+	int      locking;           /* 1 (hera-EM)                               */
+	int      filtering;         /* 1 (hera-EM)                               */
+	double   lock_threshold;    /* 1e-5, LOCK_THRESHOLD (utility.h:13)       */
+	double   filter_threshold;  /* 1e-6, FILTERING_THRESHOLD (utility.h:14)  */
+	double   stop_rel_change;   /* 1e-3, THRESHOLD1 (utility.h:10)           */
+	double   stop_min_value;    /* 1e-7, THRESHOLD2 (utility.h:11)           */
+} hem_hera_em_opts;
+
+typedef struct {
+	uint32_t iters;             /* SQUAREM loops (3 EMs each)                */
+	uint32_t converged;
+	uint32_t n_locked;          /* entries locked at the end                 */
+	uint32_t removed_members;   /* members filtered out                      */
+	uint32_t removed_classes;   /* classes removed (noise banked)            */
+	double   ms;                /* device time of the loop                   */
+} hem_hera_em_stats;
+
+/* hera-EM's constants; noise_tx = 0xFFFFFFFF (none). */
+void hem_hera_em_opts_default(hem_hera_em_opts *o);
+int hem_solve_hera_em(hem_problem *p, const double *p0,
+                      const hem_hera_em_opts *o, double *p_out,
+                      hem_hera_em_stats *st);
+
 /* ---- adjoint of the log-likelihood ----------------------------------------
  * No counterpart in the reference, which has no derivatives; the objective
  * is the one whose EM map is src/EM.c:271-302.  With pi = softmax(theta),

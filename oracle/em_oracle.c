@@ -963,7 +963,7 @@ static void he_estep(o_he *S, const double *p, double *acc, int filt)
 		total = S->P0[c];
 		for (k = S->off[c]; k < S->off[c + 1]; ++k)
 			if (S->alive[k])
-				total += p[S->ref[k]] * S->w[k];
+				total += p[S->ref[k]] * (S->w ? S->w[k] : 1.0);
 		if (total < 1e-300) {
 			if (filt) {
 				S->dead[c] = 1;
@@ -977,7 +977,7 @@ static void he_estep(o_he *S, const double *p, double *acc, int filt)
 			if (!S->alive[k])
 				continue;
 			t = S->ref[k];
-			post = (p[t] * S->w[k]) / total;
+			post = (p[t] * (S->w ? S->w[k] : 1.0)) / total;
 			if (t == S->z) {
 				zpost = post;
 				if (post >= S->thr && !S->lock[t])
@@ -993,7 +993,8 @@ static void he_estep(o_he *S, const double *p, double *acc, int filt)
 				if (filt) {
 					S->alive[k] = 0;
 					S->removed_members++;
-					S->P0[c] += p[t] * S->w[k];
+					S->P0[c] += p[t] *
+					             (S->w ? S->w[k] : 1.0);
 				}
 			} else {
 				acc[t] += S->cnt[c] * post;

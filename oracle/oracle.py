@@ -370,12 +370,15 @@ def hera_em(cls, noise_tx, p0=None, **kw):
                        o["filtering"], o["lock_thr"], o["filter_thr"],
                        o["stop_rel"], o["stop_min"])
     off, ref = _u32(cls.cls_off), _u32(cls.cls_ref)
-    cnt, w = _f64(cls.cls_count), _f64(cls.member_weight)
+    cnt = _f64(cls.cls_count)
+    w = getattr(cls, "member_weight", None)
+    w = _f64(w) if w is not None else None
     out = np.empty(cls.n_tx)
     st = OracleHeStats()
     ini = _f64(p0) if p0 is not None else None
     assert load().oracle_hera_em(cls.n_tx, cls.n_cls, _p32(off), _p32(ref),
-                                 _p64(cnt), _p64(w), opt,
+                                 _p64(cnt), _p64(w) if w is not None else None,
+                                 opt,
                                  _p64(ini) if ini is not None else None,
                                  _p64(out), st) == 0
     return out, {k: getattr(st, k) for k, _ in OracleHeStats._fields_}

@@ -50,6 +50,28 @@ __device__ __forceinline__ double em_extrapolate(double u, double u1,
 	return x;
 }
 
+// Count, mean and centred sum of squares of a run of values, merged by
+// the pairwise rule of Chan, Golub and LeVeque (1979); an empty side is
+// the identity.
+struct Moments {
+	double n, m, q;
+};
+
+__device__ __forceinline__ Moments mom_merge(Moments a, Moments b)
+{
+#pragma clang fp contract(off)
+	if (a.n == 0)
+		return b;
+	if (b.n == 0)
+		return a;
+	Moments o;
+	o.n = a.n + b.n;
+	const double d = b.m - a.m;
+	o.m = a.m + d * (b.n / o.n);
+	o.q = a.q + b.q + d * d * (a.n * b.n / o.n);
+	return o;
+}
+
 // Marks the stop test as failed: one atomic OR, skipped once the flag
 // reads as set (an L2 read, so the check sees other blocks' ORs), which
 // keeps thousands of failing blocks from serialising on one word.

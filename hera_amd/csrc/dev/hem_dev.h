@@ -327,13 +327,26 @@ int hemk_seq_norm(uint32_t n, const double *u, const double *u1,
  * and the r, v rows as map12 does.  alpha: tree partials (nparts pairs) ->
  * ctl->alpha.  rows: out[t] = A[txq[t]].  lds: dynamic LDS of one tile. */
 size_t hemk_ptile_lds(const hemk_tiles *t);
+/* f32: the FP32 bulk form (class sums, class factors n_c / D_c and terms
+ * in FP32; per-transcript sums, M-step, state and stop test in FP64);
+ * unweighted tables only. */
+/* tpart / part (map12) != NULL: the tree norms' per-tile partials of the
+ * r, v just computed, [tile][4] (step form `step`), folded by
+ * hemk_ptile_tree_alpha into ctl->alpha. */
 int hemk_ptile_map12(const hemk_tiles *t, const double *A, double *B,
                      double *C, const uint32_t *uq, const double *cntq,
-                     double *xr, double *xv, const hemk_ctl *ctl, void *s);
+                     double *xr, double *xv, double *part, int step,
+                     int f32, const hemk_ctl *ctl, void *s);
 int hemk_ptile_ex3_map12(const hemk_tiles *t, double *A, double *B,
                          double *C, const uint32_t *uq, const double *cntq,
                          const double *part, uint32_t T, hemk_step st,
-                         double *xr, double *xv, hemk_ctl *ctl, void *s);
+                         double *xr, double *xv, double *tpart, int f32,
+                         hemk_ctl *ctl, void *s);
+/* end != 0: first end the iteration (as hemk_iter_end), folding only if the
+ * loop goes on. */
+int hemk_ptile_tree_alpha(const hemk_tiles *t, const double *part,
+                          uint32_t T, hemk_step st, int end, hemk_ctl *ctl,
+                          void *s);
 int hemk_ptile_alpha(const double *part, uint32_t nparts, uint32_t T,
                      hemk_step st, hemk_ctl *ctl, void *s);
 /* Monotonicity fallback (no counterpart in the reference).  map12_fb: as
@@ -347,7 +360,8 @@ int hemk_ptile_alpha(const double *part, uint32_t nparts, uint32_t T,
  * fb_apply: A = C when ctl->fb. */
 int hemk_ptile_map12_fb(const hemk_tiles *t, const double *A, double *B,
                         double *C, const uint32_t *uq, const double *cntq,
-                        double *xr, double *xv, hemk_ctl *ctl, void *s);
+                        double *xr, double *xv, double *part, int step,
+                        hemk_ctl *ctl, void *s);
 int hemk_ptile_ex3_fb(const hemk_tiles *t, double *A, const double *B,
                       const double *C, const uint32_t *uq,
                       const double *cntq, const double *part, uint32_t T,

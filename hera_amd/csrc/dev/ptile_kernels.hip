@@ -233,15 +233,143 @@ __device__ __forceinline__ void pmap(const PLds &L, const PTile &tc,
 	__syncthreads();
 }
 
+// The map in the FP32 bulk form: class sums of the densities rounded to
+// FP32 (member order), one FP32 n_c / D_c per class, FP32 terms lam_t
+// (n_c / D_c); each transcript's terms are summed in FP64 and the M-step
+// (u_t + sum) / len_t, the state and the stop test stay FP64.  The class
+// factors and terms use the FP64 slots of D and X as FP32 arrays.
+__device__ __forceinline__ void pmap32(const PLds &L, const PTile &tc,
+                                       const double *lin, double *lout,
+                                       bool stop, double smin, double srel,
+                                       int &bad)
+{
+#pragma clang fp contract(off)
+	const int tid = threadIdx.x;
+	float *rho = (float *)L.D;
+	float *X = (float *)L.X;
+	for (uint32_t c = tid; c < tc.nc; c += PBLOCK) {
+		uint32_t k = L.coff[c];
+		const uint32_t k1 = L.coff[c + 1];
+		float acc = 0.0f;
+		for (; k + 4 <= k1; k += 4) {
+			const float x0 = (float)lin[L.kmem[k] & 0x3Fu];
+			const float x1 = (float)lin[L.kmem[k + 1] & 0x3Fu];
+			const float x2 = (float)lin[L.kmem[k + 2] & 0x3Fu];
+			const float x3 = (float)lin[L.kmem[k + 3] & 0x3Fu];
+			acc += x0;
+			acc += x1;
+			acc += x2;
+			acc += x3;
+		}
+		for (; k < k1; ++k)
+			acc += (float)lin[L.kmem[k] & 0x3Fu];
+		rho[c] = acc == 0.0f ? 0.0f : (float)L.n[c] / acc;
+	}
+	__syncthreads();
+	for (uint32_t j = tid; j < tc.nz; j += PBLOCK) {
+		const uint32_t m = L.kocc[j];
+		X[j] = (float)lin[m & 0x3Fu] * rho[m >> 6];
+	}
+	__syncthreads();
+	for (uint32_t t = tid; t < tc.nt; t += PBLOCK) {
+		uint32_t j = L.toff[t];
+		const uint32_t j1 = L.toff[t + 1];
+		double acc = 0.0;
+		for (; j + 8 <= j1; j += 8) {
+			float x[8];
+#pragma unroll
+			for (int q = 0; q < 8; ++q)
+				x[q] = X[j + q];
+#pragma unroll
+			for (int q = 0; q < 8; ++q)
+				acc += (double)x[q];
+		}
+		for (; j < j1; ++j)
+			acc += (double)X[j];
+		const double nw = ((double)L.u[t] + acc) / (double)L.len[t];
+		lout[t] = nw;
+		if (stop && nw > smin && fabs(nw - lin[t]) / nw > srel)
+			bad = 1;
+	}
+	__syncthreads();
+}
+
+// One map in the kernels' precision: FP64 in the reference's order, or
+// the FP32 bulk form (F).
+template <bool W, bool F>
+__device__ __forceinline__ void pmapx(const PLds &L, const PTile &tc,
+                                      const double *lin, double *lout,
+                                      bool stop, double smin, double srel,
+                                      int &bad)
+{
+	if constexpr (F)
+		pmap32(L, tc, lin, lout, stop, smin, srel, bad);
+	else
+		pmap<W>(L, tc, lin, lout, stop, smin, srel, bad);
+}
+
+// Sum over the 64 lanes of wave 0 (xor butterflies: every lane gets the
+// same bits).
+__device__ __forceinline__ double wave_sum(double x)
+{
+#pragma clang fp contract(off)
+#pragma unroll
+	for (int m = 32; m >= 1; m >>= 1)
+		x += __shfl_xor(x, m, 64);
+	return x;
+}
+
+// The tree norms' per-tile partials of r = u1 - u and v = (u2 - u1) - r
+// (src/EM.c:334-335) for the next step length, by wave 0 (a tile has at
+// most 64 transcripts): the sd form {mean r, centred sum of squares of r,
+// mean v, centred sum of squares of v}, merged across tiles by Chan's rule;
+// the L2 form {0, sum r^2, 0, sum v^2}.
+__device__ __forceinline__ void ptile_parts(const PTile &tc, const double *u,
+                                            const double *u1,
+                                            const double *u2, int step,
+                                            double *out)
+{
+#pragma clang fp contract(off)
+	const uint32_t t = threadIdx.x;
+	if (t >= 64)
+		return;
+	const bool in = t < tc.nt;
+	double x = 0.0, y = 0.0;
+	if (in) {
+		x = u1[t] - u[t];
+		y = (u2[t] - u1[t]) - x;
+	}
+	double a0 = 0.0, b0 = 0.0, qx, qy;
+	if (step == 0) {
+		a0 = wave_sum(x) / tc.nt;
+		b0 = wave_sum(y) / tc.nt;
+		qx = in ? (x - a0) * (x - a0) : 0.0;
+		qy = in ? (y - b0) * (y - b0) : 0.0;
+	} else {
+		qx = x * x;
+		qy = y * y;
+	}
+	qx = wave_sum(qx);
+	qy = wave_sum(qy);
+	if (t == 0) {
+		out[0] = a0;
+		out[1] = qx;
+		out[2] = b0;
+		out[3] = qy;
+	}
+}
+
 // u1 = M(A) -> B, u2 = M(u1) -> C; r = u1 - u, v = (u2 - u1) - r into the
 // rows xr, xv (transcript order) when xr != NULL.  FB (monotonicity
-// fallback): the second map's stop test is ORed into ctl->stopbad2.
-template <bool W, bool FB>
+// fallback): the second map's stop test is ORed into ctl->stopbad2.  F:
+// the FP32 bulk form.
+template <bool W, bool FB, bool F>
 __global__ __launch_bounds__(PBLOCK) void k_ptile_map12(
     hemk_tiles tl, const double *__restrict__ A, double *__restrict__ B,
     double *__restrict__ C, const uint32_t *__restrict__ uq,
     const double *__restrict__ cntq, double *__restrict__ xr,
-    double *__restrict__ xv, hemk_ctl *__restrict__ ctl)
+    double *__restrict__ xv, double *__restrict__ part, int step,
+    hemk_ctl *__restrict__ ctl)
 {
 #pragma clang fp contract(off)
 	extern __shared__ double smem[];
@@ -255,9 +383,9 @@ __global__ __launch_bounds__(PBLOCK) void k_ptile_map12(
 		L.l0[i] = A[tc.tx_base + i];
 	__syncthreads();
 	int bad = 0;
-	pmap<W>(L, tc, L.l0, L.l1, false, 0.0, 0.0, bad);
-	pmap<W>(L, tc, L.l1, L.l2, FB, FB ? ctl->stop_min : 0.0,
-	        FB ? ctl->stop_rel : 0.0, bad);
+	pmapx<W, F>(L, tc, L.l0, L.l1, false, 0.0, 0.0, bad);
+	pmapx<W, F>(L, tc, L.l1, L.l2, FB, FB ? ctl->stop_min : 0.0,
+	            FB ? ctl->stop_rel : 0.0, bad);
 	if (FB) {
 		bad = __syncthreads_or(bad);
 		if (threadIdx.x == 0 && bad)
@@ -273,6 +401,9 @@ __global__ __launch_bounds__(PBLOCK) void k_ptile_map12(
 			xv[L.tx[i]] = (c - b) - r;
 		}
 	}
+	if (part)
+		ptile_parts(tc, L.l0, L.l1, L.l2, step,
+		            part + 4 * (size_t)blockIdx.x);
 }
 
 // The end of one SQUAREM iteration and the start of the next in one pass
@@ -281,14 +412,16 @@ __global__ __launch_bounds__(PBLOCK) void k_ptile_map12(
 // lam = M(u') -> A with the stop test (src/EM.c:354-355, ORed into
 // ctl->stopbad), then the next iteration's u1 = M(lam) -> B, u2 = M(u1) -> C
 // and, when xr != NULL, its r and v rows (src/EM.c:330-336).  If the stop
-// rule ends the loop here, the extra u1, u2, r, v are never read.
-template <bool W>
+// rule ends the loop here, the extra u1, u2, r, v are never read.  F: the
+// FP32 bulk form of every map.
+template <bool W, bool F>
 __global__ __launch_bounds__(PBLOCK) void k_ptile_ex3_map12(
     hemk_tiles tl, double *__restrict__ A, double *__restrict__ B,
     double *__restrict__ C, const uint32_t *__restrict__ uq,
     const double *__restrict__ cntq, const double *__restrict__ part,
     uint32_t T, hemk_step st, double *__restrict__ xr,
-    double *__restrict__ xv, hemk_ctl *__restrict__ ctl)
+    double *__restrict__ xv, double *__restrict__ tpart,
+    hemk_ctl *__restrict__ ctl)
 {
 #pragma clang fp contract(off)
 	extern __shared__ double smem[];
@@ -308,7 +441,7 @@ __global__ __launch_bounds__(PBLOCK) void k_ptile_ex3_map12(
 	}
 	__syncthreads();
 	int bad = 0;
-	pmap<W>(L, tc, L.l0, L.l1, true, smin, srel, bad);
+	pmapx<W, F>(L, tc, L.l0, L.l1, true, smin, srel, bad);
 	for (uint32_t i = threadIdx.x; i < tc.nt; i += PBLOCK)
 		A[tc.tx_base + i] = L.l1[i];
 	bad = __syncthreads_or(bad);
@@ -318,8 +451,8 @@ __global__ __launch_bounds__(PBLOCK) void k_ptile_ex3_map12(
 		if (blockIdx.x == 0 && st.alpha_given == 2)
 			ctl->alpha = alpha;
 	}
-	pmap<W>(L, tc, L.l1, L.l2, false, 0.0, 0.0, bad);
-	pmap<W>(L, tc, L.l2, L.l3, false, 0.0, 0.0, bad);
+	pmapx<W, F>(L, tc, L.l1, L.l2, false, 0.0, 0.0, bad);
+	pmapx<W, F>(L, tc, L.l2, L.l3, false, 0.0, 0.0, bad);
 	for (uint32_t i = threadIdx.x; i < tc.nt; i += PBLOCK) {
 		const double a = L.l1[i], b = L.l2[i], c = L.l3[i];
 		B[tc.tx_base + i] = b;
@@ -330,6 +463,9 @@ __global__ __launch_bounds__(PBLOCK) void k_ptile_ex3_map12(
 			xv[L.tx[i]] = (c - b) - r;
 		}
 	}
+	if (tpart)
+		ptile_parts(tc, L.l1, L.l2, L.l3, st.step,
+		            tpart + 4 * (size_t)blockIdx.x);
 }
 
 // Class sums of lin in member order (weights applied), stored as they are
@@ -493,6 +629,145 @@ __global__ __launch_bounds__(256) void k_fb_apply(
 		A[i] = C[i];
 }
 
+// Fixed-order sums over the workgroup of N values per thread: an xor
+// butterfly within each wave (every lane ends with the same bits, since
+// each level adds the same two operands in either order), then the waves'
+// sums in wave order.  Every thread returns the totals.
+#define TA_THREADS 1024
+#define TA_WAVES (TA_THREADS / 64)
+#define TA_PER 8
+template <int N>
+__device__ __forceinline__ void ta_block_sums(double (&x)[N],
+                                              double (*sh)[TA_WAVES])
+{
+#pragma clang fp contract(off)
+	const uint32_t w = threadIdx.x / 64;
+#pragma unroll
+	for (int k = 0; k < N; ++k)
+		x[k] = wave_sum(x[k]);
+	if (threadIdx.x % 64 == 0) {
+#pragma unroll
+		for (int k = 0; k < N; ++k)
+			sh[k][w] = x[k];
+	}
+	__syncthreads();
+#pragma unroll
+	for (int k = 0; k < N; ++k) {
+		double s = sh[k][0];
+		for (int j = 1; j < TA_WAVES; ++j)
+			s += sh[k][j];
+		x[k] = s;
+	}
+	__syncthreads();
+}
+
+// Tree norms on the tiles: fold the per-tile partials into alpha.  The L2
+// form adds the tiles' sums of squares; the sd form first finds the global
+// means m = sum n_i m_i / sum n_i, then the centred sums of squares as
+// sum_i M2_i + n_i (m_i - m)^2 (the tiles' moments moved to the global
+// mean), so no division sits on a dependent chain.  Thread i takes tiles i,
+// i + 1024, ... (loads coalesced across the block, TA_PER of them issued
+// before the first add).  With end the kernel first ends the iteration
+// whose extrapolated map wrote the partials (as k_iter_end: k, the loop
+// condition of src/EM.c:328 from the stop test) and folds only if the loop
+// goes on.
+__global__ __launch_bounds__(TA_THREADS) void k_ptile_tree_alpha(
+    hemk_tiles tl, const double *__restrict__ part, uint32_t T,
+    hemk_step st, int end, hemk_ctl *__restrict__ ctl)
+{
+#pragma clang fp contract(off)
+	__shared__ double sh[3][TA_WAVES];
+	__shared__ uint32_t fin;
+	if (ctl->done)
+		return;
+	const uint32_t tid = threadIdx.x;
+	if (end) {
+		if (tid == 0) {
+			const uint32_t stop = ctl->stopbad ? 0u : 1u;
+			const uint32_t k = ctl->k + 1;
+			ctl->stopbad = 0;
+			ctl->k = k;
+			ctl->maps += 3;
+			fin = !(k < ctl->min_iter ||
+			        (k < ctl->max_iter && !stop));
+			if (fin) {
+				ctl->done = 1;
+				ctl->converged = stop;
+			}
+		}
+		__syncthreads();
+		if (fin)
+			return;
+	}
+	const uint32_t nt = tl.n_tiles;
+	if (st.step != 0) {
+		double q[2] = {0.0, 0.0};
+		for (uint32_t base = 0; base < nt; base += TA_THREADS * TA_PER) {
+			double p1[TA_PER], p3[TA_PER];
+#pragma unroll
+			for (int k = 0; k < TA_PER; ++k) {
+				const uint32_t i = base + tid + TA_THREADS * k;
+				const bool in = i < nt;
+				p1[k] = in ? part[4 * (size_t)i + 1] : 0.0;
+				p3[k] = in ? part[4 * (size_t)i + 3] : 0.0;
+			}
+#pragma unroll
+			for (int k = 0; k < TA_PER; ++k) {
+				q[0] += p1[k];
+				q[1] += p3[k];
+			}
+		}
+		ta_block_sums(q, sh);
+		if (tid == 0)
+			ctl->alpha = em_step_length(q[0], q[1], T, st);
+		return;
+	}
+	double s[3] = {0.0, 0.0, 0.0};   // sum n_i, sum n_i m_r,i, sum n_i m_v,i
+	for (uint32_t base = 0; base < nt; base += TA_THREADS * TA_PER) {
+		double n[TA_PER], p0[TA_PER], p2[TA_PER];
+#pragma unroll
+		for (int k = 0; k < TA_PER; ++k) {
+			const uint32_t i = base + tid + TA_THREADS * k;
+			const bool in = i < nt;
+			n[k] = in ? (double)tl.desc[(size_t)HEMK_TILE_DESC * i + 1]
+			          : 0.0;
+			p0[k] = in ? part[4 * (size_t)i] : 0.0;
+			p2[k] = in ? part[4 * (size_t)i + 2] : 0.0;
+		}
+#pragma unroll
+		for (int k = 0; k < TA_PER; ++k) {
+			s[0] += n[k];
+			s[1] += n[k] * p0[k];
+			s[2] += n[k] * p2[k];
+		}
+	}
+	ta_block_sums(s, sh);
+	const double mr = s[1] / s[0], mv = s[2] / s[0];
+	double q[2] = {0.0, 0.0};
+	for (uint32_t base = 0; base < nt; base += TA_THREADS * TA_PER) {
+		double n[TA_PER], p[4][TA_PER];
+#pragma unroll
+		for (int k = 0; k < TA_PER; ++k) {
+			const uint32_t i = base + tid + TA_THREADS * k;
+			const bool in = i < nt;
+			n[k] = in ? (double)tl.desc[(size_t)HEMK_TILE_DESC * i + 1]
+			          : 0.0;
+#pragma unroll
+			for (int c = 0; c < 4; ++c)
+				p[c][k] = in ? part[4 * (size_t)i + c] : 0.0;
+		}
+#pragma unroll
+		for (int k = 0; k < TA_PER; ++k) {
+			const double dr = p[0][k] - mr, dv = p[2][k] - mv;
+			q[0] += p[1][k] + n[k] * dr * dr;
+			q[1] += p[3][k] + n[k] * dv * dv;
+		}
+	}
+	ta_block_sums(q, sh);
+	if (tid == 0)
+		ctl->alpha = em_step_length(q[0], q[1], T, st);
+}
+
 // Tree norms: alpha from the block partials of the norm passes.
 __global__ __launch_bounds__(256) void k_ptile_alpha(
     const double *__restrict__ part, uint32_t nparts, uint32_t T,
@@ -549,37 +824,51 @@ size_t hemk_ptile_lds(const hemk_tiles *t)
 
 int hemk_ptile_map12(const hemk_tiles *t, const double *A, double *B,
                      double *C, const uint32_t *uq, const double *cntq,
-                     double *xr, double *xv, const hemk_ctl *ctl, void *s)
+                     double *xr, double *xv, double *part, int step,
+                     int f32, const hemk_ctl *ctl, void *s)
 {
-	if (!t->n_tiles || t->max_tx > 256)
+	if (!t->n_tiles || t->max_tx > 64 || (f32 && t->weighted))
 		return (int)hipErrorInvalidValue;
 	// the kernel writes no control word without FB
 	hemk_ctl *c = (hemk_ctl *)ctl;
-	if (t->weighted)
-		k_ptile_map12<true, false><<<t->n_tiles, PBLOCK,
-		                             hemk_ptile_lds(t), (hipStream_t)s>>>(
-		    *t, A, B, C, uq, cntq, xr, xv, c);
+	const size_t lds = hemk_ptile_lds(t);
+	hipStream_t st = (hipStream_t)s;
+	if (f32)
+		k_ptile_map12<false, false, true><<<t->n_tiles, PBLOCK, lds, st>>>(
+		    *t, A, B, C, uq, cntq, xr, xv, part, step, c);
+	else if (t->weighted)
+		k_ptile_map12<true, false, false><<<t->n_tiles, PBLOCK, lds, st>>>(
+		    *t, A, B, C, uq, cntq, xr, xv, part, step, c);
 	else
-		k_ptile_map12<false, false><<<t->n_tiles, PBLOCK,
-		                              hemk_ptile_lds(t), (hipStream_t)s>>>(
-		    *t, A, B, C, uq, cntq, xr, xv, c);
+		k_ptile_map12<false, false, false><<<t->n_tiles, PBLOCK, lds, st>>>(
+		    *t, A, B, C, uq, cntq, xr, xv, part, step, c);
 	return (int)hipGetLastError();
 }
 
 int hemk_ptile_map12_fb(const hemk_tiles *t, const double *A, double *B,
                         double *C, const uint32_t *uq, const double *cntq,
-                        double *xr, double *xv, hemk_ctl *ctl, void *s)
+                        double *xr, double *xv, double *part, int step,
+                        hemk_ctl *ctl, void *s)
 {
-	if (!t->n_tiles || t->max_tx > 256)
+	if (!t->n_tiles || t->max_tx > 64)
 		return (int)hipErrorInvalidValue;
+	const size_t lds = hemk_ptile_lds(t);
+	hipStream_t st = (hipStream_t)s;
 	if (t->weighted)
-		k_ptile_map12<true, true><<<t->n_tiles, PBLOCK, hemk_ptile_lds(t),
-		                            (hipStream_t)s>>>(*t, A, B, C, uq, cntq,
-		                                              xr, xv, ctl);
+		k_ptile_map12<true, true, false><<<t->n_tiles, PBLOCK, lds, st>>>(
+		    *t, A, B, C, uq, cntq, xr, xv, part, step, ctl);
 	else
-		k_ptile_map12<false, true><<<t->n_tiles, PBLOCK,
-		                             hemk_ptile_lds(t), (hipStream_t)s>>>(
-		    *t, A, B, C, uq, cntq, xr, xv, ctl);
+		k_ptile_map12<false, true, false><<<t->n_tiles, PBLOCK, lds, st>>>(
+		    *t, A, B, C, uq, cntq, xr, xv, part, step, ctl);
+	return (int)hipGetLastError();
+}
+
+int hemk_ptile_tree_alpha(const hemk_tiles *t, const double *part,
+                          uint32_t T, hemk_step st, int end, hemk_ctl *ctl,
+                          void *s)
+{
+	k_ptile_tree_alpha<<<1, TA_THREADS, 0, (hipStream_t)s>>>(*t, part, T,
+	                                                        st, end, ctl);
 	return (int)hipGetLastError();
 }
 
@@ -629,19 +918,22 @@ int hemk_fb_apply(uint32_t T, const double *C, double *A,
 int hemk_ptile_ex3_map12(const hemk_tiles *t, double *A, double *B,
                          double *C, const uint32_t *uq, const double *cntq,
                          const double *part, uint32_t T, hemk_step st,
-                         double *xr, double *xv, hemk_ctl *ctl, void *s)
+                         double *xr, double *xv, double *tpart, int f32,
+                         hemk_ctl *ctl, void *s)
 {
-	if (!t->n_tiles || t->max_tx > 256)
+	if (!t->n_tiles || t->max_tx > 64 || (f32 && t->weighted))
 		return (int)hipErrorInvalidValue;
-	if (t->weighted)
-		k_ptile_ex3_map12<true><<<t->n_tiles, PBLOCK, hemk_ptile_lds(t),
-		                          (hipStream_t)s>>>(*t, A, B, C, uq, cntq,
-		                                            part, T, st, xr, xv,
-		                                            ctl);
+	const size_t lds = hemk_ptile_lds(t);
+	hipStream_t sm = (hipStream_t)s;
+	if (f32)
+		k_ptile_ex3_map12<false, true><<<t->n_tiles, PBLOCK, lds, sm>>>(
+		    *t, A, B, C, uq, cntq, part, T, st, xr, xv, tpart, ctl);
+	else if (t->weighted)
+		k_ptile_ex3_map12<true, false><<<t->n_tiles, PBLOCK, lds, sm>>>(
+		    *t, A, B, C, uq, cntq, part, T, st, xr, xv, tpart, ctl);
 	else
-		k_ptile_ex3_map12<false><<<t->n_tiles, PBLOCK,
-		                           hemk_ptile_lds(t), (hipStream_t)s>>>(
-		    *t, A, B, C, uq, cntq, part, T, st, xr, xv, ctl);
+		k_ptile_ex3_map12<false, false><<<t->n_tiles, PBLOCK, lds, sm>>>(
+		    *t, A, B, C, uq, cntq, part, T, st, xr, xv, tpart, ctl);
 	return (int)hipGetLastError();
 }
 

@@ -14,6 +14,7 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "em_step.h"
 #include "hem_dev.h"
 
 #define TBLOCK 256
@@ -586,25 +587,6 @@ __global__ __launch_bounds__(TBLOCK, 4) void k_tile_ex3(
 }
 
 // (n, mean, M2) folding of Chan, Golub & LeVeque (1979).
-struct Moments {
-	double n, m, q;
-};
-
-__device__ __forceinline__ Moments mom_merge(Moments a, Moments b)
-{
-#pragma clang fp contract(off)
-	if (a.n == 0)
-		return b;
-	if (b.n == 0)
-		return a;
-	Moments o;
-	o.n = a.n + b.n;
-	const double d = b.m - a.m;
-	o.m = a.m + d * (b.n / o.n);
-	o.q = a.q + b.q + d * d * (a.n * b.n / o.n);
-	return o;
-}
-
 // One block per instance: fold the tile partials of its sample in tile
 // order.  Partials are replicate-major (part[(q * n_tiles + tile) * 4]), so
 // each thread reads its run of tiles as one contiguous stream.

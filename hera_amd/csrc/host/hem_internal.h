@@ -95,6 +95,7 @@ struct hem_problem {
 	uint32_t *d_uq;
 	double *d_cntq;
 	double *d_fbpart;  /* fallback: [tile] {ell difference, magnitudes} */
+	double *d_tpart;   /* tree norms: [tile][4] partials of r and v      */
 	double *d_part1, *d_part2, *d_out;
 	/* exact sums by speculation: r/v rows [2][T], scratch per chain
 	 * (two chains), and the sums (pass 1 at [0..1], pass 2 at [2..3]) */

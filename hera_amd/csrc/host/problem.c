@@ -139,10 +139,14 @@ static int plain_tiles(hem_problem *p, const hem_classes *c,
 	if (rc == HEM_OK)
 		rc = dalloc(p, (void **)&p->d_cntq,
 		            (size_t)(th.n_cls ? th.n_cls : 1) * 8);
-	/* the monotonicity fallback's per-tile likelihood terms */
+	/* the monotonicity fallback's per-tile likelihood terms, and the
+	 * tree norms' per-tile partials */
 	if (rc == HEM_OK)
 		rc = dalloc(p, (void **)&p->d_fbpart,
 		            (size_t)(th.n_tiles ? th.n_tiles : 1) * 16);
+	if (rc == HEM_OK)
+		rc = dalloc(p, (void **)&p->d_tpart,
+		            (size_t)(th.n_tiles ? th.n_tiles : 1) * 32);
 	hem_tiles_free_host(&th);
 	return rc;
 }
@@ -566,6 +570,7 @@ int hem_problem_destroy(hem_problem *p)
 	hemd_free(p->d_snx);
 	hemd_free(p->d_sres);
 	hemd_free(p->d_fbpart);
+	hemd_free(p->d_tpart);
 	hemd_free(p->d_cls_w);
 	hemd_free(p->d_tx_w);
 	for (int i = 0; i < HEM_TILE_ARRAYS; ++i)

@@ -114,15 +114,19 @@ static hemk_spec spec_plain(const hem_problem *p, uint32_t nchain, int xf,
 	return sp;
 }
 
-/* FP64 solves run over the component tiles when every component fits a
- * tile (one kernel for u -> u1 -> u2, one for the extrapolated map). */
+/* Solves run over the component tiles when every component fits a tile
+ * (one kernel for u -> u1 -> u2, one for the extrapolated map): FP64, and
+ * the FP32 bulk form on tables without member weights (the fallback is
+ * FP64 only). */
 static int use_tiles(const hem_problem *p, const hem_opts *o)
 {
 	size_t lds = hemk_ptile_lds(&p->dtp);
+	const int f32 = o->precision == HEM_PREC_FP32;
 	/* the fallback's kernel also keeps the class sums at u2 */
 	if (o->fallback)
 		lds = (lds + 7) / 8 * 8 + 8 * (size_t)p->dtp.max_cls;
-	return p->dtp.n_tiles && o->precision == HEM_PREC_FP64 &&
+	return p->dtp.n_tiles &&
+	       (!f32 || (!p->weighted && !o->fallback)) &&
 	       (o->norm == HEM_NORM_EXACT || o->norm == HEM_NORM_TREE) &&
 	       lds <= HEM_PTILE_LDS_MAX;
 }
@@ -136,9 +140,19 @@ static int enqueue_tiles_prologue(hem_problem *p, const hem_opts *o)
 	const uint32_t T = p->n_tx;
 	HEM_TRY_DEV(hemk_ptile_map12(&p->dtp, p->d_A, p->d_B, p->d_C, p->d_uq,
 	                             p->d_cntq, exact ? p->d_sx : NULL,
-	                             exact ? p->d_sx + T : NULL, p->d_ctl,
+	                             exact ? p->d_sx + T : NULL,
+	                             exact ? NULL : p->d_tpart, o->step,
+	                             o->precision == HEM_PREC_FP32, p->d_ctl,
 	                             p->stream),
 	            "tile maps");
+	if (!exact) {
+		/* the first step length; each later one is folded by the
+		 * kernel that ends the iteration before it */
+		hemk_step st = {o->step, o->clamp, o->alpha_clamp != 0, 1};
+		HEM_TRY_DEV(hemk_ptile_tree_alpha(&p->dtp, p->d_tpart, T, st, 0,
+		                                  p->d_ctl, p->stream),
+		            "alpha");
+	}
 	return HEM_OK;
 }
 
@@ -163,7 +177,8 @@ static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
 		                                p->d_uq, p->d_cntq,
 		                                exact ? p->d_sx : NULL,
 		                                exact ? p->d_sx + T : NULL,
-		                                p->d_ctl, s),
+		                                exact ? NULL : p->d_tpart,
+		                                o->step, p->d_ctl, s),
 		            "tile maps");
 		MARK(HEM_PK_TMAP12);
 	}
@@ -194,22 +209,13 @@ static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
 			MARK(HEM_PK_NORM2);
 			alpha_part = p->d_sres + 2;
 		}
-	} else {
-		HEM_TRY_DEV(hemk_norm_pass1(T, p->d_A, p->d_B, p->d_C,
-		                            p->d_part1, o->step, p->d_ctl, s),
-		            "norm1");
-		if (sd) {
-			MARK(HEM_PK_NORM1);
-			HEM_TRY_DEV(hemk_norm_pass2(T, p->d_A, p->d_B, p->d_C,
-			                            p->d_part1, p->d_part2,
-			                            o->step, p->d_ctl, s),
-			            "norm2");
-		}
-		HEM_TRY_DEV(hemk_ptile_alpha(sd ? p->d_part2 : p->d_part1,
-		                             HEMK_NORM_BLOCKS, T, st, p->d_ctl,
-		                             s),
+	} else if (o->fallback) {
+		/* tree norms: the tile kernel that made u1 and u2 left each
+		 * tile's partials of r and v; fold them in tile order */
+		HEM_TRY_DEV(hemk_ptile_tree_alpha(&p->dtp, p->d_tpart, T, st, 0,
+		                                  p->d_ctl, s),
 		            "alpha");
-		MARK(sd ? HEM_PK_NORM2 : HEM_PK_NORM1);
+		MARK(HEM_PK_NORM1);
 	}
 	if (o->fallback) {
 		/* extrapolate and stabilise; keep u2 instead when that lowers
@@ -231,9 +237,20 @@ static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
 		                                 alpha_part, T, st,
 		                                 exact ? p->d_sx : NULL,
 		                                 exact ? p->d_sx + T : NULL,
+		                                 exact ? NULL : p->d_tpart,
+		                                 o->precision == HEM_PREC_FP32,
 		                                 p->d_ctl, s),
 		            "tile extrapolated map");
 		MARK(HEM_PK_TEX3);
+		if (!exact) {
+			/* tree norms: end this iteration and fold the next
+			 * step length from the partials just written */
+			HEM_TRY_DEV(hemk_ptile_tree_alpha(&p->dtp, p->d_tpart, T,
+			                                  st, 1, p->d_ctl, s),
+			            "iteration end and alpha");
+			MARK(HEM_PK_NORM1);
+			return HEM_OK;
+		}
 	}
 	HEM_TRY_DEV(hemk_iter_end(p->d_ctl, s), "iter_end");
 	MARK(HEM_PK_ITER_END);
@@ -594,8 +611,12 @@ static void kernel_bytes(const hem_problem *p, int f32, int rv, double *b)
 		const double S = 32 * NT + (double)p->ptile_rec_bytes + 4 * T +
 		                 8 * TC;
 		b[HEM_PK_TMAP12] = S + 8 * T + 16 * T + 16 * T;
-		/* A, B, C in; lam, u1, u2 out; r, v rows with the exact sums */
-		b[HEM_PK_TEX3] = S + 24 * T + 24 * T + (rv ? 16 * T : 0);
+		/* A, B, C in; lam, u1, u2 out; r, v rows with the exact sums,
+		 * or each tile's norm partials (the tree norms fold those:
+		 * 32 bytes and a transcript count per tile) */
+		b[HEM_PK_TEX3] = S + 24 * T + 24 * T + (rv ? 16 * T : 32 * NT);
+		if (!rv && p->dtp.n_tiles)
+			b[HEM_PK_NORM1] = 36 * NT;
 	}
 }
 

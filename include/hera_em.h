@@ -62,10 +62,13 @@ typedef struct {
 
 /* Bulk E-step precision.  FP64 (default): the class and transcript passes
  * add in the reference's order (members in class order, classes in
- * ascending id per transcript, (lam_t * n_c) / D_c per term).  FP32:
- * gathered densities, class denominators and per-transcript sums in FP32;
- * the M-step, SQUAREM state, norms, stop test and log-likelihood stay
- * FP64. */
+ * ascending id per transcript, (lam_t * n_c) / D_c per term).  FP32 (the
+ * bulk form): on the component tiles, class sums of the densities rounded
+ * to FP32, one FP32 n_c / D_c per class and FP32 terms, each transcript's
+ * terms summed in FP64 (tables without member weights); on the two global
+ * passes (a component over the tile limits), gathered densities, class
+ * denominators and per-transcript sums in FP32.  The M-step, SQUAREM
+ * state, norms, stop test and log-likelihood stay FP64 in both. */
 #define HEM_PREC_FP32 0
 #define HEM_PREC_FP64 1
 

@@ -362,3 +362,33 @@ def test_dropin_concurrent_bootstrap_threads():
         ora = O.estimate_count(cls, c)
         np.testing.assert_array_equal(out[q][0], ora["tpm"])
         np.testing.assert_array_equal(out[q][1], ora["est"])
+
+
+def test_fp32_bulk_on_tiles_cfg2():
+    """The FP32 bulk form runs on the component tiles at 200k transcripts
+    / 2M classes: with the oracle's iteration schedule (hera's stop rule)
+    the log-likelihood within 1e-9 relative of the FP64 oracle's; the
+    abundance errors are printed (FP32 terms carry ~6e-8 relative
+    rounding)."""
+    cls, u = H.generate(200000, 2000000, 60_000_000, seed=0)
+    ora = O.estimate_count(cls, u)
+    k = ora["k"]
+    p = H.Problem(cls)
+    try:
+        g = p.solve(u, precision=H.PREC_FP32, norm=H.NORM_TREE,
+                    min_iter=k, max_iter=k)
+        prof = p.profile(u, precision=H.PREC_FP32, norm=H.NORM_TREE,
+                         min_iter=k, max_iter=k)
+    finally:
+        p.close()
+    assert prof["tile_ex3_map12"]["launches"] > 0
+    assert prof["class_pass"]["launches"] == 0
+    e = rel_errs(g["tpm"], ora["tpm"])
+    ll_o = O.loglik(cls, u, ora["lam"])
+    ll_err = abs(g["stats"]["loglik"] - ll_o) / abs(ll_o)
+    say("fp32 tiles k=%d abund max %.3e median %.3e frac>1e-6 %.5f "
+        "loglik %.3e" % (k, e.max(), np.median(e), np.mean(e > 1e-6),
+                         ll_err))
+    assert g["stats"]["iters"] == k
+    assert ll_err <= 1e-9
+    assert np.median(e) <= 1e-6

@@ -635,7 +635,7 @@ __global__ __launch_bounds__(256) void k_fb_apply(
 // sums in wave order.  Every thread returns the totals.
 #define TA_THREADS 1024
 #define TA_WAVES (TA_THREADS / 64)
-#define TA_PER 8
+#define TA_PER 5
 template <int N>
 __device__ __forceinline__ void ta_block_sums(double (&x)[N],
                                               double (*sh)[TA_WAVES])
@@ -661,16 +661,37 @@ __device__ __forceinline__ void ta_block_sums(double (&x)[N],
 	__syncthreads();
 }
 
+// One round of the fold's loads: tile base + threadIdx.x + TA_THREADS k
+// for k < TA_PER, its transcript count and four partials (zeros past the
+// last tile).
+#define TA_STEP (TA_THREADS * TA_PER)
+__device__ __forceinline__ void ta_load(const hemk_tiles &tl,
+                                        const double *__restrict__ part,
+                                        uint32_t base, double (&n)[TA_PER],
+                                        double (&p)[4][TA_PER])
+{
+#pragma unroll
+	for (int k = 0; k < TA_PER; ++k) {
+		const uint32_t i = base + threadIdx.x + TA_THREADS * k;
+		const bool in = i < tl.n_tiles;
+		n[k] = in ? (double)tl.desc[(size_t)HEMK_TILE_DESC * i + 1] : 0.0;
+#pragma unroll
+		for (int c = 0; c < 4; ++c)
+			p[c][k] = in ? part[4 * (size_t)i + c] : 0.0;
+	}
+}
+
 // Tree norms on the tiles: fold the per-tile partials into alpha.  The L2
 // form adds the tiles' sums of squares; the sd form first finds the global
 // means m = sum n_i m_i / sum n_i, then the centred sums of squares as
 // sum_i M2_i + n_i (m_i - m)^2 (the tiles' moments moved to the global
 // mean), so no division sits on a dependent chain.  Thread i takes tiles i,
-// i + 1024, ... (loads coalesced across the block, TA_PER of them issued
-// before the first add).  With end the kernel first ends the iteration
-// whose extrapolated map wrote the partials (as k_iter_end: k, the loop
-// condition of src/EM.c:328 from the stop test) and folds only if the loop
-// goes on.
+// i + 1024, ... in rounds of TA_STEP tiles; the first round's loads are
+// issued before the control block is read, and the last round loaded stays
+// in registers for the second sum, which takes it first.  With end the
+// kernel first ends the iteration whose extrapolated map wrote the
+// partials (as k_iter_end: k, the loop condition of src/EM.c:328 from the
+// stop test) and folds only if the loop goes on.
 __global__ __launch_bounds__(TA_THREADS) void k_ptile_tree_alpha(
     hemk_tiles tl, const double *__restrict__ part, uint32_t T,
     hemk_step st, int end, hemk_ctl *__restrict__ ctl)
@@ -678,9 +699,11 @@ __global__ __launch_bounds__(TA_THREADS) void k_ptile_tree_alpha(
 #pragma clang fp contract(off)
 	__shared__ double sh[3][TA_WAVES];
 	__shared__ uint32_t fin;
+	const uint32_t tid = threadIdx.x, nt = tl.n_tiles;
+	double n[TA_PER], p[4][TA_PER];
+	ta_load(tl, part, 0, n, p);
 	if (ctl->done)
 		return;
-	const uint32_t tid = threadIdx.x;
 	if (end) {
 		if (tid == 0) {
 			const uint32_t stop = ctl->stopbad ? 0u : 1u;
@@ -699,23 +722,19 @@ __global__ __launch_bounds__(TA_THREADS) void k_ptile_tree_alpha(
 		if (fin)
 			return;
 	}
-	const uint32_t nt = tl.n_tiles;
+	uint32_t last = 0;
 	if (st.step != 0) {
 		double q[2] = {0.0, 0.0};
-		for (uint32_t base = 0; base < nt; base += TA_THREADS * TA_PER) {
-			double p1[TA_PER], p3[TA_PER];
+		for (uint32_t base = 0;;) {
 #pragma unroll
 			for (int k = 0; k < TA_PER; ++k) {
-				const uint32_t i = base + tid + TA_THREADS * k;
-				const bool in = i < nt;
-				p1[k] = in ? part[4 * (size_t)i + 1] : 0.0;
-				p3[k] = in ? part[4 * (size_t)i + 3] : 0.0;
+				q[0] += p[1][k];
+				q[1] += p[3][k];
 			}
-#pragma unroll
-			for (int k = 0; k < TA_PER; ++k) {
-				q[0] += p1[k];
-				q[1] += p3[k];
-			}
+			base += TA_STEP;
+			if (base >= nt)
+				break;
+			ta_load(tl, part, base, n, p);
 		}
 		ta_block_sums(q, sh);
 		if (tid == 0)
@@ -723,45 +742,33 @@ __global__ __launch_bounds__(TA_THREADS) void k_ptile_tree_alpha(
 		return;
 	}
 	double s[3] = {0.0, 0.0, 0.0};   // sum n_i, sum n_i m_r,i, sum n_i m_v,i
-	for (uint32_t base = 0; base < nt; base += TA_THREADS * TA_PER) {
-		double n[TA_PER], p0[TA_PER], p2[TA_PER];
-#pragma unroll
-		for (int k = 0; k < TA_PER; ++k) {
-			const uint32_t i = base + tid + TA_THREADS * k;
-			const bool in = i < nt;
-			n[k] = in ? (double)tl.desc[(size_t)HEMK_TILE_DESC * i + 1]
-			          : 0.0;
-			p0[k] = in ? part[4 * (size_t)i] : 0.0;
-			p2[k] = in ? part[4 * (size_t)i + 2] : 0.0;
-		}
+	for (uint32_t base = 0;;) {
 #pragma unroll
 		for (int k = 0; k < TA_PER; ++k) {
 			s[0] += n[k];
-			s[1] += n[k] * p0[k];
-			s[2] += n[k] * p2[k];
+			s[1] += n[k] * p[0][k];
+			s[2] += n[k] * p[2][k];
 		}
+		last = base;
+		base += TA_STEP;
+		if (base >= nt)
+			break;
+		ta_load(tl, part, base, n, p);
 	}
 	ta_block_sums(s, sh);
 	const double mr = s[1] / s[0], mv = s[2] / s[0];
+	// the round in registers (the last) first, then rounds 0 .. last - 1
 	double q[2] = {0.0, 0.0};
-	for (uint32_t base = 0; base < nt; base += TA_THREADS * TA_PER) {
-		double n[TA_PER], p[4][TA_PER];
-#pragma unroll
-		for (int k = 0; k < TA_PER; ++k) {
-			const uint32_t i = base + tid + TA_THREADS * k;
-			const bool in = i < nt;
-			n[k] = in ? (double)tl.desc[(size_t)HEMK_TILE_DESC * i + 1]
-			          : 0.0;
-#pragma unroll
-			for (int c = 0; c < 4; ++c)
-				p[c][k] = in ? part[4 * (size_t)i + c] : 0.0;
-		}
+	for (uint32_t base = 0;; base += TA_STEP) {
 #pragma unroll
 		for (int k = 0; k < TA_PER; ++k) {
 			const double dr = p[0][k] - mr, dv = p[2][k] - mv;
 			q[0] += p[1][k] + n[k] * dr * dr;
 			q[1] += p[3][k] + n[k] * dv * dv;
 		}
+		if (base >= last)
+			break;
+		ta_load(tl, part, base, n, p);
 	}
 	ta_block_sums(q, sh);
 	if (tid == 0)

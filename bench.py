@@ -19,6 +19,8 @@ Modes (--modes, first one is the headline):
               norms: every iterate bitwise the reference's;
   fp64-tree   the same E-step with fixed-shape parallel norms;
   fp32-tree   FP32 bulk E-step, FP64 M-step/SQUAREM, parallel norms.
+Bootstrap modes (--boot-modes): fp64-exact, fp64-tree, fp32-exact (FP32
+bulk E-step with the reference's FP64 norms), hera-sampler.
 Each is checked against the FP64 CPU oracle on the same schedule, and the
 oracle's own throughput on the host is the cpu_baseline.  The roofline is
 the dominant memory-streaming kernel of the headline mode, timed with HIP
@@ -68,8 +70,11 @@ def main():
     ap.add_argument("--bootstrap-reps", type=int, default=100,
                     help="bootstrap replicates of the batched run (0: skip)")
     ap.add_argument("--boot-modes",
-                    default="fp64-exact,fp64-tree,hera-sampler",
+                    default="fp64-exact,fp64-tree,fp32-exact,hera-sampler",
                     help="bootstrap modes to run")
+    ap.add_argument("--seq-sample-fp32", type=int, default=10,
+                    help="replicates solved one by one in the FP32 mode "
+                         "(each runs to the iteration cap), scaled to all")
     ap.add_argument("--seq-sample", type=int, default=-1,
                     help="replicates solved one by one for the unbatched "
                          "comparison (-1: all of them; 0: skip)")
@@ -597,13 +602,18 @@ def bench_bootstrap(args, H, np, p, cls, u, sk):
     R = args.bootstrap_reps
     seeds = np.arange(R, dtype=np.uint64) + 1
     res = {"n_rep": R, "seeds": "1..%d" % R}
-    bmodes = [(n, m) for n, m in (("fp64-exact", H.NORM_EXACT),
-                                  ("fp64-tree", H.NORM_TREE))
-              if n in args.boot_modes.split(",")]
+    # fp32-exact: the FP32 bulk E-step with the reference's FP64 norms
+    # (the precision form of the batched workload); every replicate runs
+    # to the iteration cap, since FP32 E-step noise stays above a 1e-8
+    # relative change
+    allm = (("fp64-exact", dict(norm=H.NORM_EXACT)),
+            ("fp64-tree", dict(norm=H.NORM_TREE)),
+            ("fp32-exact", dict(norm=H.NORM_EXACT, precision=H.PREC_FP32)))
+    bmodes = [(n, m) for n, m in allm if n in args.boot_modes.split(",")]
     for name, nm in bmodes:
-        p.bootstrap(u, seeds[:2], norm=nm, **sk)   # capture the graphs
+        p.bootstrap(u, seeds[:2], **nm, **sk)   # capture the graphs
         t = time.perf_counter()
-        b = p.bootstrap(u, seeds, norm=nm, **sk)
+        b = p.bootstrap(u, seeds, **nm, **sk)
         dt = time.perf_counter() - t
         st = b["stats"]
         res[name] = dict(reps_per_s=R / dt, seconds=dt,
@@ -654,19 +664,23 @@ def bench_bootstrap(args, H, np, p, cls, u, sk):
     # replicate after another (all R unless --seq-sample asks for fewer)
     if args.seq_sample == 0:
         return res
-    n_seq = R if args.seq_sample < 0 else min(args.seq_sample, R)
+    n_all = R if args.seq_sample < 0 else min(args.seq_sample, R)
     uo, co = p.bootstrap_counts(u, seeds)
     for name, nm in bmodes:
+        n_seq = n_all
+        if name.startswith("fp32"):
+            n_seq = max(1, min(n_all, args.seq_sample_fp32))
         its = []
         t = time.perf_counter()
         for r in range(n_seq):
             p.set_counts(co[r])
-            its.append(p.solve(uo[r], norm=nm, **sk)["stats"]["iters"])
+            its.append(p.solve(uo[r], **nm, **sk)["stats"]["iters"])
         dt = (time.perf_counter() - t) * R / n_seq
         its = np.array(its)
         res[name]["sequential_plain_s"] = dt
         res[name]["sequential_plain_reps_per_s"] = R / dt
         res[name]["speedup_vs_sequential_gpu"] = dt / res[name]["seconds"]
+        res[name]["sequential_replicates_timed"] = n_seq
         res[name]["sequential_iters"] = {
             "min": int(its.min()), "max": int(its.max()),
             "total": int(its.sum()),
@@ -678,12 +692,12 @@ def bench_bootstrap(args, H, np, p, cls, u, sk):
                                        its.min(), its.max(), its.sum()))
     p.set_counts(cls.cls_count)
     res["sequential_sample"] = (
-        "%d of %d replicates solved one by one (hem_problem_set_counts, "
-        "then hem_solve), whole calls timed%s"
-        % (n_seq, R, "" if n_seq == R else ", scaled to all"))
+        "the first n of %d replicates (sequential_replicates_timed per "
+        "mode) solved one by one (hem_problem_set_counts, then hem_solve), "
+        "whole calls timed, scaled to all" % R)
     # the batched kernels launch by launch (HIP events) on the same counts
     for name, nm in bmodes:
-        pr = p.profile_batch(uo, co, norm=nm, **sk)
+        pr = p.profile_batch(uo, co, **nm, **sk)
         kern = {}
         for k in ("tile_map12", "norm_pass1", "norm_pass2", "tile_ex3",
                   "iter_end"):
@@ -741,7 +755,7 @@ def bootstrap_cpu(args, np, O, cls, u, boot):
     if gpu is not None:
         boot["parity_exact_vs_oracle"] = {"replicates": nthr,
                                           "est_bitwise_equal": all(same)}
-    for name in ("fp64-exact", "fp64-tree"):
+    for name in ("fp64-exact", "fp64-tree", "fp32-exact"):
         if name in boot:
             boot[name]["speedup_vs_cpu"] = (boot[name]["reps_per_s"]
                                             / (nthr / dt))

@@ -146,7 +146,9 @@ int hemk_tile_chunk(const hemk_tiles *t, uint32_t R, int cnt_bytes);
 /* Tile map kernels.  Layouts, rows in tile position order q (transcripts
  * of a tile are consecutive; txq maps a transcript id to q): state A/B/C
  * [q][R] FP64 (replicate fastest), unique counts [q][R] u32, class counts
- * [class tile position][R] (u32 when cnt_bytes == 4, FP64 when 8).
+ * [class tile position][R] u32.  f32: the FP32 bulk form of every map
+ * (FP32 class sums, class factors n_c / D_c and terms, FP64 transcript
+ * sums, as hemk_ptile_map12 computes it).
  *   map12: u1 = M(A) -> B, u2 = M(B) -> C; with part != NULL also the
  *          per-(tile, replicate) sums of r and v (Chan form) for the
  *          parallel norms, part[(r*n_tiles + tile)*4 ...]; with xr the
@@ -155,14 +157,14 @@ int hemk_tile_chunk(const hemk_tiles *t, uint32_t R, int cnt_bytes);
  *   ex3:   extrapolate from A, B, C (alpha[r] when st.alpha_given, else
  *          from q_r/q_v), map once -> A, stop test -> notconv[r]. */
 int hemk_tile_map12(const hemk_tiles *t, hemk_rctl rc, int chunk,
-                    const void *cnt, int cnt_bytes, const uint32_t *uniq,
+                    const uint32_t *cnt, int f32, const uint32_t *uniq,
                     const double *A, double *B, double *C, double *part,
                     double *xr, double *xv, uint32_t T, int step, void *s);
 /* ex3 with next != 0 also maps twice more for the next iteration:
  * B = M(A), C = M(B) and the r, v rows (xr, xv) or tree partials (part),
  * as map12 does. */
 int hemk_tile_ex3(const hemk_tiles *t, hemk_rctl rc, int chunk,
-                  const void *cnt, int cnt_bytes, const uint32_t *uniq,
+                  const uint32_t *cnt, int f32, const uint32_t *uniq,
                   double *A, double *B, double *C, uint32_t T,
                   hemk_step st, int next, double *part, double *xr,
                   double *xv, void *s);

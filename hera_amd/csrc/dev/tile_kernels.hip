@@ -81,19 +81,93 @@ __device__ __forceinline__ void stage_struct(const hemk_tiles &tl,
 //   transcript pass  lam'_t = (u_t + sum of (lam_t n_c) / D_c over its
 //                    occurrences in class order) / len_t
 // which is src/EM.c:275-291 with every sum in the reference's order.  With
-// stopf the stop test of src/EM.c:292-297 marks stopf[r].
+// stopf the stop test of src/EM.c:292-297 marks stopf[r].  F: the FP32 bulk
+// form, as the single-problem tile kernels compute it: FP32 class sums of
+// the densities rounded to FP32, rho_c = n_c / D_c in FP32 (0 for a zero
+// sum) in the D slots, FP32 terms lam_t rho_c summed per transcript in FP64
+// in the same order.
 //
 // The class pass stores 1 in place of a D_c that is 0: densities are >= 0,
 // so D_c == 0 means every member, the transcript whose term it is
 // included, has lam == 0, and (0 n_c) / 1 adds the exact 0 that the
 // reference's skip of that class leaves (src/EM.c:279-280).
 template <typename TC, int RC>
+__device__ __forceinline__ void tile_map32(
+    const TileCtx &tc, const TileStruct &ts, const double *lin, double *lout,
+    double *D, const TC *n, const uint32_t *u, const int *alive,
+    double *gout, uint32_t R, uint32_t qr, int *stopf, double smin,
+    double srel)
+{
+#pragma clang fp contract(off)
+	const int r = threadIdx.x % RC;
+	const uint32_t e0 = threadIdx.x / RC;
+	constexpr uint32_t ES = TBLOCK / RC;
+	const bool live = alive[r] != 0;
+	float *rho = (float *)D;
+	if (live) {
+		for (uint32_t c = e0; c < tc.nc; c += ES) {
+			uint32_t k = ts.coff[c];
+			const uint32_t k1 = ts.coff[c + 1];
+			float acc = 0.0f;
+			for (; k + 4 <= k1; k += 4) {
+				const float x0 = (float)lin[(ts.kmem[k] & 0xFFu) * RC + r];
+				const float x1 = (float)lin[(ts.kmem[k + 1] & 0xFFu) * RC + r];
+				const float x2 = (float)lin[(ts.kmem[k + 2] & 0xFFu) * RC + r];
+				const float x3 = (float)lin[(ts.kmem[k + 3] & 0xFFu) * RC + r];
+				acc += x0;
+				acc += x1;
+				acc += x2;
+				acc += x3;
+			}
+			for (; k < k1; ++k)
+				acc += (float)lin[(ts.kmem[k] & 0xFFu) * RC + r];
+			rho[c * RC + r] =
+			    acc == 0.0f ? 0.0f : (float)n[c * RC + r] / acc;
+		}
+	}
+	__syncthreads();
+	if (live) {
+		for (uint32_t t = e0; t < tc.nt; t += ES) {
+			const double l = lin[t * RC + r];
+			const float lf = (float)l;
+			uint32_t j = ts.toff[t];
+			const uint32_t j1 = ts.toff[t + 1];
+			double acc = 0.0;
+			for (; j + 4 <= j1; j += 4) {
+				float x[4];
+#pragma unroll
+				for (int q = 0; q < 4; ++q)
+					x[q] = lf * rho[ts.tcls[j + q] * RC + r];
+#pragma unroll
+				for (int q = 0; q < 4; ++q)
+					acc += (double)x[q];
+			}
+			for (; j < j1; ++j)
+				acc += (double)(lf * rho[ts.tcls[j] * RC + r]);
+			const double nw = ((double)u[t * RC + r] + acc) /
+			                  (double)ts.len[t];
+			lout[t * RC + r] = nw;
+			if (gout)
+				gout[(size_t)(tc.tx_base + t) * R + qr] = nw;
+			if (stopf && nw > smin && fabs(nw - l) / nw > srel)
+				stopf[r] = 1;
+		}
+	}
+	__syncthreads();
+}
+
+template <typename TC, int RC, bool F>
 __device__ __forceinline__ void tile_map(
     const hemk_tiles &tl, const TileCtx &tc, const TileStruct &ts,
     const double *lin, double *lout, double *D, const TC *n,
     const uint32_t *u, const int *alive, double *gout, uint32_t R,
     uint32_t qr, int *stopf, double smin, double srel)
 {
+	if (F) {
+		tile_map32<TC, RC>(tc, ts, lin, lout, D, n, u, alive, gout, R,
+		                   qr, stopf, smin, srel);
+		return;
+	}
 	const int r = threadIdx.x % RC;
 	const uint32_t e0 = threadIdx.x / RC;
 	constexpr uint32_t ES = TBLOCK / RC;
@@ -385,7 +459,7 @@ __device__ __forceinline__ void rv_parts(const hemk_tiles &tl,
 	}
 }
 
-template <typename TC, int RC>
+template <typename TC, int RC, bool F>
 __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
     hemk_tiles tl, hemk_rctl rc, const TC *__restrict__ cnt,
     const uint32_t *__restrict__ uniq, const double *__restrict__ A,
@@ -434,9 +508,9 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
 		if (i + 1 < nch)
 			load_rows<TC, RC, 1>(p, tc, rc, al, nrep, i + 1, cnt, uniq,
 			                     A, nullptr, nullptr);
-		tile_map<TC, RC>(tl, tc, ts, lam0, lam1, D, n, u, alive, B,
+		tile_map<TC, RC, F>(tl, tc, ts, lam0, lam1, D, n, u, alive, B,
 		                 rc.R, q, nullptr, 0, 0);
-		tile_map<TC, RC>(tl, tc, ts, lam1, lam2, D, n, u, alive, C,
+		tile_map<TC, RC, F>(tl, tc, ts, lam1, lam2, D, n, u, alive, C,
 		                 rc.R, q, nullptr, 0, 0);
 		rv_parts<TC, RC>(tl, tc, ts, tile, lam0, lam1, lam2, q, xr, xv,
 		                 part, T, step);
@@ -500,7 +574,7 @@ __device__ __forceinline__ void load_alpha(ChunkRows<TC, 3> &p,
 	}
 }
 
-template <typename TC, int RC>
+template <typename TC, int RC, bool F>
 // four waves per SIMD (at most 128 VGPRs), as many as the LDS budget allows
 __global__ __launch_bounds__(TBLOCK, 4) void k_tile_ex3(
     hemk_tiles tl, hemk_rctl rc, const TC *__restrict__ cnt,
@@ -568,16 +642,16 @@ __global__ __launch_bounds__(TBLOCK, 4) void k_tile_ex3(
 			                     A, B, C);
 			load_alpha<TC>(p, rc, smp, st);
 		}
-		tile_map<TC, RC>(tl, tc, ts, lam0, lam1, D, n, u, alive, A,
+		tile_map<TC, RC, F>(tl, tc, ts, lam0, lam1, D, n, u, alive, A,
 		                 rc.R, q, stopf, rc.stop_min, rc.stop_rel);
 		if (threadIdx.x < RC && rin && stopf[threadIdx.x])
 			rc.notconv[smp * rc.R + q] = 1;
 		if (next) {
 			// the next iteration's u1 -> B, u2 -> C (into lam0, whose
 			// extrapolated values are spent) and its r, v
-			tile_map<TC, RC>(tl, tc, ts, lam1, lam2, D, n, u, alive, B,
+			tile_map<TC, RC, F>(tl, tc, ts, lam1, lam2, D, n, u, alive, B,
 			                 rc.R, q, nullptr, 0, 0);
-			tile_map<TC, RC>(tl, tc, ts, lam2, lam0, D, n, u, alive, C,
+			tile_map<TC, RC, F>(tl, tc, ts, lam2, lam0, D, n, u, alive, C,
 			                 rc.R, q, nullptr, 0, 0);
 			rv_parts<TC, RC>(tl, tc, ts, tile, lam1, lam2, lam0, q, xr,
 			                 xv, part, T, st.step);
@@ -933,31 +1007,31 @@ size_t tile_lds(const hemk_tiles *t, int rc, int cnt_bytes)
 	       2 * ((size_t)t->max_cls + 1) + 2 * ((size_t)t->max_tx + 1) + 16;
 }
 
-template <typename TC, int RC>
-int launch_map12(const hemk_tiles *t, hemk_rctl rc, const void *cnt,
+template <int RC, bool F>
+int launch_map12(const hemk_tiles *t, hemk_rctl rc, const uint32_t *cnt,
                  const uint32_t *uniq, const double *A, double *B, double *C,
                  double *part, double *xr, double *xv, uint32_t T, int step,
                  hipStream_t s)
 {
-	const size_t lds = tile_lds(t, RC, sizeof(TC));
+	const size_t lds = tile_lds(t, RC, sizeof(uint32_t));
 	if (rc.chunk != (uint32_t)RC)
 		return (int)hipErrorInvalidValue;
-	k_tile_map12<TC, RC><<<t->n_tiles, TBLOCK, lds, s>>>(
-	    *t, rc, (const TC *)cnt, uniq, A, B, C, part, xr, xv, T, step);
+	k_tile_map12<uint32_t, RC, F><<<t->n_tiles, TBLOCK, lds, s>>>(
+	    *t, rc, cnt, uniq, A, B, C, part, xr, xv, T, step);
 	return (int)hipGetLastError();
 }
 
-template <typename TC, int RC>
-int launch_ex3(const hemk_tiles *t, hemk_rctl rc, const void *cnt,
+template <int RC, bool F>
+int launch_ex3(const hemk_tiles *t, hemk_rctl rc, const uint32_t *cnt,
                const uint32_t *uniq, double *A, double *B, double *C,
                uint32_t T, hemk_step st, int next, double *part, double *xr,
                double *xv, hipStream_t s)
 {
-	const size_t lds = tile_lds(t, RC, sizeof(TC));
+	const size_t lds = tile_lds(t, RC, sizeof(uint32_t));
 	if (rc.chunk != (uint32_t)RC)
 		return (int)hipErrorInvalidValue;
-	k_tile_ex3<TC, RC><<<t->n_tiles, TBLOCK, lds, s>>>(
-	    *t, rc, (const TC *)cnt, uniq, A, B, C, T, st, next, part, xr, xv);
+	k_tile_ex3<uint32_t, RC, F><<<t->n_tiles, TBLOCK, lds, s>>>(
+	    *t, rc, cnt, uniq, A, B, C, T, st, next, part, xr, xv);
 	return (int)hipGetLastError();
 }
 
@@ -993,25 +1067,25 @@ int hemk_tile_chunk(const hemk_tiles *t, uint32_t R, int cnt_bytes)
 }
 
 int hemk_tile_map12(const hemk_tiles *t, hemk_rctl rc, int chunk,
-                    const void *cnt, int cnt_bytes, const uint32_t *uniq,
+                    const uint32_t *cnt, int f32, const uint32_t *uniq,
                     const double *A, double *B, double *C, double *part,
                     double *xr, double *xv, uint32_t T, int step, void *s)
 {
 	hipStream_t st = (hipStream_t)s;
 	if (t->n_tiles == 0)
 		return 0;
-	if (cnt_bytes == 4) {
-#define M12U(RC) launch_map12<uint32_t, RC>(t, rc, cnt, uniq, A, B, C, part, xr, xv, T, step, st)
-		HEM_RC_SWITCH(chunk, M12U)
-#undef M12U
+	if (f32) {
+#define M12F(RC) launch_map12<RC, true>(t, rc, cnt, uniq, A, B, C, part, xr, xv, T, step, st)
+		HEM_RC_SWITCH(chunk, M12F)
+#undef M12F
 	}
-#define M12D(RC) launch_map12<double, RC>(t, rc, cnt, uniq, A, B, C, part, xr, xv, T, step, st)
+#define M12D(RC) launch_map12<RC, false>(t, rc, cnt, uniq, A, B, C, part, xr, xv, T, step, st)
 	HEM_RC_SWITCH(chunk, M12D)
 #undef M12D
 }
 
 int hemk_tile_ex3(const hemk_tiles *t, hemk_rctl rc, int chunk,
-                  const void *cnt, int cnt_bytes, const uint32_t *uniq,
+                  const uint32_t *cnt, int f32, const uint32_t *uniq,
                   double *A, double *B, double *C, uint32_t T,
                   hemk_step st, int next, double *part, double *xr,
                   double *xv, void *s)
@@ -1019,12 +1093,12 @@ int hemk_tile_ex3(const hemk_tiles *t, hemk_rctl rc, int chunk,
 	hipStream_t sm = (hipStream_t)s;
 	if (t->n_tiles == 0)
 		return 0;
-	if (cnt_bytes == 4) {
-#define EXU(RC) launch_ex3<uint32_t, RC>(t, rc, cnt, uniq, A, B, C, T, st, next, part, xr, xv, sm)
-		HEM_RC_SWITCH(chunk, EXU)
-#undef EXU
+	if (f32) {
+#define EXF(RC) launch_ex3<RC, true>(t, rc, cnt, uniq, A, B, C, T, st, next, part, xr, xv, sm)
+		HEM_RC_SWITCH(chunk, EXF)
+#undef EXF
 	}
-#define EXD(RC) launch_ex3<double, RC>(t, rc, cnt, uniq, A, B, C, T, st, next, part, xr, xv, sm)
+#define EXD(RC) launch_ex3<RC, false>(t, rc, cnt, uniq, A, B, C, T, st, next, part, xr, xv, sm)
 	HEM_RC_SWITCH(chunk, EXD)
 #undef EXD
 }

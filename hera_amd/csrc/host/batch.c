@@ -330,7 +330,8 @@ static int batch_prologue(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 	const int exact = o->norm != HEM_NORM_TREE;
 	const size_t TR = (size_t)p->n_tx * rc.R;
 	double *xr = exact ? w->X : NULL, *xv = exact ? w->X + TR : NULL;
-	HEM_TRY_DEV(hemk_tile_map12(&p->dt, rc, chunk, w->cnt, 4, w->uq, w->A,
+	HEM_TRY_DEV(hemk_tile_map12(&p->dt, rc, chunk, w->cnt,
+	                            o->precision == HEM_PREC_FP32, w->uq, w->A,
 	                            w->B, w->C, exact ? NULL : w->part, xr, xv,
 	                            p->n_tx, o->step, p->stream),
 	            "tile map12");
@@ -393,7 +394,8 @@ static int batch_iteration(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 		            "batch alpha");
 		BMARK(HEM_BK_NORM1);
 	}
-	HEM_TRY_DEV(hemk_tile_ex3(&p->dt, rc, chunk, w->cnt, 4, w->uq, w->A,
+	HEM_TRY_DEV(hemk_tile_ex3(&p->dt, rc, chunk, w->cnt,
+	                          o->precision == HEM_PREC_FP32, w->uq, w->A,
 	                          w->B, w->C, p->n_tx, st, 1,
 	                          exact ? NULL : w->part, xr, xv, s),
 	            "tile ex3");
@@ -406,7 +408,8 @@ static int batch_iteration(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 static int batch_graph_key(const hem_opts *o, int chunk)
 {
 	return o->step | (o->clamp << 1) | ((o->alpha_clamp != 0) << 2) |
-	       (o->norm << 3) | (chunk << 5);
+	       (o->norm << 3) | (chunk << 5) |
+	       ((o->precision == HEM_PREC_FP32) << 12);
 }
 
 static int batch_graph(hem_problem *p, const hem_opts *o, hemk_rctl rc,
@@ -455,9 +458,8 @@ static int check_batch_opts(const hem_problem *p, const hem_opts *o)
 	if (p->weighted)
 		return hem_fail(HEM_ERR_INVALID, "batched solves take tables "
 		                                 "without member weights");
-	if (o->precision != HEM_PREC_FP64)
-		return hem_fail(HEM_ERR_INVALID,
-		                "batched solves compute in FP64 (precision %d)",
+	if (o->precision != HEM_PREC_FP64 && o->precision != HEM_PREC_FP32)
+		return hem_fail(HEM_ERR_INVALID, "unknown precision %d",
 		                o->precision);
 	if (o->step != HEM_STEP_SD && o->step != HEM_STEP_L2)
 		return hem_fail(HEM_ERR_INVALID, "unknown step form %d", o->step);

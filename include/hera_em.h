@@ -203,8 +203,9 @@ int hem_loglik(hem_problem *p, const uint32_t *unique, const double *lambda,
  * Replicates share the device class table and run together, R lanes per
  * component tile ([transcript][replicate] layouts, replicate fastest);
  * each keeps its own iteration count, step length and stop test, exactly as
- * R separate hem_solve calls would (bitwise so with HEM_PREC_FP64 and
- * HEM_NORM_EXACT or HEM_NORM_SEQ).  Batches compute in FP64 (precision must be FP64).
+ * R separate hem_solve calls would: bitwise so with HEM_NORM_EXACT in
+ * either precision (the FP32 bulk form is the arithmetic of hem_solve's
+ * tile kernels) and with HEM_NORM_SEQ in FP64.
  * They need every connected component of the class graph to fit a tile
  * (hem_problem_tiles reports n_tiles = 0 otherwise); class counts must be
  * integers below 2^32. */

@@ -186,7 +186,48 @@ def test_bootstrap_after_plain_solves_on_same_handle(cfg1):
     np.testing.assert_array_equal(g["lam"], g2["lam"])
 
 
-def test_batch_rejects_untileable_and_fp32():
+@pytest.mark.parametrize("case", ["cfg1", "zero_component"])
+def test_batch_fp32_equals_plain_fp32(cfg1, case):
+    """The FP32 bulk form batched: each replicate is hem_solve's FP32 solve
+    of its own counts, bit for bit, with exact norms (the same arithmetic
+    on small and large tiles), iteration count included."""
+    if case == "cfg1":
+        cls, u = cfg1
+        U, CC = perturbed(cls, u, 4, 31)
+    else:
+        cls, u = zero_component_problem(5)
+        U, CC = perturbed(cls, u, 3, 32)
+        U[:, -4:] = 0
+        CC[:, -3:] = 0
+    kw = dict(precision=H.PREC_FP32, norm=H.NORM_EXACT,
+              stop_rel_change=1e-6, max_iter=300)
+    p = H.Problem(cls)
+    res = p.solve_batch(U, CC, **kw)
+    tree = p.solve_batch(U, CC, precision=H.PREC_FP32, norm=H.NORM_TREE,
+                         stop_rel_change=1e-6, max_iter=300)
+    p.close()
+    say("%s fp32 iters %s (tree %s)" % (case, res["iters"].tolist(),
+                                        tree["iters"].tolist()))
+    for r in range(U.shape[0]):
+        cr = with_counts(cls, CC[r])
+        q = H.Problem(cr)
+        g = q.solve(U[r], **kw)
+        q.close()
+        assert res["iters"][r] == g["stats"]["iters"]
+        np.testing.assert_array_equal(res["est"][r], g["est"])
+        np.testing.assert_array_equal(res["tpm"][r], g["tpm"])
+        # the FP32 form against the FP64 oracle on the same counts, held
+        # to the log-likelihood bar of that mode
+        ora = O.estimate_count(cr, U[r], stop_rel=1e-6, max_round=300)
+        l32 = O.loglik(cr, U[r], g["lam"])
+        l64 = O.loglik(cr, U[r], ora["lam"])
+        say("replicate %d: k %d (oracle %d), loglik rel diff %.2e" % (
+            r, g["stats"]["iters"], ora["k"], abs(l32 - l64) / abs(l64)))
+        assert abs(l32 - l64) <= 1e-9 * abs(l64)
+        assert np.all(np.isfinite(tree["est"][r]))
+
+
+def test_batch_rejects_untileable_and_unknown_precision():
     members = [[i, i + 1] for i in range(99)]
     big = H.Classes.from_lists(100, members, [3.0] * 99, [500] * 100)
     p = H.Problem(big)
@@ -198,7 +239,7 @@ def test_batch_rejects_untileable_and_fp32():
     cls, u = tiny_problem(1)
     p = H.Problem(cls)
     with pytest.raises(H.HemError) as ei:
-        p.solve_batch(np.stack([u, u]), precision=H.PREC_FP32)
+        p.solve_batch(np.stack([u, u]), precision=7)
     assert ei.value.code == _lib.HEM_ERR_INVALID
     p.close()
 

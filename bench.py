@@ -80,12 +80,19 @@ def main():
                          "comparison (-1: all of them; 0: skip)")
     ap.add_argument("--cpu-boot-threads", type=int, default=16,
                     help="threads (= replicates) of the CPU bootstrap sample")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="use the default tile configuration instead of "
+                         "the autotuner's")
+    ap.add_argument("--tune-cache", default=None,
+                    help="autotuner cache file (default: none; tune now)")
     ap.add_argument("--skip-plain", action="store_true",
                     help="skip the plain solves (profiling runs of the other "
                          "sections); the JSON line then has no headline")
     ap.add_argument("--ragged-samples", type=int, default=64,
                     help="samples of the ragged multi-sample batch (0: skip)")
     ap.add_argument("--ragged-reps", type=int, default=20)
+    ap.add_argument("--fallback-solves", type=int, default=3,
+                    help="solves with the monotonicity fallback (0: skip)")
     ap.add_argument("--weighted-solves", type=int, default=3,
                     help="solves of the workload with member weights "
                          "(hera-EM's per-alignment probabilities); 0 skips")
@@ -116,15 +123,30 @@ def main():
     cls, u = H.generate(args.n_tx, args.n_cls, args.n_frag, seed=args.seed)
     log("generated T=%d C=%d Z=%d in %.2fs" % (cls.n_tx, cls.n_cls, cls.nnz,
                                                time.time() - t0))
+    # the tile configuration: the autotuner's choice for this table and
+    # replicate count (timed before, and outside, every measured region)
+    cfg, tuning = None, None
+    if not args.no_tune:
+        t0 = time.time()
+        cfg, rep = H.autotune(cls, u, n_rep=max(args.bootstrap_reps, 0),
+                              iters=20, repeats=2, cache=args.tune_cache)
+        tuning = dict(config=cfg, seconds=time.time() - t0, **rep)
+        log("autotune %.1fs (%d candidates%s): %s; plain %.1f -> %.1f us/it, "
+            "batched %.2f -> %.2f us per replicate-iteration" % (
+                tuning["seconds"], rep["n_candidates"],
+                ", cached" if rep["from_cache"] else "", cfg,
+                rep["us_plain_default"], rep["us_plain_best"],
+                rep["us_batch_default"], rep["us_batch_best"]))
     t0 = time.time()
-    p = H.Problem(cls)
+    p = H.Problem(cls, config=cfg)
     t_upload = time.time() - t0
     log("uploaded class table in %.3fs" % t_upload)
 
     sk = dict(stop_rel_change=args.stop_rel, min_iter=args.min_iter,
               max_iter=args.max_iter)
     if args.skip_plain:
-        out = {"metric": METRIC, "skipped": "plain solves (--skip-plain)"}
+        out = {"metric": METRIC, "skipped": "plain solves (--skip-plain)",
+               "tuning": tuning}
         # the copy of known size that calibrates profiler byte counters
         out["copy_bandwidth_GBps"] = H.copy_bandwidth(1 << 30, 10)
         if args.bootstrap_reps > 0:
@@ -242,6 +264,7 @@ def main():
         "kernels_headline": {k: dict(prof[k]) for k in kinds
                              if prof[k]["launches"]},
         "upload_s": t_upload,
+        "tuning": tuning,
     }
 
     out["copy_bandwidth_GBps"] = H.copy_bandwidth(1 << 30, 10)
@@ -258,6 +281,9 @@ def main():
 
     if args.weighted_solves > 0:
         out["weighted"] = bench_weighted(args, H, np, cls, u, sk)
+
+    if args.fallback_solves > 0:
+        out["fallback"] = bench_fallback(args, H, p, u, sk, head_result)
 
     boot = None
     if args.bootstrap_reps > 0:
@@ -465,6 +491,33 @@ def bench_writer(H, np, cls, u, res):
             "note": "hem_write_result: abundance.tsv + abundance.gene.tsv "
                     "(4 transcripts per gene), including packing the name "
                     "blocks in Python"}
+
+
+def bench_fallback(args, H, p, u, sk, plain):
+    """The monotonicity fallback (hem_opts.fallback) on the headline table:
+    FP64 exact solves whose iterations end at u2 when the extrapolated map
+    lowers the log-likelihood, whole calls timed; the final log-likelihood
+    beside the plain solve's."""
+    mk = dict(precision=H.PREC_FP64, norm=H.NORM_EXACT, fallback=1)
+    p.solve(u, **mk, **sk)
+    it = 0
+    t = time.perf_counter()
+    for _ in range(args.fallback_solves):
+        r = p.solve(u, **mk, **sk)
+        it += r["stats"]["iters"]
+    dt = time.perf_counter() - t
+    st = r["stats"]
+    res = dict(solves=args.fallback_solves, iters_per_solve=st["iters"],
+               iters_per_s=it / dt,
+               ms_per_solve=dt / args.fallback_solves * 1e3,
+               fallbacks_per_solve=st["fallbacks"], loglik=st["loglik"],
+               loglik_plain=plain["stats"]["loglik"] if plain else None,
+               iters_plain=plain["stats"]["iters"] if plain else None)
+    log("fallback  %d solves %.4fs  %.1f it/s  k=%d (%d fell back); loglik "
+        "%.12e vs plain %.12e" % (args.fallback_solves, dt, it / dt,
+                                  st["iters"], st["fallbacks"], st["loglik"],
+                                  res["loglik_plain"] or 0.0))
+    return res
 
 
 def bench_weighted(args, H, np, cls, u, sk):

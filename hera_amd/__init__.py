@@ -214,17 +214,77 @@ def opts(**kw):
     return o
 
 
+def tile_config(config=None):
+    """A hem_tile_config from a dict (missing fields: the defaults)."""
+    g = _lib.HemTileConfig()
+    _lib.load().hem_tile_config_default(g)
+    for k, v in (config or {}).items():
+        if not hasattr(g, k):
+            raise KeyError(k)
+        setattr(g, k, int(v))
+    return g
+
+
+def autotune(classes, unique, n_rep=0, iters=20, repeats=3, cache=None):
+    """Tile configuration for a class table, timed on the device
+    (hem_autotune).  Returns (config dict, report dict)."""
+    u = as_u32(unique)
+    assert u.shape == (classes.n_tx,)
+    o = _lib.HemTuneOpts(n_rep, iters, repeats,
+                         os.fsencode(cache) if cache else None)
+    g = _lib.HemTileConfig()
+    r = _lib.HemTuneReport()
+    c = classes.to_c()
+    check(_lib.load().hem_autotune(c, u32p(u), o, g, r))
+    cfg = {k: getattr(g, k) for k, _ in _lib.HemTileConfig._fields_}
+    rep = {k: getattr(r, k) for k, _ in _lib.HemTuneReport._fields_}
+    rep["key"] = r.key.decode()
+    return cfg, rep
+
+
+def tune_key(n_tx, n_cls, nnz, n_rep, arch):
+    buf = _lib.C.create_string_buffer(160)
+    check(_lib.load().hem_tune_key(n_tx, n_cls, nnz, n_rep, arch.encode(),
+                                   buf, 160))
+    return buf.value.decode()
+
+
+def tune_cache_lookup(path, key):
+    g = _lib.HemTileConfig()
+    check(_lib.load().hem_tune_cache_lookup(os.fsencode(path), key.encode(),
+                                            g))
+    return {k: getattr(g, k) for k, _ in _lib.HemTileConfig._fields_}
+
+
+def tune_cache_store(path, key, config):
+    check(_lib.load().hem_tune_cache_store(os.fsencode(path), key.encode(),
+                                           tile_config(config)))
+
+
 class Problem:
     """A device-resident class table (hem_problem)."""
 
-    def __init__(self, classes):
+    def __init__(self, classes, config=None):
+        """config: a tile configuration (dict with the fields of
+        hem_tile_config, e.g. from autotune(); missing fields keep their
+        defaults), or None for the defaults."""
         lib = _lib.load()
         self.classes = classes
         self._c = classes.to_c()
         self._h = _lib.C.c_void_p()
-        check(lib.hem_problem_create(self._c, self._h))
+        if config is None:
+            check(lib.hem_problem_create(self._c, self._h))
+        else:
+            check(lib.hem_problem_create_config(self._c, tile_config(config),
+                                                self._h))
         self.n_tx = classes.n_tx
         self.n_samples = 1
+
+    def config(self):
+        """The tile configuration of the handle (hem_problem_get_config)."""
+        g = _lib.HemTileConfig()
+        check(_lib.load().hem_problem_get_config(self._h, g))
+        return {k: getattr(g, k) for k, _ in _lib.HemTileConfig._fields_}
 
     @classmethod
     def from_samples(cls, samples):

@@ -111,6 +111,26 @@ class HemSeStats(C.Structure):
                 ("ms_total", C.c_double)]
 
 
+class HemTileConfig(C.Structure):
+    _fields_ = [("ptile_tx", C.c_uint32), ("ptile_cls", C.c_uint32),
+                ("ptile_nnz", C.c_uint32), ("tile_tx", C.c_uint32),
+                ("tile_cls", C.c_uint32), ("tile_nnz", C.c_uint32),
+                ("chunk", C.c_uint32)]
+
+
+class HemTuneOpts(C.Structure):
+    _fields_ = [("n_rep", C.c_uint32), ("iters", C.c_uint32),
+                ("repeats", C.c_uint32), ("cache_path", C.c_char_p)]
+
+
+class HemTuneReport(C.Structure):
+    _fields_ = [("from_cache", C.c_int), ("n_candidates", C.c_uint32),
+                ("us_plain_default", C.c_double),
+                ("us_plain_best", C.c_double),
+                ("us_batch_default", C.c_double),
+                ("us_batch_best", C.c_double), ("key", C.c_char * 160)]
+
+
 class HemGenParams(C.Structure):
     _fields_ = [("n_tx", C.c_uint32), ("n_cls_all", C.c_uint32),
                 ("n_frag", C.c_uint64), ("cluster_min", C.c_uint32),
@@ -135,6 +155,23 @@ SIGNATURES = {
     "hem_opts_default": (None, [C.POINTER(HemOpts)]),
     "hem_problem_create": (C.c_int, [C.POINTER(HemClasses), C.POINTER(C.c_void_p)]),
     "hem_problem_destroy": (C.c_int, [C.c_void_p]),
+    "hem_tile_config_default": (None, [C.POINTER(HemTileConfig)]),
+    "hem_problem_create_config": (C.c_int, [C.POINTER(HemClasses),
+                                            C.POINTER(HemTileConfig),
+                                            C.POINTER(C.c_void_p)]),
+    "hem_problem_get_config": (C.c_int, [C.c_void_p,
+                                         C.POINTER(HemTileConfig)]),
+    "hem_autotune": (C.c_int, [C.POINTER(HemClasses), _u32p,
+                               C.POINTER(HemTuneOpts),
+                               C.POINTER(HemTileConfig),
+                               C.POINTER(HemTuneReport)]),
+    "hem_tune_key": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint64,
+                               C.c_uint32, C.c_char_p, C.c_char_p,
+                               C.c_size_t]),
+    "hem_tune_cache_lookup": (C.c_int, [C.c_char_p, C.c_char_p,
+                                        C.POINTER(HemTileConfig)]),
+    "hem_tune_cache_store": (C.c_int, [C.c_char_p, C.c_char_p,
+                                       C.POINTER(HemTileConfig)]),
     "hem_samples_create": (C.c_int, [C.POINTER(HemClasses), C.c_uint32,
                                      C.POINTER(C.c_void_p)]),
     "hem_problem_set_counts": (C.c_int, [C.c_void_p, _f64p]),

@@ -481,9 +481,6 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
 	uint32_t *u;
 	TileStruct ts;
 	carve<TC, RC>(smem, tl, lam0, lam1, lam2, D, n, u, ts);
-	const int r = threadIdx.x % RC;
-	const uint32_t e0 = threadIdx.x / RC;
-	constexpr uint32_t ES = TBLOCK / RC;
 	ChunkRows<TC, 1> p;
 	load_rows<TC, RC, 1>(p, tc, rc, al, nrep, 0, cnt, uniq, A, nullptr,
 	                     nullptr);

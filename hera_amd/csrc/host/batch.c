@@ -75,6 +75,14 @@ int hem_tiles_upload(hem_problem *p, const hem_tiles_host *h, hemk_tiles *dt,
 	return HEM_OK;
 }
 
+int hem_batch_chunk(const hem_problem *p, uint32_t R)
+{
+	int m = hemk_tile_chunk(&p->dt, R, 4);
+	if (p->cfg.chunk && (int)p->cfg.chunk < m)
+		m = (int)p->cfg.chunk;
+	return m;
+}
+
 int hem_problem_tiles(const hem_problem *p, uint32_t *n_tiles,
                       uint32_t *max_tx, uint32_t *max_cls)
 {
@@ -507,7 +515,7 @@ static int run_batch(hem_problem *p, const hem_opts *o, uint32_t R,
 {
 	hem_batch_ws *w = &p->bw;
 	hemk_rctl rc = rctl_for(p, R, o);
-	const int chunk = hemk_tile_chunk(&p->dt, R, 4);
+	const int chunk = hem_batch_chunk(p, R);
 	void *graph = NULL;
 	uint32_t launched = 0;
 	const uint32_t limit = o->max_iter > o->min_iter ? o->max_iter
@@ -758,7 +766,7 @@ int hem_profile_batch(hem_problem *p, uint32_t n_rep, const uint32_t *unique,
 	{
 		hem_batch_ws *w = &p->bw;
 		hemk_rctl bc = rctl_for(p, n_rep, &o);
-		const int chunk = hemk_tile_chunk(&p->dt, n_rep, 4);
+		const int chunk = hem_batch_chunk(p, n_rep);
 		const uint32_t limit = o.max_iter > o.min_iter ? o.max_iter
 		                                                : o.min_iter;
 		uint32_t act = n_rep * p->ns;

@@ -58,6 +58,12 @@ int hem_device_ready(void)
 	return HEM_OK;
 }
 
+const char *hem_device_arch(void)
+{
+	pthread_once(&hem_once, hem_init_once);
+	return hem_arch;
+}
+
 /* Device-to-device copy bandwidth (read + write bytes per second), the
  * achievable reference beside the HBM peak and the known-size kernel that
  * scales rocprof's FETCH_SIZE / WRITE_SIZE counters. */

@@ -73,6 +73,7 @@ struct hem_problem {
 	pthread_mutex_t mu;
 	uint32_t n_tx, n_cls;
 	uint64_t nnz;
+	hem_tile_config cfg;     /* tile targets and batched chunk */
 	uint64_t dev_bytes;
 	int group_cls, group_tx;
 	void *stream;
@@ -248,6 +249,11 @@ void hem_hera_em_free(hem_problem *p);
 int hem_last_code(void);
 int hem_dev_fail(int hip_code, const char *what);
 int hem_device_ready(void);
+/* The current device's arch name (after hem_device_ready). */
+const char *hem_device_arch(void);
+/* Replicates per batched workgroup for R replicates: the configured chunk,
+ * lowered to the largest whose rows fit the LDS budget (0: none fits). */
+int hem_batch_chunk(const hem_problem *p, uint32_t R);
 
 #define HEM_TRY_DEV(call, what)                                   \
 	do {                                                      \

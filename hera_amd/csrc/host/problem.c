@@ -131,8 +131,8 @@ static int plain_tiles(hem_problem *p, const hem_classes *c,
 	hem_tiles_host th;
 	int rc = hem_tiles_build(c->n_tx, c->n_cls, c->cls_off, c->cls_ref,
 	                         c->tx_len, tx_off, NULL, 1, c->member_weight,
-	                         HEM_PTILE_TX_TARGET, HEM_PTILE_CLS_TARGET,
-	                         HEM_PTILE_NNZ_TARGET, &th);
+	                         p->cfg.ptile_tx, p->cfg.ptile_cls,
+	                         p->cfg.ptile_nnz, &th);
 	if (rc == HEM_OK)
 		rc = hem_tiles_upload(p, &th, &p->dtp, p->d_ptile_mem,
 		                      &p->d_cls_orig_p, &p->ptile_rec_bytes);
@@ -151,8 +151,40 @@ static int plain_tiles(hem_problem *p, const hem_classes *c,
 	return rc;
 }
 
+void hem_tile_config_default(hem_tile_config *cfg)
+{
+	if (!cfg)
+		return;
+	cfg->ptile_tx = HEM_PTILE_TX_TARGET;
+	cfg->ptile_cls = HEM_PTILE_CLS_TARGET;
+	cfg->ptile_nnz = HEM_PTILE_NNZ_TARGET;
+	cfg->tile_tx = HEM_TILE_TX_TARGET;
+	cfg->tile_cls = HEM_TILE_CLS_TARGET;
+	cfg->tile_nnz = HEM_TILE_NNZ_TARGET;
+	cfg->chunk = 0;
+}
+
+static int check_config(const hem_tile_config *g)
+{
+	const uint32_t ch = g->chunk;
+	if (!g->ptile_tx || !g->ptile_cls || !g->ptile_nnz || !g->tile_tx ||
+	    !g->tile_cls || !g->tile_nnz)
+		return hem_fail(HEM_ERR_INVALID, "tile targets must be > 0");
+	if (g->ptile_tx > HEM_TILE_TX || g->tile_tx > HEM_TILE_TX)
+		return hem_fail(HEM_ERR_INVALID, "tile transcript targets are at "
+		                                 "most %u", HEM_TILE_TX);
+	if (g->ptile_cls > HEM_TILE_CLS || g->tile_cls > HEM_TILE_CLS)
+		return hem_fail(HEM_ERR_INVALID, "tile class targets are at "
+		                                 "most %u", HEM_TILE_CLS);
+	if (ch > 32 || (ch & (ch - 1)))
+		return hem_fail(HEM_ERR_INVALID, "chunk must be 0 or a power of "
+		                                 "two up to 32 (%u)", ch);
+	return HEM_OK;
+}
+
 static int create_impl(const hem_classes *c, const uint32_t *tx0,
-                       const uint32_t *cls0, uint32_t ns, hem_problem **out)
+                       const uint32_t *cls0, uint32_t ns,
+                       const hem_tile_config *cfg, hem_problem **out)
 {
 	int rc;
 	hem_problem *p;
@@ -167,6 +199,8 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 		return hem_fail(HEM_ERR_INVALID, "null output handle");
 	*out = NULL;
 	if ((rc = validate(c)))
+		return rc;
+	if (cfg && (rc = check_config(cfg)))
 		return rc;
 	if ((rc = hem_device_ready()))
 		return rc;
@@ -224,6 +258,10 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 	p->n_cls = C;
 	p->nnz = nnz;
 	p->ns = ns;
+	if (cfg)
+		p->cfg = *cfg;
+	else
+		hem_tile_config_default(&p->cfg);
 	p->group_cls = 4;
 	p->group_tx = 16;
 
@@ -382,8 +420,8 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 		hem_tiles_host th;
 		int trc = hem_tiles_build(T, C, c->cls_off, c->cls_ref,
 		                          c->tx_len, tx_off, tx0, ns, NULL,
-		                          HEM_TILE_TX_TARGET, HEM_TILE_CLS_TARGET,
-		                          HEM_TILE_NNZ_TARGET, &th);
+		                          p->cfg.tile_tx, p->cfg.tile_cls,
+		                          p->cfg.tile_nnz, &th);
 		if (trc == HEM_OK) {
 			trc = hem_tiles_upload(p, &th, &p->dt, p->d_tile_mem,
 			                       &p->d_cls_orig, NULL);
@@ -441,7 +479,21 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 
 int hem_problem_create(const hem_classes *c, hem_problem **out)
 {
-	return create_impl(c, NULL, NULL, 1, out);
+	return create_impl(c, NULL, NULL, 1, NULL, out);
+}
+
+int hem_problem_create_config(const hem_classes *c,
+                              const hem_tile_config *cfg, hem_problem **out)
+{
+	return create_impl(c, NULL, NULL, 1, cfg, out);
+}
+
+int hem_problem_get_config(const hem_problem *p, hem_tile_config *cfg)
+{
+	if (!p || !cfg)
+		return hem_fail(HEM_ERR_INVALID, "null argument");
+	*cfg = p->cfg;
+	return HEM_OK;
 }
 
 int hem_samples_create(const hem_classes *samples, uint32_t n_samples,
@@ -509,7 +561,7 @@ int hem_samples_create(const hem_classes *samples, uint32_t n_samples,
 	all.cls_count = cnt;
 	all.tx_len = len;
 	all.member_weight = NULL;
-	rc = create_impl(&all, tx0, cls0, n_samples, out);
+	rc = create_impl(&all, tx0, cls0, n_samples, NULL, out);
 out:
 	free(tx0);
 	free(cls0);

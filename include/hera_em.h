@@ -238,6 +238,84 @@ int hem_problem_tiles(const hem_problem *p, uint32_t *n_tiles,
 int hem_classes_tiling(const hem_classes *cls, uint32_t *n_tiles,
                        uint32_t *max_tx, uint32_t *max_cls);
 
+/* ---- tile configuration and the autotuner --------------------------------
+ * No counterpart in the reference (a CPU loop has no tiles).  The packing
+ * targets of the two tilings and the batched kernels' replicate chunk.
+ * Results do not depend on them in the modes whose sums follow the
+ * reference's order (HEM_NORM_EXACT / HEM_NORM_SEQ, either precision):
+ * every class and transcript sum runs in the order its list gives,
+ * whichever tile holds it.  The tree norms' grouping follows the tiles, and
+ * so do their last bits. */
+typedef struct {
+	uint32_t ptile_tx, ptile_cls, ptile_nnz; /* single-problem tile kernels:
+	                                            64 / 320 / 1024           */
+	uint32_t tile_tx, tile_cls, tile_nnz;    /* batched and standard-error
+	                                            kernels: 24 / 120 / 576   */
+	uint32_t chunk;  /* replicates per batched workgroup (1, 2, 4, ...,
+	                    32): the most rows of one entity a wave serves at
+	                    once is 64 / chunk; 0 = the largest chunk whose
+	                    rows fit the LDS budget (default)              */
+} hem_tile_config;
+
+/* The defaults above. */
+void hem_tile_config_default(hem_tile_config *cfg);
+/* hem_problem_create with a tile configuration (NULL: the defaults).
+ * Targets are packing targets: a component larger than a target but within
+ * the tile limits (64 transcripts, 320 classes) gets a tile of its own.
+ * HEM_ERR_INVALID for a target of 0, ptile_tx or tile_tx above 64,
+ * ptile_cls or tile_cls above 320, or a chunk that is not 0 or a power of
+ * two up to 32.  A chunk larger than fits the LDS budget is lowered to
+ * the largest that fits. */
+int hem_problem_create_config(const hem_classes *cls,
+                              const hem_tile_config *cfg, hem_problem **out);
+/* The configuration a handle was created with. */
+int hem_problem_get_config(const hem_problem *p, hem_tile_config *cfg);
+
+typedef struct {
+	uint32_t n_rep;       /* replicates of the batched workload to tune
+	                         for; 0: tune the single-problem tiles only   */
+	uint32_t iters;       /* SQUAREM iterations per timed run (default 20
+	                         when 0); every run does exactly this many    */
+	uint32_t repeats;     /* timed runs per candidate, fastest kept
+	                         (default 3 when 0)                           */
+	const char *cache_path; /* text file of tuned configurations, or NULL;
+	                         read first, appended to after a search       */
+} hem_tune_opts;
+
+typedef struct {
+	int      from_cache;       /* 1: the configuration came from the file */
+	uint32_t n_candidates;     /* configurations timed                    */
+	double   us_plain_default; /* device time per SQUAREM iteration of    */
+	double   us_plain_best;    /* a plain exact solve (default, chosen)   */
+	double   us_batch_default; /* device time per replicate-iteration of  */
+	double   us_batch_best;    /* the batched exact solve                 */
+	char     key[160];         /* the cache key (T, C, Z, R, device arch) */
+} hem_tune_report;
+
+/* Picks the tile configuration for a class table by timing candidates on
+ * the current device: the single-problem tile targets (transcripts and
+ * members per tile) on plain exact solves of unique, and, with n_rep > 0,
+ * the batched tile targets and replicate chunk on an exact batch of n_rep
+ * replicates of the table's own counts.  Each candidate is a temporary
+ * handle; the defaults are among the candidates, so the choice is never
+ * slower than them as measured.  With cache_path, a configuration stored
+ * under the same key (n_tx, n_cls, members, n_rep, device arch) is
+ * returned without timing, and a new search appends one line to the file
+ * (concurrent tuners of one key may both append; the last line read
+ * wins).  rep may be NULL. */
+int hem_autotune(const hem_classes *cls, const uint32_t *unique,
+                 const hem_tune_opts *o, hem_tile_config *cfg,
+                 hem_tune_report *rep);
+/* Host only: the cache key of a table and replicate count on a device
+ * arch, and a lookup of it in a cache file (HEM_ERR_INVALID when absent or
+ * malformed; the last matching line wins). */
+int hem_tune_key(uint32_t n_tx, uint32_t n_cls, uint64_t nnz,
+                 uint32_t n_rep, const char *arch, char *key, size_t len);
+int hem_tune_cache_lookup(const char *path, const char *key,
+                          hem_tile_config *cfg);
+int hem_tune_cache_store(const char *path, const char *key,
+                         const hem_tile_config *cfg);
+
 /* n_rep EM solves with per-replicate counts: unique[r*n_tx + t] and
  * cls_count[r*n_cls + c] (NULL: the table's own class counts for every
  * replicate).  Outputs (any may be NULL): est[r*n_tx + t] (est_counts),

@@ -1,0 +1,89 @@
+"""The autotuner on the device, and tile configurations against each other.
+
+In the order-faithful modes (exact norms, FP64 or the FP32 bulk form) every
+class and transcript sum runs in the order its list gives, whichever tile
+holds it, so any tile configuration gives the default's results bit for
+bit; the autotuner can only change speed.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import hera_amd as H
+
+pytestmark = pytest.mark.gpu
+
+STOP = dict(stop_rel_change=1e-8)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+CONFIGS = [
+    dict(ptile_tx=32, ptile_cls=160, ptile_nnz=512, tile_tx=16, tile_cls=80,
+         tile_nnz=384, chunk=8),
+    dict(ptile_tx=64, ptile_cls=320, ptile_nnz=2048, tile_tx=32,
+         tile_cls=160, tile_nnz=768, chunk=32),
+    dict(ptile_tx=5, ptile_cls=9, ptile_nnz=40, tile_tx=3, tile_cls=4,
+         tile_nnz=10, chunk=2),
+]
+
+
+@pytest.fixture(scope="module")
+def reference(cfg1):
+    cls, u = cfg1
+    rng = np.random.default_rng(3)
+    U = np.array([rng.poisson(u) for _ in range(5)], dtype=np.uint32)
+    p = H.Problem(cls)
+    out = dict(plain=p.solve(u, **STOP),
+               f32=p.solve(u, precision=H.PREC_FP32, max_iter=200),
+               batch=p.solve_batch(U, **STOP), U=U,
+               tiles=p.tiles())
+    p.close()
+    return out
+
+
+@pytest.mark.parametrize("ci", range(len(CONFIGS)))
+def test_config_results_equal_default(cfg1, reference, ci):
+    cls, u = cfg1
+    cfg = CONFIGS[ci]
+    p = H.Problem(cls, config=cfg)
+    try:
+        assert p.config() == cfg
+        t = p.tiles()
+        say("config %d: %d batched tiles (default %d)" % (
+            ci, t["n_tiles"], reference["tiles"]["n_tiles"]))
+        g = p.solve(u, **STOP)
+        np.testing.assert_array_equal(g["lam"], reference["plain"]["lam"])
+        assert g["stats"]["iters"] == reference["plain"]["stats"]["iters"]
+        f = p.solve(u, precision=H.PREC_FP32, max_iter=200)
+        np.testing.assert_array_equal(f["lam"], reference["f32"]["lam"])
+        b = p.solve_batch(reference["U"], **STOP)
+        np.testing.assert_array_equal(b["iters"], reference["batch"]["iters"])
+        np.testing.assert_array_equal(b["est"], reference["batch"]["est"])
+        assert b["stats"]["chunk"] <= cfg["chunk"]
+    finally:
+        p.close()
+
+
+def test_autotune_and_cache(cfg1, reference, tmp_path):
+    cls, u = cfg1
+    cache = str(tmp_path / "tune.txt")
+    cfg, rep = H.autotune(cls, u, n_rep=8, iters=10, repeats=1, cache=cache)
+    say("tuned %s; report %s" % (cfg, rep))
+    assert rep["from_cache"] == 0 and rep["n_candidates"] >= 6
+    assert rep["us_plain_best"] <= rep["us_plain_default"]
+    assert rep["us_batch_best"] <= rep["us_batch_default"]
+    assert rep["key"].startswith("T=%d,C=%d,Z=%d,R=8,arch=gfx950" % (
+        cls.n_tx, cls.n_cls, cls.nnz))
+    cfg2, rep2 = H.autotune(cls, u, n_rep=8, cache=cache)
+    assert rep2["from_cache"] == 1 and cfg2 == cfg
+    p = H.Problem(cls, config=cfg)
+    try:
+        g = p.solve(u, **STOP)
+    finally:
+        p.close()
+    np.testing.assert_array_equal(g["lam"], reference["plain"]["lam"])

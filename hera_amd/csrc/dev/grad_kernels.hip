@@ -70,13 +70,19 @@ __device__ __forceinline__ double bmax(double a)
 	return a;
 }
 
-// Sum of the NB block partials, the same in every block.
-__device__ __forceinline__ double fold(const double *__restrict__ part)
+// Sum of np block (or tile) partials, the same in every block.
+__device__ __forceinline__ double foldn(const double *__restrict__ part,
+                                        uint32_t np)
 {
 	double a = 0;
-	for (int i = threadIdx.x; i < NB; i += GB)
+	for (uint32_t i = threadIdx.x; i < np; i += GB)
 		a += part[i];
 	return bsum(a);
+}
+
+__device__ __forceinline__ double fold(const double *__restrict__ part)
+{
+	return foldn(part, NB);
 }
 
 __global__ __launch_bounds__(GB) void k_gmax(const double *__restrict__ th,
@@ -222,13 +228,13 @@ __global__ __launch_bounds__(GB) void k_gfin(
     const double *__restrict__ E, const double *__restrict__ pi,
     const uint32_t *__restrict__ unique, uint32_t T,
     const double *__restrict__ partN, const double *__restrict__ partL1,
-    const double *__restrict__ partL2, double *__restrict__ gth,
+    const double *__restrict__ partL2, uint32_t np, double *__restrict__ gth,
     double *__restrict__ gell, double *__restrict__ a,
     double *__restrict__ fd, double *__restrict__ out)
 {
-	const double N = fold(partN);
-	const double L1 = fold(partL1);
-	const double L2 = fold(partL2);
+	const double N = foldn(partN, np);
+	const double L1 = foldn(partL1, np);
+	const double L2 = foldn(partL2, np);
 	if (blockIdx.x == 0 && threadIdx.x == 0) {
 		out[0] = L1 + L2;
 		out[1] = N;
@@ -241,6 +247,16 @@ __global__ __launch_bounds__(GB) void k_gfin(
 		a[t] = g - (double)unique[t];
 		fd[t] += N * p * (1 - p);
 	}
+}
+
+// out[b] = the sum of part[b np .. (b+1) np), one block each.
+__global__ __launch_bounds__(GB) void k_gfold(const double *__restrict__ part,
+                                              uint32_t np,
+                                              double *__restrict__ out)
+{
+	const double v = foldn(part + (size_t)blockIdx.x * np, np);
+	if (threadIdx.x == 0)
+		out[blockIdx.x] = v;
 }
 
 // Block partials of sum_t a_t b_t.
@@ -341,8 +357,22 @@ int hemk_grad_passes(const hemk_prob *p, int fp32, const double *x,
 		                                    p->cnt64, x, unique, p->n_tx,
 		                                    E, fd, pN, pL2);
 	}
-	k_gfin<<<NB, GB, 0, st>>>(E, pi, unique, p->n_tx, pN, pL1, pL2, gth,
-	                          gell, a, fd, scal);
+	k_gfin<<<NB, GB, 0, st>>>(E, pi, unique, p->n_tx, pN, pL1, pL2, NB,
+	                          gth, gell, a, fd, scal);
+	return (int)hipGetLastError();
+}
+
+int hemk_grad_fin(const double *E, const double *pi, const uint32_t *unique,
+                  uint32_t T, double *part, uint32_t np, double *gth,
+                  double *gell, double *a, double *fd, double *scal, void *s)
+{
+	hipStream_t st = (hipStream_t)s;
+	double *tot = part + 3 * (size_t)np;
+	// the three runs of partials to one value each first, so the blocks
+	// of k_gfin do not each fold all of them
+	k_gfold<<<3, GB, 0, st>>>(part, np, tot);
+	k_gfin<<<NB, GB, 0, st>>>(E, pi, unique, T, tot + 1, tot, tot + 2, 1,
+	                          gth, gell, a, fd, scal);
 	return (int)hipGetLastError();
 }
 

@@ -240,6 +240,31 @@ int hemk_grad_passes(const hemk_prob *p, int fp32, const double *x,
 int hemk_hvp(const hemk_prob *p, const double *x, const double *a,
              const double *pi, const double *kappa, const double *scal,
              const double *v, double *kd, double *part, double *hv, void *s);
+/* The gradient's last pass (g_theta, g_ell, a, the Fisher diagonal and
+ * scal) from np partials each of sum n_c log D_c (part[0..np)), N
+ * (part[np..2np)) and sum u_t log x_t (part[2np..3np)); part[3np..3np+3)
+ * is scratch for their totals. */
+int hemk_grad_fin(const double *E, const double *pi, const uint32_t *unique,
+                  uint32_t T, double *part, uint32_t np, double *gth,
+                  double *gell, double *a, double *fd, double *scal, void *s);
+/* The adjoint's class and transcript passes over the single-problem tiles
+ * (the classes of a tile in LDS): kappa_p by class tile position, E and fd
+ * by transcript id, as hemk_grad_passes computes them, and per-tile
+ * partials for hemk_grad_fin (np = n_tiles).  cntq: class counts by tile
+ * position; x (or x32 with fp32), unique: by transcript id. */
+int hemk_ptile_grad(const hemk_tiles *t, const double *cntq, int fp32,
+                    const double *x, const float *x32,
+                    const uint32_t *unique, double *kappa_p, double *E,
+                    double *fd, double *part, void *s);
+/* HEMK_NORM_BLOCKS block partials of a . b (T values). */
+int hemk_dot_parts(const double *a, const double *b, uint32_t T,
+                   double *part, void *s);
+/* hv = H v over the tiles: kappa_p by class tile position, pv_part holds
+ * HEMK_NORM_BLOCKS partials of pi . v (hemk_dot_parts), scal[1] = N. */
+int hemk_ptile_hvp(const hemk_tiles *t, const double *x, const double *v,
+                   const double *kappa_p, const double *a, const double *pi,
+                   const double *pv_part, const double *scal, double *hv,
+                   void *s);
 
 /* ---- standard errors by batched CG (se_kernels.hip) ----------------------
  * Vectors [q][K] in tile position order; per-direction partials [rb][K]
@@ -272,6 +297,9 @@ int hemk_cg_out(const double *X, const uint32_t *jq, const double *pj,
                 uint32_t K, double *se, double *resid, void *s);
 int hemk_gather_f64(const double *in, const uint32_t *idx, uint32_t n,
                     double *out, void *s);
+/* out[idx[i]] = in[i] (idx a permutation of its range) */
+int hemk_scatter_f64(const double *in, const uint32_t *idx, uint32_t n,
+                     double *out, void *s);
 int hemk_precond(const double *fdq, const double *piq, const double *scal,
                  double delta, uint32_t n, double *mq, void *s);
 

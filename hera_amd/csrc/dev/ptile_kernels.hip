@@ -809,6 +809,239 @@ __global__ __launch_bounds__(256) void k_ptile_alpha(
 	}
 }
 
+// ---- derivatives of the log-likelihood over the tiles ---------------------
+// L = sum_c n_c log D_c + sum_t u_t log x_t, D_c = sum_{t in c} x_t (the
+// objective of grad_kernels.hip).  The tile's record and its classes sit in
+// LDS, so the class pass and the transcript pass read D_c, n_c / D_c and
+// n_c without the scattered global loads of the two-pass form.
+
+// The record of tile tc into LDS (no counts: the derivative kernels read
+// them by class id).
+__device__ __forceinline__ void pstage_rec(const hemk_tiles &tl,
+                                           const PTile &tc, const PLds &L)
+{
+	const uint32_t n16 = rec_bytes<false>(tc.nt, tc.nc, tc.nz) / 16;
+	const uint4 *src = (const uint4 *)tl.rec + tc.rec;
+	uint4 rv[PK_R];
+#pragma unroll
+	for (int q = 0; q < PK_R; ++q)
+		rv[q] = threadIdx.x + q * PBLOCK < n16 ? src[threadIdx.x + q * PBLOCK]
+		                                       : make_uint4(0, 0, 0, 0);
+#pragma unroll
+	for (int q = 0; q < PK_R; ++q)
+		if (threadIdx.x + q * PBLOCK < n16)
+			L.rec[threadIdx.x + q * PBLOCK] = rv[q];
+	for (uint32_t i = threadIdx.x + PK_R * PBLOCK; i < n16; i += PBLOCK)
+		L.rec[i] = src[i];
+}
+
+// Workgroup sum (PBLOCK threads) in a fixed order; thread 0 has it.
+__device__ __forceinline__ double pblock_total(double a, double *sh)
+{
+#pragma clang fp contract(off)
+	a = wave_sum(a);
+	if (threadIdx.x % 64 == 0)
+		sh[threadIdx.x / 64] = a;
+	__syncthreads();
+	double t = sh[0];
+	for (int w = 1; w < PBLOCK / 64; ++w)
+		t += sh[w];
+	__syncthreads();
+	return t;
+}
+
+// The adjoint's passes over one tile: D_c (members in class order, in TX),
+// rho_c = n_c / D_c, kappa_c = rho_c / D_c (0 when n_c or D_c is 0, kept
+// by class tile position for the HVP) and n_c log D_c; then per transcript
+// E_t = x_t sum rho_c + u_t and the class part of the Fisher diagonal
+// fd_t = -sum n_c q (1 - q), q = x_t / D_c; tile partials of
+// sum n_c log D_c, N = sum E_t and sum u_t log x_t.
+template <typename TX>
+__global__ __launch_bounds__(PBLOCK) void k_ptile_grad(
+    hemk_tiles tl, const double *__restrict__ cntq,
+    const TX *__restrict__ x, const uint32_t *__restrict__ unique,
+    double *__restrict__ kappa_p, double *__restrict__ E,
+    double *__restrict__ fd, double *__restrict__ part)
+{
+	extern __shared__ double smem[];
+	__shared__ double sh[PBLOCK / 64];
+	const uint32_t tid = threadIdx.x, tile = blockIdx.x;
+	const PTile tc = ptile(tl, tile);
+	PLds L = pcarve(smem, tl);
+	pfields<false>(L, tc);
+	pstage_rec(tl, tc, L);
+	__syncthreads();
+	double *xl = L.l0;     // x_t of the tile's transcripts
+	for (uint32_t i = tid; i < tc.nt; i += PBLOCK) {
+		const uint32_t id = L.tx[i];
+		xl[i] = (double)x[id];
+		L.u[i] = unique[id];
+	}
+	for (uint32_t c = tid; c < tc.nc; c += PBLOCK)
+		L.n[c] = cntq[tc.cls_base + c];
+	__syncthreads();
+	double Lc = 0.0;
+	for (uint32_t c = tid; c < tc.nc; c += PBLOCK) {
+		uint32_t k = L.coff[c];
+		const uint32_t k1 = L.coff[c + 1];
+		TX acc = 0;
+		for (; k + 4 <= k1; k += 4) {
+			const TX x0 = (TX)xl[L.kmem[k] & 0x3Fu];
+			const TX x1 = (TX)xl[L.kmem[k + 1] & 0x3Fu];
+			const TX x2 = (TX)xl[L.kmem[k + 2] & 0x3Fu];
+			const TX x3 = (TX)xl[L.kmem[k + 3] & 0x3Fu];
+			acc += x0;
+			acc += x1;
+			acc += x2;
+			acc += x3;
+		}
+		for (; k < k1; ++k)
+			acc += (TX)xl[L.kmem[k] & 0x3Fu];
+		const double D = (double)acc, n = L.n[c];
+		double r = 0.0, q = 0.0;
+		if (n != 0 && D != 0) {
+			r = n / D;
+			q = r / D;
+			Lc += n * log(D);
+		}
+		L.D[c] = D;
+		L.n[c] = r;    // rho_c from here on (each thread its own c)
+		kappa_p[tc.cls_base + c] = q;
+	}
+	__syncthreads();
+	// four lanes per transcript, each taking every fourth occurrence, then
+	// combined across the four (n_c q (1 - q) = rho_c x_t (1 - x_t / D_c))
+	double Nn = 0.0, Lt = 0.0;
+	const uint32_t sub = tid & 3u;
+	for (uint32_t t0 = 0; t0 < tc.nt; t0 += PBLOCK / 4) {
+		const uint32_t t = t0 + tid / 4;
+		const bool in = t < tc.nt;
+		const double xt = in ? xl[t] : 0.0;
+		double sr = 0.0, h = 0.0;
+		if (in) {
+			const uint32_t j1 = L.toff[t + 1];
+			uint32_t j = L.toff[t] + sub;
+			for (; j + 4 < j1; j += 8) {
+				const uint32_t c0 = L.kocc[j] >> 6;
+				const uint32_t c1 = L.kocc[j + 4] >> 6;
+				const double r0 = L.n[c0], d0 = L.D[c0];
+				const double r1 = L.n[c1], d1 = L.D[c1];
+				sr += r0;
+				sr += r1;
+				if (r0 != 0)
+					h += r0 * xt * (1 - xt / d0);
+				if (r1 != 0)
+					h += r1 * xt * (1 - xt / d1);
+			}
+			for (; j < j1; j += 4) {
+				const uint32_t c = L.kocc[j] >> 6;
+				const double r = L.n[c];
+				sr += r;
+				if (r != 0)
+					h += r * xt * (1 - xt / L.D[c]);
+			}
+		}
+		sr += __shfl_xor(sr, 1, 64);
+		h += __shfl_xor(h, 1, 64);
+		sr += __shfl_xor(sr, 2, 64);
+		h += __shfl_xor(h, 2, 64);
+		if (in && sub == 0) {
+			const uint32_t u = L.u[t], id = L.tx[t];
+			const double e = xt * sr + (double)u;
+			E[id] = e;
+			fd[id] = -h;
+			Nn += e;
+			if (u)
+				Lt += (double)u * log(xt);
+		}
+	}
+	Lc = pblock_total(Lc, sh);
+	Nn = pblock_total(Nn, sh);
+	Lt = pblock_total(Lt, sh);
+	if (tid == 0) {
+		part[tile] = Lc;
+		part[tl.n_tiles + tile] = Nn;
+		part[2 * (size_t)tl.n_tiles + tile] = Lt;
+	}
+}
+
+// H v over one tile: kd_c = kappa_c sum_{s in c} x_s v_s (members in class
+// order), hv_t = v_t a_t - x_t sum_{c ni t} kd_c + N (pi . v) pi_t.
+__global__ __launch_bounds__(PBLOCK) void k_ptile_hvp(
+    hemk_tiles tl, const double *__restrict__ x, const double *__restrict__ v,
+    const double *__restrict__ kappa_p, const double *__restrict__ a,
+    const double *__restrict__ pi, const double *__restrict__ pv_part,
+    const double *__restrict__ scal, double *__restrict__ hv)
+{
+	extern __shared__ double smem[];
+	__shared__ double sh[PBLOCK / 64];
+	__shared__ double pvs;
+	const uint32_t tid = threadIdx.x;
+	const PTile tc = ptile(tl, blockIdx.x);
+	PLds L = pcarve(smem, tl);
+	pfields<false>(L, tc);
+	pstage_rec(tl, tc, L);
+	double pv = 0.0;
+	for (uint32_t i = tid; i < HEMK_NORM_BLOCKS; i += PBLOCK)
+		pv += pv_part[i];
+	pv = pblock_total(pv, sh);   // also orders the record's stores
+	if (tid == 0)
+		pvs = pv;
+	double *xv = L.l0;
+	for (uint32_t i = tid; i < tc.nt; i += PBLOCK) {
+		const uint32_t id = L.tx[i];
+		xv[i] = x[id] * v[id];
+	}
+	for (uint32_t c = tid; c < tc.nc; c += PBLOCK)
+		L.n[c] = kappa_p[tc.cls_base + c];
+	__syncthreads();
+	for (uint32_t c = tid; c < tc.nc; c += PBLOCK) {
+		uint32_t k = L.coff[c];
+		const uint32_t k1 = L.coff[c + 1];
+		double acc = 0.0;
+		for (; k + 4 <= k1; k += 4) {
+			const double x0 = xv[L.kmem[k] & 0x3Fu];
+			const double x1 = xv[L.kmem[k + 1] & 0x3Fu];
+			const double x2 = xv[L.kmem[k + 2] & 0x3Fu];
+			const double x3 = xv[L.kmem[k + 3] & 0x3Fu];
+			acc += x0;
+			acc += x1;
+			acc += x2;
+			acc += x3;
+		}
+		for (; k < k1; ++k)
+			acc += xv[L.kmem[k] & 0x3Fu];
+		L.D[c] = L.n[c] * acc;
+	}
+	__syncthreads();
+	// four lanes per transcript, combined across the four
+	const double Npv = scal[1] * pvs;
+	const uint32_t sub = tid & 3u;
+	for (uint32_t t0 = 0; t0 < tc.nt; t0 += PBLOCK / 4) {
+		const uint32_t t = t0 + tid / 4;
+		const bool in = t < tc.nt;
+		double sk = 0.0;
+		if (in) {
+			const uint32_t j1 = L.toff[t + 1];
+			uint32_t j = L.toff[t] + sub;
+			for (; j + 4 < j1; j += 8) {
+				const double k0 = L.D[L.kocc[j] >> 6];
+				const double k1 = L.D[L.kocc[j + 4] >> 6];
+				sk += k0;
+				sk += k1;
+			}
+			for (; j < j1; j += 4)
+				sk += L.D[L.kocc[j] >> 6];
+		}
+		sk += __shfl_xor(sk, 1, 64);
+		sk += __shfl_xor(sk, 2, 64);
+		if (in && sub == 0) {
+			const uint32_t id = L.tx[t];
+			hv[id] = v[id] * a[id] - x[id] * sk + Npv * pi[id];
+		}
+	}
+}
+
 // out[t] = A[txq[t]]: the state in transcript order.
 __global__ __launch_bounds__(256) void k_ptile_rows(
     uint32_t T, const uint32_t *__restrict__ txq, const double *__restrict__ A,
@@ -867,6 +1100,36 @@ int hemk_ptile_map12_fb(const hemk_tiles *t, const double *A, double *B,
 	else
 		k_ptile_map12<false, true, false><<<t->n_tiles, PBLOCK, lds, st>>>(
 		    *t, A, B, C, uq, cntq, xr, xv, part, step, ctl);
+	return (int)hipGetLastError();
+}
+
+int hemk_ptile_grad(const hemk_tiles *t, const double *cntq, int fp32,
+                    const double *x, const float *x32,
+                    const uint32_t *unique, double *kappa_p, double *E,
+                    double *fd, double *part, void *s)
+{
+	if (!t->n_tiles || t->max_tx > 64 || t->weighted)
+		return (int)hipErrorInvalidValue;
+	const size_t lds = hemk_ptile_lds(t);
+	hipStream_t st = (hipStream_t)s;
+	if (fp32)
+		k_ptile_grad<float><<<t->n_tiles, PBLOCK, lds, st>>>(
+		    *t, cntq, x32, unique, kappa_p, E, fd, part);
+	else
+		k_ptile_grad<double><<<t->n_tiles, PBLOCK, lds, st>>>(
+		    *t, cntq, x, unique, kappa_p, E, fd, part);
+	return (int)hipGetLastError();
+}
+
+int hemk_ptile_hvp(const hemk_tiles *t, const double *x, const double *v,
+                   const double *kappa_p, const double *a, const double *pi,
+                   const double *pv_part, const double *scal, double *hv,
+                   void *s)
+{
+	if (!t->n_tiles || t->max_tx > 64 || t->weighted)
+		return (int)hipErrorInvalidValue;
+	k_ptile_hvp<<<t->n_tiles, PBLOCK, hemk_ptile_lds(t), (hipStream_t)s>>>(
+	    *t, x, v, kappa_p, a, pi, pv_part, scal, hv);
 	return (int)hipGetLastError();
 }
 

@@ -344,6 +344,16 @@ __global__ __launch_bounds__(SB) void k_gather_f64(const double *__restrict__ in
 		out[i] = in[idx[i]];
 }
 
+__global__ __launch_bounds__(SB) void k_scatter_f64(const double *__restrict__ in,
+                                                    const uint32_t *__restrict__ idx,
+                                                    uint32_t n,
+                                                    double *__restrict__ out)
+{
+	for (uint32_t i = blockIdx.x * SB + threadIdx.x; i < n;
+	     i += gridDim.x * SB)
+		out[idx[i]] = in[i];
+}
+
 // m_t = 1 / (fd_t + delta), floored so a non-positive diagonal stays usable
 __global__ __launch_bounds__(SB) void k_precond(const double *__restrict__ fdq,
                                                 const double *__restrict__ piq,
@@ -485,6 +495,18 @@ int hemk_gather_f64(const double *in, const uint32_t *idx, uint32_t n,
 	if (g > 4096)
 		g = 4096;
 	k_gather_f64<<<g, SB, 0, (hipStream_t)s>>>(in, idx, n, out);
+	return (int)hipGetLastError();
+}
+
+int hemk_scatter_f64(const double *in, const uint32_t *idx, uint32_t n,
+                     double *out, void *s)
+{
+	if (!n)
+		return 0;
+	uint32_t g = sdiv(n, SB);
+	if (g > 4096)
+		g = 4096;
+	k_scatter_f64<<<g, SB, 0, (hipStream_t)s>>>(in, idx, n, out);
 	return (int)hipGetLastError();
 }
 

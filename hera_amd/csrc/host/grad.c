@@ -28,6 +28,7 @@ void hem_grad_free(hem_problem *p)
 	void *ptrs[] = {g->theta, g->ell, g->pi,  g->w,   g->x,    g->E,
 	                g->fd,    g->gth, g->gell, g->a,  g->v,    g->hv,
 	                g->x32,   g->u,   g->rho, g->D,   g->kappa, g->kd,
+	                g->kappa_p,
 	                g->part,  g->scal, g->piq, g->xq, g->aq,   g->mq,
 	                g->fdq,   g->kq,  g->jq,  g->pj,  g->X,    g->R,
 	                g->P,     g->Q,   g->prz, g->prr, g->ppv,  g->ppfp,
@@ -73,7 +74,14 @@ static int ensure_grad(hem_problem *p)
 	GALLOC(g->D, C * 8);
 	GALLOC(g->kappa, C * 8);
 	GALLOC(g->kd, C * 8);
-	GALLOC(g->part, (size_t)3 * HEMK_NORM_BLOCKS * 8);
+	GALLOC(g->kappa_p, (size_t)(p->dtp.n_cls ? p->dtp.n_cls : 1) * 8);
+	{
+		/* block partials, or per-tile partials of the tile passes */
+		size_t np = HEMK_NORM_BLOCKS;
+		if (p->dtp.n_tiles > np)
+			np = p->dtp.n_tiles;
+		GALLOC(g->part, (3 * np + 3) * 8);
+	}
 	GALLOC(g->scal, 2 * 8);
 	g->ready = 1;
 	return HEM_OK;
@@ -125,6 +133,15 @@ static int ensure_se(hem_problem *p, uint32_t K)
 	return HEM_OK;
 }
 
+/* The adjoint and the Hessian-vector product run over the single-problem
+ * tiles (classes in LDS) when every component fits one and the tile fits
+ * the LDS bound, else over the global CSR/CSC passes. */
+static int grad_tiles(const hem_problem *p)
+{
+	return p->dtp.n_tiles && !p->weighted &&
+	       hemk_ptile_lds(&p->dtp) <= HEM_PTILE_LDS_MAX;
+}
+
 /* Uploads the point and runs the softmax and the two adjoint passes. */
 static int grad_setup(hem_problem *p, const uint32_t *unique,
                       const double *theta, const double *ell, int fp32)
@@ -138,6 +155,20 @@ static int grad_setup(hem_problem *p, const uint32_t *unique,
 	HEM_TRY_DEV(hemk_grad_softmax(g->theta, g->ell, p->n_tx, g->pi, g->w,
 	                              g->x, g->x32, g->part, s),
 	            "softmax");
+	g->kappa_tiled = 0;
+	if (grad_tiles(p)) {
+		HEM_TRY_DEV(hem_cntq_update(p), "class count gather");
+		HEM_TRY_DEV(hemk_ptile_grad(&p->dtp, p->d_cntq, fp32, g->x,
+		                            g->x32, g->u, g->kappa_p, g->E, g->fd,
+		                            g->part, s),
+		            "adjoint tile passes");
+		g->kappa_tiled = 1;
+		HEM_TRY_DEV(hemk_grad_fin(g->E, g->pi, g->u, p->n_tx, g->part,
+		                          p->dtp.n_tiles, g->gth, g->gell, g->a,
+		                          g->fd, g->scal, s),
+		            "adjoint end");
+		return HEM_OK;
+	}
 	HEM_TRY_DEV(hemk_grad_passes(&p->dp, fp32, g->x, g->x32, g->u, g->pi,
 	                             g->rho, g->D, g->kappa, g->E, g->fd, g->gth,
 	                             g->gell, g->a, g->part, g->scal, s),
@@ -234,9 +265,16 @@ int hem_hvp(hem_problem *p, const uint32_t *unique, const double *theta,
 		hem_grad_ws *g = &p->gw;
 		void *s = p->stream;
 		int e = hemd_h2d(g->v, v, (size_t)p->n_tx * 8, s);
-		if (!e)
+		if (!e && grad_tiles(p)) {
+			e = hemk_dot_parts(g->pi, g->v, p->n_tx, g->part, s);
+			if (!e)
+				e = hemk_ptile_hvp(&p->dtp, g->x, g->v, g->kappa_p,
+				                   g->a, g->pi, g->part, g->scal,
+				                   g->hv, s);
+		} else if (!e) {
 			e = hemk_hvp(&p->dp, g->x, g->a, g->pi, g->kappa, g->scal,
 			             g->v, g->kd, g->part, g->hv, s);
+		}
 		if (!e)
 			e = hemd_d2h(hv, g->hv, (size_t)p->n_tx * 8, s);
 		if (!e)
@@ -407,6 +445,10 @@ int hem_stderr(hem_problem *p, const uint32_t *unique, const double *theta,
 			e = hemk_gather_f64(g->a, p->dt.tx, T, g->aq, s);
 		if (!e)
 			e = hemk_gather_f64(g->fd, p->dt.tx, T, g->fdq, s);
+		/* kappa by class id from the tile passes' tile order */
+		if (!e && g->kappa_tiled)
+			e = hemk_scatter_f64(g->kappa_p, p->d_cls_orig_p,
+			                     p->dtp.n_cls, g->kappa, s);
 		if (!e)
 			e = hemk_gather_f64(g->kappa, p->d_cls_orig, p->dt.n_cls,
 			                    g->kq, s);

@@ -58,8 +58,12 @@ typedef struct {
 	double *theta, *ell, *pi, *w, *x, *E, *fd, *gth, *gell, *a, *v, *hv;
 	float *x32;
 	uint32_t *u;
-	double *rho, *D, *kappa, *kd;  /* n_cls                             */
-	double *part, *scal;           /* 3 * HEMK_NORM_BLOCKS, 2           */
+	double *rho, *D, *kappa, *kd;  /* n_cls, by class id                */
+	double *kappa_p;               /* kappa by single-problem tile
+	                                  position (the tile passes)        */
+	int kappa_tiled;               /* the last adjoint ran on the tiles:
+	                                  kappa_p is current, kappa is not  */
+	double *part, *scal;           /* 3 * max(HEMK_NORM_BLOCKS, tiles), 2 */
 	/* standard errors: tile-order constants and K-wide CG state */
 	uint32_t K;                    /* capacity in directions            */
 	double *piq, *xq, *aq, *mq, *fdq, *kq;
@@ -95,6 +99,7 @@ struct hem_problem {
 	 * in tile order (state vectors A, B, C are then in tile order too) */
 	uint32_t *d_uq;
 	double *d_cntq;
+	int cntq_ok;               /* d_cntq holds the current class counts */
 	double *d_fbpart;  /* fallback: [tile] {ell difference, magnitudes} */
 	double *d_tpart;   /* tree norms: [tile][4] partials of r and v      */
 	double *d_part1, *d_part2, *d_out;
@@ -254,6 +259,9 @@ const char *hem_device_arch(void);
 /* Replicates per batched workgroup for R replicates: the configured chunk,
  * lowered to the largest whose rows fit the LDS budget (0: none fits). */
 int hem_batch_chunk(const hem_problem *p, uint32_t R);
+/* Enqueues the gather of the class counts into the single-problem tile
+ * order (d_cntq) unless it already holds the current counts. */
+int hem_cntq_update(hem_problem *p);
 
 #define HEM_TRY_DEV(call, what)                                   \
 	do {                                                      \

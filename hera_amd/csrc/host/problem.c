@@ -477,6 +477,18 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 	return HEM_OK;
 }
 
+int hem_cntq_update(hem_problem *p)
+{
+	int e;
+	if (p->cntq_ok)
+		return 0;
+	e = hemk_gather_f64(p->d_cnt64, p->d_cls_orig_p, p->dtp.n_cls,
+	                    p->d_cntq, p->stream);
+	if (!e)
+		p->cntq_ok = 1;
+	return e;
+}
+
 int hem_problem_create(const hem_classes *c, hem_problem **out)
 {
 	return create_impl(c, NULL, NULL, 1, NULL, out);
@@ -692,6 +704,7 @@ int hem_problem_set_counts(hem_problem *p, const double *cnt)
 		nd[2 * (size_t)i + 1] = 0;
 	}
 	pthread_mutex_lock(&p->mu);
+	p->cntq_ok = 0;
 	rc = hemd_h2d(p->d_cnt64, cnt, (size_t)C * 8, p->stream);
 	if (!rc)
 		rc = hemd_h2d(p->d_cnt32, c32, (size_t)C * 4, p->stream);

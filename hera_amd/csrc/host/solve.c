@@ -445,8 +445,7 @@ static int solve_setup(hem_problem *p, const uint32_t *unique,
 		rc = hemk_gather_rows_u32(p->d_unique, p->dtp.tx, p->n_tx, 1,
 		                          p->d_uq, p->stream);
 	if (!rc && use_tiles(p, o))
-		rc = hemk_gather_f64(p->d_cnt64, p->d_cls_orig_p, p->dtp.n_cls,
-		                     p->d_cntq, p->stream);
+		rc = hem_cntq_update(p);
 	if (!rc)
 		rc = hemk_init_state(p->n_tx, p->d_A, p->d_A32,
 		                     (double)1 / p->n_tx, NULL, p->stream);

@@ -154,6 +154,36 @@ def test_hvp_matches_cpu_and_gradient_differences(sub):
     assert np.max(np.abs(h1)) <= 1e-8 * np.max(np.abs(hv))
 
 
+def test_grad_and_hvp_on_untiled_table():
+    """A component over the tile limits (a chain of 100 transcripts) sends
+    the adjoint and the HVP to the global CSR/CSC passes; both still equal
+    the CPU reference."""
+    rng = np.random.default_rng(11)
+    T = 300
+    members = [[i, i + 1] for i in range(99)]
+    members += [sorted(rng.choice(np.arange(100, T), rng.integers(1, 5),
+                                  replace=False).tolist())
+                for _ in range(600)]
+    counts = rng.integers(1, 50, len(members)).astype(float).tolist()
+    cls = H.Classes.from_lists(T, members, counts,
+                               rng.integers(200, 3000, T).tolist())
+    u = rng.integers(0, 20, T).astype(np.uint32)
+    theta, ell = point(cls, 12)
+    v = rng.normal(size=T)
+    p = H.Problem(cls)
+    try:
+        assert p.tiles()["n_tiles"] == 0
+        g = p.grad(u, theta, ell)
+        hv = p.hvp(u, theta, ell, v)
+    finally:
+        p.close()
+    L, gt, ge = O.grad(cls, u, theta, ell)
+    assert abs(g["loglik"] - L) <= 1e-12 * abs(L)
+    assert np.max(np.abs(g["g_theta"] - gt)) <= 1e-10 * np.max(np.abs(gt))
+    ref = O.hvp(cls, u, theta, ell, v)
+    assert np.linalg.norm(hv - ref) <= 1e-11 * np.linalg.norm(ref)
+
+
 def dense_fisher(cls, u, theta, ell):
     T = cls.n_tx
     Hm = np.empty((T, T))

@@ -430,12 +430,27 @@ def bench_grad(args, H, np, p, cls, u, sk):
     ell = np.log(cls.tx_len.astype(float))
     reps = 10
     res = {}
+    b_map = map_bytes(cls)
     for name, prec in (("fp64", H.PREC_FP64), ("fp32_forward", H.PREC_FP32)):
         p.grad(u, theta, ell, precision=prec)
-        ms = []
+        ms, ma = [], []
         for _ in range(reps):
-            ms.append(p.grad(u, theta, ell, precision=prec)["ms"])
+            r = p.grad(u, theta, ell, precision=prec)
+            ms.append(r["ms"])
+            ma.append(r["ms_adjoint"])
         res["grad_" + name + "_ms"] = float(np.median(ms))
+        res["grad_" + name + "_adjoint_ms"] = float(np.median(ma))
+    # the adjoint's class and transcript passes read what one EM map reads
+    # (a class pass and a transcript pass over the same table)
+    ach = b_map / (res["grad_fp64_adjoint_ms"] * 1e-3) / 1e9
+    res["roofline"] = {
+        "bound": "hbm", "kernel": "k_ptile_grad (adjoint passes)",
+        "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": ach / HBM_PEAK_GBS,
+        "algorithmic_bytes": b_map,
+        "bytes_rule": "8Z + 16C + 24T (one class pass and one transcript "
+                      "pass, u32 indices, FP32 values)",
+        "ms": res["grad_fp64_adjoint_ms"]}
     gg = p.grad(u, theta, ell)
     E = -gg["g_log_efflen"]
     res["grad_rel_stationarity"] = float(
@@ -462,6 +477,9 @@ def bench_grad(args, H, np, p, cls, u, sk):
     O.grad(cls, u, np.where(np.isfinite(theta), theta, -745.0), ell)
     res["cpu_grad_ms"] = (O.now() - t0) * 1e3
     res["cpu_grad_note"] = "oracle FP64 gradient, 1 thread"
+    log("adjoint passes %.4f ms: %.0f GB/s by algorithmic bytes (%.1f%% "
+        "of peak)" % (res["grad_fp64_adjoint_ms"], ach,
+                      100 * ach / HBM_PEAK_GBS))
     log("grad fp64 %.3f ms, fp32-forward %.3f ms, HVP %.2f ms (wall), "
         "CPU grad %.1f ms; SE top %d, %d CG steps: %.1f ms (resid max %.1e)"
         % (res["grad_fp64_ms"], res["grad_fp32_forward_ms"],
@@ -562,7 +580,8 @@ def bench_ragged(args, H, np):
     each with R bootstrap replicates, all in one multi-sample handle (every
     (sample, replicate) EM instance in the same launches, per-instance stop
     tests); against the same samples bootstrapped one handle at a time, and
-    a gradient pass per sample at its EM solution."""
+    every sample's gradient in one call at its first replicate's
+    estimate."""
     S, R = args.ragged_samples, args.ragged_reps
     rng = np.random.default_rng(12345)
     ntx = np.exp(rng.uniform(np.log(5000), np.log(200000), S)).astype(int)
@@ -604,7 +623,39 @@ def bench_ragged(args, H, np):
         log("ragged %-10s %d samples x %d reps: %.3fs (%.0f instances/s) "
             "iters %d..%d" % (name, S, R, dt, S * R / dt,
                               b["iters"].min(), b["iters"].max()))
+    # a gradient pass for every sample in one call on the same handle, at
+    # each sample's first exact replicate (theta = log est, softmax within
+    # the sample), against the samples' own handles one at a time
+    est0 = out_multi["fp64-exact"]["est"][0]
+    with np.errstate(divide="ignore"):
+        th_all = np.log(est0)
+    th_all = np.where(np.isfinite(th_all), th_all, -745.0)
+    ell_all = np.log(np.concatenate([c.tx_len for c in cls]).astype(float))
+    mp.grad_samples(u_all, th_all, ell_all)
+    gm = [mp.grad_samples(u_all, th_all, ell_all) for _ in range(5)]
     mp.close()
+    g_ms = float(np.median([g["ms"] for g in gm]))
+    g_ma = float(np.median([g["ms_adjoint"] for g in gm]))
+    tg, worst = 0.0, 0.0
+    for q in range(S):
+        a, e = int(mp.tx0[q]), int(mp.tx0[q + 1])
+        p = H.Problem(cls[q])
+        p.grad(us[q], th_all[a:e], ell_all[a:e])
+        own = p.grad(us[q], th_all[a:e], ell_all[a:e])
+        p.close()
+        tg += own["ms"]
+        worst = max(worst, float(np.max(np.abs(gm[0]["g_theta"][a:e]
+                                               - own["g_theta"]))
+                                 / np.max(np.abs(own["g_theta"]))))
+    res["gradient"] = {
+        "one_call_ms": g_ms, "adjoint_ms": g_ma,
+        "own_handles_ms_total": tg, "speedup": tg / g_ms,
+        "max_rel_diff_vs_own_handles": worst,
+        "note": "hem_grad_samples on the multi-sample handle (device ms, "
+                "uploads included) against hem_grad on each sample's own "
+                "handle, summed"}
+    log("ragged gradient: one call %.2f ms (adjoint %.2f ms) vs %.2f ms on "
+        "%d own handles; max rel diff %.1e" % (g_ms, g_ma, tg, S, worst))
     # the same samples one handle at a time (a sample of them, scaled)
     nseq = min(args.ragged_seq_sample, S)
     pick = np.linspace(0, S - 1, nseq).astype(int)
@@ -630,21 +681,6 @@ def bench_ragged(args, H, np):
             "transcripts) -> %.2fs for all; x%.2f; bitwise %s" % (
                 name, tsum, nseq, 100 * frac, est_all,
                 res[name]["speedup_vs_one_at_a_time"], same))
-    # a gradient pass per sample at its EM solution
-    tg = 0.0
-    for q in range(S):
-        p = H.Problem(cls[q])
-        g = p.solve(us[q], norm=H.NORM_TREE, **sk)
-        with np.errstate(divide="ignore"):
-            th = np.log(g["lam"] * cls[q].tx_len)
-        th -= th[np.isfinite(th)].max()
-        ell = np.log(cls[q].tx_len.astype(float))
-        p.grad(us[q], th, ell)
-        tg += p.grad(us[q], th, ell)["ms"]
-        p.close()
-    res["grad_pass_ms_total"] = tg
-    res["grad_pass_note"] = "hem_grad at each sample's EM solution, device ms"
-    log("ragged gradient passes: %.2f ms for %d samples" % (tg, S))
     return res
 
 

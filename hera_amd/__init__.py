@@ -519,7 +519,7 @@ class Problem:
     def grad(self, unique, theta, log_efflen, precision=PREC_FP64):
         """Log-likelihood and its gradient in the softmax logits theta and
         the log effective lengths (include/hera_em.h, hem_grad).  Returns
-        dict(loglik, n_frag, g_theta, g_log_efflen, ms)."""
+        dict(loglik, n_frag, g_theta, g_log_efflen, ms, ms_adjoint)."""
         u = as_u32(unique)
         th = as_f64(theta)
         el = as_f64(log_efflen)
@@ -529,7 +529,26 @@ class Problem:
         check(_lib.load().hem_grad(self._h, u32p(u), f64p(th), f64p(el),
                                    precision, f64p(gt), f64p(ge), st))
         return dict(loglik=st.loglik, n_frag=st.n_frag, g_theta=gt,
-                    g_log_efflen=ge, ms=st.ms)
+                    g_log_efflen=ge, ms=st.ms, ms_adjoint=st.ms_adjoint)
+
+    def grad_samples(self, unique, theta, log_efflen, precision=PREC_FP64):
+        """Every sample's gradient in one pass (hem_grad_samples): the
+        softmax within each sample.  Returns dict(loglik[ns], n_frag[ns],
+        g_theta, g_log_efflen, ms, ms_adjoint)."""
+        u = as_u32(unique)
+        th = as_f64(theta)
+        el = as_f64(log_efflen)
+        assert u.shape == th.shape == el.shape == (self.n_tx,)
+        gt = np.empty(self.n_tx)
+        ge = np.empty(self.n_tx)
+        L = np.empty(self.n_samples)
+        N = np.empty(self.n_samples)
+        st = _lib.HemGradStats()
+        check(_lib.load().hem_grad_samples(self._h, u32p(u), f64p(th),
+                                           f64p(el), precision, f64p(gt),
+                                           f64p(ge), f64p(L), f64p(N), st))
+        return dict(loglik=L, n_frag=N, g_theta=gt, g_log_efflen=ge,
+                    ms=st.ms, ms_adjoint=st.ms_adjoint)
 
     def hvp(self, unique, theta, log_efflen, v):
         """Hessian-vector product in theta (hem_hvp)."""

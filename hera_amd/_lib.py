@@ -101,7 +101,7 @@ class HemHeraEmStats(C.Structure):
 
 class HemGradStats(C.Structure):
     _fields_ = [("loglik", C.c_double), ("n_frag", C.c_double),
-                ("ms", C.c_double)]
+                ("ms", C.c_double), ("ms_adjoint", C.c_double)]
 
 
 class HemSeStats(C.Structure):
@@ -227,6 +227,9 @@ SIGNATURES = {
                                     C.POINTER(HemHeraEmStats)]),
     "hem_grad": (C.c_int, [C.c_void_p, _u32p, _f64p, _f64p, C.c_int, _f64p,
                            _f64p, C.POINTER(HemGradStats)]),
+    "hem_grad_samples": (C.c_int, [C.c_void_p, _u32p, _f64p, _f64p,
+                                   C.c_int, _f64p, _f64p, _f64p, _f64p,
+                                   C.POINTER(HemGradStats)]),
     "hem_hvp": (C.c_int, [C.c_void_p, _u32p, _f64p, _f64p, _f64p, _f64p]),
     "hem_stderr": (C.c_int, [C.c_void_p, _u32p, _f64p, _f64p, C.c_uint32,
                              C.c_uint32, C.c_double, _u32p, _f64p, _f64p,

@@ -259,6 +259,72 @@ __global__ __launch_bounds__(GB) void k_gfold(const double *__restrict__ part,
 		out[blockIdx.x] = v;
 }
 
+// ---- several samples side by side: a softmax and N per sample ----------
+// info[4q..4q+3] = {first transcript, transcripts, first tile, tiles}.
+
+// pi = softmax(theta) within sample q (block q), w = exp(-ell), x = pi w.
+__global__ __launch_bounds__(GB) void k_gsoftmax_seg(
+    const double *__restrict__ th, const double *__restrict__ ell,
+    const uint32_t *__restrict__ info, double *__restrict__ pi,
+    double *__restrict__ w, double *__restrict__ x, float *__restrict__ x32)
+{
+	const uint32_t q = blockIdx.x;
+	const uint32_t a = info[4 * q], b = a + info[4 * q + 1];
+	double m = -INFINITY;
+	for (uint32_t t = a + threadIdx.x; t < b; t += GB)
+		m = fmax(m, th[t]);
+	m = bmax(m);
+	double sum = 0;
+	for (uint32_t t = a + threadIdx.x; t < b; t += GB) {
+		const double v = exp(th[t] - m);
+		pi[t] = v;
+		w[t] = exp(-ell[t]);
+		sum += v;
+	}
+	const double S = bsum(sum);
+	for (uint32_t t = a + threadIdx.x; t < b; t += GB) {
+		const double pq = pi[t] / S;
+		pi[t] = pq;
+		const double v = pq * w[t];
+		x[t] = v;
+		x32[t] = (float)v;
+	}
+}
+
+// tot[3q + k] = the sum of sample q's tile partials of kind k (sum n_c log
+// D_c, N, sum u_t log x_t), block q.
+__global__ __launch_bounds__(GB) void k_gfold_seg(
+    const double *__restrict__ part, uint32_t np,
+    const uint32_t *__restrict__ info, double *__restrict__ tot)
+{
+	const uint32_t q = blockIdx.x;
+	const uint32_t a = info[4 * q + 2], n = info[4 * q + 3];
+	for (int k = 0; k < 3; ++k) {
+		const double v = foldn(part + (size_t)k * np + a, n);
+		if (threadIdx.x == 0)
+			tot[3 * q + k] = v;
+	}
+}
+
+// k_gfin with the N of each transcript's sample.
+__global__ __launch_bounds__(GB) void k_gfin_seg(
+    const double *__restrict__ E, const double *__restrict__ pi,
+    const uint32_t *__restrict__ unique, uint32_t T,
+    const uint32_t *__restrict__ txsamp, const double *__restrict__ tot,
+    double *__restrict__ gth, double *__restrict__ gell,
+    double *__restrict__ a, double *__restrict__ fd)
+{
+	for (uint32_t t = blockIdx.x * GB + threadIdx.x; t < T; t += NB * GB) {
+		const double N = tot[3 * (txsamp ? txsamp[t] : 0u) + 1];
+		const double e = E[t], p = pi[t];
+		const double g = e - N * p;
+		gth[t] = g;
+		gell[t] = -e;
+		a[t] = g - (double)unique[t];
+		fd[t] += N * p * (1 - p);
+	}
+}
+
 // Block partials of sum_t a_t b_t.
 __global__ __launch_bounds__(GB) void k_gdot(const double *__restrict__ a,
                                              const double *__restrict__ b,
@@ -388,6 +454,29 @@ int hemk_hvp(const hemk_prob *p, const double *x, const double *a,
 	                                        kappa, p->n_cls, kd);
 	k_htx<8><<<gt ? gt : 1, GB, 0, st>>>(p->tx_off, p->tx_ref, kd, x, a, pi,
 	                                     v, part, scal, p->n_tx, hv);
+	return (int)hipGetLastError();
+}
+
+int hemk_grad_softmax_seg(const double *theta, const double *ell,
+                          const uint32_t *info, uint32_t ns, double *pi,
+                          double *w, double *x, float *x32, void *s)
+{
+	k_gsoftmax_seg<<<ns, GB, 0, (hipStream_t)s>>>(theta, ell, info, pi, w,
+	                                              x, x32);
+	return (int)hipGetLastError();
+}
+
+int hemk_grad_fin_seg(const double *E, const double *pi,
+                      const uint32_t *unique, uint32_t T,
+                      const uint32_t *txsamp, const uint32_t *info,
+                      uint32_t ns, const double *part, uint32_t np,
+                      double *tot, double *gth, double *gell, double *a,
+                      double *fd, void *s)
+{
+	hipStream_t st = (hipStream_t)s;
+	k_gfold_seg<<<ns, GB, 0, st>>>(part, np, info, tot);
+	k_gfin_seg<<<NB, GB, 0, st>>>(E, pi, unique, T, txsamp, tot, gth, gell,
+	                              a, fd);
 	return (int)hipGetLastError();
 }
 

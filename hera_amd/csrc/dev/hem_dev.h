@@ -256,6 +256,21 @@ int hemk_ptile_grad(const hemk_tiles *t, const double *cntq, int fp32,
                     const double *x, const float *x32,
                     const uint32_t *unique, double *kappa_p, double *E,
                     double *fd, double *part, void *s);
+/* Several samples side by side (info: per sample {first transcript,
+ * transcripts, first tile, tiles} of the single-problem tiling): pi, w, x
+ * with a softmax within each sample; then, from the tile passes' partials,
+ * the per-sample totals tot[3q..3q+2] = {sum n_c log D_c, N_q,
+ * sum u_t log x_t} and the gradient with each transcript's sample's N
+ * (txsamp: transcript -> sample, NULL for one sample). */
+int hemk_grad_softmax_seg(const double *theta, const double *ell,
+                          const uint32_t *info, uint32_t ns, double *pi,
+                          double *w, double *x, float *x32, void *s);
+int hemk_grad_fin_seg(const double *E, const double *pi,
+                      const uint32_t *unique, uint32_t T,
+                      const uint32_t *txsamp, const uint32_t *info,
+                      uint32_t ns, const double *part, uint32_t np,
+                      double *tot, double *gth, double *gell, double *a,
+                      double *fd, void *s);
 /* HEMK_NORM_BLOCKS block partials of a . b (T values). */
 int hemk_dot_parts(const double *a, const double *b, uint32_t T,
                    double *part, void *s);

@@ -28,7 +28,7 @@ void hem_grad_free(hem_problem *p)
 	void *ptrs[] = {g->theta, g->ell, g->pi,  g->w,   g->x,    g->E,
 	                g->fd,    g->gth, g->gell, g->a,  g->v,    g->hv,
 	                g->x32,   g->u,   g->rho, g->D,   g->kappa, g->kd,
-	                g->kappa_p,
+	                g->kappa_p, g->tot,
 	                g->part,  g->scal, g->piq, g->xq, g->aq,   g->mq,
 	                g->fdq,   g->kq,  g->jq,  g->pj,  g->X,    g->R,
 	                g->P,     g->Q,   g->prz, g->prr, g->ppv,  g->ppfp,
@@ -83,6 +83,7 @@ static int ensure_grad(hem_problem *p)
 		GALLOC(g->part, (3 * np + 3) * 8);
 	}
 	GALLOC(g->scal, 2 * 8);
+	GALLOC(g->tot, (size_t)3 * p->ns * 8);
 	g->ready = 1;
 	return HEM_OK;
 }
@@ -158,10 +159,12 @@ static int grad_setup(hem_problem *p, const uint32_t *unique,
 	g->kappa_tiled = 0;
 	if (grad_tiles(p)) {
 		HEM_TRY_DEV(hem_cntq_update(p), "class count gather");
+		HEM_TRY_DEV(hemd_event_record(p->ev2, s), "event");
 		HEM_TRY_DEV(hemk_ptile_grad(&p->dtp, p->d_cntq, fp32, g->x,
 		                            g->x32, g->u, g->kappa_p, g->E, g->fd,
 		                            g->part, s),
 		            "adjoint tile passes");
+		HEM_TRY_DEV(hemd_event_record(p->ev3, s), "event");
 		g->kappa_tiled = 1;
 		HEM_TRY_DEV(hemk_grad_fin(g->E, g->pi, g->u, p->n_tx, g->part,
 		                          p->dtp.n_tiles, g->gth, g->gell, g->a,
@@ -169,20 +172,34 @@ static int grad_setup(hem_problem *p, const uint32_t *unique,
 		            "adjoint end");
 		return HEM_OK;
 	}
+	HEM_TRY_DEV(hemd_event_record(p->ev2, s), "event");
 	HEM_TRY_DEV(hemk_grad_passes(&p->dp, fp32, g->x, g->x32, g->u, g->pi,
 	                             g->rho, g->D, g->kappa, g->E, g->fd, g->gth,
 	                             g->gell, g->a, g->part, g->scal, s),
 	            "adjoint passes");
+	/* (the global passes' end, k_gfin, falls inside the bracket) */
+	HEM_TRY_DEV(hemd_event_record(p->ev3, s), "event");
 	return HEM_OK;
 }
+
+static int check_point(hem_problem *p, const uint32_t *unique,
+                       const double *theta, const double *ell);
 
 static int check_args(hem_problem *p, const uint32_t *unique,
                       const double *theta, const double *ell)
 {
+	if (p && p->ns > 1)
+		return hem_fail(HEM_ERR_INVALID, "one-sample handles only "
+		                                 "(hem_grad_samples takes "
+		                                 "several)");
+	return check_point(p, unique, theta, ell);
+}
+
+static int check_point(hem_problem *p, const uint32_t *unique,
+                       const double *theta, const double *ell)
+{
 	if (!p || !unique || !theta || !ell)
 		return hem_fail(HEM_ERR_INVALID, "null argument");
-	if (p->ns > 1)
-		return hem_fail(HEM_ERR_INVALID, "one-sample handles only");
 	if (p->weighted)
 		return hem_fail(HEM_ERR_INVALID, "no likelihood derivatives for "
 		                                 "a table with member weights");
@@ -239,12 +256,117 @@ int hem_grad(hem_problem *p, const uint32_t *unique, const double *theta,
 		}
 	}
 	if (st) {
+		float ma = 0;
+		if (hemd_event_elapsed(p->ev2, p->ev3, &ma))
+			ma = 0;
 		st->loglik = sc[0];
 		st->n_frag = sc[1];
 		st->ms = ms;
+		st->ms_adjoint = ma;
 	}
 out:
 	pthread_mutex_unlock(&p->mu);
+	return rc;
+}
+
+int hem_grad_samples(hem_problem *p, const uint32_t *unique,
+                     const double *theta, const double *log_efflen,
+                     int precision, double *g_theta, double *g_log_efflen,
+                     double *loglik, double *n_frag, hem_grad_stats *st)
+{
+	hem_grad_ws *g;
+	void *s;
+	double *tot = NULL;
+	float ms = 0, ma = 0;
+	int rc;
+	const size_t T = p ? p->n_tx : 0;
+	if ((rc = check_point(p, unique, theta, log_efflen)))
+		return rc;
+	if (precision != HEM_PREC_FP32 && precision != HEM_PREC_FP64)
+		return hem_fail(HEM_ERR_INVALID, "unknown precision %d",
+		                precision);
+	if (!grad_tiles(p))
+		return hem_fail(HEM_ERR_RANGE, "per-sample gradients run on the "
+		                               "component tiles; a component "
+		                               "exceeds the tile limits");
+	tot = malloc((size_t)3 * p->ns * 8);
+	if (!tot)
+		return hem_fail(HEM_ERR_NOMEM, "host allocation");
+	pthread_mutex_lock(&p->mu);
+	g = &p->gw;
+	s = p->stream;
+	if ((rc = ensure_grad(p)))
+		goto out;
+	{
+		int e = hemd_event_record(p->ev0, s);
+		if (!e)
+			e = hemd_h2d(g->theta, theta, T * 8, s);
+		if (!e)
+			e = hemd_h2d(g->ell, log_efflen, T * 8, s);
+		if (!e)
+			e = hemd_h2d(g->u, unique, T * 4, s);
+		if (!e)
+			e = hemk_grad_softmax_seg(g->theta, g->ell, p->d_ptinfo,
+			                          p->ns, g->pi, g->w, g->x, g->x32,
+			                          s);
+		if (!e)
+			e = hem_cntq_update(p);
+		if (!e)
+			e = hemd_event_record(p->ev2, s);
+		if (!e)
+			e = hemk_ptile_grad(&p->dtp, p->d_cntq,
+			                    precision == HEM_PREC_FP32, g->x, g->x32,
+			                    g->u, g->kappa_p, g->E, g->fd, g->part,
+			                    s);
+		if (!e)
+			e = hemd_event_record(p->ev3, s);
+		if (!e)
+			e = hemk_grad_fin_seg(g->E, g->pi, g->u, p->n_tx,
+			                      p->ns > 1 ? p->d_txsamp : NULL,
+			                      p->d_ptinfo, p->ns, g->part,
+			                      p->dtp.n_tiles, g->tot, g->gth,
+			                      g->gell, g->a, g->fd, s);
+		if (!e)
+			e = hemd_event_record(p->ev1, s);
+		if (!e && g_theta)
+			e = hemd_d2h(g_theta, g->gth, T * 8, s);
+		if (!e && g_log_efflen)
+			e = hemd_d2h(g_log_efflen, g->gell, T * 8, s);
+		if (!e)
+			e = hemd_d2h(tot, g->tot, (size_t)3 * p->ns * 8, s);
+		if (!e)
+			e = hemd_stream_sync(s);
+		if (!e)
+			e = hemd_event_elapsed(p->ev0, p->ev1, &ms);
+		if (!e)
+			e = hemd_event_elapsed(p->ev2, p->ev3, &ma);
+		if (e) {
+			rc = hem_dev_fail(e, "per-sample gradient");
+			goto out;
+		}
+	}
+	/* the kappa of these passes is per sample; no HVP follows */
+	g->kappa_tiled = 1;
+	for (uint32_t q = 0; q < p->ns; ++q) {
+		if (loglik)
+			loglik[q] = tot[3 * q] + tot[3 * q + 2];
+		if (n_frag)
+			n_frag[q] = tot[3 * q + 1];
+	}
+	if (st) {
+		double L = 0, N = 0;
+		for (uint32_t q = 0; q < p->ns; ++q) {
+			L += tot[3 * q] + tot[3 * q + 2];
+			N += tot[3 * q + 1];
+		}
+		st->loglik = L;
+		st->n_frag = N;
+		st->ms = ms;
+		st->ms_adjoint = ma;
+	}
+out:
+	pthread_mutex_unlock(&p->mu);
+	free(tot);
 	return rc;
 }
 

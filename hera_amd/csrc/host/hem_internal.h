@@ -64,6 +64,7 @@ typedef struct {
 	int kappa_tiled;               /* the last adjoint ran on the tiles:
 	                                  kappa_p is current, kappa is not  */
 	double *part, *scal;           /* 3 * max(HEMK_NORM_BLOCKS, tiles), 2 */
+	double *tot;                   /* 3 per sample                      */
 	/* standard errors: tile-order constants and K-wide CG state */
 	uint32_t K;                    /* capacity in directions            */
 	double *piq, *xq, *aq, *mq, *fdq, *kq;
@@ -127,6 +128,8 @@ struct hem_problem {
 	uint32_t *d_cls_orig_p;    /* its tile position -> class id         */
 	uint32_t *d_ptile_mem[HEM_TILE_ARRAYS];
 	uint64_t ptile_rec_bytes;  /* its tile records (algorithmic bytes)  */
+	uint32_t *d_ptinfo;        /* per sample {tx0, n_tx, first tile,
+	                              tiles} of this tiling                 */
 	int counts_integral;       /* class counts are integers < 2^32      */
 	double *h_cls_count;       /* host copy of the class counts         */
 	hem_batch_ws bw;

@@ -126,16 +126,43 @@ static int samples_upload(hem_problem *p, const hem_tiles_host *th,
  * into larger tiles, and room for the class counts in its tile order
  * (gathered at each solve). */
 static int plain_tiles(hem_problem *p, const hem_classes *c,
-                       const uint32_t *tx_off)
+                       const uint32_t *tx_off, const uint32_t *tx0,
+                       uint32_t ns)
 {
 	hem_tiles_host th;
 	int rc = hem_tiles_build(c->n_tx, c->n_cls, c->cls_off, c->cls_ref,
-	                         c->tx_len, tx_off, NULL, 1, c->member_weight,
+	                         c->tx_len, tx_off, tx0, ns, c->member_weight,
 	                         p->cfg.ptile_tx, p->cfg.ptile_cls,
 	                         p->cfg.ptile_nnz, &th);
 	if (rc == HEM_OK)
 		rc = hem_tiles_upload(p, &th, &p->dtp, p->d_ptile_mem,
 		                      &p->d_cls_orig_p, &p->ptile_rec_bytes);
+	/* per sample: its transcripts and its run of tiles (a sample's tiles
+	 * are consecutive; one sample: all of them) */
+	if (rc == HEM_OK) {
+		uint32_t *pi = calloc((size_t)ns * 4, 4);
+		if (!pi)
+			rc = hem_fail(HEM_ERR_NOMEM, "host allocation failed");
+		for (uint32_t q = 0; rc == HEM_OK && q < ns; ++q) {
+			pi[4 * q] = ns > 1 ? tx0[q] : 0;
+			pi[4 * q + 1] = ns > 1 ? tx0[q + 1] - tx0[q] : c->n_tx;
+		}
+		if (rc == HEM_OK && ns == 1)
+			pi[3] = th.n_tiles;
+		for (uint32_t i = th.n_tiles; rc == HEM_OK && ns > 1 && i-- > 0;) {
+			const uint32_t q = th.tsamp[i];
+			pi[4 * q + 2] = i;
+			pi[4 * q + 3]++;
+		}
+		if (rc == HEM_OK)
+			rc = dalloc(p, (void **)&p->d_ptinfo, (size_t)ns * 16);
+		if (rc == HEM_OK &&
+		    hemd_h2d(p->d_ptinfo, pi, (size_t)ns * 16, p->stream))
+			rc = hem_fail(HEM_ERR_DEVICE, "tile layout upload");
+		if (rc == HEM_OK && hemd_stream_sync(p->stream))
+			rc = hem_fail(HEM_ERR_DEVICE, "tile layout upload");
+		free(pi);
+	}
 	if (rc == HEM_OK)
 		rc = dalloc(p, (void **)&p->d_cntq,
 		            (size_t)(th.n_cls ? th.n_cls : 1) * 8);
@@ -404,7 +431,7 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 	 * member weights has only the single-problem tiling (weights in its
 	 * records) */
 	if (c->member_weight) {
-		int trc = plain_tiles(p, c, tx_off);
+		int trc = plain_tiles(p, c, tx_off, NULL, 1);
 		if (trc == HEM_OK)
 			trc = hemd_stream_sync(p->stream) ? HEM_ERR_DEVICE : HEM_OK;
 		if (trc && trc != HEM_ERR_RANGE) {
@@ -427,8 +454,8 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 			                       &p->d_cls_orig, NULL);
 			if (trc == HEM_OK && ns > 1)
 				trc = samples_upload(p, &th, tx0, cls0);
-			if (trc == HEM_OK && ns == 1)
-				trc = plain_tiles(p, c, tx_off);
+			if (trc == HEM_OK)
+				trc = plain_tiles(p, c, tx_off, tx0, ns);
 			p->h_cls_orig = th.cls_orig;
 			th.cls_orig = NULL;
 			hem_tiles_free_host(&th);
@@ -618,6 +645,7 @@ int hem_problem_destroy(hem_problem *p)
 	hemd_free(p->d_uq);
 	hemd_free(p->d_cntq);
 	hemd_free(p->d_sinfo);
+	hemd_free(p->d_ptinfo);
 	hemd_free(p->d_tsamp);
 	hemd_free(p->d_txsamp);
 	free(p->h_tx0);

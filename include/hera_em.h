@@ -543,12 +543,28 @@ int hem_solve_hera_em(hem_problem *p, const double *p0,
 typedef struct {
 	double loglik;   /* L                                                 */
 	double n_frag;   /* N                                                 */
-	double ms;       /* device time                                       */
+	double ms;       /* device time (uploads of the point included)       */
+	double ms_adjoint; /* device time of the class and transcript passes
+	                      (one tile launch, or the two global passes)     */
 } hem_grad_stats;
 
 int hem_grad(hem_problem *p, const uint32_t *unique, const double *theta,
              const double *log_efflen, int precision, double *g_theta,
              double *g_log_efflen, hem_grad_stats *st);
+
+/* The gradient of every sample of a multi-sample handle (hem_samples_create;
+ * a one-sample handle works too) in one pass: sample q's objective L_q with
+ * pi = softmax(theta) within the sample, its gradient E_t - N_q pi_t and
+ * -E_t in the sample's columns of g_theta and g_log_efflen, loglik[q] and
+ * n_frag[q] = N_q (n_samples entries each).  Each sample's values are what
+ * hem_grad gives on the sample's own handle, up to rounding (the softmax
+ * and the sums run in another order).  Needs every component within the
+ * tile limits (HEM_ERR_RANGE otherwise); st sums over samples.  Outputs
+ * may be NULL. */
+int hem_grad_samples(hem_problem *p, const uint32_t *unique,
+                     const double *theta, const double *log_efflen,
+                     int precision, double *g_theta, double *g_log_efflen,
+                     double *loglik, double *n_frag, hem_grad_stats *st);
 
 /* Hessian-vector product in theta, FP64: hv = (d^2 L / d theta^2) v. */
 int hem_hvp(hem_problem *p, const uint32_t *unique, const double *theta,

@@ -81,6 +81,51 @@ def test_samples_solve_batch_equals_oracle(samples):
                                       ora["tpm"])
 
 
+@pytest.mark.parametrize("prec", [H.PREC_FP64, H.PREC_FP32])
+def test_samples_gradient_equals_own_handles(samples, prec):
+    """One pass over the multi-sample handle gives each sample the gradient
+    its own handle gives (softmax within the sample), to rounding."""
+    cls, us = samples
+    rng = np.random.default_rng(21)
+    ths = [rng.normal(0.0, 1.0, c.n_tx) for c in cls]
+    ells = [np.log(c.tx_len.astype(float)) + rng.normal(0.0, 0.1, c.n_tx)
+            for c in cls]
+    mp = H.Problem.from_samples(cls)
+    try:
+        g = mp.grad_samples(np.concatenate(us), np.concatenate(ths),
+                            np.concatenate(ells), precision=prec)
+        with pytest.raises(H.HemError):    # hem_grad is one-sample only
+            mp.grad(np.concatenate(us), np.concatenate(ths),
+                    np.concatenate(ells))
+    finally:
+        mp.close()
+    say("per-sample loglik %s, %.3f ms (adjoint %.3f ms)" % (
+        g["loglik"].tolist(), g["ms"], g["ms_adjoint"]))
+    t0 = 0
+    for q, (c, u) in enumerate(zip(cls, us)):
+        p = H.Problem(c)
+        own = p.grad(u, ths[q], ells[q], precision=prec)
+        p.close()
+        sl = slice(t0, t0 + c.n_tx)
+        t0 += c.n_tx
+        assert g["loglik"][q] == pytest.approx(own["loglik"], rel=1e-12)
+        assert g["n_frag"][q] == pytest.approx(own["n_frag"], rel=1e-12)
+        gt = own["g_theta"]
+        np.testing.assert_allclose(g["g_theta"][sl], gt, rtol=0,
+                                   atol=1e-10 * np.abs(gt).max())
+        np.testing.assert_allclose(g["g_log_efflen"][sl],
+                                   own["g_log_efflen"], rtol=1e-12)
+        # softmax invariance within each sample
+        assert abs(g["g_theta"][sl].sum()) <= 1e-9 * np.abs(gt).sum()
+    if prec == H.PREC_FP64:
+        # the middle sample against the CPU gradient
+        L, ogt, oge = O.grad(cls[1], us[1], ths[1], ells[1])
+        sl = slice(cls[0].n_tx, cls[0].n_tx + cls[1].n_tx)
+        assert g["loglik"][1] == pytest.approx(L, rel=1e-12)
+        np.testing.assert_allclose(g["g_theta"][sl], ogt, rtol=0,
+                                   atol=1e-10 * np.abs(ogt).max())
+
+
 def test_samples_reject_single_problem_entry_points(samples):
     cls, us = samples
     mp = H.Problem.from_samples(cls)

@@ -7,8 +7,8 @@ change or 1,000 iterations in one batched run.
 
 Ragged multi-sample batch: 64 samples with transcript counts log-uniform in
 5,000-200,000, 10x as many drawn classes, fragments uniform in 5M-80M, 20
-replicates each, in one multi-sample handle, plus a gradient pass per
-sample.
+replicates each, in one multi-sample handle, plus every sample's gradient
+in one call on the same handle.
 
 Every replicate of the default mode (FP64, exact sums) is the reference EM
 of its own counts, bit for bit; the oracle checks a few replicates chosen
@@ -115,8 +115,16 @@ def test_ragged_64_samples_20_replicates():
     mp = H.Problem.from_samples(cls)
     try:
         res = mp.bootstrap(np.concatenate(us), seeds, **STOP)
+        # every sample's gradient in one call at its first replicate's
+        # estimate (theta = log est, softmax within the sample)
+        with np.errstate(divide="ignore"):
+            th_all = np.log(res["est"][0])
+        th_all = np.where(np.isfinite(th_all), th_all, -745.0)
+        ell_all = np.log(np.concatenate([c.tx_len for c in cls]).astype(float))
+        gs = mp.grad_samples(np.concatenate(us), th_all, ell_all)
     finally:
         mp.close()
+    say("ragged gradient in one call: %.2f ms" % gs["ms"])
     it = res["iters"]
     say("ragged: iterations %d..%d, %.1f ms solve" % (
         it.min(), it.max(), res["stats"]["ms_solve"]))
@@ -152,28 +160,29 @@ def test_ragged_64_samples_20_replicates():
             q, cls[q].n_tx, ks.tolist(), it[q, :2].tolist()))
         np.testing.assert_array_equal(ks, it[q, :2])
         np.testing.assert_array_equal(res["est"][:2, a:e], est)
-    # a gradient pass per sample at its first replicate's estimate, with
-    # the sample's own counts; the median sample against the CPU gradient
+    # the one-call gradient against each sample's own handle at the same
+    # point; the median sample also against the CPU gradient
     for q in range(S):
         a, e = int(tx0[q]), int(tx0[q + 1])
-        with np.errstate(divide="ignore"):
-            th = np.log(res["est"][0, a:e])
-        th = np.where(np.isfinite(th), th - th[np.isfinite(th)].max(), -745.0)
-        ell = np.log(cls[q].tx_len.astype(float))
+        th, ell = th_all[a:e], ell_all[a:e]
         p = H.Problem(cls[q])
         try:
             g = p.grad(us[q], th, ell)
         finally:
             p.close()
-        gt, ge = g["g_theta"], g["g_log_efflen"]
+        gt, ge = gs["g_theta"][a:e], gs["g_log_efflen"][a:e]
         assert np.all(np.isfinite(gt)) and np.all(np.isfinite(ge))
         # softmax invariance, and d/d(log efflen) = -E[count] <= 0
         assert abs(gt.sum()) <= 1e-9 * np.abs(gt).sum() + 1e-9
         assert np.all(ge <= 0)
+        np.testing.assert_allclose(gt, g["g_theta"], rtol=0,
+                                   atol=1e-10 * np.abs(g["g_theta"]).max())
+        np.testing.assert_allclose(ge, g["g_log_efflen"], rtol=1e-12)
+        assert gs["loglik"][q] == pytest.approx(g["loglik"], rel=1e-12)
         if q == check[1]:
             L, ogt, oge = O.grad(cls[q], us[q], th, ell)
             np.testing.assert_allclose(gt, ogt, rtol=1e-10,
                                        atol=1e-10 * np.abs(ogt).max())
             np.testing.assert_allclose(ge, oge, rtol=1e-10,
                                        atol=1e-10 * np.abs(oge).max())
-            assert g["loglik"] == pytest.approx(L, rel=1e-12)
+            assert gs["loglik"][q] == pytest.approx(L, rel=1e-12)

@@ -148,6 +148,8 @@ static int time_batch(hem_problem *p, tune_ctx *t, uint32_t R, double *us)
 }
 
 #define NCAND(a) (sizeof(a) / sizeof((a)[0]))
+/* the fraction by which a candidate must beat the default */
+#define HEM_TUNE_MARGIN 0.02
 
 /* Single-problem tile targets tried (transcripts, classes, members); the
  * first is the default. */
@@ -286,6 +288,15 @@ int hem_autotune(const hem_classes *cls, const uint32_t *unique,
 		if (rc)
 			goto out;
 	}
+	/* a candidate replaces the default only when it is clearly faster:
+	 * run-to-run noise of the short timed runs is about 1% */
+	if (best_plain < DBL_MAX &&
+	    best_plain > r.us_plain_default * (1.0 - HEM_TUNE_MARGIN)) {
+		best.ptile_tx = HEM_PTILE_TX_TARGET;
+		best.ptile_cls = HEM_PTILE_CLS_TARGET;
+		best.ptile_nnz = HEM_PTILE_NNZ_TARGET;
+		best_plain = r.us_plain_default;
+	}
 	r.us_plain_best = best_plain == DBL_MAX ? 0 : best_plain;
 
 	/* batched tiles and chunks (the default tiles were timed above) */
@@ -310,6 +321,13 @@ int hem_autotune(const hem_classes *cls, const uint32_t *unique,
 			p = NULL;
 			if (rc)
 				goto out;
+		}
+		if (best_batch > r.us_batch_default * (1.0 - HEM_TUNE_MARGIN)) {
+			best.tile_tx = HEM_TILE_TX_TARGET;
+			best.tile_cls = HEM_TILE_CLS_TARGET;
+			best.tile_nnz = HEM_TILE_NNZ_TARGET;
+			best.chunk = 0;
+			best_batch = r.us_batch_default;
 		}
 		r.us_batch_best = best_batch;
 	}

@@ -297,8 +297,8 @@ typedef struct {
  * members per tile) on plain exact solves of unique, and, with n_rep > 0,
  * the batched tile targets and replicate chunk on an exact batch of n_rep
  * replicates of the table's own counts.  Each candidate is a temporary
- * handle; the defaults are among the candidates, so the choice is never
- * slower than them as measured.  With cache_path, a configuration stored
+ * handle; the defaults are among the candidates and stay the choice
+ * unless a candidate is at least 2% faster (the timed runs are short).  With cache_path, a configuration stored
  * under the same key (n_tx, n_cls, members, n_rep, device arch) is
  * returned without timing, and a new search appends one line to the file
  * (concurrent tuners of one key may both append; the last line read

@@ -77,6 +77,14 @@ def test_autotune_and_cache(cfg1, reference, tmp_path):
     assert rep["from_cache"] == 0 and rep["n_candidates"] >= 6
     assert rep["us_plain_best"] <= rep["us_plain_default"]
     assert rep["us_batch_best"] <= rep["us_batch_default"]
+    # a choice other than the defaults is at least 2% faster as measured
+    d = H.tile_config()
+    if (cfg["ptile_tx"], cfg["ptile_cls"], cfg["ptile_nnz"]) != (
+            d.ptile_tx, d.ptile_cls, d.ptile_nnz):
+        assert rep["us_plain_best"] <= 0.98 * rep["us_plain_default"]
+    if (cfg["tile_tx"], cfg["tile_cls"], cfg["tile_nnz"], cfg["chunk"]) != (
+            d.tile_tx, d.tile_cls, d.tile_nnz, d.chunk):
+        assert rep["us_batch_best"] <= 0.98 * rep["us_batch_default"]
     assert rep["key"].startswith("T=%d,C=%d,Z=%d,R=8,arch=gfx950" % (
         cls.n_tx, cls.n_cls, cls.nnz))
     cfg2, rep2 = H.autotune(cls, u, n_rep=8, cache=cache)

@@ -173,6 +173,7 @@ __device__ __forceinline__ void tile_map(
 	constexpr uint32_t ES = TBLOCK / RC;
 	const bool live = alive[r] != 0;
 	(void)tl;
+	int out = 0;   // a value outside em_quot's range
 	if (live) {
 		for (uint32_t c = e0; c < tc.nc; c += ES) {
 			uint32_t k = ts.coff[c];
@@ -190,11 +191,76 @@ __device__ __forceinline__ void tile_map(
 			}
 			for (; k < k1; ++k)
 				acc += lin[(ts.kmem[k] & 0xFFu) * RC + r];
-			D[c * RC + r] = acc == 0.0 ? 1.0 : acc;
+			const double d = acc == 0.0 ? 1.0 : acc;
+			D[c * RC + r] = d;
+			out |= !(d >= EM_QUOT_LO && d <= EM_QUOT_HI);
+		}
+		// lam_t n_c with n_c < 2^32 stays in em_quot's range when lam_t
+		// is 0 or in [2^-299, 2^260]
+		for (uint32_t t = e0; t < tc.nt; t += ES) {
+			const double l = lin[t * RC + r];
+			out |= l != 0.0 && !(l >= 0x1p-299 && l <= 0x1p260);
 		}
 	}
-	__syncthreads();
-	if (live) {
+	const bool fast = !__syncthreads_or(out);
+	if (live && fast) {
+		// every divisor and dividend in em_quot's range (the usual
+		// case): the same quotients with fewer instructions, no masked
+		// slots, and eight occurrences at a time with all their LDS
+		// loads issued before the first quotient (the pass waits on
+		// dependent LDS loads more than on arithmetic)
+		for (uint32_t t = e0; t < tc.nt; t += ES) {
+			const double l = lin[t * RC + r];
+			uint32_t j = ts.toff[t];
+			const uint32_t j1 = ts.toff[t + 1];
+			double acc = 0.0;
+			for (; j + 8 <= j1; j += 8) {
+				uint32_t c[8];
+				double x[8];
+#pragma unroll
+				for (int q = 0; q < 8; ++q)
+					c[q] = ts.tcls[j + q];
+				double nn[8], dd[8];
+#pragma unroll
+				for (int q = 0; q < 8; ++q) {
+					nn[q] = (double)n[c[q] * RC + r];
+					dd[q] = D[c[q] * RC + r];
+				}
+#pragma unroll
+				for (int q = 0; q < 8; ++q)
+					x[q] = em_quot(l * nn[q], dd[q]);
+#pragma unroll
+				for (int q = 0; q < 8; ++q)
+					acc += x[q];
+			}
+			for (; j + 4 <= j1; j += 4) {
+				uint32_t c[4];
+				double x[4];
+#pragma unroll
+				for (int q = 0; q < 4; ++q)
+					c[q] = ts.tcls[j + q];
+#pragma unroll
+				for (int q = 0; q < 4; ++q)
+					x[q] = em_quot(l * (double)n[c[q] * RC + r],
+					               D[c[q] * RC + r]);
+#pragma unroll
+				for (int q = 0; q < 4; ++q)
+					acc += x[q];
+			}
+			for (; j < j1; ++j) {
+				const uint32_t c = ts.tcls[j];
+				acc += em_quot(l * (double)n[c * RC + r],
+				               D[c * RC + r]);
+			}
+			const double nw = ((double)u[t * RC + r] + acc) /
+			                  (double)ts.len[t];
+			lout[t * RC + r] = nw;
+			if (gout)
+				gout[(size_t)(tc.tx_base + t) * R + qr] = nw;
+			if (stopf && nw > smin && fabs(nw - l) / nw > srel)
+				stopf[r] = 1;
+		}
+	} else if (live) {
 		// each occurrence's term (lam_t n_c) / D_c computed where the
 		// transcript's ordered sum takes it; four occurrences at a time,
 		// their LDS reads and divisions straight-line (a slot past the

@@ -286,16 +286,17 @@ int hemk_ptile_hvp(const hemk_tiles *t, const double *x, const double *v,
  * over hemk_se_rowblocks(T) row blocks, folded by hemk_cg_fold. */
 int hemk_se_chunk(const hemk_tiles *t, uint32_t K);
 uint32_t hemk_se_rowblocks(uint32_t T);
+/* Q = F P = -(H P) + delta P for every direction (pv: pi . P per
+ * direction, scal[1] = N), and per (tile, direction) partials of P . F P
+ * in part[tile * K + k], folded over n_tiles by hemk_cg_fold. */
 int hemk_se_tile(const hemk_tiles *t, int chunk, uint32_t K, const double *xq,
-                 const double *aq, const double *kq, const double *P,
-                 double *Q, void *s);
+                 const double *aq, const double *kq, const double *piq,
+                 const double *pv, const double *scal, double delta,
+                 const double *P, double *Q, double *part, void *s);
 int hemk_cg_init(const double *piq, const double *mq, const uint32_t *jq,
                  const double *pj, uint32_t T, uint32_t K, double *X,
                  double *R, double *P, double *prz, double *prr, double *ppv,
                  void *s);
-int hemk_cg_fp(const double *piq, const double *pv, const double *scal,
-               double delta, uint32_t T, uint32_t K, const double *P,
-               double *Q, double *ppfp, void *s);
 int hemk_cg_xr(const double *mq, const double *rz, const double *pfp,
                const double *rr, const double *bb, uint32_t T, uint32_t K,
                const double *P, const double *FP, double *X, double *R,

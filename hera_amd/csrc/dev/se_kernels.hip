@@ -49,22 +49,31 @@ __device__ __forceinline__ STile s_tile(const hemk_tiles &t, uint32_t i)
 
 #define SPF 2  // register slots per thread for a chunk's P rows
 
-// Q = M(pi . P) for one tile, K directions in chunks of RC:
+// The damped Fisher product of one tile, K directions in chunks of RC:
 //   Dd_c = sum_{s in c} x_s P_s,  Q_t = P_t a_t - x_t sum_{c ni t} kappa_c Dd_c
+// is the HVP without its rank-one term; F P = -(H P) + delta P then is
+//   Q_t <- -Q_t - N (pi . P) pi_t + delta P_t,
+// written in place of Q, and the tile's p . F p per direction goes to
+// part[tile * K + k] (its transcripts in order of the four row groups).
 template <int RC>
 __global__ __launch_bounds__(SB) void k_se_tile(
     hemk_tiles tl, uint32_t K, const double *__restrict__ xq,
     const double *__restrict__ aq, const double *__restrict__ kq,
-    const double *__restrict__ P, double *__restrict__ Q)
+    const double *__restrict__ piq, const double *__restrict__ pv,
+    const double *__restrict__ scal, double delta,
+    const double *__restrict__ P, double *__restrict__ Q,
+    double *__restrict__ part)
 {
 	extern __shared__ double smem[];
+	__shared__ double red[SB];
 	const STile tc = s_tile(tl, blockIdx.x);
 	const uint32_t mt = tl.max_tx, mc = tl.max_cls, mz = tl.max_nnz;
 	double *Pl = smem;             // mt * RC
 	double *Dd = Pl + mt * RC;     // mc * RC
 	double *xs = Dd + mc * RC;     // mt
 	double *as = xs + mt;          // mt
-	double *ks = as + mt;          // mc
+	double *ps = as + mt;          // mt
+	double *ks = ps + mt;          // mc
 	uint32_t *kmem = (uint32_t *)(ks + mc);
 	uint16_t *coff = (uint16_t *)(kmem + mz);
 	uint16_t *toff = coff + (mc + 1);
@@ -80,7 +89,9 @@ __global__ __launch_bounds__(SB) void k_se_tile(
 	for (uint32_t i = threadIdx.x; i < tc.nt; i += SB) {
 		xs[i] = xq[tc.tx_base + i];
 		as[i] = aq[tc.tx_base + i];
+		ps[i] = piq[tc.tx_base + i];
 	}
+	const double N = scal[1];
 	for (uint32_t i = threadIdx.x; i < tc.nc; i += SB)
 		ks[i] = kq[tc.cls_base + i];
 	const uint32_t r = threadIdx.x % RC;
@@ -116,24 +127,70 @@ __global__ __launch_bounds__(SB) void k_se_tile(
 				            : 0.0;
 			}
 		}
+		// (four members / eight occurrences at a time, their LDS loads
+		// issued before the first product or sum)
 		for (uint32_t c = e0; c < tc.nc; c += ES) {
 			double acc = 0;
-			for (uint32_t j = coff[c]; j < coff[c + 1]; ++j) {
+			uint32_t j = coff[c];
+			const uint32_t j1 = coff[c + 1];
+			for (; j + 4 <= j1; j += 4) {
+				uint32_t t[4];
+				double xv[4], pv4[4];
+#pragma unroll
+				for (int q = 0; q < 4; ++q)
+					t[q] = kmem[j + q] & 0xFFu;
+#pragma unroll
+				for (int q = 0; q < 4; ++q) {
+					xv[q] = xs[t[q]];
+					pv4[q] = Pl[t[q] * RC + r];
+				}
+#pragma unroll
+				for (int q = 0; q < 4; ++q)
+					acc += xv[q] * pv4[q];
+			}
+			for (; j < j1; ++j) {
 				const uint32_t t = kmem[j] & 0xFFu;
 				acc += xs[t] * Pl[t * RC + r];
 			}
 			Dd[c * RC + r] = ks[c] * acc;
 		}
 		__syncthreads();
+		const double cv = kin ? N * pv[k] : 0.0;
+		double pf = 0;
 		for (uint32_t t = e0; t < tc.nt; t += ES) {
 			double acc = 0;
-			for (uint32_t j = toff[t]; j < toff[t + 1]; ++j) {
-				const uint32_t c = tcls[j];
-				acc += Dd[c * RC + r];
+			uint32_t j = toff[t];
+			const uint32_t j1 = toff[t + 1];
+			for (; j + 8 <= j1; j += 8) {
+				uint32_t c[8];
+				double d[8];
+#pragma unroll
+				for (int q = 0; q < 8; ++q)
+					c[q] = tcls[j + q];
+#pragma unroll
+				for (int q = 0; q < 8; ++q)
+					d[q] = Dd[c[q] * RC + r];
+#pragma unroll
+				for (int q = 0; q < 8; ++q)
+					acc += d[q];
 			}
-			if (kin)
-				Q[(size_t)(tc.tx_base + t) * K + k] =
-				    Pl[t * RC + r] * as[t] - xs[t] * acc;
+			for (; j < j1; ++j)
+				acc += Dd[tcls[j] * RC + r];
+			if (kin) {
+				const double p = Pl[t * RC + r];
+				const double hq = p * as[t] - xs[t] * acc;
+				const double f = -hq - cv * ps[t] + delta * p;
+				Q[(size_t)(tc.tx_base + t) * K + k] = f;
+				pf += p * f;
+			}
+		}
+		red[threadIdx.x] = pf;
+		__syncthreads();
+		if (e0 == 0 && kin) {
+			double sum = red[r];
+			for (uint32_t i = 1; i < ES; ++i)
+				sum += red[i * RC + r];
+			part[(size_t)blockIdx.x * K + k] = sum;
 		}
 		__syncthreads();
 	}
@@ -212,28 +269,6 @@ __global__ __launch_bounds__(SB) void k_cg_init(
 }
 
 // F p = -Q - N pv pi + delta p (into Q); partials of p.F p
-__global__ __launch_bounds__(SB) void k_cg_fp(
-    const double *__restrict__ piq, const double *__restrict__ pv,
-    const double *__restrict__ scal, double delta, uint32_t T, uint32_t K,
-    const double *__restrict__ P, double *__restrict__ Q,
-    double *__restrict__ ppfp)
-{
-	const SIdx ix = s_idx(T, K);
-	double v[1] = {0};
-	if (ix.kin) {
-		const double c = scal[1] * pv[ix.k];
-		for (uint32_t q = ix.q0; q < ix.q1; q += SB / SCOLS) {
-			const size_t g = (size_t)q * K + ix.k;
-			const double p = P[g];
-			const double f = -Q[g] - c * piq[q] + delta * p;
-			Q[g] = f;
-			v[0] += p * f;
-		}
-	}
-	double *const part[1] = {ppfp};
-	s_partials<1>(v, K, part);
-}
-
 // A direction whose residual is down to CG_TOL |b| is finished: its x and r
 // stay as they are (further steps would only stir rounding noise).
 #define CG_TOL2 1e-28
@@ -304,18 +339,32 @@ __global__ __launch_bounds__(SB) void k_cg_pix(const double *__restrict__ piq,
 	s_partials<1>(v, K, part);
 }
 
-// out[k] = sum over row blocks of part[rb*K + k], in row-block order.
+// out[k] = sum over row blocks of part[rb*K + k]: FOLD_K directions per
+// block; thread (g, kk) adds the row blocks g, g + FOLD_G, ... of direction
+// kk in order, then the FOLD_G group sums are added in group order (a
+// fixed order, with every block's loads in flight at once rather than one
+// thread per direction walking all row blocks).
+#define FOLD_K 8
+#define FOLD_G (SB / FOLD_K)
 __global__ __launch_bounds__(SB) void k_cg_fold(const double *__restrict__ part,
                                                 uint32_t nrb, uint32_t K,
                                                 double *__restrict__ out)
 {
-	const uint32_t k = blockIdx.x * SB + threadIdx.x;
-	if (k >= K)
-		return;
+	__shared__ double sh[FOLD_G][FOLD_K];
+	const uint32_t kk = threadIdx.x % FOLD_K, g = threadIdx.x / FOLD_K;
+	const uint32_t k = blockIdx.x * FOLD_K + kk;
 	double s = 0;
-	for (uint32_t b = 0; b < nrb; ++b)
-		s += part[(size_t)b * K + k];
-	out[k] = s;
+	if (k < K)
+		for (uint32_t b = g; b < nrb; b += FOLD_G)
+			s += part[(size_t)b * K + k];
+	sh[g][kk] = s;
+	__syncthreads();
+	if (g == 0 && k < K) {
+		double t = sh[0][kk];
+		for (uint32_t i = 1; i < FOLD_G; ++i)
+			t += sh[i][kk];
+		out[k] = t;
+	}
 }
 
 // se_k = sqrt(max(0, pj (x[jq] - pi.x))), resid_k = sqrt(rr / bb)
@@ -375,18 +424,19 @@ inline uint32_t sdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b
 size_t se_lds(const hemk_tiles *t, int rc)
 {
 	return (size_t)rc * ((size_t)t->max_tx * 8 + (size_t)t->max_cls * 8) +
-	       (size_t)t->max_tx * 16 + (size_t)t->max_cls * 8 +
+	       (size_t)t->max_tx * 24 + (size_t)t->max_cls * 8 +
 	       (size_t)t->max_nnz * 6 + 2 * ((size_t)t->max_cls + 1) +
 	       2 * ((size_t)t->max_tx + 1) + 16;
 }
 
 template <int RC>
 int launch_se_tile(const hemk_tiles *t, uint32_t K, const double *xq,
-                   const double *aq, const double *kq, const double *P,
-                   double *Q, hipStream_t s)
+                   const double *aq, const double *kq, const double *piq,
+                   const double *pv, const double *scal, double delta,
+                   const double *P, double *Q, double *part, hipStream_t s)
 {
-	k_se_tile<RC><<<t->n_tiles, SB, se_lds(t, RC), s>>>(*t, K, xq, aq, kq,
-	                                                    P, Q);
+	k_se_tile<RC><<<t->n_tiles, SB, se_lds(t, RC), s>>>(
+	    *t, K, xq, aq, kq, piq, pv, scal, delta, P, Q, part);
 	return (int)hipGetLastError();
 }
 
@@ -405,19 +455,22 @@ int hemk_se_chunk(const hemk_tiles *t, uint32_t K)
 uint32_t hemk_se_rowblocks(uint32_t T) { return sdiv(T, SROWS); }
 
 int hemk_se_tile(const hemk_tiles *t, int chunk, uint32_t K, const double *xq,
-                 const double *aq, const double *kq, const double *P,
-                 double *Q, void *s)
+                 const double *aq, const double *kq, const double *piq,
+                 const double *pv, const double *scal, double delta,
+                 const double *P, double *Q, double *part, void *s)
 {
 	hipStream_t st = (hipStream_t)s;
+#define SE_T(RC) launch_se_tile<RC>(t, K, xq, aq, kq, piq, pv, scal, delta, P, Q, part, st)
 	switch (chunk) {
-	case 1: return launch_se_tile<1>(t, K, xq, aq, kq, P, Q, st);
-	case 2: return launch_se_tile<2>(t, K, xq, aq, kq, P, Q, st);
-	case 4: return launch_se_tile<4>(t, K, xq, aq, kq, P, Q, st);
-	case 8: return launch_se_tile<8>(t, K, xq, aq, kq, P, Q, st);
-	case 16: return launch_se_tile<16>(t, K, xq, aq, kq, P, Q, st);
-	case 32: return launch_se_tile<32>(t, K, xq, aq, kq, P, Q, st);
+	case 1: return SE_T(1);
+	case 2: return SE_T(2);
+	case 4: return SE_T(4);
+	case 8: return SE_T(8);
+	case 16: return SE_T(16);
+	case 32: return SE_T(32);
 	default: return (int)hipErrorInvalidValue;
 	}
+#undef SE_T
 }
 
 int hemk_cg_init(const double *piq, const double *mq, const uint32_t *jq,
@@ -428,16 +481,6 @@ int hemk_cg_init(const double *piq, const double *mq, const uint32_t *jq,
 	dim3 g(sdiv(T, SROWS), sdiv(K, SCOLS));
 	k_cg_init<<<g, SB, 0, (hipStream_t)s>>>(piq, mq, jq, pj, 0.0, T, K, X, R,
 	                                        P, prz, prr, ppv);
-	return (int)hipGetLastError();
-}
-
-int hemk_cg_fp(const double *piq, const double *pv, const double *scal,
-               double delta, uint32_t T, uint32_t K, const double *P,
-               double *Q, double *ppfp, void *s)
-{
-	dim3 g(sdiv(T, SROWS), sdiv(K, SCOLS));
-	k_cg_fp<<<g, SB, 0, (hipStream_t)s>>>(piq, pv, scal, delta, T, K, P, Q,
-	                                      ppfp);
 	return (int)hipGetLastError();
 }
 
@@ -473,7 +516,7 @@ int hemk_cg_pix(const double *piq, uint32_t T, uint32_t K, const double *X,
 int hemk_cg_fold(const double *part, uint32_t nrb, uint32_t K, double *out,
                  void *s)
 {
-	k_cg_fold<<<sdiv(K, SB), SB, 0, (hipStream_t)s>>>(part, nrb, K, out);
+	k_cg_fold<<<sdiv(K, FOLD_K), SB, 0, (hipStream_t)s>>>(part, nrb, K, out);
 	return (int)hipGetLastError();
 }
 

@@ -120,7 +120,7 @@ static int ensure_se(hem_problem *p, uint32_t K)
 	GALLOC(g->prz, NRB * K * 8);
 	GALLOC(g->prr, NRB * K * 8);
 	GALLOC(g->ppv, NRB * K * 8);
-	GALLOC(g->ppfp, NRB * K * 8);
+	GALLOC(g->ppfp, (size_t)(p->dt.n_tiles ? p->dt.n_tiles : 1) * K * 8);
 	GALLOC(g->rz0, (size_t)K * 8);
 	GALLOC(g->rz1, (size_t)K * 8);
 	GALLOC(g->rr, (size_t)K * 8);
@@ -477,12 +477,11 @@ static int cg_run(hem_problem *p, uint32_t K, uint32_t steps, double damping,
 	HEM_TRY_DEV(hemk_cg_fold(g->ppv, NRB, K, g->pv, s), "cg fold");
 	for (uint32_t it = 0; it < steps; ++it) {
 		HEM_TRY_DEV(hemk_se_tile(&p->dt, chunk, K, g->xq, g->aq, g->kq,
-		                         g->P, g->Q, s),
+		                         g->piq, g->pv, g->scal, damping, g->P,
+		                         g->Q, g->ppfp, s),
 		            "cg tile pass");
-		HEM_TRY_DEV(hemk_cg_fp(g->piq, g->pv, g->scal, damping, T, K,
-		                       g->P, g->Q, g->ppfp, s),
-		            "cg F p");
-		HEM_TRY_DEV(hemk_cg_fold(g->ppfp, NRB, K, g->pfp, s), "cg fold");
+		HEM_TRY_DEV(hemk_cg_fold(g->ppfp, p->dt.n_tiles, K, g->pfp, s),
+		            "cg fold");
 		HEM_TRY_DEV(hemk_cg_xr(g->mq, rz, g->pfp, g->rr, g->bb, T, K,
 		                       g->P, g->Q, g->X, g->R, g->prz, g->prr, s),
 		            "cg update");

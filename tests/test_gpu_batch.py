@@ -227,6 +227,43 @@ def test_batch_fp32_equals_plain_fp32(cfg1, case):
         assert np.all(np.isfinite(tree["est"][r]))
 
 
+@pytest.mark.parametrize("k", [20, 50, 100])
+def test_batch_tiny_densities_bitwise(k):
+    """Pairs {A, B} where only A has unique reads: under hera-EM's clamp
+    B's density decays through (0, 2^-299) to denormals and 0 within 100
+    iterations (checked on the oracle below), so the batched kernel's
+    tiles leave its fast quotient for the division; every replicate stays
+    the oracle's, bit for bit, next to tiles of ordinary densities."""
+    P = 40
+    members = [[2 * i, 2 * i + 1] for i in range(P)]
+    counts = [float(50 + 3 * i) for i in range(P)]
+    big, ub = tiny_problem(9, n_tx=60, n_cls=240, dup=True)
+    T0 = big.n_tx
+    members = [big.cls_ref[big.cls_off[i]:big.cls_off[i + 1]].tolist()
+               for i in range(big.n_cls)] + [[T0 + a for a in m]
+                                             for m in members]
+    counts = big.cls_count.tolist() + counts
+    lens = big.tx_len.tolist() + [1000 + 10 * i for i in range(2 * P)]
+    cls = H.Classes.from_lists(T0 + 2 * P, members, counts, lens)
+    u = np.concatenate([ub, np.zeros(2 * P, np.uint32)])
+    u[T0::2] = 1000 + np.arange(P)
+    U, CC = perturbed(cls, u, 3, 40 + k)
+    U[:, T0 + 1::2] = 0
+    kw = dict(step_form=1, clamp_form=1, stop_rel=0.0, min_round=k,
+              max_round=k)
+    ora = O.estimate_count(with_counts(cls, CC[0]), U[0], **kw)
+    b = ora["lam"][T0 + 1::2]
+    if k <= 50:
+        assert np.any((b > 0) & (b < 2.0 ** -299))
+    p = H.Problem(cls)
+    try:
+        res = p.solve_batch(U, CC, step=1, clamp=1, stop_rel_change=0.0,
+                            min_iter=k, max_iter=k)
+    finally:
+        p.close()
+    check_replicates(cls, U, CC, res, **kw)
+
+
 def test_batch_rejects_untileable_and_unknown_precision():
     members = [[i, i + 1] for i in range(99)]
     big = H.Classes.from_lists(100, members, [3.0] * 99, [500] * 100)

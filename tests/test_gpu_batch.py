@@ -227,13 +227,15 @@ def test_batch_fp32_equals_plain_fp32(cfg1, case):
         assert np.all(np.isfinite(tree["est"][r]))
 
 
-@pytest.mark.parametrize("k", [20, 50, 100])
+@pytest.mark.parametrize("k", [50, 80, 100])
 def test_batch_tiny_densities_bitwise(k):
     """Pairs {A, B} where only A has unique reads: under hera-EM's clamp
     B's density decays through (0, 2^-299) to denormals and 0 within 100
-    iterations (checked on the oracle below), so the batched kernel's
-    tiles leave its fast quotient for the division; every replicate stays
-    the oracle's, bit for bit, next to tiles of ordinary densities."""
+    iterations (checked on the oracle below: at 50, 80 and 100 iterations
+    some replicate ends with densities in (0, 2^-299), at 100 with
+    denormals), so the batched kernel's tiles leave its fast quotient for
+    the division; every replicate stays the oracle's, bit for bit, next to
+    tiles of ordinary densities."""
     P = 40
     members = [[2 * i, 2 * i + 1] for i in range(P)]
     counts = [float(50 + 3 * i) for i in range(P)]
@@ -251,10 +253,12 @@ def test_batch_tiny_densities_bitwise(k):
     U[:, T0 + 1::2] = 0
     kw = dict(step_form=1, clamp_form=1, stop_rel=0.0, min_round=k,
               max_round=k)
-    ora = O.estimate_count(with_counts(cls, CC[0]), U[0], **kw)
-    b = ora["lam"][T0 + 1::2]
-    if k <= 50:
-        assert np.any((b > 0) & (b < 2.0 ** -299))
+    tiny = False
+    for r in range(U.shape[0]):
+        b = O.estimate_count(with_counts(cls, CC[r]), U[r],
+                             **kw)["lam"][T0 + 1::2]
+        tiny |= bool(np.any((b > 0) & (b < 2.0 ** -299)))
+    assert tiny
     p = H.Problem(cls)
     try:
         res = p.solve_batch(U, CC, step=1, clamp=1, stop_rel_change=0.0,

@@ -74,9 +74,12 @@ __device__ __forceinline__ void stage_struct(const hemk_tiles &tl,
 }
 
 // One EM map of the tile for the block's replicate chunk, lin -> lout in
-// LDS (and to column qr, the lane's replicate, of the global [q][R] vector
-// gout when given), in two passes
-// separated by barriers:
+// LDS (and to column qlist[r], the replicate of slot r, of the global
+// [q][R] vector gout when given), in two passes separated by barriers.
+// Slot r of an entity's RC slots holds the chunk's r-th replicate; rcl
+// (a power of two, at most RC, at least the chunk's live replicates)
+// lanes serve each entity, so a chunk with few live replicates spreads
+// over more entities at once instead of idling lanes:
 //   class pass       D_c = sum of lam over the members, in member order
 //   transcript pass  lam'_t = (u_t + sum of (lam_t n_c) / D_c over its
 //                    occurrences in class order) / len_t
@@ -95,14 +98,15 @@ template <typename TC, int RC>
 __device__ __forceinline__ void tile_map32(
     const TileCtx &tc, const TileStruct &ts, const double *lin, double *lout,
     double *D, const TC *n, const uint32_t *u, const int *alive,
-    double *gout, uint32_t R, uint32_t qr, int *stopf, double smin,
-    double srel)
+    double *gout, uint32_t R, const uint32_t *qlist, uint32_t rcl,
+    int *stopf, double smin, double srel)
 {
 #pragma clang fp contract(off)
-	const int r = threadIdx.x % RC;
-	const uint32_t e0 = threadIdx.x / RC;
-	constexpr uint32_t ES = TBLOCK / RC;
+	const int r = threadIdx.x & (rcl - 1);
+	const uint32_t e0 = threadIdx.x / rcl;
+	const uint32_t ES = TBLOCK / rcl;
 	const bool live = alive[r] != 0;
+	const uint32_t qr = live ? qlist[r] : 0u;
 	float *rho = (float *)D;
 	if (live) {
 		for (uint32_t c = e0; c < tc.nc; c += ES) {
@@ -161,17 +165,19 @@ __device__ __forceinline__ void tile_map(
     const hemk_tiles &tl, const TileCtx &tc, const TileStruct &ts,
     const double *lin, double *lout, double *D, const TC *n,
     const uint32_t *u, const int *alive, double *gout, uint32_t R,
-    uint32_t qr, int *stopf, double smin, double srel)
+    const uint32_t *qlist, uint32_t rcl, int *stopf, double smin,
+    double srel)
 {
 	if (F) {
 		tile_map32<TC, RC>(tc, ts, lin, lout, D, n, u, alive, gout, R,
-		                   qr, stopf, smin, srel);
+		                   qlist, rcl, stopf, smin, srel);
 		return;
 	}
-	const int r = threadIdx.x % RC;
-	const uint32_t e0 = threadIdx.x / RC;
-	constexpr uint32_t ES = TBLOCK / RC;
+	const int r = threadIdx.x & (rcl - 1);
+	const uint32_t e0 = threadIdx.x / rcl;
+	const uint32_t ES = TBLOCK / rcl;
 	const bool live = alive[r] != 0;
+	const uint32_t qr = live ? qlist[r] : 0u;
 	(void)tl;
 	int out = 0;   // a value outside em_quot's range
 	if (live) {
@@ -356,6 +362,18 @@ __device__ __forceinline__ uint32_t sample_T(const hemk_rctl &rc,
                                              uint32_t smp, uint32_t T)
 {
 	return rc.sinfo ? rc.sinfo[4 * smp + 1] : T;
+}
+
+// Lanes per entity for chunk i of an active list of nrep replicates: the
+// smallest power of two that holds the chunk's replicates, at most RC.
+template <int RC>
+__device__ __forceinline__ uint32_t chunk_lanes(uint32_t nrep, uint32_t i)
+{
+	const uint32_t left = nrep - i * RC;
+	uint32_t l = 1;
+	while (l < RC && l < left)
+		l <<= 1;
+	return l;
 }
 
 // The replicate of this thread's lane in chunk i of an active list.
@@ -571,10 +589,12 @@ __global__ __launch_bounds__(TBLOCK) void k_tile_map12(
 		if (i + 1 < nch)
 			load_rows<TC, RC, 1>(p, tc, rc, al, nrep, i + 1, cnt, uniq,
 			                     A, nullptr, nullptr);
+		const uint32_t *ql = al + 1 + (size_t)i * RC;
+		const uint32_t rcl = chunk_lanes<RC>(nrep, i);
 		tile_map<TC, RC, F>(tl, tc, ts, lam0, lam1, D, n, u, alive, B,
-		                 rc.R, q, nullptr, 0, 0);
+		                    rc.R, ql, rcl, nullptr, 0, 0);
 		tile_map<TC, RC, F>(tl, tc, ts, lam1, lam2, D, n, u, alive, C,
-		                 rc.R, q, nullptr, 0, 0);
+		                    rc.R, ql, rcl, nullptr, 0, 0);
 		rv_parts<TC, RC>(tl, tc, ts, tile, lam0, lam1, lam2, q, xr, xv,
 		                 part, T, step);
 		__syncthreads();
@@ -705,17 +725,20 @@ __global__ __launch_bounds__(TBLOCK, 4) void k_tile_ex3(
 			                     A, B, C);
 			load_alpha<TC>(p, rc, smp, st);
 		}
+		const uint32_t *ql = al + 1 + (size_t)i * RC;
+		const uint32_t rcl = chunk_lanes<RC>(nrep, i);
 		tile_map<TC, RC, F>(tl, tc, ts, lam0, lam1, D, n, u, alive, A,
-		                 rc.R, q, stopf, rc.stop_min, rc.stop_rel);
+		                    rc.R, ql, rcl, stopf, rc.stop_min,
+		                    rc.stop_rel);
 		if (threadIdx.x < RC && rin && stopf[threadIdx.x])
 			rc.notconv[smp * rc.R + q] = 1;
 		if (next) {
 			// the next iteration's u1 -> B, u2 -> C (into lam0, whose
 			// extrapolated values are spent) and its r, v
 			tile_map<TC, RC, F>(tl, tc, ts, lam1, lam2, D, n, u, alive, B,
-			                 rc.R, q, nullptr, 0, 0);
+			                    rc.R, ql, rcl, nullptr, 0, 0);
 			tile_map<TC, RC, F>(tl, tc, ts, lam2, lam0, D, n, u, alive, C,
-			                 rc.R, q, nullptr, 0, 0);
+			                    rc.R, ql, rcl, nullptr, 0, 0);
 			rv_parts<TC, RC>(tl, tc, ts, tile, lam1, lam2, lam0, q, xr,
 			                 xv, part, T, st.step);
 		}

@@ -113,6 +113,8 @@ def main():
 
     all_modes = {
         "fp64-exact": dict(precision=H.PREC_FP64, norm=H.NORM_EXACT),
+        "fp64-exact-win": dict(precision=H.PREC_FP64,
+                               norm=H.NORM_EXACT_WIN),
         "fp64-tree": dict(precision=H.PREC_FP64, norm=H.NORM_TREE),
         "fp32-tree": dict(precision=H.PREC_FP32, norm=H.NORM_TREE),
     }
@@ -195,7 +197,8 @@ def main():
     # the roofline names the dominant memory-streaming kernel; the exact
     # sums' in-order chunk walk is latency-bound by construction
     stream_kinds = [k for k in kinds if k not in ("norm_pass1", "norm_pass2")
-                    or modes[head_name]["norm"] != H.NORM_EXACT]
+                    or modes[head_name]["norm"] not in (H.NORM_EXACT,
+                                                        H.NORM_EXACT_WIN)]
     dom = max(stream_kinds, key=lambda k: prof[k]["ms_total"])
     d = prof[dom]
     # algorithmic bytes: 8Z + 16C + 24T per EM map (u32 indices, FP32
@@ -696,6 +699,7 @@ def bench_bootstrap(args, H, np, p, cls, u, sk):
     # to the iteration cap, since FP32 E-step noise stays above a 1e-8
     # relative change
     allm = (("fp64-exact", dict(norm=H.NORM_EXACT)),
+            ("fp64-exact-win", dict(norm=H.NORM_EXACT_WIN)),
             ("fp64-tree", dict(norm=H.NORM_TREE)),
             ("fp32-exact", dict(norm=H.NORM_EXACT, precision=H.PREC_FP32)))
     bmodes = [(n, m) for n, m in allm if n in args.boot_modes.split(",")]

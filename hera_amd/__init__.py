@@ -25,6 +25,7 @@ CLAMP_DIV10 = 1
 NORM_EXACT = 0
 NORM_TREE = 1
 NORM_SEQ = 2
+NORM_EXACT_WIN = 3
 
 
 @dataclass
@@ -124,6 +125,7 @@ def copy_bandwidth(nbytes=1 << 30, reps=10):
 SUM_PLAIN = 0
 SUM_NORM_SD = 1
 SUM_SQUARES = 2
+SUM_WINDOWS = 16   # or'ed into mode: the window method of the exact sums
 
 
 def exact_sums(rows, mode=SUM_PLAIN):

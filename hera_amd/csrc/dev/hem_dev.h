@@ -501,7 +501,26 @@ typedef struct {
 	uint8_t *nx;           /* [nchain][P][W] links to the next window     */
 	double *res;           /* sums by row                                 */
 	uint32_t *miss;
+	/* Chunk programs (walk_sum.hip).  When wa is set, the finishing step
+	 * describes each chunk by at most L/2 records and walks them in order
+	 * (no drifts, windows or links; W, tab, nx and base unused):
+	 *   wa  [nchain][P][L/2] what the record adds;
+	 *   wlh [nchain][P][L/2] pairs {lo, hi}: the starts it is valid for;
+	 *   wrg [nchain][P][L/2] its values' range in the chunk, beg | end << 11,
+	 *       and HEMK_WS_EVEN when the record needs a start whose last
+	 *       mantissa bit is 0;
+	 *   wcnt [nchain][P] records per chunk, or HEMK_WS_RAW (the chunk is
+	 *       added value by value). */
+	double *wa, *wlh;
+	uint32_t *wrg, *wcnt;
 } hemk_spec;
+#define HEMK_WS_RAW 0xFFFFFFFFu
+#define HEMK_WS_EVEN 0x80000000u
+/* Walk-step records of launches of `chains` chains of P chunks of L. */
+size_t hemk_ws_records(uint32_t L, uint32_t chains, uint32_t P);
+/* The finishing step by chunk programs (called by hemk_spec_sum and
+ * hemk_spec_phases when sp->wa is set). */
+int hemk_ws_finish(const hemk_spec *sp, const hemk_ctl *ctl, void *s);
 /* The steps of hemk_spec_sum by themselves, in this order: chunk sums,
  * drifts, then windows and resolve (produce == 0). */
 #define HEMK_SPEC_PH_CSUM   1

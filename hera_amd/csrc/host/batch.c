@@ -156,6 +156,10 @@ void hem_batch_free(hem_problem *p)
 	hemd_free(w->sbase);
 	hemd_free(w->stab);
 	hemd_free(w->snx);
+	hemd_free(w->swa);
+	hemd_free(w->swlh);
+	hemd_free(w->swrg);
+	hemd_free(w->swcnt);
 	hemd_host_free(w->h_active);
 	memset(w, 0, sizeof(*w));
 	for (int i = 0; i < HEM_TILE_ARRAYS; ++i) {
@@ -219,6 +223,15 @@ static int ws_ensure(hem_problem *p, uint32_t R)
 		WALLOC(w->stab, ent * 8);
 		WALLOC(w->snx, ent);
 	}
+	{
+		/* chunk programs of the r/v pairs (the final sums' R rows use
+		 * the first R chains) */
+		const size_t nrec = hemk_ws_records(p->spec_L, 2 * R, p->spec_P);
+		WALLOC(w->swa, nrec * 8);
+		WALLOC(w->swlh, nrec * 16);
+		WALLOC(w->swrg, nrec * 4);
+		WALLOC(w->swcnt, (size_t)2 * R * p->spec_P * 4);
+	}
 #undef WALLOC
 	if (hemd_host_alloc((void **)&w->h_active,
 	                    (size_t)(p->ns > 16 ? p->ns : 16) * 4))
@@ -280,7 +293,8 @@ static hemk_rctl rctl_for(hem_problem *p, uint32_t R, const hem_opts *o)
 /* Exact-sum chains over the batch's rows.  pairs: r and v rows of the
  * replicates in alist (X[rep] and X[R + rep]); otherwise one row per
  * replicate (every replicate, no alist). */
-static hemk_spec spec_batch(hem_problem *p, hemk_rctl rc, int pairs, int xf,
+static hemk_spec spec_batch(hem_problem *p, const hem_opts *o,
+                            hemk_rctl rc, int pairs, int xf,
                             const double *mres, double *res)
 {
 	hem_batch_ws *w = &p->bw;
@@ -306,6 +320,12 @@ static hemk_spec spec_batch(hem_problem *p, hemk_rctl rc, int pairs, int xf,
 	sp.nx = w->snx;
 	sp.res = res;
 	sp.miss = reruns_word(p);
+	if (o->norm == HEM_NORM_EXACT) {
+		sp.wa = w->swa;
+		sp.wlh = w->swlh;
+		sp.wrg = w->swrg;
+		sp.wcnt = w->swcnt;
+	}
 	return sp;
 }
 
@@ -358,7 +378,7 @@ static int batch_iteration(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 	/* exact sums by speculation for one sample; several samples (rows of
 	 * different lengths) use the single-lane chains, with the same
 	 * results */
-	const int spec = o->norm == HEM_NORM_EXACT && p->ns == 1;
+	const int spec = hem_norm_exact(o->norm) && p->ns == 1;
 	const int exact = o->norm != HEM_NORM_TREE;
 	/* exact: alpha from the sums inside ex3; tree: alpha given */
 	hemk_step st = {o->step, o->clamp, o->alpha_clamp != 0, !exact};
@@ -370,14 +390,14 @@ static int batch_iteration(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 		 * then the centred sums of the sd form into q_r/q_v */
 		const int sd = o->step == HEM_STEP_SD;
 		double *sums1 = pass1_sums(p);
-		hemk_spec sp = spec_batch(p, rc, 1,
+		hemk_spec sp = spec_batch(p, o, rc, 1,
 		                          sd ? HEMK_SPEC_IDENT : HEMK_SPEC_SQUARE,
 		                          NULL, sd ? sums1 : rc.q_r);
 		HEM_TRY_DEV(hemk_spec_sum(&sp, 0, NULL, NULL, NULL, NULL, s),
 		            "batch exact sums 1");
 		BMARK(HEM_BK_NORM1);
 		if (sd) {
-			sp = spec_batch(p, rc, 1, HEMK_SPEC_CENTRED, sums1,
+			sp = spec_batch(p, o, rc, 1, HEMK_SPEC_CENTRED, sums1,
 			                rc.q_r);
 			HEM_TRY_DEV(hemk_spec_sum(&sp, 0, NULL, NULL, NULL, NULL,
 			                          s),
@@ -474,7 +494,7 @@ static int check_batch_opts(const hem_problem *p, const hem_opts *o)
 	if (o->clamp != HEM_CLAMP_ZERO && o->clamp != HEM_CLAMP_DIV10)
 		return hem_fail(HEM_ERR_INVALID, "unknown clamp form %d",
 		                o->clamp);
-	if (o->norm != HEM_NORM_EXACT && o->norm != HEM_NORM_TREE &&
+	if (!hem_norm_exact(o->norm) && o->norm != HEM_NORM_TREE &&
 	    o->norm != HEM_NORM_SEQ)
 		return hem_fail(HEM_ERR_INVALID, "unknown norm mode %d", o->norm);
 	if (o->fallback)
@@ -566,7 +586,7 @@ static int run_batch(hem_problem *p, const hem_opts *o, uint32_t R,
 		                             p->stream),
 		            "final sums");
 	} else {
-		hemk_spec sp = spec_batch(p, rc, 0, HEMK_SPEC_IDENT, NULL,
+		hemk_spec sp = spec_batch(p, o, rc, 0, HEMK_SPEC_IDENT, NULL,
 		                          rc.lam_sum);
 		HEM_TRY_DEV(hemk_spec_sum(&sp, 0, NULL, NULL, NULL, NULL,
 		                          p->stream),

@@ -42,6 +42,8 @@ typedef struct {
 	long long *sbase;              /* [2R][P] window bases              */
 	double *stab;                  /* [rows][P][W] candidates           */
 	uint8_t *snx;                  /* [rows][P][W] window links         */
+	double *swa, *swlh;           /* [2R][P][L/2] chunk programs       */
+	uint32_t *swrg, *swcnt;
 	uint32_t *h_active;            /* pinned                            */
 	/* captured iterations; the loop bounds and stop thresholds are kernel
 	 * arguments, so a graph is reused only for identical options */
@@ -111,6 +113,9 @@ struct hem_problem {
 	double *d_scsq;             /* pass-1 chunk sums of x * x         */
 	long long *d_sbase;
 	uint8_t *d_snx;
+	/* chunk programs of the exact sums (two chains) */
+	double *d_swa, *d_swlh;
+	uint32_t *d_swrg, *d_swcnt;
 	double *d_cls_w, *d_tx_w;  /* member weights (weighted tables)   */
 	int weighted;
 	hemk_ctl *d_ctl;
@@ -217,6 +222,12 @@ int hem_tiles_build(uint32_t T, uint32_t C, const uint32_t *cls_off,
                     uint32_t nnz_target, hem_tiles_host *h);
 void hem_tiles_free_host(hem_tiles_host *h);
 /* Bytes of a tile's record (see tiles.c), a multiple of 16. */
+/* Either method of the bitwise left-to-right sums. */
+static inline int hem_norm_exact(int norm)
+{
+	return norm == HEM_NORM_EXACT || norm == HEM_NORM_EXACT_WIN;
+}
+
 static inline uint64_t hem_tile_rec_bytes(uint32_t nt, uint32_t nc,
                                           uint32_t nz, int weighted)
 {

@@ -358,6 +358,13 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 	DALLOC_H(p->d_sbase, 2 * (size_t)p->spec_P * 8);
 	DALLOC_H(p->d_stab, 2 * (size_t)p->spec_P * HEMK_SPEC_W * 8);
 	DALLOC_H(p->d_snx, 2 * (size_t)p->spec_P * HEMK_SPEC_W);
+	{
+		const size_t nrec = hemk_ws_records(p->spec_L, 2, p->spec_P);
+		DALLOC_H(p->d_swa, nrec * 8);
+		DALLOC_H(p->d_swlh, nrec * 16);
+		DALLOC_H(p->d_swrg, nrec * 4);
+		DALLOC_H(p->d_swcnt, 2 * (size_t)p->spec_P * 4);
+	}
 	DALLOC_H(p->d_sres, 4 * 8);
 	DALLOC_H(p->d_ctl, sizeof(hemk_ctl));
 #undef DALLOC_H
@@ -660,6 +667,10 @@ int hem_problem_destroy(hem_problem *p)
 	hemd_free(p->d_sbase);
 	hemd_free(p->d_stab);
 	hemd_free(p->d_snx);
+	hemd_free(p->d_swa);
+	hemd_free(p->d_swlh);
+	hemd_free(p->d_swrg);
+	hemd_free(p->d_swcnt);
 	hemd_free(p->d_sres);
 	hemd_free(p->d_fbpart);
 	hemd_free(p->d_tpart);

@@ -53,7 +53,7 @@ static int check_opts(const hem_opts *o)
 	if (o->clamp != HEM_CLAMP_ZERO && o->clamp != HEM_CLAMP_DIV10)
 		return hem_fail(HEM_ERR_INVALID, "unknown clamp form %d",
 		                o->clamp);
-	if (o->norm != HEM_NORM_EXACT && o->norm != HEM_NORM_TREE &&
+	if (!hem_norm_exact(o->norm) && o->norm != HEM_NORM_TREE &&
 	    o->norm != HEM_NORM_SEQ)
 		return hem_fail(HEM_ERR_INVALID, "unknown norm mode %d", o->norm);
 	if (o->fallback != 0 && o->fallback != 1)
@@ -88,9 +88,9 @@ typedef struct {
 
 /* The exact-sum chains of the plain solve: rows of x (r/v rows, or the
  * state for the finalisation sum), results into res[row]. */
-static hemk_spec spec_plain(const hem_problem *p, uint32_t nchain, int xf,
-                            const double *x, const double *mres,
-                            double *res)
+static hemk_spec spec_plain(const hem_problem *p, const hem_opts *o,
+                            uint32_t nchain, int xf, const double *x,
+                            const double *mres, double *res)
 {
 	hemk_spec sp;
 	memset(&sp, 0, sizeof(sp));
@@ -111,6 +111,12 @@ static hemk_spec spec_plain(const hem_problem *p, uint32_t nchain, int xf,
 	sp.nx = p->d_snx;
 	sp.res = res;
 	sp.miss = &p->d_ctl->reruns;
+	if (o->norm == HEM_NORM_EXACT) {
+		sp.wa = p->d_swa;
+		sp.wlh = p->d_swlh;
+		sp.wrg = p->d_swrg;
+		sp.wcnt = p->d_swcnt;
+	}
 	return sp;
 }
 
@@ -127,7 +133,7 @@ static int use_tiles(const hem_problem *p, const hem_opts *o)
 		lds = (lds + 7) / 8 * 8 + 8 * (size_t)p->dtp.max_cls;
 	return p->dtp.n_tiles &&
 	       (!f32 || (!p->weighted && !o->fallback)) &&
-	       (o->norm == HEM_NORM_EXACT || o->norm == HEM_NORM_TREE) &&
+	       (hem_norm_exact(o->norm) || o->norm == HEM_NORM_TREE) &&
 	       lds <= HEM_PTILE_LDS_MAX;
 }
 
@@ -136,7 +142,7 @@ static int use_tiles(const hem_problem *p, const hem_opts *o)
  * kernel that ends the iteration before it. */
 static int enqueue_tiles_prologue(hem_problem *p, const hem_opts *o)
 {
-	const int exact = o->norm == HEM_NORM_EXACT;
+	const int exact = hem_norm_exact(o->norm);
 	const uint32_t T = p->n_tx;
 	HEM_TRY_DEV(hemk_ptile_map12(&p->dtp, p->d_A, p->d_B, p->d_C, p->d_uq,
 	                             p->d_cntq, exact ? p->d_sx : NULL,
@@ -162,7 +168,7 @@ static int enqueue_tiles_prologue(hem_problem *p, const hem_opts *o)
 static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
                                    prof_ctx *pc)
 {
-	const int exact = o->norm == HEM_NORM_EXACT;
+	const int exact = hem_norm_exact(o->norm);
 	const int sd = o->step == HEM_STEP_SD;
 	const uint32_t T = p->n_tx;
 	hemk_step st = {o->step, o->clamp, o->alpha_clamp != 0, exact ? 2 : 1};
@@ -187,7 +193,7 @@ static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
 		 * with the sd form pass 1 also keeps each chunk's sum of
 		 * squares, from which pass 2 (centred sums of squares) seeds
 		 * its guesses without a chunk-sums step of its own */
-		hemk_spec sp = spec_plain(p, 2,
+		hemk_spec sp = spec_plain(p, o, 2,
 		                          sd ? HEMK_SPEC_IDENT : HEMK_SPEC_SQUARE,
 		                          p->d_sx, NULL, p->d_sres);
 		if (sd)
@@ -197,7 +203,7 @@ static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
 		MARK(HEM_PK_NORM1);
 		alpha_part = p->d_sres;
 		if (sd) {
-			sp = spec_plain(p, 2, HEMK_SPEC_CENTRED, p->d_sx,
+			sp = spec_plain(p, o, 2, HEMK_SPEC_CENTRED, p->d_sx,
 			                p->d_sres, p->d_sres + 2);
 			sp.gsum = p->d_scsum;
 			sp.gsq = p->d_scsq;
@@ -269,7 +275,7 @@ static int enqueue_iteration(hem_problem *p, const hem_opts *o, prof_ctx *pc)
 	const void *gB = f32 ? (const void *)p->d_B32 : (const void *)p->d_B;
 	const void *gD = f32 ? (const void *)p->d_D32 : (const void *)p->d_D;
 	void *rho = f32 ? p->d_rho : (void *)p->d_nd;
-	const int exact = o->norm == HEM_NORM_EXACT;
+	const int exact = hem_norm_exact(o->norm);
 	const int seq = o->norm == HEM_NORM_SEQ;
 	hemk_step st = {o->step, o->clamp, o->alpha_clamp != 0,
 	                seq ? 1 : exact ? 2 : 0};
@@ -300,7 +306,7 @@ static int enqueue_iteration(hem_problem *p, const hem_opts *o, prof_ctx *pc)
 		 * the centred sums of the sd form; the extrapolation takes
 		 * alpha from the last pair */
 		const int sd = o->step == HEM_STEP_SD;
-		hemk_spec sp = spec_plain(p, 2,
+		hemk_spec sp = spec_plain(p, o, 2,
 		                          sd ? HEMK_SPEC_IDENT : HEMK_SPEC_SQUARE,
 		                          p->d_sx, NULL, p->d_sres);
 		HEM_TRY_DEV(hemk_spec_sum(&sp, 1, p->d_A, p->d_B, p->d_C,
@@ -309,7 +315,7 @@ static int enqueue_iteration(hem_problem *p, const hem_opts *o, prof_ctx *pc)
 		MARK(HEM_PK_NORM1);
 		alpha_part = p->d_sres;
 		if (sd) {
-			sp = spec_plain(p, 2, HEMK_SPEC_CENTRED, p->d_sx,
+			sp = spec_plain(p, o, 2, HEMK_SPEC_CENTRED, p->d_sx,
 			                p->d_sres, p->d_sres + 2);
 			HEM_TRY_DEV(hemk_spec_sum(&sp, 0, NULL, NULL, NULL,
 			                          p->d_ctl, s),
@@ -516,9 +522,9 @@ int hem_solve(hem_problem *p, const uint32_t *unique, const hem_opts *opts,
 		lam = p->d_D;
 	}
 	if (rc) {
-	} else if (o.norm == HEM_NORM_EXACT) {
+	} else if (hem_norm_exact(o.norm)) {
 		/* sum of the final state (src/EM.c:362-363) into lam_sum */
-		hemk_spec sp = spec_plain(p, 1, HEMK_SPEC_IDENT, lam, NULL,
+		hemk_spec sp = spec_plain(p, &o, 1, HEMK_SPEC_IDENT, lam, NULL,
 		                          &p->d_ctl->lam_sum);
 		rc = hemk_spec_sum(&sp, 0, NULL, NULL, NULL, NULL, p->stream);
 		if (!rc)
@@ -637,7 +643,7 @@ int hem_profile_solve(hem_problem *p, const uint32_t *unique,
 	memset(out, 0, sizeof(*out));
 	memset(&pc, 0, sizeof(pc));
 	kernel_bytes(p, o.precision == HEM_PREC_FP32,
-	             o.norm == HEM_NORM_EXACT, out->bytes);
+	             hem_norm_exact(o.norm), out->bytes);
 	for (i = 0; i < 16 && !rc; ++i)
 		rc = hemd_event_create(&pc.ev[i]);
 	if (!rc)
@@ -735,7 +741,11 @@ int hem_exact_sums(const double *x, uint32_t n, uint32_t n_rows, int mode,
 	uint8_t *d_nx = NULL;
 	long long *d_base = NULL;
 	uint32_t *d_miss = NULL, miss = 0;
+	double *d_wa = NULL, *d_wlh = NULL;
+	uint32_t *d_wrg = NULL, *d_wcnt = NULL;
 	hemk_spec sp;
+	const int windows = (mode & HEM_SUM_WINDOWS) != 0;
+	mode &= ~HEM_SUM_WINDOWS;
 	if (!x || !out || n == 0 || n_rows == 0)
 		return hem_fail(HEM_ERR_INVALID, "null rows or empty size");
 	if (mode < HEM_SUM_PLAIN || mode > HEM_SUM_SQUARES)
@@ -761,6 +771,13 @@ int hem_exact_sums(const double *x, uint32_t n, uint32_t n_rows, int mode,
 	if (!e) e = hemd_malloc((void **)&d_nx, cp * HEMK_SPEC_W);
 	if (!e) e = hemd_malloc((void **)&d_res, (size_t)n_rows * 2 * 8);
 	if (!e) e = hemd_malloc((void **)&d_miss, 4);
+	if (!windows) {
+		const size_t nrec = hemk_ws_records(sp.L, n_rows, sp.P);
+		if (!e) e = hemd_malloc((void **)&d_wa, nrec * 8);
+		if (!e) e = hemd_malloc((void **)&d_wlh, nrec * 16);
+		if (!e) e = hemd_malloc((void **)&d_wrg, nrec * 4);
+		if (!e) e = hemd_malloc((void **)&d_wcnt, cp * 4);
+	}
 	if (e) {
 		rc = hem_fail(HEM_ERR_NOMEM, "device allocation: %s",
 		              hemd_error_string(e));
@@ -774,6 +791,10 @@ int hem_exact_sums(const double *x, uint32_t n, uint32_t n_rows, int mode,
 	sp.nx = d_nx;
 	sp.miss = d_miss;
 	sp.res = d_res;
+	sp.wa = d_wa;
+	sp.wlh = d_wlh;
+	sp.wrg = d_wrg;
+	sp.wcnt = d_wcnt;
 	sp.xf = mode == HEM_SUM_SQUARES ? HEMK_SPEC_SQUARE : HEMK_SPEC_IDENT;
 	sp.W = hemk_spec_width(sp.L, sp.xf, sp.nchain * sp.P);
 	e = hemd_h2d(d_x, x, (size_t)n_rows * n * 8, s);
@@ -810,6 +831,10 @@ out:
 	hemd_free(d_nx);
 	hemd_free(d_res);
 	hemd_free(d_miss);
+	hemd_free(d_wa);
+	hemd_free(d_wlh);
+	hemd_free(d_wrg);
+	hemd_free(d_wcnt);
 	if (s)
 		hemd_stream_destroy(s);
 	return rc;

@@ -80,15 +80,20 @@ typedef struct {
 
 /* SQUAREM norm reductions.  EXACT: the left-to-right sums of Norm
  * (src/hash_align.c:150-164) and of the finalisation (src/EM.c:362-363),
- * bitwise, computed in parallel chunks from speculated start values with
- * an in-order lookup (a chunk whose start falls outside its window is
- * added again from the exact start), so with HEM_PREC_FP64 every iterate
- * is bitwise the reference's.  SEQ: the same sums, each run by one device
- * lane (slow; kept as the plain restatement).  TREE: fixed-shape parallel
- * sums (bitwise reproducible run to run, ~1 ulp from the reference's). */
-#define HEM_NORM_EXACT 0
-#define HEM_NORM_TREE  1
-#define HEM_NORM_SEQ   2
+ * bitwise, so with HEM_PREC_FP64 every iterate is bitwise the reference's:
+ * each chunk of values becomes a short program (runs whose partial sums
+ * stay in one binade, as exact integer sums on its grid, and the other
+ * values one by one) and one wave walks the programs in order.  EXACT_WIN:
+ * the same bits by an earlier method (chunks run from windows of
+ * speculated start values, an in-order lookup, and a chunk whose start
+ * falls outside its window added again from the exact start).  SEQ: the
+ * same sums, each run by one device lane (slow; kept as the plain
+ * restatement).  TREE: fixed-shape parallel sums (bitwise reproducible run
+ * to run, ~1 ulp from the reference's). */
+#define HEM_NORM_EXACT     0
+#define HEM_NORM_TREE      1
+#define HEM_NORM_SEQ       2
+#define HEM_NORM_EXACT_WIN 3
 
 /* Treatment of extrapolated values.  ZERO: negatives become 0
  * (src/EM.c:351).  DIV10: values <= 1e-300 become u2/10
@@ -414,12 +419,16 @@ int hem_copy_bandwidth(size_t bytes, int reps, double *gb_per_s);
  * ((+0 + x_0) + x_1) + ... in index order (src/EM.c:362-363); HEM_SUM_NORM_SD
  * out[r] = Norm(x_r) (src/hash_align.c:150-164: mean, centred sum of
  * squares, / (n-1), sqrt); HEM_SUM_SQUARES out[r] = the same ordered sum of
- * x_i * x_i.  Computed in parallel chunks (see HEM_NORM_EXACT); reruns, if
- * not NULL, receives the number of chunks added again from their exact
- * start.  No counterpart in the reference as an entry point. */
+ * x_i * x_i.  Computed in parallel chunks by chunk programs (see
+ * HEM_NORM_EXACT), or, with HEM_SUM_WINDOWS or'ed into mode, from windows
+ * of speculated starts (HEM_NORM_EXACT_WIN); reruns, if not NULL, receives
+ * the number of chunks (or program records) added again value by value
+ * from their exact start.  No counterpart in the reference as an entry
+ * point. */
 #define HEM_SUM_PLAIN   0
 #define HEM_SUM_NORM_SD 1
 #define HEM_SUM_SQUARES 2
+#define HEM_SUM_WINDOWS 16  /* or'ed into mode: the EXACT_WIN method */
 int hem_exact_sums(const double *x, uint32_t n, uint32_t n_rows, int mode,
                    double *out, uint32_t *reruns);
 
@@ -434,7 +443,7 @@ int hem_exact_sums(const double *x, uint32_t n, uint32_t n_rows, int mode,
  * above, must not exceed 4096 (HEM_ERR_RANGE otherwise).  Calls on one
  * device share that array, so calls that overlap on a device race: the
  * caller orders them (one stream, or events).  Results are bitwise those
- * of hem_exact_sums. */
+ * of hem_exact_sums (the window method, as with HEM_SUM_WINDOWS). */
 int hem_exact_sums_noalloc(const double *x, uint32_t n, uint32_t n_rows,
                            int mode, double *out, uint32_t *reruns,
                            void *stream);

@@ -1,13 +1,17 @@
 """Exact left-to-right sums on the device (hem_exact_sums; the reductions of
 Norm, src/hash_align.c:150-164, and of the finalisation, src/EM.c:362-363).
 
-The device computes them in parallel chunks from speculated start values
-with an in-order lookup; the bar is bitwise equality with the sequential
-loop, (((+0 + x0) + x1) + ...), for every input, including inputs built to
-defeat the speculation (partial sums that return to zero, huge dynamic
-range, denormals, signed zeros, infinities).  The expected values come
-from numpy's sequential cumulative sum (np.add.accumulate adds strictly left
-to right) and from the oracle's restatement of Norm.
+The device computes them in parallel by two methods: chunk programs (runs
+whose partial sums stay in one binade, summed exactly on its grid, the
+other values one by one) walked in order, and chunks run from windows of
+speculated start values with an in-order lookup (HEM_SUM_WINDOWS).  The bar
+for both is bitwise equality with the sequential loop, (((+0 + x0) + x1) +
+...), for every input, including inputs built to defeat either method
+(partial sums that return to zero or hover on a power of two, round-half
+ties on every step, huge dynamic range, denormals, signed zeros,
+infinities).  The expected values come from numpy's sequential cumulative
+sum (np.add.accumulate adds strictly left to right) and from the oracle's
+restatement of Norm.
 """
 import sys
 
@@ -63,18 +67,47 @@ def cases():
     inf[1, 3000] = -np.inf
     inf[2, 2500] = np.nan
     out["nonfinite"] = inf
+    # every step a round-half tie: from 2^52 (ulp 1) add halves and odd
+    # halves, so each result depends on the parity of the running sum
+    ties = rng.integers(0, 4, size=(2, 30000)) + 0.5
+    ties[:, 0] = 2.0 ** 52
+    ties[1, 1::2] *= -1.0
+    out["ties"] = ties
+    # partial sums hovering on 1.0, a binade boundary, by steps of a few
+    # ulps of 1, then far below it
+    hov = rng.integers(-3, 4, size=(2, 50000)) * 2.0 ** -53
+    hov[:, 0] = 1.0
+    hov[1, 25000:] *= 2.0 ** 20
+    out["boundary_hover"] = hov
+    # partial sums near the top of the double range and near the bottom of
+    # the normal range
+    out["huge"] = rng.normal(size=(2, 20000)) * 2.0 ** 1010
+    out["tiny_normal"] = rng.normal(size=(2, 20000)) * 2.0 ** -1000
+    # chunk sums that miss the sequential sum by 1 (1e30 + 1 - 1e30 is 0
+    # left to right), so every later chunk's start is guessed 1 too high
+    # against running sums of order 1e-3: the guessed runs do not hold for
+    # the true starts and the walk adds those values one by one
+    mis = rng.normal(size=(2, 50000)) * 1e-3
+    mis[:, :3] = [1e30, 1.0, -1e30]
+    mis[1, 20000:20003] = [1e30, 1.0, -1e30]
+    out["misguided"] = mis
     return out
 
 
+ENGINES = {"programs": 0, "windows": H.SUM_WINDOWS}
+
+
+@pytest.mark.parametrize("engine", sorted(ENGINES))
 @pytest.mark.parametrize("name", sorted(cases().keys()))
-def test_exact_sums_bitwise(name):
+def test_exact_sums_bitwise(name, engine):
     x = cases()[name]
-    got, reruns = H.exact_sums(x, H.SUM_PLAIN)
+    e = ENGINES[engine]
+    got, reruns = H.exact_sums(x, H.SUM_PLAIN | e)
     exp = np.array([seq_sum(r) for r in x])
-    sq, _ = H.exact_sums(x, H.SUM_SQUARES)
+    sq, _ = H.exact_sums(x, H.SUM_SQUARES | e)
     exp_sq = np.array([seq_sum(r * r) for r in x])
-    say("%-16s n=%6d rows=%d reruns=%d" % (name, x.shape[1], x.shape[0],
-                                          reruns))
+    say("%-16s %-8s n=%6d rows=%d reruns=%d" % (
+        name, engine, x.shape[1], x.shape[0], reruns))
     assert same_bits(got, exp), (got, exp)
     assert same_bits(sq, exp_sq), (sq, exp_sq)
 
@@ -103,9 +136,9 @@ def test_exact_sums_bitwise_narrow_windows(name):
     scale = 2.0 ** (np.arange(reps) % 5 - 2)
     xs = np.concatenate([x * s for s in scale])
     assert launch_chunks(xs.shape[0], xs.shape[1]) > 512
-    got, reruns = H.exact_sums(xs, H.SUM_PLAIN)
+    got, reruns = H.exact_sums(xs, H.SUM_PLAIN | H.SUM_WINDOWS)
     exp = np.array([seq_sum(r) for r in xs])
-    sq, _ = H.exact_sums(xs, H.SUM_SQUARES)
+    sq, _ = H.exact_sums(xs, H.SUM_SQUARES | H.SUM_WINDOWS)
     exp_sq = np.array([seq_sum(r * r) for r in xs])
     say("%-16s n=%6d rows=%d reruns=%d (64-wide)" % (
         name, xs.shape[1], xs.shape[0], reruns))
@@ -113,12 +146,14 @@ def test_exact_sums_bitwise_narrow_windows(name):
     assert same_bits(sq, exp_sq)
 
 
+@pytest.mark.parametrize("engine", sorted(ENGINES))
 @pytest.mark.parametrize("name", ["normal_1025", "normal_200000",
                                   "lognormal_wide", "return_to_zero",
-                                  "alternating", "denormal"])
-def test_norm_sd_bitwise_oracle(name):
+                                  "alternating", "denormal", "ties",
+                                  "boundary_hover"])
+def test_norm_sd_bitwise_oracle(name, engine):
     x = cases()[name]
-    got, _ = H.exact_sums(x, H.SUM_NORM_SD)
+    got, _ = H.exact_sums(x, H.SUM_NORM_SD | ENGINES[engine])
     exp = np.array([O.norm_sd(r) for r in x])
     assert same_bits(got, exp), (got, exp)
 
@@ -136,26 +171,30 @@ def test_norm_sd_on_squarem_vectors(cfg1):
         r = u1 - lam
         rows += [r, (u2 - u1) - r]
     x = np.stack(rows)
-    got, reruns = H.exact_sums(x, H.SUM_NORM_SD)
     exp = np.array([O.norm_sd(r) for r in x])
-    say("squarem vectors: reruns=%d over %d rows" % (reruns, len(rows)))
-    assert same_bits(got, exp)
+    for engine, e in sorted(ENGINES.items()):
+        got, reruns = H.exact_sums(x, H.SUM_NORM_SD | e)
+        say("squarem vectors, %s: reruns=%d over %d rows" % (
+            engine, reruns, len(rows)))
+        assert same_bits(got, exp)
 
 
 def test_solve_exact_equals_single_lane(cfg1):
-    """The speculated sums and the single-lane chains give the same
-    solve, bit for bit, under both step forms."""
+    """Both methods of the parallel exact sums and the single-lane chains
+    give the same solve, bit for bit, under both step forms."""
     cls, u = cfg1
     p = H.Problem(cls)
     try:
         for step in (H.STEP_SD, H.STEP_L2):
-            a = p.solve(u, norm=H.NORM_EXACT, step=step, max_iter=120)
             b = p.solve(u, norm=H.NORM_SEQ, step=step, max_iter=120)
-            say("step=%d k=%d reruns=%d" % (step, a["stats"]["iters"],
-                                             a["stats"]["sum_reruns"]))
-            assert a["stats"]["iters"] == b["stats"]["iters"]
-            np.testing.assert_array_equal(a["lam"], b["lam"])
-            np.testing.assert_array_equal(a["tpm"], b["tpm"])
+            for norm in (H.NORM_EXACT, H.NORM_EXACT_WIN):
+                a = p.solve(u, norm=norm, step=step, max_iter=120)
+                say("step=%d norm=%d k=%d reruns=%d" % (
+                    step, norm, a["stats"]["iters"],
+                    a["stats"]["sum_reruns"]))
+                assert a["stats"]["iters"] == b["stats"]["iters"]
+                np.testing.assert_array_equal(a["lam"], b["lam"])
+                np.testing.assert_array_equal(a["tpm"], b["tpm"])
     finally:
         p.close()
 

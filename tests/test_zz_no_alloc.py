@@ -1,7 +1,9 @@
 """hem_exact_sums_noalloc: the exact left-to-right sums with the library's
 fixed device scratch (no allocation), called through the C ABI on device
-rows; every result bitwise hem_exact_sums's, on two calls in a row (the
-second smaller, so stale scratch from the first would show).
+rows; every result bitwise hem_exact_sums's (by the same method, windows of
+speculated starts, with the same reruns, and by the default method, chunk
+programs), on two calls in a row (the second smaller, so stale scratch from
+the first would show).
 
 Device memory and streams for the rows come from the HIP runtime the
 library itself is linked against (found among the process's mappings), so
@@ -113,12 +115,18 @@ def test_noalloc_sums_equal_the_allocating_entry_point():
             # a stream of its own
             for x, stream in ((rows_big(), None), (rows_small(), side)):
                 got, rr = noalloc(rt, x, mode, stream)
-                exp, rr_exp = H.exact_sums(x, mode)
+                # the same method (windows of speculated starts) on
+                # allocated scratch: same bits, same chunks run again
+                exp, rr_exp = H.exact_sums(x, mode | H.SUM_WINDOWS)
                 say("mode %d rows %d n %d reruns %d/%d" % (
                     mode, x.shape[0], x.shape[1], rr, rr_exp))
                 np.testing.assert_array_equal(got.view(np.int64),
                                               exp.view(np.int64))
                 assert rr == rr_exp
+                # and the default method (chunk programs): same bits
+                prog, _ = H.exact_sums(x, mode)
+                np.testing.assert_array_equal(got.view(np.int64),
+                                              prog.view(np.int64))
     finally:
         rt.hipStreamDestroy(side)
 

@@ -435,18 +435,7 @@ __global__ void k_iter_end(hemk_ctl *__restrict__ ctl)
 {
 	if (ctl->done)
 		return;
-	const uint32_t any = ctl->stopbad;
-	ctl->stopbad = 0;
-	{
-		const uint32_t k = ctl->k + 1;
-		const uint32_t stop = any ? 0u : 1u;
-		ctl->k = k;
-		ctl->maps += 3;
-		if (!(k < ctl->min_iter || (k < ctl->max_iter && !stop))) {
-			ctl->done = 1;
-			ctl->converged = stop;
-		}
-	}
+	em_iter_end(ctl);
 }
 
 __global__ void k_ctl_reset(hemk_ctl *ctl, uint32_t min_iter,

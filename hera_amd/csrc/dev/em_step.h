@@ -8,6 +8,22 @@
 
 #include "hem_dev.h"
 
+// The end of one SQUAREM iteration (src/EM.c:328 and 357): k++, the loop
+// condition from the stop test of its last map (ctl->stopbad, cleared for
+// the next iteration), done set when the loop ends.  One thread.
+__device__ __forceinline__ void em_iter_end(hemk_ctl *ctl)
+{
+	const uint32_t stop = ctl->stopbad ? 0u : 1u;
+	const uint32_t k = ctl->k + 1;
+	ctl->stopbad = 0;
+	ctl->k = k;
+	ctl->maps += 3;
+	if (!(k < ctl->min_iter || (k < ctl->max_iter && !stop))) {
+		ctl->done = 1;
+		ctl->converged = stop;
+	}
+}
+
 // Step length from the two reductions (src/EM.c:338-346): sample sd (A, B
 // = centred sums of squares, src/hash_align.c:158-163) or L2 (A, B = sums
 // of squares, hera-EM/utility.c:16-23).

@@ -513,6 +513,10 @@ typedef struct {
 	 *       added value by value). */
 	double *wa, *wlh;
 	uint32_t *wrg, *wcnt;
+	/* chunk programs: when set, the walk's workgroup 0 ends the previous
+	 * SQUAREM iteration on this control block (as hemk_iter_end) once its
+	 * sum is written */
+	hemk_ctl *iend;
 } hemk_spec;
 #define HEMK_WS_RAW 0xFFFFFFFFu
 #define HEMK_WS_EVEN 0x80000000u

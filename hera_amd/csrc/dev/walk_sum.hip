@@ -38,6 +38,7 @@
 
 #include <climits>
 
+#include "em_step.h"
 #include "hem_dev.h"
 #include "spec_common.h"
 
@@ -151,7 +152,7 @@ struct seg_rng {
 };
 
 struct sum_abs {
-	double s, a;
+	double s, a, q;   // sum, sum of magnitudes, sum of squares
 };
 
 // Program of chunk j of each chain: one workgroup of WS_T threads per
@@ -250,23 +251,28 @@ __global__ __launch_bounds__(WS_T) void k_ws_prog(hemk_spec sp,
 	double prev;
 	double tau;
 	{
-		double tl = 0.0, ta = 0.0;
+		double tl = 0.0, ta = 0.0, tq = 0.0;
 #pragma unroll
 		for (int k = 0; k < WS_E; ++k) {
 			tl += y[k];
 			a[k] = tl;
 			ta += fabs(y[k]);
+			tq += y[k] * y[k];
 		}
 		const auto pad = [](sum_abs p, sum_abs q) {
 #pragma clang fp contract(off)
 			sum_abs r;
 			r.s = p.s + q.s;
 			r.a = p.a + q.a;
+			r.q = p.q + q.q;
 			return r;
 		};
 		sum_abs tt;
-		const sum_abs ex = block_excl(sum_abs{tl, ta}, sum_abs{0.0, 0.0},
-		                              pad, w_t, tt);
+		const sum_abs ex = block_excl(sum_abs{tl, ta, tq},
+		                              sum_abs{0.0, 0.0, 0.0}, pad, w_t, tt);
+		// the chunk's sum of squares, for the guesses of a centred pass
+		if (XF == HEMK_SPEC_IDENT && sp.csq && t == 0)
+			sp.csq[cb + j] = tt.q;
 		prev = g + ex.s;
 #pragma unroll
 		for (int k = 0; k < WS_E; ++k)
@@ -786,6 +792,10 @@ __global__ __launch_bounds__(SBLOCK) void k_ws_walk(hemk_spec sp,
 		sp.res[row] = s;
 		if (reruns && sp.miss)
 			atomicAdd(sp.miss, reruns);
+		// the previous iteration's end, off the longer walk's path: the
+		// kernels after this one read its done flag
+		if (sp.iend && blockIdx.x == 0)
+			em_iter_end(sp.iend);
 	}
 }
 

@@ -164,9 +164,14 @@ static int enqueue_tiles_prologue(hem_problem *p, const hem_opts *o)
 
 /* One SQUAREM iteration over the tiles, its u1 and u2 already in B and C:
  * the norms, then one kernel that extrapolates, maps with the stop test
- * and maps twice more for the next iteration. */
+ * and maps twice more for the next iteration.  pos: the iteration's place
+ * in a captured graph (-1: enqueued by itself).  With chunk programs the
+ * iteration end of graph iterations but the last is done by the next
+ * iteration's first walk (its kernels before that run regardless and are
+ * never read once the loop has ended); the last one, and every iteration
+ * enqueued by itself, ends with its own kernel. */
 static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
-                                   prof_ctx *pc)
+                                   prof_ctx *pc, int pos)
 {
 	const int exact = hem_norm_exact(o->norm);
 	const int sd = o->step == HEM_STEP_SD;
@@ -174,6 +179,8 @@ static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
 	hemk_step st = {o->step, o->clamp, o->alpha_clamp != 0, exact ? 2 : 1};
 	const double *alpha_part = NULL;
 	void *s = p->stream;
+	const int fuse_end = o->norm == HEM_NORM_EXACT && !o->fallback &&
+	                     pos >= 0;
 
 	if (o->fallback) {
 		/* u1, u2 (and the r/v rows) at the start of every iteration,
@@ -198,6 +205,8 @@ static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
 		                          p->d_sx, NULL, p->d_sres);
 		if (sd)
 			sp.csq = p->d_scsq;
+		if (fuse_end && pos > 0)
+			sp.iend = p->d_ctl;
 		HEM_TRY_DEV(hemk_spec_sum(&sp, 0, NULL, NULL, NULL, p->d_ctl, s),
 		            "exact sums 1");
 		MARK(HEM_PK_NORM1);
@@ -258,16 +267,20 @@ static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
 			return HEM_OK;
 		}
 	}
-	HEM_TRY_DEV(hemk_iter_end(p->d_ctl, s), "iter_end");
-	MARK(HEM_PK_ITER_END);
+	if (!fuse_end || pos == HEM_ITERS_PER_GRAPH - 1) {
+		HEM_TRY_DEV(hemk_iter_end(p->d_ctl, s), "iter_end");
+		MARK(HEM_PK_ITER_END);
+	}
 	return HEM_OK;
 }
 
-/* Enqueue one SQUAREM iteration on the problem's stream. */
-static int enqueue_iteration(hem_problem *p, const hem_opts *o, prof_ctx *pc)
+/* Enqueue one SQUAREM iteration on the problem's stream (pos: its place in
+ * a captured graph, -1 by itself). */
+static int enqueue_iteration(hem_problem *p, const hem_opts *o, prof_ctx *pc,
+                             int pos)
 {
 	if (use_tiles(p, o))
-		return enqueue_iteration_tiles(p, o, pc);
+		return enqueue_iteration_tiles(p, o, pc, pos);
 	{
 	const int f32 = o->precision == HEM_PREC_FP32;
 	const int prec = f32 ? 0 : 1;
@@ -390,7 +403,7 @@ static int ensure_graph(hem_problem *p, const hem_opts *o, void **graph)
 	if (e)
 		return hem_dev_fail(e, "graph capture begin");
 	for (int i = 0; i < HEM_ITERS_PER_GRAPH; ++i) {
-		rc = enqueue_iteration(p, o, NULL);
+		rc = enqueue_iteration(p, o, NULL, i);
 		if (rc) {
 			hemd_graph_end(p->stream, &exec);
 			hemd_graph_destroy(exec);
@@ -663,7 +676,7 @@ int hem_profile_solve(hem_problem *p, const uint32_t *unique,
 			rc = hem_dev_fail(e, "event");
 			goto out;
 		}
-		if ((rc = enqueue_iteration(p, &o, &pc)))
+		if ((rc = enqueue_iteration(p, &o, &pc, -1)))
 			goto out;
 		e = hemd_d2h(p->h_ctl, p->d_ctl, sizeof(hemk_ctl), p->stream);
 		if (!e)

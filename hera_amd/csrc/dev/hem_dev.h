@@ -506,20 +506,30 @@ typedef struct {
 	 * (no drifts, windows or links; W, tab, nx and base unused):
 	 *   wa  [nchain][P][L/2] what the record adds;
 	 *   wlh [nchain][P][L/2] pairs {lo, hi}: the starts it is valid for;
-	 *   wrg [nchain][P][L/2] its values' range in the chunk, beg | end << 11,
-	 *       and HEMK_WS_EVEN when the record needs a start whose last
-	 *       mantissa bit is 0;
+	 *   wrg [nchain][P][L/2] its first value's index in the chunk (bits
+	 *       0-10; its values run to the next record's, or to the chunk's
+	 *       end) and, in bits 11-21, 0 or the binade code plus one of the
+	 *       unit 2^(code - 1075) whose multiple the start must be;
 	 *   wcnt [nchain][P] records per chunk, or HEMK_WS_RAW (the chunk is
 	 *       added value by value). */
 	double *wa, *wlh;
 	uint32_t *wrg, *wcnt;
+	/* chunk programs without a seeded guess: when set ([nchain][P]
+	 * words, zeroed when allocated and written by nothing else), each
+	 * program publishes its chunk's sum here as one word, the double with
+	 * its low 16 significand bits replaced by the launch's epoch (its own
+	 * word's last epoch plus one, mod 2^16), and takes the earlier chunks'
+	 * sums from their words once they carry that epoch, so no chunk sums
+	 * step runs (csum is not written); every launch must cover all P
+	 * chunks of a chain.  The words read as doubles are the chunk sums to
+	 * a relative 2^-36, for the seeds of a centred pass */
+	unsigned long long *cpub;
 	/* chunk programs: when set, the walk's workgroup 0 ends the previous
 	 * SQUAREM iteration on this control block (as hemk_iter_end) once its
 	 * sum is written */
 	hemk_ctl *iend;
 } hemk_spec;
 #define HEMK_WS_RAW 0xFFFFFFFFu
-#define HEMK_WS_EVEN 0x80000000u
 /* Walk-step records of launches of `chains` chains of P chunks of L. */
 size_t hemk_ws_records(uint32_t L, uint32_t chains, uint32_t P);
 /* The finishing step by chunk programs (called by hemk_spec_sum and

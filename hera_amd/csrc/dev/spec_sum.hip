@@ -589,7 +589,7 @@ int launch_xf(const hemk_spec &sp, int produce, int phases, const double *u,
 	if (produce)
 		k_spec_rv<XF><<<grid, SBLOCK, 0, st>>>(sp, u, u1, u2,
 		                                       (double *)sp.x, ctl);
-	else if (phases & HEMK_SPEC_PH_CSUM)
+	else if ((phases & HEMK_SPEC_PH_CSUM) && !(sp.wa && sp.cpub))
 		k_spec_csum<XF><<<grid, SBLOCK, 0, st>>>(sp, ctl);
 	if (sp.wa) {
 		// chunk programs and the walk need no drifts or windows

@@ -2,37 +2,44 @@
 // in-order walk.
 //
 // The reference adds its norm sums one value after another (Norm,
-// src/hash_align.c:153-154 and 158-159).  Inside one binade [2^E, 2^(E+1))
-// every double is an integer multiple S of the binade's ulp u = 2^(E-52),
-// and while a running sum s = S u and its result stay in that binade,
-//     fl(s + y) = (S + q) u,   q = round(y / u) to the nearest integer,
-// except at a tie (y / u = k + 1/2), which goes to the even one of S + k and
-// S + k + 1.  So a run of values whose partial sums stay in one binade adds
-// a fixed integer D to S once the parity of S before its first tie is
-// known: the run is cut at that tie, which is added by itself (the
-// hardware's addition rounds it as the reference does), and the rest of
-// the run starts from an even S (a tie always ends on an even integer).  A
+// src/hash_align.c:153-154 and 158-159).  Fix a unit u = 2^G.  While a
+// running sum s = S u is a multiple of u below 2^53 u in magnitude, adding
+// a value y gives
+//   * (S + y / u) u exactly when y is a multiple of u and the result stays
+//     below 2^53 u, whatever binade the sum is in, and
+//   * (S + q) u, q = round(y / u) to the nearest integer, when the sums
+//     before and after stay in the top binade [2^52 u, 2^53 u), whose ulp
+//     is u, except at a tie (y / u = k + 1/2), which goes to the even one
+//     of S + k and S + k + 1.
+// So a run of such values adds a fixed integer D to S once the parity of S
+// before its first tie is known: the run is cut at that tie, which is
+// added by itself (the hardware's addition rounds it as the reference
+// does), and the rest of the run starts from an even S (a tie always ends
+// on an even integer).  Each value's unit is either its own binade's ulp
+// (the run then stays in that binade) or, for a row of 64 values, the ulp
+// of the binade just above every partial sum of the row (the run can then
+// cross the binades below without a record, as sums near zero do).  A
 // chunk of L values becomes a short program of records, each one
-//   * a segment: such a run, summarised as D u and the interval of starts
-//     [lo, hi] for which every partial sum stays in the binade with a
-//     margin of one ulp (and, after a cut, the need for an even start), or
-//   * a single value (one that crosses a binade, lies near a binade
-//     boundary or zero, or is a run's first tie), added as it is.
+//   * a segment: such a run, summarised as D u, the interval of starts
+//     [lo, hi] for which every partial sum stays where its rule needs it
+//     with a margin of one unit, and, where it matters, the unit (or twice
+//     the unit, after a cut) whose multiple the start must be, or
+//   * a single value (one that crosses a binade off the grid, lies near a
+//     binade boundary or zero, or is a run's first tie), added as it is.
 // Which values form segments is decided from an approximate trajectory (the
-// chunk's start guessed from the earlier chunks' plain sums); that choice
-// only affects speed.  Exactness comes from the walk: one wave adds the
-// records of every chunk in order from +0, and a segment whose actual start
-// lies outside [lo, hi] (or is odd where it must be even) is added value by
-// value instead.  Adding D u to a
-// start in [lo, hi] is exact (the result is on the binade's grid) and equals
-// the run's rounded additions, and a single value is the reference's own
-// addition, so the walk's sum is the reference's, bit for bit, whatever the
-// guesses were.
+// chunk's start guessed from the earlier chunks' plain sums, which the
+// programs publish to each other); that choice only affects speed.
+// Exactness comes from the walk: one wave adds the records of every chunk
+// in order from +0, and a segment whose actual start lies outside [lo, hi]
+// or off its grid is added value by value instead.  Adding D u to a valid
+// start is exact and equals the run's additions, and a single value is the
+// reference's own addition, so the walk's sum is the reference's, bit for
+// bit, whatever the guesses were.
 //
 // On SQUAREM's r and v rows of 200,000 values a chunk of 1,024 values needs
-// a handful of records where the running sum stays away from zero and
-// powers of two, and its single-value records are where it crosses them:
-// about 760 (r) and 1,370 (v) records per row against 200,000 chained adds.
+// a handful of records: a few hundred per row early in a solve, up to about
+// 2,000 (r) and 3,000 (v) late, when the running sums hover near zero,
+// against 200,000 chained adds.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -47,12 +54,44 @@
 #define WS_E (HEMK_SPEC_LMAX / WS_T)     // values per thread, at most 4
 #define WS_RB 2048                       // records per walk batch
 #define WS_TAU 0x1p-30   // segment margin, relative to the largest |sum|
+#define WS_SPIN (1u << 14)  // polls of a published chunk sum
 
 namespace {
 
-__device__ __forceinline__ uint32_t lo32(double v)
+// biased exponent field of v
+__device__ __forceinline__ uint32_t bexp(double v)
 {
-	return (uint32_t)__double_as_longlong(v);
+	return (uint32_t)((unsigned long long)__double_as_longlong(v) >> 52) &
+	       0x7FFu;
+}
+
+// max(e, 1) + trailing zeros of the significand (hidden bit included) of
+// a nonzero v with exponent field e: its lowest set bit is 2^(that - 1075)
+__device__ __forceinline__ int low_code(unsigned long long b)
+{
+	const int e = (int)((b >> 52) & 0x7FF);
+	unsigned long long m = b & 0xFFFFFFFFFFFFFull;
+	if (e)
+		m |= 1ull << 52;
+	return max(e, 1) + (int)__builtin_ctzll(m | (1ull << 63));
+}
+
+// y is +0, or a finite nonzero multiple of 2^(code - 1075)
+__device__ __forceinline__ bool on_grid(double y, int code)
+{
+	const unsigned long long b = (unsigned long long)__double_as_longlong(y);
+	return (b == 0) | (((b << 1) != 0) & (((b >> 52) & 0x7FF) != 0x7FF) &
+	                   (low_code(b) >= code));
+}
+
+// a start s that is a signed zero, a NaN or a multiple of 2^(code - 1075)
+// (s scaled by 2^(1075 - code) is an integer; a nonzero multiple scales to
+// at least 1, so a scaled value that underflows to 0 is not one)
+__device__ __forceinline__ bool start_on_grid(double s, int code)
+{
+	const double t = ldexp(s, 1075 - code);
+	return ((t == __builtin_trunc(t)) & ((t != 0.0) | (s == 0.0))) |
+	       (s != s);
 }
 
 // ---- scans across lanes (DPP moves) and across the workgroup --------------
@@ -80,6 +119,7 @@ __device__ __forceinline__ T dpp_mov(const T &old, const T &v)
 }
 
 #define DPP_ROW_SHR(n) (0x110 + (n))
+#define DPP_ROW_ROR(n) (0x120 + (n))
 #define DPP_WAVE_SHR1 0x138
 #define DPP_ROW_BCAST15 0x142
 #define DPP_ROW_BCAST31 0x143
@@ -147,8 +187,8 @@ struct tie_seen {
 };
 
 struct seg_rng {
-	uint32_t h, pad;
-	long long mn, mx;   // smallest / largest partial increment since start
+	uint32_t h, x;      // x: an exact row-grid value since the start
+	long long lo, hi;   // bounds on the segment's start since its start
 };
 
 struct sum_abs {
@@ -169,7 +209,8 @@ __global__ __launch_bounds__(WS_T) void k_ws_prog(hemk_spec sp,
 	__shared__ tie_seen w_f[WS_NW];
 	__shared__ seg_sum w_s[WS_NW];
 	__shared__ seg_rng w_r[WS_NW];
-	__shared__ uint32_t sm[WS_T], sm2[WS_T];
+	__shared__ tie_seen w_a[WS_NW];
+	__shared__ uint32_t sg[WS_T], sl[WS_T], sf[WS_T], sl2[WS_T];
 	uint32_t row;
 	if (!chain_row(sp, blockIdx.y, row))
 		return;
@@ -181,8 +222,11 @@ __global__ __launch_bounds__(WS_T) void k_ws_prog(hemk_spec sp,
 	const uint32_t e0 = t * E;
 	const double *x = sp.x + (size_t)row * sp.ld + (size_t)j * sp.L;
 	const bool seeded = XF == HEMK_SPEC_CENTRED && sp.gsq;
-	// the chunk's values and the first 256 earlier chunk sums are
-	// requested before the control word is read, so the loads overlap
+	// programs that publish their chunk sums (cpub) read the earlier ones
+	// once published; otherwise the first 256 earlier chunk sums are
+	// requested with the chunk's values, before the control word is read,
+	// so the loads overlap
+	const bool pub = sp.cpub && !seeded;
 	double y[WS_E];
 	uint32_t vmask = 0;
 #pragma unroll
@@ -194,7 +238,13 @@ __global__ __launch_bounds__(WS_T) void k_ws_prog(hemk_spec sp,
 		}
 	}
 	double c1 = 0.0, c2 = 0.0;
-	if ((uint32_t)t < j) {
+	uint32_t ep0 = 0;
+	if (pub) {
+		// the epoch of the last launch: only thread 0 reads it, as only
+		// thread 0 writes this word, after the read
+		if (t == 0)
+			ep0 = (uint32_t)sp.cpub[cb + j] & 0xFFFFu;
+	} else if ((uint32_t)t < j) {
 		if (seeded) {
 			c1 = sp.gsq[cb + t];
 			c2 = sp.gsum[cb + t];
@@ -215,43 +265,14 @@ __global__ __launch_bounds__(WS_T) void k_ws_prog(hemk_spec sp,
 	};
 	const auto fmx = [](double a, double b) { return fmax(a, b); };
 
-	// guessed start g of the chunk (the earlier chunks' sums, any order)
-	// and the largest magnitude M of the earlier chunk starts
-	double g = 0.0, M = 0.0;
-	{
-		const double lm2 = (double)sp.L * m * m;
-		for (uint32_t i0 = 0; i0 < j; i0 += WS_T) {
-			const uint32_t i = i0 + t;
-			if (i0 && i < j) {
-				if (seeded) {
-					c1 = sp.gsq[cb + i];
-					c2 = sp.gsum[cb + i];
-				} else {
-					c1 = sp.csum[cb + i];
-				}
-			}
-			double c = 0.0;
-			if (i < j)
-				c = seeded ? (c1 - 2.0 * m * c2) + lm2 : c1;
-			double tot;
-			const double ex = block_excl(c, 0.0, fadd, w_g, tot);
-			if (i < j)
-				M = fmax(M, fabs(g + ex));
-			g += tot;
-			__syncthreads();
-		}
-		double mt;
-		(void)block_excl(M, 0.0, fmx, w_m, mt);
-		M = fmax(mt, fabs(g));
-	}
-
-	// approximate partial sums a_k after each value, and a bound on every
-	// partial sum's magnitude in this chunk
+	// the chunk's own sums (any order), each thread's partial sums before
+	// its values, and with publishing the chunk sum and the epoch
+	__shared__ uint32_t sh_ep;
+	double tl = 0.0;
 	double a[WS_E];
-	double prev;
-	double tau;
+	sum_abs tt, ex;
 	{
-		double tl = 0.0, ta = 0.0, tq = 0.0;
+		double ta = 0.0, tq = 0.0;
 #pragma unroll
 		for (int k = 0; k < WS_E; ++k) {
 			tl += y[k];
@@ -267,12 +288,77 @@ __global__ __launch_bounds__(WS_T) void k_ws_prog(hemk_spec sp,
 			r.q = p.q + q.q;
 			return r;
 		};
-		sum_abs tt;
-		const sum_abs ex = block_excl(sum_abs{tl, ta, tq},
-		                              sum_abs{0.0, 0.0, 0.0}, pad, w_t, tt);
+		if (pub && t == 0)
+			sh_ep = (ep0 + 1u) & 0xFFFFu;
+		ex = block_excl(sum_abs{tl, ta, tq}, sum_abs{0.0, 0.0, 0.0}, pad,
+		                w_t, tt);
 		// the chunk's sum of squares, for the guesses of a centred pass
 		if (XF == HEMK_SPEC_IDENT && sp.csq && t == 0)
 			sp.csq[cb + j] = tt.q;
+	}
+	const uint32_t epoch = pub ? sh_ep : 0u;
+	// one word carries the sum and the epoch, so a relaxed store and
+	// relaxed reads suffice (no fence, which would write back the L2)
+	if (pub && t == 0)
+		__hip_atomic_store(sp.cpub + cb + j,
+		                   ((unsigned long long)__double_as_longlong(tt.s) &
+		                    ~0xFFFFull) | epoch,
+		                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+
+	// guessed start g of the chunk (the earlier chunks' sums, any order)
+	// and the largest magnitude M of the earlier chunk starts.  A
+	// published sum is taken once its word holds this launch's epoch; the
+	// wait is bounded, and a sum not seen in time counts as 0 (the guess
+	// only steers which values form runs)
+	double g = 0.0, M = 0.0;
+	{
+		const double lm2 = (double)sp.L * m * m;
+		for (uint32_t i0 = 0; i0 < j; i0 += WS_T) {
+			const uint32_t i = i0 + t;
+			if (pub) {
+				c1 = 0.0;
+				if (i < j) {
+					uint32_t spin = 0;
+					unsigned long long w;
+					for (;;) {
+						w = __hip_atomic_load(sp.cpub + cb + i,
+						                      __ATOMIC_RELAXED,
+						                      __HIP_MEMORY_SCOPE_AGENT);
+						if ((w & 0xFFFFull) == epoch || ++spin >= WS_SPIN)
+							break;
+						__builtin_amdgcn_s_sleep(1);
+					}
+					if ((w & 0xFFFFull) == epoch)
+						c1 = __longlong_as_double((long long)w);
+				}
+			} else if (i0 && i < j) {
+				if (seeded) {
+					c1 = sp.gsq[cb + i];
+					c2 = sp.gsum[cb + i];
+				} else {
+					c1 = sp.csum[cb + i];
+				}
+			}
+			double c = 0.0;
+			if (i < j)
+				c = seeded ? (c1 - 2.0 * m * c2) + lm2 : c1;
+			double tot;
+			const double exg = block_excl(c, 0.0, fadd, w_g, tot);
+			if (i < j)
+				M = fmax(M, fabs(g + exg));
+			g += tot;
+			__syncthreads();
+		}
+		double mt;
+		(void)block_excl(M, 0.0, fmx, w_m, mt);
+		M = fmax(mt, fabs(g));
+	}
+
+	// approximate partial sums a_k after each value, and a bound on every
+	// partial sum's magnitude in this chunk
+	double prev;
+	double tau;
+	{
 		prev = g + ex.s;
 #pragma unroll
 		for (int k = 0; k < WS_E; ++k)
@@ -280,14 +366,46 @@ __global__ __launch_bounds__(WS_T) void k_ws_prog(hemk_spec sp,
 		tau = (M + fabs(g) + tt.a) * WS_TAU;
 	}
 
-	// segment candidates: the partial sums before and after stay in one
-	// binade (sign and exponent) with a margin; q = the value in units of
-	// the binade's ulp, rounded to the nearest integer, a round-half tie
-	// (tmask) kept at floor for now: its rounding goes to the even
-	// neighbour and so depends on the running sum's last bit
-	uint32_t cmask = 0, tmask = 0;
-	uint32_t key[WS_E];
-	long long q[WS_E];
+	// Each value is summarised on one of two grids.
+	//  * Its binade's: while the partial sums before and after it stay in
+	//    one binade (sign and exponent) with the margin tau, the value adds
+	//    q = round(y / u) ulps u of that binade.
+	//  * Its row's: gc is the largest binade code of this DPP row's partial
+	//    sums (16 threads, 16 E values), margin included, so every partial
+	//    sum is below 2^53 units u = 2^(gc - 1075).  A value that is a
+	//    multiple of u (or +0) adds exactly y / u units wherever the sum is
+	//    below that bound, and a value whose partial sums stay in the top
+	//    binade (gc) adds round(y / u) as above, so a run of such values
+	//    can cross the binades below the top one without a record.  Its
+	//    start must be a multiple of u: a row-grid run begins at an anchor
+	//    (a value whose partial sum before lies in the top binade) and goes
+	//    on over the values after it with the same grid and the same sign
+	//    of the partial sums on both sides, which keeps its total below
+	//    2^53 units.
+	// A round-half tie (only a top-binade value can be one) is kept at
+	// floor for now: it rounds to the even neighbour, so it depends on the
+	// running sum's last bit.
+	uint32_t gc = 0;
+	{
+#pragma unroll
+		for (int k = 0; k < WS_E; ++k) {
+			const double pk = k ? a[k - 1] : prev;
+			gc = max(gc, bexp(fabs(pk) + tau));
+			gc = max(gc, bexp(fabs(a[k]) + tau));
+		}
+		gc = max(gc, dpp_mov<DPP_ROW_ROR(8), 0xF>(gc, gc));
+		gc = max(gc, dpp_mov<DPP_ROW_ROR(4), 0xF>(gc, gc));
+		gc = max(gc, dpp_mov<DPP_ROW_ROR(2), 0xF>(gc, gc));
+		gc = max(gc, dpp_mov<DPP_ROW_ROR(1), 0xF>(gc, gc));
+	}
+	const bool gok = gc >= 2 && gc <= 2044;
+	const double gtop = __longlong_as_double((long long)gc << 52);
+	uint32_t bmask = 0, btie = 0;   // binade grid: usable values, ties
+	uint32_t emask_x = 0;           // multiples of the row's unit (or +0)
+	uint32_t amask = 0, nmask = 0;  // anchors; partial sum after < 0
+	uint32_t gkey[WS_E], bkey[WS_E];
+	uint32_t glast = 0;   // gkey of the thread's last value
+	long long q[WS_E], qx[WS_E];
 #pragma unroll
 	for (int k = 0; k < WS_E; ++k) {
 		const double ak = a[k], pk = k ? a[k - 1] : prev;
@@ -296,7 +414,8 @@ __global__ __launch_bounds__(WS_T) void k_ws_prog(hemk_spec sp,
 		const uint32_t kp =
 		    (uint32_t)((unsigned long long)__double_as_longlong(pk) >> 52);
 		const uint32_t be = ka & 0x7FFu;
-		bool s = ((vmask >> k) & 1u) && ka == kp && be >= 2 && be <= 2044;
+		const bool valid = (vmask >> k) & 1u;
+		bool s = valid && ka == kp && be >= 2 && be <= 2044;
 		const double blo = __longlong_as_double((long long)be << 52);
 		const double bhi = __longlong_as_double((long long)(be + 1) << 52);
 		const double A = fabs(ak), B = fabs(pk);
@@ -306,22 +425,104 @@ __global__ __launch_bounds__(WS_T) void k_ws_prog(hemk_spec sp,
 		const double kf = floor(xs);
 		const double f = xs - kf;
 		s = s && fabs(xs) < 0x1p52;
-		key[k] = ka;
+		bkey[k] = ka;
 		q[k] = s ? (long long)kf + (f > 0.5 ? 1 : 0) : 0;
-		cmask |= (uint32_t)s << k;
-		tmask |= (uint32_t)(s && f == 0.5) << k;
+		bmask |= (uint32_t)s << k;
+		btie |= (uint32_t)(s && f == 0.5) << k;
+		const bool same = (ak > 0.0 && pk > 0.0) || (ak < 0.0 && pk < 0.0);
+		const bool xk = gok && valid && same && on_grid(y[k], (int)gc);
+		const bool rk = s && be == gc;
+		const bool ge = xk || rk;
+		qx[k] = xk ? (long long)ldexp(y[k], 1075 - (int)gc) : 0;
+		emask_x |= (uint32_t)xk << k;
+		nmask |= (uint32_t)(ak < 0.0) << k;
+		gkey[k] = ge ? ((gc << 1) | (uint32_t)(ak < 0.0)) + 1u : 0u;
+		if ((uint32_t)k + 1 == E)
+			glast = gkey[k];
+		amask |= (uint32_t)(ge && B - tau >= gtop) << k;
 	}
-	const uint32_t fullm = (1u << E) - 1u;
-	// the first tie of each candidate run is added by itself (its
-	// rounding needs the true start's last bit); after it the running sum
-	// is even, so the rest of the run is a segment whose start must be
-	// even, its later ties settled by the parity below
-	sm[t] = cmask;
+	// row-grid values: those of a stretch (consecutive values with the
+	// same nonzero gkey) from its first anchor on
+	sg[t] = glast;
+	__syncthreads();
+	uint32_t gmask = 0;
+	{
+		const auto tsop = [](tie_seen l, tie_seen r) {
+			tie_seen o;
+			o.h = l.h | r.h;
+			o.t = r.h ? r.t : (l.t | r.t);
+			return o;
+		};
+		uint32_t pkey = t ? sg[t - 1] : 0u, hm = 0;
+		tie_seen la = {0u, 0u};
+#pragma unroll
+		for (int k = 0; k < WS_E; ++k) {
+			if (gkey[k] && gkey[k] != pkey) {
+				la.h = 1;
+				la.t = 0;
+				hm |= 1u << k;
+			}
+			la.t |= (amask >> k) & 1u;
+			pkey = gkey[k];
+		}
+		tie_seen atot;
+		const tie_seen ca = block_excl(la, tie_seen{0u, 0u}, tsop, w_a, atot);
+		uint32_t seen = ca.t;
+#pragma unroll
+		for (int k = 0; k < WS_E; ++k) {
+			if ((hm >> k) & 1u)
+				seen = 0;
+			seen |= (amask >> k) & 1u;
+			if (gkey[k] && seen)
+				gmask |= 1u << k;
+		}
+	}
+	// labels: the unit's binade code and the sign of the partial sums
+	// (values of both kinds on one unit form one run), 0 for a value added
+	// by itself; units and increments; X marks exact row-grid values,
+	// whose partial sums only need to stay below 2^53 units
+	uint32_t lab[WS_E], code[WS_E];
+	uint32_t llast = 0;   // label of the thread's last value
+	uint32_t cmask = 0, tmask = 0, xmask = 0;
+#pragma unroll
+	for (int k = 0; k < WS_E; ++k) {
+		const uint32_t bit = 1u << k;
+		lab[k] = 0;
+		code[k] = 0;
+		if (gmask & bit) {
+			lab[k] = gkey[k];
+			code[k] = gc;
+			if (emask_x & bit) {
+				q[k] = qx[k];
+				xmask |= bit;
+			} else {
+				tmask |= btie & bit;
+			}
+		} else if (bmask & bit) {
+			code[k] = bkey[k] & 0x7FFu;
+			lab[k] = ((code[k] << 1) | (bkey[k] >> 11)) + 1u;
+			tmask |= btie & bit;
+		}
+		cmask |= (uint32_t)(lab[k] != 0) << k;
+		if ((uint32_t)k + 1 == E)
+			llast = lab[k];
+	}
+	// the first tie of each candidate run (consecutive values with the
+	// same label) is added by itself (its rounding needs the true start's
+	// last bit); after it the running sum is even, so the rest of the run
+	// is a segment whose start must be even, its later ties settled by the
+	// parity below
+	sl[t] = llast;
 	__syncthreads();
 	uint32_t split;   // first ties of candidate runs
 	{
-		const uint32_t pc = t ? (sm[t - 1] >> (E - 1)) & 1u : 0u;
-		const uint32_t ch = cmask & ~(((cmask << 1) | pc) & fullm);
+		uint32_t plab = t ? sl[t - 1] : 0u, ch = 0;
+#pragma unroll
+		for (int k = 0; k < WS_E; ++k) {
+			if (lab[k] && lab[k] != plab)
+				ch |= 1u << k;
+			plab = lab[k];
+		}
 		tie_seen lt = {0u, 0u};
 #pragma unroll
 		for (int k = 0; k < WS_E; ++k) {
@@ -355,18 +556,39 @@ __global__ __launch_bounds__(WS_T) void k_ws_prog(hemk_spec sp,
 	}
 	const uint32_t smask = cmask & ~split;
 	tmask &= ~split;
-	// segment starts (safe after a value that is not; an even start when
-	// that value is a split tie), ends (safe before one that is not) and
-	// records (values that are not safe, and starts)
-	sm2[t] = smask | (split << 8);
+	// segment starts (in smask after a value that is not, or after one
+	// with another label; an even start when that value is a split tie),
+	// ends (before such a value) and records (values not in smask, and
+	// starts)
+	sf[t] = (lab[0] << 2) | (smask & 1u);
+	sl2[t] = (llast << 2) | ((smask >> (E - 1)) & 1u) |
+	         (((split >> (E - 1)) & 1u) << 1);
 	__syncthreads();
-	const uint32_t pw = t ? sm2[t - 1] : 0u;
-	const uint32_t psafe = (pw >> (E - 1)) & 1u;
-	const uint32_t psplit = (pw >> (8 + E - 1)) & 1u;
-	const uint32_t nsafe = t + 1 < WS_T ? sm2[t + 1] & 1u : 0u;
-	const uint32_t hmask = smask & ~(((smask << 1) | psafe) & fullm);
-	const uint32_t evmask = hmask & (((split << 1) | psplit) & fullm);
-	const uint32_t emask = smask & ~((smask >> 1) | (nsafe << (E - 1)));
+	uint32_t hmask = 0, evmask = 0, emask = 0;
+	{
+		const uint32_t pw = t ? sl2[t - 1] : 0u;
+		const uint32_t nw = t + 1 < WS_T ? sf[t + 1] : 0u;
+		uint32_t pin = pw & 1u, psp = (pw >> 1) & 1u, plab = pw >> 2;
+#pragma unroll
+		for (int k = 0; k < WS_E; ++k) {
+			const uint32_t bit = 1u << k;
+			const bool in = smask & bit;
+			if (in && (!pin || plab != lab[k])) {
+				hmask |= bit;
+				if (psp)
+					evmask |= bit;
+			}
+			const bool inside = k + 1 < WS_E && (uint32_t)k + 1 < E;
+			const uint32_t nin = inside ? (smask >> (k + 1)) & 1u : nw & 1u;
+			const uint32_t nlab =
+			    inside ? lab[k + 1 < WS_E ? k + 1 : k] : nw >> 2;
+			if (in && (!nin || nlab != lab[k]))
+				emask |= bit;
+			pin = in;
+			psp = (split >> k) & 1u;
+			plab = lab[k];
+		}
+	}
 	const uint32_t rmask = vmask & (~smask | hmask);
 
 	// parity before each segment value for a segment started at an even
@@ -450,8 +672,28 @@ __global__ __launch_bounds__(WS_T) void k_ws_prog(hemk_spec sp,
 	};
 	seg_sum stot;
 	const seg_sum cs = block_excl(ls, seg_sum{0u, -1, 0}, ssop, w_s, stot);
-	// smallest and largest P since each segment start
-	seg_rng lr = {0u, 0u, LLONG_MAX, LLONG_MIN};
+	// the starts S (in units) each segment value allows: with P before and
+	// P' after it, S + P and S + P' in [2^52 + 1, 2^53 - 1] (negated for
+	// a negative binade) for a value rounded in its binade, and
+	// |S + P'| <= 2^53 - 1 for an exact row-grid value; lo and hi are the
+	// tightest bounds since the segment's start
+	// (with P' = P + q: S >= L - min(0, q) - P and S <= U - max(0, q) - P)
+	const long long B52 = 1LL << 52, B53 = 1LL << 53;
+	long long wl[WS_E], wu[WS_E];
+#pragma unroll
+	for (int k = 0; k < WS_E; ++k) {
+		const bool xk = (xmask >> k) & 1u, nk = (nmask >> k) & 1u;
+		const long long L = xk || nk ? -(B53 - 1) : B52 + 1;
+		const long long U = xk || !nk ? B53 - 1 : -(B52 + 1);
+		wl[k] = L - min(0LL, q[k]);
+		wu[k] = U - max(0LL, q[k]);
+	}
+	const auto bounds = [&](uint32_t k, long long p0, long long &lo,
+	                        long long &hi) {
+		lo = max(lo, wl[k] - p0);
+		hi = min(hi, wu[k] - p0);
+	};
+	seg_rng lr = {0u, 0u, LLONG_MIN, LLONG_MAX};
 	{
 		long long acc = cs.acc;
 #pragma unroll
@@ -460,34 +702,39 @@ __global__ __launch_bounds__(WS_T) void k_ws_prog(hemk_spec sp,
 				continue;
 			if ((hmask >> k) & 1u) {
 				lr.h = 1;
-				lr.mn = LLONG_MAX;
-				lr.mx = LLONG_MIN;
+				lr.x = 0;
+				lr.lo = LLONG_MIN;
+				lr.hi = LLONG_MAX;
 				acc = 0;
 			}
+			bounds(k, acc, lr.lo, lr.hi);
 			acc += q[k];
-			lr.mn = min(lr.mn, acc);
-			lr.mx = max(lr.mx, acc);
+			lr.x |= (xmask >> k) & 1u;
 		}
 	}
 	const auto srop = [](seg_rng l, seg_rng r) {
 		seg_rng o;
 		o.h = l.h | r.h;
-		o.pad = 0;
-		o.mn = r.h ? r.mn : min(l.mn, r.mn);
-		o.mx = r.h ? r.mx : max(l.mx, r.mx);
+		o.x = r.h ? r.x : (l.x | r.x);
+		o.lo = r.h ? r.lo : max(l.lo, r.lo);
+		o.hi = r.h ? r.hi : min(l.hi, r.hi);
 		return o;
 	};
 	seg_rng rtot;
 	const seg_rng cr = block_excl(
-	    lr, seg_rng{0u, 0u, LLONG_MAX, LLONG_MIN}, srop, w_r, rtot);
+	    lr, seg_rng{0u, 0u, LLONG_MIN, LLONG_MAX}, srop, w_r, rtot);
 
 	// records: single values at their own index, a segment at its
 	// start's index (no record lies between a segment's start and end).
-	// A segment adds D = P(end) ulps; every partial sum S + P_i stays in
-	// [2^52 + 1, 2^53 - 1] ulps (in magnitude) for starts S in [lo, hi]
-	// (and, after a split tie, S even)
+	// A segment adds D = P(end) units for starts S in [lo, hi] that are
+	// multiples of its unit (of twice the unit after a split tie); the
+	// record keeps its first value's index and, when the start's grid
+	// needs a check (an exact row-grid value or a split tie: any start in
+	// [lo, hi] of a run rounded in its binade is on the unit's grid), the
+	// binade code of the unit it must be a multiple of, plus one
 	seg_sum r = cs;
-	long long mn = cr.mn, mx = cr.mx;
+	long long slo = cr.lo, shi = cr.hi;
+	uint32_t sx = cr.x;
 	uint32_t nr = 0;   // records at this thread's values so far
 	const size_t rb0 = (cb + j) * cap + rbase;
 	double2 *wlh = (double2 *)sp.wlh;
@@ -498,12 +745,13 @@ __global__ __launch_bounds__(WS_T) void k_ws_prog(hemk_spec sp,
 			if (hmask & bit) {
 				r.acc = 0;
 				r.hp = (int)(e0 + k) | ((evmask & bit) ? WS_EVEN : 0);
-				mn = LLONG_MAX;
-				mx = LLONG_MIN;
+				slo = LLONG_MIN;
+				shi = LLONG_MAX;
+				sx = 0;
 			}
+			bounds(k, r.acc, slo, shi);
 			r.acc += q[k];
-			mn = min(mn, r.acc);
-			mx = max(mx, r.acc);
+			sx |= (xmask >> k) & 1u;
 		}
 		if (rmask & bit)
 			++nr;
@@ -511,34 +759,23 @@ __global__ __launch_bounds__(WS_T) void k_ws_prog(hemk_spec sp,
 			const size_t ri = rb0 + nr - 1;
 			sp.wa[ri] = y[k];
 			wlh[ri] = make_double2(-__builtin_inf(), __builtin_inf());
-			sp.wrg[ri] = (e0 + k) | ((e0 + k + 1) << 11);
+			sp.wrg[ri] = e0 + k;
 		}
 		if (emask & bit) {
 			const size_t ri = rb0 + nr - 1;
-			const int be = (int)(key[k] & 0x7FFu);
-			const bool neg = key[k] >> 11;
-			const long long B52 = 1LL << 52, B53 = 1LL << 53;
-			long long loS, hiS;
-			if (!neg) {
-				loS = max(B52, B52 + 1 - mn);
-				hiS = min(B53 - 1, B53 - 1 - mx);
-			} else {
-				loS = max(-(B53 - 1), -(B53 - 1) - mn);
-				hiS = min(-B52, -(B52 + 1) - mx);
-			}
+			const int sh = (int)code[k] - 1075;
 			double lo = __builtin_inf(), hi = -__builtin_inf();
 			double da = 0.0;
-			if (loS <= hiS) {
-				const int sh = be - 1075;
-				lo = ldexp((double)loS, sh);
-				hi = ldexp((double)hiS, sh);
+			if (slo <= shi && r.acc > -B53 && r.acc < B53) {
+				lo = ldexp((double)slo, sh);
+				hi = ldexp((double)shi, sh);
 				da = ldexp((double)r.acc, sh);
 			}
+			const uint32_t ev = (r.hp & WS_EVEN) ? 1u : 0u;
+			const uint32_t chk = sx | ev ? code[k] + ev + 1u : 0u;
 			sp.wa[ri] = da;
 			wlh[ri] = make_double2(lo, hi);
-			sp.wrg[ri] = (uint32_t)(r.hp & 0x7FF) |
-			             ((e0 + k + 1) << 11) |
-			             ((r.hp & WS_EVEN) ? HEMK_WS_EVEN : 0u);
+			sp.wrg[ri] = (uint32_t)(r.hp & 0x7FF) | (chk << 11);
 		}
 	}
 }
@@ -696,26 +933,25 @@ __global__ __launch_bounds__(SBLOCK) void k_ws_walk(hemk_spec sp,
 			for (uint32_t r = inc - cc; r < inc; ++r)
 				lch[r] = (uint8_t)tid;
 		__syncthreads();
-		// each thread's records requested RPT / 2 at a time, then stored
+		// each thread's records requested RPT / 2 at a time, then stored;
+		// a slot past the batch's last record reads that record again, so
+		// the loads are unconditional and stay in flight together
 #pragma unroll
 		for (int h = 0; h < 2; ++h) {
 			constexpr int HR = RPT / 2;
+			if ((uint32_t)(h * HR * SBLOCK) >= nr)
+				break;
 			double av[HR], lo[HR], hi[HR];
 			uint32_t rg[HR];
 #pragma unroll
 			for (int i = 0; i < HR; ++i) {
-				const uint32_t r = tid + (h * HR + i) * SBLOCK;
-				av[i] = lo[i] = hi[i] = 0.0;
-				rg[i] = 0;
-				if (r < nr) {
-					const uint32_t t = lch[r];
-					const size_t src =
-					    (cb + j + t) * cap + (r - loff[t]);
-					av[i] = sp.wa[src];
-					lo[i] = sp.wlh[2 * src];
-					hi[i] = sp.wlh[2 * src + 1];
-					rg[i] = sp.wrg[src] | (t << 22);   // chunk of the batch
-				}
+				const uint32_t r = min(tid + (h * HR + i) * SBLOCK, nr - 1);
+				const uint32_t t = lch[r];
+				const size_t src = (cb + j + t) * cap + (r - loff[t]);
+				av[i] = sp.wa[src];
+				lo[i] = sp.wlh[2 * src];
+				hi[i] = sp.wlh[2 * src + 1];
+				rg[i] = sp.wrg[src];
 			}
 #pragma unroll
 			for (int i = 0; i < HR; ++i) {
@@ -742,21 +978,34 @@ __global__ __launch_bounds__(SBLOCK) void k_ws_walk(hemk_spec sp,
 			uint32_t myf = 0xFFFFFFFFu;
 			double myv = 0.0;
 			for (uint32_t g = tid; g < ng; g += SBLOCK) {
+				// all WS_WB records without branches (a slot past the
+				// last reads it again, masked); the first that fails and
+				// its start are kept, the sums after it are not used
 				double v = lsb[g];
 				const uint32_t rb = from + g * WS_WB;
 				const uint32_t re = min(rb + WS_WB, nr);
-				for (uint32_t r = rb; r < re; ++r) {
-					const double2 lh = llh[r];
-					const bool odd = (lrg[r] & HEMK_WS_EVEN) &&
-					                 (lo32(v) & 1u);
-					if (v < lh.x || v > lh.y || odd) {
-						if (r < myf) {
-							myf = r;
-							myv = v;
-						}
-						break;
-					}
-					v = v + la[r];
+				uint32_t gf = 0xFFFFFFFFu;
+				double gv = 0.0;
+#pragma unroll
+				for (int i = 0; i < WS_WB; ++i) {
+					const uint32_t r = rb + i;
+					const uint32_t rr = min(r, nr - 1);
+					const double2 lh = llh[rr];
+					const uint32_t rg = lrg[rr];
+					const double add = la[rr];
+					const int gcode = (int)((rg >> 11) & 0x7FFu);
+					const bool bad =
+					    (r < re) &
+					    ((v < lh.x) | (v > lh.y) |
+					     ((gcode != 0) & !start_on_grid(v, gcode - 1)));
+					const bool first = bad & (gf == 0xFFFFFFFFu);
+					gf = first ? r : gf;
+					gv = first ? v : gv;
+					v = v + add;
+				}
+				if (gf < myf) {
+					myf = gf;
+					myv = gv;
 				}
 			}
 			if (myf != 0xFFFFFFFFu)
@@ -771,10 +1020,15 @@ __global__ __launch_bounds__(SBLOCK) void k_ws_walk(hemk_spec sp,
 			if (walker) {
 				// the segment's values one by one from its start
 				const double m = chain_mean<XF>(sp, row);
-				const uint32_t rg = lrg[f];
-				const uint32_t beg = rg & 0x7FFu;
-				const uint32_t end = (rg >> 11) & 0x7FFu;
-				const uint32_t t = (rg >> 22) & 0xFFu;
+				// its values: up to the next record of the chunk, or
+				// the chunk's end
+				const uint32_t t = lch[f];
+				const uint32_t n1 = min(sp.L, sp.n - (j + t) * sp.L);
+				const uint32_t beg = min(lrg[f] & 0x7FFu, n1);
+				const uint32_t end =
+				    f + 1 < nr && lch[f + 1] == t
+				        ? min(max(lrg[f + 1] & 0x7FFu, beg), n1)
+				        : n1;
 				double v[CH_REG];
 				chunk_load(xr + (size_t)(j + t) * sp.L + beg, end - beg, v);
 				chunk_store<XF>(v, end - beg, m, ys);

@@ -160,6 +160,7 @@ void hem_batch_free(hem_problem *p)
 	hemd_free(w->swlh);
 	hemd_free(w->swrg);
 	hemd_free(w->swcnt);
+	hemd_free(w->scpub);
 	hemd_host_free(w->h_active);
 	memset(w, 0, sizeof(*w));
 	for (int i = 0; i < HEM_TILE_ARRAYS; ++i) {
@@ -231,6 +232,10 @@ static int ws_ensure(hem_problem *p, uint32_t R)
 		WALLOC(w->swlh, nrec * 16);
 		WALLOC(w->swrg, nrec * 4);
 		WALLOC(w->swcnt, (size_t)2 * R * p->spec_P * 4);
+		WALLOC(w->scpub, (size_t)2 * R * p->spec_P * 8);
+		if (hemk_fill_u32((uint32_t *)w->scpub,
+		                  (size_t)4 * R * p->spec_P, 0, p->stream))
+			return hem_fail(HEM_ERR_DEVICE, "batch workspace fill");
 	}
 #undef WALLOC
 	if (hemd_host_alloc((void **)&w->h_active,
@@ -325,6 +330,7 @@ static hemk_spec spec_batch(hem_problem *p, const hem_opts *o,
 		sp.wlh = w->swlh;
 		sp.wrg = w->swrg;
 		sp.wcnt = w->swcnt;
+		sp.cpub = w->scpub;
 	}
 	return sp;
 }

@@ -44,6 +44,7 @@ typedef struct {
 	uint8_t *snx;                  /* [rows][P][W] window links         */
 	double *swa, *swlh;           /* [2R][P][L/2] chunk programs       */
 	uint32_t *swrg, *swcnt;
+	unsigned long long *scpub;
 	uint32_t *h_active;            /* pinned                            */
 	/* captured iterations; the loop bounds and stop thresholds are kernel
 	 * arguments, so a graph is reused only for identical options */
@@ -116,6 +117,7 @@ struct hem_problem {
 	/* chunk programs of the exact sums (two chains) */
 	double *d_swa, *d_swlh;
 	uint32_t *d_swrg, *d_swcnt;
+	unsigned long long *d_scpub;
 	double *d_cls_w, *d_tx_w;  /* member weights (weighted tables)   */
 	int weighted;
 	hemk_ctl *d_ctl;

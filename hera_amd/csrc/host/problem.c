@@ -364,6 +364,7 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 		DALLOC_H(p->d_swlh, nrec * 16);
 		DALLOC_H(p->d_swrg, nrec * 4);
 		DALLOC_H(p->d_swcnt, 2 * (size_t)p->spec_P * 4);
+		DALLOC_H(p->d_scpub, 2 * (size_t)p->spec_P * 8);
 	}
 	DALLOC_H(p->d_sres, 4 * 8);
 	DALLOC_H(p->d_ctl, sizeof(hemk_ctl));
@@ -404,6 +405,10 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 		              p->stream);
 	if (!rc)
 		rc = hemd_h2d(p->d_len, c->tx_len, (size_t)T * 4, p->stream);
+	/* the chunk programs' epochs start at 0 */
+	if (!rc)
+		rc = hemk_fill_u32((uint32_t *)p->d_scpub,
+		                   4 * (size_t)p->spec_P, 0, p->stream);
 	if (!rc)
 		rc = hemd_stream_sync(p->stream);
 	if (rc) {
@@ -671,6 +676,7 @@ int hem_problem_destroy(hem_problem *p)
 	hemd_free(p->d_swlh);
 	hemd_free(p->d_swrg);
 	hemd_free(p->d_swcnt);
+	hemd_free(p->d_scpub);
 	hemd_free(p->d_sres);
 	hemd_free(p->d_fbpart);
 	hemd_free(p->d_tpart);

@@ -116,6 +116,7 @@ static hemk_spec spec_plain(const hem_problem *p, const hem_opts *o,
 		sp.wlh = p->d_swlh;
 		sp.wrg = p->d_swrg;
 		sp.wcnt = p->d_swcnt;
+		sp.cpub = p->d_scpub;
 	}
 	return sp;
 }
@@ -214,7 +215,10 @@ static int enqueue_iteration_tiles(hem_problem *p, const hem_opts *o,
 		if (sd) {
 			sp = spec_plain(p, o, 2, HEMK_SPEC_CENTRED, p->d_sx,
 			                p->d_sres, p->d_sres + 2);
-			sp.gsum = p->d_scsum;
+			/* pass 1's chunk sums: its published words, or the chunk
+			 * sums step's */
+			sp.gsum = sp.cpub ? (const double *)p->d_scpub
+			                  : p->d_scsum;
 			sp.gsq = p->d_scsq;
 			HEM_TRY_DEV(hemk_spec_phases(&sp,
 			                             HEMK_SPEC_PH_DRIFT |
@@ -756,6 +760,7 @@ int hem_exact_sums(const double *x, uint32_t n, uint32_t n_rows, int mode,
 	uint32_t *d_miss = NULL, miss = 0;
 	double *d_wa = NULL, *d_wlh = NULL;
 	uint32_t *d_wrg = NULL, *d_wcnt = NULL;
+	unsigned long long *d_cpub = NULL;
 	hemk_spec sp;
 	const int windows = (mode & HEM_SUM_WINDOWS) != 0;
 	mode &= ~HEM_SUM_WINDOWS;
@@ -790,6 +795,7 @@ int hem_exact_sums(const double *x, uint32_t n, uint32_t n_rows, int mode,
 		if (!e) e = hemd_malloc((void **)&d_wlh, nrec * 16);
 		if (!e) e = hemd_malloc((void **)&d_wrg, nrec * 4);
 		if (!e) e = hemd_malloc((void **)&d_wcnt, cp * 4);
+		if (!e) e = hemd_malloc((void **)&d_cpub, cp * 8);
 	}
 	if (e) {
 		rc = hem_fail(HEM_ERR_NOMEM, "device allocation: %s",
@@ -808,10 +814,13 @@ int hem_exact_sums(const double *x, uint32_t n, uint32_t n_rows, int mode,
 	sp.wlh = d_wlh;
 	sp.wrg = d_wrg;
 	sp.wcnt = d_wcnt;
+	sp.cpub = d_cpub;
 	sp.xf = mode == HEM_SUM_SQUARES ? HEMK_SPEC_SQUARE : HEMK_SPEC_IDENT;
 	sp.W = hemk_spec_width(sp.L, sp.xf, sp.nchain * sp.P);
 	e = hemd_h2d(d_x, x, (size_t)n_rows * n * 8, s);
 	if (!e) e = hemk_fill_u32(d_miss, 1, 0, s);
+	if (!e && d_cpub)
+		e = hemk_fill_u32((uint32_t *)d_cpub, 2 * cp, 0, s);
 	if (!e) e = hemk_spec_sum(&sp, 0, NULL, NULL, NULL, NULL, s);
 	if (!e && mode == HEM_SUM_NORM_SD) {
 		/* second pass of Norm: centred squares about sum / n */
@@ -848,6 +857,7 @@ out:
 	hemd_free(d_wlh);
 	hemd_free(d_wrg);
 	hemd_free(d_wcnt);
+	hemd_free(d_cpub);
 	if (s)
 		hemd_stream_destroy(s);
 	return rc;

@@ -440,8 +440,9 @@ __global__ void k_iter_end(hemk_ctl *__restrict__ ctl)
 
 __global__ void k_ctl_reset(hemk_ctl *ctl, uint32_t min_iter,
                             uint32_t max_iter, double stop_min,
-                            double stop_rel)
+                            double stop_rel, uint32_t *hdone)
 {
+	ctl->hdone = hdone;
 	ctl->k = 0;
 	ctl->maps = 0;
 	ctl->converged = 0;
@@ -456,7 +457,9 @@ __global__ void k_ctl_reset(hemk_ctl *ctl, uint32_t min_iter,
 	ctl->stop_min = stop_min;
 	ctl->stop_rel = stop_rel;
 	// Loop condition of src/EM.c:328 at k = 0 with stop = 0.
-	ctl->done = (0 < min_iter || 0 < max_iter) ? 0u : 1u;
+	ctl->done = 0;
+	if (!(0 < min_iter || 0 < max_iter))
+		em_set_done(ctl, 0);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_init_state(
@@ -660,10 +663,11 @@ int hemk_init_state(uint32_t n, double *lam64, float *lam32, double value,
 }
 
 int hemk_ctl_reset(hemk_ctl *ctl, uint32_t min_iter, uint32_t max_iter,
-                   double stop_min, double stop_rel, void *s)
+                   double stop_min, double stop_rel, uint32_t *hdone,
+                   void *s)
 {
 	k_ctl_reset<<<1, 1, 0, (hipStream_t)s>>>(ctl, min_iter, max_iter,
-	                                         stop_min, stop_rel);
+	                                         stop_min, stop_rel, hdone);
 	return (int)hipGetLastError();
 }
 

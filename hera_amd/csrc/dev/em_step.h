@@ -8,6 +8,16 @@
 
 #include "hem_dev.h"
 
+// The loop's end: done, converged, and the host's copy of done.
+__device__ __forceinline__ void em_set_done(hemk_ctl *ctl, uint32_t stop)
+{
+	ctl->done = 1;
+	ctl->converged = stop;
+	if (ctl->hdone)
+		__hip_atomic_store(ctl->hdone, 1u, __ATOMIC_RELAXED,
+		                   __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // The end of one SQUAREM iteration (src/EM.c:328 and 357): k++, the loop
 // condition from the stop test of its last map (ctl->stopbad, cleared for
 // the next iteration), done set when the loop ends.  One thread.
@@ -18,10 +28,8 @@ __device__ __forceinline__ void em_iter_end(hemk_ctl *ctl)
 	ctl->stopbad = 0;
 	ctl->k = k;
 	ctl->maps += 3;
-	if (!(k < ctl->min_iter || (k < ctl->max_iter && !stop))) {
-		ctl->done = 1;
-		ctl->converged = stop;
-	}
+	if (!(k < ctl->min_iter || (k < ctl->max_iter && !stop)))
+		em_set_done(ctl, stop);
 }
 
 // Step length from the two reductions (src/EM.c:338-346): sample sd (A, B

@@ -713,10 +713,8 @@ __global__ __launch_bounds__(TA_THREADS) void k_ptile_tree_alpha(
 			ctl->maps += 3;
 			fin = !(k < ctl->min_iter ||
 			        (k < ctl->max_iter && !stop));
-			if (fin) {
-				ctl->done = 1;
-				ctl->converged = stop;
-			}
+			if (fin)
+				em_set_done(ctl, stop);
 		}
 		__syncthreads();
 		if (fin)

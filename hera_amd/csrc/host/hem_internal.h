@@ -86,6 +86,7 @@ struct hem_problem {
 	int group_cls, group_tx;
 	void *stream;
 	void *ev0, *ev1;
+	void *evq[2];         /* the loop's control readbacks */
 
 	/* structure (device) */
 	uint32_t *d_cls_off, *d_cls_ref, *d_tx_off, *d_tx_ref, *d_len;
@@ -122,6 +123,7 @@ struct hem_problem {
 	int weighted;
 	hemk_ctl *d_ctl;
 	hemk_ctl *h_ctl; /* pinned */
+	uint32_t *h_done; /* pinned, the loop's done flag (set by the device) */
 
 	/* component tiles (n_tiles == 0: some component is too large) for
 	 * the batched and standard-error kernels */

@@ -301,6 +301,10 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 		rc = hemd_event_create(&p->ev2);
 	if (!rc)
 		rc = hemd_event_create(&p->ev3);
+	if (!rc)
+		rc = hemd_event_create(&p->evq[0]);
+	if (!rc)
+		rc = hemd_event_create(&p->evq[1]);
 	if (rc) {
 		free(tx_off);
 		free(tx_ref);
@@ -370,6 +374,8 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 	DALLOC_H(p->d_ctl, sizeof(hemk_ctl));
 #undef DALLOC_H
 	rc = hemd_host_alloc((void **)&p->h_ctl, sizeof(hemk_ctl));
+	if (!rc)
+		rc = hemd_host_alloc((void **)&p->h_done, sizeof(uint32_t));
 	if (rc) {
 		free(tx_off);
 		free(tx_ref);
@@ -409,6 +415,7 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 	if (!rc)
 		rc = hemk_fill_u32((uint32_t *)p->d_scpub,
 		                   4 * (size_t)p->spec_P, 0, p->stream);
+
 	if (!rc)
 		rc = hemd_stream_sync(p->stream);
 	if (rc) {
@@ -686,6 +693,10 @@ int hem_problem_destroy(hem_problem *p)
 		hemd_free(p->d_ptile_mem[i]);
 	hemd_free(p->d_ctl);
 	hemd_host_free(p->h_ctl);
+	hemd_host_free(p->h_done);
+	for (int i = 0; i < 2; ++i)
+		if (p->evq[i])
+			hemd_event_destroy(p->evq[i]);
 	if (p->ev0)
 		hemd_event_destroy(p->ev0);
 	if (p->ev1)

@@ -423,12 +423,18 @@ static int ensure_graph(hem_problem *p, const hem_opts *o, void **graph)
 	return HEM_OK;
 }
 
-/* Runs the iteration loop on the device from the state already in A. */
+/* Runs the iteration loop on the device from the state already in A.  From
+ * min_iter on the host checks the done flag the device mirrors into pinned
+ * memory (no copy on the stream) one graph behind: graph g + 1 is queued
+ * before the event after graph g is waited for, so the device never idles
+ * on the host's check; the graph queued after convergence does nothing
+ * (every kernel reads the done flag first). */
 static int run_loop(hem_problem *p, const hem_opts *o, float *ms)
 {
 	int rc;
 	void *graph = NULL;
 	uint32_t launched = 0;
+	int slot = 0, pending = 0;
 	const uint32_t limit = o->max_iter > o->min_iter ? o->max_iter
 	                                                  : o->min_iter;
 	if ((rc = ensure_graph(p, o, &graph)))
@@ -440,16 +446,21 @@ static int run_loop(hem_problem *p, const hem_opts *o, float *ms)
 		launched += HEM_ITERS_PER_GRAPH;
 		if (launched < o->min_iter && launched < limit)
 			continue;
-		HEM_TRY_DEV(hemd_d2h(p->h_ctl, p->d_ctl, sizeof(hemk_ctl),
-		                     p->stream),
-		            "control readback");
-		HEM_TRY_DEV(hemd_stream_sync(p->stream), "stream sync");
-		if (p->h_ctl->done)
-			break;
-		if (launched > limit + HEM_ITERS_PER_GRAPH)
+		HEM_TRY_DEV(hemd_event_record(p->evq[slot], p->stream), "event");
+		slot ^= 1;
+		if (pending) {
+			/* the graph before this one has finished */
+			HEM_TRY_DEV(hemd_event_sync(p->evq[slot]), "event sync");
+			if (*(volatile uint32_t *)p->h_done)
+				break;
+		}
+		pending = 1;
+		if (launched > limit + 2 * HEM_ITERS_PER_GRAPH)
 			return hem_fail(HEM_ERR_DEVICE,
 			                "iteration loop did not terminate");
 	}
+	HEM_TRY_DEV(hemd_d2h(p->h_ctl, p->d_ctl, sizeof(hemk_ctl), p->stream),
+	            "control readback");
 	HEM_TRY_DEV(hemd_event_record(p->ev1, p->stream), "event");
 	HEM_TRY_DEV(hemd_event_sync(p->ev1), "event sync");
 	HEM_TRY_DEV(hemd_event_elapsed(p->ev0, p->ev1, ms), "event time");
@@ -472,10 +483,11 @@ static int solve_setup(hem_problem *p, const uint32_t *unique,
 	if (!rc)
 		rc = hemk_init_state(p->n_tx, p->d_A, p->d_A32,
 		                     (double)1 / p->n_tx, NULL, p->stream);
+	*(volatile uint32_t *)p->h_done = 0;
 	if (!rc)
 		rc = hemk_ctl_reset(p->d_ctl, o->min_iter, o->max_iter,
 		                    o->stop_min_value, o->stop_rel_change,
-		                    p->stream);
+		                    p->h_done, p->stream);
 	if (rc)
 		return hem_dev_fail(rc, "solve setup");
 	return use_tiles(p, o) && !o->fallback ? enqueue_tiles_prologue(p, o)

@@ -385,8 +385,11 @@ __global__ __launch_bounds__(WS_T) void k_ws_prog(hemk_spec sp,
 	// A round-half tie (only a top-binade value can be one) is kept at
 	// floor for now: it rounds to the even neighbour, so it depends on the
 	// running sum's last bit.
+	// (only for plain sums: squares and centred squares only grow, so
+	// their runs never cross binades on a row's grid)
+	constexpr bool GRID = XF == HEMK_SPEC_IDENT;
 	uint32_t gc = 0;
-	{
+	if (GRID) {
 #pragma unroll
 		for (int k = 0; k < WS_E; ++k) {
 			const double pk = k ? a[k - 1] : prev;
@@ -443,10 +446,10 @@ __global__ __launch_bounds__(WS_T) void k_ws_prog(hemk_spec sp,
 	}
 	// row-grid values: those of a stretch (consecutive values with the
 	// same nonzero gkey) from its first anchor on
-	sg[t] = glast;
-	__syncthreads();
 	uint32_t gmask = 0;
-	{
+	if (GRID) {
+		sg[t] = glast;
+		__syncthreads();
 		const auto tsop = [](tie_seen l, tie_seen r) {
 			tie_seen o;
 			o.h = l.h | r.h;

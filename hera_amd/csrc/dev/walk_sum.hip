@@ -164,14 +164,16 @@ __device__ __forceinline__ T block_excl(T v, const T &id, Op op, T *wt,
 }
 
 // Segmented carries.  Each thread summarises its values as (h: a segment
-// starts among them, then the value since its last start; otherwise the
-// value over all of them); the exclusive scan gives every thread the value
+// starts among them, then the part since its last start; otherwise the
+// part over all of them); the exclusive scan gives every thread the state
 // carried in from the threads before it.
 struct seg_sum {
 	uint32_t h;
 	int hp;          // position of the last segment start (-1: none),
 	                 // + WS_EVEN when that segment needs an even start
+	uint32_t x, pad; // x: an exact row-grid value since the start
 	long long acc;   // increments since the last segment start
+	long long lo, hi; // bounds on the segment's start since its start
 };
 #define WS_EVEN 0x40000000
 
@@ -184,11 +186,6 @@ struct cnt_par {
 // has been seen since the last start (or over all the values)
 struct tie_seen {
 	uint32_t h, t;
-};
-
-struct seg_rng {
-	uint32_t h, x;      // x: an exact row-grid value since the start
-	long long lo, hi;   // bounds on the segment's start since its start
 };
 
 struct sum_abs {
@@ -208,7 +205,6 @@ __global__ __launch_bounds__(WS_T) void k_ws_prog(hemk_spec sp,
 	__shared__ cnt_par w_c[WS_NW];
 	__shared__ tie_seen w_f[WS_NW];
 	__shared__ seg_sum w_s[WS_NW];
-	__shared__ seg_rng w_r[WS_NW];
 	__shared__ tie_seen w_a[WS_NW];
 	__shared__ uint32_t sg[WS_T], sl[WS_T], sf[WS_T], sl2[WS_T];
 	uint32_t row;
@@ -654,34 +650,17 @@ __global__ __launch_bounds__(WS_T) void k_ws_prog(hemk_spec sp,
 
 	// increments since each segment start (P) and the start's position,
 	// carried across threads
-	seg_sum ls = {0u, -1, 0};
-#pragma unroll
-	for (int k = 0; k < WS_E; ++k) {
-		if (!((smask >> k) & 1u))
-			continue;
-		if ((hmask >> k) & 1u) {
-			ls.h = 1;
-			ls.acc = 0;
-			ls.hp = (int)(e0 + k) | (((evmask >> k) & 1u) ? WS_EVEN : 0);
-		}
-		ls.acc += q[k];
-	}
-	const auto ssop = [](seg_sum l, seg_sum r) {
-		seg_sum o;
-		o.h = l.h | r.h;
-		o.acc = r.h ? r.acc : l.acc + r.acc;
-		o.hp = r.h ? r.hp : l.hp;
-		return o;
-	};
-	seg_sum stot;
-	const seg_sum cs = block_excl(ls, seg_sum{0u, -1, 0}, ssop, w_s, stot);
+	// Increments since each segment start (P), the start's position, and
 	// the starts S (in units) each segment value allows: with P before and
 	// P' after it, S + P and S + P' in [2^52 + 1, 2^53 - 1] (negated for
 	// a negative binade) for a value rounded in its binade, and
-	// |S + P'| <= 2^53 - 1 for an exact row-grid value; lo and hi are the
-	// tightest bounds since the segment's start
-	// (with P' = P + q: S >= L - min(0, q) - P and S <= U - max(0, q) - P)
+	// |S + P'| <= 2^53 - 1 for an exact row-grid value (with P' = P + q:
+	// S >= L - min(0, q) - P and S <= U - max(0, q) - P); lo and hi are
+	// the tightest bounds since the segment's start.  Each thread's part
+	// is relative to P at its first value (or at its last segment start);
+	// the scan shifts a later part's bounds by the P of the part before.
 	const long long B52 = 1LL << 52, B53 = 1LL << 53;
+	const long long NOB = 1LL << 62;   // no bound yet (|bounds| < 2^56)
 	long long wl[WS_E], wu[WS_E];
 #pragma unroll
 	for (int k = 0; k < WS_E; ++k) {
@@ -696,36 +675,37 @@ __global__ __launch_bounds__(WS_T) void k_ws_prog(hemk_spec sp,
 		lo = max(lo, wl[k] - p0);
 		hi = min(hi, wu[k] - p0);
 	};
-	seg_rng lr = {0u, 0u, LLONG_MIN, LLONG_MAX};
-	{
-		long long acc = cs.acc;
+	seg_sum ls = {0u, -1, 0u, 0u, 0, -NOB, NOB};
 #pragma unroll
-		for (int k = 0; k < WS_E; ++k) {
-			if (!((smask >> k) & 1u))
-				continue;
-			if ((hmask >> k) & 1u) {
-				lr.h = 1;
-				lr.x = 0;
-				lr.lo = LLONG_MIN;
-				lr.hi = LLONG_MAX;
-				acc = 0;
-			}
-			bounds(k, acc, lr.lo, lr.hi);
-			acc += q[k];
-			lr.x |= (xmask >> k) & 1u;
+	for (int k = 0; k < WS_E; ++k) {
+		if (!((smask >> k) & 1u))
+			continue;
+		if ((hmask >> k) & 1u) {
+			ls.h = 1;
+			ls.hp = (int)(e0 + k) | (((evmask >> k) & 1u) ? WS_EVEN : 0);
+			ls.x = 0;
+			ls.acc = 0;
+			ls.lo = -NOB;
+			ls.hi = NOB;
 		}
+		bounds(k, ls.acc, ls.lo, ls.hi);
+		ls.acc += q[k];
+		ls.x |= (xmask >> k) & 1u;
 	}
-	const auto srop = [](seg_rng l, seg_rng r) {
-		seg_rng o;
+	const auto ssop = [](seg_sum l, seg_sum r) {
+		seg_sum o;
 		o.h = l.h | r.h;
+		o.hp = r.h ? r.hp : l.hp;
 		o.x = r.h ? r.x : (l.x | r.x);
-		o.lo = r.h ? r.lo : max(l.lo, r.lo);
-		o.hi = r.h ? r.hi : min(l.hi, r.hi);
+		o.pad = 0;
+		o.acc = r.h ? r.acc : l.acc + r.acc;
+		o.lo = r.h ? r.lo : max(l.lo, r.lo - l.acc);
+		o.hi = r.h ? r.hi : min(l.hi, r.hi - l.acc);
 		return o;
 	};
-	seg_rng rtot;
-	const seg_rng cr = block_excl(
-	    lr, seg_rng{0u, 0u, LLONG_MIN, LLONG_MAX}, srop, w_r, rtot);
+	seg_sum stot;
+	const seg_sum cs = block_excl(
+	    ls, seg_sum{0u, -1, 0u, 0u, 0, -NOB, NOB}, ssop, w_s, stot);
 
 	// records: single values at their own index, a segment at its
 	// start's index (no record lies between a segment's start and end).
@@ -736,8 +716,8 @@ __global__ __launch_bounds__(WS_T) void k_ws_prog(hemk_spec sp,
 	// [lo, hi] of a run rounded in its binade is on the unit's grid), the
 	// binade code of the unit it must be a multiple of, plus one
 	seg_sum r = cs;
-	long long slo = cr.lo, shi = cr.hi;
-	uint32_t sx = cr.x;
+	long long slo = cs.lo, shi = cs.hi;
+	uint32_t sx = cs.x;
 	uint32_t nr = 0;   // records at this thread's values so far
 	const size_t rb0 = (cb + j) * cap + rbase;
 	double2 *wlh = (double2 *)sp.wlh;

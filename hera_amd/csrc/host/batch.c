@@ -468,7 +468,7 @@ static int batch_graph(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 	e = hemd_graph_begin(p->stream);
 	if (e)
 		return hem_dev_fail(e, "batch graph capture");
-	for (int i = 0; i < HEM_ITERS_PER_GRAPH; ++i) {
+	for (int i = 0; i < HEM_BATCH_ITERS_PER_GRAPH; ++i) {
 		rc2 = batch_iteration(p, o, rc, chunk, NULL);
 		if (rc2) {
 			hemd_graph_end(p->stream, &exec);
@@ -562,7 +562,7 @@ static int run_batch(hem_problem *p, const hem_opts *o, uint32_t R,
 	HEM_TRY_DEV(hemd_event_record(p->ev0, p->stream), "event");
 	for (;;) {
 		HEM_TRY_DEV(hemd_graph_launch(graph, p->stream), "graph launch");
-		launched += HEM_ITERS_PER_GRAPH;
+		launched += HEM_BATCH_ITERS_PER_GRAPH;
 		if (launched < o->min_iter && launched < limit)
 			continue;
 		HEM_TRY_DEV(hemd_d2h(w->h_active, rc.active,
@@ -576,7 +576,7 @@ static int run_batch(hem_problem *p, const hem_opts *o, uint32_t R,
 			if (act == 0)
 				break;
 		}
-		if (launched > limit + HEM_ITERS_PER_GRAPH)
+		if (launched > limit + HEM_BATCH_ITERS_PER_GRAPH)
 			return hem_fail(HEM_ERR_DEVICE,
 			                "batched loop did not terminate");
 	}

@@ -8,9 +8,11 @@
 #include "../../../include/hera_em.h"
 #include "../dev/hem_dev.h"
 
-/* Iterations per captured graph.  Iterations past convergence are no-ops
- * (every kernel reads the control block's done flag first). */
-#define HEM_ITERS_PER_GRAPH 4
+/* Iterations per captured graph of the plain solve and of the batched
+ * solve.  Iterations past convergence are no-ops (every kernel reads the
+ * control block's done flag first). */
+#define HEM_ITERS_PER_GRAPH 8
+#define HEM_BATCH_ITERS_PER_GRAPH 4
 /* Monotonicity fallback: an iteration keeps u2 when ell(lam) - ell(u2) <
  * -HEM_FB_TOL times the sum of the difference's terms' magnitudes. */
 #define HEM_FB_TOL 1e-10

@@ -91,6 +91,18 @@ def cases():
     mis[:, :3] = [1e30, 1.0, -1e30]
     mis[1, 20000:20003] = [1e30, 1.0, -1e30]
     out["misguided"] = mis
+    # short values (a few significant bits, as late SQUAREM r and v rows
+    # have: differences of nearby doubles) whose running sums wander
+    # around zero across many binades, so runs on a row's grid cross
+    # binades; then short values mixed with full-length ones, and values
+    # on one coarse grid interleaved with values on finer ones
+    walk = rng.integers(-2000, 2001, size=(2, 80000)) * 2.0 ** -60
+    walk[1, ::7] *= 2.0 ** -9
+    out["short_walk"] = walk
+    mixed = rng.integers(-64, 65, size=(2, 60000)) * 2.0 ** -40
+    mixed[0, ::5] = rng.normal(size=12000) * 2.0 ** -34
+    mixed[1, ::3] *= -(2.0 ** -20)
+    out["short_mixed"] = mixed
     return out
 
 

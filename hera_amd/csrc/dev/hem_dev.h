@@ -493,10 +493,13 @@ typedef struct {
 	size_t ld;
 	const uint32_t *alist;
 	const double *mres;    /* CENTRED: pass-1 sums by row                 */
-	double *csq;           /* IDENT, chain slot = row: when set, the chunk
+	double *csq;           /* IDENT, chain slot = row (any chains with chunk
+	                          programs, by chain slot): when set, the chunk
 	                          sums step also writes each chunk's sum of
 	                          x * x here ([row][P])                    */
-	const double *gsum, *gsq; /* CENTRED, chain slot = row: pass-1 chunk
+	const double *gsum, *gsq; /* CENTRED, chain slot = row (any chains
+	                          with chunk programs, by chain slot, pass 1
+	                          on the same slots): pass-1 chunk
 	                          sums of x and x * x ([row][P]); when set,
 	                          the drifts take each chunk's sum of
 	                          (x - m)^2 as gsq - 2 m gsum + L m^2 (it only

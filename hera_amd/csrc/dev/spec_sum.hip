@@ -735,9 +735,10 @@ static int spec_launch(const hemk_spec *sp, int produce, int phases,
 	    (sp->wa && (!sp->wlh || !sp->wrg || !sp->wcnt)) ||
 	    (produce && (sp->nchain != 2 || sp->pairs || sp->alist ||
 	                 sp->xf == HEMK_SPEC_CENTRED)) ||
-	    (sp->csq && (sp->xf != HEMK_SPEC_IDENT || !by_row || produce)) ||
+	    (sp->csq && (sp->xf != HEMK_SPEC_IDENT || (!by_row && !sp->wa) ||
+	                 produce)) ||
 	    (!sp->gsq != !sp->gsum) ||
-	    (sp->gsq && (sp->xf != HEMK_SPEC_CENTRED || !by_row)))
+	    (sp->gsq && (sp->xf != HEMK_SPEC_CENTRED || (!by_row && !sp->wa))))
 		return (int)hipErrorInvalidValue;
 	switch (sp->xf) {
 	case HEMK_SPEC_IDENT:

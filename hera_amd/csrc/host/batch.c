@@ -161,6 +161,7 @@ void hem_batch_free(hem_problem *p)
 	hemd_free(w->swrg);
 	hemd_free(w->swcnt);
 	hemd_free(w->scpub);
+	hemd_free(w->scsq);
 	hemd_host_free(w->h_active);
 	memset(w, 0, sizeof(*w));
 	for (int i = 0; i < HEM_TILE_ARRAYS; ++i) {
@@ -233,6 +234,7 @@ static int ws_ensure(hem_problem *p, uint32_t R)
 		WALLOC(w->swrg, nrec * 4);
 		WALLOC(w->swcnt, (size_t)2 * R * p->spec_P * 4);
 		WALLOC(w->scpub, (size_t)2 * R * p->spec_P * 8);
+		WALLOC(w->scsq, (size_t)2 * R * p->spec_P * 8);
 		if (hemk_fill_u32((uint32_t *)w->scpub,
 		                  (size_t)4 * R * p->spec_P, 0, p->stream))
 			return hem_fail(HEM_ERR_DEVICE, "batch workspace fill");
@@ -399,12 +401,22 @@ static int batch_iteration(hem_problem *p, const hem_opts *o, hemk_rctl rc,
 		hemk_spec sp = spec_batch(p, o, rc, 1,
 		                          sd ? HEMK_SPEC_IDENT : HEMK_SPEC_SQUARE,
 		                          NULL, sd ? sums1 : rc.q_r);
+		/* with chunk programs, pass 1 keeps each chunk's sum of squares
+		 * and its published sum, from which pass 2 (the same chain
+		 * slots) seeds its guesses */
+		const int seed = sd && sp.cpub;
+		if (seed)
+			sp.csq = w->scsq;
 		HEM_TRY_DEV(hemk_spec_sum(&sp, 0, NULL, NULL, NULL, NULL, s),
 		            "batch exact sums 1");
 		BMARK(HEM_BK_NORM1);
 		if (sd) {
 			sp = spec_batch(p, o, rc, 1, HEMK_SPEC_CENTRED, sums1,
 			                rc.q_r);
+			if (seed) {
+				sp.gsum = (const double *)w->scpub;
+				sp.gsq = w->scsq;
+			}
 			HEM_TRY_DEV(hemk_spec_sum(&sp, 0, NULL, NULL, NULL, NULL,
 			                          s),
 			            "batch exact sums 2");

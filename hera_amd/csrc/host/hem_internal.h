@@ -47,6 +47,7 @@ typedef struct {
 	double *swa, *swlh;           /* [2R][P][L/2] chunk programs       */
 	uint32_t *swrg, *swcnt;
 	unsigned long long *scpub;
+	double *scsq;   /* pass-1 chunk sums of squares (pass 2's seeds) */
 	uint32_t *h_active;            /* pinned                            */
 	/* captured iterations; the loop bounds and stop thresholds are kernel
 	 * arguments, so a graph is reused only for identical options */

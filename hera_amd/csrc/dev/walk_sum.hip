@@ -182,8 +182,9 @@ struct cnt_par {
 	uint32_t n, d, v;
 };
 
-// a segment-candidate run starts among the values (h), then whether a tie
-// has been seen since the last start (or over all the values)
+// a segmented flag: a stretch starts among the values (h), then whether the
+// flag (a tie, or an anchor) has been seen since the last start (or over
+// all the values)
 struct tie_seen {
 	uint32_t h, t;
 };

@@ -440,9 +440,11 @@ __global__ void k_iter_end(hemk_ctl *__restrict__ ctl)
 
 __global__ void k_ctl_reset(hemk_ctl *ctl, uint32_t min_iter,
                             uint32_t max_iter, double stop_min,
-                            double stop_rel, uint32_t *hdone)
+                            double stop_rel, uint32_t *hdone,
+                            uint32_t *hk)
 {
 	ctl->hdone = hdone;
+	ctl->hk = hk;
 	ctl->k = 0;
 	ctl->maps = 0;
 	ctl->converged = 0;
@@ -664,10 +666,10 @@ int hemk_init_state(uint32_t n, double *lam64, float *lam32, double value,
 
 int hemk_ctl_reset(hemk_ctl *ctl, uint32_t min_iter, uint32_t max_iter,
                    double stop_min, double stop_rel, uint32_t *hdone,
-                   void *s)
+                   uint32_t *hk, void *s)
 {
 	k_ctl_reset<<<1, 1, 0, (hipStream_t)s>>>(ctl, min_iter, max_iter,
-	                                         stop_min, stop_rel, hdone);
+	                                         stop_min, stop_rel, hdone, hk);
 	return (int)hipGetLastError();
 }
 

@@ -8,6 +8,15 @@
 
 #include "hem_dev.h"
 
+// k after an iteration end, and the host's copy of it.
+__device__ __forceinline__ void em_set_k(hemk_ctl *ctl, uint32_t k)
+{
+	ctl->k = k;
+	if (ctl->hk)
+		__hip_atomic_store(ctl->hk, k, __ATOMIC_RELAXED,
+		                   __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // The loop's end: done, converged, and the host's copy of done.
 __device__ __forceinline__ void em_set_done(hemk_ctl *ctl, uint32_t stop)
 {
@@ -26,7 +35,7 @@ __device__ __forceinline__ void em_iter_end(hemk_ctl *ctl)
 	const uint32_t stop = ctl->stopbad ? 0u : 1u;
 	const uint32_t k = ctl->k + 1;
 	ctl->stopbad = 0;
-	ctl->k = k;
+	em_set_k(ctl, k);
 	ctl->maps += 3;
 	if (!(k < ctl->min_iter || (k < ctl->max_iter && !stop)))
 		em_set_done(ctl, stop);

@@ -66,6 +66,8 @@ typedef struct {
 	uint32_t *hdone;          /* pinned host word set to 1 with done (the
 	                             host's loop reads it without a copy), or
 	                             NULL                                  */
+	uint32_t *hk;             /* pinned host word: k after each iteration
+	                             end, or NULL                          */
 } hemk_ctl;
 
 /* Device view of one problem (all pointers are device pointers). */
@@ -327,11 +329,12 @@ int hemk_precond(const double *fdq, const double *piq, const double *scal,
  * init64 when non-NULL. */
 int hemk_init_state(uint32_t n, double *lam64, float *lam32, double value,
                     const double *init64, void *s);
-/* ctl_reset: hdone (pinned host memory, or NULL) is set to 1 when the loop
- * ends; the caller clears it before. */
+/* ctl_reset: hdone and hk (pinned host memory, or NULL) receive 1 when the
+ * loop ends and k after each iteration end; the caller clears them
+ * before. */
 int hemk_ctl_reset(hemk_ctl *ctl, uint32_t min_iter, uint32_t max_iter,
                    double stop_min, double stop_rel, uint32_t *hdone,
-                   void *s);
+                   uint32_t *hk, void *s);
 
 /* CSR pass: rho_c = n_c / sum_{t in c} lam_t (0 if the sum is 0).
  * prec 0: lam32/rho32; prec 1: lam64/rho64.  group = lanes per class. */

@@ -709,7 +709,7 @@ __global__ __launch_bounds__(TA_THREADS) void k_ptile_tree_alpha(
 			const uint32_t stop = ctl->stopbad ? 0u : 1u;
 			const uint32_t k = ctl->k + 1;
 			ctl->stopbad = 0;
-			ctl->k = k;
+			em_set_k(ctl, k);
 			ctl->maps += 3;
 			fin = !(k < ctl->min_iter ||
 			        (k < ctl->max_iter && !stop));

@@ -89,7 +89,6 @@ struct hem_problem {
 	int group_cls, group_tx;
 	void *stream;
 	void *ev0, *ev1;
-	void *evq[2];         /* the loop's control readbacks */
 
 	/* structure (device) */
 	uint32_t *d_cls_off, *d_cls_ref, *d_tx_off, *d_tx_ref, *d_len;
@@ -126,7 +125,8 @@ struct hem_problem {
 	int weighted;
 	hemk_ctl *d_ctl;
 	hemk_ctl *h_ctl; /* pinned */
-	uint32_t *h_done; /* pinned, the loop's done flag (set by the device) */
+	uint32_t *h_done; /* pinned, set by the device: [0] the loop's done flag,
+	                     [1] iterations completed */
 
 	/* component tiles (n_tiles == 0: some component is too large) for
 	 * the batched and standard-error kernels */

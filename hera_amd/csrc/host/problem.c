@@ -301,10 +301,6 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 		rc = hemd_event_create(&p->ev2);
 	if (!rc)
 		rc = hemd_event_create(&p->ev3);
-	if (!rc)
-		rc = hemd_event_create(&p->evq[0]);
-	if (!rc)
-		rc = hemd_event_create(&p->evq[1]);
 	if (rc) {
 		free(tx_off);
 		free(tx_ref);
@@ -375,7 +371,7 @@ static int create_impl(const hem_classes *c, const uint32_t *tx0,
 #undef DALLOC_H
 	rc = hemd_host_alloc((void **)&p->h_ctl, sizeof(hemk_ctl));
 	if (!rc)
-		rc = hemd_host_alloc((void **)&p->h_done, sizeof(uint32_t));
+		rc = hemd_host_alloc((void **)&p->h_done, 2 * sizeof(uint32_t));
 	if (rc) {
 		free(tx_off);
 		free(tx_ref);
@@ -694,9 +690,6 @@ int hem_problem_destroy(hem_problem *p)
 	hemd_free(p->d_ctl);
 	hemd_host_free(p->h_ctl);
 	hemd_host_free(p->h_done);
-	for (int i = 0; i < 2; ++i)
-		if (p->evq[i])
-			hemd_event_destroy(p->evq[i]);
 	if (p->ev0)
 		hemd_event_destroy(p->ev0);
 	if (p->ev1)

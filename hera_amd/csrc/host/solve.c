@@ -423,18 +423,40 @@ static int ensure_graph(hem_problem *p, const hem_opts *o, void **graph)
 	return HEM_OK;
 }
 
+/* Waits until the device has completed `target` iterations or ended the
+ * loop, as it mirrors both into pinned memory; after 10 s without either the
+ * stream is synchronised instead (a device error surfaces there). */
+static int wait_iters(hem_problem *p, uint32_t target)
+{
+	volatile const uint32_t *h = p->h_done;
+	struct timespec t0, t1;
+	clock_gettime(CLOCK_MONOTONIC, &t0);
+	for (uint32_t n = 1;; ++n) {
+		if (h[0] || h[1] >= target)
+			return HEM_OK;
+		if ((n & 0xFFFu) == 0) {
+			clock_gettime(CLOCK_MONOTONIC, &t1);
+			if (t1.tv_sec - t0.tv_sec > 10) {
+				const int e = hemd_stream_sync(p->stream);
+				return e ? hem_dev_fail(e, "stream sync") : HEM_OK;
+			}
+		}
+	}
+}
+
 /* Runs the iteration loop on the device from the state already in A.  From
  * min_iter on the host checks the done flag the device mirrors into pinned
- * memory (no copy on the stream) one graph behind: graph g + 1 is queued
- * before the event after graph g is waited for, so the device never idles
- * on the host's check; the graph queued after convergence does nothing
- * (every kernel reads the done flag first). */
+ * memory one graph behind, with no copy or event on the stream: graph g + 1
+ * is queued before the host waits for the device's iteration count to
+ * reach the end of graph g, so the device never idles on the host's check;
+ * the graph queued after convergence does nothing (every kernel reads the
+ * done flag first). */
 static int run_loop(hem_problem *p, const hem_opts *o, float *ms)
 {
 	int rc;
 	void *graph = NULL;
 	uint32_t launched = 0;
-	int slot = 0, pending = 0;
+	int pending = 0;
 	const uint32_t limit = o->max_iter > o->min_iter ? o->max_iter
 	                                                  : o->min_iter;
 	if ((rc = ensure_graph(p, o, &graph)))
@@ -446,12 +468,11 @@ static int run_loop(hem_problem *p, const hem_opts *o, float *ms)
 		launched += HEM_ITERS_PER_GRAPH;
 		if (launched < o->min_iter && launched < limit)
 			continue;
-		HEM_TRY_DEV(hemd_event_record(p->evq[slot], p->stream), "event");
-		slot ^= 1;
 		if (pending) {
 			/* the graph before this one has finished */
-			HEM_TRY_DEV(hemd_event_sync(p->evq[slot]), "event sync");
-			if (*(volatile uint32_t *)p->h_done)
+			if ((rc = wait_iters(p, launched - HEM_ITERS_PER_GRAPH)))
+				return rc;
+			if (((volatile uint32_t *)p->h_done)[0])
 				break;
 		}
 		pending = 1;
@@ -483,11 +504,12 @@ static int solve_setup(hem_problem *p, const uint32_t *unique,
 	if (!rc)
 		rc = hemk_init_state(p->n_tx, p->d_A, p->d_A32,
 		                     (double)1 / p->n_tx, NULL, p->stream);
-	*(volatile uint32_t *)p->h_done = 0;
+	((volatile uint32_t *)p->h_done)[0] = 0;
+	((volatile uint32_t *)p->h_done)[1] = 0;
 	if (!rc)
 		rc = hemk_ctl_reset(p->d_ctl, o->min_iter, o->max_iter,
 		                    o->stop_min_value, o->stop_rel_change,
-		                    p->h_done, p->stream);
+		                    p->h_done, p->h_done + 1, p->stream);
 	if (rc)
 		return hem_dev_fail(rc, "solve setup");
 	return use_tiles(p, o) && !o->fallback ? enqueue_tiles_prologue(p, o)
